@@ -1,0 +1,293 @@
+/*
+ * nimbus_hip.h - C ABI of the MI355X (gfx950) HIP compute backend for the
+ * Nimbus Klemp-Wilhelmson split-explicit large step.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative NIMBUS_E* code on
+ *    failure; nimbus_hip_last_error() returns the thread's last message. A
+ *    failed dispatch leaves caller-owned host fields untouched, as the
+ *    reference dispatch contract requires (include/compute/dispatch/dynamics.hpp:682-683).
+ *  - Host 3-D arrays use the reference layout [nx][ny][nz] with k contiguous,
+ *    index (i*ny + j)*nz + k (include/core/field/field3d.hpp:543-574). nx and ny
+ *    include the one-cell lateral ghost layer on each side.
+ *  - Device memory is owned by the context; host memory by the caller. Calls
+ *    on one context are serialised by the caller (one context per host thread),
+ *    as the reference backend's single orchestration thread does.
+ */
+#ifndef NIMBUS_HIP_H
+#define NIMBUS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum
+{
+    NIMBUS_OK = 0,
+    NIMBUS_EINVAL = -1,    /* invalid argument or shape: nothing was touched */
+    NIMBUS_ENODEV = -2,    /* no HIP device / device index out of range      */
+    NIMBUS_ENOMEM = -3,    /* device allocation or admission budget refused  */
+    NIMBUS_EHIP = -4,      /* a HIP runtime call failed                      */
+    NIMBUS_ENOENT = -5     /* unknown member id                              */
+};
+
+enum
+{
+    NIMBUS_PRECISION_FP64 = 0,  /* FP64 storage, FP64 arithmetic                      */
+    NIMBUS_PRECISION_MIXED = 1  /* FP32 storage, FP64 acoustic/EOS/base/Thomas terms  */
+};
+
+enum
+{
+    NIMBUS_ACOUSTIC_EXPLICIT = 0, /* reference forward-backward substep               */
+    NIMBUS_ACOUSTIC_HEVI = 1      /* vertically implicit w-p' with FP64 Thomas solve  */
+};
+
+/* Lateral / vertical boundary codes (boundary_conditions_cartesian_cgrid.cpp:26-60). */
+enum
+{
+    NIMBUS_BC_PERIODIC = 0,
+    NIMBUS_BC_RADIATIVE = 1,
+    NIMBUS_BC_ZERO_GRADIENT = 2,
+    NIMBUS_BC_RIGID = 3,
+    NIMBUS_BC_RIGID_SLIP = 4
+};
+enum { NIMBUS_BOTTOM_FREE_SLIP = 0, NIMBUS_BOTTOM_NO_SLIP = 1 };
+enum { NIMBUS_TOP_RIGID = 0, NIMBUS_TOP_OPEN = 1 };
+
+typedef struct nimbus_hip_ctx nimbus_hip_ctx;
+
+typedef struct
+{
+    int nx, ny, nz;       /* cells including lateral ghosts */
+    double dx, dy, dz;    /* metres */
+} nimbus_grid;
+
+/* Large-step configuration (split_explicit.cpp:67-258; kessler.hpp:40-46). */
+typedef struct
+{
+    double dt;            /* large step (s) */
+    int substeps;         /* requested acoustic substeps; promoted to the CFL minimum */
+    int acoustic_mode;    /* NIMBUS_ACOUSTIC_* */
+    double hevi_beta;     /* off-centering of the implicit vertical step (HE-VI) */
+    double coriolis_f;    /* f-plane parameter (1/s) */
+    int bc_west, bc_east, bc_south, bc_north, bc_bottom, bc_top;
+    double qc0, c_auto, c_accr, evap_eff, vt_max;
+} nimbus_step_config;
+
+/* FP64 base-state profiles at cell centres, each nz long
+ * (include/core/runtime/simulation_state.hpp:104-111). */
+typedef struct
+{
+    const double *rho0, *p0, *theta0, *qv0, *u0, *v0;
+} nimbus_base_state;
+
+/* Host views of the nine prognostic fields, reference layout. The element type
+ * is float for the *_f32 entry points and double for *_f64. rain_mm (nx*ny
+ * doubles, accumulated surface rain) may be null. */
+typedef struct
+{
+    void *u, *v, *w, *rho, *p, *theta, *qv, *qc, *qr;
+    double* rain_mm;
+} nimbus_host_state;
+
+/* Analytic Weisman-Klemp-type sounding with a quarter-circle hodograph. */
+typedef struct
+{
+    double theta_sfc;        /* K (300) */
+    double theta_tropo;      /* K (343) */
+    double z_tropo;          /* m (12000) */
+    double T_tropo;          /* K (213) */
+    double rh_sfc, rh_top;   /* 0.95, 0.25 */
+    double qv_cap;           /* kg/kg (0.014) */
+    double p_sfc;            /* Pa (100000) */
+    double shear;            /* |V(6 km) - V(0)| m/s (30) */
+    double shear_depth;      /* m (6000) */
+    double curve_depth;      /* m (4000): depth of the quarter circle */
+    double curvature;        /* 1 = quarter circle, 0 = straight line */
+} nimbus_sounding;
+
+typedef struct
+{
+    double xc, yc, zc;       /* centre (m); x, y measured from the west/south wall */
+    double rh, rz;           /* horizontal / vertical radius (m) */
+    double dtheta;           /* K */
+} nimbus_bubble;
+
+typedef struct
+{
+    nimbus_sounding sounding;
+    int n_bubbles;
+    nimbus_bubble bubbles[4];
+    double noise_amp;        /* K, uniform in [-amp, amp) on physical cells */
+    uint64_t seed;
+    uint32_t member;
+} nimbus_ic_params;
+
+typedef struct
+{
+    nimbus_grid grid;
+    nimbus_step_config config;
+    int precision;           /* NIMBUS_PRECISION_* */
+    nimbus_base_state base;  /* copied at creation */
+} nimbus_member_desc;
+
+typedef struct
+{
+    double max_w;            /* max w over the physical domain (m/s) */
+    double min_w;
+    double rms_theta_p;      /* RMS of theta - theta0 over physical cells (K) */
+    double rain_total_mm;    /* sum of accumulated surface rain over columns (mm) */
+    double max_rain_mm;
+    double mean_qc, mean_qr; /* domain means over physical cells (kg/kg) */
+    int64_t steps;           /* large steps completed */
+} nimbus_diag;
+
+/* ---------------- context ---------------- */
+const char* nimbus_hip_last_error(void);
+int nimbus_hip_create(int device, nimbus_hip_ctx** out);
+int nimbus_hip_destroy(nimbus_hip_ctx* ctx);
+int nimbus_hip_mem_info(nimbus_hip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+/* Counts completed dispatches per kernel family (backend_interface.hpp:46-64 analog). */
+uint64_t nimbus_hip_dispatch_count(nimbus_hip_ctx* ctx);
+
+/* ---------------- initial conditions (host only, no device needed) ---------------- */
+void nimbus_default_sounding(nimbus_sounding* s);
+/* Hydrostatic FP64 base state of the sounding at cell centres z_k = (k+1/2) dz. */
+int nimbus_ic_base_state(const nimbus_grid* g, const nimbus_sounding* s, double* rho0, double* p0,
+                         double* theta0, double* qv0, double* u0, double* v0);
+/* Full initial state: base + bubbles (RH preserving) + seeded noise; p = p0, rho from the EOS. */
+int nimbus_ic_fill_f32(const nimbus_grid* g, const nimbus_ic_params* prm, const nimbus_host_state* out);
+int nimbus_ic_fill_f64(const nimbus_grid* g, const nimbus_ic_params* prm, const nimbus_host_state* out);
+/* Counter-based uniform [0,1) hash of (seed, member, i, j, k) used for the noise. */
+double nimbus_ic_uniform(uint64_t seed, uint32_t member, int i, int j, int k);
+/* Effective acoustic substep count (split_explicit.cpp:67-156; HE-VI checks only the horizontal CFL). */
+int nimbus_resolve_substeps(const nimbus_grid* g, const nimbus_base_state* b, const nimbus_step_config* c);
+
+/* ---------------- resident ensemble members ---------------- */
+int nimbus_hip_member_create(nimbus_hip_ctx* ctx, const nimbus_member_desc* desc, uint32_t* id);
+int nimbus_hip_member_destroy(nimbus_hip_ctx* ctx, uint32_t id);
+/* Bytes the member would hold on the device (admission planning). */
+uint64_t nimbus_hip_member_bytes(const nimbus_member_desc* desc);
+int nimbus_hip_member_upload(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s);
+int nimbus_hip_member_download(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s);
+int nimbus_hip_member_diag(nimbus_hip_ctx* ctx, uint32_t id, nimbus_diag* d);
+/* Advance each listed member by n_steps large steps; returns after the device finished. */
+int nimbus_hip_step(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_steps);
+/* Same, enqueued on the context stream without waiting. */
+int nimbus_hip_step_async(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_steps);
+int nimbus_hip_synchronize(nimbus_hip_ctx* ctx);
+/* Enable hipGraph capture of whole large steps (per member and substep count). */
+int nimbus_hip_set_graphs(nimbus_hip_ctx* ctx, int enable);
+/* Per-kernel timing with HIP events on the launch stream (0 disables). */
+int nimbus_hip_set_kernel_timing(nimbus_hip_ctx* ctx, int enable);
+/* Kernel timing table: name, launches, total milliseconds. Returns the number of rows. */
+int nimbus_hip_kernel_timing(nimbus_hip_ctx* ctx, int max_rows, char (*names)[48], uint64_t* launches,
+                             double* total_ms);
+int nimbus_hip_reset_kernel_timing(nimbus_hip_ctx* ctx);
+/* Elapsed device time (ms) of the last nimbus_hip_step call measured with HIP events. */
+double nimbus_hip_last_step_ms(nimbus_hip_ctx* ctx);
+
+/* ---------------- per-operation dispatch (reference dispatch contracts) ----------------
+ * Each takes caller-owned host float arrays in the reference layout, runs the
+ * same device kernels the resident step uses (FP32 storage, mixed arithmetic)
+ * and writes the outputs back. */
+
+/* Replaces CpuComputeBackend::dispatch_acoustic_substep_fused_cartesian_cgrid
+ * (src/compute/cpu/acoustic_cgrid.cpp:50-76; struct include/compute/dispatch/dynamics.hpp:220-277). */
+typedef struct
+{
+    float *u, *v, *w, *rho, *p;
+    const float* theta_data;
+    const double* rho0_base_host;
+    const double* p0_base_host;
+    int base_len;
+    int nx, ny, nz;
+    double dx_host, dy_host, dz_host, dt_small_host, gamma_host;
+    float rho_min, rho_max, p_min, p_max, p_ref_fallback, wind_clamp_h, wind_clamp_v;
+} nimbus_acoustic_substep_fused_dispatch;
+int nimbus_hip_dispatch_acoustic_substep_fused(nimbus_hip_ctx* ctx, const nimbus_acoustic_substep_fused_dispatch* d);
+
+/* Replaces DynamicsComputeKernels::dispatch_acoustic_substeps_batched_cartesian_cgrid
+ * (include/compute/dispatch/dynamics.hpp:279-325,726-731): n_substeps substeps with
+ * per-substep BCs and time-averaged mass-flux export (weight per substep). The
+ * KW78 radiative stash is seeded from the input faces. */
+typedef struct
+{
+    float *u, *v, *w, *rho, *p;
+    const float* theta_data;
+    const double* rho0_base_host;
+    const double* p0_base_host;
+    int nx, ny, nz;
+    double dx, dy, dz, dt_small;
+    int n_substeps;
+    int acoustic_mode;
+    double hevi_beta;
+    int bc_west, bc_east, bc_south, bc_north, bc_bottom, bc_top;
+    float *mass_flux_x, *mass_flux_y, *mass_flux_z;
+    int accumulate_mass_flux;
+    double mass_flux_weight;
+} nimbus_acoustic_substeps_batched_dispatch;
+int nimbus_hip_dispatch_acoustic_substeps_batched(nimbus_hip_ctx* ctx,
+                                                  const nimbus_acoustic_substeps_batched_dispatch* d);
+
+/* Replaces dispatch_cartesian_cgrid_slow_tendencies (include/compute/dispatch/dynamics.hpp:374-420,780):
+ * UP5 momentum advection, buoyancy, moisture buoyancy, loading, f-plane Coriolis. */
+typedef struct
+{
+    const float *u_data, *v_data, *w_data, *rho_data, *qv_data, *qc_data, *qr_data;
+    const double *rho0_base, *qv0_base, *u0_base, *v0_base;
+    float *du_dt_data, *dv_dt_data, *dw_dt_data;
+    int nr, nth, nz;
+    double dx, dy, dz, coriolis_f;
+} nimbus_slow_tendencies_dispatch;
+int nimbus_hip_dispatch_slow_tendencies(nimbus_hip_ctx* ctx, const nimbus_slow_tendencies_dispatch* d);
+
+/* Heun slow RK2 + slow apply (dynamics.cpp:5123-5292,1955-2095) on u, v, w. */
+typedef struct
+{
+    float *u, *v, *w;
+    const float *rho, *qv, *qc, *qr;
+    const double *rho0_base, *qv0_base, *u0_base, *v0_base;
+    int nx, ny, nz;
+    double dx, dy, dz, dt_large, coriolis_f;
+} nimbus_slow_rk2_dispatch;
+int nimbus_hip_dispatch_slow_rk2(nimbus_hip_ctx* ctx, const nimbus_slow_rk2_dispatch* d);
+
+/* Replaces dispatch_advect_scalars_conservative_cartesian_cgrid
+ * (include/compute/dispatch/advection.hpp:46-103,226-231): WENO5 retained-flux
+ * transport of up to 4 scalars with the positive-definite donor limiter. */
+typedef struct
+{
+    int n_scalars;
+    float* q[4];              /* in/out */
+    int positive_definite[4];
+    const float *mass_flux_x, *mass_flux_y, *mass_flux_z;
+    const float *rho_begin, *rho_end;
+    int nx, ny, nz;
+    double dx, dy, dz, dt;
+    int bc_top;
+} nimbus_advect_scalars_dispatch;
+int nimbus_hip_dispatch_advect_scalars(nimbus_hip_ctx* ctx, const nimbus_advect_scalars_dispatch* d);
+
+/* Replaces dispatch_kessler_pointwise_backend + dispatch_kessler_sedimentation_backend
+ * (include/compute/dispatch/microphysics.hpp:436,726,752; kessler.cpp:103-531):
+ * saturation adjustment, warm rain and sedimentation tendencies on the physical domain. */
+typedef struct
+{
+    const float *p, *theta, *rho, *qv, *qc, *qr;
+    float *dtheta_dt, *dqv_dt, *dqc_dt, *dqr_dt;
+    int nx, ny, nz;
+    double dz, dt;
+    double qc0, c_auto, c_accr, evap_eff, vt_max;
+} nimbus_kessler_dispatch;
+int nimbus_hip_dispatch_kessler(nimbus_hip_ctx* ctx, const nimbus_kessler_dispatch* d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NIMBUS_HIP_H */

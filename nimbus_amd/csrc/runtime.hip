@@ -1,0 +1,970 @@
+// Host runtime of the HIP backend: device context, resident ensemble members
+// (device-owned state in [k][j][i] layout), hipGraph capture of whole large
+// steps, per-kernel HIP-event timing and the C ABI declared in nimbus_hip.h.
+#include "kernels.hpp"
+#include "nimbus_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace nimbus
+{
+namespace
+{
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(NIMBUS_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define NIMBUS_HIP_TRY(expr)                                    \
+    do                                                          \
+    {                                                           \
+        const hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);       \
+    } while (0)
+
+int pitch_for(int nx, int elem)
+{
+    const int per = 64 / elem;  // 64-byte rows
+    return (nx + per - 1) / per * per;
+}
+
+struct Member
+{
+    nimbus_member_desc desc{};
+    std::vector<double> host_base;  // kBaseCount * nz
+    DevGrid g{};
+    int elem = 4;
+    int substeps = 0;
+    StepParams params{};
+    void* block = nullptr;  // one device allocation for everything
+    std::size_t bytes = 0;
+    DevFields f{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::int64_t steps = 0;
+};
+
+// Field slots of a member allocation, in order.
+enum FieldSlot
+{
+    F_U, F_V, F_W, F_RP, F_PP, F_TH, F_QV, F_QC, F_QR,
+    F_MFX, F_MFY, F_MFZ, F_RB,
+    F_K1U, F_K1V, F_K1W, F_SU, F_SV, F_SW,
+    F_RALT,
+    F_TTH, F_TQV, F_TQC, F_TQR,
+    F_COUNT
+};
+
+std::size_t align256(std::size_t x) { return (x + 255) / 256 * 256; }
+
+struct Layout
+{
+    std::size_t field, rl, tri, rad_w, rad_s, rain, base, total;
+};
+
+Layout layout_for(const DevGrid& g, int elem, bool hevi)
+{
+    Layout L{};
+    L.field = align256(g.cells() * std::size_t(elem));
+    L.rl = align256(g.cells() * sizeof(float));
+    L.tri = hevi ? align256(g.cells() * sizeof(double)) : 0;
+    L.rad_w = align256(std::size_t(g.ny) * g.nz * sizeof(float));
+    L.rad_s = align256(std::size_t(g.nx) * g.nz * sizeof(float));
+    L.rain = align256(std::size_t(g.nx) * g.ny * sizeof(double));
+    L.base = align256(std::size_t(kBaseCount) * g.nz * sizeof(double));
+    L.total = L.field * F_COUNT + 3 * L.rl + 2 * L.tri + 2 * L.rad_w + 2 * L.rad_s + L.rain + L.base;
+    return L;
+}
+
+DevGrid dev_grid(const nimbus_grid& g, int elem)
+{
+    DevGrid d{};
+    d.nx = g.nx;
+    d.ny = g.ny;
+    d.nz = g.nz;
+    d.px = pitch_for(g.nx, elem);
+    d.dx = g.dx;
+    d.dy = g.dy;
+    d.dz = g.dz;
+    return d;
+}
+
+bool valid_grid(const nimbus_grid& g)
+{
+    return g.nx >= 5 && g.ny >= 5 && g.nz >= 4 && std::isfinite(g.dx) && g.dx > 0 && std::isfinite(g.dy) && g.dy > 0 &&
+           std::isfinite(g.dz) && g.dz > 0 && std::size_t(g.nx) * g.ny * g.nz < (std::size_t(1) << 31);
+}
+
+bool valid_config(const nimbus_step_config& c)
+{
+    auto bc_ok = [](int b) { return b >= NIMBUS_BC_PERIODIC && b <= NIMBUS_BC_RIGID_SLIP; };
+    return std::isfinite(c.dt) && c.dt > 0 && c.substeps >= 0 && c.substeps < 10000 &&
+           (c.acoustic_mode == NIMBUS_ACOUSTIC_EXPLICIT || c.acoustic_mode == NIMBUS_ACOUSTIC_HEVI) &&
+           std::isfinite(c.hevi_beta) && c.hevi_beta >= 0.0 && c.hevi_beta <= 1.0 && std::isfinite(c.coriolis_f) &&
+           bc_ok(c.bc_west) && bc_ok(c.bc_east) && bc_ok(c.bc_south) && bc_ok(c.bc_north) &&
+           (c.bc_bottom == NIMBUS_BOTTOM_FREE_SLIP || c.bc_bottom == NIMBUS_BOTTOM_NO_SLIP) &&
+           (c.bc_top == NIMBUS_TOP_RIGID || c.bc_top == NIMBUS_TOP_OPEN) &&
+           !(c.acoustic_mode == NIMBUS_ACOUSTIC_HEVI && c.bc_top == NIMBUS_TOP_OPEN) && c.vt_max > 0 &&
+           std::isfinite(c.vt_max) && c.qc0 >= 0 && c.c_auto >= 0 && c.c_accr >= 0 && c.evap_eff >= 0;
+}
+
+}  // namespace
+}  // namespace nimbus
+
+struct nimbus_hip_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<std::uint32_t, std::unique_ptr<nimbus::Member>> members;
+    std::uint32_t next_id = 1;
+    bool graphs = false;
+    bool timing = false;
+    std::vector<nimbus::TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    struct Row
+    {
+        std::uint64_t launches = 0;
+        double ms = 0.0;
+    };
+    std::map<std::string, Row> table;
+    std::uint64_t launches = 0;
+    hipEvent_t step_a = nullptr, step_b = nullptr;
+    double last_step_ms = 0.0;
+    double* diag_partials = nullptr;
+    double* diag_out = nullptr;
+};
+
+namespace nimbus
+{
+namespace
+{
+
+Launcher make_launcher(nimbus_hip_ctx* ctx)
+{
+    Launcher L;
+    L.stream = ctx->stream;
+    L.timing = ctx->timing;
+    L.pending = &ctx->pending;
+    L.event_pool = &ctx->event_pool;
+    L.launches = &ctx->launches;
+    return L;
+}
+
+// Moves completed per-kernel event pairs into the timing table (after a sync).
+void harvest(nimbus_hip_ctx* ctx)
+{
+    for (auto& t : ctx->pending)
+    {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess)
+        {
+            auto& row = ctx->table[t.name];
+            row.launches += 1;
+            row.ms += ms;
+        }
+        ctx->event_pool.push_back(t.start);
+        ctx->event_pool.push_back(t.stop);
+    }
+    ctx->pending.clear();
+}
+
+Member* find(nimbus_hip_ctx* ctx, std::uint32_t id)
+{
+    auto it = ctx->members.find(id);
+    return it == ctx->members.end() ? nullptr : it->second.get();
+}
+
+void release_graph(Member& m)
+{
+    if (m.graph_exec) hipGraphExecDestroy(m.graph_exec);
+    if (m.graph) hipGraphDestroy(m.graph);
+    m.graph_exec = nullptr;
+    m.graph = nullptr;
+}
+
+int allocate_member(const nimbus_member_desc& desc, Member& m)
+{
+    m.desc = desc;
+    m.elem = desc.precision == NIMBUS_PRECISION_MIXED ? 4 : 8;
+    m.g = dev_grid(desc.grid, m.elem);
+    const int nz = desc.grid.nz;
+    const bool hevi = desc.config.acoustic_mode == NIMBUS_ACOUSTIC_HEVI;
+    m.host_base.assign(std::size_t(kBaseCount) * nz, 0.0);
+    const double* src[kBaseCount] = {desc.base.rho0, desc.base.p0, desc.base.theta0, desc.base.qv0, desc.base.u0,
+                                     desc.base.v0};
+    for (int s = 0; s < kBaseCount; ++s)
+        for (int k = 0; k < nz; ++k)
+        {
+            double v = src[s] ? src[s][k] : 0.0;
+            if (s == kBaseQv0Stored) v = double(float(v));
+            m.host_base[std::size_t(s) * nz + k] = v;
+        }
+    m.substeps = resolve_substeps(desc.grid, m.host_base.data() + std::size_t(kBaseRho0) * nz,
+                                  m.host_base.data() + std::size_t(kBaseP0) * nz, desc.config);
+    const Layout L = layout_for(m.g, m.elem, hevi);
+    m.bytes = L.total;
+    hipError_t e = hipMalloc(&m.block, L.total);
+    if (e != hipSuccess)
+    {
+        m.block = nullptr;
+        return fail(NIMBUS_ENOMEM, std::string("member allocation failed: ") + hipGetErrorString(e));
+    }
+    char* p = static_cast<char*>(m.block);
+    void* fields[F_COUNT];
+    for (int s = 0; s < F_COUNT; ++s)
+    {
+        fields[s] = p;
+        p += L.field;
+    }
+    DevFields& f = m.f;
+    f.u = fields[F_U]; f.v = fields[F_V]; f.w = fields[F_W]; f.rp = fields[F_RP]; f.pp = fields[F_PP];
+    f.th = fields[F_TH]; f.qv = fields[F_QV]; f.qc = fields[F_QC]; f.qr = fields[F_QR];
+    f.mfx = fields[F_MFX]; f.mfy = fields[F_MFY]; f.mfz = fields[F_MFZ]; f.rb = fields[F_RB];
+    f.k1u = fields[F_K1U]; f.k1v = fields[F_K1V]; f.k1w = fields[F_K1W];
+    f.su = fields[F_SU]; f.sv = fields[F_SV]; f.sw = fields[F_SW];
+    f.ralt = fields[F_RALT];
+    f.tth = fields[F_TTH]; f.tqv = fields[F_TQV]; f.tqc = fields[F_TQC]; f.tqr = fields[F_TQR];
+    f.rl_qv = reinterpret_cast<float*>(p); p += L.rl;
+    f.rl_qc = reinterpret_cast<float*>(p); p += L.rl;
+    f.rl_qr = reinterpret_cast<float*>(p); p += L.rl;
+    if (hevi)
+    {
+        f.tri_c = reinterpret_cast<double*>(p); p += L.tri;
+        f.tri_r = reinterpret_cast<double*>(p); p += L.tri;
+    }
+    f.rad_w = reinterpret_cast<float*>(p); p += L.rad_w;
+    f.rad_e = reinterpret_cast<float*>(p); p += L.rad_w;
+    f.rad_s = reinterpret_cast<float*>(p); p += L.rad_s;
+    f.rad_n = reinterpret_cast<float*>(p); p += L.rad_s;
+    f.rain = reinterpret_cast<double*>(p); p += L.rain;
+    f.base = reinterpret_cast<double*>(p); p += L.base;
+    // Zero everything once (not inside any graph), then upload the base state.
+    e = hipMemset(m.block, 0, L.total);
+    if (e == hipSuccess)
+        e = hipMemcpy(f.base, m.host_base.data(), m.host_base.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "member initialisation");
+    StepParams& sp = m.params;
+    const nimbus_step_config& c = desc.config;
+    sp.dt = c.dt;
+    sp.substeps = m.substeps;
+    sp.acoustic_mode = c.acoustic_mode;
+    sp.hevi_beta = c.hevi_beta;
+    sp.coriolis_f = c.coriolis_f;
+    sp.bc_west = c.bc_west; sp.bc_east = c.bc_east; sp.bc_south = c.bc_south; sp.bc_north = c.bc_north;
+    sp.bc_bottom = c.bc_bottom; sp.bc_top = c.bc_top;
+    sp.kessler = KesslerConst{c.qc0, c.c_auto, c.c_accr, c.evap_eff, c.vt_max};
+    sp.bounds = Bounds{};
+    return NIMBUS_OK;
+}
+
+void free_member(Member& m)
+{
+    release_graph(m);
+    if (m.block) hipFree(m.block);
+    m.block = nullptr;
+}
+
+const double* base_slot(const Member& m, int s) { return m.f.base + std::size_t(s) * m.g.nz; }
+
+// Host field h (reference layout, element H) -> device field d via a staging buffer.
+template <class H>
+int upload_field(nimbus_hip_ctx* ctx, Member& m, const void* h, void* d, const double* subtract, void* staging)
+{
+    const std::size_t n = std::size_t(m.g.nx) * m.g.ny * m.g.nz;
+    NIMBUS_HIP_TRY(hipMemcpyAsync(staging, h, n * sizeof(H), hipMemcpyHostToDevice, ctx->stream));
+    Launcher L = make_launcher(ctx);
+    L.timing = false;
+    if (m.elem == 4) launch_host_to_dev<H, float>(L, m.g, staging, d, subtract);
+    else launch_host_to_dev<H, double>(L, m.g, staging, d, subtract);
+    if (L.error != hipSuccess) return hip_fail(L.error, "upload transpose");
+    return NIMBUS_OK;
+}
+
+template <class H>
+int download_field(nimbus_hip_ctx* ctx, Member& m, const void* d, void* h, const double* add, void* staging)
+{
+    const std::size_t n = std::size_t(m.g.nx) * m.g.ny * m.g.nz;
+    Launcher L = make_launcher(ctx);
+    L.timing = false;
+    if (m.elem == 4) launch_dev_to_host<float, H>(L, m.g, d, staging, add);
+    else launch_dev_to_host<double, H>(L, m.g, d, staging, add);
+    if (L.error != hipSuccess) return hip_fail(L.error, "download transpose");
+    NIMBUS_HIP_TRY(hipMemcpyAsync(h, staging, n * sizeof(H), hipMemcpyDeviceToHost, ctx->stream));
+    NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return NIMBUS_OK;
+}
+
+struct Staging
+{
+    void* p = nullptr;
+    ~Staging()
+    {
+        if (p) hipFree(p);
+    }
+};
+
+template <class H>
+int upload_state(nimbus_hip_ctx* ctx, Member& m, const nimbus_host_state& s)
+{
+    Staging st;
+    const std::size_t n = std::size_t(m.g.nx) * m.g.ny * m.g.nz;
+    NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(H)));
+    const void* hs[9] = {s.u, s.v, s.w, s.rho, s.p, s.theta, s.qv, s.qc, s.qr};
+    void* ds[9] = {m.f.u, m.f.v, m.f.w, m.f.rp, m.f.pp, m.f.th, m.f.qv, m.f.qc, m.f.qr};
+    for (int q = 0; q < 9; ++q)
+    {
+        if (!hs[q]) return fail(NIMBUS_EINVAL, "upload: null field");
+        const double* sub = q == 3 ? base_slot(m, kBaseRho0) : (q == 4 ? base_slot(m, kBaseP0) : nullptr);
+        const int rc = upload_field<H>(ctx, m, hs[q], ds[q], sub, st.p);
+        if (rc != NIMBUS_OK) return rc;
+        NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    const std::size_t cols = std::size_t(m.g.nx) * m.g.ny;
+    std::vector<double> rain(cols, 0.0);
+    if (s.rain_mm)
+        for (int i = 0; i < m.g.nx; ++i)
+            for (int j = 0; j < m.g.ny; ++j) rain[std::size_t(j) * m.g.nx + i] = s.rain_mm[std::size_t(i) * m.g.ny + j];
+    NIMBUS_HIP_TRY(hipMemcpy(m.f.rain, rain.data(), cols * sizeof(double), hipMemcpyHostToDevice));
+    return NIMBUS_OK;
+}
+
+template <class H>
+int download_state(nimbus_hip_ctx* ctx, Member& m, const nimbus_host_state& s)
+{
+    Staging st;
+    const std::size_t n = std::size_t(m.g.nx) * m.g.ny * m.g.nz;
+    NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(H)));
+    void* hs[9] = {s.u, s.v, s.w, s.rho, s.p, s.theta, s.qv, s.qc, s.qr};
+    void* ds[9] = {m.f.u, m.f.v, m.f.w, m.f.rp, m.f.pp, m.f.th, m.f.qv, m.f.qc, m.f.qr};
+    for (int q = 0; q < 9; ++q)
+    {
+        if (!hs[q]) continue;
+        const double* add = q == 3 ? base_slot(m, kBaseRho0) : (q == 4 ? base_slot(m, kBaseP0) : nullptr);
+        const int rc = download_field<H>(ctx, m, ds[q], hs[q], add, st.p);
+        if (rc != NIMBUS_OK) return rc;
+    }
+    if (s.rain_mm)
+    {
+        const std::size_t cols = std::size_t(m.g.nx) * m.g.ny;
+        std::vector<double> rain(cols);
+        NIMBUS_HIP_TRY(hipMemcpy(rain.data(), m.f.rain, cols * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < m.g.nx; ++i)
+            for (int j = 0; j < m.g.ny; ++j) s.rain_mm[std::size_t(i) * m.g.ny + j] = rain[std::size_t(j) * m.g.nx + i];
+    }
+    return NIMBUS_OK;
+}
+
+void enqueue_step(Launcher& L, Member& m)
+{
+    if (m.elem == 4) launch_large_step<float>(L, m.g, m.f, m.params);
+    else launch_large_step<double>(L, m.g, m.f, m.params);
+}
+
+int run_member_step(nimbus_hip_ctx* ctx, Member& m)
+{
+    if (ctx->graphs && !ctx->timing)
+    {
+        if (!m.graph_exec)
+        {
+            Launcher L = make_launcher(ctx);
+            L.timing = false;
+            NIMBUS_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(L, m);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+            if (L.error != hipSuccess) return hip_fail(L.error, "step capture launch");
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            m.graph = graph;
+            NIMBUS_HIP_TRY(hipGraphInstantiate(&m.graph_exec, m.graph, nullptr, nullptr, 0));
+        }
+        NIMBUS_HIP_TRY(hipGraphLaunch(m.graph_exec, ctx->stream));
+    }
+    else
+    {
+        Launcher L = make_launcher(ctx);
+        enqueue_step(L, m);
+        if (L.error != hipSuccess) return hip_fail(L.error, "step launch");
+    }
+    ++m.steps;
+    return NIMBUS_OK;
+}
+
+int step_members(nimbus_hip_ctx* ctx, const std::uint32_t* ids, int n_ids, int n_steps, bool wait)
+{
+    if (!ctx || (!ids && n_ids > 0) || n_ids < 0 || n_steps < 0) return fail(NIMBUS_EINVAL, "step: bad arguments");
+    std::vector<Member*> ms;
+    for (int q = 0; q < n_ids; ++q)
+    {
+        Member* m = find(ctx, ids[q]);
+        if (!m) return fail(NIMBUS_ENOENT, "step: unknown member");
+        ms.push_back(m);
+    }
+    NIMBUS_HIP_TRY(hipEventRecord(ctx->step_a, ctx->stream));
+    for (int s = 0; s < n_steps; ++s)
+        for (Member* m : ms)
+        {
+            const int rc = run_member_step(ctx, *m);
+            if (rc != NIMBUS_OK) return rc;
+        }
+    NIMBUS_HIP_TRY(hipEventRecord(ctx->step_b, ctx->stream));
+    if (wait)
+    {
+        NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        float ms = 0.0f;
+        NIMBUS_HIP_TRY(hipEventElapsedTime(&ms, ctx->step_a, ctx->step_b));
+        ctx->last_step_ms = ms;
+        harvest(ctx);
+    }
+    return NIMBUS_OK;
+}
+
+// A scratch member for one per-operation dispatch: mixed precision, given
+// grid and base profiles, destroyed when the dispatch returns.
+struct ScratchMember
+{
+    nimbus_hip_ctx* ctx;
+    Member m;
+    ~ScratchMember() { free_member(m); }
+};
+
+int make_scratch(nimbus_hip_ctx* ctx, ScratchMember& sm, int nx, int ny, int nz, double dx, double dy, double dz,
+                 const double* rho0, const double* p0, const double* th0, const double* qv0, const double* u0,
+                 const double* v0, int acoustic_mode)
+{
+    nimbus_member_desc d{};
+    d.grid = nimbus_grid{nx, ny, nz, dx, dy, dz};
+    d.config.dt = 1.0;
+    d.config.substeps = 1;
+    d.config.acoustic_mode = acoustic_mode;
+    d.config.hevi_beta = 0.1;
+    d.config.vt_max = 20.0;
+    d.precision = NIMBUS_PRECISION_MIXED;
+    d.base = nimbus_base_state{rho0, p0, th0, qv0, u0, v0};
+    if (!valid_grid(d.grid)) return fail(NIMBUS_EINVAL, "dispatch: invalid grid");
+    sm.ctx = ctx;
+    return allocate_member(d, sm.m);
+}
+
+template <class T>
+bool all_nonnull(std::initializer_list<T> ps)
+{
+    for (auto p : ps)
+        if (!p) return false;
+    return true;
+}
+
+}  // namespace
+}  // namespace nimbus
+
+using namespace nimbus;
+
+extern "C" {
+
+const char* nimbus_hip_last_error(void) { return g_last_error.c_str(); }
+
+int nimbus_hip_create(int device, nimbus_hip_ctx** out)
+{
+    if (!out) return fail(NIMBUS_EINVAL, "create: null output");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(NIMBUS_ENODEV, "no HIP device");
+    if (device < 0 || device >= count) return fail(NIMBUS_ENODEV, "device index out of range");
+    NIMBUS_HIP_TRY(hipSetDevice(device));
+    auto ctx = std::make_unique<nimbus_hip_ctx>();
+    ctx->device = device;
+    NIMBUS_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    NIMBUS_HIP_TRY(hipEventCreate(&ctx->step_a));
+    NIMBUS_HIP_TRY(hipEventCreate(&ctx->step_b));
+    NIMBUS_HIP_TRY(hipMalloc(&ctx->diag_partials, sizeof(double) * kDiagBlocks * kDiagValues));
+    NIMBUS_HIP_TRY(hipMalloc(&ctx->diag_out, sizeof(double) * kDiagValues));
+    *out = ctx.release();
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_destroy(nimbus_hip_ctx* ctx)
+{
+    if (!ctx) return NIMBUS_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->members) free_member(*kv.second);
+    ctx->members.clear();
+    harvest(ctx);
+    for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
+    if (ctx->step_a) hipEventDestroy(ctx->step_a);
+    if (ctx->step_b) hipEventDestroy(ctx->step_b);
+    if (ctx->diag_partials) hipFree(ctx->diag_partials);
+    if (ctx->diag_out) hipFree(ctx->diag_out);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_mem_info(nimbus_hip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes)
+{
+    if (!ctx || !free_bytes || !total_bytes) return fail(NIMBUS_EINVAL, "mem_info: null argument");
+    size_t f = 0, t = 0;
+    NIMBUS_HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return NIMBUS_OK;
+}
+
+uint64_t nimbus_hip_dispatch_count(nimbus_hip_ctx* ctx) { return ctx ? ctx->launches : 0; }
+
+uint64_t nimbus_hip_member_bytes(const nimbus_member_desc* desc)
+{
+    if (!desc || !valid_grid(desc->grid)) return 0;
+    const int elem = desc->precision == NIMBUS_PRECISION_MIXED ? 4 : 8;
+    return layout_for(dev_grid(desc->grid, elem), elem, desc->config.acoustic_mode == NIMBUS_ACOUSTIC_HEVI).total;
+}
+
+int nimbus_hip_member_create(nimbus_hip_ctx* ctx, const nimbus_member_desc* desc, uint32_t* id)
+{
+    if (!ctx || !desc || !id) return fail(NIMBUS_EINVAL, "member_create: null argument");
+    if (!valid_grid(desc->grid)) return fail(NIMBUS_EINVAL, "member_create: invalid grid");
+    if (!valid_config(desc->config)) return fail(NIMBUS_EINVAL, "member_create: invalid step config");
+    if (desc->precision != NIMBUS_PRECISION_FP64 && desc->precision != NIMBUS_PRECISION_MIXED)
+        return fail(NIMBUS_EINVAL, "member_create: invalid precision");
+    if (!desc->base.rho0 || !desc->base.p0 || !desc->base.theta0)
+        return fail(NIMBUS_EINVAL, "member_create: base state needs rho0, p0, theta0");
+    for (int k = 0; k < desc->grid.nz; ++k)
+        if (!(desc->base.rho0[k] > 0.0) || !(desc->base.p0[k] > 0.0) || !std::isfinite(desc->base.rho0[k]) ||
+            !std::isfinite(desc->base.p0[k]))
+            return fail(NIMBUS_EINVAL, "member_create: base pressure and density must be positive and finite");
+    hipSetDevice(ctx->device);
+    auto m = std::make_unique<Member>();
+    const int rc = allocate_member(*desc, *m);
+    if (rc != NIMBUS_OK)
+    {
+        free_member(*m);
+        return rc;
+    }
+    *id = ctx->next_id++;
+    ctx->members[*id] = std::move(m);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_member_destroy(nimbus_hip_ctx* ctx, uint32_t id)
+{
+    if (!ctx) return fail(NIMBUS_EINVAL, "member_destroy: null context");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "member_destroy: unknown member");
+    hipStreamSynchronize(ctx->stream);
+    free_member(*m);
+    ctx->members.erase(id);
+    return NIMBUS_OK;
+}
+
+static int member_io(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s, bool up, int elem_host)
+{
+    if (!ctx || !s) return fail(NIMBUS_EINVAL, "member io: null argument");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "member io: unknown member");
+    hipSetDevice(ctx->device);
+    if (up) return elem_host == 4 ? upload_state<float>(ctx, *m, *s) : upload_state<double>(ctx, *m, *s);
+    return elem_host == 4 ? download_state<float>(ctx, *m, *s) : download_state<double>(ctx, *m, *s);
+}
+
+// The host element type follows the member precision.
+int nimbus_hip_member_upload(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s)
+{
+    Member* m = ctx ? find(ctx, id) : nullptr;
+    return member_io(ctx, id, s, true, m ? m->elem : 4);
+}
+
+int nimbus_hip_member_download(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s)
+{
+    Member* m = ctx ? find(ctx, id) : nullptr;
+    return member_io(ctx, id, s, false, m ? m->elem : 4);
+}
+
+int nimbus_hip_member_diag(nimbus_hip_ctx* ctx, uint32_t id, nimbus_diag* d)
+{
+    if (!ctx || !d) return fail(NIMBUS_EINVAL, "diag: null argument");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "diag: unknown member");
+    Launcher L = make_launcher(ctx);
+    L.timing = false;
+    if (m->elem == 4) launch_diagnostics<float>(L, m->g, m->f, ctx->diag_partials, kDiagBlocks, ctx->diag_out);
+    else launch_diagnostics<double>(L, m->g, m->f, ctx->diag_partials, kDiagBlocks, ctx->diag_out);
+    if (L.error != hipSuccess) return hip_fail(L.error, "diag launch");
+    double v[kDiagValues];
+    NIMBUS_HIP_TRY(hipMemcpyAsync(v, ctx->diag_out, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const double cells = double(m->g.nx - 2) * double(m->g.ny - 2) * double(m->g.nz);
+    d->max_w = v[0];
+    d->min_w = v[1];
+    d->rms_theta_p = std::sqrt(v[2] / cells);
+    d->rain_total_mm = v[3];
+    d->max_rain_mm = v[6];
+    d->mean_qc = v[4] / cells;
+    d->mean_qr = v[5] / cells;
+    d->steps = m->steps;
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_step(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_steps)
+{
+    if (ctx) hipSetDevice(ctx->device);
+    return step_members(ctx, ids, n_ids, n_steps, true);
+}
+
+int nimbus_hip_step_async(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_steps)
+{
+    if (ctx) hipSetDevice(ctx->device);
+    return step_members(ctx, ids, n_ids, n_steps, false);
+}
+
+int nimbus_hip_synchronize(nimbus_hip_ctx* ctx)
+{
+    if (!ctx) return fail(NIMBUS_EINVAL, "synchronize: null context");
+    NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->step_a, ctx->step_b) == hipSuccess) ctx->last_step_ms = ms;
+    harvest(ctx);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_set_graphs(nimbus_hip_ctx* ctx, int enable)
+{
+    if (!ctx) return fail(NIMBUS_EINVAL, "set_graphs: null context");
+    ctx->graphs = enable != 0;
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_set_kernel_timing(nimbus_hip_ctx* ctx, int enable)
+{
+    if (!ctx) return fail(NIMBUS_EINVAL, "set_kernel_timing: null context");
+    ctx->timing = enable != 0;
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_kernel_timing(nimbus_hip_ctx* ctx, int max_rows, char (*names)[48], uint64_t* launches,
+                             double* total_ms)
+{
+    if (!ctx || max_rows < 0) return fail(NIMBUS_EINVAL, "kernel_timing: bad arguments");
+    int r = 0;
+    for (auto& kv : ctx->table)
+    {
+        if (r >= max_rows) break;
+        if (names)
+        {
+            std::snprintf(names[r], 48, "%s", kv.first.c_str());
+        }
+        if (launches) launches[r] = kv.second.launches;
+        if (total_ms) total_ms[r] = kv.second.ms;
+        ++r;
+    }
+    return r;
+}
+
+int nimbus_hip_reset_kernel_timing(nimbus_hip_ctx* ctx)
+{
+    if (!ctx) return fail(NIMBUS_EINVAL, "reset_kernel_timing: null context");
+    hipStreamSynchronize(ctx->stream);
+    harvest(ctx);
+    ctx->table.clear();
+    return NIMBUS_OK;
+}
+
+double nimbus_hip_last_step_ms(nimbus_hip_ctx* ctx) { return ctx ? ctx->last_step_ms : 0.0; }
+
+// ---------------- per-operation dispatch ----------------
+
+int nimbus_hip_dispatch_acoustic_substep_fused(nimbus_hip_ctx* ctx, const nimbus_acoustic_substep_fused_dispatch* d)
+{
+    // Validation mirrors valid_acoustic_cgrid_dispatch (src/compute/cpu/acoustic_cgrid.cpp:11-46).
+    if (!ctx || !d) return fail(NIMBUS_EINVAL, "acoustic: null argument");
+    auto pos = [](double x) { return std::isfinite(x) && x > 0.0; };
+    auto bounds_ok = [](float lo, float hi) { return std::isfinite(lo) && std::isfinite(hi) && lo > 0.0f && lo <= hi; };
+    if (!all_nonnull<const void*>({d->u, d->v, d->w, d->rho, d->p, d->theta_data, d->rho0_base_host, d->p0_base_host}) ||
+        d->nx < 5 || d->ny < 5 || d->nz < 4 || d->base_len < d->nz || !pos(d->dx_host) || !pos(d->dy_host) ||
+        !pos(d->dz_host) || !pos(d->dt_small_host) || !pos(d->gamma_host) || !bounds_ok(d->rho_min, d->rho_max) ||
+        !bounds_ok(d->p_min, d->p_max) || !pos(d->p_ref_fallback) || !(d->wind_clamp_h >= 0.0f) ||
+        !(d->wind_clamp_v >= 0.0f) || std::fabs(d->gamma_host - kGamma) > 1e-12)
+        return fail(NIMBUS_EINVAL, "acoustic: invalid dispatch");
+    hipSetDevice(ctx->device);
+    ScratchMember sm{};
+    int rc = make_scratch(ctx, sm, d->nx, d->ny, d->nz, d->dx_host, d->dy_host, d->dz_host, d->rho0_base_host,
+                          d->p0_base_host, nullptr, nullptr, nullptr, nullptr, NIMBUS_ACOUSTIC_EXPLICIT);
+    if (rc != NIMBUS_OK) return rc;
+    Member& m = sm.m;
+    m.params.bounds = Bounds{d->rho_min, d->rho_max, d->p_min, d->p_max, d->p_ref_fallback, d->wind_clamp_h,
+                             d->wind_clamp_v};
+    std::vector<float> zero(std::size_t(d->nx) * d->ny * d->nz, 0.0f);
+    nimbus_host_state hs{d->u, d->v, d->w, d->rho, d->p, const_cast<float*>(d->theta_data), zero.data(), zero.data(),
+                         zero.data(), nullptr};
+    rc = upload_state<float>(ctx, m, hs);
+    if (rc != NIMBUS_OK) return rc;
+    Launcher L = make_launcher(ctx);
+    launch_acoustic_substep<float>(L, m.g, m.f, m.params, d->dt_small_host, 0.0, false, false);
+    if (L.error != hipSuccess) return hip_fail(L.error, "acoustic launch");
+    nimbus_host_state out{d->u, d->v, d->w, d->rho, d->p, nullptr, nullptr, nullptr, nullptr, nullptr};
+    rc = download_state<float>(ctx, m, out);
+    harvest(ctx);
+    return rc;
+}
+
+int nimbus_hip_dispatch_acoustic_substeps_batched(nimbus_hip_ctx* ctx,
+                                                  const nimbus_acoustic_substeps_batched_dispatch* d)
+{
+    if (!ctx || !d) return fail(NIMBUS_EINVAL, "batched acoustic: null argument");
+    if (!all_nonnull<const void*>({d->u, d->v, d->w, d->rho, d->p, d->theta_data, d->rho0_base_host, d->p0_base_host}) ||
+        d->n_substeps < 1 || !(d->dt_small > 0.0) ||
+        (d->accumulate_mass_flux && !all_nonnull<const void*>({d->mass_flux_x, d->mass_flux_y, d->mass_flux_z})))
+        return fail(NIMBUS_EINVAL, "batched acoustic: invalid dispatch");
+    nimbus_step_config cfg{};
+    cfg.dt = d->dt_small * d->n_substeps;
+    cfg.acoustic_mode = d->acoustic_mode;
+    cfg.hevi_beta = d->hevi_beta;
+    cfg.bc_west = d->bc_west; cfg.bc_east = d->bc_east; cfg.bc_south = d->bc_south; cfg.bc_north = d->bc_north;
+    cfg.bc_bottom = d->bc_bottom; cfg.bc_top = d->bc_top;
+    cfg.vt_max = 20.0;
+    if (!valid_config(cfg)) return fail(NIMBUS_EINVAL, "batched acoustic: invalid boundary or mode");
+    hipSetDevice(ctx->device);
+    ScratchMember sm{};
+    int rc = make_scratch(ctx, sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->rho0_base_host, d->p0_base_host,
+                          nullptr, nullptr, nullptr, nullptr, d->acoustic_mode);
+    if (rc != NIMBUS_OK) return rc;
+    Member& m = sm.m;
+    StepParams& p = m.params;
+    p.acoustic_mode = d->acoustic_mode;
+    p.hevi_beta = d->hevi_beta;
+    p.bc_west = d->bc_west; p.bc_east = d->bc_east; p.bc_south = d->bc_south; p.bc_north = d->bc_north;
+    p.bc_bottom = d->bc_bottom; p.bc_top = d->bc_top;
+    std::vector<float> zero(std::size_t(d->nx) * d->ny * d->nz, 0.0f);
+    nimbus_host_state hs{d->u, d->v, d->w, d->rho, d->p, const_cast<float*>(d->theta_data), zero.data(), zero.data(),
+                         zero.data(), nullptr};
+    rc = upload_state<float>(ctx, m, hs);
+    if (rc != NIMBUS_OK) return rc;
+    Staging st;
+    const std::size_t n = zero.size();
+    NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(float)));
+    if (d->accumulate_mass_flux)
+    {
+        const void* src[3] = {d->mass_flux_x, d->mass_flux_y, d->mass_flux_z};
+        void* dst[3] = {m.f.mfx, m.f.mfy, m.f.mfz};
+        for (int q = 0; q < 3; ++q)
+        {
+            rc = upload_field<float>(ctx, m, src[q], dst[q], nullptr, st.p);
+            if (rc != NIMBUS_OK) return rc;
+            NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+    }
+    Launcher L = make_launcher(ctx);
+    launch_seed_radiation<float>(L, m.g, m.f);
+    for (int s = 0; s < d->n_substeps; ++s)
+    {
+        launch_acoustic_substep<float>(L, m.g, m.f, p, d->dt_small, d->mass_flux_weight, false,
+                                       d->accumulate_mass_flux != 0);
+        launch_boundaries<float>(L, m.g, m.f, p, d->dt_small);
+    }
+    if (L.error != hipSuccess) return hip_fail(L.error, "batched acoustic launch");
+    nimbus_host_state out{d->u, d->v, d->w, d->rho, d->p, nullptr, nullptr, nullptr, nullptr, nullptr};
+    rc = download_state<float>(ctx, m, out);
+    if (rc != NIMBUS_OK) return rc;
+    if (d->accumulate_mass_flux)
+    {
+        void* dst[3] = {d->mass_flux_x, d->mass_flux_y, d->mass_flux_z};
+        const void* src[3] = {m.f.mfx, m.f.mfy, m.f.mfz};
+        for (int q = 0; q < 3; ++q)
+        {
+            rc = download_field<float>(ctx, m, src[q], dst[q], nullptr, st.p);
+            if (rc != NIMBUS_OK) return rc;
+        }
+    }
+    harvest(ctx);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_dispatch_slow_tendencies(nimbus_hip_ctx* ctx, const nimbus_slow_tendencies_dispatch* d)
+{
+    if (!ctx || !d) return fail(NIMBUS_EINVAL, "slow tendencies: null argument");
+    if (!all_nonnull<const void*>({d->u_data, d->v_data, d->w_data, d->rho_data, d->qv_data, d->qc_data, d->qr_data,
+                                   d->rho0_base, d->du_dt_data, d->dv_dt_data, d->dw_dt_data}))
+        return fail(NIMBUS_EINVAL, "slow tendencies: null field");
+    hipSetDevice(ctx->device);
+    std::vector<double> p0(d->nz, kP00);
+    ScratchMember sm{};
+    int rc = make_scratch(ctx, sm, d->nr, d->nth, d->nz, d->dx, d->dy, d->dz, d->rho0_base, p0.data(), nullptr,
+                          d->qv0_base, d->u0_base, d->v0_base, NIMBUS_ACOUSTIC_EXPLICIT);
+    if (rc != NIMBUS_OK) return rc;
+    Member& m = sm.m;
+    m.params.coriolis_f = d->coriolis_f;
+    const std::size_t n = std::size_t(d->nr) * d->nth * d->nz;
+    std::vector<float> zero(n, 0.0f);
+    nimbus_host_state hs{const_cast<float*>(d->u_data), const_cast<float*>(d->v_data), const_cast<float*>(d->w_data),
+                         const_cast<float*>(d->rho_data), zero.data(), zero.data(), const_cast<float*>(d->qv_data),
+                         const_cast<float*>(d->qc_data), const_cast<float*>(d->qr_data), nullptr};
+    rc = upload_state<float>(ctx, m, hs);
+    if (rc != NIMBUS_OK) return rc;
+    Launcher L = make_launcher(ctx);
+    launch_slow_tendencies<float>(L, m.g, m.f, m.params, m.f.k1u, m.f.k1v, m.f.k1w);
+    if (L.error != hipSuccess) return hip_fail(L.error, "slow tendencies launch");
+    Staging st;
+    NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(float)));
+    void* outs[3] = {d->du_dt_data, d->dv_dt_data, d->dw_dt_data};
+    void* srcs[3] = {m.f.k1u, m.f.k1v, m.f.k1w};
+    for (int q = 0; q < 3; ++q)
+    {
+        rc = download_field<float>(ctx, m, srcs[q], outs[q], nullptr, st.p);
+        if (rc != NIMBUS_OK) return rc;
+    }
+    harvest(ctx);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_dispatch_slow_rk2(nimbus_hip_ctx* ctx, const nimbus_slow_rk2_dispatch* d)
+{
+    if (!ctx || !d) return fail(NIMBUS_EINVAL, "slow rk2: null argument");
+    if (!all_nonnull<const void*>({d->u, d->v, d->w, d->rho, d->qv, d->qc, d->qr, d->rho0_base}) || !(d->dt_large > 0))
+        return fail(NIMBUS_EINVAL, "slow rk2: invalid dispatch");
+    hipSetDevice(ctx->device);
+    std::vector<double> p0(d->nz, kP00);
+    ScratchMember sm{};
+    int rc = make_scratch(ctx, sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->rho0_base, p0.data(), nullptr,
+                          d->qv0_base, d->u0_base, d->v0_base, NIMBUS_ACOUSTIC_EXPLICIT);
+    if (rc != NIMBUS_OK) return rc;
+    Member& m = sm.m;
+    m.params.coriolis_f = d->coriolis_f;
+    m.params.dt = d->dt_large;
+    const std::size_t n = std::size_t(d->nx) * d->ny * d->nz;
+    std::vector<float> zero(n, 0.0f);
+    nimbus_host_state hs{d->u, d->v, d->w, const_cast<float*>(d->rho), zero.data(), zero.data(),
+                         const_cast<float*>(d->qv), const_cast<float*>(d->qc), const_cast<float*>(d->qr), nullptr};
+    rc = upload_state<float>(ctx, m, hs);
+    if (rc != NIMBUS_OK) return rc;
+    Launcher L = make_launcher(ctx);
+    launch_slow_rk2<float>(L, m.g, m.f, m.params);
+    if (L.error != hipSuccess) return hip_fail(L.error, "slow rk2 launch");
+    nimbus_host_state out{d->u, d->v, d->w, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    rc = download_state<float>(ctx, m, out);
+    harvest(ctx);
+    return rc;
+}
+
+int nimbus_hip_dispatch_advect_scalars(nimbus_hip_ctx* ctx, const nimbus_advect_scalars_dispatch* d)
+{
+    if (!ctx || !d) return fail(NIMBUS_EINVAL, "advect: null argument");
+    if (d->n_scalars < 0 || d->n_scalars > 4 ||
+        !all_nonnull<const void*>({d->mass_flux_x, d->mass_flux_y, d->mass_flux_z, d->rho_begin, d->rho_end}) ||
+        !(d->dt > 0.0) || (d->bc_top != NIMBUS_TOP_RIGID && d->bc_top != NIMBUS_TOP_OPEN))
+        return fail(NIMBUS_EINVAL, "advect: invalid dispatch");
+    int npd = 0;
+    for (int s = 0; s < d->n_scalars; ++s)
+    {
+        if (!d->q[s]) return fail(NIMBUS_EINVAL, "advect: null scalar");
+        npd += d->positive_definite[s] != 0;
+    }
+    if (npd > 3) return fail(NIMBUS_EINVAL, "advect: at most three positive-definite scalars");
+    if (d->n_scalars == 0) return NIMBUS_OK;
+    hipSetDevice(ctx->device);
+    std::vector<double> rho0(d->nz, 0.0), p0(d->nz, kP00);
+    ScratchMember sm{};
+    int rc = make_scratch(ctx, sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, rho0.data(), p0.data(), nullptr,
+                          nullptr, nullptr, nullptr, NIMBUS_ACOUSTIC_EXPLICIT);
+    if (rc == NIMBUS_EINVAL) return rc;
+    if (rc != NIMBUS_OK) return rc;
+    Member& m = sm.m;
+    m.params.dt = d->dt;
+    m.params.bc_top = d->bc_top;
+    const std::size_t n = std::size_t(d->nx) * d->ny * d->nz;
+    Staging st;
+    NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(float)));
+    // rho0 = 0 in the scratch member, so rho' slots hold the full densities.
+    const void* src[5] = {d->mass_flux_x, d->mass_flux_y, d->mass_flux_z, d->rho_begin, d->rho_end};
+    void* dst[5] = {m.f.mfx, m.f.mfy, m.f.mfz, m.f.rb, m.f.rp};
+    for (int q = 0; q < 5; ++q)
+    {
+        rc = upload_field<float>(ctx, m, src[q], dst[q], nullptr, st.p);
+        if (rc != NIMBUS_OK) return rc;
+        NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    void* qin[4] = {m.f.th, m.f.qv, m.f.qc, m.f.qr};
+    void* qout[4] = {m.f.tth, m.f.tqv, m.f.tqc, m.f.tqr};
+    int pd[4] = {0, 0, 0, 0};
+    for (int s = 0; s < d->n_scalars; ++s)
+    {
+        rc = upload_field<float>(ctx, m, d->q[s], qin[s], nullptr, st.p);
+        if (rc != NIMBUS_OK) return rc;
+        NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        pd[s] = d->positive_definite[s] != 0;
+    }
+    Launcher L = make_launcher(ctx);
+    launch_transport<float>(L, m.g, m.f, m.params, d->n_scalars, qin, qout, pd);
+    if (L.error != hipSuccess) return hip_fail(L.error, "advect launch");
+    for (int s = 0; s < d->n_scalars; ++s)
+    {
+        rc = download_field<float>(ctx, m, qout[s], d->q[s], nullptr, st.p);
+        if (rc != NIMBUS_OK) return rc;
+    }
+    harvest(ctx);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_dispatch_kessler(nimbus_hip_ctx* ctx, const nimbus_kessler_dispatch* d)
+{
+    if (!ctx || !d) return fail(NIMBUS_EINVAL, "kessler: null argument");
+    if (!all_nonnull<const void*>({d->p, d->theta, d->rho, d->qv, d->qc, d->qr, d->dtheta_dt, d->dqv_dt, d->dqc_dt,
+                                   d->dqr_dt}) ||
+        !(d->dt > 0.0) || !(d->dz > 0.0) || !(d->vt_max > 0.0))
+        return fail(NIMBUS_EINVAL, "kessler: invalid dispatch");
+    hipSetDevice(ctx->device);
+    std::vector<double> rho0(d->nz, 0.0), p0(d->nz, 0.0);
+    ScratchMember sm{};
+    // p0 = rho0 = 0: the perturbation slots carry the full fields.
+    nimbus_member_desc desc{};
+    desc.grid = nimbus_grid{d->nx, d->ny, d->nz, 1.0, 1.0, d->dz};
+    if (!valid_grid(desc.grid)) return fail(NIMBUS_EINVAL, "kessler: invalid grid");
+    desc.config.dt = d->dt;
+    desc.config.vt_max = d->vt_max;
+    desc.precision = NIMBUS_PRECISION_MIXED;
+    std::vector<double> one(d->nz, 1.0);
+    desc.base = nimbus_base_state{one.data(), one.data(), nullptr, nullptr, nullptr, nullptr};
+    sm.ctx = ctx;
+    int rc = allocate_member(desc, sm.m);
+    if (rc != NIMBUS_OK) return rc;
+    Member& m = sm.m;
+    NIMBUS_HIP_TRY(hipMemset(m.f.base, 0, sizeof(double) * kBaseCount * d->nz));
+    m.params.kessler = KesslerConst{d->qc0, d->c_auto, d->c_accr, d->evap_eff, d->vt_max};
+    const std::size_t n = std::size_t(d->nx) * d->ny * d->nz;
+    std::vector<float> zero(n, 0.0f);
+    nimbus_host_state hs{zero.data(), zero.data(), zero.data(), const_cast<float*>(d->rho), const_cast<float*>(d->p),
+                         const_cast<float*>(d->theta), const_cast<float*>(d->qv), const_cast<float*>(d->qc),
+                         const_cast<float*>(d->qr), nullptr};
+    // upload_state subtracts base slots that are zero here
+    rc = upload_state<float>(ctx, m, hs);
+    if (rc != NIMBUS_OK) return rc;
+    Launcher L = make_launcher(ctx);
+    launch_kessler_tendencies<float>(L, m.g, m.f, m.params, d->dt, m.f.k1u, m.f.k1v, m.f.k1w, m.f.su);
+    if (L.error != hipSuccess) return hip_fail(L.error, "kessler launch");
+    Staging st;
+    NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(float)));
+    void* outs[4] = {d->dtheta_dt, d->dqv_dt, d->dqc_dt, d->dqr_dt};
+    void* srcs[4] = {m.f.k1u, m.f.k1v, m.f.k1w, m.f.su};
+    for (int q = 0; q < 4; ++q)
+    {
+        rc = download_field<float>(ctx, m, srcs[q], outs[q], nullptr, st.p);
+        if (rc != NIMBUS_OK) return rc;
+    }
+    harvest(ctx);
+    return NIMBUS_OK;
+}
+
+}  // extern "C"

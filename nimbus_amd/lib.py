@@ -1,0 +1,395 @@
+"""ctypes binding of libnimbus_hip.so (the C ABI in include/nimbus_hip.h).
+
+Host arrays use the reference layout: numpy arrays of shape (nx, ny, nz) in C
+order, i.e. index (i*ny + j)*nz + k with k contiguous
+(include/core/field/field3d.hpp:543-574 of the Nimbus reference).
+
+The library is loaded from the package directory only; a missing or
+unloadable library raises, there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnimbus_hip.so")
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+ACOUSTIC_EXPLICIT = 0
+ACOUSTIC_HEVI = 1
+BC_PERIODIC, BC_RADIATIVE, BC_ZERO_GRADIENT, BC_RIGID, BC_RIGID_SLIP = 0, 1, 2, 3, 4
+BOTTOM_FREE_SLIP, BOTTOM_NO_SLIP = 0, 1
+TOP_RIGID, TOP_OPEN = 0, 1
+
+FIELDS = ("u", "v", "w", "rho", "p", "theta", "qv", "qc", "qr")
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class NimbusError(RuntimeError):
+    pass
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
+
+
+class StepConfig(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("substeps", ctypes.c_int), ("acoustic_mode", ctypes.c_int),
+                ("hevi_beta", ctypes.c_double), ("coriolis_f", ctypes.c_double),
+                ("bc_west", ctypes.c_int), ("bc_east", ctypes.c_int), ("bc_south", ctypes.c_int),
+                ("bc_north", ctypes.c_int), ("bc_bottom", ctypes.c_int), ("bc_top", ctypes.c_int),
+                ("qc0", ctypes.c_double), ("c_auto", ctypes.c_double), ("c_accr", ctypes.c_double),
+                ("evap_eff", ctypes.c_double), ("vt_max", ctypes.c_double)]
+
+
+class BaseState(ctypes.Structure):
+    _fields_ = [(n, _c_double_p) for n in ("rho0", "p0", "theta0", "qv0", "u0", "v0")]
+
+
+class HostState(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in FIELDS] + [("rain_mm", _c_double_p)]
+
+
+class Sounding(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "theta_sfc", "theta_tropo", "z_tropo", "T_tropo", "rh_sfc", "rh_top", "qv_cap", "p_sfc",
+        "shear", "shear_depth", "curve_depth", "curvature")]
+
+
+class Bubble(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("xc", "yc", "zc", "rh", "rz", "dtheta")]
+
+
+class ICParams(ctypes.Structure):
+    _fields_ = [("sounding", Sounding), ("n_bubbles", ctypes.c_int), ("bubbles", Bubble * 4),
+                ("noise_amp", ctypes.c_double), ("seed", ctypes.c_uint64), ("member", ctypes.c_uint32)]
+
+
+class MemberDesc(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("config", StepConfig), ("precision", ctypes.c_int), ("base", BaseState)]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "max_w", "min_w", "rms_theta_p", "rain_total_mm", "max_rain_mm", "mean_qc", "mean_qr")] + [
+        ("steps", ctypes.c_int64)]
+
+
+class AcousticFusedDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("u", "v", "w", "rho", "p")] +
+                [("theta_data", _c_float_p), ("rho0_base_host", _c_double_p), ("p0_base_host", _c_double_p),
+                 ("base_len", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)] +
+                [(n, ctypes.c_double) for n in ("dx_host", "dy_host", "dz_host", "dt_small_host", "gamma_host")] +
+                [(n, ctypes.c_float) for n in ("rho_min", "rho_max", "p_min", "p_max", "p_ref_fallback",
+                                               "wind_clamp_h", "wind_clamp_v")])
+
+
+class AcousticBatchedDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("u", "v", "w", "rho", "p")] +
+                [("theta_data", _c_float_p), ("rho0_base_host", _c_double_p), ("p0_base_host", _c_double_p),
+                 ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)] +
+                [(n, ctypes.c_double) for n in ("dx", "dy", "dz", "dt_small")] +
+                [("n_substeps", ctypes.c_int), ("acoustic_mode", ctypes.c_int), ("hevi_beta", ctypes.c_double)] +
+                [(n, ctypes.c_int) for n in ("bc_west", "bc_east", "bc_south", "bc_north", "bc_bottom", "bc_top")] +
+                [(n, _c_float_p) for n in ("mass_flux_x", "mass_flux_y", "mass_flux_z")] +
+                [("accumulate_mass_flux", ctypes.c_int), ("mass_flux_weight", ctypes.c_double)])
+
+
+class SlowTendenciesDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("u_data", "v_data", "w_data", "rho_data", "qv_data", "qc_data",
+                                          "qr_data")] +
+                [(n, _c_double_p) for n in ("rho0_base", "qv0_base", "u0_base", "v0_base")] +
+                [(n, _c_float_p) for n in ("du_dt_data", "dv_dt_data", "dw_dt_data")] +
+                [("nr", ctypes.c_int), ("nth", ctypes.c_int), ("nz", ctypes.c_int)] +
+                [(n, ctypes.c_double) for n in ("dx", "dy", "dz", "coriolis_f")])
+
+
+class SlowRk2Dispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("u", "v", "w", "rho", "qv", "qc", "qr")] +
+                [(n, _c_double_p) for n in ("rho0_base", "qv0_base", "u0_base", "v0_base")] +
+                [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)] +
+                [(n, ctypes.c_double) for n in ("dx", "dy", "dz", "dt_large", "coriolis_f")])
+
+
+class AdvectScalarsDispatch(ctypes.Structure):
+    _fields_ = [("n_scalars", ctypes.c_int), ("q", _c_float_p * 4), ("positive_definite", ctypes.c_int * 4),
+                ("mass_flux_x", _c_float_p), ("mass_flux_y", _c_float_p), ("mass_flux_z", _c_float_p),
+                ("rho_begin", _c_float_p), ("rho_end", _c_float_p),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
+                ("bc_top", ctypes.c_int)]
+
+
+class KesslerDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("p", "theta", "rho", "qv", "qc", "qr")] +
+                [(n, _c_float_p) for n in ("dtheta_dt", "dqv_dt", "dqc_dt", "dqr_dt")] +
+                [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)] +
+                [(n, ctypes.c_double) for n in ("dz", "dt", "qc0", "c_auto", "c_accr", "evap_eff", "vt_max")])
+
+
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libnimbus_hip.so and declares argument types. Raises if absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise NimbusError(f"{path} is missing: build it with __graft_entry__.build() "
+                          f"(python -m nimbus_amd._build)")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    decl = {
+        "nimbus_hip_last_error": (ctypes.c_char_p, []),
+        "nimbus_hip_create": (ctypes.c_int, [ctypes.c_int, P(vp)]),
+        "nimbus_hip_destroy": (ctypes.c_int, [vp]),
+        "nimbus_hip_mem_info": (ctypes.c_int, [vp, P(ctypes.c_uint64), P(ctypes.c_uint64)]),
+        "nimbus_hip_dispatch_count": (ctypes.c_uint64, [vp]),
+        "nimbus_default_sounding": (None, [P(Sounding)]),
+        "nimbus_ic_base_state": (ctypes.c_int, [P(Grid), P(Sounding)] + [_c_double_p] * 6),
+        "nimbus_ic_fill_f32": (ctypes.c_int, [P(Grid), P(ICParams), P(HostState)]),
+        "nimbus_ic_fill_f64": (ctypes.c_int, [P(Grid), P(ICParams), P(HostState)]),
+        "nimbus_ic_uniform": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int]),
+        "nimbus_resolve_substeps": (ctypes.c_int, [P(Grid), P(BaseState), P(StepConfig)]),
+        "nimbus_hip_member_create": (ctypes.c_int, [vp, P(MemberDesc), P(ctypes.c_uint32)]),
+        "nimbus_hip_member_destroy": (ctypes.c_int, [vp, ctypes.c_uint32]),
+        "nimbus_hip_member_bytes": (ctypes.c_uint64, [P(MemberDesc)]),
+        "nimbus_hip_member_upload": (ctypes.c_int, [vp, ctypes.c_uint32, P(HostState)]),
+        "nimbus_hip_member_download": (ctypes.c_int, [vp, ctypes.c_uint32, P(HostState)]),
+        "nimbus_hip_member_diag": (ctypes.c_int, [vp, ctypes.c_uint32, P(Diag)]),
+        "nimbus_hip_step": (ctypes.c_int, [vp, P(ctypes.c_uint32), ctypes.c_int, ctypes.c_int]),
+        "nimbus_hip_step_async": (ctypes.c_int, [vp, P(ctypes.c_uint32), ctypes.c_int, ctypes.c_int]),
+        "nimbus_hip_synchronize": (ctypes.c_int, [vp]),
+        "nimbus_hip_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
+        "nimbus_hip_set_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+        "nimbus_hip_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int, vp, P(ctypes.c_uint64),
+                                                    P(ctypes.c_double)]),
+        "nimbus_hip_reset_kernel_timing": (ctypes.c_int, [vp]),
+        "nimbus_hip_last_step_ms": (ctypes.c_double, [vp]),
+        "nimbus_hip_dispatch_acoustic_substep_fused": (ctypes.c_int, [vp, P(AcousticFusedDispatch)]),
+        "nimbus_hip_dispatch_acoustic_substeps_batched": (ctypes.c_int, [vp, P(AcousticBatchedDispatch)]),
+        "nimbus_hip_dispatch_slow_tendencies": (ctypes.c_int, [vp, P(SlowTendenciesDispatch)]),
+        "nimbus_hip_dispatch_slow_rk2": (ctypes.c_int, [vp, P(SlowRk2Dispatch)]),
+        "nimbus_hip_dispatch_advect_scalars": (ctypes.c_int, [vp, P(AdvectScalarsDispatch)]),
+        "nimbus_hip_dispatch_kessler": (ctypes.c_int, [vp, P(KesslerDispatch)]),
+    }
+    for name, (res, args) in decl.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = load_library().nimbus_hip_last_error().decode()
+        raise NimbusError(f"{what} failed ({rc}): {msg}")
+
+
+def fptr(a):
+    return a.ctypes.data_as(_c_float_p)
+
+
+def dptr(a):
+    return a.ctypes.data_as(_c_double_p)
+
+
+# ---------------------------------------------------------------- host-side helpers
+
+def default_sounding(**overrides):
+    s = Sounding()
+    load_library().nimbus_default_sounding(ctypes.byref(s))
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    return s
+
+
+def make_grid(nx, ny, nz, dx, dy, dz):
+    return Grid(int(nx), int(ny), int(nz), float(dx), float(dy), float(dz))
+
+
+def step_config(dt, substeps, acoustic_mode=ACOUSTIC_EXPLICIT, hevi_beta=0.1, coriolis_f=0.0,
+                lateral=BC_RADIATIVE, bottom=BOTTOM_FREE_SLIP, top=TOP_RIGID,
+                qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0):
+    lat = lateral if isinstance(lateral, (tuple, list)) else (lateral,) * 4
+    return StepConfig(float(dt), int(substeps), int(acoustic_mode), float(hevi_beta), float(coriolis_f),
+                      lat[0], lat[1], lat[2], lat[3], int(bottom), int(top),
+                      qc0, c_auto, c_accr, evap_eff, vt_max)
+
+
+def base_state(grid, sounding=None):
+    """FP64 base-state profiles (dict of numpy arrays, each nz long)."""
+    lib = load_library()
+    s = sounding if sounding is not None else default_sounding()
+    out = {n: np.zeros(grid.nz) for n in ("rho0", "p0", "theta0", "qv0", "u0", "v0")}
+    check(lib.nimbus_ic_base_state(ctypes.byref(grid), ctypes.byref(s),
+                                   *[dptr(out[n]) for n in ("rho0", "p0", "theta0", "qv0", "u0", "v0")]),
+          "nimbus_ic_base_state")
+    return out
+
+
+def base_struct(base):
+    return BaseState(*[dptr(base[n]) for n in ("rho0", "p0", "theta0", "qv0", "u0", "v0")])
+
+
+def ic_params(grid, sounding=None, bubbles=None, noise_amp=0.1, seed=0, member=0):
+    """Initial-condition parameters; the default is one +2 K bubble of 10 km x 1.4 km
+    radius, centred 1.4 km above the domain centre."""
+    p = ICParams()
+    p.sounding = sounding if sounding is not None else default_sounding()
+    if bubbles is None:
+        xc = (grid.nx - 2) * grid.dx / 2.0
+        yc = (grid.ny - 2) * grid.dy / 2.0
+        bubbles = [(xc, yc, 1400.0, 10000.0, 1400.0, 2.0)]
+    p.n_bubbles = len(bubbles)
+    for q, b in enumerate(bubbles):
+        p.bubbles[q] = Bubble(*b)
+    p.noise_amp = noise_amp
+    p.seed = seed
+    p.member = member
+    return p
+
+
+def new_state(grid, dtype):
+    st = {n: np.zeros((grid.nx, grid.ny, grid.nz), dtype=dtype) for n in FIELDS}
+    st["rain_mm"] = np.zeros((grid.nx, grid.ny), dtype=np.float64)
+    return st
+
+
+def host_state_struct(st):
+    return HostState(*[st[n].ctypes.data_as(ctypes.c_void_p) if st.get(n) is not None else None for n in FIELDS],
+                     dptr(st["rain_mm"]) if st.get("rain_mm") is not None else None)
+
+
+def initial_state(grid, params, dtype=np.float32):
+    """Seeded initial state in the reference host layout."""
+    lib = load_library()
+    st = new_state(grid, dtype)
+    hs = host_state_struct(st)
+    fill = lib.nimbus_ic_fill_f32 if np.dtype(dtype) == np.float32 else lib.nimbus_ic_fill_f64
+    check(fill(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(hs)), "nimbus_ic_fill")
+    return st
+
+
+def resolve_substeps(grid, base, config):
+    return load_library().nimbus_resolve_substeps(ctypes.byref(grid), ctypes.byref(base_struct(base)),
+                                                  ctypes.byref(config))
+
+
+# ---------------------------------------------------------------- device context
+
+class Context:
+    """One HIP device context (one per host thread)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self._h = ctypes.c_void_p()
+        check(self.lib.nimbus_hip_create(int(device), ctypes.byref(self._h)), "nimbus_hip_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            self.lib.nimbus_hip_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mem_info(self):
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        check(self.lib.nimbus_hip_mem_info(self._h, ctypes.byref(f), ctypes.byref(t)), "mem_info")
+        return f.value, t.value
+
+    def member(self, grid, config, base, precision=PRECISION_MIXED):
+        return Member(self, grid, config, base, precision)
+
+    def step(self, members, n_steps=1, wait=True):
+        ids = (ctypes.c_uint32 * len(members))(*[m.id for m in members])
+        fn = self.lib.nimbus_hip_step if wait else self.lib.nimbus_hip_step_async
+        check(fn(self._h, ids, len(members), int(n_steps)), "nimbus_hip_step")
+
+    def synchronize(self):
+        check(self.lib.nimbus_hip_synchronize(self._h), "synchronize")
+
+    def set_graphs(self, enable):
+        check(self.lib.nimbus_hip_set_graphs(self._h, int(bool(enable))), "set_graphs")
+
+    def set_kernel_timing(self, enable):
+        check(self.lib.nimbus_hip_set_kernel_timing(self._h, int(bool(enable))), "set_kernel_timing")
+
+    def reset_kernel_timing(self):
+        check(self.lib.nimbus_hip_reset_kernel_timing(self._h), "reset_kernel_timing")
+
+    def kernel_timing(self, max_rows=64):
+        names = (ctypes.c_char * 48 * max_rows)()
+        launches = (ctypes.c_uint64 * max_rows)()
+        ms = (ctypes.c_double * max_rows)()
+        n = self.lib.nimbus_hip_kernel_timing(self._h, max_rows, ctypes.cast(names, ctypes.c_void_p), launches, ms)
+        check(0 if n >= 0 else n, "kernel_timing")
+        return {bytes(names[r]).split(b"\0")[0].decode(): (launches[r], ms[r]) for r in range(n)}
+
+    def last_step_ms(self):
+        return self.lib.nimbus_hip_last_step_ms(self._h)
+
+    def dispatch_count(self):
+        return self.lib.nimbus_hip_dispatch_count(self._h)
+
+
+class Member:
+    """A device-resident ensemble member."""
+
+    def __init__(self, ctx, grid, config, base, precision):
+        self.ctx = ctx
+        self.grid = grid
+        self.config = config
+        self.precision = precision
+        self.dtype = np.float32 if precision == PRECISION_MIXED else np.float64
+        self._base = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in base.items()}
+        desc = MemberDesc(grid, config, precision, base_struct(self._base))
+        self.bytes = ctx.lib.nimbus_hip_member_bytes(ctypes.byref(desc))
+        mid = ctypes.c_uint32()
+        check(ctx.lib.nimbus_hip_member_create(ctx.handle, ctypes.byref(desc), ctypes.byref(mid)), "member_create")
+        self.id = mid.value
+        self.substeps = resolve_substeps(grid, self._base, config)
+
+    def upload(self, st):
+        st = {k: (np.ascontiguousarray(v, dtype=self.dtype) if k in FIELDS else
+                  (np.ascontiguousarray(v, dtype=np.float64) if v is not None else None)) for k, v in st.items()}
+        hs = host_state_struct(st)
+        check(self.ctx.lib.nimbus_hip_member_upload(self.ctx.handle, self.id, ctypes.byref(hs)), "upload")
+
+    def download(self):
+        st = new_state(self.grid, self.dtype)
+        hs = host_state_struct(st)
+        check(self.ctx.lib.nimbus_hip_member_download(self.ctx.handle, self.id, ctypes.byref(hs)), "download")
+        return st
+
+    def diag(self):
+        d = Diag()
+        check(self.ctx.lib.nimbus_hip_member_diag(self.ctx.handle, self.id, ctypes.byref(d)), "diag")
+        return {n: getattr(d, n) for n, _ in Diag._fields_}
+
+    def destroy(self):
+        if self.id:
+            check(self.ctx.lib.nimbus_hip_member_destroy(self.ctx.handle, self.id), "member_destroy")
+            self.id = 0

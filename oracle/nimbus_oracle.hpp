@@ -1,0 +1,1281 @@
+// CPU oracle for the Klemp-Wilhelmson split-explicit large step.
+//
+// TEST INFRASTRUCTURE ONLY. This file is a plain C++ restatement of the
+// Nimbus reference CPU path for one large step of the Cartesian C-grid
+// dynamical core. It is used by tests/, by __graft_entry__.smoke() and by the
+// cpu_baseline leg of bench.py as the checker and as the CPU baseline; the
+// shipped HIP backend (nimbus_amd/) never links, loads or calls it.
+//
+// Layout mirrors the reference host storage: every 3-D field is row-major
+// [nx][ny][nz] with k (vertical) contiguous, index (i*ny + j)*nz + k
+// (include/core/field/field3d.hpp:543-574). The C-grid placement follows
+// src/boundary_conditions/boundary_conditions_cartesian_cgrid.cpp:11-24:
+// u[i] at x_{i+1/2}, v[j] at y_{j+1/2}, w[k] at z_{k+1/2}, scalars at cell
+// centres, one ghost column on each lateral side.
+//
+// The template parameter S is the storage type. S = float reproduces the
+// reference CPU path (FP32 storage, FP64 arithmetic, FP32 rounding at every
+// store). S = double is the all-FP64 oracle. Arithmetic is FP64 in both.
+//
+// Parity pins (see tests/): every function below is checked against the
+// reference's own known-answer tests, against compiled reference
+// translation units (oracle/_ref, built by oracle/Makefile from the
+// reference sources where they lie) and against closed forms / invariants.
+// The HE-VI vertically implicit acoustic solve (acoustic_substep_hevi) has
+// no reference counterpart; it is pinned by closed forms (rest state, beta=0
+// structure, dense-solve brute force on tiny columns, conservation).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <stdexcept>
+#include <vector>
+
+namespace oracle
+{
+
+// ---- constants: include/core/infra/physical_constants.hpp:14-34,
+// include/microphysics/microphysics_base.hpp:1071-1081,
+// include/core/field/field_bounds.hpp:10-20 ----
+constexpr double kG = 9.81;
+constexpr double kRd = 287.0;
+constexpr double kRv = 461.5;
+constexpr double kCp = 1004.0;
+constexpr double kLv = 2.5e6;
+constexpr double kP00 = 100000.0;
+constexpr double kT0 = 273.15;
+constexpr double kGamma = kCp / (kCp - kRd);
+constexpr double kKappa = kRd / kCp;
+constexpr float kThetaMin = 200.0f, kThetaMax = 700.0f;
+constexpr float kPMin = 1000.0f, kPMax = 200000.0f;
+constexpr float kRhoMin = 0.05f, kRhoMax = 2.0f;
+constexpr float kWindH = 180.0f, kWindV = 120.0f;
+constexpr float kQvMax = 0.05f, kQhMax = 0.05f;
+constexpr double kCStar = 30.0;  // boundary_conditions_cartesian_cgrid.cpp:293
+
+enum LateralBC : int { kPeriodic = 0, kRadiative = 1, kZeroGradient = 2, kRigid = 3, kRigidSlip = 4 };
+enum BottomBC : int { kBottomFreeSlip = 0, kBottomNoSlip = 1 };
+enum TopBC : int { kTopRigid = 0, kTopOpen = 1 };
+enum AcousticMode : int { kAcousticExplicit = 0, kAcousticHEVI = 1 };
+
+struct Grid
+{
+    int nx = 0, ny = 0, nz = 0;
+    double dx = 0, dy = 0, dz = 0;
+    std::size_t cells() const { return std::size_t(nx) * ny * nz; }
+    std::size_t at(int i, int j, int k) const
+    {
+        return (std::size_t(i) * std::size_t(ny) + std::size_t(j)) * std::size_t(nz) + std::size_t(k);
+    }
+};
+
+// FP64 base state at cell centres (include/core/runtime/simulation_state.hpp:104-111).
+struct Base
+{
+    std::vector<double> rho0, p0, theta0, qv0, u0, v0;
+};
+
+struct Config
+{
+    double dt = 1.0;
+    int substeps = 6;           // requested minimum (split_explicit.cpp:127-137)
+    int acoustic_mode = kAcousticExplicit;
+    double hevi_beta = 0.1;     // off-centering of the implicit vertical step
+    double coriolis_f = 0.0;
+    int bc_west = kRadiative, bc_east = kRadiative, bc_south = kRadiative, bc_north = kRadiative;
+    int bc_bottom = kBottomFreeSlip, bc_top = kTopRigid;
+    // Kessler parameters (kessler.hpp:40-46)
+    double qc0 = 1.0e-3, c_auto = 1.0e-3, c_accr = 2.2, evap_eff = 1.0, vt_max = 20.0;
+};
+
+template <class S>
+struct State
+{
+    std::vector<S> u, v, w, rho, p, theta, qv, qc, qr;
+    std::vector<double> rain_mm;  // accumulated surface rain per column [nx*ny]
+    // KW78 radiative-face stashes (boundary_conditions_cartesian_cgrid.cpp:798-823)
+    std::vector<float> rad_w, rad_e, rad_s, rad_n;
+    void allocate(const Grid& g)
+    {
+        for (auto* f : {&u, &v, &w, &rho, &p, &theta, &qv, &qc, &qr}) f->assign(g.cells(), S(0));
+        rain_mm.assign(std::size_t(g.nx) * g.ny, 0.0);
+    }
+};
+
+// ======================================================================
+// Thermodynamics (src/microphysics/base/thermodynamics.hpp:29-470)
+// ======================================================================
+
+inline double theta_to_temperature(double theta, double p)
+{
+    // thermodynamics.hpp:29-45
+    if (!std::isfinite(theta) || !std::isfinite(p)) return kT0;
+    const double th = std::clamp(theta, 150.0, 700.0);
+    const double pp = std::clamp(p, 100.0, 200000.0);
+    const double T = th * std::pow(pp / kP00, kKappa);
+    return std::isfinite(T) ? T : kT0;
+}
+
+inline double theta_tendency_from_temperature(double dT, double p)
+{
+    // thermodynamics.hpp:225-238
+    if (!std::isfinite(dT) || !std::isfinite(p)) return 0.0;
+    const double pp = std::clamp(p, 100.0, 200000.0);
+    const double r = dT * std::pow(kP00 / pp, kKappa);
+    return std::isfinite(r) ? r : 0.0;
+}
+
+inline double buck_es(double T)
+{
+    // thermodynamics.hpp:67-88 (Buck water equation at all temperatures)
+    if (!std::isfinite(T)) return 0.0;
+    const double Tc = std::clamp(T, 150.0, 400.0) - kT0;
+    const double es = 611.21 * std::exp((18.678 - Tc / 234.5) * Tc / (257.14 + Tc));
+    return std::isfinite(es) ? std::max(0.0, es) : 0.0;
+}
+
+inline double qvs_water(double T, double p)
+{
+    // thermodynamics.hpp:104-118
+    const double es = buck_es(T);
+    if (!std::isfinite(es) || !std::isfinite(p)) return 0.0;
+    const double pp = std::clamp(p, 100.0, 200000.0);
+    const double q = 0.62198 * es / std::max(pp - es, 1.0);
+    return std::clamp(q, 0.0, 0.2);
+}
+
+// d(qvs)/dT along the Buck curve, evaluated from (T, qvs) as the reference's
+// Newton slope (thermodynamics.hpp:300-325).
+inline double qvs_slope(double T, double qs, double p)
+{
+    if (!(T > 150.0 && T < 400.0) || !(qs > 0.0 && qs < 0.2)) return 0.0;
+    const double pp = std::clamp(p, 100.0, 200000.0);
+    const double e = qs * pp / (0.62198 + qs);
+    const double Tc = T - kT0;
+    const double den = 257.14 + Tc;
+    const double num = (18.678 - Tc / 234.5) * Tc;
+    const double dnum = 18.678 - 2.0 * Tc / 234.5;
+    const double dlog = (dnum * den - num) / (den * den);
+    return qs * pp / std::max(pp - e, 1.0) * dlog;
+}
+
+// Saturation adjustment: qv' = qvs(T'), T' = T + (Lv/cp)(qv - qv'), solved by a
+// bracketed Newton iteration of at most 10 steps (thermodynamics.hpp:282-470).
+inline double saturation_adjust(double T_in, double p, double& qv, double& qc)
+{
+    constexpr double L = kLv / kCp;
+    constexpr int iters = 10;
+    double T = std::isfinite(T_in) ? std::clamp(T_in, 150.0, 400.0) : kT0;
+    qv = std::isfinite(qv) ? std::max(0.0, qv) : 0.0;
+    qc = std::isfinite(qc) ? std::max(0.0, qc) : 0.0;
+    double qs = qvs_water(T, p);
+    if (!std::isfinite(qs) || qs < 0.0) qs = 0.0;
+    const double tol_eps = 8.0 * std::numeric_limits<double>::epsilon();
+    if (qv > qs)
+    {
+        const double cmax = std::min(qv, std::max(0.0, (400.0 - T) / L));
+        double c = cmax;
+        const double r_at_max = qv - cmax - qvs_water(T + L * cmax, p);
+        if (r_at_max <= 0.0)
+        {
+            double lo = 0.0, hi = cmax;
+            c = std::clamp((qv - qs) / (1.0 + L * qvs_slope(T, qs, p)), lo, hi);
+            for (int it = 0; it < iters; ++it)
+            {
+                const double Ta = T + L * c;
+                const double qa = qvs_water(Ta, p);
+                const double f = qv - c - qa;
+                if (std::abs(f) <= tol_eps * std::max({1.0, std::abs(qv), std::abs(qa)})) break;
+                if (f > 0.0) lo = c; else hi = c;
+                const double cand = c - f / (-1.0 - L * qvs_slope(Ta, qa, p));
+                c = (cand > lo && cand < hi) ? cand : 0.5 * (lo + hi);
+            }
+        }
+        qv -= c;
+        qc += c;
+        T += L * c;
+    }
+    else if (qv < qs && qc > 0.0)
+    {
+        const double emax = std::min(qc, std::max(0.0, (T - 150.0) / L));
+        double e = emax;
+        const double r_at_max = qv + emax - qvs_water(T - L * emax, p);
+        if (r_at_max >= 0.0)
+        {
+            double lo = 0.0, hi = emax;
+            e = std::clamp((qs - qv) / (1.0 + L * qvs_slope(T, qs, p)), lo, hi);
+            for (int it = 0; it < iters; ++it)
+            {
+                const double Ta = T - L * e;
+                const double qa = qvs_water(Ta, p);
+                const double f = qv + e - qa;
+                if (std::abs(f) <= tol_eps * std::max({1.0, std::abs(qv), std::abs(qa)})) break;
+                if (f < 0.0) lo = e; else hi = e;
+                const double cand = e - f / (1.0 + L * qvs_slope(Ta, qa, p));
+                e = (cand > lo && cand < hi) ? cand : 0.5 * (lo + hi);
+            }
+        }
+        qc -= e;
+        qv += e;
+        T -= L * e;
+    }
+    qc = std::max(0.0, qc);
+    qv = std::max(0.0, qv);
+    return T;
+}
+
+// Kessler rain speed without the density correction (fall_speeds.hpp:125-130).
+inline double kessler_rain_speed(double q, double rho)
+{
+    if (q < 1.0e-9) return 0.0;
+    return 36.34 * std::pow(1.0e-3 * rho * q, 0.1346);
+}
+
+// Dry rho-theta EOS evaluated in FP64 (include/dynamics/eos_pressure.hpp:30-38).
+inline double eos_pressure(double rho, double theta)
+{
+    return kP00 * std::pow(rho * kRd * theta / kP00, kGamma);
+}
+
+inline float sanitize(float v, float fallback, float lo, float hi)
+{
+    // field_bounds.hpp:41-53 (repairs enabled)
+    if (!std::isfinite(v)) return fallback;
+    return std::clamp(v, lo, hi);
+}
+
+// ======================================================================
+// WENO5-JS face value (include/numerics/advection/weno5_face.hpp:43-86)
+// and the MC-limited slope used near walls (advection_cartesian.cpp:135-162)
+// ======================================================================
+
+inline double weno5_face(double a, double b, double c, double d, double e)
+{
+    // candidate polynomials at the downwind face of cell c (flow c -> d)
+    const double q0 = a / 3.0 - 7.0 * b / 6.0 + 11.0 * c / 6.0;
+    const double q1 = -b / 6.0 + 5.0 * c / 6.0 + d / 3.0;
+    const double q2 = c / 3.0 + 5.0 * d / 6.0 - e / 6.0;
+    // Jiang-Shu smoothness indicators
+    const double t0a = a - 2.0 * b + c, t0b = a - 4.0 * b + 3.0 * c;
+    const double t1a = b - 2.0 * c + d, t1b = b - d;
+    const double t2a = c - 2.0 * d + e, t2b = 3.0 * c - 4.0 * d + e;
+    const double s0 = 13.0 / 12.0 * t0a * t0a + 0.25 * t0b * t0b;
+    const double s1 = 13.0 / 12.0 * t1a * t1a + 0.25 * t1b * t1b;
+    const double s2 = 13.0 / 12.0 * t2a * t2a + 0.25 * t2b * t2b;
+    // epsilon relative to the stencil magnitude squared plus a tiny floor
+    const double m2 = std::max(std::max(std::max(a * a, b * b), std::max(c * c, d * d)), e * e);
+    const double eps = 1.0e-6 * m2 + 1.0e-40;
+    const double w0 = 0.1 / ((eps + s0) * (eps + s0));
+    const double w1 = 0.6 / ((eps + s1) * (eps + s1));
+    const double w2 = 0.3 / ((eps + s2) * (eps + s2));
+    const double ws = w0 + w1 + w2;
+    if (!(ws > 0.0) || !std::isfinite(ws)) return c;
+    const double f = (w0 * q0 + w1 * q1 + w2 * q2) / ws;
+    return std::isfinite(f) ? f : c;
+}
+
+inline double minmod3(double a, double b, double c)
+{
+    if (a > 0.0 && b > 0.0 && c > 0.0) return std::min(a, std::min(b, c));
+    if (a < 0.0 && b < 0.0 && c < 0.0) return std::max(a, std::max(b, c));
+    return 0.0;
+}
+
+inline double mc_slope(double qm, double q0, double qp)
+{
+    const double r = qp - q0, l = q0 - qm;
+    return minmod3(0.5 * (r + l), 2.0 * r, 2.0 * l);
+}
+
+// Conservative PD-limiter storage bound (include/numerics/advection/advection_cartesian.hpp:101-108).
+inline float pd_limiter_store(double ratio, bool active_at_one)
+{
+    if (!(ratio > 0.0 && (ratio < 1.0 || (active_at_one && ratio == 1.0)))) return float(ratio);
+    constexpr float shrink = 1.0f - 32.0f * std::numeric_limits<float>::epsilon();
+    return std::nextafter(float(ratio) * shrink, 0.0f);
+}
+
+// ======================================================================
+// UP5/UP3/UP1 upwind-biased first derivatives
+// (include/numerics/derivatives/derivative_operators.hpp:324-401)
+// ======================================================================
+
+inline double upwind_fd1(double m1, double c, double p1, double a, double inv)
+{
+    return a >= 0.0 ? (c - m1) * inv : (p1 - c) * inv;
+}
+inline double upwind_fd3(double m2, double m1, double c, double p1, double p2, double a, double inv)
+{
+    return a >= 0.0 ? (2.0 * p1 + 3.0 * c - 6.0 * m1 + m2) * (inv / 6.0)
+                    : (-p2 + 6.0 * p1 - 3.0 * c - 2.0 * m1) * (inv / 6.0);
+}
+inline double upwind_fd5(double m3, double m2, double m1, double c, double p1, double p2, double p3,
+                         double a, double inv)
+{
+    return a >= 0.0 ? (-2.0 * m3 + 15.0 * m2 - 60.0 * m1 + 20.0 * c + 30.0 * p1 - 3.0 * p2) * (inv / 60.0)
+                    : (2.0 * p3 - 15.0 * p2 + 60.0 * p1 - 20.0 * c - 30.0 * m1 + 3.0 * m2) * (inv / 60.0);
+}
+
+// Upwind derivative along one axis: `at(o)` returns f at offset o from the
+// centre, `pos` the index along the axis and `n` its extent.
+template <class F>
+inline double upwind_axis(F at, int pos, int n, double a, double inv)
+{
+    const int half = std::min(pos, n - 1 - pos);
+    if (half <= 0) return 0.0;
+    const double c = at(0), m1 = at(-1), p1 = at(1);
+    if (half == 1) return upwind_fd1(m1, c, p1, a, inv);
+    const double m2 = at(-2), p2 = at(2);
+    if (half == 2) return upwind_fd3(m2, m1, c, p1, p2, a, inv);
+    return upwind_fd5(at(-3), m2, m1, c, p1, p2, at(3), a, inv);
+}
+
+// ======================================================================
+// Large-step pieces
+// ======================================================================
+
+inline int resolve_substeps(const Grid& g, const Base& b, const Config& cfg)
+{
+    // split_explicit.cpp:67-156 with c_max from the base state (runtime.cpp:860-878)
+    double cmax = 0.0;
+    for (int k = 0; k < g.nz; ++k) cmax = std::max(cmax, std::sqrt(kGamma * b.p0[k] / b.rho0[k]));
+    double dmin = g.dz;
+    if (cfg.acoustic_mode == kAcousticHEVI)
+        dmin = std::min(g.dx, g.dy);  // the vertical acoustic mode is implicit
+    else
+        dmin = std::min({g.dx, g.dy, g.dz});
+    const int required = std::max(int(std::ceil(cfg.dt * cmax / (0.5 * dmin))) + 1, 2);
+    return cfg.substeps > 0 ? std::max(cfg.substeps, required) : required;
+}
+
+// C-grid lateral + vertical boundary closure with optional KW78 radiation
+// (boundary_conditions_cartesian_cgrid.cpp:798-1175).
+template <class S>
+void seed_radiation(const Grid& g, State<S>& s)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    s.rad_w.resize(std::size_t(NY) * NZ);
+    s.rad_e.resize(std::size_t(NY) * NZ);
+    s.rad_s.resize(std::size_t(NX) * NZ);
+    s.rad_n.resize(std::size_t(NX) * NZ);
+    for (int j = 0; j < NY; ++j)
+        for (int k = 0; k < NZ; ++k)
+        {
+            s.rad_w[std::size_t(j) * NZ + k] = float(s.u[g.at(0, j, k)]);
+            s.rad_e[std::size_t(j) * NZ + k] = float(s.u[g.at(NX - 2, j, k)]);
+        }
+    for (int i = 0; i < NX; ++i)
+        for (int k = 0; k < NZ; ++k)
+        {
+            s.rad_s[std::size_t(i) * NZ + k] = float(s.v[g.at(i, 0, k)]);
+            s.rad_n[std::size_t(i) * NZ + k] = float(s.v[g.at(i, NY - 2, k)]);
+        }
+}
+
+inline float kw78_face(float prev, float phase, float grad, float dt, float inv_d, bool west_like)
+{
+    // include/dynamics/radiative_face.hpp:5-15 (four FP32 operations)
+    if (west_like ? phase >= 0.0f : phase <= 0.0f) return prev;
+    volatile float a = dt * phase;
+    volatile float b = a * grad;
+    volatile float c = b * inv_d;
+    return prev - c;
+}
+
+template <class S>
+void apply_boundaries(const Grid& g, const Config& cfg, State<S>& s, double dt_rad)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    if (s.rad_w.size() != std::size_t(NY) * NZ || s.rad_s.size() != std::size_t(NX) * NZ) seed_radiation(g, s);
+    const bool radiate = dt_rad > 0.0;
+    const float dtf = float(dt_rad);
+    const float inv_dx = 1.0f / float(g.dx), inv_dy = 1.0f / float(g.dy);
+    const float cstar = float(kCStar);
+    auto& u = s.u; auto& v = s.v; auto& w = s.w;
+    auto copy_rp = [&](std::size_t d, std::size_t src) { s.rho[d] = s.rho[src]; s.p[d] = s.p[src]; };
+    // west (i = 0 ghost, wall face u[0])
+    for (int j = 0; j < NY; ++j)
+        for (int k = 0; k < NZ; ++k)
+        {
+            const std::size_t c = g.at(0, j, k);
+            const int bc = cfg.bc_west;
+            if (bc == kPeriodic)
+            {
+                const std::size_t sidx = g.at(NX - 2, j, k);
+                u[c] = u[sidx]; v[c] = v[sidx]; w[c] = w[sidx]; copy_rp(c, sidx);
+                continue;
+            }
+            const std::size_t in = g.at(1, j, k);
+            if (bc == kRadiative)
+            {
+                if (radiate)
+                {
+                    float& up = s.rad_w[std::size_t(j) * NZ + k];
+                    const float un = kw78_face(up, up - 0.0f - cstar, float(u[in]) - up, dtf, inv_dx, true);
+                    u[c] = S(un);
+                    up = un;
+                }
+            }
+            else if (bc == kZeroGradient) u[c] = u[in];
+            else u[c] = S(0);
+            if (bc == kRigid) { v[c] = S(0); w[c] = S(0); }
+            else { v[c] = v[in]; w[c] = w[in]; }
+            copy_rp(c, in);
+        }
+    // east (i = NX-1 ghost, wall face u[NX-2])
+    for (int j = 0; j < NY; ++j)
+        for (int k = 0; k < NZ; ++k)
+        {
+            const int bc = cfg.bc_east;
+            const std::size_t gi = g.at(NX - 1, j, k);
+            if (bc == kPeriodic)
+            {
+                const std::size_t sidx = g.at(1, j, k);
+                u[gi] = u[sidx]; v[gi] = v[sidx]; w[gi] = w[sidx]; copy_rp(gi, sidx);
+                continue;
+            }
+            const std::size_t a = g.at(NX - 2, j, k), am = g.at(NX - 3, j, k);
+            if (bc == kRigid || bc == kRigidSlip) u[a] = S(0);
+            if (bc == kRadiative && radiate)
+            {
+                float& up = s.rad_e[std::size_t(j) * NZ + k];
+                const float un = kw78_face(up, up - 0.0f + cstar, up - float(u[am]), dtf, inv_dx, false);
+                u[a] = S(un);
+                up = un;
+            }
+            else if (bc == kZeroGradient) u[a] = u[am];
+            u[gi] = (bc == kRadiative || bc == kZeroGradient) ? u[a] : S(0);
+            if (bc == kRigid) { v[gi] = S(0); w[gi] = S(0); }
+            else { v[gi] = v[a]; w[gi] = w[a]; }
+            copy_rp(gi, a);
+        }
+    // south (j = 0 ghost, wall face v[i][0])
+    for (int i = 0; i < NX; ++i)
+        for (int k = 0; k < NZ; ++k)
+        {
+            const std::size_t c = g.at(i, 0, k);
+            const int bc = cfg.bc_south;
+            if (bc == kPeriodic)
+            {
+                const std::size_t sidx = g.at(i, NY - 2, k);
+                v[c] = v[sidx]; u[c] = u[sidx]; w[c] = w[sidx]; copy_rp(c, sidx);
+                continue;
+            }
+            const std::size_t in = g.at(i, 1, k);
+            if (bc == kRadiative)
+            {
+                if (radiate)
+                {
+                    float& vp = s.rad_s[std::size_t(i) * NZ + k];
+                    const float vn = kw78_face(vp, vp - 0.0f - cstar, float(v[in]) - vp, dtf, inv_dy, true);
+                    v[c] = S(vn);
+                    vp = vn;
+                }
+            }
+            else if (bc == kZeroGradient) v[c] = v[in];
+            else v[c] = S(0);
+            if (bc == kRigid) { u[c] = S(0); w[c] = S(0); }
+            else { u[c] = u[in]; w[c] = w[in]; }
+            copy_rp(c, in);
+        }
+    // north (j = NY-1 ghost, wall face v[i][NY-2])
+    for (int i = 0; i < NX; ++i)
+        for (int k = 0; k < NZ; ++k)
+        {
+            const int bc = cfg.bc_north;
+            const std::size_t gj = g.at(i, NY - 1, k);
+            if (bc == kPeriodic)
+            {
+                const std::size_t sidx = g.at(i, 1, k);
+                v[gj] = v[sidx]; u[gj] = u[sidx]; w[gj] = w[sidx]; copy_rp(gj, sidx);
+                continue;
+            }
+            const std::size_t a = g.at(i, NY - 2, k), am = g.at(i, NY - 3, k);
+            if (bc == kRigid || bc == kRigidSlip) v[a] = S(0);
+            if (bc == kRadiative && radiate)
+            {
+                float& vp = s.rad_n[std::size_t(i) * NZ + k];
+                const float vn = kw78_face(vp, vp - 0.0f + cstar, vp - float(v[am]), dtf, inv_dy, false);
+                v[a] = S(vn);
+                vp = vn;
+            }
+            else if (bc == kZeroGradient) v[a] = v[am];
+            v[gj] = (bc == kRadiative || bc == kZeroGradient) ? v[a] : S(0);
+            if (bc == kRigid) { u[gj] = S(0); w[gj] = S(0); }
+            else { u[gj] = u[a]; w[gj] = w[a]; }
+            copy_rp(gj, a);
+        }
+    // vertical: bottom no-slip pins u,v; top rigid lid w = 0, u,v zero-gradient
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j)
+        {
+            if (cfg.bc_bottom == kBottomNoSlip) { u[g.at(i, j, 0)] = S(0); v[g.at(i, j, 0)] = S(0); }
+            const std::size_t t = g.at(i, j, NZ - 1), tm = g.at(i, j, NZ - 2);
+            w[t] = cfg.bc_top == kTopOpen ? w[tm] : S(0);
+            u[t] = u[tm];
+            v[t] = v[tm];
+        }
+}
+
+// ----------------------------------------------------------------------
+// Slow tendencies (cartesian_cgrid_3d.cpp:1038-1356). dp/dt is not formed:
+// the first acoustic substep re-diagnoses p from the EOS in every cell
+// before any operator reads it, so the slow pressure tendency never reaches
+// an output of the large step.
+// ----------------------------------------------------------------------
+template <class S>
+void slow_tendencies(const Grid& g, const Base& b, const Config& cfg, const State<S>& s,
+                     std::vector<S>& du, std::vector<S>& dv, std::vector<S>& dw)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const double inv_dx = 1.0 / g.dx, inv_dy = 1.0 / g.dy, inv_dz = 1.0 / g.dz;
+    const double f = cfg.coriolis_f;
+    du.assign(g.cells(), S(0)); dv.assign(g.cells(), S(0)); dw.assign(g.cells(), S(0));
+    const auto& u = s.u; const auto& v = s.v; const auto& w = s.w;
+    auto F = [&](const std::vector<S>& fld, int i, int j, int k) { return double(fld[g.at(i, j, k)]); };
+    auto finite0 = [](double x) { return std::isfinite(x) ? x : 0.0; };
+    auto safe_rho = [](double r) { return std::isfinite(r) && r > 1.0e-6 ? r : 1.0; };
+    auto stored = [](double x) { return double(float(x)); };
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < NX - 1; ++i)
+        for (int j = 1; j < NY - 1; ++j)
+            for (int k = 0; k < NZ - 1; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                // ---- u at x-face ----
+                {
+                    const double ux = F(u, i, j, k);
+                    const double vy = 0.25 * (F(v, i, j - 1, k) + F(v, i, j, k) + F(v, i + 1, j - 1, k) + F(v, i + 1, j, k));
+                    const double wz = k == 0 ? 0.25 * (F(w, i, j, 0) + F(w, i + 1, j, 0))
+                                             : 0.25 * (F(w, i, j, k - 1) + F(w, i, j, k) + F(w, i + 1, j, k - 1) + F(w, i + 1, j, k));
+                    const double ddx = upwind_axis([&](int o) { return F(u, i + o, j, k); }, i, NX, ux, inv_dx);
+                    const double ddy = upwind_axis([&](int o) { return F(u, i, j + o, k); }, j, NY, vy, inv_dy);
+                    const double ddz = k == 0 ? (wz >= 0.0 ? 0.0 : (F(u, i, j, 1) - F(u, i, j, 0)) * inv_dz)
+                                              : upwind_axis([&](int o) { return F(u, i, j, k + o); }, k, NZ, wz, inv_dz);
+                    const double v0k = (int(b.v0.size()) > k) ? b.v0[k] : 0.0;
+                    const double cor = f * (vy - v0k);
+                    du[c] = S(finite0(-ux * ddx - vy * ddy - wz * ddz + cor));
+                }
+                // ---- v at y-face ----
+                {
+                    const double vy = F(v, i, j, k);
+                    const double ux = 0.25 * (F(u, i - 1, j, k) + F(u, i, j, k) + F(u, i - 1, j + 1, k) + F(u, i, j + 1, k));
+                    const double wz = k == 0 ? 0.25 * (F(w, i, j, 0) + F(w, i, j + 1, 0))
+                                             : 0.25 * (F(w, i, j, k - 1) + F(w, i, j, k) + F(w, i, j + 1, k - 1) + F(w, i, j + 1, k));
+                    const double ddx = upwind_axis([&](int o) { return F(v, i + o, j, k); }, i, NX, ux, inv_dx);
+                    const double ddy = upwind_axis([&](int o) { return F(v, i, j + o, k); }, j, NY, vy, inv_dy);
+                    const double ddz = k == 0 ? (wz >= 0.0 ? 0.0 : (F(v, i, j, 1) - F(v, i, j, 0)) * inv_dz)
+                                              : upwind_axis([&](int o) { return F(v, i, j, k + o); }, k, NZ, wz, inv_dz);
+                    const double u0k = (int(b.u0.size()) > k) ? b.u0[k] : 0.0;
+                    const double cor = f * (ux - u0k);
+                    dv[c] = S(finite0(-ux * ddx - vy * ddy - wz * ddz - cor));
+                }
+                // ---- w at z-face ----
+                {
+                    const double wz = F(w, i, j, k);
+                    const double ux = 0.25 * (F(u, i - 1, j, k) + F(u, i, j, k) + F(u, i - 1, j, k + 1) + F(u, i, j, k + 1));
+                    const double vy = 0.25 * (F(v, i, j - 1, k) + F(v, i, j, k) + F(v, i, j - 1, k + 1) + F(v, i, j, k + 1));
+                    const double ddx = upwind_axis([&](int o) { return F(w, i + o, j, k); }, i, NX, ux, inv_dx);
+                    const double ddy = upwind_axis([&](int o) { return F(w, i, j + o, k); }, j, NY, vy, inv_dy);
+                    const double ddz = k == 0 ? (wz >= 0.0 ? F(w, i, j, 0) * inv_dz : (F(w, i, j, 1) - F(w, i, j, 0)) * inv_dz)
+                                              : upwind_axis([&](int o) { return F(w, i, j, k + o); }, k, NZ, wz, inv_dz);
+                    const double rf = 0.5 * (F(s.rho, i, j, k) + F(s.rho, i, j, k + 1));
+                    const double r0f = 0.5 * (b.rho0[k] + b.rho0[k + 1]);
+                    const double buoy = -kG * (rf - r0f) / safe_rho(rf);
+                    const double qvf = 0.5 * (F(s.qv, i, j, k) + F(s.qv, i, j, k + 1));
+                    const double qv0f = 0.5 * (stored(b.qv0[k]) + stored(b.qv0[k + 1]));
+                    const double mbuoy = kG * 0.608 * (qvf - qv0f);
+                    const double ql_lo = F(s.qc, i, j, k) + F(s.qr, i, j, k);
+                    const double ql_hi = F(s.qc, i, j, k + 1) + F(s.qr, i, j, k + 1);
+                    const double load = -kG * (0.5 * (ql_lo + ql_hi));
+                    dw[c] = S(finite0(-ux * ddx - vy * ddy - wz * ddz + buoy + mbuoy + load));
+                }
+            }
+}
+
+// Slow apply u += dt*du (dynamics.cpp:2055-2095); non-finite tendencies count as 0.
+template <class S>
+void slow_apply(State<S>& s, const std::vector<S>& du, const std::vector<S>& dv, const std::vector<S>& dw, double dt)
+{
+    const std::size_t n = s.u.size();
+#pragma omp parallel for schedule(static)
+    for (std::size_t c = 0; c < n; ++c)
+    {
+        double a = double(du[c]), b = double(dv[c]), d = double(dw[c]);
+        if (!std::isfinite(a)) a = 0.0;
+        if (!std::isfinite(b)) b = 0.0;
+        if (!std::isfinite(d)) d = 0.0;
+        s.u[c] = S(double(s.u[c]) + a * dt);
+        s.v[c] = S(double(s.v[c]) + b * dt);
+        s.w[c] = S(double(s.w[c]) + d * dt);
+    }
+}
+
+// Heun slow RK2 (dynamics.cpp:5123-5292): k1 at u_n, predictor u_n + dt*k1,
+// k2 at the predictor, state restored, tendency = 0.5*(k1 + k2) in storage
+// precision, then the slow apply with dt.
+template <class S>
+void slow_rk2_and_apply(const Grid& g, const Base& b, const Config& cfg, State<S>& s)
+{
+    std::vector<S> k1u, k1v, k1w, k2u, k2v, k2w;
+    slow_tendencies(g, b, cfg, s, k1u, k1v, k1w);
+    const std::vector<S> us = s.u, vs = s.v, ws = s.w;
+    slow_apply(s, k1u, k1v, k1w, cfg.dt);
+    slow_tendencies(g, b, cfg, s, k2u, k2v, k2w);
+    s.u = us; s.v = vs; s.w = ws;
+    const std::size_t n = s.u.size();
+    for (std::size_t c = 0; c < n; ++c)
+    {
+        k2u[c] = S(S(0.5) * (k2u[c] + k1u[c]));
+        k2v[c] = S(S(0.5) * (k2v[c] + k1v[c]));
+        k2w[c] = S(S(0.5) * (k2w[c] + k1w[c]));
+    }
+    slow_apply(s, k2u, k2v, k2w, cfg.dt);
+}
+
+// Time-averaged face mass flux, sampled from the pre-continuity state of each
+// substep (advection_cartesian.cpp:610-830, Host precision: FP64 add, storage round).
+template <class S>
+void accumulate_mass_flux(const Grid& g, const Config& cfg, const State<S>& s, double weight,
+                          std::vector<S>& mfx, std::vector<S>& mfy, std::vector<S>& mfz)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j)
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                const double rc = double(s.rho[c]);
+                if (i < NX - 1)
+                {
+                    const double fl = 0.5 * (rc + double(s.rho[g.at(i + 1, j, k)])) * double(s.u[c]);
+                    mfx[c] = S(double(mfx[c]) + weight * fl);
+                }
+                if (j < NY - 1)
+                {
+                    const double fl = 0.5 * (rc + double(s.rho[g.at(i, j + 1, k)])) * double(s.v[c]);
+                    mfy[c] = S(double(mfy[c]) + weight * fl);
+                }
+                double fl = 0.0;
+                if (k < NZ - 1) fl = 0.5 * (rc + double(s.rho[g.at(i, j, k + 1)])) * double(s.w[c]);
+                else fl = cfg.bc_top == kTopOpen ? rc * double(s.w[c]) : 0.0;
+                mfz[c] = S(double(mfz[c]) + weight * fl);
+            }
+}
+
+
+// The reference clamps in FP32 on the stored value (field_bounds.hpp:41-53);
+// the FP64 oracle applies the same bounds to the FP64 value.
+template <class S>
+inline double clamp_store(double x, double fallback, float lo, float hi)
+{
+    if constexpr (sizeof(S) == sizeof(float))
+    {
+        float f = float(x);
+        if (!std::isfinite(f)) f = float(fallback);
+        return double(std::clamp(f, lo, hi));
+    }
+    else
+    {
+        if (!std::isfinite(x)) x = fallback;
+        return std::clamp(x, double(lo), double(hi));
+    }
+}
+
+inline double finite_or_zero(double x) { return std::isfinite(x) ? x : 0.0; }
+inline double safe_rho(double r) { return std::isfinite(r) && r > 1.0e-6 ? r : 1.0; }
+
+// Backward momentum update of the acoustic substep with the new pressure
+// (acoustic_cgrid_fast.cpp:159-215). w is skipped when `skip_w` (HE-VI solves it).
+template <class S>
+void momentum_backward(const Grid& g, const Base& b, State<S>& s, double dts, bool skip_w)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const double inv_dx = 1.0 / g.dx, inv_dy = 1.0 / g.dy, inv_dz = 1.0 / g.dz;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j)
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                double du = 0.0, dv = 0.0, dw = 0.0;
+                if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k < NZ - 1)
+                {
+                    const std::size_t ip = g.at(i + 1, j, k), jp = g.at(i, j + 1, k), kp = g.at(i, j, k + 1);
+                    const double rc = double(s.rho[c]);
+                    const double rx = 0.5 * (rc + double(s.rho[ip]));
+                    const double ry = 0.5 * (rc + double(s.rho[jp]));
+                    const double rz = 0.5 * (rc + double(s.rho[kp]));
+                    const double dp0 = (b.p0[k + 1] - b.p0[k]) * inv_dz;
+                    // tendencies are rounded to FP32 before the update in the reference
+                    du = double(S(finite_or_zero(-(double(s.p[ip]) - double(s.p[c])) * inv_dx / safe_rho(rx))));
+                    dv = double(S(finite_or_zero(-(double(s.p[jp]) - double(s.p[c])) * inv_dy / safe_rho(ry))));
+                    dw = double(S(finite_or_zero(-(((double(s.p[kp]) - double(s.p[c])) * inv_dz) - dp0) / safe_rho(rz))));
+                }
+                s.u[c] = S(clamp_store<S>(double(s.u[c]) + du * dts, 0.0, -kWindH, kWindH));
+                s.v[c] = S(clamp_store<S>(double(s.v[c]) + dv * dts, 0.0, -kWindH, kWindH));
+                if (!skip_w) s.w[c] = S(clamp_store<S>(double(s.w[c]) + dw * dts, 0.0, -kWindV, kWindV));
+            }
+}
+
+// Continuity flux divergence of the explicit substep (acoustic_cgrid_fast.cpp:49-116);
+// `horizontal_only` drops the vertical terms (HE-VI treats them implicitly).
+template <class S>
+double continuity_divergence(const Grid& g, const State<S>& s, int i, int j, int k, bool horizontal_only)
+{
+    const int NZ = g.nz;
+    const double inv_dx = 1.0 / g.dx, inv_dy = 1.0 / g.dy, inv_dz = 1.0 / g.dz;
+    auto R = [&](std::size_t c) { return double(s.rho[c]); };
+    const std::size_t c = g.at(i, j, k);
+    const std::size_t im = g.at(i - 1, j, k), jm = g.at(i, j - 1, k);
+    const double fxp = 0.5 * (R(c) + R(g.at(i + 1, j, k))) * double(s.u[c]);
+    const double fxm = 0.5 * (R(im) + R(c)) * double(s.u[im]);
+    const double fyp = 0.5 * (R(c) + R(g.at(i, j + 1, k))) * double(s.v[c]);
+    const double fym = 0.5 * (R(jm) + R(c)) * double(s.v[jm]);
+    double d = (fxp - fxm) * inv_dx + (fyp - fym) * inv_dy;
+    if (!horizontal_only)
+    {
+        const double fzp = k + 1 < NZ ? 0.5 * (R(c) + R(g.at(i, j, k + 1))) * double(s.w[c]) : R(c) * double(s.w[c]);
+        const double fzm = k > 0 ? 0.5 * (R(g.at(i, j, k - 1)) + R(c)) * double(s.w[g.at(i, j, k - 1)]) : 0.0;
+        d += (fzp - fzm) * inv_dz;
+    }
+    return -d;
+}
+
+// Density update + EOS pressure of pass 2 (acoustic_cgrid_fast.cpp:118-157).
+template <class S>
+void density_eos_update(const Base& b, State<S>& s, std::size_t c, int k, double rho_new)
+{
+    double rn = rho_new;
+    if (!std::isfinite(rn) || rn <= 0.0) rn = std::max(0.1, b.rho0[k]);
+    const double rs = clamp_store<S>(rn, 1.0, kRhoMin, kRhoMax);
+    s.rho[c] = S(rs);
+    double pn = kP00;
+    const double th = double(s.theta[c]);
+    if (th > 0.0) pn = eos_pressure(double(s.rho[c]), th);
+    if (!std::isfinite(pn) || pn <= 0.0) pn = kP00;
+    s.p[c] = S(clamp_store<S>(pn, 101325.0, kPMin, kPMax));
+}
+
+// Explicit forward-backward acoustic substep (src/compute/cpu/acoustic_cgrid_fast.cpp:12-219).
+template <class S>
+void acoustic_substep_explicit(const Grid& g, const Base& b, State<S>& s, double dts)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    std::vector<S> drho(g.cells(), S(0));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < NX - 1; ++i)
+        for (int j = 1; j < NY - 1; ++j)
+            for (int k = 0; k < NZ; ++k)
+                drho[g.at(i, j, k)] = S(finite_or_zero(continuity_divergence(g, s, i, j, k, false)));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j)
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                density_eos_update(b, s, c, k, double(s.rho[c]) + double(drho[c]) * dts);
+            }
+    momentum_backward(g, b, s, dts, false);
+}
+
+// Thomas algorithm for a tridiagonal system a_k x_{k-1} + d_k x_k + c_k x_{k+1} = r_k
+// in FP64 with a positive-pivot guard (analog of implicit.cpp:465-500).
+inline bool thomas_solve(int n, const double* a, const double* d, const double* c, const double* r, double* x)
+{
+    std::vector<double> cp(n), rp(n);
+    double piv = d[0];
+    if (!(piv > 0.0) || !std::isfinite(piv)) return false;
+    cp[0] = c[0] / piv;
+    rp[0] = r[0] / piv;
+    for (int k = 1; k < n; ++k)
+    {
+        piv = d[k] - a[k] * cp[k - 1];
+        if (!(piv > 0.0) || !std::isfinite(piv)) return false;
+        cp[k] = c[k] / piv;
+        rp[k] = (r[k] - a[k] * rp[k - 1]) / piv;
+    }
+    x[n - 1] = rp[n - 1];
+    for (int k = n - 2; k >= 0; --k) x[k] = rp[k] - cp[k] * x[k + 1];
+    return true;
+}
+
+// Coefficients of the HE-VI w system for one column. Inputs per level k:
+// rho_bar[k] (face density k+1/2, k = 0..nz-2), C[k] = dp/drho at fixed theta
+// (cell k), G[k] explicit vertical PGF at face k+1/2 from p*, e[k] explicit
+// part of the vertical density change of cell k, w_old[k]. Unknowns are the
+// interior faces k = 0..nz-2 (ground and lid fluxes are zero).
+inline void hevi_column_system(int nz, double dt, double dz, double alpha, const double* rho_bar,
+                               const double* C, const double* G, const double* e, const double* w_old,
+                               double* A, double* D, double* U, double* Rhs)
+{
+    const int n = nz - 1;
+    const double Bc = alpha * dt / dz;
+    for (int k = 0; k < n; ++k)
+    {
+        const double Ak = dt * alpha / (rho_bar[k] * dz);
+        const double rk = rho_bar[k];
+        A[k] = k > 0 ? -Ak * Bc * C[k] * rho_bar[k - 1] : 0.0;
+        U[k] = k < n - 1 ? -Ak * Bc * C[k + 1] * rho_bar[k + 1] : 0.0;
+        D[k] = 1.0 + Ak * Bc * (C[k + 1] + C[k]) * rk;
+        Rhs[k] = w_old[k] - dt / rk * G[k] - Ak * (C[k + 1] * e[k + 1] - C[k] * e[k]);
+    }
+}
+
+// Vertically implicit (HE-VI) acoustic substep. Horizontal continuity and
+// momentum are explicit forward-backward as in the reference substep; the
+// vertical w-rho-p coupling is off-centred implicit (alpha = (1+beta)/2)
+// with p linearised about p* = EOS(rho*), solved per column by an FP64 Thomas
+// sweep, after which rho is updated in flux form and p re-diagnosed from the
+// EOS. The vertical mass flux alpha*w_new + (1-alpha)*w_old is returned in
+// `wflux_weighted` for the transport mass-flux accumulation.
+template <class S>
+void acoustic_substep_hevi(const Grid& g, const Base& b, State<S>& s, double dts, double beta,
+                           std::vector<double>& wz_eff)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const double alpha = 0.5 * (1.0 + beta);
+    const double inv_dz = 1.0 / g.dz;
+    wz_eff.assign(g.cells(), 0.0);
+    // columns are independent: horizontal divergence uses old u, v and old rho
+    std::vector<S> rho_old = s.rho;
+    State<S> snap;  // read-only view of the substep-start state for neighbours
+    snap.u = s.u; snap.v = s.v; snap.w = s.w; snap.rho = rho_old;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < NX - 1; ++i)
+        for (int j = 1; j < NY - 1; ++j)
+        {
+            std::vector<double> rstar(NZ), pstar(NZ), C(NZ + 1, 0.0), G(NZ), e(NZ + 1, 0.0), rb(NZ),
+                wo(NZ), A(NZ), D(NZ), U(NZ), Rh(NZ), x(NZ, 0.0);
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                rstar[k] = double(rho_old[c]) + dts * finite_or_zero(continuity_divergence(g, snap, i, j, k, true));
+                const double th = double(s.theta[c]);
+                pstar[k] = eos_pressure(rstar[k], th);
+                C[k] = kGamma * pstar[k] / rstar[k];
+                wo[k] = double(s.w[c]);
+            }
+            for (int k = 0; k < NZ - 1; ++k)
+            {
+                rb[k] = 0.5 * (double(rho_old[g.at(i, j, k)]) + double(rho_old[g.at(i, j, k + 1)]));
+                G[k] = (pstar[k + 1] - pstar[k]) * inv_dz - (b.p0[k + 1] - b.p0[k]) * inv_dz;
+            }
+            rb[NZ - 1] = 0.0;
+            for (int k = 0; k < NZ; ++k)
+            {
+                const double ftop = k < NZ - 1 ? rb[k] * wo[k] : 0.0;
+                const double fbot = k > 0 ? rb[k - 1] * wo[k - 1] : 0.0;
+                e[k] = -(1.0 - alpha) * dts * inv_dz * (ftop - fbot);
+            }
+            hevi_column_system(NZ, dts, g.dz, alpha, rb.data(), C.data(), G.data(), e.data(), wo.data(),
+                               A.data(), D.data(), U.data(), Rh.data());
+            const bool ok = thomas_solve(NZ - 1, A.data(), D.data(), U.data(), Rh.data(), x.data());
+            if (!ok)
+                for (int k = 0; k < NZ - 1; ++k) x[k] = wo[k];
+            // clamp and commit w, then flux-form density update with the weighted flux
+            for (int k = 0; k < NZ - 1; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                s.w[c] = S(clamp_store<S>(x[k], 0.0, -kWindV, kWindV));
+                x[k] = double(s.w[c]);
+            }
+            for (int k = 0; k < NZ; ++k)
+            {
+                const double wt = k < NZ - 1 ? alpha * x[k] + (1.0 - alpha) * wo[k] : 0.0;
+                const double wb = k > 0 ? alpha * x[k - 1] + (1.0 - alpha) * wo[k - 1] : 0.0;
+                const double ft = k < NZ - 1 ? rb[k] * wt : 0.0;
+                const double fb = k > 0 ? rb[k - 1] * wb : 0.0;
+                wz_eff[g.at(i, j, k)] = ft;
+                const double rn = rstar[k] - dts * inv_dz * (ft - fb);
+                density_eos_update(b, s, g.at(i, j, k), k, rn);
+            }
+        }
+    // ghost columns: density unchanged, clamped, pressure re-diagnosed (pass 2 of the reference)
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j)
+        {
+            if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1) continue;
+            for (int k = 0; k < NZ; ++k) density_eos_update(b, s, g.at(i, j, k), k, double(s.rho[g.at(i, j, k)]));
+        }
+    momentum_backward(g, b, s, dts, true);
+    // ghost-column and lid w follow the reference clamp-only rule
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j)
+        {
+            const bool interior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
+            for (int k = 0; k < NZ; ++k)
+                if (!interior || k == NZ - 1)
+                {
+                    const std::size_t c = g.at(i, j, k);
+                    s.w[c] = S(clamp_store<S>(double(s.w[c]), 0.0, -kWindV, kWindV));
+                }
+        }
+}
+
+// ----------------------------------------------------------------------
+// WENO5 conservative scalar transport with retained mass fluxes and the
+// positive-definite donor limiter (advection_cartesian.cpp:1941-2650).
+// ----------------------------------------------------------------------
+template <class S>
+void transport_scalar(const Grid& g, const Config& cfg, const std::vector<S>& q, std::vector<S>& out,
+                      const std::vector<S>& mfx, const std::vector<S>& mfy, const std::vector<S>& mfz,
+                      const std::vector<S>& rho_begin, const std::vector<S>& rho_end, double dt,
+                      bool positive_definite)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const double inv_dx = 1.0 / g.dx, inv_dy = 1.0 / g.dy, inv_dz = 1.0 / g.dz;
+    const bool open_top = cfg.bc_top == kTopOpen;
+    std::vector<S> sfx(g.cells(), S(0)), sfy(g.cells(), S(0)), sfz(g.cells(), S(0)), rl(g.cells(), S(1));
+    auto Q = [&](int i, int j, int k) { return double(q[g.at(i, j, k)]); };
+    // face value along one axis: `at(o)` reads q at offset o from the lower cell
+    auto face = [&](auto at, int pos, int n, double mf) {
+        if (mf >= 0.0)
+        {
+            const double q0 = at(0);
+            if (pos >= 2 && pos + 2 <= n - 1) return weno5_face(at(-2), at(-1), q0, at(1), at(2));
+            const double sl = pos >= 1 ? mc_slope(at(-1), q0, at(1)) : 0.0;
+            return q0 + 0.5 * sl;
+        }
+        const double q0 = at(1);
+        if (pos - 1 >= 0 && pos + 3 <= n - 1) return weno5_face(at(3), at(2), q0, at(0), at(-1));
+        const double sl = pos + 1 <= n - 2 ? mc_slope(at(0), q0, at(2)) : 0.0;
+        return q0 - 0.5 * sl;
+    };
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j)
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t f = g.at(i, j, k);
+                if (i < NX - 1)
+                {
+                    const double m = double(mfx[f]);
+                    sfx[f] = S(m * face([&](int o) { return Q(i + o, j, k); }, i, NX, m));
+                }
+                if (j < NY - 1)
+                {
+                    const double m = double(mfy[f]);
+                    sfy[f] = S(m * face([&](int o) { return Q(i, j + o, k); }, j, NY, m));
+                }
+                if (k < NZ - 1)
+                {
+                    const double m = double(mfz[f]);
+                    sfz[f] = S(m * face([&](int o) { return Q(i, j, k + o); }, k, NZ, m));
+                }
+                else if (open_top)
+                {
+                    sfz[f] = S(double(mfz[f]) * Q(i, j, k));
+                }
+            }
+    if (positive_definite)
+    {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 0; i < NX; ++i)
+            for (int j = 0; j < NY; ++j)
+                for (int k = 0; k < NZ; ++k)
+                {
+                    const std::size_t c = g.at(i, j, k);
+                    double out_rate = 0.0;
+                    if (i <= NX - 2 && sfx[c] > S(0)) out_rate += double(sfx[c]) * inv_dx;
+                    if (i >= 1 && sfx[g.at(i - 1, j, k)] < S(0)) out_rate -= double(sfx[g.at(i - 1, j, k)]) * inv_dx;
+                    if (j <= NY - 2 && sfy[c] > S(0)) out_rate += double(sfy[c]) * inv_dy;
+                    if (j >= 1 && sfy[g.at(i, j - 1, k)] < S(0)) out_rate -= double(sfy[g.at(i, j - 1, k)]) * inv_dy;
+                    if ((k <= NZ - 2 || (open_top && k == NZ - 1)) && sfz[c] > S(0)) out_rate += double(sfz[c]) * inv_dz;
+                    if (k >= 1 && sfz[g.at(i, j, k - 1)] < S(0)) out_rate -= double(sfz[g.at(i, j, k - 1)]) * inv_dz;
+                    double r = 1.0;
+                    if (out_rate > 0.0)
+                    {
+                        const double avail = double(rho_begin[c]) * std::max(double(q[c]), 0.0) / dt;
+                        r = std::min(1.0, avail / out_rate);
+                    }
+                    float rs = float(r);
+                    if (r > 0.0 && r < 1.0) rs = pd_limiter_store(r, false);
+                    rl[c] = S(rs);
+                }
+    }
+    auto limited = [&](S raw, std::size_t lo, std::size_t hi) -> double {
+        if (!positive_definite || raw == S(0)) return double(raw);
+        const S r = raw > S(0) ? rl[lo] : rl[hi];
+        return double(raw) * double(r);
+    };
+    out = q;
+    const double rho_floor = double(kRhoMin);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < NX - 1; ++i)
+        for (int j = 1; j < NY - 1; ++j)
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                const std::size_t xm = g.at(i - 1, j, k), xp = g.at(i + 1, j, k);
+                const std::size_t ym = g.at(i, j - 1, k), yp = g.at(i, j + 1, k);
+                const bool has_top = k <= NZ - 2 || (open_top && k == NZ - 1);
+                const double sxp = limited(sfx[c], c, xp);
+                const double sxm = limited(sfx[xm], xm, c);
+                const double syp = limited(sfy[c], c, yp);
+                const double sym = limited(sfy[ym], ym, c);
+                const double szp = has_top ? limited(sfz[c], c, k < NZ - 1 ? g.at(i, j, k + 1) : c) : 0.0;
+                const double szm = k >= 1 ? limited(sfz[g.at(i, j, k - 1)], g.at(i, j, k - 1), c) : 0.0;
+                const double rb = double(rho_begin[c]);
+                double rho_phi;
+                if (positive_definite)
+                {
+                    double outgoing = 0.0, incoming = 0.0;
+                    auto plus = [&](double fl, double inv) { if (fl >= 0.0) outgoing += fl * inv; else incoming -= fl * inv; };
+                    auto minus = [&](double fl, double inv) { if (fl <= 0.0) outgoing -= fl * inv; else incoming += fl * inv; };
+                    plus(sxp, inv_dx); minus(sxm, inv_dx);
+                    plus(syp, inv_dy); minus(sym, inv_dy);
+                    if (has_top) plus(szp, inv_dz);
+                    if (k >= 1) minus(szm, inv_dz);
+                    rho_phi = (rb * double(q[c]) - dt * outgoing) + dt * incoming;
+                }
+                else
+                {
+                    const double div = (sxp - sxm) * inv_dx + (syp - sym) * inv_dy + (szp - szm) * inv_dz;
+                    rho_phi = rb * double(q[c]) - dt * div;
+                }
+                const double re = double(rho_end[c]);
+                const double den = re > rho_floor ? re : rho_floor;
+                out[c] = S(rho_phi / den);
+            }
+}
+
+// ----------------------------------------------------------------------
+// Kessler warm rain (src/microphysics/schemes/kessler/kessler.cpp:103-531).
+// Returns per-cell tendencies for theta, qv, qc, qr on the physical domain
+// i in [1, nx-1), j in [1, ny-1), all k; and the bottom sink rate per column
+// (kessler.cpp:533-561).
+// ----------------------------------------------------------------------
+struct KesslerParams
+{
+    double qc0 = 1.0e-3, c_auto = 1.0e-3, c_accr = 2.2, evap_eff = 1.0, vt_max = 20.0;
+};
+
+template <class S>
+void kessler_tendencies(const Grid& g, const KesslerParams& kp, const std::vector<S>& p, const std::vector<S>& theta,
+                        const std::vector<S>& rho, const std::vector<S>& qv, const std::vector<S>& qc,
+                        const std::vector<S>& qr, double dt, std::vector<S>& dth, std::vector<S>& dqv,
+                        std::vector<S>& dqc, std::vector<S>& dqr)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    for (auto* f : {&dth, &dqv, &dqc, &dqr}) f->assign(g.cells(), S(0));
+    if (!(dt > 0.0) || !std::isfinite(dt)) return;
+    const double inv_dt = 1.0 / dt;
+    const double L = kLv / kCp;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < NX - 1; ++i)
+        for (int j = 1; j < NY - 1; ++j)
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                // saturation adjustment on copies (kessler.cpp:169-208)
+                const S T0s = S(theta_to_temperature(double(theta[c]), double(p[c])));
+                double T = double(T0s), qva = double(qv[c]), qca = double(qc[c]);
+                T = saturation_adjust(T, double(p[c]), qva, qca);
+                const S Ts = S(T), qvs_ = S(qva), qcs_ = S(qca);
+                const double dqc_adj = double(qcs_) - double(qc[c]);
+                if (dqc_adj != 0.0)
+                {
+                    const double rate = dqc_adj * inv_dt;
+                    dqc[c] = S(double(dqc[c]) + double(S(rate)));
+                    dqv[c] = S(double(dqv[c]) - double(S(rate)));
+                    dth[c] = S(double(dth[c]) + double(S(L * rate)));
+                }
+                // warm rain on the adjusted state (kessler.cpp:346-460)
+                const S qcv = qcs_, qrv = qr[c], qvv = qvs_;
+                S prod = S(0);
+                if (qcv > S(0))
+                {
+                    const double factor = 1.0 / (1.0 + kp.c_accr * dt * std::pow(double(std::max(S(0), qrv)), 0.875));
+                    const double amount = double(qcv) * (1.0 - factor)
+                                        + factor * kp.c_auto * dt * std::max(double(qcv) - kp.qc0, 0.0);
+                    prod = S(std::clamp(amount, 0.0, double(qcv)) / dt);
+                    dqc[c] = dqc[c] - prod;
+                    dqr[c] = dqr[c] + prod;
+                }
+                if (qrv > S(0) || prod > S(0))
+                {
+                    const double pv = double(p[c]), rv = double(rho[c]);
+                    const double qsat = qvs_water(double(Ts), pv);
+                    if (qsat > 0.0 && double(qvv) < qsat && rv > 0.0 && pv > 0.0)
+                    {
+                        const double qr_av = double(qrv) + double(prod) * dt;
+                        const double rcgs = 1.0e-3 * rv;
+                        const double rc = rcgs * std::max(qr_av, 0.0);
+                        const double vent = (1.6 + 124.9 * std::pow(rc, 0.2046)) * std::pow(rc, 0.525);
+                        const double resist = 2.55e8 / (pv * qsat) + 5.4e5;
+                        const double raw = kp.evap_eff * (vent / resist) * ((qsat - double(qvv)) / (rcgs * qsat));
+                        const double gsat = (kLv * kLv * qsat) / (kCp * kRv * std::max(double(Ts) * double(Ts), 1.0));
+                        const double bound = (qsat - double(qvv)) / (1.0 + gsat);
+                        const double ev = std::clamp(raw, 0.0, std::min(bound / dt, qr_av / dt));
+                        dqv[c] = S(double(dqv[c]) + ev);
+                        dqr[c] = S(double(dqr[c]) - ev);
+                        dth[c] = S(double(dth[c]) - L * ev);
+                    }
+                }
+            }
+    // sedimentation of the pre-process rain, donor then receiver (kessler.cpp:465-531)
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < NX - 1; ++i)
+        for (int j = 1; j < NY - 1; ++j)
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                const double rk = double(rho[c]);
+                const double rkm = k > 0 ? double(rho[g.at(i, j, k - 1)]) : rk;
+                const double qv_ = std::max(0.0, double(qr[c]));
+                if (qv_ > 0.0)
+                {
+                    const double Vt = std::min(kessler_rain_speed(qv_, rk) * std::sqrt(1.2 / rk), kp.vt_max);
+                    const S flux = S(Vt * qv_);
+                    dqr[c] = dqr[c] - S(double(flux) / g.dz);
+                    if (k > 0)
+                    {
+                        const std::size_t cm = g.at(i, j, k - 1);
+                        dqr[cm] = dqr[cm] + S(rk / rkm * double(flux) / g.dz);
+                    }
+                }
+            }
+    // temperature tendency to theta tendency (kessler.cpp:326-338)
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < NX - 1; ++i)
+        for (int j = 1; j < NY - 1; ++j)
+            for (int k = 0; k < NZ; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                dth[c] = S(theta_tendency_from_temperature(double(dth[c]), double(p[c])));
+            }
+}
+
+template <class S>
+double kessler_bottom_sink(const Grid& g, const KesslerParams& kp, const std::vector<S>& rho,
+                           const std::vector<S>& qr, int i, int j)
+{
+    // kessler.cpp:533-561
+    const std::size_t c = g.at(i, j, 0);
+    const double r0 = double(rho[c]);
+    const S q = qr[c];
+    if (!(q > S(0))) return 0.0;
+    const double Vt = std::min(kessler_rain_speed(double(q), r0) * std::sqrt(1.2 / r0), kp.vt_max);
+    return double(S(S(Vt) * q)) / g.dz;
+}
+
+// Commit of the microphysics tendencies with the reference's bounds
+// (microphysics_step.cpp:1090-1140: theta step limited to +-50 K and clamped
+// to [200, 700] K, water species clamped to [0, 0.05]) and the surface rain
+// ledger (microphysics_step.cpp:1917-2024: precipitation flux = sink*rho*dz,
+// 1 kg m^-2 = 1 mm).
+template <class S>
+void microphysics_step(const Grid& g, const Config& cfg, State<S>& s, double dt)
+{
+    const KesslerParams kp{cfg.qc0, cfg.c_auto, cfg.c_accr, cfg.evap_eff, cfg.vt_max};
+    // sedimentation subcycling limit 0.8*dz/Vt_max(q_total) (microphysics_step.cpp:3501-3560)
+    double vmax = 0.0;
+    for (int i = 1; i < g.nx - 1; ++i)
+        for (int j = 1; j < g.ny - 1; ++j)
+            for (int k = 0; k < g.nz; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                const double qt = double(s.qv[c]) + double(s.qc[c]) + double(s.qr[c]);
+                const double r = double(s.rho[c]);
+                vmax = std::max(vmax, std::min(cfg.vt_max, kessler_rain_speed(qt, r) * std::sqrt(1.2 / r)));
+            }
+    const double limit = vmax > 0.0 ? 0.8 * g.dz / vmax : std::numeric_limits<double>::infinity();
+    const int nsub = dt <= limit ? 1 : int(std::ceil(dt / limit));
+    const double h = dt / nsub;
+    std::vector<S> dth, dqv, dqc, dqr;
+    for (int n = 0; n < nsub; ++n)
+    {
+        kessler_tendencies(g, kp, s.p, s.theta, s.rho, s.qv, s.qc, s.qr, h, dth, dqv, dqc, dqr);
+        for (int i = 1; i < g.nx - 1; ++i)
+            for (int j = 1; j < g.ny - 1; ++j)
+            {
+                const double sink = kessler_bottom_sink(g, kp, s.rho, s.qr, i, j);
+                s.rain_mm[std::size_t(i) * g.ny + j] += sink * double(s.rho[g.at(i, j, 0)]) * g.dz * h;
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 1; i < g.nx - 1; ++i)
+            for (int j = 1; j < g.ny - 1; ++j)
+                for (int k = 0; k < g.nz; ++k)
+                {
+                    const std::size_t c = g.at(i, j, k);
+                    const double step = std::clamp(double(dth[c]) * h, -50.0, 50.0);
+                    s.theta[c] = S(clamp_store<S>(double(s.theta[c]) + step, 300.0, kThetaMin, kThetaMax));
+                    s.qv[c] = S(clamp_store<S>(double(s.qv[c]) + double(dqv[c]) * h, 0.0, 0.0f, kQvMax));
+                    s.qc[c] = S(clamp_store<S>(double(s.qc[c]) + double(dqc[c]) * h, 0.0, 0.0f, kQhMax));
+                    s.qr[c] = S(clamp_store<S>(double(s.qr[c]) + double(dqr[c]) * h, 0.0, 0.0f, kQhMax));
+                }
+    }
+}
+
+// EOS pressure projection after physics (src/core/orchestration/dynamics/eos_apply.cpp:50-110).
+template <class S>
+void eos_projection(State<S>& s)
+{
+    const std::size_t n = s.p.size();
+#pragma omp parallel for schedule(static)
+    for (std::size_t c = 0; c < n; ++c)
+    {
+        const double r = double(s.rho[c]), th = double(s.theta[c]);
+        if (!(r > 0.0) || !(th > 0.0)) continue;
+        const double pd = eos_pressure(r, th);
+        if (!std::isfinite(pd) || pd <= 0.0) continue;
+        s.p[c] = S(clamp_store<S>(pd, 101325.0, kPMin, kPMax));
+    }
+}
+
+// ----------------------------------------------------------------------
+// One large step in the reference order (dynamics.cpp:6585-7420):
+// slow Heun RK2 + apply, zero-time BC closure and KW78 seed, N acoustic
+// substeps with mass-flux accumulation and per-substep BCs, WENO5 transport
+// of theta, qv, qc, qr with the retained fluxes, Kessler, EOS projection and
+// the final BC closure.
+// ----------------------------------------------------------------------
+template <class S>
+int large_step(const Grid& g, const Base& b, const Config& cfg, State<S>& s)
+{
+    const int N = resolve_substeps(g, b, cfg);
+    const double dts = cfg.dt / N;
+    const double weight = dts / cfg.dt;
+    slow_rk2_and_apply(g, b, cfg, s);
+    apply_boundaries(g, cfg, s, 0.0);
+    seed_radiation(g, s);
+    std::vector<S> mfx(g.cells(), S(0)), mfy(g.cells(), S(0)), mfz(g.cells(), S(0));
+    const std::vector<S> rho_begin = s.rho;
+    std::vector<double> wz;
+    for (int n = 0; n < N; ++n)
+    {
+        if (cfg.acoustic_mode == kAcousticHEVI)
+        {
+            std::vector<S> zero(g.cells(), S(0));
+            accumulate_mass_flux(g, cfg, s, weight, mfx, mfy, zero);
+            acoustic_substep_hevi(g, b, s, dts, cfg.hevi_beta, wz);
+            for (std::size_t c = 0; c < g.cells(); ++c) mfz[c] = S(double(mfz[c]) + weight * wz[c]);
+        }
+        else
+        {
+            accumulate_mass_flux(g, cfg, s, weight, mfx, mfy, mfz);
+            acoustic_substep_explicit(g, b, s, dts);
+        }
+        apply_boundaries(g, cfg, s, dts);
+    }
+    std::vector<S> tmp;
+    transport_scalar(g, cfg, s.theta, tmp, mfx, mfy, mfz, rho_begin, s.rho, cfg.dt, false);
+    s.theta.swap(tmp);
+    transport_scalar(g, cfg, s.qv, tmp, mfx, mfy, mfz, rho_begin, s.rho, cfg.dt, true);
+    s.qv.swap(tmp);
+    transport_scalar(g, cfg, s.qc, tmp, mfx, mfy, mfz, rho_begin, s.rho, cfg.dt, true);
+    s.qc.swap(tmp);
+    transport_scalar(g, cfg, s.qr, tmp, mfx, mfy, mfz, rho_begin, s.rho, cfg.dt, true);
+    s.qr.swap(tmp);
+    microphysics_step(g, cfg, s, cfg.dt);
+    eos_projection(s);
+    apply_boundaries(g, cfg, s, 0.0);
+    return N;
+}
+
+}  // namespace oracle

@@ -1,0 +1,295 @@
+"""HIP backend parity against the CPU oracle, through the C ABI.
+
+Per-operation tests drive the reference dispatch contracts
+(nimbus_hip_dispatch_*) with the same seeded inputs the oracle gets in its
+reference-storage (FP32 storage, FP64 arithmetic) mode; the tolerances are
+the reference's own parity tiers (src/compute/parity_comparison.cpp:118-121:
+Pointwise rel 1e-5 / abs 1e-6, Stencil rel 1e-4 / abs 5e-5, Microphysics rel
+1e-4 / abs 1e-7, MicrophysicsHeating rel 1e-4 / abs 6e-4). Whole-step tests
+compare the FP64 HIP path with the FP64 oracle (reordering only: rel 1e-9)
+and the mixed path with the FP64 oracle at physics-level tolerances.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from nimbus_amd import lib as L
+from oracle import oracle_py as O
+from tests.helpers import max_abs, oracle_grid, small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def within_tier(gpu, ref, rel, abs_):
+    gpu = np.asarray(gpu, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    bad = np.abs(gpu - ref) > abs_ + rel * np.abs(ref)
+    return int(bad.sum()), float(np.max(np.abs(gpu - ref)))
+
+
+# ---------------------------------------------------------------- per-operation dispatch
+
+def test_acoustic_substep_fused_matches_oracle(hip_ctx):
+    g, base, st = small_case(wind_noise=2.0)
+    dts = 0.5
+    ref = O.acoustic_substep(oracle_grid(g), base, st, dts, np.float32)
+    a = {k: np.ascontiguousarray(st[k], dtype=np.float32).copy() for k in ("u", "v", "w", "rho", "p", "theta")}
+    d = L.AcousticFusedDispatch(
+        L.fptr(a["u"]), L.fptr(a["v"]), L.fptr(a["w"]), L.fptr(a["rho"]), L.fptr(a["p"]), L.fptr(a["theta"]),
+        L.dptr(base["rho0"]), L.dptr(base["p0"]), g.nz, g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, dts,
+        1004.0 / 717.0, 0.05, 2.0, 1000.0, 200000.0, 100000.0, 180.0, 120.0)
+    L.check(hip_ctx.lib.nimbus_hip_dispatch_acoustic_substep_fused(hip_ctx.handle, ctypes.byref(d)), "acoustic")
+    for f, (rel, ab) in {"u": (1e-5, 1e-5), "v": (1e-5, 1e-5), "w": (1e-5, 1e-5), "rho": (1e-6, 1e-7),
+                         "p": (1e-6, 0.05)}.items():
+        nbad, err = within_tier(a[f], ref[f], rel, ab)
+        print(f"acoustic {f}: max abs diff {err:.3e}")
+        assert nbad == 0, f"{f}: {nbad} cells outside tier, max diff {err}"
+
+
+def test_acoustic_hydrostatic_rest_is_exact(hip_ctx):
+    """Hydrostatic identity: rho = rho0, p = EOS(rho0, theta0), u = v = w = 0 stays at rest
+    (tests/dynamics/test_acoustic_cartesian_cgrid_gpu_parity.cpp section 1 of the reference)."""
+    g = L.make_grid(10, 9, 8, 1000.0, 1000.0, 400.0)
+    base = L.base_state(g)
+    shape = (g.nx, g.ny, g.nz)
+    th = np.broadcast_to(base["theta0"], shape).astype(np.float32).copy()
+    rho = np.broadcast_to(base["rho0"], shape).astype(np.float32).copy()
+    p = np.broadcast_to(base["p0"], shape).astype(np.float32).copy()
+    u = np.zeros(shape, np.float32)
+    v = np.zeros(shape, np.float32)
+    w = np.zeros(shape, np.float32)
+    d = L.AcousticFusedDispatch(L.fptr(u), L.fptr(v), L.fptr(w), L.fptr(rho), L.fptr(p), L.fptr(th),
+                                L.dptr(base["rho0"]), L.dptr(base["p0"]), g.nz, g.nx, g.ny, g.nz, g.dx, g.dy, g.dz,
+                                0.5, 1004.0 / 717.0, 0.05, 2.0, 1000.0, 200000.0, 100000.0, 180.0, 120.0)
+    L.check(hip_ctx.lib.nimbus_hip_dispatch_acoustic_substep_fused(hip_ctx.handle, ctypes.byref(d)), "acoustic")
+    # With p re-diagnosed from the stored FP32 rho and theta the residual
+    # pressure-gradient force is at FP32 rounding of the inputs.
+    assert np.max(np.abs(w)) < 2e-3
+    assert np.max(np.abs(u)) < 2e-3 and np.max(np.abs(v)) < 2e-3
+
+
+def test_kessler_matches_oracle(hip_ctx):
+    g, base, st = small_case(cloudy=True)
+    dt = 3.0
+    cfg = O.config(dt, 6)
+    ref = O.kessler_tendencies(oracle_grid(g), cfg, st["p"], st["theta"], st["rho"], st["qv"], st["qc"], st["qr"], dt)
+    a = {k: np.ascontiguousarray(st[k], dtype=np.float32) for k in ("p", "theta", "rho", "qv", "qc", "qr")}
+    outs = [np.zeros_like(a["p"]) for _ in range(4)]
+    d = L.KesslerDispatch(*[L.fptr(a[k]) for k in ("p", "theta", "rho", "qv", "qc", "qr")],
+                          *[L.fptr(o) for o in outs], g.nx, g.ny, g.nz, g.dz, dt, 1e-3, 1e-3, 2.2, 1.0, 20.0)
+    L.check(hip_ctx.lib.nimbus_hip_dispatch_kessler(hip_ctx.handle, ctypes.byref(d)), "kessler")
+    # FP32 microphysics arithmetic against the FP64-arithmetic oracle: the
+    # reference's Microphysics / MicrophysicsHeating tiers, widened by the
+    # 0.01 % knife-edge allowance for branch flips (parity_check.hpp).
+    tiers = {0: (1e-3, 6e-4), 1: (1e-3, 1e-7), 2: (1e-3, 1e-7), 3: (1e-3, 1e-7)}
+    for q, name in enumerate(("dth", "dqv", "dqc", "dqr")):
+        nbad, err = within_tier(outs[q], ref[q], *tiers[q])
+        print(f"kessler {name}: max abs diff {err:.3e}, outside {nbad}")
+        assert nbad <= max(1, outs[q].size // 10000), f"{name}: {nbad} cells outside tier (max {err})"
+
+
+def _transport_inputs(g, base, st, dt, rng):
+    shape = (g.nx, g.ny, g.nz)
+    rho_b = st["rho"].astype(np.float32)
+    u = st["u"] + 5.0 * rng.standard_normal(shape)
+    v = st["v"] + 5.0 * rng.standard_normal(shape)
+    w = 2.0 * rng.standard_normal(shape)
+    w[:, :, -1] = 0.0
+    rb = rho_b.astype(np.float64)
+    mfx = 0.5 * (rb + np.roll(rb, -1, 0)) * u
+    mfy = 0.5 * (rb + np.roll(rb, -1, 1)) * v
+    mfz = 0.5 * (rb + np.roll(rb, -1, 2)) * w
+    mfx[-1] = 0.0
+    mfy[:, -1] = 0.0
+    mfz[:, :, -1] = 0.0
+    div = np.zeros(shape)
+    div[1:] += mfx[1:] - mfx[:-1]
+    div[0] += mfx[0]
+    div /= g.dx
+    dy_ = np.zeros(shape)
+    dy_[:, 1:] += mfy[:, 1:] - mfy[:, :-1]
+    dy_[:, 0] += mfy[:, 0]
+    dz_ = np.zeros(shape)
+    dz_[:, :, 1:] += mfz[:, :, 1:] - mfz[:, :, :-1]
+    dz_[:, :, 0] += mfz[:, :, 0]
+    rho_e = (rb - dt * (div + dy_ / g.dy + dz_ / g.dz)).astype(np.float32)
+    return [x.astype(np.float32) for x in (mfx, mfy, mfz)], rho_b, rho_e
+
+
+def test_transport_matches_oracle(hip_ctx, rng):
+    g, base, st = small_case(nx=16, ny=14, nz=12, cloudy=True)
+    dt = 6.0
+    (mfx, mfy, mfz), rho_b, rho_e = _transport_inputs(g, base, st, dt, rng)
+    og = oracle_grid(g)
+    cfg = O.config(dt, 6)
+    names = ("theta", "qv", "qc", "qr")
+    pd = (0, 1, 1, 1)
+    ref = [O.transport(og, cfg, st[n], mfx, mfy, mfz, rho_b, rho_e, dt, p) for n, p in zip(names, pd)]
+    qs = [np.ascontiguousarray(st[n], dtype=np.float32).copy() for n in names]
+    d = L.AdvectScalarsDispatch()
+    d.n_scalars = 4
+    for s in range(4):
+        d.q[s] = L.fptr(qs[s])
+        d.positive_definite[s] = pd[s]
+    d.mass_flux_x, d.mass_flux_y, d.mass_flux_z = L.fptr(mfx), L.fptr(mfy), L.fptr(mfz)
+    d.rho_begin, d.rho_end = L.fptr(rho_b), L.fptr(rho_e)
+    d.nx, d.ny, d.nz, d.dx, d.dy, d.dz, d.dt, d.bc_top = g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, dt, 0
+    L.check(hip_ctx.lib.nimbus_hip_dispatch_advect_scalars(hip_ctx.handle, ctypes.byref(d)), "advect")
+    # FP32 offset-form WENO5 (the reference shader's form) against the FP64
+    # reconstruction: Stencil tier.
+    for n, q, r in zip(names, qs, ref):
+        nbad, err = within_tier(q, r, 1e-4, 5e-5 if n == "theta" else 1e-8)
+        print(f"transport {n}: max abs diff {err:.3e}, outside {nbad}")
+        assert nbad == 0, f"{n}: {nbad} outside tier (max {err})"
+        assert np.all(q[:, :, :] >= 0.0) or n == "theta"
+
+
+def test_slow_tendencies_match_oracle(hip_ctx):
+    g, base, st = small_case(nx=16, ny=14, nz=12, cloudy=True, wind_noise=1.0)
+    cfg = O.config(3.0, 6)
+    ref = O.slow_tendencies(oracle_grid(g), base, cfg, st)
+    a = {k: np.ascontiguousarray(st[k], dtype=np.float32) for k in ("u", "v", "w", "rho", "qv", "qc", "qr")}
+    outs = [np.zeros_like(a["u"]) for _ in range(3)]
+    d = L.SlowTendenciesDispatch(*[L.fptr(a[k]) for k in ("u", "v", "w", "rho", "qv", "qc", "qr")],
+                                 L.dptr(base["rho0"]), L.dptr(base["qv0"]), L.dptr(base["u0"]), L.dptr(base["v0"]),
+                                 *[L.fptr(o) for o in outs], g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, 0.0)
+    L.check(hip_ctx.lib.nimbus_hip_dispatch_slow_tendencies(hip_ctx.handle, ctypes.byref(d)), "slow")
+    for name, o, r in zip(("du", "dv", "dw"), outs, ref):
+        nbad, err = within_tier(o, r, 1e-4, 5e-5)
+        print(f"slow {name}: max abs diff {err:.3e}")
+        assert nbad == 0, f"{name}: {nbad} outside tier (max {err})"
+
+
+def test_slow_rk2_matches_oracle(hip_ctx):
+    g, base, st = small_case(nx=16, ny=14, nz=12, cloudy=True, wind_noise=1.0)
+    dt = 3.0
+    cfg = O.config(dt, 6)
+    ref = O.slow_rk2(oracle_grid(g), base, cfg, st, np.float32)
+    a = {k: np.ascontiguousarray(st[k], dtype=np.float32).copy() for k in ("u", "v", "w", "rho", "qv", "qc", "qr")}
+    d = L.SlowRk2Dispatch(*[L.fptr(a[k]) for k in ("u", "v", "w", "rho", "qv", "qc", "qr")],
+                          L.dptr(base["rho0"]), L.dptr(base["qv0"]), L.dptr(base["u0"]), L.dptr(base["v0"]),
+                          g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, dt, 0.0)
+    L.check(hip_ctx.lib.nimbus_hip_dispatch_slow_rk2(hip_ctx.handle, ctypes.byref(d)), "slow rk2")
+    for f in ("u", "v", "w"):
+        nbad, err = within_tier(a[f], ref[f], 1e-5, 1e-5)
+        print(f"slow rk2 {f}: max abs diff {err:.3e}")
+        assert nbad == 0
+
+
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_batched_substeps_with_boundaries_and_mass_flux(hip_ctx, mode):
+    """N substeps + per-substep KW78 radiative BCs + time-averaged mass-flux
+    export (split_explicit.cpp:209-236; dynamics.hpp:279-325) against the oracle."""
+    g, base, st = small_case(wind_noise=1.0)
+    n_sub, dts = 4, 0.5
+    weight = 0.25
+    cfg = O.config(n_sub * dts, n_sub, acoustic_mode=mode)
+    ref, mf_ref = O.acoustic_substeps_batched(oracle_grid(g), base, cfg, st, n_sub, dts, weight)
+    a = {k: np.ascontiguousarray(st[k], dtype=np.float32).copy() for k in ("u", "v", "w", "rho", "p", "theta")}
+    mf = [np.zeros_like(a["u"]) for _ in range(3)]
+    d = L.AcousticBatchedDispatch(
+        *[L.fptr(a[k]) for k in ("u", "v", "w", "rho", "p", "theta")], L.dptr(base["rho0"]), L.dptr(base["p0"]),
+        g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, dts, n_sub, mode, 0.1, 1, 1, 1, 1, 0, 0, L.fptr(mf[0]), L.fptr(mf[1]),
+        L.fptr(mf[2]), 1, weight)
+    L.check(hip_ctx.lib.nimbus_hip_dispatch_acoustic_substeps_batched(hip_ctx.handle, ctypes.byref(d)), "batched")
+    for f, (rel, ab) in {"u": (1e-5, 1e-4), "v": (1e-5, 1e-4), "w": (1e-5, 1e-4), "rho": (1e-6, 1e-6),
+                         "p": (1e-6, 0.1)}.items():
+        nbad, err = within_tier(a[f], ref[f], rel, ab)
+        print(f"batched mode {mode} {f}: max abs diff {err:.3e}")
+        assert nbad == 0, f
+    for q in range(3):
+        nbad, err = within_tier(mf[q], mf_ref[q], 1e-4, 5e-5)  # Stencil tier
+        print(f"batched mode {mode} mass flux {q}: max abs diff {err:.3e}")
+        assert nbad == 0
+
+
+# ---------------------------------------------------------------- whole large step
+
+def _run_member(ctx, g, base, cfg, st, precision, n_steps):
+    m = ctx.member(g, cfg, base, precision)
+    try:
+        m.upload(st)
+        ctx.step([m], n_steps)
+        return m.download(), m.diag()
+    finally:
+        m.destroy()
+
+
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_large_step_fp64_matches_fp64_oracle(hip_ctx, mode):
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, dtype=np.float64)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode)
+    n_steps = 4
+    ref, n = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
+    out, diag = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, n_steps)
+    scale = {"u": 30.0, "v": 30.0, "w": 10.0, "rho": 1.0, "p": 1e5, "theta": 300.0, "qv": 0.02, "qc": 1e-3,
+             "qr": 1e-3}
+    for f in L.FIELDS:
+        err = max_abs(out[f], ref[f]) / scale[f]
+        print(f"fp64 mode {mode} {f}: scaled max diff {err:.3e}")
+        assert err < 1e-9, f
+    assert max_abs(out["rain_mm"], ref["rain_mm"]) < 1e-9 * max(1.0, float(np.max(ref["rain_mm"])))
+
+
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_large_step_mixed_close_to_fp64_oracle(hip_ctx, mode):
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, dtype=np.float64)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode)
+    n_steps = 4
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
+    out, diag = _run_member(hip_ctx, g, base, cfg, {k: (v.astype(np.float32) if k in L.FIELDS else v)
+                                                    for k, v in st.items()}, L.PRECISION_MIXED, n_steps)
+    # FP32 storage rounding (2^-23 of the field scale) amplified at most 1000x
+    # over four steps: u, v, w ~ 30 m/s, rho ~ 1, p ~ 1e5 Pa, theta ~ 300 K, q ~ 0.014.
+    eps32 = 2.0 ** -23
+    tol = {"u": 30 * eps32 * 1e3, "v": 30 * eps32 * 1e3, "w": 30 * eps32 * 1e3, "rho": eps32 * 1e3,
+           "p": 1e5 * eps32 * 1e3, "theta": 300 * eps32 * 1e3, "qv": 0.014 * eps32 * 1e3,
+           "qc": 0.014 * eps32 * 1e3, "qr": 0.014 * eps32 * 1e3}
+    for f in L.FIELDS:
+        err = max_abs(out[f], ref[f])
+        print(f"mixed mode {mode} {f}: max diff {err:.3e}")
+        assert err < tol[f], f
+
+
+def test_graph_replay_matches_eager(hip_ctx):
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True)
+    cfg = L.step_config(3.0, 6)
+    eager, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_MIXED, 3)
+    hip_ctx.set_graphs(True)
+    try:
+        m = hip_ctx.member(g, cfg, base, L.PRECISION_MIXED)
+        m.upload(st)
+        for _ in range(3):  # several replays with host work in between
+            hip_ctx.step([m], 1)
+            _ = m.diag()
+        graphed = m.download()
+        m.destroy()
+    finally:
+        hip_ctx.set_graphs(False)
+    for f in L.FIELDS:
+        assert np.array_equal(graphed[f], eager[f]), f
+
+
+def test_member_alone_equals_member_in_mix(hip_ctx):
+    """Bitwise identity of a member advanced alone and interleaved with others
+    of different grid sizes (serving's 'same alone and in the mix')."""
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True)
+    cfg = L.step_config(3.0, 6)
+    alone, d_alone = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_MIXED, 3)
+    g2, base2, st2 = small_case(nx=26, ny=15, nz=11, seed=3)
+    others = []
+    m = hip_ctx.member(g, cfg, base, L.PRECISION_MIXED)
+    m.upload(st)
+    for q in range(2):
+        o = hip_ctx.member(g2, L.step_config(2.0, 4), base2, L.PRECISION_MIXED)
+        o.upload(st2)
+        others.append(o)
+    hip_ctx.step([others[0], m, others[1]], 3)
+    mixed = m.download()
+    d_mixed = m.diag()
+    for o in others + [m]:
+        o.destroy()
+    for f in L.FIELDS:
+        assert np.array_equal(mixed[f], alone[f]), f
+    assert d_mixed["max_w"] == d_alone["max_w"] and d_mixed["rms_theta_p"] == d_alone["rms_theta_p"]
