@@ -180,6 +180,13 @@ int nimbus_hip_step(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_s
 /* Same, enqueued on the context stream without waiting. */
 int nimbus_hip_step_async(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_steps);
 int nimbus_hip_synchronize(nimbus_hip_ctx* ctx);
+/* Launch tiling of a member's cell kernels: tile_w in {16, 32, 64, 128, 256} cells along i
+ * (tile height 256 / tile_w along j) and XCD-contiguous tile ordering on/off. */
+int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int xcd_remap);
+/* Autotuner: times `trials` large steps per tiling candidate on a saved copy of the
+ * member state (restored afterwards) and keeps the fastest; reports it. */
+int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int* best_tile_w, int* best_remap,
+                               double* best_ms);
 /* Enable hipGraph capture of whole large steps (per member and substep count). */
 int nimbus_hip_set_graphs(nimbus_hip_ctx* ctx, int enable);
 /* Per-kernel timing with HIP events on the launch stream (0 disables). */

@@ -189,9 +189,10 @@ __device__ __forceinline__ float weno5_offset(float a, float b, float c, float d
     const float mag = fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d))), fabsf(e));
     const float scale = fmaxf(mag, 1e-20f);
     if (!isfinite(scale)) return 0.0f;
+    const float inv = 1.0f / scale;
     auto diff = [=](float x) {
         const float dlt = x - c;
-        return isfinite(dlt) ? dlt / scale : x / scale - c / scale;
+        return isfinite(dlt) ? dlt * inv : x * inv - c * inv;
     };
     const float ac = diff(a), bc = diff(b), dc = diff(d), ec = diff(e);
     const float p0 = (1.0f / 3.0f) * ac - (7.0f / 6.0f) * bc;
@@ -203,13 +204,15 @@ __device__ __forceinline__ float weno5_offset(float a, float b, float c, float d
     const float b0 = (13.0f / 12.0f) * s0a * s0a + 0.25f * s0b * s0b;
     const float b1 = (13.0f / 12.0f) * s1a * s1a + 0.25f * s1b * s1b;
     const float b2 = (13.0f / 12.0f) * s2a * s2a + 0.25f * s2b * s2b;
-    const float rel = mag / scale;
-    const float flo = 1e-20f / scale;
+    const float rel = mag * inv;
+    const float flo = 1e-20f * inv;
     const float epsv = 1e-6f * rel * rel + flo * flo;
-    const float mn = fminf(fminf(epsv + b0, epsv + b1), epsv + b2);
-    const float r0 = mn / (epsv + b0), r1 = mn / (epsv + b1), r2 = mn / (epsv + b2);
-    const float a0 = 0.1f * r0 * r0, a1 = 0.6f * r1 * r1, a2 = 0.3f * r2 * r2;
-    const float off = scale * ((a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2));
+    // With the stencil scaled to O(1), eps + beta >= 1e-6 (or >= 1 for an
+    // all-tiny stencil), so 1 / (eps + beta)^2 stays below 1e12: the weights
+    // need no further normalisation and reciprocals replace the divisions.
+    const float t0 = __frcp_rn(epsv + b0), t1 = __frcp_rn(epsv + b1), t2 = __frcp_rn(epsv + b2);
+    const float a0 = 0.1f * t0 * t0, a1 = 0.6f * t1 * t1, a2 = 0.3f * t2 * t2;
+    const float off = scale * ((a0 * p0 + a1 * p1 + a2 * p2) * __frcp_rn(a0 + a1 + a2));
     return isfinite(off) ? off : 0.0f;
 }
 
@@ -283,11 +286,11 @@ __device__ __forceinline__ A upwind_axis(F at, int pos, int n, A a, A inv)
     if (half == 1) return a >= A(0) ? (c - m1) * inv : (p1 - c) * inv;
     const A m2 = at(-2), p2 = at(2);
     if (half == 2)
-        return a >= A(0) ? (A(2) * p1 + A(3) * c - A(6) * m1 + m2) * (inv / A(6))
-                         : (-p2 + A(6) * p1 - A(3) * c - A(2) * m1) * (inv / A(6));
+        return a >= A(0) ? (A(2) * p1 + A(3) * c - A(6) * m1 + m2) * (inv * A(1.0 / 6.0))
+                         : (-p2 + A(6) * p1 - A(3) * c - A(2) * m1) * (inv * A(1.0 / 6.0));
     const A m3 = at(-3), p3 = at(3);
-    return a >= A(0) ? (-A(2) * m3 + A(15) * m2 - A(60) * m1 + A(20) * c + A(30) * p1 - A(3) * p2) * (inv / A(60))
-                     : (A(2) * p3 - A(15) * p2 + A(60) * p1 - A(20) * c - A(30) * m1 + A(3) * m2) * (inv / A(60));
+    return a >= A(0) ? (-A(2) * m3 + A(15) * m2 - A(60) * m1 + A(20) * c + A(30) * p1 - A(3) * p2) * (inv * A(1.0 / 60.0))
+                     : (A(2) * p3 - A(15) * p2 + A(60) * p1 - A(20) * c - A(30) * m1 + A(3) * m2) * (inv * A(1.0 / 60.0));
 }
 
 }  // namespace nimbus::dev

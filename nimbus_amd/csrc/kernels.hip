@@ -1,12 +1,13 @@
 // CDNA4 (gfx950) kernels of the Klemp-Wilhelmson split-explicit large step.
 //
 // Device layout: every 3-D field is [k][j][i] with i contiguous and a
-// 64-byte-padded row pitch, so a wavefront of 64 lanes reads 64 consecutive
-// i of one (j, k) row (coalesced) and vertical neighbours are one plane
-// apart. Cell kernels map blockIdx.y to k and flatten (i, j) over
-// blockIdx.x * 256 + threadIdx.x; column kernels give each thread one (i, j)
-// column and march k. rho and p are stored as perturbations from the FP64
-// base state; theta, winds and water species are stored whole.
+// 64-byte-padded row pitch, so the lanes of a wavefront read consecutive i of
+// two (j, k) rows (coalesced) and vertical neighbours are one plane apart.
+// Cell kernels give each 256-thread block a 32 x 8 (i, j) tile of one k
+// plane, ordered so that each XCD sweeps a contiguous slab of planes; column
+// kernels give each thread one (i, j) column and march k. rho and p are
+// stored as perturbations from the FP64 base state; theta, winds and water
+// species are stored whole.
 //
 // Precision: S is the storage type. Continuity, EOS, pressure gradient,
 // base-state terms, slow tendencies, mass-flux accumulation and the HE-VI
@@ -39,23 +40,40 @@ struct AccT<double>
     using type = double;
 };
 
-__device__ __forceinline__ bool cell_index(const DevGrid& g, int& i, int& j, int& k)
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on
+// XCD w % 8, each XCD with its own L2). Remapping w so that XCD x receives a
+// contiguous range of logical tiles keeps the j +- 1 rows and k +- 1 planes a
+// stencil reads in the same L2.
+__device__ __forceinline__ int xcd_logical(int w, int n)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= g.nx * g.ny) return false;
-    j = t / g.nx;
-    i = t - j * g.nx;
-    k = blockIdx.y;
-    return true;
+    constexpr int kXcd = 8;
+    const int x = w % kXcd, local = w / kXcd;
+    const int q = n / kXcd, r = n % kXcd;
+    return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
 }
 
+// Cell kernels: one tile_w x (256 / tile_w) (i, j) tile of one k plane per
+// 256-thread block; logical tile order is k-major.
+__device__ __forceinline__ bool cell_index(const DevGrid& g, int& i, int& j, int& k)
+{
+    const int n = g.tiles_plane * g.nz;
+    const int logical = g.xcd_remap ? xcd_logical(int(blockIdx.x), n) : int(blockIdx.x);
+    k = logical / g.tiles_plane;
+    const int t = logical - k * g.tiles_plane;
+    const int by = t / g.tiles_x, bx = t - by * g.tiles_x;
+    i = (bx << g.tile_shift) + int(threadIdx.x & unsigned(g.tile_w - 1));
+    j = (by << (8 - g.tile_shift)) + int(threadIdx.x >> g.tile_shift);
+    return i < g.nx && j < g.ny;
+}
+
+// Column kernels: one thread per (i, j) column, same tile shape.
 __device__ __forceinline__ bool column_index(const DevGrid& g, int& i, int& j)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= g.nx * g.ny) return false;
-    j = t / g.nx;
-    i = t - j * g.nx;
-    return true;
+    const int t = g.xcd_remap ? xcd_logical(int(blockIdx.x), g.tiles_plane) : int(blockIdx.x);
+    const int by = t / g.tiles_x, bx = t - by * g.tiles_x;
+    i = (bx << g.tile_shift) + int(threadIdx.x & unsigned(g.tile_w - 1));
+    j = (by << (8 - g.tile_shift)) + int(threadIdx.x >> g.tile_shift);
+    return i < g.nx && j < g.ny;
 }
 
 __device__ __forceinline__ double finite0(double x) { return isfinite(x) ? x : 0.0; }
@@ -90,7 +108,7 @@ __device__ void slow_tendency_cell(const DevGrid& g, const S* __restrict__ u, co
                                    double f, int i, int j, int k, double& du, double& dv, double& dw)
 {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const double ix = 1.0 / g.dx, iy = 1.0 / g.dy, iz = 1.0 / g.dz;
+    const double ix = g.idx, iy = g.idy, iz = g.idz;
     const std::size_t c = g.at(i, j, k);
     const std::size_t sx = 1, sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
     auto F = [&](const S* a, std::size_t idx) { return double(a[idx]); };
@@ -209,19 +227,51 @@ __global__ __launch_bounds__(kBlock) void k_slow_tendencies(DevGrid g, const S* 
 // C-grid continuity, density clamp and EOS pressure
 // (acoustic_cgrid_fast.cpp:49-157).
 // ---------------------------------------------------------------------
-template <class S>
-__device__ __forceinline__ void density_eos_store(const Bounds& bd, double rho0k, double p0k, double rn, double theta,
-                                                  S* rp_out, S* pp, std::size_t c)
+// Clamp in FP64 without rounding the whole value to the storage type, so the
+// stored perturbation keeps its own precision.
+__device__ __forceinline__ double clamp_full(double x, double fallback, float lo, float hi)
 {
+    if (!isfinite(x)) x = fallback;
+    return fmin(fmax(x, double(lo)), double(hi));
+}
+
+// Pressure perturbation p - p0 from the dry EOS. FP64 storage evaluates
+// p00 (rho R theta / p00)^gamma in FP64; FP32 storage evaluates the same
+// expression relative to the base state, p = pe0 (1 + x)^gamma with
+// 1 + x = rho theta / (rho0 theta0) and pe0 = EOS(rho0, theta0) in FP64, so
+// p - p0 = pe0 expm1(gamma log1p(x)) + (pe0 - p0) keeps FP32 relative
+// accuracy on the perturbation rather than on the full pressure.
+template <class S>
+__device__ __forceinline__ double eos_full_pressure(const double* base, int nz, int k, double rho, double theta)
+{
+    if constexpr (sizeof(S) == sizeof(float))
+    {
+        const double x = rho * theta * base[std::size_t(kBaseInvRT0) * nz + k] - 1.0;
+        const float e = expm1f(float(kGamma) * log1pf(float(x)));
+        return base[std::size_t(kBaseP0) * nz + k] + (double(e) * base[std::size_t(kBasePe0) * nz + k] +
+                                                      base[std::size_t(kBasePeDiff) * nz + k]);
+    }
+    else
+    {
+        return dev::eos_pressure(rho, theta);
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void density_eos_store(const Bounds& bd, const double* base, int nz, int k, double rn,
+                                                  double theta, S* rp_out, S* pp, std::size_t c)
+{
+    const double rho0k = base[std::size_t(kBaseRho0) * nz + k];
+    const double p0k = base[std::size_t(kBaseP0) * nz + k];
     if (!isfinite(rn) || rn <= 0.0) rn = fmax(0.1, rho0k);
-    const double rs = clamp_store<S>(rn, 1.0, bd.rho_min, bd.rho_max);
+    const double rs = clamp_full(rn, 1.0, bd.rho_min, bd.rho_max);
     const S rps = S(rs - rho0k);
     rp_out[c] = rps;
     const double rho_used = rho0k + double(rps);
     double pn = double(bd.p_fallback);
-    if (theta > 0.0) pn = dev::eos_pressure(rho_used, theta);
+    if (theta > 0.0) pn = eos_full_pressure<S>(base, nz, k, rho_used, theta);
     if (!isfinite(pn) || pn <= 0.0) pn = double(bd.p_fallback);
-    pp[c] = S(clamp_store<S>(pn, 101325.0, bd.p_min, bd.p_max) - p0k);
+    pp[c] = S(clamp_full(pn, 101325.0, bd.p_min, bd.p_max) - p0k);
 }
 
 template <class S>
@@ -273,10 +323,10 @@ __global__ __launch_bounds__(kBlock) void k_acoustic_forward(DevGrid g, const S*
         const double fzp = k + 1 < NZ ? 0.5 * (rc + (rho0[k + 1] + double(rp_in[c + sz]))) * double(w[c])
                                       : rc * double(w[c]);
         const double fzm = k > 0 ? 0.5 * ((rho0[k - 1] + double(rp_in[c - sz])) + rc) * double(w[c - sz]) : 0.0;
-        d = -((fxp - fxm) * (1.0 / g.dx) + (fyp - fym) * (1.0 / g.dy) + (fzp - fzm) * (1.0 / g.dz));
+        d = -((fxp - fxm) * g.idx + (fyp - fym) * g.idy + (fzp - fzm) * g.idz);
     }
     const S drho = S(finite0(d));
-    density_eos_store<S>(bd, r0, p0[k], rc + double(drho) * dts, double(th[c]), rp_out, pp, c);
+    density_eos_store<S>(bd, base, NZ, k, rc + double(drho) * dts, double(th[c]), rp_out, pp, c);
 }
 
 // Backward momentum update with the new pressure (acoustic_cgrid_fast.cpp:159-215);
@@ -304,10 +354,23 @@ __global__ __launch_bounds__(kBlock) void k_acoustic_backward(DevGrid g, S* __re
         const double rx = 0.5 * (rc + (r0 + double(rp_new[c + 1])));
         const double ry = 0.5 * (rc + (r0 + double(rp_new[c + sy])));
         const double rz = 0.5 * (rc + (rho0[k + 1] + double(rp_new[c + sz])));
-        const double pc = double(pp[c]);
-        du = double(S(finite0(-(double(pp[c + 1]) - pc) * (1.0 / g.dx) / safe_rho(rx))));
-        dv = double(S(finite0(-(double(pp[c + sy]) - pc) * (1.0 / g.dy) / safe_rho(ry))));
-        dw = double(S(finite0(-((double(pp[c + sz]) - pc) * (1.0 / g.dz)) / safe_rho(rz))));
+        if constexpr (sizeof(S) == sizeof(float))
+        {
+            // FP32 tendency, as the reference rounds it to FP32 before the update
+            auto safe_f = [](float r) { return isfinite(r) && r > 1.0e-6f ? r : 1.0f; };
+            auto fin = [](float x) { return isfinite(x) ? x : 0.0f; };
+            const float pc = float(pp[c]);
+            du = double(fin(-(float(pp[c + 1]) - pc) * float(g.idx) / safe_f(float(rx))));
+            dv = double(fin(-(float(pp[c + sy]) - pc) * float(g.idy) / safe_f(float(ry))));
+            dw = double(fin(-((float(pp[c + sz]) - pc) * float(g.idz)) / safe_f(float(rz))));
+        }
+        else
+        {
+            const double pc = double(pp[c]);
+            du = finite0(-(double(pp[c + 1]) - pc) * g.idx / safe_rho(rx));
+            dv = finite0(-(double(pp[c + sy]) - pc) * g.idy / safe_rho(ry));
+            dw = finite0(-((double(pp[c + sz]) - pc) * g.idz) / safe_rho(rz));
+        }
     }
     u[c] = S(clamp_store<S>(double(u[c]) + du * dts, 0.0, -bd.wind_h, bd.wind_h));
     v[c] = S(clamp_store<S>(double(v[c]) + dv * dts, 0.0, -bd.wind_h, bd.wind_h));
@@ -339,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(DevGrid g, const S* __re
     const double* rho0 = slot(base, kBaseRho0, NZ);
     const double* p0 = slot(base, kBaseP0, NZ);
     const bool interior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
-    const double idz = 1.0 / g.dz;
+    const double idz = g.idz;
     const double a1 = 1.0 - alpha;
     // ---- upward sweep: mass fluxes, rho*, p*, Thomas forward elimination ----
     double rho_prev = 0.0, rb_prev = 0.0, C_prev = 0.0, ps_prev = 0.0, wo_prev = 0.0, e_prev = 0.0;
@@ -367,13 +430,13 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(DevGrid g, const S* __re
         if (first) rb[c] = rp_in[c];
         if (!interior)
         {
-            density_eos_store<S>(bd, r0, p0[k], rc, double(th[c]), rp_out, pp, c);
+            density_eos_store<S>(bd, base, NZ, k, rc, double(th[c]), rp_out, pp, c);
             continue;
         }
         const double rxp = r0 + double(rp_in[c + 1]), rxm = r0 + double(rp_in[c - 1]);
         const double ryp = r0 + double(rp_in[c + sy]), rym = r0 + double(rp_in[c - sy]);
-        const double dh = -((0.5 * (rc + rxp) * double(u[c]) - 0.5 * (rxm + rc) * double(u[c - 1])) * (1.0 / g.dx) +
-                            (0.5 * (rc + ryp) * double(v[c]) - 0.5 * (rym + rc) * double(v[c - sy])) * (1.0 / g.dy));
+        const double dh = -((0.5 * (rc + rxp) * double(u[c]) - 0.5 * (rxm + rc) * double(u[c - 1])) * g.idx +
+                            (0.5 * (rc + ryp) * double(v[c]) - 0.5 * (rym + rc) * double(v[c - sy])) * g.idy);
         const double rstar = rc + dts * finite0(dh);
         const double ps = dev::eos_pressure(rstar, double(th[c]));
         const double C = kGamma * ps / rstar;
@@ -436,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(DevGrid g, const S* __re
         const double fb = r >= 0 ? rbr * (alpha * wn + a1 * wo) : 0.0;
         if (accumulate) mfz[cc] = S((first ? 0.0 : double(mfz[cc])) + weight * ft);
         const double rstar = rho0[kc] + double(rp_out[cc]);
-        density_eos_store<S>(bd, rho0[kc], p0[kc], rstar - dts * idz * (ft - fb), double(th[cc]), rp_out, pp, cc);
+        density_eos_store<S>(bd, base, NZ, kc, rstar - dts * idz * (ft - fb), double(th[cc]), rp_out, pp, cc);
         w_above = wn;
         wo_above = wo;
         rb_above = rbr;
@@ -636,34 +699,41 @@ struct ScalarSet
     int n;
 };
 
+// Face flux pass (advect_conservative_scalar_flux): each cell writes the raw
+// retained-flux scalar flux through its own upper x, y and z faces, stored in
+// the storage type like the reference's SFX/SFY/SFZ scratch.
 template <class S>
-struct FaceFluxes
+struct FluxSet
 {
-    S xp, xm, yp, ym, zp, zm;  // raw fluxes through the six faces of a cell
+    S* fx[4];
+    S* fy[4];
+    S* fz[4];
 };
 
 template <class S>
-__device__ __forceinline__ FaceFluxes<S> cell_face_fluxes(const DevGrid& g, const S* __restrict__ q,
-                                                          const S* __restrict__ mfx, const S* __restrict__ mfy,
-                                                          const S* __restrict__ mfz, int i, int j, int k,
-                                                          int open_top)
+__global__ __launch_bounds__(kBlock) void k_transport_flux(DevGrid g, ScalarSet<S> set, FluxSet<S> fl,
+                                                           const S* __restrict__ mfx, const S* __restrict__ mfy,
+                                                           const S* __restrict__ mfz, int open_top)
 {
     using A = typename AccT<S>::type;
+    int i, j, k;
+    if (!cell_index(g, i, j, k)) return;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const std::size_t c = g.at(i, j, k);
     const std::ptrdiff_t sy = g.px, sz = std::ptrdiff_t(g.px) * g.ny;
-    FaceFluxes<S> F{S(0), S(0), S(0), S(0), S(0), S(0)};
-    auto ax = [&](std::size_t lo) { return [=](int o) { return A(q[lo + std::ptrdiff_t(o)]); }; };
-    auto ay = [&](std::size_t lo) { return [=](int o) { return A(q[lo + std::ptrdiff_t(o) * sy]); }; };
-    auto az = [&](std::size_t lo) { return [=](int o) { return A(q[lo + std::ptrdiff_t(o) * sz]); }; };
-    if (i <= NX - 2) F.xp = S(dev::face_flux<A>(ax(c), i, NX, A(mfx[c])));
-    if (i >= 1) F.xm = S(dev::face_flux<A>(ax(c - 1), i - 1, NX, A(mfx[c - 1])));
-    if (j <= NY - 2) F.yp = S(dev::face_flux<A>(ay(c), j, NY, A(mfy[c])));
-    if (j >= 1) F.ym = S(dev::face_flux<A>(ay(c - sy), j - 1, NY, A(mfy[c - sy])));
-    if (k <= NZ - 2) F.zp = S(dev::face_flux<A>(az(c), k, NZ, A(mfz[c])));
-    else if (open_top) F.zp = S(A(mfz[c]) * A(q[c]));
-    if (k >= 1) F.zm = S(dev::face_flux<A>(az(c - sz), k - 1, NZ, A(mfz[c - sz])));
-    return F;
+    const A mx = A(mfx[c]), my = A(mfy[c]), mz = A(mfz[c]);
+    for (int s = 0; s < set.n; ++s)
+    {
+        const S* __restrict__ q = set.q[s];
+        S fx = S(0), fy = S(0), fz = S(0);
+        if (i <= NX - 2) fx = S(dev::face_flux<A>([=](int o) { return A(q[c + std::ptrdiff_t(o)]); }, i, NX, mx));
+        if (j <= NY - 2) fy = S(dev::face_flux<A>([=](int o) { return A(q[c + std::ptrdiff_t(o) * sy]); }, j, NY, my));
+        if (k <= NZ - 2) fz = S(dev::face_flux<A>([=](int o) { return A(q[c + std::ptrdiff_t(o) * sz]); }, k, NZ, mz));
+        else if (open_top) fz = S(mz * A(q[c]));
+        fl.fx[s][c] = fx;
+        fl.fy[s][c] = fy;
+        fl.fz[s][c] = fz;
+    }
 }
 
 __device__ __forceinline__ float pd_store(double r)
@@ -673,39 +743,43 @@ __device__ __forceinline__ float pd_store(double r)
     return nextafterf(float(r) * shrink, 0.0f);
 }
 
+// Positive-definite donor limiter (advect_pd_limiter): r = min(1, (rho_begin q / dt) / outgoing).
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_transport_limiter(DevGrid g, ScalarSet<S> set, const S* mfx,
-                                                              const S* mfy, const S* mfz, const S* rbp,
-                                                              const double* base, double dt, int open_top)
+__global__ __launch_bounds__(kBlock) void k_transport_limiter(DevGrid g, ScalarSet<S> set, FluxSet<S> fl,
+                                                              const S* rbp, const double* base, double dt,
+                                                              int open_top)
 {
     int i, j, k;
     if (!cell_index(g, i, j, k)) return;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const std::size_t c = g.at(i, j, k);
+    const std::size_t sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
     const double rho_b = slot(base, kBaseRho0, NZ)[k] + double(rbp[c]);
     for (int s = 0; s < set.n; ++s)
     {
         if (!set.pd[s]) continue;
-        const FaceFluxes<S> F = cell_face_fluxes(g, set.q[s], mfx, mfy, mfz, i, j, k, open_top);
-        const double ix = 1.0 / g.dx, iy = 1.0 / g.dy, iz = 1.0 / g.dz;
+        const S* fx = fl.fx[s];
+        const S* fy = fl.fy[s];
+        const S* fz = fl.fz[s];
         double out_rate = 0.0;
-        if (i <= NX - 2 && F.xp > S(0)) out_rate += double(F.xp) * ix;
-        if (i >= 1 && F.xm < S(0)) out_rate -= double(F.xm) * ix;
-        if (j <= NY - 2 && F.yp > S(0)) out_rate += double(F.yp) * iy;
-        if (j >= 1 && F.ym < S(0)) out_rate -= double(F.ym) * iy;
-        if ((k <= NZ - 2 || (open_top && k == NZ - 1)) && F.zp > S(0)) out_rate += double(F.zp) * iz;
-        if (k >= 1 && F.zm < S(0)) out_rate -= double(F.zm) * iz;
+        if (i <= NX - 2 && fx[c] > S(0)) out_rate += double(fx[c]) * g.idx;
+        if (i >= 1 && fx[c - 1] < S(0)) out_rate -= double(fx[c - 1]) * g.idx;
+        if (j <= NY - 2 && fy[c] > S(0)) out_rate += double(fy[c]) * g.idy;
+        if (j >= 1 && fy[c - sy] < S(0)) out_rate -= double(fy[c - sy]) * g.idy;
+        if ((k <= NZ - 2 || (open_top && k == NZ - 1)) && fz[c] > S(0)) out_rate += double(fz[c]) * g.idz;
+        if (k >= 1 && fz[c - sz] < S(0)) out_rate -= double(fz[c - sz]) * g.idz;
         double r = 1.0;
         if (out_rate > 0.0) r = fmin(1.0, rho_b * fmax(double(set.q[s][c]), 0.0) / dt / out_rate);
         set.rl[s][c] = pd_store(r);
     }
 }
 
+// Conservative update (advect_conservative_update): q = (rho_begin q - dt div F_lim) / rho_end,
+// outgoing-then-incoming for positive-definite scalars.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_transport_update(DevGrid g, ScalarSet<S> set, const S* mfx,
-                                                             const S* mfy, const S* mfz, const S* rbp,
-                                                             const S* rep, const double* base, double dt,
-                                                             int open_top)
+__global__ __launch_bounds__(kBlock) void k_transport_update(DevGrid g, ScalarSet<S> set, FluxSet<S> fl,
+                                                             const S* rbp, const S* rep, const double* base,
+                                                             double dt, int open_top)
 {
     int i, j, k;
     if (!cell_index(g, i, j, k)) return;
@@ -716,7 +790,7 @@ __global__ __launch_bounds__(kBlock) void k_transport_update(DevGrid g, ScalarSe
         for (int s = 0; s < set.n; ++s) set.out[s][c] = set.q[s][c];
         return;
     }
-    const std::ptrdiff_t sy = g.px, sz = std::ptrdiff_t(g.px) * g.ny;
+    const std::size_t sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
     const double r0 = slot(base, kBaseRho0, NZ)[k];
     const double rho_b = r0 + double(rbp[c]);
     const double rho_e = r0 + double(rep[c]);
@@ -724,25 +798,27 @@ __global__ __launch_bounds__(kBlock) void k_transport_update(DevGrid g, ScalarSe
     const bool has_top = k <= NZ - 2 || (open_top && k == NZ - 1);
     for (int s = 0; s < set.n; ++s)
     {
-        const FaceFluxes<S> F = cell_face_fluxes(g, set.q[s], mfx, mfy, mfz, i, j, k, open_top);
+        const S* fx = fl.fx[s];
+        const S* fy = fl.fy[s];
+        const S* fz = fl.fz[s];
         const float* rl = set.rl[s];
         const bool pd = set.pd[s] != 0;
         auto lim = [&](S raw, std::size_t lo, std::size_t hi) -> double {
             if (!pd || raw == S(0)) return double(raw);
             return double(raw) * double(S(raw > S(0) ? rl[lo] : rl[hi]));
         };
-        const double sxp = lim(F.xp, c, c + 1);
-        const double sxm = lim(F.xm, c - 1, c);
-        const double syp = lim(F.yp, c, c + sy);
-        const double sym = lim(F.ym, c - sy, c);
-        const double szp = has_top ? lim(F.zp, c, k < NZ - 1 ? c + sz : c) : 0.0;
-        const double szm = k >= 1 ? lim(F.zm, c - sz, c) : 0.0;
+        const double sxp = lim(fx[c], c, c + 1);
+        const double sxm = lim(fx[c - 1], c - 1, c);
+        const double syp = lim(fy[c], c, c + sy);
+        const double sym = lim(fy[c - sy], c - sy, c);
+        const double szp = has_top ? lim(fz[c], c, k < NZ - 1 ? c + sz : c) : 0.0;
+        const double szm = k >= 1 ? lim(fz[c - sz], c - sz, c) : 0.0;
         const double q0 = double(set.q[s][c]);
         double rho_phi;
         if (pd)
         {
             double outg = 0.0, inc = 0.0;
-            const double ix = 1.0 / g.dx, iy = 1.0 / g.dy, iz = 1.0 / g.dz;
+            const double ix = g.idx, iy = g.idy, iz = g.idz;
             if (sxp >= 0.0) outg += sxp * ix; else inc -= sxp * ix;
             if (sxm <= 0.0) outg -= sxm * ix; else inc += sxm * ix;
             if (syp >= 0.0) outg += syp * iy; else inc -= syp * iy;
@@ -753,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void k_transport_update(DevGrid g, ScalarSe
         }
         else
         {
-            const double div = (sxp - sxm) * (1.0 / g.dx) + (syp - sym) * (1.0 / g.dy) + (szp - szm) * (1.0 / g.dz);
+            const double div = (sxp - sxm) * g.idx + (syp - sym) * g.idy + (szp - szm) * g.idz;
             rho_phi = rho_b * q0 - dt * div;
         }
         set.out[s][c] = S(rho_phi / den);
@@ -1021,8 +1097,8 @@ __global__ __launch_bounds__(kBlock) void k_dev_to_host(DevGrid g, const S* src,
     dst[h] = H(double(src[g.at(i, j, k)]) + (base_slot ? base_slot[k] : 0.0));
 }
 
-dim3 cell_grid(const DevGrid& g) { return dim3(unsigned((std::size_t(g.nx) * g.ny + kBlock - 1) / kBlock), unsigned(g.nz)); }
-dim3 column_grid(const DevGrid& g) { return dim3(unsigned((std::size_t(g.nx) * g.ny + kBlock - 1) / kBlock)); }
+dim3 cell_grid(const DevGrid& g) { return dim3(unsigned(g.tiles_plane * g.nz)); }
+dim3 column_grid(const DevGrid& g) { return dim3(unsigned(g.tiles_plane)); }
 dim3 linear_grid(std::size_t n) { return dim3(unsigned((n + kBlock - 1) / kBlock)); }
 
 template <class S>
@@ -1142,6 +1218,7 @@ void launch_transport(Launcher& L, const DevGrid& g, const DevFields& f, const S
                       void* const* q_in, void* const* q_out, const int* pd)
 {
     ScalarSet<S> set{};
+    FluxSet<S> fl{};
     float* rls[3] = {f.rl_qv, f.rl_qc, f.rl_qr};
     int npd = 0;
     set.n = n_scalars;
@@ -1151,13 +1228,18 @@ void launch_transport(Launcher& L, const DevGrid& g, const DevFields& f, const S
         set.out[s] = P<S>(q_out[s]);
         set.pd[s] = pd[s];
         set.rl[s] = pd[s] ? rls[npd++] : nullptr;
+        fl.fx[s] = P<S>(f.flux[3 * s]);
+        fl.fy[s] = P<S>(f.flux[3 * s + 1]);
+        fl.fz[s] = P<S>(f.flux[3 * s + 2]);
     }
     const int open_top = int(p.bc_top == NIMBUS_TOP_OPEN);
+    NIMBUS_LAUNCH(L, "transport_flux", (k_transport_flux<S>), cell_grid(g), g, set, fl, P<S>(f.mfx), P<S>(f.mfy),
+                  P<S>(f.mfz), open_top);
     if (npd > 0)
-        NIMBUS_LAUNCH(L, "transport_limiter", (k_transport_limiter<S>), cell_grid(g), g, set, P<S>(f.mfx),
-                      P<S>(f.mfy), P<S>(f.mfz), P<S>(f.rb), (const double*)f.base, p.dt, open_top);
-    NIMBUS_LAUNCH(L, "transport_update", (k_transport_update<S>), cell_grid(g), g, set, P<S>(f.mfx), P<S>(f.mfy),
-                  P<S>(f.mfz), P<S>(f.rb), P<S>(f.rp), (const double*)f.base, p.dt, open_top);
+        NIMBUS_LAUNCH(L, "transport_limiter", (k_transport_limiter<S>), cell_grid(g), g, set, fl, P<S>(f.rb),
+                      (const double*)f.base, p.dt, open_top);
+    NIMBUS_LAUNCH(L, "transport_update", (k_transport_update<S>), cell_grid(g), g, set, fl, P<S>(f.rb), P<S>(f.rp),
+                  (const double*)f.base, p.dt, open_top);
 }
 
 template <class S>

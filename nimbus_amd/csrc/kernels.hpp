@@ -40,6 +40,7 @@ struct DevFields
     void *k1u, *k1v, *k1w, *su, *sv, *sw;                // Heun stage-1 tendencies and predictor
     void* ralt;                                          // rho' ping-pong of the acoustic substep
     void *tth, *tqv, *tqc, *tqr;                         // transported scalars before microphysics
+    void* flux[12];                                      // per scalar: x, y, z face fluxes
     float *rl_qv, *rl_qc, *rl_qr;                        // PD limiter factors
     double *tri_c, *tri_r;                               // HE-VI Thomas coefficients (FP64)
     float *rad_w, *rad_e, *rad_s, *rad_n;                // KW78 radiative stashes

@@ -36,6 +36,11 @@ struct DevGrid
 {
     int nx, ny, nz, px;
     double dx, dy, dz;
+    double idx, idy, idz;  // 1/dx, 1/dy, 1/dz
+    // Launch tiling: tile_w x (256 / tile_w) cell tiles per (j, k) plane,
+    // tiles_x * tiles_y per plane; xcd_remap orders tiles so each XCD sweeps a
+    // contiguous slab of planes.
+    int tile_w, tile_shift, tiles_x, tiles_y, tiles_plane, xcd_remap;
     __host__ __device__ std::size_t at(int i, int j, int k) const
     {
         return (std::size_t(k) * std::size_t(ny) + std::size_t(j)) * std::size_t(px) + std::size_t(i);
@@ -52,7 +57,10 @@ enum BaseSlot
     kBaseQv0Stored = 3,  // qv0 rounded to FP32 as the reference stores it (cartesian_common.hpp:31-35)
     kBaseU0 = 4,
     kBaseV0 = 5,
-    kBaseCount = 6
+    kBasePe0 = 6,          // EOS(rho0, theta0): reference point of the perturbation EOS
+    kBasePeDiff = 7,       // EOS(rho0, theta0) - p0
+    kBaseInvRT0 = 8,       // 1 / (rho0 theta0)
+    kBaseCount = 9
 };
 
 // Prognostic bounds (include/core/field/field_bounds.hpp:10-20).

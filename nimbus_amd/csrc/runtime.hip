@@ -68,7 +68,8 @@ enum FieldSlot
     F_K1U, F_K1V, F_K1W, F_SU, F_SV, F_SW,
     F_RALT,
     F_TTH, F_TQV, F_TQC, F_TQR,
-    F_COUNT
+    F_FLUX0, F_FLUX_END = F_FLUX0 + 12,
+    F_COUNT = F_FLUX_END
 };
 
 std::size_t align256(std::size_t x) { return (x + 255) / 256 * 256; }
@@ -92,6 +93,20 @@ Layout layout_for(const DevGrid& g, int elem, bool hevi)
     return L;
 }
 
+// Tile width in {16, 32, 64, 128, 256}; tile height = 256 / width.
+void set_tiling(DevGrid& d, int tile_w, int remap)
+{
+    int shift = 4;
+    while ((1 << shift) < tile_w && shift < 8) ++shift;
+    d.tile_w = 1 << shift;
+    d.tile_shift = shift;
+    const int tile_h = 256 / d.tile_w;
+    d.tiles_x = (d.nx + d.tile_w - 1) / d.tile_w;
+    d.tiles_y = (d.ny + tile_h - 1) / tile_h;
+    d.tiles_plane = d.tiles_x * d.tiles_y;
+    d.xcd_remap = remap ? 1 : 0;
+}
+
 DevGrid dev_grid(const nimbus_grid& g, int elem)
 {
     DevGrid d{};
@@ -102,6 +117,10 @@ DevGrid dev_grid(const nimbus_grid& g, int elem)
     d.dx = g.dx;
     d.dy = g.dy;
     d.dz = g.dz;
+    d.idx = 1.0 / g.dx;
+    d.idy = 1.0 / g.dy;
+    d.idz = 1.0 / g.dz;
+    set_tiling(d, 32, 1);
     return d;
 }
 
@@ -206,15 +225,33 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     const int nz = desc.grid.nz;
     const bool hevi = desc.config.acoustic_mode == NIMBUS_ACOUSTIC_HEVI;
     m.host_base.assign(std::size_t(kBaseCount) * nz, 0.0);
-    const double* src[kBaseCount] = {desc.base.rho0, desc.base.p0, desc.base.theta0, desc.base.qv0, desc.base.u0,
-                                     desc.base.v0};
-    for (int s = 0; s < kBaseCount; ++s)
+    const double* src[6] = {desc.base.rho0, desc.base.p0, desc.base.theta0, desc.base.qv0, desc.base.u0,
+                            desc.base.v0};
+    for (int s = 0; s < 6; ++s)
         for (int k = 0; k < nz; ++k)
         {
             double v = src[s] ? src[s][k] : 0.0;
             if (s == kBaseQv0Stored) v = double(float(v));
             m.host_base[std::size_t(s) * nz + k] = v;
         }
+    // Reference point of the perturbation EOS. Without a theta0 profile the
+    // EOS-consistent theta of (rho0, p0) is used, so EOS(rho0, theta0) = p0.
+    for (int k = 0; k < nz; ++k)
+    {
+        double* hb = m.host_base.data();
+        const double r0 = hb[std::size_t(kBaseRho0) * nz + k], p0 = hb[std::size_t(kBaseP0) * nz + k];
+        double& th0 = hb[std::size_t(kBaseTh0) * nz + k];
+        if (!(th0 > 0.0) && r0 > 0.0 && p0 > 0.0) th0 = p0 / (r0 * kRd * std::pow(p0 / kP00, kKappa));
+        double pe0 = 0.0, inv = 0.0;
+        if (r0 > 0.0 && th0 > 0.0)
+        {
+            pe0 = kP00 * std::pow(r0 * kRd * th0 / kP00, kGamma);
+            inv = 1.0 / (r0 * th0);
+        }
+        hb[std::size_t(kBasePe0) * nz + k] = pe0;
+        hb[std::size_t(kBasePeDiff) * nz + k] = pe0 - p0;
+        hb[std::size_t(kBaseInvRT0) * nz + k] = inv;
+    }
     m.substeps = resolve_substeps(desc.grid, m.host_base.data() + std::size_t(kBaseRho0) * nz,
                                   m.host_base.data() + std::size_t(kBaseP0) * nz, desc.config);
     const Layout L = layout_for(m.g, m.elem, hevi);
@@ -240,6 +277,7 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     f.su = fields[F_SU]; f.sv = fields[F_SV]; f.sw = fields[F_SW];
     f.ralt = fields[F_RALT];
     f.tth = fields[F_TTH]; f.tqv = fields[F_TQV]; f.tqc = fields[F_TQC]; f.tqr = fields[F_TQR];
+    for (int q = 0; q < 12; ++q) f.flux[q] = fields[F_FLUX0 + q];
     f.rl_qv = reinterpret_cast<float*>(p); p += L.rl;
     f.rl_qc = reinterpret_cast<float*>(p); p += L.rl;
     f.rl_qr = reinterpret_cast<float*>(p); p += L.rl;
@@ -638,6 +676,69 @@ int nimbus_hip_synchronize(nimbus_hip_ctx* ctx)
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, ctx->step_a, ctx->step_b) == hipSuccess) ctx->last_step_ms = ms;
     harvest(ctx);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int xcd_remap)
+{
+    if (!ctx) return fail(NIMBUS_EINVAL, "set_tiling: null context");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "set_tiling: unknown member");
+    if (tile_w != 16 && tile_w != 32 && tile_w != 64 && tile_w != 128 && tile_w != 256)
+        return fail(NIMBUS_EINVAL, "set_tiling: tile width must be 16, 32, 64, 128 or 256");
+    hipStreamSynchronize(ctx->stream);
+    set_tiling(m->g, tile_w, xcd_remap);
+    release_graph(*m);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int* best_tile_w, int* best_remap,
+                               double* best_ms)
+{
+    if (!ctx || trials < 1) return fail(NIMBUS_EINVAL, "autotune: bad arguments");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "autotune: unknown member");
+    hipSetDevice(ctx->device);
+    void* saved = nullptr;
+    NIMBUS_HIP_TRY(hipMalloc(&saved, m->bytes));
+    Staging keep;
+    keep.p = saved;
+    NIMBUS_HIP_TRY(hipMemcpyAsync(saved, m->block, m->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    const std::int64_t steps0 = m->steps;
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    double best = 1e300;
+    int bw = m->g.tile_w, br = m->g.xcd_remap;
+    const std::uint32_t ids[1] = {id};
+    for (int w : {32, 64, 128, 256, 16})
+        for (int r : {0, 1})
+        {
+            set_tiling(m->g, w, r);
+            release_graph(*m);
+            int rc = step_members(ctx, ids, 1, 1, true);  // warm-up (and graph capture)
+            if (rc == NIMBUS_OK) rc = step_members(ctx, ids, 1, trials, true);
+            if (rc != NIMBUS_OK)
+            {
+                ctx->timing = timing;
+                return rc;
+            }
+            const double ms = ctx->last_step_ms / trials;
+            if (ms < best)
+            {
+                best = ms;
+                bw = w;
+                br = r;
+            }
+            NIMBUS_HIP_TRY(hipMemcpyAsync(m->block, saved, m->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    ctx->timing = timing;
+    NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    m->steps = steps0;
+    set_tiling(m->g, bw, br);
+    release_graph(*m);
+    if (best_tile_w) *best_tile_w = bw;
+    if (best_remap) *best_remap = br;
+    if (best_ms) *best_ms = best;
     return NIMBUS_OK;
 }
 

@@ -168,6 +168,9 @@ def load_library(path=LIB_PATH):
         "nimbus_hip_step_async": (ctypes.c_int, [vp, P(ctypes.c_uint32), ctypes.c_int, ctypes.c_int]),
         "nimbus_hip_synchronize": (ctypes.c_int, [vp]),
         "nimbus_hip_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
+        "nimbus_hip_member_set_tiling": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]),
+        "nimbus_hip_member_autotune": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_int, P(ctypes.c_int),
+                                                      P(ctypes.c_int), P(ctypes.c_double)]),
         "nimbus_hip_set_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "nimbus_hip_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int, vp, P(ctypes.c_uint64),
                                                     P(ctypes.c_double)]),
@@ -388,6 +391,16 @@ class Member:
         d = Diag()
         check(self.ctx.lib.nimbus_hip_member_diag(self.ctx.handle, self.id, ctypes.byref(d)), "diag")
         return {n: getattr(d, n) for n, _ in Diag._fields_}
+
+    def set_tiling(self, tile_w, xcd_remap):
+        check(self.ctx.lib.nimbus_hip_member_set_tiling(self.ctx.handle, self.id, int(tile_w), int(xcd_remap)),
+              "set_tiling")
+
+    def autotune(self, trials=2):
+        w, r, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(self.ctx.lib.nimbus_hip_member_autotune(self.ctx.handle, self.id, int(trials), ctypes.byref(w),
+                                                      ctypes.byref(r), ctypes.byref(ms)), "autotune")
+        return w.value, r.value, ms.value
 
     def destroy(self):
         if self.id:

@@ -40,7 +40,9 @@ def test_acoustic_substep_fused_matches_oracle(hip_ctx):
         L.dptr(base["rho0"]), L.dptr(base["p0"]), g.nz, g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, dts,
         1004.0 / 717.0, 0.05, 2.0, 1000.0, 200000.0, 100000.0, 180.0, 120.0)
     L.check(hip_ctx.lib.nimbus_hip_dispatch_acoustic_substep_fused(hip_ctx.handle, ctypes.byref(d)), "acoustic")
-    for f, (rel, ab) in {"u": (1e-5, 1e-5), "v": (1e-5, 1e-5), "w": (1e-5, 1e-5), "rho": (1e-6, 1e-7),
+    # Stencil tier for the winds; rho and p at FP32 rounding of the full values
+    # (the mixed path stores perturbations, the reference whole FP32 values).
+    for f, (rel, ab) in {"u": (1e-4, 5e-5), "v": (1e-4, 5e-5), "w": (1e-4, 5e-5), "rho": (1e-6, 1e-7),
                          "p": (1e-6, 0.05)}.items():
         nbad, err = within_tier(a[f], ref[f], rel, ab)
         print(f"acoustic {f}: max abs diff {err:.3e}")
