@@ -255,9 +255,9 @@ inline float sanitize(float v, float fallback, float lo, float hi)
 inline double weno5_face(double a, double b, double c, double d, double e)
 {
     // candidate polynomials at the downwind face of cell c (flow c -> d)
-    const double q0 = a / 3.0 - 7.0 * b / 6.0 + 11.0 * c / 6.0;
-    const double q1 = -b / 6.0 + 5.0 * c / 6.0 + d / 3.0;
-    const double q2 = c / 3.0 + 5.0 * d / 6.0 - e / 6.0;
+    const double q0 = (1.0 / 3.0) * a - (7.0 / 6.0) * b + (11.0 / 6.0) * c;
+    const double q1 = -(1.0 / 6.0) * b + (5.0 / 6.0) * c + (1.0 / 3.0) * d;
+    const double q2 = (1.0 / 3.0) * c + (5.0 / 6.0) * d - (1.0 / 6.0) * e;
     // Jiang-Shu smoothness indicators
     const double t0a = a - 2.0 * b + c, t0b = a - 4.0 * b + 3.0 * c;
     const double t1a = b - 2.0 * c + d, t1b = b - d;

@@ -105,6 +105,41 @@ std::vector<S> vec(const void* p, std::size_t n)
     const S* q = static_cast<const S*>(p);
     return std::vector<S>(q, q + n);
 }
+template <class S>
+int substeps_batched(const oc_grid* gp, const oc_base* bp, const oc_config* cp, const oc_state* st, int n,
+                     double dts, double weight, S* mfx, S* mfy, S* mfz)
+{
+    const Grid g = to_grid(gp);
+    const Base b = to_base(bp, g.nz);
+    const Config c = to_config(cp);
+    State<S> s;
+    load(g, st, s);
+    std::vector<S> fx(mfx, mfx + g.cells()), fy(mfy, mfy + g.cells()), fz(mfz, mfz + g.cells());
+    seed_radiation(g, s);
+    std::vector<double> wz;
+    for (int q = 0; q < n; ++q)
+    {
+        if (c.acoustic_mode == kAcousticHEVI)
+        {
+            std::vector<S> zero(g.cells(), S(0));
+            accumulate_mass_flux(g, c, s, weight, fx, fy, zero);
+            acoustic_substep_hevi(g, b, s, dts, c.hevi_beta, wz);
+            for (std::size_t k = 0; k < g.cells(); ++k) fz[k] = S(double(fz[k]) + weight * wz[k]);
+        }
+        else
+        {
+            accumulate_mass_flux(g, c, s, weight, fx, fy, fz);
+            acoustic_substep_explicit(g, b, s, dts);
+        }
+        apply_boundaries(g, c, s, dts);
+    }
+    store(g, s, st);
+    std::memcpy(mfx, fx.data(), sizeof(S) * g.cells());
+    std::memcpy(mfy, fy.data(), sizeof(S) * g.cells());
+    std::memcpy(mfz, fz.data(), sizeof(S) * g.cells());
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -183,35 +218,13 @@ int oracle_hevi_substep_f32(const oc_grid* gp, const oc_base* bp, const oc_state
 int oracle_acoustic_substeps_batched_f32(const oc_grid* gp, const oc_base* bp, const oc_config* cp, const oc_state* st,
                                          int n, double dts, double weight, float* mfx, float* mfy, float* mfz)
 {
-    const Grid g = to_grid(gp);
-    const Base b = to_base(bp, g.nz);
-    const Config c = to_config(cp);
-    State<float> s;
-    load(g, st, s);
-    std::vector<float> fx(mfx, mfx + g.cells()), fy(mfy, mfy + g.cells()), fz(mfz, mfz + g.cells());
-    seed_radiation(g, s);
-    std::vector<double> wz;
-    for (int q = 0; q < n; ++q)
-    {
-        if (c.acoustic_mode == kAcousticHEVI)
-        {
-            std::vector<float> zero(g.cells(), 0.0f);
-            accumulate_mass_flux(g, c, s, weight, fx, fy, zero);
-            acoustic_substep_hevi(g, b, s, dts, c.hevi_beta, wz);
-            for (std::size_t k = 0; k < g.cells(); ++k) fz[k] = float(double(fz[k]) + weight * wz[k]);
-        }
-        else
-        {
-            accumulate_mass_flux(g, c, s, weight, fx, fy, fz);
-            acoustic_substep_explicit(g, b, s, dts);
-        }
-        apply_boundaries(g, c, s, dts);
-    }
-    store(g, s, st);
-    std::memcpy(mfx, fx.data(), sizeof(float) * g.cells());
-    std::memcpy(mfy, fy.data(), sizeof(float) * g.cells());
-    std::memcpy(mfz, fz.data(), sizeof(float) * g.cells());
-    return 0;
+    return substeps_batched<float>(gp, bp, cp, st, n, dts, weight, mfx, mfy, mfz);
+}
+
+int oracle_acoustic_substeps_batched_f64(const oc_grid* gp, const oc_base* bp, const oc_config* cp, const oc_state* st,
+                                         int n, double dts, double weight, double* mfx, double* mfy, double* mfz)
+{
+    return substeps_batched<double>(gp, bp, cp, st, n, dts, weight, mfx, mfy, mfz);
 }
 
 int oracle_apply_boundaries_f32(const oc_grid* gp, const oc_config* cp, const oc_state* st, double dt_rad)
@@ -338,6 +351,16 @@ double oracle_upwind_fd3(double m2, double m1, double c, double p1, double p2, d
 {
     return upwind_fd3(m2, m1, c, p1, p2, a, inv);
 }
+// UP5/UP3/UP1 derivative of a float field (reference layout) at (i, j, k) along axis 0/1/2.
+double oracle_upwind_point(const float* f, int nx, int ny, int nz, int i, int j, int k, int axis, double a,
+                           double inv)
+{
+    const Grid g{nx, ny, nz, 1.0, 1.0, 1.0};
+    if (axis == 0) return upwind_axis([&](int o) { return double(f[g.at(i + o, j, k)]); }, i, nx, a, inv);
+    if (axis == 1) return upwind_axis([&](int o) { return double(f[g.at(i, j + o, k)]); }, j, ny, a, inv);
+    return upwind_axis([&](int o) { return double(f[g.at(i, j, k + o)]); }, k, nz, a, inv);
+}
+
 double oracle_qvs_water(double T, double p) { return qvs_water(T, p); }
 double oracle_theta_to_temperature(double th, double p) { return theta_to_temperature(th, p); }
 double oracle_eos_pressure(double rho, double theta) { return eos_pressure(rho, theta); }

@@ -58,6 +58,9 @@ def lib():
         L.oracle_acoustic_substeps_batched_f32.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState),
                                                            ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                            _fp, _fp, _fp]
+        L.oracle_acoustic_substeps_batched_f64.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState),
+                                                           ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                           _dp, _dp, _dp]
         L.oracle_apply_boundaries_f32.argtypes = [P(OcGrid), P(OcConfig), P(OcState), ctypes.c_double]
         L.oracle_slow_tendencies_f32.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), _fp, _fp, _fp]
         L.oracle_slow_rk2_f32.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState)]
@@ -75,6 +78,9 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = [ctypes.c_double] * n
             fn.restype = ctypes.c_double
+        L.oracle_upwind_point.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.oracle_upwind_point.restype = ctypes.c_double
         L.oracle_saturation_adjust.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.oracle_saturation_adjust.restype = ctypes.c_double
         L.oracle_thomas.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
@@ -145,15 +151,18 @@ def hevi_substep(g, base, st, dts, beta, dtype=np.float64):
     return s, wz
 
 
-def acoustic_substeps_batched(g, base, cfg, st, n, dts, weight, mf=None):
-    """Reference batched substep contract in reference storage (FP32)."""
-    s = copy_state(st, np.float32)
+def acoustic_substeps_batched(g, base, cfg, st, n, dts, weight, mf=None, dtype=np.float32):
+    """Reference batched substep contract: n substeps with per-substep boundaries."""
+    s = copy_state(st, dtype)
     b = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in base.items()}
     shape = st["u"].shape
-    mf = [np.zeros(shape, np.float32) for _ in range(3)] if mf is None else [np.array(x, np.float32) for x in mf]
-    lib().oracle_acoustic_substeps_batched_f32(ctypes.byref(g), ctypes.byref(base_struct(b)), ctypes.byref(cfg),
-                                               ctypes.byref(_state_struct(s)), int(n), float(dts), float(weight),
-                                               *[x.ctypes.data_as(_fp) for x in mf])
+    mf = [np.zeros(shape, dtype) for _ in range(3)] if mf is None else [np.array(x, dtype) for x in mf]
+    if np.dtype(dtype) == np.float32:
+        fn, ptr = lib().oracle_acoustic_substeps_batched_f32, _fp
+    else:
+        fn, ptr = lib().oracle_acoustic_substeps_batched_f64, _dp
+    fn(ctypes.byref(g), ctypes.byref(base_struct(b)), ctypes.byref(cfg), ctypes.byref(_state_struct(s)), int(n),
+       float(dts), float(weight), *[x.ctypes.data_as(ptr) for x in mf])
     return s, mf
 
 
