@@ -254,6 +254,9 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     }
     m.substeps = resolve_substeps(desc.grid, m.host_base.data() + std::size_t(kBaseRho0) * nz,
                                   m.host_base.data() + std::size_t(kBaseP0) * nz, desc.config);
+    if (m.substeps < 1)
+        return fail(NIMBUS_EINVAL, "acoustic substep count unresolvable (non-positive base density or "
+                                   "more than 1e6 substeps required)");
     const Layout L = layout_for(m.g, m.elem, hevi);
     m.bytes = L.total;
     hipError_t e = hipMalloc(&m.block, L.total);
