@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the Klemp-Wilhelmson large step on one MI355X.
 
-Default workload: 400x400x100 cells, dx = dy = 250 m, dz = 200 m, dt = 1.5 s,
-8 acoustic substeps, three +2 K bubbles 15 km apart, FP32-storage mixed
-precision with hipGraph-captured steps; the plain all-FP64 path is timed on the
-same grid for the FP32-mixed / FP64 ratio. A "step" is one large step of the
-whole domain with the state resident in HBM. Prints one JSON line.
+Headline workload: 400x400x100 cells, dx = dy = 250 m, dz = 200 m, dt = 1.5 s,
+8 acoustic substeps, three +2 K bubbles 15 km apart. `value` is the
+cell-updates/s of the mixed-precision path (FP32 bulk in perturbation form,
+FP64 base state and FP64 vertically implicit w-p' Thomas solve), hipGraph
+captured, state resident in HBM; `fp64_value` is the plain all-FP64 path
+(explicit acoustic substeps, the reference method) on the same grid. Also
+reported: the other precision/method combinations, the 32-member uneven
+ensemble run as one batch, and a short continuous-batching serving run with
+p50/p99 completion latency. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2] [--mode explicit|hevi]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+                  [--serving-seconds S] [--no-ensemble] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -26,18 +31,18 @@ METRIC = "cell-updates/s per large step (FP32-mixed vs FP64) + p50/p99 ensemble 
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per cell of each kernel, in units of the storage size s
-# (4 B mixed, 8 B FP64): every field a kernel must read or write, once.
-# The phase totals follow the large-step budget B(N) = (68 + 17 N) s:
-# acoustic substep 17 s = forward 13 s (u, v, w, rho, theta, 3 fluxes in;
-# rho, p, 3 fluxes out) + backward 4 s of the remaining momentum writes
-# (u, v, w written, p read; the inputs shared with the forward pass are not
-# counted twice); scalar transport 13 s (9 in + 4 out) split 5 s limiter /
-# 8 s update; Kessler 11 s; slow stage 1 + predictor 16 s; stage 2 +
-# average + apply 28 s.
+# (4 B mixed, 8 B FP64): every field a phase must read or write, once, with
+# the phase totals of the large-step budget B(N) = (68 + 17 N) s:
+# acoustic substep 17 s (u, v, w, rho, p, theta in; u, v, w, rho, p out;
+# three retained mass fluxes read and written), split for HE-VI into the
+# column solve (13 s) and the u, v momentum update (4 s); scalar transport
+# 13 s (4 scalars, 3 mass fluxes, 2 densities in; 4 scalars out) split over
+# its flux (7 s), limiter (1 s) and update (5 s) passes; Kessler with
+# sedimentation, apply and EOS 11 s; slow stage 1 + predictor 16 s; stage 2 +
+# average + apply 28 s. Face-flux and Thomas scratch are not algorithmic.
 ALGO_BYTES_PER_CELL = {
-    "acoustic_forward": 13, "acoustic_hevi_column": 13, "acoustic_backward": 4,
-    "acoustic_substep": 17,
-    "transport_limiter": 5, "transport_update": 8, "transport": 13,
+    "acoustic_substep": 17, "acoustic_hevi_column": 13, "acoustic_hevi_momentum": 4,
+    "transport_flux": 7, "transport_limiter": 1, "transport_update": 5,
     "kessler_apply": 11,
     "slow_stage1": 16, "slow_stage2": 28,
 }
@@ -82,16 +87,17 @@ def kernel_table(ctx, m, steps):
     return tab
 
 
-def cpu_baseline(wl, mode, sample_steps=2):
+def cpu_baseline(wl, sample_steps=2):
+    """The oracle in the reference's CPU arithmetic (FP32 storage, FP64
+    arithmetic, explicit acoustic substeps) on the host cores."""
     from oracle import oracle_py as O
-    from nimbus_amd import lib as L
 
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     threads = max(1, min(threads, 16))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     g = wl.grid
     og = O.grid(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz)
-    cfg = O.from_step_config(wl.config(acoustic_mode=mode))
+    cfg = O.from_step_config(wl.config())
     base = wl.base()
     st = wl.initial_state(np.float32)
     st, _ = O.large_steps(og, base, cfg, st, 1, np.float32)  # warm-up step
@@ -100,8 +106,72 @@ def cpu_baseline(wl, mode, sample_steps=2):
     el = time.perf_counter() - t0
     return {"value": wl.cells * sample_steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{sample_steps} large steps of the full {wl.describe()} grid after 1 warm-up step, "
-                      f"reference storage mode (FP32 storage, FP64 arithmetic), {n_sub} substeps",
-            "seconds": el}
+                      f"reference arithmetic (FP32 storage, FP64 arithmetic, explicit substeps), {n_sub} substeps",
+            "seconds": round(el, 3)}
+
+
+def measured_traffic(workload, mode, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary for this workload and acoustic mode, scaled by the copy
+    calibration recorded with it; None when no matching summary exists."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{mode}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return int(k["fetch_kib"] * 1024 * d["fetch_scale"] + k["write_kib"] * 1024 * d["write_scale"])
+
+
+def roofline_for(tab, cells, elem, workload, mode):
+    """Dominant kernel of a step: algorithmic bytes per launch / measured launch time."""
+    name, (launches, ms) = max(tab.items(), key=lambda kv: kv[1][1])
+    per_cell = ALGO_BYTES_PER_CELL.get(name)
+    if per_cell is None:
+        return None
+    avg_ms = ms / launches
+    algo = per_cell * elem * cells
+    achieved = algo / (avg_ms / 1e3) / 1e9
+    traffic = measured_traffic(workload, mode, name)
+    total_ms = sum(v[1] for v in tab.values())
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
+            "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": algo,
+            "share_of_step": round(ms / total_ms, 3)}
+
+
+def ensemble_run(ctx, L, W, precision, mode):
+    """The 32-member uneven ensemble as one resident batch, 30 simulated minutes
+    per member; members leave the batch as they finish."""
+    from nimbus_amd import serving as SV
+    reqs = [SV.Request(q, 0.0, wl) for q, wl in enumerate(W.ensemble_c3())]
+    srv = SV.Server(ctx, budget_bytes=250e9, precision=precision, acoustic_mode=mode, quantum=50)
+    rep = srv.run(reqs)
+    s = rep.summary()
+    lat = sorted(r.latency for r in rep.results)
+    return {"members": len(reqs), "cells_total": sum(r.workload.cells for r in reqs),
+            "cell_updates_per_s": s["cell_updates_per_s"], "wall_seconds": s["wall_seconds"],
+            "device_step_seconds": s["device_step_seconds"], "p50_completion_s": s["p50_latency_s"],
+            "p99_completion_s": s["p99_latency_s"], "max_completion_s": round(lat[-1], 3),
+            "rounds": s["rounds"], "acoustic_mode": "hevi" if mode == L.ACOUSTIC_HEVI else "explicit",
+            "data": "seed 3: nx = ny in {48..160}, nz in {40, 60, 80}, dx in {500, 1000} m, dt = 6 dx/1000 s, "
+                    "6 substeps, perturbed soundings, 1800 simulated s each"}
+
+
+def serving_run(ctx, L, seconds, precision, mode):
+    """Continuous batching of seeded Poisson requests (2/s) arriving over `seconds`."""
+    from nimbus_amd import serving as SV
+    reqs = SV.poisson_requests(seed=4, rate=2.0, horizon=float(seconds))
+    srv = SV.Server(ctx, budget_bytes=200e9, precision=precision, acoustic_mode=mode, quantum=16)
+    rep = srv.run(reqs)
+    s = rep.summary()
+    s["arrival_seconds"] = seconds
+    s["data"] = ("seed 4 Poisson arrivals at 2/s; nx = ny in {48..256}, nz in {40, 60, 80}, dx 500-1000 m, "
+                 "15-90 simulated minutes each; 200 GB HBM budget")
+    return s
 
 
 def main():
@@ -110,11 +180,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--mode", default="explicit", choices=["explicit", "hevi"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
-    ap.add_argument("--profile", action="store_true",
-                    help="only the mixed member: warm-up + timed steps, for rocprofv3 kernel/counter passes")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=20.0,
+                    help="arrival window of the serving run (600 = the full 10-minute stream; 0 skips it)")
+    ap.add_argument("--profile", choices=["hevi", "explicit"], default=None,
+                    help="only the mixed member in this mode: warm-up + timed steps, for rocprofv3 passes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -129,7 +201,7 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the large step of one domain per device and "
+        print(f"bench.py measures one GPU: this build runs the large step on one device and "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
@@ -137,67 +209,75 @@ def main():
     from nimbus_amd import workloads as W
 
     wl = W.NAMED[args.workload]
-    mode = L.ACOUSTIC_HEVI if args.mode == "hevi" else L.ACOUSTIC_EXPLICIT
     ctx = L.Context(0)
     graphs = not args.no_graphs
+    modes = {"hevi": L.ACOUSTIC_HEVI, "explicit": L.ACOUSTIC_EXPLICIT}
 
-    mixed, mixed_ms, mixed_host = run_member(ctx, L, wl, L.PRECISION_MIXED, mode, args.steps, args.warmup, graphs)
-    n_sub = mixed.substeps
     if args.profile:
-        print(json.dumps({"profile": True, "workload": wl.describe(), "steps": args.steps,
-                          "ms_per_step": mixed_ms / args.steps}))
-        mixed.destroy()
+        m, ms, _ = run_member(ctx, L, wl, L.PRECISION_MIXED, modes[args.profile], args.steps, args.warmup, graphs)
+        print(json.dumps({"profile": True, "workload": wl.describe(), "mode": args.profile, "steps": args.steps,
+                          "ms_per_step": ms / args.steps}))
+        m.destroy()
         ctx.close()
         return
-    tab = kernel_table(ctx, mixed, 2)
-    diag = mixed.diag()
-    mixed.destroy()
-    fp64_steps = max(1, min(args.steps, 10))
-    fp64, fp64_ms, _ = run_member(ctx, L, wl, L.PRECISION_FP64, mode, fp64_steps, 1, graphs)
-    tab64 = kernel_table(ctx, fp64, 1)
-    fp64.destroy()
 
-    ms_per_step = mixed_ms / args.steps
-    value = wl.cells * args.steps / (mixed_ms / 1e3)
-    fp64_value = wl.cells * fp64_steps / (fp64_ms / 1e3)
+    runs = {}
+    tables = {}
+    for label, prec, mode in (("mixed_hevi", L.PRECISION_MIXED, "hevi"),
+                              ("mixed_explicit", L.PRECISION_MIXED, "explicit"),
+                              ("fp64_explicit", L.PRECISION_FP64, "explicit"),
+                              ("fp64_hevi", L.PRECISION_FP64, "hevi")):
+        steps = args.steps if prec == L.PRECISION_MIXED else max(1, min(args.steps, 10))
+        m, ms, _ = run_member(ctx, L, wl, prec, modes[mode], steps, args.warmup if prec == L.PRECISION_MIXED else 1,
+                              graphs)
+        runs[label] = {"ms_per_step": ms / steps, "value": wl.cells * steps / (ms / 1e3), "substeps": m.substeps}
+        tables[label] = kernel_table(ctx, m, 2)
+        if label == "mixed_hevi":
+            diag = m.diag()
+        m.destroy()
 
-    # dominant kernel of the mixed step, with its algorithmic bytes per launch
-    total_ms = sum(v[1] for v in tab.values())
-    name, (launches, ms) = max(tab.items(), key=lambda kv: kv[1][1])
-    avg_ms = ms / launches
-    per_cell = ALGO_BYTES_PER_CELL.get(name)
-    roofline = None
-    if per_cell is not None:
-        algo = per_cell * 4 * wl.cells
-        achieved = algo / (avg_ms / 1e3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
-                    "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": algo,
-                    "share_of_step": round(ms / total_ms, 3)}
+    head = runs["mixed_hevi"]
+    value = head["value"]
+    fp64_value = runs["fp64_explicit"]["value"]
+    n_sub = head["substeps"]
+    ms_per_step = head["ms_per_step"]
     step_bytes = (68 + 17 * n_sub) * 4 * wl.cells
+
+    def per_step(tab):
+        return {k: round(v[1] / 2, 4) for k, v in sorted(tab.items(), key=lambda kv: -kv[1][1])}
+
     result = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 storage, f64 acoustic/EOS/base-state",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 storage and bulk arithmetic; f64 base state, continuity/EOS, vertical Thomas solve",
         "data": "synthetic (seeded analytic sounding, warm bubbles, uniform theta noise)",
         "config": {"workload": f"{args.workload}: {wl.describe()}", "cells": wl.cells,
-                   "acoustic_mode": args.mode, "substeps": n_sub, "graphs": graphs,
-                   "parallelism": "single GPU, one domain"},
-        "fp64_value": fp64_value, "fp64_ms_per_step": round(fp64_ms / fp64_steps, 4),
+                   "acoustic_mode": "hevi (vertically implicit, FP64 Thomas)", "substeps": n_sub,
+                   "graphs": graphs, "parallelism": "single GPU, one domain"},
+        "fp64_value": fp64_value, "fp64_ms_per_step": round(runs["fp64_explicit"]["ms_per_step"], 4),
         "mixed_over_fp64": round(value / fp64_value, 3),
+        "modes": {k: {"cell_updates_per_s": v["value"], "ms_per_step": round(v["ms_per_step"], 4),
+                      "substeps": v["substeps"]} for k, v in runs.items()},
+        "mixed_over_fp64_same_method": {
+            "hevi": round(runs["mixed_hevi"]["value"] / runs["fp64_hevi"]["value"], 3),
+            "explicit": round(runs["mixed_explicit"]["value"] / runs["fp64_explicit"]["value"], 3)},
         "step_algorithmic_GBps": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
-        "host_timed_ms_per_step": round(mixed_host * 1e3 / args.steps, 4),
-        "kernels_mixed_ms_per_step": {k: round(v[1] / 2, 4) for k, v in sorted(tab.items(), key=lambda kv: -kv[1][1])},
-        "kernels_fp64_ms_per_step": {k: round(v[1], 4) for k, v in sorted(tab64.items(), key=lambda kv: -kv[1][1])},
+        "kernels_ms_per_step": {k: per_step(t) for k, t in tables.items()},
         "diag_after_steps": {k: diag[k] for k in ("max_w", "rms_theta_p", "rain_total_mm")},
-        "roofline": roofline,
+        "roofline": roofline_for(tables["mixed_hevi"], wl.cells, 4, args.workload, "hevi"),
+        "roofline_mixed_explicit": roofline_for(tables["mixed_explicit"], wl.cells, 4, args.workload, "explicit"),
         "cpu_baseline": None,
         "host": {"platform": platform.platform(), **cpu_info()},
     }
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(wl, mode)
-        result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+    if not args.no_ensemble:
+        result["ensemble_c3"] = ensemble_run(ctx, L, W, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
+    if args.serving_seconds > 0:
+        result["serving_c4"] = serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
     ctx.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(wl)
+        result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result))
 
 

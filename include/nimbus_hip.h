@@ -175,19 +175,24 @@ uint64_t nimbus_hip_member_bytes(const nimbus_member_desc* desc);
 int nimbus_hip_member_upload(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s);
 int nimbus_hip_member_download(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s);
 int nimbus_hip_member_diag(nimbus_hip_ctx* ctx, uint32_t id, nimbus_diag* d);
-/* Advance each listed member by n_steps large steps; returns after the device finished. */
+/* Advance each listed member by n_steps large steps; returns after the device finished.
+ * The listed members of one precision run as one batch: every kernel launch of
+ * the step covers all of them (uneven grids included), and the batch's whole
+ * large step is one captured hipGraph when graphs are enabled. Members with
+ * different substep or microphysics subcycle counts share the batch. */
 int nimbus_hip_step(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_steps);
 /* Same, enqueued on the context stream without waiting. */
 int nimbus_hip_step_async(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_steps);
 int nimbus_hip_synchronize(nimbus_hip_ctx* ctx);
-/* Launch tiling of a member's cell kernels: tile_w in {16, 32, 64, 128, 256} cells along i
- * (tile height 256 / tile_w along j) and XCD-contiguous tile ordering on/off. */
-int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int xcd_remap);
-/* Autotuner: times `trials` large steps per tiling candidate on a saved copy of the
- * member state (restored afterwards) and keeps the fastest; reports it. */
-int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int* best_tile_w, int* best_remap,
-                               double* best_ms);
-/* Enable hipGraph capture of whole large steps (per member and substep count). */
+/* Launch geometry of a member: column tiles of tile_w x (256 / tile_w) columns
+ * (tile_w 32 or 64), column marches split into k_chunks vertical chunks
+ * (1 .. nz), and XCD-contiguous workgroup ordering on/off. */
+int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int k_chunks, int xcd_remap);
+/* Autotuner: times `trials` large steps per launch-geometry candidate on a saved
+ * copy of the member state (restored afterwards), keeps the fastest and reports it. */
+int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int* best_tile_w, int* best_k_chunks,
+                               int* best_remap, double* best_ms);
+/* Enable hipGraph capture of whole large steps (one graph per batch of members). */
 int nimbus_hip_set_graphs(nimbus_hip_ctx* ctx, int enable);
 /* Per-kernel timing with HIP events on the launch stream (0 disables). */
 int nimbus_hip_set_kernel_timing(nimbus_hip_ctx* ctx, int enable);

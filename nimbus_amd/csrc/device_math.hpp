@@ -231,48 +231,36 @@ __device__ __forceinline__ A mc_slope(A qm, A q0, A qp)
     return minmod3(A(0.5) * (r + l), A(2) * r, A(2) * l);
 }
 
-// Retained-flux scalar face flux through the face between lower cell `lo`
-// and upper cell `lo+1` along one axis. `at(o)` reads the scalar at offset o
-// from the lower cell, `pos` is the lower cell's index and `n` the extent.
-// The reconstruction is WENO5 where the 5-point upwind stencil fits and
-// MC-limited linear otherwise (advection_cartesian.cpp:1941-2173). FP32
+// Retained-flux scalar face flux through the face between lower cell `pos`
+// and upper cell `pos + 1` along one axis of extent n, from the six values
+// v(-2) .. v(3) at offsets -2 .. 3 from the lower cell (values outside the
+// axis may be any finite number: they are never selected). The
+// reconstruction is WENO5 where the 5-point upwind stencil fits and
+// MC-limited linear otherwise (advection_cartesian.cpp:1941-2173). The
+// upwind stencil is selected without branching: the downwind case is the
+// upwind case on the mirrored stencil, and the MC slope of a mirrored stencil
+// is exactly the negated slope, so both cases share one evaluation. FP32
 // forms the flux as fma(m, base, m * offset) like the reference shader.
-template <class A, class F>
-__device__ __forceinline__ A face_flux(F at, int pos, int n, A m)
+template <class A>
+__device__ __forceinline__ A face_flux6(A vm2, A vm1, A v0, A v1, A v2, A v3, int pos, int n, A m)
 {
+    const bool up = m >= A(0);
+    const A a = up ? vm2 : v3, b = up ? vm1 : v2, c = up ? v0 : v1, d = up ? v1 : v0, e = up ? v2 : vm1;
+    const bool fits = up ? (pos >= 2 && pos + 2 <= n - 1) : (pos >= 1 && pos + 3 <= n - 1);
+    const bool has_mc = up ? pos >= 1 : pos + 1 <= n - 2;
     if constexpr (sizeof(A) == sizeof(double))
     {
         double phi;
-        if (m >= 0.0)
-        {
-            const double q0 = at(0);
-            if (pos >= 2 && pos + 2 <= n - 1) phi = weno5_face(at(-2), at(-1), q0, at(1), at(2));
-            else phi = q0 + 0.5 * (pos >= 1 ? mc_slope<double>(at(-1), q0, at(1)) : 0.0);
-        }
-        else
-        {
-            const double q0 = at(1);
-            if (pos - 1 >= 0 && pos + 3 <= n - 1) phi = weno5_face(at(3), at(2), q0, at(0), at(-1));
-            else phi = q0 - 0.5 * (pos + 1 <= n - 2 ? mc_slope<double>(at(0), q0, at(2)) : 0.0);
-        }
+        if (fits) phi = weno5_face(a, b, c, d, e);
+        else phi = c + 0.5 * (has_mc ? mc_slope<double>(b, c, d) : 0.0);
         return m * phi;
     }
     else
     {
-        float base, off;
-        if (m >= 0.0f)
-        {
-            base = at(0);
-            if (pos >= 2 && pos + 2 <= n - 1) off = weno5_offset(at(-2), at(-1), base, at(1), at(2));
-            else off = 0.5f * (pos >= 1 ? mc_slope<float>(at(-1), base, at(1)) : 0.0f);
-        }
-        else
-        {
-            base = at(1);
-            if (pos - 1 >= 0 && pos + 3 <= n - 1) off = weno5_offset(at(3), at(2), base, at(0), at(-1));
-            else off = -0.5f * (pos + 1 <= n - 2 ? mc_slope<float>(at(0), base, at(2)) : 0.0f);
-        }
-        return fmaf(m, base, m * off);
+        float off;
+        if (fits) off = weno5_offset(a, b, c, d, e);
+        else off = 0.5f * (has_mc ? mc_slope<float>(b, c, d) : 0.0f);
+        return fmaf(m, c, m * off);
     }
 }
 

@@ -1,13 +1,20 @@
 // CDNA4 (gfx950) kernels of the Klemp-Wilhelmson split-explicit large step.
 //
 // Device layout: every 3-D field is [k][j][i] with i contiguous and a
-// 64-byte-padded row pitch, so the lanes of a wavefront read consecutive i of
-// two (j, k) rows (coalesced) and vertical neighbours are one plane apart.
-// Cell kernels give each 256-thread block a 32 x 8 (i, j) tile of one k
-// plane, ordered so that each XCD sweeps a contiguous slab of planes; column
-// kernels give each thread one (i, j) column and march k. rho and p are
-// stored as perturbations from the FP64 base state; theta, winds and water
-// species are stored whole.
+// 64-byte-padded row pitch, so the 64 lanes of a wavefront read consecutive
+// i of one (j, k) row and vertical neighbours are one plane apart. rho and p
+// are stored as perturbations from the FP64 base state; theta, winds and
+// water species are stored whole.
+//
+// Launch shapes. Every launch covers all members of a batch: workgroup b
+// reads entry b of a block map {member, local tile}. Stencil kernels give
+// each 256-thread workgroup a tile of columns (four waves) and march k
+// upward through one vertical chunk, holding the vertical stencil of the
+// marched fields in registers, so each plane is fetched from HBM once per
+// tile and the horizontal halo comes from L1/L2. The fused acoustic
+// substep works on strips: a wave owns 63 consecutive columns of one row and
+// its 64th lane computes the eastern neighbour column, whose pressure the
+// momentum update needs, exchanged by a lane shuffle.
 //
 // Precision: S is the storage type. Continuity, EOS, pressure gradient,
 // base-state terms, slow tendencies, mass-flux accumulation and the HE-VI
@@ -28,52 +35,15 @@ namespace
 constexpr int kBlock = 256;
 
 template <class S>
-struct AccT;
-template <>
-struct AccT<float>
+struct AccT
 {
-    using type = float;
-};
-template <>
-struct AccT<double>
-{
-    using type = double;
+    using type = S;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on
-// XCD w % 8, each XCD with its own L2). Remapping w so that XCD x receives a
-// contiguous range of logical tiles keeps the j +- 1 rows and k +- 1 planes a
-// stencil reads in the same L2.
-__device__ __forceinline__ int xcd_logical(int w, int n)
+template <class S>
+__device__ __forceinline__ S* P(void* p)
 {
-    constexpr int kXcd = 8;
-    const int x = w % kXcd, local = w / kXcd;
-    const int q = n / kXcd, r = n % kXcd;
-    return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
-}
-
-// Cell kernels: one tile_w x (256 / tile_w) (i, j) tile of one k plane per
-// 256-thread block; logical tile order is k-major.
-__device__ __forceinline__ bool cell_index(const DevGrid& g, int& i, int& j, int& k)
-{
-    const int n = g.tiles_plane * g.nz;
-    const int logical = g.xcd_remap ? xcd_logical(int(blockIdx.x), n) : int(blockIdx.x);
-    k = logical / g.tiles_plane;
-    const int t = logical - k * g.tiles_plane;
-    const int by = t / g.tiles_x, bx = t - by * g.tiles_x;
-    i = (bx << g.tile_shift) + int(threadIdx.x & unsigned(g.tile_w - 1));
-    j = (by << (8 - g.tile_shift)) + int(threadIdx.x >> g.tile_shift);
-    return i < g.nx && j < g.ny;
-}
-
-// Column kernels: one thread per (i, j) column, same tile shape.
-__device__ __forceinline__ bool column_index(const DevGrid& g, int& i, int& j)
-{
-    const int t = g.xcd_remap ? xcd_logical(int(blockIdx.x), g.tiles_plane) : int(blockIdx.x);
-    const int by = t / g.tiles_x, bx = t - by * g.tiles_x;
-    i = (bx << g.tile_shift) + int(threadIdx.x & unsigned(g.tile_w - 1));
-    j = (by << (8 - g.tile_shift)) + int(threadIdx.x >> g.tile_shift);
-    return i < g.nx && j < g.ny;
+    return static_cast<S*>(p);
 }
 
 __device__ __forceinline__ double finite0(double x) { return isfinite(x) ? x : 0.0; }
@@ -96,137 +66,6 @@ __device__ __forceinline__ double clamp_store(double x, double fallback, float l
     }
 }
 
-__device__ __forceinline__ const double* slot(const double* base, int s, int nz) { return base + std::size_t(s) * nz; }
-
-// ---------------------------------------------------------------------
-// Slow tendencies at one interior cell (cartesian_cgrid_3d.cpp:1064-1356)
-// ---------------------------------------------------------------------
-template <class S>
-__device__ void slow_tendency_cell(const DevGrid& g, const S* __restrict__ u, const S* __restrict__ v,
-                                   const S* __restrict__ w, const S* __restrict__ rp, const S* __restrict__ qv,
-                                   const S* __restrict__ qc, const S* __restrict__ qr, const double* __restrict__ base,
-                                   double f, int i, int j, int k, double& du, double& dv, double& dw)
-{
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const double ix = g.idx, iy = g.idy, iz = g.idz;
-    const std::size_t c = g.at(i, j, k);
-    const std::size_t sx = 1, sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
-    auto F = [&](const S* a, std::size_t idx) { return double(a[idx]); };
-    // u at x-face
-    {
-        const double ux = F(u, c);
-        const double vy = 0.25 * (F(v, c - sy) + F(v, c) + F(v, c + sx - sy) + F(v, c + sx));
-        const double wz = k == 0 ? 0.25 * (F(w, c) + F(w, c + sx))
-                                 : 0.25 * (F(w, c - sz) + F(w, c) + F(w, c + sx - sz) + F(w, c + sx));
-        const double ddx = dev::upwind_axis<double>([&](int o) { return F(u, c + std::ptrdiff_t(o)); }, i, NX, ux, ix);
-        const double ddy = dev::upwind_axis<double>([&](int o) { return F(u, c + std::ptrdiff_t(o) * std::ptrdiff_t(sy)); }, j, NY, vy, iy);
-        const double ddz = k == 0 ? (wz >= 0.0 ? 0.0 : (F(u, c + sz) - F(u, c)) * iz)
-                                  : dev::upwind_axis<double>([&](int o) { return F(u, c + std::ptrdiff_t(o) * std::ptrdiff_t(sz)); }, k, NZ, wz, iz);
-        const double cor = f * (vy - slot(base, kBaseV0, NZ)[k]);
-        du = finite0(-ux * ddx - vy * ddy - wz * ddz + cor);
-    }
-    // v at y-face
-    {
-        const double vy = F(v, c);
-        const double ux = 0.25 * (F(u, c - sx) + F(u, c) + F(u, c - sx + sy) + F(u, c + sy));
-        const double wz = k == 0 ? 0.25 * (F(w, c) + F(w, c + sy))
-                                 : 0.25 * (F(w, c - sz) + F(w, c) + F(w, c + sy - sz) + F(w, c + sy));
-        const double ddx = dev::upwind_axis<double>([&](int o) { return F(v, c + std::ptrdiff_t(o)); }, i, NX, ux, ix);
-        const double ddy = dev::upwind_axis<double>([&](int o) { return F(v, c + std::ptrdiff_t(o) * std::ptrdiff_t(sy)); }, j, NY, vy, iy);
-        const double ddz = k == 0 ? (wz >= 0.0 ? 0.0 : (F(v, c + sz) - F(v, c)) * iz)
-                                  : dev::upwind_axis<double>([&](int o) { return F(v, c + std::ptrdiff_t(o) * std::ptrdiff_t(sz)); }, k, NZ, wz, iz);
-        const double cor = f * (ux - slot(base, kBaseU0, NZ)[k]);
-        dv = finite0(-ux * ddx - vy * ddy - wz * ddz - cor);
-    }
-    // w at z-face with buoyancy, moisture buoyancy and condensate loading
-    {
-        const double wz = F(w, c);
-        const double ux = 0.25 * (F(u, c - sx) + F(u, c) + F(u, c - sx + sz) + F(u, c + sz));
-        const double vy = 0.25 * (F(v, c - sy) + F(v, c) + F(v, c - sy + sz) + F(v, c + sz));
-        const double ddx = dev::upwind_axis<double>([&](int o) { return F(w, c + std::ptrdiff_t(o)); }, i, NX, ux, ix);
-        const double ddy = dev::upwind_axis<double>([&](int o) { return F(w, c + std::ptrdiff_t(o) * std::ptrdiff_t(sy)); }, j, NY, vy, iy);
-        const double ddz = k == 0 ? (wz >= 0.0 ? F(w, c) * iz : (F(w, c + sz) - F(w, c)) * iz)
-                                  : dev::upwind_axis<double>([&](int o) { return F(w, c + std::ptrdiff_t(o) * std::ptrdiff_t(sz)); }, k, NZ, wz, iz);
-        const double* rho0 = slot(base, kBaseRho0, NZ);
-        const double* qv0s = slot(base, kBaseQv0Stored, NZ);
-        const double rpf = 0.5 * (F(rp, c) + F(rp, c + sz));
-        const double rf = 0.5 * (rho0[k] + rho0[k + 1]) + rpf;
-        const double buoy = -kG * rpf / safe_rho(rf);
-        const double qvf = 0.5 * (F(qv, c) + F(qv, c + sz));
-        const double mbuoy = kG * 0.608 * (qvf - 0.5 * (qv0s[k] + qv0s[k + 1]));
-        const double load = -kG * (0.5 * ((F(qc, c) + F(qr, c)) + (F(qc, c + sz) + F(qr, c + sz))));
-        dw = finite0(-ux * ddx - vy * ddy - wz * ddz + buoy + mbuoy + load);
-    }
-}
-
-template <class S>
-__device__ __forceinline__ bool slow_interior(const DevGrid& g, int i, int j, int k)
-{
-    return i >= 1 && i < g.nx - 1 && j >= 1 && j < g.ny - 1 && k < g.nz - 1;
-}
-
-// Heun stage 1: k1 at u_n and the predictor u_n + dt k1 (dynamics.cpp:5178-5232).
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_slow_stage1(DevGrid g, const S* u, const S* v, const S* w, const S* rp,
-                                                        const S* qv, const S* qc, const S* qr, const double* base,
-                                                        double f, double dt, S* k1u, S* k1v, S* k1w, S* su, S* sv,
-                                                        S* sw)
-{
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
-    const std::size_t c = g.at(i, j, k);
-    double du = 0.0, dv = 0.0, dw = 0.0;
-    if (slow_interior<S>(g, i, j, k)) slow_tendency_cell(g, u, v, w, rp, qv, qc, qr, base, f, i, j, k, du, dv, dw);
-    const S a = S(du), b = S(dv), d = S(dw);
-    k1u[c] = a;
-    k1v[c] = b;
-    k1w[c] = d;
-    su[c] = S(double(u[c]) + double(a) * dt);
-    sv[c] = S(double(v[c]) + double(b) * dt);
-    sw[c] = S(double(w[c]) + double(d) * dt);
-}
-
-// Heun stage 2 + average in storage precision + slow apply (dynamics.cpp:5233-5292, 2055-2095).
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_slow_stage2(DevGrid g, S* u, S* v, S* w, const S* rp, const S* qv,
-                                                        const S* qc, const S* qr, const double* base, double f,
-                                                        double dt, const S* k1u, const S* k1v, const S* k1w,
-                                                        const S* su, const S* sv, const S* sw)
-{
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
-    const std::size_t c = g.at(i, j, k);
-    double du = 0.0, dv = 0.0, dw = 0.0;
-    if (slow_interior<S>(g, i, j, k)) slow_tendency_cell(g, su, sv, sw, rp, qv, qc, qr, base, f, i, j, k, du, dv, dw);
-    const S au = S(S(0.5) * (S(du) + k1u[c]));
-    const S av = S(S(0.5) * (S(dv) + k1v[c]));
-    const S aw = S(S(0.5) * (S(dw) + k1w[c]));
-    u[c] = S(double(u[c]) + finite0(double(au)) * dt);
-    v[c] = S(double(v[c]) + finite0(double(av)) * dt);
-    w[c] = S(double(w[c]) + finite0(double(aw)) * dt);
-}
-
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_slow_tendencies(DevGrid g, const S* u, const S* v, const S* w,
-                                                            const S* rp, const S* qv, const S* qc, const S* qr,
-                                                            const double* base, double f, S* du_o, S* dv_o, S* dw_o)
-{
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
-    const std::size_t c = g.at(i, j, k);
-    double du = 0.0, dv = 0.0, dw = 0.0;
-    if (slow_interior<S>(g, i, j, k)) slow_tendency_cell(g, u, v, w, rp, qv, qc, qr, base, f, i, j, k, du, dv, dw);
-    du_o[c] = S(du);
-    dv_o[c] = S(dv);
-    dw_o[c] = S(dw);
-}
-
-// ---------------------------------------------------------------------
-// Acoustic substep, explicit forward step: retained mass-flux
-// accumulation from the pre-continuity state (advection_cartesian.cpp:610-830),
-// C-grid continuity, density clamp and EOS pressure
-// (acoustic_cgrid_fast.cpp:49-157).
-// ---------------------------------------------------------------------
 // Clamp in FP64 without rounding the whole value to the storage type, so the
 // stored perturbation keeps its own precision.
 __device__ __forceinline__ double clamp_full(double x, double fallback, float lo, float hi)
@@ -235,12 +74,250 @@ __device__ __forceinline__ double clamp_full(double x, double fallback, float lo
     return fmin(fmax(x, double(lo)), double(hi));
 }
 
+__device__ __forceinline__ const double* slot(const double* base, int s, int nz) { return base + std::size_t(s) * nz; }
+
+// ---------------------------------------------------------------------
+// Work decoding
+// ---------------------------------------------------------------------
+struct Work
+{
+    int m, local;
+};
+
+__device__ __forceinline__ Work work_of(const int2* __restrict__ map)
+{
+    const int2 e = map[blockIdx.x];
+    return Work{e.x, e.y};
+}
+
+// Column tile of a tile_w x tile_h workgroup, vertical chunk kc of n_chunks.
+struct Tile
+{
+    int i, j, k0, k1;
+};
+
+__device__ __forceinline__ Tile tile_of(const MemberArgs& A, int local, int n_chunks)
+{
+    const MemberGeom& G = A.geo;
+    const int tiles = G.tiles_x * G.tiles_y;
+    const int kc = local / tiles, t = local - kc * tiles;
+    const int ty = t / G.tiles_x, tx = t - ty * G.tiles_x;
+    const int tid = int(threadIdx.x);
+    Tile r;
+    r.i = tx * G.tile_w + (tid & (G.tile_w - 1));
+    r.j = ty * G.tile_h + tid / G.tile_w;
+    const int nz = A.g.nz;
+    r.k0 = (kc * nz) / n_chunks;
+    r.k1 = ((kc + 1) * nz) / n_chunks;
+    return r;
+}
+
+// Strip of 63 owned columns per wave, four rows per workgroup; `west` puts
+// the halo lane on the west (lane 0) instead of the east (lane 63).
+struct Strip
+{
+    int i, j, lane, k0, k1;
+    bool own;
+};
+
+__device__ __forceinline__ Strip strip_of(const MemberArgs& A, int local, bool west)
+{
+    const MemberGeom& G = A.geo;
+    const int strips = G.strips_x * G.strips_y;
+    const int kc = local / strips, t = local - kc * strips;
+    const int sy = t / G.strips_x, sx = t - sy * G.strips_x;
+    Strip r;
+    r.lane = int(threadIdx.x & 63u);
+    r.j = sy * 4 + int(threadIdx.x >> 6);
+    r.i = sx * 63 + r.lane - (west ? 1 : 0);
+    r.own = west ? r.lane > 0 : r.lane < 63;
+    const int nz = A.g.nz;
+    r.k0 = (kc * nz) / G.k_chunks;
+    r.k1 = ((kc + 1) * nz) / G.k_chunks;
+    return r;
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// ---------------------------------------------------------------------
+// Slow tendencies (cartesian_cgrid_3d.cpp:1064-1356) at one cell. The own
+// column's u, v, w are supplied as vertical windows k-3 .. k+3; everything
+// else is read directly.
+// ---------------------------------------------------------------------
+template <class S>
+struct SlowFields
+{
+    const S *u, *v, *w, *rp, *qv, *qc, *qr;
+};
+
+template <class S>
+__device__ void slow_tendency_cell(const DevGrid& g, const SlowFields<S>& F, const double* __restrict__ base,
+                                   double fcor, int i, int j, int k, const double* uz, const double* vz,
+                                   const double* wz_, double rpk, double rpk1, double qvk, double qvk1, double lk,
+                                   double lk1, double& du, double& dv, double& dw)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const double ix = g.idx, iy = g.idy, iz = g.idz;
+    const std::size_t c = g.at(i, j, k);
+    const std::ptrdiff_t sx = 1, sy = g.px, sz = std::ptrdiff_t(g.plane());
+    auto R = [&](const S* a, std::ptrdiff_t o) { return double(a[std::ptrdiff_t(c) + o]); };
+    // u at x-face
+    {
+        const double ux = uz[3];
+        const double vy = 0.25 * (R(F.v, -sy) + vz[3] + R(F.v, sx - sy) + R(F.v, sx));
+        const double wz = k == 0 ? 0.25 * (wz_[3] + R(F.w, sx))
+                                 : 0.25 * (wz_[2] + wz_[3] + R(F.w, sx - sz) + R(F.w, sx));
+        const double ddx = dev::upwind_axis<double>([&](int o) { return R(F.u, o); }, i, NX, ux, ix);
+        const double ddy = dev::upwind_axis<double>([&](int o) { return R(F.u, o * sy); }, j, NY, vy, iy);
+        const double ddz = k == 0 ? (wz >= 0.0 ? 0.0 : (uz[4] - uz[3]) * iz)
+                                  : dev::upwind_axis<double>([&](int o) { return uz[3 + o]; }, k, NZ, wz, iz);
+        const double cor = fcor * (vy - slot(base, kBaseV0, NZ)[k]);
+        du = finite0(-ux * ddx - vy * ddy - wz * ddz + cor);
+    }
+    // v at y-face
+    {
+        const double vy = vz[3];
+        const double ux = 0.25 * (R(F.u, -sx) + uz[3] + R(F.u, -sx + sy) + R(F.u, sy));
+        const double wz = k == 0 ? 0.25 * (wz_[3] + R(F.w, sy))
+                                 : 0.25 * (wz_[2] + wz_[3] + R(F.w, sy - sz) + R(F.w, sy));
+        const double ddx = dev::upwind_axis<double>([&](int o) { return R(F.v, o); }, i, NX, ux, ix);
+        const double ddy = dev::upwind_axis<double>([&](int o) { return R(F.v, o * sy); }, j, NY, vy, iy);
+        const double ddz = k == 0 ? (wz >= 0.0 ? 0.0 : (vz[4] - vz[3]) * iz)
+                                  : dev::upwind_axis<double>([&](int o) { return vz[3 + o]; }, k, NZ, wz, iz);
+        const double cor = fcor * (ux - slot(base, kBaseU0, NZ)[k]);
+        dv = finite0(-ux * ddx - vy * ddy - wz * ddz - cor);
+    }
+    // w at z-face with buoyancy, moisture buoyancy and condensate loading
+    {
+        const double wz = wz_[3];
+        const double ux = 0.25 * (R(F.u, -sx) + uz[3] + R(F.u, -sx + sz) + uz[4]);
+        const double vy = 0.25 * (R(F.v, -sy) + vz[3] + R(F.v, -sy + sz) + vz[4]);
+        const double ddx = dev::upwind_axis<double>([&](int o) { return R(F.w, o); }, i, NX, ux, ix);
+        const double ddy = dev::upwind_axis<double>([&](int o) { return R(F.w, o * sy); }, j, NY, vy, iy);
+        const double ddz = k == 0 ? (wz >= 0.0 ? wz_[3] * iz : (wz_[4] - wz_[3]) * iz)
+                                  : dev::upwind_axis<double>([&](int o) { return wz_[3 + o]; }, k, NZ, wz, iz);
+        const double* rho0 = slot(base, kBaseRho0, NZ);
+        const double* qv0s = slot(base, kBaseQv0Stored, NZ);
+        const double rpf = 0.5 * (rpk + rpk1);
+        const double rf = 0.5 * (rho0[k] + rho0[k + 1]) + rpf;
+        const double buoy = -kG * rpf / safe_rho(rf);
+        const double qvf = 0.5 * (qvk + qvk1);
+        const double mbuoy = kG * 0.608 * (qvf - 0.5 * (qv0s[k] + qv0s[k + 1]));
+        const double load = -kG * (0.5 * (lk + lk1));
+        dw = finite0(-ux * ddx - vy * ddy - wz * ddz + buoy + mbuoy + load);
+    }
+}
+
+// Vertical window of seven values k-3 .. k+3 of one column (clamped reads).
+template <class S>
+struct Win7
+{
+    double v[7];
+    __device__ __forceinline__ void fill(const S* __restrict__ a, std::size_t col, std::size_t sz, int k, int nz)
+    {
+#pragma unroll
+        for (int o = 0; o < 7; ++o) v[o] = double(a[col + std::size_t(clampi(k + o - 3, 0, nz - 1)) * sz]);
+    }
+    __device__ __forceinline__ void shift(const S* __restrict__ a, std::size_t col, std::size_t sz, int k, int nz)
+    {
+#pragma unroll
+        for (int o = 0; o < 6; ++o) v[o] = v[o + 1];
+        v[6] = double(a[col + std::size_t(clampi(k + 4, 0, nz - 1)) * sz]);
+    }
+};
+
+// Heun slow RK2 (dynamics.cpp:5178-5292, 2055-2095) in two column marches.
+// MODE 0: stage 1 at u_n -> k1 and predictor u_n + dt k1 (into u2, v2, w2).
+// MODE 1: stage 2 at the predictor, average in storage precision, apply to u_n.
+// MODE 2: tendencies only (per-operation dispatch) into out_u/v/w.
+template <class S, int MODE>
+__global__ __launch_bounds__(kBlock) void k_slow(const MemberArgs* __restrict__ ma, const int2* __restrict__ map,
+                                                 S* out_u, S* out_v, S* out_w)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const bool valid = t.i < NX && t.j < NY;
+    const int i = min(t.i, NX - 1), j = min(t.j, NY - 1);
+    const DevFields& f = A.f;
+    SlowFields<S> F;
+    F.u = P<S>(MODE == 1 ? f.u2 : f.u);
+    F.v = P<S>(MODE == 1 ? f.v2 : f.v);
+    F.w = P<S>(MODE == 1 ? f.w2 : f.w);
+    F.rp = P<S>(f.rp);
+    F.qv = P<S>(f.qv);
+    F.qc = P<S>(f.qc);
+    F.qr = P<S>(f.qr);
+    const double* base = f.base;
+    const double dt = A.p.dt, fcor = A.p.coriolis_f;
+    const std::size_t sz = g.plane();
+    const std::size_t col = g.at(i, j, 0);
+    const bool hinterior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
+    Win7<S> U, V, Wz;
+    U.fill(F.u, col, sz, t.k0, NZ);
+    V.fill(F.v, col, sz, t.k0, NZ);
+    Wz.fill(F.w, col, sz, t.k0, NZ);
+    auto at_k = [&](const S* a, int k) { return double(a[col + std::size_t(clampi(k, 0, NZ - 1)) * sz]); };
+    double rp0 = at_k(F.rp, t.k0), qv0 = at_k(F.qv, t.k0), l0 = at_k(F.qc, t.k0) + at_k(F.qr, t.k0);
+    for (int k = t.k0; k < t.k1; ++k)
+    {
+        const std::size_t c = col + std::size_t(k) * sz;
+        const double rp1 = at_k(F.rp, k + 1), qv1 = at_k(F.qv, k + 1);
+        const double l1 = at_k(F.qc, k + 1) + at_k(F.qr, k + 1);
+        double du = 0.0, dv = 0.0, dw = 0.0;
+        if (hinterior && k < NZ - 1)
+            slow_tendency_cell<S>(g, F, base, fcor, i, j, k, U.v, V.v, Wz.v, rp0, rp1, qv0, qv1, l0, l1, du, dv, dw);
+        if (valid)
+        {
+            if constexpr (MODE == 0)
+            {
+                const S a = S(du), b = S(dv), d = S(dw);
+                P<S>(f.k1u)[c] = a;
+                P<S>(f.k1v)[c] = b;
+                P<S>(f.k1w)[c] = d;
+                P<S>(f.u2)[c] = S(U.v[3] + double(a) * dt);
+                P<S>(f.v2)[c] = S(V.v[3] + double(b) * dt);
+                P<S>(f.w2)[c] = S(Wz.v[3] + double(d) * dt);
+            }
+            else if constexpr (MODE == 1)
+            {
+                const S au = S(S(0.5) * (S(du) + P<S>(f.k1u)[c]));
+                const S av = S(S(0.5) * (S(dv) + P<S>(f.k1v)[c]));
+                const S aw = S(S(0.5) * (S(dw) + P<S>(f.k1w)[c]));
+                S* u = P<S>(f.u);
+                S* v = P<S>(f.v);
+                S* w = P<S>(f.w);
+                u[c] = S(double(u[c]) + finite0(double(au)) * dt);
+                v[c] = S(double(v[c]) + finite0(double(av)) * dt);
+                w[c] = S(double(w[c]) + finite0(double(aw)) * dt);
+            }
+            else
+            {
+                out_u[c] = S(du);
+                out_v[c] = S(dv);
+                out_w[c] = S(dw);
+            }
+        }
+        rp0 = rp1;
+        qv0 = qv1;
+        l0 = l1;
+        U.shift(F.u, col, sz, k, NZ);
+        V.shift(F.v, col, sz, k, NZ);
+        Wz.shift(F.w, col, sz, k, NZ);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Density clamp and EOS (acoustic_cgrid_fast.cpp:118-157).
 // Pressure perturbation p - p0 from the dry EOS. FP64 storage evaluates
 // p00 (rho R theta / p00)^gamma in FP64; FP32 storage evaluates the same
 // expression relative to the base state, p = pe0 (1 + x)^gamma with
 // 1 + x = rho theta / (rho0 theta0) and pe0 = EOS(rho0, theta0) in FP64, so
 // p - p0 = pe0 expm1(gamma log1p(x)) + (pe0 - p0) keeps FP32 relative
 // accuracy on the perturbation rather than on the full pressure.
+// ---------------------------------------------------------------------
 template <class S>
 __device__ __forceinline__ double eos_full_pressure(const double* base, int nz, int k, double rho, double theta)
 {
@@ -258,124 +335,207 @@ __device__ __forceinline__ double eos_full_pressure(const double* base, int nz, 
 }
 
 template <class S>
-__device__ __forceinline__ void density_eos_store(const Bounds& bd, const double* base, int nz, int k, double rn,
-                                                  double theta, S* rp_out, S* pp, std::size_t c)
+struct RhoP
+{
+    S r, p;  // stored rho' and p'
+};
+
+template <class S>
+__device__ __forceinline__ RhoP<S> density_eos(const Bounds& bd, const double* base, int nz, int k, double rn,
+                                               double theta)
 {
     const double rho0k = base[std::size_t(kBaseRho0) * nz + k];
     const double p0k = base[std::size_t(kBaseP0) * nz + k];
     if (!isfinite(rn) || rn <= 0.0) rn = fmax(0.1, rho0k);
     const double rs = clamp_full(rn, 1.0, bd.rho_min, bd.rho_max);
-    const S rps = S(rs - rho0k);
-    rp_out[c] = rps;
-    const double rho_used = rho0k + double(rps);
+    RhoP<S> o;
+    o.r = S(rs - rho0k);
+    const double rho_used = rho0k + double(o.r);
     double pn = double(bd.p_fallback);
     if (theta > 0.0) pn = eos_full_pressure<S>(base, nz, k, rho_used, theta);
     if (!isfinite(pn) || pn <= 0.0) pn = double(bd.p_fallback);
-    pp[c] = S(clamp_full(pn, 101325.0, bd.p_min, bd.p_max) - p0k);
+    o.p = S(clamp_full(pn, 101325.0, bd.p_min, bd.p_max) - p0k);
+    return o;
 }
 
+// ---------------------------------------------------------------------
+// Fused explicit acoustic substep (acoustic_cgrid_fast.cpp:12-219) with the
+// retained mass-flux accumulation of the pre-continuity state
+// (advection_cartesian.cpp:610-830). Substep s reads u, v, w, rho' from
+// buffer s & 1 and writes buffer (s + 1) & 1, so no workgroup reads what
+// another writes. Per wave and plane: forward pass (continuity, clamp, EOS)
+// of the own column, of the northern neighbour column (recomputed) and, in
+// lane 63, of the eastern neighbour; the momentum update of plane k runs
+// once the forward pass of plane k + 1 is known.
+// ---------------------------------------------------------------------
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_acoustic_forward(DevGrid g, const S* __restrict__ u,
-                                                             const S* __restrict__ v, const S* __restrict__ w,
-                                                             const S* __restrict__ rp_in, S* __restrict__ rp_out,
-                                                             S* __restrict__ pp, const S* __restrict__ th,
-                                                             S* __restrict__ mfx, S* __restrict__ mfy,
-                                                             S* __restrict__ mfz, S* __restrict__ rb,
-                                                             const double* __restrict__ base, Bounds bd, double dts,
-                                                             double weight, int first, int accumulate, int top_open)
+struct AcousticBufs
 {
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
+    const S *uo, *vo, *wo, *ro, *th;
+    S *un, *vn, *wn, *rn, *pp, *mfx, *mfy, *mfz, *rb;
+};
+
+template <class S>
+__device__ __forceinline__ RhoP<S> acoustic_forward(const DevGrid& g, const AcousticBufs<S>& B,
+                                                    const double* __restrict__ base, const Bounds& bd, double dts,
+                                                    double weight, int i, int j, int k, bool store, bool accumulate,
+                                                    bool first, bool top_open)
+{
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const std::size_t c = g.at(i, j, k);
-    const std::size_t sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
+    const std::size_t sy = std::size_t(g.px), sz = g.plane();
     const double* rho0 = slot(base, kBaseRho0, NZ);
-    const double* p0 = slot(base, kBaseP0, NZ);
     const double r0 = rho0[k];
-    const double rc = r0 + double(rp_in[c]);
-    if (accumulate)
+    const double rc = r0 + double(B.ro[c]);
+    const double rxp = i < NX - 1 ? r0 + double(B.ro[c + 1]) : 0.0;
+    const double ryp = j < NY - 1 ? r0 + double(B.ro[c + sy]) : 0.0;
+    const double rzp = k < NZ - 1 ? rho0[k + 1] + double(B.ro[c + sz]) : 0.0;
+    const double uc = double(B.uo[c]), vc = double(B.vo[c]), wc = double(B.wo[c]);
+    if (store && accumulate)
     {
-        if (i < NX - 1)
-        {
-            const double fl = 0.5 * (rc + (r0 + double(rp_in[c + 1]))) * double(u[c]);
-            mfx[c] = S((first ? 0.0 : double(mfx[c])) + weight * fl);
-        }
-        if (j < NY - 1)
-        {
-            const double fl = 0.5 * (rc + (r0 + double(rp_in[c + sy]))) * double(v[c]);
-            mfy[c] = S((first ? 0.0 : double(mfy[c])) + weight * fl);
-        }
+        if (i < NX - 1) B.mfx[c] = S((first ? 0.0 : double(B.mfx[c])) + weight * (0.5 * (rc + rxp) * uc));
+        if (j < NY - 1) B.mfy[c] = S((first ? 0.0 : double(B.mfy[c])) + weight * (0.5 * (rc + ryp) * vc));
         double fl = 0.0;
-        if (k < NZ - 1) fl = 0.5 * (rc + (rho0[k + 1] + double(rp_in[c + sz]))) * double(w[c]);
-        else if (top_open) fl = rc * double(w[c]);
-        mfz[c] = S((first ? 0.0 : double(mfz[c])) + weight * fl);
+        if (k < NZ - 1) fl = 0.5 * (rc + rzp) * wc;
+        else if (top_open) fl = rc * wc;
+        B.mfz[c] = S((first ? 0.0 : double(B.mfz[c])) + weight * fl);
     }
-    if (first) rb[c] = rp_in[c];
+    if (store && first) B.rb[c] = B.ro[c];
     double d = 0.0;
     if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1)
     {
-        const double rxp = r0 + double(rp_in[c + 1]), rxm = r0 + double(rp_in[c - 1]);
-        const double ryp = r0 + double(rp_in[c + sy]), rym = r0 + double(rp_in[c - sy]);
-        const double fxp = 0.5 * (rc + rxp) * double(u[c]);
-        const double fxm = 0.5 * (rxm + rc) * double(u[c - 1]);
-        const double fyp = 0.5 * (rc + ryp) * double(v[c]);
-        const double fym = 0.5 * (rym + rc) * double(v[c - sy]);
-        const double fzp = k + 1 < NZ ? 0.5 * (rc + (rho0[k + 1] + double(rp_in[c + sz]))) * double(w[c])
-                                      : rc * double(w[c]);
-        const double fzm = k > 0 ? 0.5 * ((rho0[k - 1] + double(rp_in[c - sz])) + rc) * double(w[c - sz]) : 0.0;
+        const double rxm = r0 + double(B.ro[c - 1]);
+        const double rym = r0 + double(B.ro[c - sy]);
+        const double fxp = 0.5 * (rc + rxp) * uc;
+        const double fxm = 0.5 * (rxm + rc) * double(B.uo[c - 1]);
+        const double fyp = 0.5 * (rc + ryp) * vc;
+        const double fym = 0.5 * (rym + rc) * double(B.vo[c - sy]);
+        const double fzp = k + 1 < NZ ? 0.5 * (rc + rzp) * wc : rc * wc;
+        const double fzm = k > 0 ? 0.5 * ((rho0[k - 1] + double(B.ro[c - sz])) + rc) * double(B.wo[c - sz]) : 0.0;
         d = -((fxp - fxm) * g.idx + (fyp - fym) * g.idy + (fzp - fzm) * g.idz);
     }
     const S drho = S(finite0(d));
-    density_eos_store<S>(bd, base, NZ, k, rc + double(drho) * dts, double(th[c]), rp_out, pp, c);
+    const RhoP<S> o = density_eos<S>(bd, base, NZ, k, rc + double(drho) * dts, double(B.th[c]));
+    if (store)
+    {
+        B.rn[c] = o.r;
+        B.pp[c] = o.p;
+    }
+    return o;
 }
 
-// Backward momentum update with the new pressure (acoustic_cgrid_fast.cpp:159-215);
-// also commits the new density to the home buffer. With skip_w the interior
-// w (solved by the HE-VI column kernel) is left untouched.
+// Backward momentum update of one cell with the new rho' and p' of the cell
+// and its +x, +y, +z neighbours (acoustic_cgrid_fast.cpp:159-215).
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_acoustic_backward(DevGrid g, S* __restrict__ u, S* __restrict__ v,
-                                                              S* __restrict__ w, const S* __restrict__ rp_new,
-                                                              S* __restrict__ rp_home, const S* __restrict__ pp,
-                                                              const double* __restrict__ base, Bounds bd,
-                                                              double dts, int skip_w)
+__device__ __forceinline__ void momentum_update(const DevGrid& g, const double* __restrict__ base, const Bounds& bd,
+                                                double dts, int i, int j, int k, RhoP<S> cc, RhoP<S> xp, RhoP<S> yp,
+                                                RhoP<S> zp, double u, double v, double w, bool skip_w, S* un, S* vn,
+                                                S* wn, std::size_t c)
 {
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const std::size_t c = g.at(i, j, k);
-    const std::size_t sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
     const double* rho0 = slot(base, kBaseRho0, NZ);
     double du = 0.0, dv = 0.0, dw = 0.0;
     const bool interior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k < NZ - 1;
     if (interior)
     {
         const double r0 = rho0[k];
-        const double rc = r0 + double(rp_new[c]);
-        const double rx = 0.5 * (rc + (r0 + double(rp_new[c + 1])));
-        const double ry = 0.5 * (rc + (r0 + double(rp_new[c + sy])));
-        const double rz = 0.5 * (rc + (rho0[k + 1] + double(rp_new[c + sz])));
+        const double rc = r0 + double(cc.r);
+        const double rx = 0.5 * (rc + (r0 + double(xp.r)));
+        const double ry = 0.5 * (rc + (r0 + double(yp.r)));
+        const double rz = 0.5 * (rc + (rho0[k + 1] + double(zp.r)));
         if constexpr (sizeof(S) == sizeof(float))
         {
             // FP32 tendency, as the reference rounds it to FP32 before the update
             auto safe_f = [](float r) { return isfinite(r) && r > 1.0e-6f ? r : 1.0f; };
             auto fin = [](float x) { return isfinite(x) ? x : 0.0f; };
-            const float pc = float(pp[c]);
-            du = double(fin(-(float(pp[c + 1]) - pc) * float(g.idx) / safe_f(float(rx))));
-            dv = double(fin(-(float(pp[c + sy]) - pc) * float(g.idy) / safe_f(float(ry))));
-            dw = double(fin(-((float(pp[c + sz]) - pc) * float(g.idz)) / safe_f(float(rz))));
+            const float pc = float(cc.p);
+            du = double(fin(-(float(xp.p) - pc) * float(g.idx) / safe_f(float(rx))));
+            dv = double(fin(-(float(yp.p) - pc) * float(g.idy) / safe_f(float(ry))));
+            dw = double(fin(-((float(zp.p) - pc) * float(g.idz)) / safe_f(float(rz))));
         }
         else
         {
-            const double pc = double(pp[c]);
-            du = finite0(-(double(pp[c + 1]) - pc) * g.idx / safe_rho(rx));
-            dv = finite0(-(double(pp[c + sy]) - pc) * g.idy / safe_rho(ry));
-            dw = finite0(-((double(pp[c + sz]) - pc) * g.idz) / safe_rho(rz));
+            const double pc = double(cc.p);
+            du = finite0(-(double(xp.p) - pc) * g.idx / safe_rho(rx));
+            dv = finite0(-(double(yp.p) - pc) * g.idy / safe_rho(ry));
+            dw = finite0(-((double(zp.p) - pc) * g.idz) / safe_rho(rz));
         }
     }
-    u[c] = S(clamp_store<S>(double(u[c]) + du * dts, 0.0, -bd.wind_h, bd.wind_h));
-    v[c] = S(clamp_store<S>(double(v[c]) + dv * dts, 0.0, -bd.wind_h, bd.wind_h));
-    if (!(skip_w && interior)) w[c] = S(clamp_store<S>(double(w[c]) + dw * dts, 0.0, -bd.wind_v, bd.wind_v));
-    if (rp_home != rp_new) rp_home[c] = rp_new[c];
+    un[c] = S(clamp_store<S>(u + du * dts, 0.0, -bd.wind_h, bd.wind_h));
+    vn[c] = S(clamp_store<S>(v + dv * dts, 0.0, -bd.wind_h, bd.wind_h));
+    if (!(skip_w && interior)) wn[c] = S(clamp_store<S>(w + dw * dts, 0.0, -bd.wind_v, bd.wind_v));
+}
+
+template <class S>
+__device__ __forceinline__ RhoP<S> shfl_down1(RhoP<S> v)
+{
+    RhoP<S> o;
+    o.r = __shfl_down(v.r, 1);
+    o.p = __shfl_down(v.p, 1);
+    return o;
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_acoustic_fused(const MemberArgs* __restrict__ ma,
+                                                           const int2* __restrict__ map, int s, int accumulate,
+                                                           int init)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (s >= A.p.substeps) return;
+    const DevGrid& g = A.g;
+    const Strip st = strip_of(A, wk.local, false);
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    if (st.j >= NY) return;  // whole wave (rows are wave-uniform); no workgroup barrier follows
+    const int j = st.j;
+    const bool have = st.i < NX;
+    const bool own = st.own && have;
+    const int i = min(st.i, NX - 1);
+    const DevFields& f = A.f;
+    const int par = s & 1;
+    AcousticBufs<S> B;
+    B.uo = P<S>(par ? f.u2 : f.u);
+    B.vo = P<S>(par ? f.v2 : f.v);
+    B.wo = P<S>(par ? f.w2 : f.w);
+    B.ro = P<S>(par ? f.rp2 : f.rp);
+    B.un = P<S>(par ? f.u : f.u2);
+    B.vn = P<S>(par ? f.v : f.v2);
+    B.wn = P<S>(par ? f.w : f.w2);
+    B.rn = P<S>(par ? f.rp : f.rp2);
+    B.th = P<S>(f.th);
+    B.pp = P<S>(f.pp);
+    B.mfx = P<S>(f.mfx);
+    B.mfy = P<S>(f.mfy);
+    B.mfz = P<S>(f.mfz);
+    B.rb = P<S>(f.rb);
+    const double* base = f.base;
+    const Bounds bd = A.p.bounds;
+    const double dts = A.p.dts, weight = A.p.mf_weight;
+    const bool acc = accumulate != 0, first = init != 0 && s == 0;
+    const bool top_open = A.p.bc_top == NIMBUS_TOP_OPEN;
+    const bool yn = j + 1 < NY;
+    const int k0 = st.k0, k1 = st.k1;
+    RhoP<S> cur = acoustic_forward<S>(g, B, base, bd, dts, weight, i, j, k0, own, acc, first, top_open);
+    RhoP<S> ycur{};
+    if (yn) ycur = acoustic_forward<S>(g, B, base, bd, dts, weight, i, j + 1, k0, false, acc, first, top_open);
+    for (int k = k0; k < k1; ++k)
+    {
+        RhoP<S> nxt{}, ynxt{};
+        if (k + 1 < NZ)
+            nxt = acoustic_forward<S>(g, B, base, bd, dts, weight, i, j, k + 1, own && k + 1 < k1, acc, first,
+                                      top_open);
+        if (yn && k + 1 < k1)
+            ynxt = acoustic_forward<S>(g, B, base, bd, dts, weight, i, j + 1, k + 1, false, acc, first, top_open);
+        const RhoP<S> xcur = shfl_down1(cur);
+        if (own)
+        {
+            const std::size_t c = g.at(i, j, k);
+            momentum_update<S>(g, base, bd, dts, i, j, k, cur, xcur, ycur, nxt, double(B.uo[c]), double(B.vo[c]),
+                               double(B.wo[c]), false, B.un, B.vn, B.wn, c);
+        }
+        cur = nxt;
+        ycur = ynxt;
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -383,29 +543,48 @@ __global__ __launch_bounds__(kBlock) void k_acoustic_backward(DevGrid g, S* __re
 // coupling off-centred implicit (alpha = (1+beta)/2), p linearised about
 // p* = EOS(rho*), FP64 Thomas sweep per column, flux-form density update and
 // EOS re-diagnosis. Ghost columns get the reference's pass-2 clamp/EOS only.
+// rho' is read from buffer s & 1 and written to (s + 1) & 1; u, v, w stay in
+// their home buffers (w is only touched by its own column).
 // ---------------------------------------------------------------------
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_hevi_column(DevGrid g, const S* __restrict__ u,
-                                                        const S* __restrict__ v, S* __restrict__ w,
-                                                        const S* __restrict__ rp_in, S* __restrict__ rp_out,
-                                                        S* __restrict__ pp, const S* __restrict__ th,
-                                                        S* __restrict__ mfx, S* __restrict__ mfy,
-                                                        S* __restrict__ mfz, S* __restrict__ rb,
-                                                        double* __restrict__ tc, double* __restrict__ tr,
-                                                        const double* __restrict__ base, Bounds bd, double dts,
-                                                        double alpha, double weight, int first, int accumulate)
+__global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __restrict__ ma,
+                                                        const int2* __restrict__ map, int s, int accumulate, int init)
 {
-    int i, j;
-    if (!column_index(g, i, j)) return;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (s >= A.p.substeps) return;
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, 1);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const std::size_t sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
+    if (t.i >= NX || t.j >= NY) return;  // no barrier in this kernel
+    const int i = t.i, j = t.j;
+    const DevFields& f = A.f;
+    const int par = s & 1;
+    const S* __restrict__ u = P<S>(f.u);
+    const S* __restrict__ v = P<S>(f.v);
+    S* __restrict__ w = P<S>(f.w);
+    const S* __restrict__ rp_in = P<S>(par ? f.rp2 : f.rp);
+    S* __restrict__ rp_out = P<S>(par ? f.rp : f.rp2);
+    S* __restrict__ pp = P<S>(f.pp);
+    const S* __restrict__ th = P<S>(f.th);
+    S* __restrict__ mfx = P<S>(f.mfx);
+    S* __restrict__ mfy = P<S>(f.mfy);
+    S* __restrict__ mfz = P<S>(f.mfz);
+    S* __restrict__ rb = P<S>(f.rb);
+    double* __restrict__ tc = f.tri_c;
+    double* __restrict__ tr = f.tri_r;
+    const double* base = f.base;
+    const Bounds bd = A.p.bounds;
+    const double dts = A.p.dts, alpha = A.p.hevi_alpha, weight = A.p.mf_weight;
+    const bool acc = accumulate != 0, first = init != 0 && s == 0;
+    const std::size_t sy = std::size_t(g.px), sz = g.plane();
     const double* rho0 = slot(base, kBaseRho0, NZ);
     const double* p0 = slot(base, kBaseP0, NZ);
     const bool interior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
     const double idz = g.idz;
     const double a1 = 1.0 - alpha;
     // ---- upward sweep: mass fluxes, rho*, p*, Thomas forward elimination ----
-    double rho_prev = 0.0, rb_prev = 0.0, C_prev = 0.0, ps_prev = 0.0, wo_prev = 0.0, e_prev = 0.0;
+    double rb_prev = 0.0, C_prev = 0.0, ps_prev = 0.0, wo_prev = 0.0, e_prev = 0.0;
     double rb_prev2 = 0.0, cp_prev = 0.0, rr_prev = 0.0;
     bool ok = true;
     for (int k = 0; k < NZ; ++k)
@@ -413,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(DevGrid g, const S* __re
         const std::size_t c = g.at(i, j, k);
         const double r0 = rho0[k];
         const double rc = r0 + double(rp_in[c]);
-        if (accumulate)
+        if (acc)
         {
             if (i < NX - 1)
             {
@@ -430,7 +609,9 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(DevGrid g, const S* __re
         if (first) rb[c] = rp_in[c];
         if (!interior)
         {
-            density_eos_store<S>(bd, base, NZ, k, rc, double(th[c]), rp_out, pp, c);
+            const RhoP<S> o = density_eos<S>(bd, base, NZ, k, rc, double(th[c]));
+            rp_out[c] = o.r;
+            pp[c] = o.p;
             continue;
         }
         const double rxp = r0 + double(rp_in[c + 1]), rxm = r0 + double(rp_in[c - 1]);
@@ -451,22 +632,21 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(DevGrid g, const S* __re
             const double rbr = rb_prev;
             const double Ak = dts * alpha / (rbr * g.dz);
             const double Bc = alpha * dts * idz;
-            const double A = (k - 1) > 0 ? -Ak * Bc * C_prev * rb_prev2 : 0.0;
+            const double Aa = (k - 1) > 0 ? -Ak * Bc * C_prev * rb_prev2 : 0.0;
             const double U = (k - 1) < NZ - 2 ? -Ak * Bc * C * rbk : 0.0;
             const double D = 1.0 + Ak * Bc * (C + C_prev) * rbr;
             const double G = (ps - ps_prev) * idz - (p0[k] - p0[k - 1]) * idz;
             const double R = wo_prev - dts / rbr * G - Ak * (C * e - C_prev * e_prev);
-            const double piv = D - A * cp_prev;
+            const double piv = D - Aa * cp_prev;
             if (!(piv > 0.0) || !isfinite(piv)) ok = false;
             const double cpr = U / piv;
-            const double rrr = (R - A * rr_prev) / piv;
+            const double rrr = (R - Aa * rr_prev) / piv;
             const std::size_t cr = g.at(i, j, k - 1);
             tc[cr] = cpr;
             tr[cr] = rrr;
             cp_prev = cpr;
             rr_prev = rrr;
         }
-        rho_prev = rc;
         rb_prev2 = rb_prev;
         rb_prev = rbk;
         C_prev = C;
@@ -497,19 +677,54 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(DevGrid g, const S* __re
         const std::size_t cc = g.at(i, j, kc);
         const double ft = kc < NZ - 1 ? rb_above * (alpha * w_above + a1 * wo_above) : 0.0;
         const double fb = r >= 0 ? rbr * (alpha * wn + a1 * wo) : 0.0;
-        if (accumulate) mfz[cc] = S((first ? 0.0 : double(mfz[cc])) + weight * ft);
+        if (acc) mfz[cc] = S((first ? 0.0 : double(mfz[cc])) + weight * ft);
         const double rstar = rho0[kc] + double(rp_out[cc]);
-        density_eos_store<S>(bd, base, NZ, kc, rstar - dts * idz * (ft - fb), double(th[cc]), rp_out, pp, cc);
+        const RhoP<S> o = density_eos<S>(bd, base, NZ, kc, rstar - dts * idz * (ft - fb), double(th[cc]));
+        rp_out[cc] = o.r;
+        pp[cc] = o.p;
         w_above = wn;
         wo_above = wo;
         rb_above = rbr;
     }
 }
 
+// HE-VI backward step for u and v with the new pressure; w is clamped only
+// where the column solve did not set it (ghost columns, lid).
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_hevi_momentum(const MemberArgs* __restrict__ ma,
+                                                          const int2* __restrict__ map, int s)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (s >= A.p.substeps) return;
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    if (t.i >= NX || t.j >= NY) return;
+    const int i = t.i, j = t.j;
+    const DevFields& f = A.f;
+    const S* __restrict__ rn = P<S>((s & 1) ? f.rp : f.rp2);
+    const S* __restrict__ pp = P<S>(f.pp);
+    S* u = P<S>(f.u);
+    S* v = P<S>(f.v);
+    S* w = P<S>(f.w);
+    const std::size_t sy = std::size_t(g.px), sz = g.plane();
+    for (int k = t.k0; k < t.k1; ++k)
+    {
+        const std::size_t c = g.at(i, j, k);
+        const RhoP<S> cc{rn[c], pp[c]};
+        const RhoP<S> xp = i < NX - 1 ? RhoP<S>{rn[c + 1], pp[c + 1]} : cc;
+        const RhoP<S> yp = j < NY - 1 ? RhoP<S>{rn[c + sy], pp[c + sy]} : cc;
+        const RhoP<S> zp = k < NZ - 1 ? RhoP<S>{rn[c + sz], pp[c + sz]} : cc;
+        momentum_update<S>(g, f.base, A.p.bounds, A.p.dts, i, j, k, cc, xp, yp, zp, double(u[c]), double(v[c]),
+                           double(w[c]), true, u, v, w, c);
+    }
+}
+
 // ---------------------------------------------------------------------
 // Boundary closure (boundary_conditions_cartesian_cgrid.cpp:798-1175):
 // west/east faces, then south/north, then the vertical closure; KW78
-// radiative faces integrate their FP32 stash when dt_rad > 0.
+// radiative faces integrate their FP32 stash when radiating.
 // ---------------------------------------------------------------------
 __device__ __forceinline__ float kw78_face(float prev, float phase, float grad, float dt, float inv_d, bool west_like)
 {
@@ -521,23 +736,55 @@ __device__ __forceinline__ float kw78_face(float prev, float phase, float grad, 
 }
 
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_bc_west_east(DevGrid g, S* u, S* v, S* w, S* rp, S* pp, float* rad_w,
-                                                         float* rad_e, int bcw, int bce, float dt_rad, float inv_dx)
+struct BcBufs
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
+    S *u, *v, *w, *rp, *pp;
+    float dt_rad;
+};
+
+// Buffers current after substep s (s < 0: home buffers, no radiation).
+template <class S>
+__device__ __forceinline__ bool bc_buffers(const MemberArgs& A, int s, BcBufs<S>& b)
+{
+    const DevFields& f = A.f;
+    if (s >= 0 && s >= A.p.substeps) return false;
+    const int par = s >= 0 ? ((s + 1) & 1) : 0;
+    const bool explicit_mode = A.p.acoustic_mode == NIMBUS_ACOUSTIC_EXPLICIT;
+    const int pu = explicit_mode ? par : 0;
+    b.u = P<S>(pu ? f.u2 : f.u);
+    b.v = P<S>(pu ? f.v2 : f.v);
+    b.w = P<S>(pu ? f.w2 : f.w);
+    b.rp = P<S>(par ? f.rp2 : f.rp);
+    b.pp = P<S>(f.pp);
+    b.dt_rad = s >= 0 ? float(A.p.dts) : 0.0f;
+    return true;
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_bc_west_east(const MemberArgs* __restrict__ ma,
+                                                         const int2* __restrict__ map, int s)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    BcBufs<S> b;
+    if (!bc_buffers<S>(A, s, b)) return;
+    const DevGrid& g = A.g;
+    const int t = wk.local * kBlock + int(threadIdx.x);
     if (t >= g.ny * g.nz) return;
     const int k = t / g.ny, j = t - k * g.ny;
     const int NX = g.nx, NZ = g.nz;
-    const bool radiate = dt_rad > 0.0f;
-    const float cstar = 30.0f;
+    const int bcw = A.p.bc_west, bce = A.p.bc_east;
+    const bool radiate = b.dt_rad > 0.0f;
+    const float cstar = 30.0f, inv_dx = 1.0f / float(g.dx);
+    S *u = b.u, *v = b.v, *w = b.w, *rp = b.rp, *pp = b.pp;
     const std::size_t row = g.at(0, j, k);
-    auto cp_rp = [&](std::size_t d, std::size_t s) { rp[d] = rp[s]; pp[d] = pp[s]; };
+    auto cp_rp = [&](std::size_t d, std::size_t src) { rp[d] = rp[src]; pp[d] = pp[src]; };
     {
         const std::size_t c = row, in = row + 1;
         if (bcw == NIMBUS_BC_PERIODIC)
         {
-            const std::size_t s = row + (NX - 2);
-            u[c] = u[s]; v[c] = v[s]; w[c] = w[s]; cp_rp(c, s);
+            const std::size_t src = row + (NX - 2);
+            u[c] = u[src]; v[c] = v[src]; w[c] = w[src]; cp_rp(c, src);
         }
         else
         {
@@ -545,8 +792,8 @@ __global__ __launch_bounds__(kBlock) void k_bc_west_east(DevGrid g, S* u, S* v, 
             {
                 if (radiate)
                 {
-                    float& up = rad_w[std::size_t(j) * NZ + k];
-                    const float un = kw78_face(up, up - 0.0f - cstar, float(u[in]) - up, dt_rad, inv_dx, true);
+                    float& up = A.f.rad_w[std::size_t(j) * NZ + k];
+                    const float un = kw78_face(up, up - 0.0f - cstar, float(u[in]) - up, b.dt_rad, inv_dx, true);
                     u[c] = S(un);
                     up = un;
                 }
@@ -562,16 +809,16 @@ __global__ __launch_bounds__(kBlock) void k_bc_west_east(DevGrid g, S* u, S* v, 
         const std::size_t gi = row + (NX - 1), a = row + (NX - 2), am = row + (NX - 3);
         if (bce == NIMBUS_BC_PERIODIC)
         {
-            const std::size_t s = row + 1;
-            u[gi] = u[s]; v[gi] = v[s]; w[gi] = w[s]; cp_rp(gi, s);
+            const std::size_t src = row + 1;
+            u[gi] = u[src]; v[gi] = v[src]; w[gi] = w[src]; cp_rp(gi, src);
         }
         else
         {
             if (bce == NIMBUS_BC_RIGID || bce == NIMBUS_BC_RIGID_SLIP) u[a] = S(0);
             if (bce == NIMBUS_BC_RADIATIVE && radiate)
             {
-                float& up = rad_e[std::size_t(j) * NZ + k];
-                const float un = kw78_face(up, up - 0.0f + cstar, up - float(u[am]), dt_rad, inv_dx, false);
+                float& up = A.f.rad_e[std::size_t(j) * NZ + k];
+                const float un = kw78_face(up, up - 0.0f + cstar, up - float(u[am]), b.dt_rad, inv_dx, false);
                 u[a] = S(un);
                 up = un;
             }
@@ -585,23 +832,30 @@ __global__ __launch_bounds__(kBlock) void k_bc_west_east(DevGrid g, S* u, S* v, 
 }
 
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_bc_south_north(DevGrid g, S* u, S* v, S* w, S* rp, S* pp, float* rad_s,
-                                                           float* rad_n, int bcs, int bcn, float dt_rad, float inv_dy)
+__global__ __launch_bounds__(kBlock) void k_bc_south_north(const MemberArgs* __restrict__ ma,
+                                                           const int2* __restrict__ map, int s)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    BcBufs<S> b;
+    if (!bc_buffers<S>(A, s, b)) return;
+    const DevGrid& g = A.g;
+    const int t = wk.local * kBlock + int(threadIdx.x);
     if (t >= g.nx * g.nz) return;
     const int k = t / g.nx, i = t - k * g.nx;
     const int NY = g.ny, NZ = g.nz;
-    const bool radiate = dt_rad > 0.0f;
-    const float cstar = 30.0f;
+    const int bcs = A.p.bc_south, bcn = A.p.bc_north;
+    const bool radiate = b.dt_rad > 0.0f;
+    const float cstar = 30.0f, inv_dy = 1.0f / float(g.dy);
+    S *u = b.u, *v = b.v, *w = b.w, *rp = b.rp, *pp = b.pp;
     const std::size_t sy = std::size_t(g.px);
-    auto cp_rp = [&](std::size_t d, std::size_t s) { rp[d] = rp[s]; pp[d] = pp[s]; };
+    auto cp_rp = [&](std::size_t d, std::size_t src) { rp[d] = rp[src]; pp[d] = pp[src]; };
     {
         const std::size_t c = g.at(i, 0, k), in = c + sy;
         if (bcs == NIMBUS_BC_PERIODIC)
         {
-            const std::size_t s = g.at(i, NY - 2, k);
-            v[c] = v[s]; u[c] = u[s]; w[c] = w[s]; cp_rp(c, s);
+            const std::size_t src = g.at(i, NY - 2, k);
+            v[c] = v[src]; u[c] = u[src]; w[c] = w[src]; cp_rp(c, src);
         }
         else
         {
@@ -609,8 +863,8 @@ __global__ __launch_bounds__(kBlock) void k_bc_south_north(DevGrid g, S* u, S* v
             {
                 if (radiate)
                 {
-                    float& vp = rad_s[std::size_t(i) * NZ + k];
-                    const float vn = kw78_face(vp, vp - 0.0f - cstar, float(v[in]) - vp, dt_rad, inv_dy, true);
+                    float& vp = A.f.rad_s[std::size_t(i) * NZ + k];
+                    const float vn = kw78_face(vp, vp - 0.0f - cstar, float(v[in]) - vp, b.dt_rad, inv_dy, true);
                     v[c] = S(vn);
                     vp = vn;
                 }
@@ -626,16 +880,16 @@ __global__ __launch_bounds__(kBlock) void k_bc_south_north(DevGrid g, S* u, S* v
         const std::size_t gj = g.at(i, NY - 1, k), a = g.at(i, NY - 2, k), am = g.at(i, NY - 3, k);
         if (bcn == NIMBUS_BC_PERIODIC)
         {
-            const std::size_t s = g.at(i, 1, k);
-            v[gj] = v[s]; u[gj] = u[s]; w[gj] = w[s]; cp_rp(gj, s);
+            const std::size_t src = g.at(i, 1, k);
+            v[gj] = v[src]; u[gj] = u[src]; w[gj] = w[src]; cp_rp(gj, src);
         }
         else
         {
             if (bcn == NIMBUS_BC_RIGID || bcn == NIMBUS_BC_RIGID_SLIP) v[a] = S(0);
             if (bcn == NIMBUS_BC_RADIATIVE && radiate)
             {
-                float& vp = rad_n[std::size_t(i) * NZ + k];
-                const float vn = kw78_face(vp, vp - 0.0f + cstar, vp - float(v[am]), dt_rad, inv_dy, false);
+                float& vp = A.f.rad_n[std::size_t(i) * NZ + k];
+                const float vn = kw78_face(vp, vp - 0.0f + cstar, vp - float(v[am]), b.dt_rad, inv_dy, false);
                 v[a] = S(vn);
                 vp = vn;
             }
@@ -649,90 +903,171 @@ __global__ __launch_bounds__(kBlock) void k_bc_south_north(DevGrid g, S* u, S* v
 }
 
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_bc_vertical(DevGrid g, S* u, S* v, S* w, int bottom, int top)
+__global__ __launch_bounds__(kBlock) void k_bc_vertical(const MemberArgs* __restrict__ ma,
+                                                        const int2* __restrict__ map, int s)
 {
-    int i, j;
-    if (!column_index(g, i, j)) return;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    BcBufs<S> b;
+    if (!bc_buffers<S>(A, s, b)) return;
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, 1);
+    if (t.i >= g.nx || t.j >= g.ny) return;
     const int NZ = g.nz;
-    if (bottom == NIMBUS_BOTTOM_NO_SLIP)
+    if (A.p.bc_bottom == NIMBUS_BOTTOM_NO_SLIP)
     {
-        u[g.at(i, j, 0)] = S(0);
-        v[g.at(i, j, 0)] = S(0);
+        b.u[g.at(t.i, t.j, 0)] = S(0);
+        b.v[g.at(t.i, t.j, 0)] = S(0);
     }
-    const std::size_t t = g.at(i, j, NZ - 1), tm = g.at(i, j, NZ - 2);
-    w[t] = top == NIMBUS_TOP_OPEN ? w[tm] : S(0);
-    u[t] = u[tm];
-    v[t] = v[tm];
+    const std::size_t top = g.at(t.i, t.j, NZ - 1), tm = g.at(t.i, t.j, NZ - 2);
+    b.w[top] = A.p.bc_top == NIMBUS_TOP_OPEN ? b.w[tm] : S(0);
+    b.u[top] = b.u[tm];
+    b.v[top] = b.v[tm];
 }
 
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_seed_radiation(DevGrid g, const S* u, const S* v, float* rad_w,
-                                                           float* rad_e, float* rad_s, float* rad_n)
+__global__ __launch_bounds__(kBlock) void k_seed_radiation(const MemberArgs* __restrict__ ma,
+                                                           const int2* __restrict__ map)
 {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const DevGrid& g = A.g;
+    const int t = wk.local * kBlock + int(threadIdx.x);
     const int NZ = g.nz;
+    const S* u = P<S>(A.f.u);
+    const S* v = P<S>(A.f.v);
     if (t < g.ny * NZ)
     {
         const int j = t / NZ, k = t - j * NZ;
-        rad_w[t] = float(u[g.at(0, j, k)]);
-        rad_e[t] = float(u[g.at(g.nx - 2, j, k)]);
+        A.f.rad_w[t] = float(u[g.at(0, j, k)]);
+        A.f.rad_e[t] = float(u[g.at(g.nx - 2, j, k)]);
     }
     if (t < g.nx * NZ)
     {
         const int i = t / NZ, k = t - i * NZ;
-        rad_s[t] = float(v[g.at(i, 0, k)]);
-        rad_n[t] = float(v[g.at(i, g.ny - 2, k)]);
+        A.f.rad_s[t] = float(v[g.at(i, 0, k)]);
+        A.f.rad_n[t] = float(v[g.at(i, g.ny - 2, k)]);
+    }
+}
+
+// Moves u, v, w (explicit mode) and rho' from buffer 1 to buffer 0 when the
+// member's substep count leaves them there (n_done odd; n_done < 0 means the
+// member's own count).
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_commit(const MemberArgs* __restrict__ ma, const int2* __restrict__ map,
+                                                   int n_done)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const int n = n_done < 0 ? A.p.substeps : min(n_done, A.p.substeps);
+    if ((n & 1) == 0) return;
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
+    if (t.i >= g.nx || t.j >= g.ny) return;
+    const DevFields& f = A.f;
+    const bool all = A.p.acoustic_mode == NIMBUS_ACOUSTIC_EXPLICIT;
+    for (int k = t.k0; k < t.k1; ++k)
+    {
+        const std::size_t c = g.at(t.i, t.j, k);
+        P<S>(f.rp)[c] = P<S>(f.rp2)[c];
+        if (all)
+        {
+            P<S>(f.u)[c] = P<S>(f.u2)[c];
+            P<S>(f.v)[c] = P<S>(f.v2)[c];
+            P<S>(f.w)[c] = P<S>(f.w2)[c];
+        }
     }
 }
 
 // ---------------------------------------------------------------------
 // WENO5 retained-flux transport with the PD donor limiter
-// (advection_cartesian.cpp:1941-2650), all scalars of the step in one pass.
+// (advection_cartesian.cpp:1941-2650): face flux pass, limiter pass,
+// conservative update pass, each a column march. Scalars are the member's
+// theta, qv, qc, qr slots; the result goes to the tth .. tqr slots.
 // ---------------------------------------------------------------------
+constexpr int kMaxScalars = 4;
+
 template <class S>
-struct ScalarSet
+__device__ __forceinline__ const S* scalar_in(const DevFields& f, int q)
 {
-    const S* q[4];
-    S* out[4];
-    float* rl[4];
-    int pd[4];
-    int n;
-};
+    return P<S>(q == 0 ? f.th : q == 1 ? f.qv : q == 2 ? f.qc : f.qr);
+}
+template <class S>
+__device__ __forceinline__ S* scalar_out(const DevFields& f, int q)
+{
+    return P<S>(q == 0 ? f.tth : q == 1 ? f.tqv : q == 2 ? f.tqc : f.tqr);
+}
+// Limiter factor slot of scalar q: the PD scalars take rl_qv, rl_qc, rl_qr in order.
+__device__ __forceinline__ float* limiter_slot(const DevFields& f, int pd_mask, int q)
+{
+    int n = 0;
+    for (int r = 0; r < q; ++r) n += (pd_mask >> r) & 1;
+    return n == 0 ? f.rl_qv : n == 1 ? f.rl_qc : f.rl_qr;
+}
 
 // Face flux pass (advect_conservative_scalar_flux): each cell writes the raw
-// retained-flux scalar flux through its own upper x, y and z faces, stored in
-// the storage type like the reference's SFX/SFY/SFZ scratch.
+// flux through its own upper x, y and z faces, stored in the storage type
+// like the reference's SFX/SFY/SFZ scratch. The own column's scalar is held
+// as a window k-2 .. k+3.
 template <class S>
-struct FluxSet
+__global__ __launch_bounds__(kBlock) void k_transport_flux(const MemberArgs* __restrict__ ma,
+                                                           const int2* __restrict__ map, int n_scalars)
 {
-    S* fx[4];
-    S* fy[4];
-    S* fz[4];
-};
-
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_transport_flux(DevGrid g, ScalarSet<S> set, FluxSet<S> fl,
-                                                           const S* __restrict__ mfx, const S* __restrict__ mfy,
-                                                           const S* __restrict__ mfz, int open_top)
-{
-    using A = typename AccT<S>::type;
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
+    using A_ = typename AccT<S>::type;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const std::size_t c = g.at(i, j, k);
-    const std::ptrdiff_t sy = g.px, sz = std::ptrdiff_t(g.px) * g.ny;
-    const A mx = A(mfx[c]), my = A(mfy[c]), mz = A(mfz[c]);
-    for (int s = 0; s < set.n; ++s)
+    const bool valid = t.i < NX && t.j < NY;
+    const int i = min(t.i, NX - 1), j = min(t.j, NY - 1);
+    const DevFields& f = A.f;
+    const std::size_t sz = g.plane();
+    const std::size_t col = g.at(i, j, 0);
+    const bool open_top = A.p.bc_top == NIMBUS_TOP_OPEN;
+    const S* __restrict__ mfx = P<S>(f.mfx);
+    const S* __restrict__ mfy = P<S>(f.mfy);
+    const S* __restrict__ mfz = P<S>(f.mfz);
+    int xo[6], yo[6];
+#pragma unroll
+    for (int o = 0; o < 6; ++o)
     {
-        const S* __restrict__ q = set.q[s];
-        S fx = S(0), fy = S(0), fz = S(0);
-        if (i <= NX - 2) fx = S(dev::face_flux<A>([=](int o) { return A(q[c + std::ptrdiff_t(o)]); }, i, NX, mx));
-        if (j <= NY - 2) fy = S(dev::face_flux<A>([=](int o) { return A(q[c + std::ptrdiff_t(o) * sy]); }, j, NY, my));
-        if (k <= NZ - 2) fz = S(dev::face_flux<A>([=](int o) { return A(q[c + std::ptrdiff_t(o) * sz]); }, k, NZ, mz));
-        else if (open_top) fz = S(mz * A(q[c]));
-        fl.fx[s][c] = fx;
-        fl.fy[s][c] = fy;
-        fl.fz[s][c] = fz;
+        xo[o] = clampi(i + o - 2, 0, NX - 1) - i;
+        yo[o] = (clampi(j + o - 2, 0, NY - 1) - j) * g.px;
+    }
+    for (int q = 0; q < n_scalars; ++q)
+    {
+        const S* __restrict__ qs = scalar_in<S>(f, q);
+        S* __restrict__ fx = P<S>(f.flux[3 * q]);
+        S* __restrict__ fy = P<S>(f.flux[3 * q + 1]);
+        S* __restrict__ fz = P<S>(f.flux[3 * q + 2]);
+        A_ zw[6];
+#pragma unroll
+        for (int o = 0; o < 5; ++o) zw[o + 1] = A_(qs[col + std::size_t(clampi(t.k0 + o - 2, 0, NZ - 1)) * sz]);
+        for (int k = t.k0; k < t.k1; ++k)
+        {
+#pragma unroll
+            for (int o = 0; o < 5; ++o) zw[o] = zw[o + 1];
+            zw[5] = A_(qs[col + std::size_t(clampi(k + 3, 0, NZ - 1)) * sz]);
+            const std::size_t c = col + std::size_t(k) * sz;
+            S sx = S(0), sy_ = S(0), sz_ = S(0);
+            if (i <= NX - 2)
+                sx = S(dev::face_flux6<A_>(A_(qs[c + xo[0]]), A_(qs[c + xo[1]]), zw[2], A_(qs[c + xo[3]]),
+                                           A_(qs[c + xo[4]]), A_(qs[c + xo[5]]), i, NX, A_(mfx[c])));
+            if (j <= NY - 2)
+                sy_ = S(dev::face_flux6<A_>(A_(qs[c + yo[0]]), A_(qs[c + yo[1]]), zw[2], A_(qs[c + yo[3]]),
+                                            A_(qs[c + yo[4]]), A_(qs[c + yo[5]]), j, NY, A_(mfy[c])));
+            if (k <= NZ - 2)
+                sz_ = S(dev::face_flux6<A_>(zw[0], zw[1], zw[2], zw[3], zw[4], zw[5], k, NZ, A_(mfz[c])));
+            else if (open_top)
+                sz_ = S(A_(mfz[c]) * zw[2]);
+            if (valid)
+            {
+                fx[c] = sx;
+                fy[c] = sy_;
+                fz[c] = sz_;
+            }
+        }
     }
 }
 
@@ -745,94 +1080,131 @@ __device__ __forceinline__ float pd_store(double r)
 
 // Positive-definite donor limiter (advect_pd_limiter): r = min(1, (rho_begin q / dt) / outgoing).
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_transport_limiter(DevGrid g, ScalarSet<S> set, FluxSet<S> fl,
-                                                              const S* rbp, const double* base, double dt,
-                                                              int open_top)
+__global__ __launch_bounds__(kBlock) void k_transport_limiter(const MemberArgs* __restrict__ ma,
+                                                              const int2* __restrict__ map, int n_scalars,
+                                                              int pd_mask)
 {
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const std::size_t c = g.at(i, j, k);
-    const std::size_t sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
-    const double rho_b = slot(base, kBaseRho0, NZ)[k] + double(rbp[c]);
-    for (int s = 0; s < set.n; ++s)
+    if (t.i >= NX || t.j >= NY) return;
+    const int i = t.i, j = t.j;
+    const DevFields& f = A.f;
+    const std::size_t sy = std::size_t(g.px), sz = g.plane();
+    const bool open_top = A.p.bc_top == NIMBUS_TOP_OPEN;
+    const double dt = A.p.dt;
+    const double* rho0 = slot(f.base, kBaseRho0, NZ);
+    const S* rbp = P<S>(f.rb);
+    for (int q = 0; q < n_scalars; ++q)
     {
-        if (!set.pd[s]) continue;
-        const S* fx = fl.fx[s];
-        const S* fy = fl.fy[s];
-        const S* fz = fl.fz[s];
-        double out_rate = 0.0;
-        if (i <= NX - 2 && fx[c] > S(0)) out_rate += double(fx[c]) * g.idx;
-        if (i >= 1 && fx[c - 1] < S(0)) out_rate -= double(fx[c - 1]) * g.idx;
-        if (j <= NY - 2 && fy[c] > S(0)) out_rate += double(fy[c]) * g.idy;
-        if (j >= 1 && fy[c - sy] < S(0)) out_rate -= double(fy[c - sy]) * g.idy;
-        if ((k <= NZ - 2 || (open_top && k == NZ - 1)) && fz[c] > S(0)) out_rate += double(fz[c]) * g.idz;
-        if (k >= 1 && fz[c - sz] < S(0)) out_rate -= double(fz[c - sz]) * g.idz;
-        double r = 1.0;
-        if (out_rate > 0.0) r = fmin(1.0, rho_b * fmax(double(set.q[s][c]), 0.0) / dt / out_rate);
-        set.rl[s][c] = pd_store(r);
+        if (!((pd_mask >> q) & 1)) continue;
+        const S* __restrict__ fx = P<S>(f.flux[3 * q]);
+        const S* __restrict__ fy = P<S>(f.flux[3 * q + 1]);
+        const S* __restrict__ fz = P<S>(f.flux[3 * q + 2]);
+        const S* __restrict__ qs = scalar_in<S>(f, q);
+        float* rl = limiter_slot(f, pd_mask, q);
+        S fz_below = t.k0 >= 1 ? fz[g.at(i, j, t.k0 - 1)] : S(0);
+        for (int k = t.k0; k < t.k1; ++k)
+        {
+            const std::size_t c = g.at(i, j, k);
+            const S fzc = fz[c];
+            double out_rate = 0.0;
+            if (i <= NX - 2 && fx[c] > S(0)) out_rate += double(fx[c]) * g.idx;
+            if (i >= 1 && fx[c - 1] < S(0)) out_rate -= double(fx[c - 1]) * g.idx;
+            if (j <= NY - 2 && fy[c] > S(0)) out_rate += double(fy[c]) * g.idy;
+            if (j >= 1 && fy[c - sy] < S(0)) out_rate -= double(fy[c - sy]) * g.idy;
+            if ((k <= NZ - 2 || (open_top && k == NZ - 1)) && fzc > S(0)) out_rate += double(fzc) * g.idz;
+            if (k >= 1 && fz_below < S(0)) out_rate -= double(fz_below) * g.idz;
+            double r = 1.0;
+            const double rho_b = rho0[k] + double(rbp[c]);
+            if (out_rate > 0.0) r = fmin(1.0, rho_b * fmax(double(qs[c]), 0.0) / dt / out_rate);
+            rl[c] = pd_store(r);
+            fz_below = fzc;
+        }
     }
 }
 
 // Conservative update (advect_conservative_update): q = (rho_begin q - dt div F_lim) / rho_end,
-// outgoing-then-incoming for positive-definite scalars.
+// outgoing-then-incoming for positive-definite scalars; ghost columns copy through.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_transport_update(DevGrid g, ScalarSet<S> set, FluxSet<S> fl,
-                                                             const S* rbp, const S* rep, const double* base,
-                                                             double dt, int open_top)
+__global__ __launch_bounds__(kBlock) void k_transport_update(const MemberArgs* __restrict__ ma,
+                                                             const int2* __restrict__ map, int n_scalars,
+                                                             int pd_mask)
 {
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const std::size_t c = g.at(i, j, k);
+    if (t.i >= NX || t.j >= NY) return;
+    const int i = t.i, j = t.j;
+    const DevFields& f = A.f;
+    const std::size_t sy = std::size_t(g.px), sz = g.plane();
     if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
     {
-        for (int s = 0; s < set.n; ++s) set.out[s][c] = set.q[s][c];
+        for (int q = 0; q < n_scalars; ++q)
+        {
+            const S* qs = scalar_in<S>(f, q);
+            S* out = scalar_out<S>(f, q);
+            for (int k = t.k0; k < t.k1; ++k) out[g.at(i, j, k)] = qs[g.at(i, j, k)];
+        }
         return;
     }
-    const std::size_t sy = std::size_t(g.px), sz = std::size_t(g.px) * g.ny;
-    const double r0 = slot(base, kBaseRho0, NZ)[k];
-    const double rho_b = r0 + double(rbp[c]);
-    const double rho_e = r0 + double(rep[c]);
-    const double den = rho_e > 0.05 ? rho_e : 0.05;
-    const bool has_top = k <= NZ - 2 || (open_top && k == NZ - 1);
-    for (int s = 0; s < set.n; ++s)
+    const bool open_top = A.p.bc_top == NIMBUS_TOP_OPEN;
+    const double dt = A.p.dt;
+    const double* rho0 = slot(f.base, kBaseRho0, NZ);
+    const S* rbp = P<S>(f.rb);
+    const S* rep = P<S>(f.rp);
+    const double ix = g.idx, iy = g.idy, iz = g.idz;
+    for (int q = 0; q < n_scalars; ++q)
     {
-        const S* fx = fl.fx[s];
-        const S* fy = fl.fy[s];
-        const S* fz = fl.fz[s];
-        const float* rl = set.rl[s];
-        const bool pd = set.pd[s] != 0;
+        const S* __restrict__ fx = P<S>(f.flux[3 * q]);
+        const S* __restrict__ fy = P<S>(f.flux[3 * q + 1]);
+        const S* __restrict__ fz = P<S>(f.flux[3 * q + 2]);
+        const S* __restrict__ qs = scalar_in<S>(f, q);
+        S* __restrict__ out = scalar_out<S>(f, q);
+        const bool pd = (pd_mask >> q) & 1;
+        const float* rl = pd ? limiter_slot(f, pd_mask, q) : nullptr;
         auto lim = [&](S raw, std::size_t lo, std::size_t hi) -> double {
             if (!pd || raw == S(0)) return double(raw);
             return double(raw) * double(S(raw > S(0) ? rl[lo] : rl[hi]));
         };
-        const double sxp = lim(fx[c], c, c + 1);
-        const double sxm = lim(fx[c - 1], c - 1, c);
-        const double syp = lim(fy[c], c, c + sy);
-        const double sym = lim(fy[c - sy], c - sy, c);
-        const double szp = has_top ? lim(fz[c], c, k < NZ - 1 ? c + sz : c) : 0.0;
-        const double szm = k >= 1 ? lim(fz[c - sz], c - sz, c) : 0.0;
-        const double q0 = double(set.q[s][c]);
-        double rho_phi;
-        if (pd)
+        for (int k = t.k0; k < t.k1; ++k)
         {
-            double outg = 0.0, inc = 0.0;
-            const double ix = g.idx, iy = g.idy, iz = g.idz;
-            if (sxp >= 0.0) outg += sxp * ix; else inc -= sxp * ix;
-            if (sxm <= 0.0) outg -= sxm * ix; else inc += sxm * ix;
-            if (syp >= 0.0) outg += syp * iy; else inc -= syp * iy;
-            if (sym <= 0.0) outg -= sym * iy; else inc += sym * iy;
-            if (has_top) { if (szp >= 0.0) outg += szp * iz; else inc -= szp * iz; }
-            if (k >= 1) { if (szm <= 0.0) outg -= szm * iz; else inc += szm * iz; }
-            rho_phi = (rho_b * q0 - dt * outg) + dt * inc;
+            const std::size_t c = g.at(i, j, k);
+            const double r0 = rho0[k];
+            const double rho_b = r0 + double(rbp[c]);
+            const double rho_e = r0 + double(rep[c]);
+            const double den = rho_e > 0.05 ? rho_e : 0.05;
+            const bool has_top = k <= NZ - 2 || (open_top && k == NZ - 1);
+            const double sxp = lim(fx[c], c, c + 1);
+            const double sxm = lim(fx[c - 1], c - 1, c);
+            const double syp = lim(fy[c], c, c + sy);
+            const double sym = lim(fy[c - sy], c - sy, c);
+            const double szp = has_top ? lim(fz[c], c, k < NZ - 1 ? c + sz : c) : 0.0;
+            const double szm = k >= 1 ? lim(fz[c - sz], c - sz, c) : 0.0;
+            const double q0 = double(qs[c]);
+            double rho_phi;
+            if (pd)
+            {
+                double outg = 0.0, inc = 0.0;
+                if (sxp >= 0.0) outg += sxp * ix; else inc -= sxp * ix;
+                if (sxm <= 0.0) outg -= sxm * ix; else inc += sxm * ix;
+                if (syp >= 0.0) outg += syp * iy; else inc -= syp * iy;
+                if (sym <= 0.0) outg -= sym * iy; else inc += sym * iy;
+                if (has_top) { if (szp >= 0.0) outg += szp * iz; else inc -= szp * iz; }
+                if (k >= 1) { if (szm <= 0.0) outg -= szm * iz; else inc += szm * iz; }
+                rho_phi = (rho_b * q0 - dt * outg) + dt * inc;
+            }
+            else
+            {
+                const double div = (sxp - sxm) * ix + (syp - sym) * iy + (szp - szm) * iz;
+                rho_phi = rho_b * q0 - dt * div;
+            }
+            out[c] = S(rho_phi / den);
         }
-        else
-        {
-            const double div = (sxp - sxm) * g.idx + (syp - sym) * g.idy + (szp - szm) * g.idz;
-            rho_phi = rho_b * q0 - dt * div;
-        }
-        set.out[s][c] = S(rho_phi / den);
     }
 }
 
@@ -850,8 +1222,8 @@ struct KesslerOut
 };
 
 template <class S>
-__device__ KesslerOut<S> kessler_cell(const DevGrid& g, const KesslerConst& kc, double dt, double p_full,
-                                      double rho_c, double rho_up, S theta, S qv, S qc, S qr, S qr_up, bool has_up)
+__device__ KesslerOut<S> kessler_cell(double dz, const KesslerConst& kc, double dt, double p_full, double rho_c,
+                                      double rho_up, S theta, S qv, S qc, S qr, S qr_up, bool has_up)
 {
     using A = typename AccT<S>::type;
     const A L = A(kLv / kCp);
@@ -898,7 +1270,7 @@ __device__ KesslerOut<S> kessler_cell(const DevGrid& g, const KesslerConst& kc, 
             o.dth = S(A(o.dth) - L * ev);
         }
     }
-    const A dzA = A(g.dz);
+    const A dzA = A(dz);
     const A qr_own = fmax(A(0), A(qr));
     if (qr_own > A(0))
     {
@@ -922,102 +1294,165 @@ __device__ KesslerOut<S> kessler_cell(const DevGrid& g, const KesslerConst& kc, 
 }
 
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_kessler_tendencies(DevGrid g, const S* th, const S* qv, const S* qc,
-                                                               const S* qr, const S* rp, const S* pp,
-                                                               const double* base, KesslerConst kc, double dt,
-                                                               S* dth, S* dqv, S* dqc, S* dqr)
+__global__ __launch_bounds__(kBlock) void k_kessler_tendencies(const MemberArgs* __restrict__ ma,
+                                                               const int2* __restrict__ map, double dt, S* dth,
+                                                               S* dqv, S* dqc, S* dqr)
 {
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
-    const std::size_t c = g.at(i, j, k);
-    const int NZ = g.nz;
-    if (!(i >= 1 && i < g.nx - 1 && j >= 1 && j < g.ny - 1))
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    if (t.i >= NX || t.j >= NY) return;
+    const int i = t.i, j = t.j;
+    const DevFields& f = A.f;
+    const double* rho0 = slot(f.base, kBaseRho0, NZ);
+    const double* p0 = slot(f.base, kBaseP0, NZ);
+    const S *th = P<S>(f.th), *qv = P<S>(f.qv), *qc = P<S>(f.qc), *qr = P<S>(f.qr), *rp = P<S>(f.rp),
+            *pp = P<S>(f.pp);
+    const std::size_t sz = g.plane();
+    for (int k = t.k0; k < t.k1; ++k)
     {
-        dth[c] = S(0); dqv[c] = S(0); dqc[c] = S(0); dqr[c] = S(0);
-        return;
+        const std::size_t c = g.at(i, j, k);
+        if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
+        {
+            dth[c] = S(0); dqv[c] = S(0); dqc[c] = S(0); dqr[c] = S(0);
+            continue;
+        }
+        const bool up = k + 1 < NZ;
+        const KesslerOut<S> o =
+            kessler_cell<S>(g.dz, A.p.kessler, dt, p0[k] + double(pp[c]), rho0[k] + double(rp[c]),
+                            up ? rho0[k + 1] + double(rp[c + sz]) : 0.0, th[c], qv[c], qc[c], qr[c],
+                            up ? qr[c + sz] : S(0), up);
+        dth[c] = o.dth; dqv[c] = o.dqv; dqc[c] = o.dqc; dqr[c] = o.dqr;
     }
-    const double* rho0 = slot(base, kBaseRho0, NZ);
-    const double* p0 = slot(base, kBaseP0, NZ);
-    const std::size_t sz = std::size_t(g.px) * g.ny;
-    const bool up = k + 1 < NZ;
-    const KesslerOut<S> o = kessler_cell<S>(g, kc, dt, p0[k] + double(pp[c]), rho0[k] + double(rp[c]),
-                                            up ? rho0[k + 1] + double(rp[c + sz]) : 0.0, th[c], qv[c], qc[c], qr[c],
-                                            up ? qr[c + sz] : S(0), up);
-    dth[c] = o.dth; dqv[c] = o.dqv; dqc[c] = o.dqc; dqr[c] = o.dqr;
 }
 
 // Fused microphysics commit: Kessler tendencies, bounded apply
 // (microphysics_step.cpp:1090-1140), surface rain ledger
 // (microphysics_step.cpp:1917-2024; kessler.cpp:533-561) and the EOS
-// pressure projection of every cell (eos_apply.cpp:50-110).
+// pressure projection of every cell (eos_apply.cpp:50-110). Subcycle n reads
+// the transported slots (n even) or the home slots (n odd) and writes the
+// other set; a column marches downward so the level above is still the
+// subcycle's input.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_kessler_apply(DevGrid g, const S* th_in, const S* qv_in, const S* qc_in,
-                                                          const S* qr_in, S* th, S* qv, S* qc, S* qr, const S* rp,
-                                                          S* pp, const double* base, KesslerConst kc, double dt,
-                                                          double* rain, Bounds bd, int do_eos)
+__global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __restrict__ ma,
+                                                          const int2* __restrict__ map, int n)
 {
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
-    const std::size_t c = g.at(i, j, k);
-    const int NZ = g.nz;
-    const double* rho0 = slot(base, kBaseRho0, NZ);
-    const double* p0 = slot(base, kBaseP0, NZ);
-    const std::size_t sz = std::size_t(g.px) * g.ny;
-    const double rho = rho0[k] + double(rp[c]);
-    S thn = th_in[c], qvn = qv_in[c], qcn = qc_in[c], qrn = qr_in[c];
-    if (i >= 1 && i < g.nx - 1 && j >= 1 && j < g.ny - 1)
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (n >= A.p.kessler_sub) return;
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    if (t.i >= NX || t.j >= NY) return;
+    const int i = t.i, j = t.j;
+    const DevFields& f = A.f;
+    const bool from_temp = (n & 1) == 0;
+    const S* th_in = P<S>(from_temp ? f.tth : f.th);
+    const S* qv_in = P<S>(from_temp ? f.tqv : f.qv);
+    const S* qc_in = P<S>(from_temp ? f.tqc : f.qc);
+    const S* qr_in = P<S>(from_temp ? f.tqr : f.qr);
+    S* th = P<S>(from_temp ? f.th : f.tth);
+    S* qv = P<S>(from_temp ? f.qv : f.tqv);
+    S* qc = P<S>(from_temp ? f.qc : f.tqc);
+    S* qr = P<S>(from_temp ? f.qr : f.tqr);
+    const S* rp = P<S>(f.rp);
+    S* pp = P<S>(f.pp);
+    const double* rho0 = slot(f.base, kBaseRho0, NZ);
+    const double* p0 = slot(f.base, kBaseP0, NZ);
+    const KesslerConst kc = A.p.kessler;
+    const Bounds bd = A.p.bounds;
+    const double h = A.p.dt / A.p.kessler_sub;
+    const bool do_eos = n == A.p.kessler_sub - 1;
+    const bool phys = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
+    const std::size_t col = g.at(i, j, 0), sz = g.plane();
+    // level above the chunk (input values of this subcycle)
+    S qr_up = t.k1 < NZ ? qr_in[col + std::size_t(t.k1) * sz] : S(0);
+    double rho_up = t.k1 < NZ ? rho0[t.k1] + double(rp[col + std::size_t(t.k1) * sz]) : 0.0;
+    for (int k = t.k1 - 1; k >= t.k0; --k)
     {
-        const bool up = k + 1 < NZ;
-        const KesslerOut<S> o = kessler_cell<S>(g, kc, dt, p0[k] + double(pp[c]), rho,
-                                                up ? rho0[k + 1] + double(rp[c + sz]) : 0.0, thn, qvn, qcn, qrn,
-                                                up ? qr_in[c + sz] : S(0), up);
-        if (k == 0)
+        const std::size_t c = col + std::size_t(k) * sz;
+        const double rho = rho0[k] + double(rp[c]);
+        S thn = th_in[c], qvn = qv_in[c], qcn = qc_in[c], qrn = qr_in[c];
+        const S qr_here = qrn;
+        if (phys)
         {
-            const double q0 = double(qrn);
-            if (q0 > 0.0)
+            const bool up = k + 1 < NZ;
+            const KesslerOut<S> o = kessler_cell<S>(g.dz, kc, h, p0[k] + double(pp[c]), rho, up ? rho_up : 0.0,
+                                                    thn, qvn, qcn, qrn, up ? qr_up : S(0), up);
+            if (k == 0)
             {
-                double Vt = fmin(double(dev::kessler_rain_speed<double>(q0, rho)) * sqrt(1.2 / rho), kc.vt_max);
-                const double sink = double(S(S(Vt) * qrn)) / g.dz;
-                rain[std::size_t(j) * g.nx + i] += sink * rho * g.dz * dt;
+                const double q0 = double(qrn);
+                if (q0 > 0.0)
+                {
+                    double Vt = fmin(double(dev::kessler_rain_speed<double>(q0, rho)) * sqrt(1.2 / rho), kc.vt_max);
+                    const double sink = double(S(S(Vt) * qrn)) / g.dz;
+                    f.rain[std::size_t(j) * NX + i] += sink * rho * g.dz * h;
+                }
             }
+            const double step = fmin(fmax(double(o.dth) * h, -50.0), 50.0);
+            thn = S(clamp_store<S>(double(thn) + step, 300.0, 200.0f, 700.0f));
+            qvn = S(clamp_store<S>(double(qvn) + double(o.dqv) * h, 0.0, 0.0f, 0.05f));
+            qcn = S(clamp_store<S>(double(qcn) + double(o.dqc) * h, 0.0, 0.0f, 0.05f));
+            qrn = S(clamp_store<S>(double(qrn) + double(o.dqr) * h, 0.0, 0.0f, 0.05f));
         }
-        const double step = fmin(fmax(double(o.dth) * dt, -50.0), 50.0);
-        thn = S(clamp_store<S>(double(thn) + step, 300.0, 200.0f, 700.0f));
-        qvn = S(clamp_store<S>(double(qvn) + double(o.dqv) * dt, 0.0, 0.0f, 0.05f));
-        qcn = S(clamp_store<S>(double(qcn) + double(o.dqc) * dt, 0.0, 0.0f, 0.05f));
-        qrn = S(clamp_store<S>(double(qrn) + double(o.dqr) * dt, 0.0, 0.0f, 0.05f));
-    }
-    th[c] = thn; qv[c] = qvn; qc[c] = qcn; qr[c] = qrn;
-    const double thv = double(thn);
-    if (do_eos && rho > 0.0 && thv > 0.0)
-    {
-        const double pd = dev::eos_pressure(rho, thv);
-        if (isfinite(pd) && pd > 0.0) pp[c] = S(clamp_store<S>(pd, 101325.0, bd.p_min, bd.p_max) - p0[k]);
+        th[c] = thn; qv[c] = qvn; qc[c] = qcn; qr[c] = qrn;
+        const double thv = double(thn);
+        if (do_eos && rho > 0.0 && thv > 0.0)
+        {
+            const double pd = dev::eos_pressure(rho, thv);
+            if (isfinite(pd) && pd > 0.0) pp[c] = S(clamp_store<S>(pd, 101325.0, bd.p_min, bd.p_max) - p0[k]);
+        }
+        qr_up = qr_here;
+        rho_up = rho;
     }
 }
 
+// Copies the transported-slot result home after an even number of subcycles.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_copy(std::size_t n, const S* a, S* b)
+__global__ __launch_bounds__(kBlock) void k_scalars_home(const MemberArgs* __restrict__ ma,
+                                                         const int2* __restrict__ map)
 {
-    const std::size_t t = std::size_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (t < n) b[t] = a[t];
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (A.p.kessler_sub & 1) return;
+    const DevGrid& g = A.g;
+    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
+    if (t.i >= g.nx || t.j >= g.ny) return;
+    const DevFields& f = A.f;
+    for (int k = t.k0; k < t.k1; ++k)
+    {
+        const std::size_t c = g.at(t.i, t.j, k);
+        P<S>(f.th)[c] = P<S>(f.tth)[c];
+        P<S>(f.qv)[c] = P<S>(f.tqv)[c];
+        P<S>(f.qc)[c] = P<S>(f.tqc)[c];
+        P<S>(f.qr)[c] = P<S>(f.tqr)[c];
+    }
 }
 
 // ---------------------------------------------------------------------
-// Deterministic diagnostics: fixed-shape block partials, then one block
-// reduces them in a fixed order (so a member's numbers do not depend on
-// what else runs on the device).
+// Deterministic diagnostics: kDiagBlocksPerMember fixed-shape partials per
+// member, then one workgroup per member reduces them in a fixed order, so a
+// member's numbers do not depend on what else is in the batch.
 // ---------------------------------------------------------------------
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_diag_partials(DevGrid g, const S* w, const S* th, const S* qc,
-                                                          const S* qr, const double* rain, const double* base,
-                                                          double* partials)
+__global__ __launch_bounds__(kBlock) void k_diag_partials(const MemberArgs* __restrict__ ma,
+                                                          const int2* __restrict__ map)
 {
     __shared__ double sm[kDiagValues][kBlock];
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const DevGrid& g = A.g;
+    const DevFields& f = A.f;
+    const S* w = P<S>(f.w);
+    const S* th = P<S>(f.th);
+    const S* qc = P<S>(f.qc);
+    const S* qr = P<S>(f.qr);
     double v[kDiagValues] = {-1e300, 1e300, 0.0, 0.0, 0.0, 0.0, -1e300, 0.0};
     const std::size_t ncell = std::size_t(g.nx) * g.ny * g.nz;
-    const std::size_t stride = std::size_t(gridDim.x) * kBlock;
-    for (std::size_t t = std::size_t(blockIdx.x) * kBlock + threadIdx.x; t < ncell; t += stride)
+    const std::size_t stride = std::size_t(kDiagBlocksPerMember) * kBlock;
+    for (std::size_t t = std::size_t(wk.local) * kBlock + threadIdx.x; t < ncell; t += stride)
     {
         const int k = int(t / (std::size_t(g.nx) * g.ny));
         const int rem = int(t - std::size_t(k) * g.nx * g.ny);
@@ -1027,13 +1462,13 @@ __global__ __launch_bounds__(kBlock) void k_diag_partials(DevGrid g, const S* w,
         const double wv = double(w[c]);
         v[0] = fmax(v[0], wv);
         v[1] = fmin(v[1], wv);
-        const double tp = double(th[c]) - slot(base, kBaseTh0, g.nz)[k];
+        const double tp = double(th[c]) - slot(f.base, kBaseTh0, g.nz)[k];
         v[2] += tp * tp;
         v[4] += double(qc[c]);
         v[5] += double(qr[c]);
         if (k == 0)
         {
-            const double r = rain[std::size_t(j) * g.nx + i];
+            const double r = f.rain[std::size_t(j) * g.nx + i];
             v[3] += r;
             v[6] = fmax(v[6], r);
         }
@@ -1056,16 +1491,18 @@ __global__ __launch_bounds__(kBlock) void k_diag_partials(DevGrid g, const S* w,
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < kDiagValues; ++q) partials[std::size_t(blockIdx.x) * kDiagValues + q] = sm[q][0];
+        for (int q = 0; q < kDiagValues; ++q) f.diag[std::size_t(wk.local) * kDiagValues + q] = sm[q][0];
 }
 
-__global__ void k_diag_final(const double* partials, int n, double* out)
+// One workgroup per member (blockIdx.x = member index).
+__global__ void k_diag_final(const MemberArgs* __restrict__ ma, int n_members)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (threadIdx.x != 0 || int(blockIdx.x) >= n_members) return;
+    double* d = ma[blockIdx.x].f.diag;
     double v[kDiagValues] = {-1e300, 1e300, 0.0, 0.0, 0.0, 0.0, -1e300, 0.0};
-    for (int b = 0; b < n; ++b)
+    for (int b = 0; b < kDiagBlocksPerMember; ++b)
     {
-        const double* p = partials + std::size_t(b) * kDiagValues;
+        const double* p = d + std::size_t(b) * kDiagValues;
         v[0] = fmax(v[0], p[0]);
         v[1] = fmin(v[1], p[1]);
         v[6] = fmax(v[6], p[6]);
@@ -1075,36 +1512,65 @@ __global__ void k_diag_final(const double* partials, int n, double* out)
         v[5] += p[5];
         v[7] += p[7];
     }
-    for (int q = 0; q < kDiagValues; ++q) out[q] = v[q];
+    for (int q = 0; q < kDiagValues; ++q) d[std::size_t(kDiagBlocksPerMember) * kDiagValues + q] = v[q];
 }
 
 // Host layout [i][j][k] <-> device [k][j][i], subtracting / adding a base slot.
+// Grid: (ceil(nx / 64), ny, nz) workgroups of 64 threads.
 template <class H, class S>
-__global__ __launch_bounds__(kBlock) void k_host_to_dev(DevGrid g, const H* src, S* dst, const double* base_slot)
+__global__ __launch_bounds__(64) void k_host_to_dev(DevGrid g, const H* src, S* dst, const double* base_slot)
 {
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
+    const int i = int(blockIdx.x) * 64 + int(threadIdx.x), j = int(blockIdx.y), k = int(blockIdx.z);
+    if (i >= g.nx) return;
     const std::size_t h = (std::size_t(i) * g.ny + j) * g.nz + k;
     dst[g.at(i, j, k)] = S(double(src[h]) - (base_slot ? base_slot[k] : 0.0));
 }
 
 template <class S, class H>
-__global__ __launch_bounds__(kBlock) void k_dev_to_host(DevGrid g, const S* src, H* dst, const double* base_slot)
+__global__ __launch_bounds__(64) void k_dev_to_host(DevGrid g, const S* src, H* dst, const double* base_slot)
 {
-    int i, j, k;
-    if (!cell_index(g, i, j, k)) return;
+    const int i = int(blockIdx.x) * 64 + int(threadIdx.x), j = int(blockIdx.y), k = int(blockIdx.z);
+    if (i >= g.nx) return;
     const std::size_t h = (std::size_t(i) * g.ny + j) * g.nz + k;
     dst[h] = H(double(src[g.at(i, j, k)]) + (base_slot ? base_slot[k] : 0.0));
 }
 
-dim3 cell_grid(const DevGrid& g) { return dim3(unsigned(g.tiles_plane * g.nz)); }
-dim3 column_grid(const DevGrid& g) { return dim3(unsigned(g.tiles_plane)); }
-dim3 linear_grid(std::size_t n) { return dim3(unsigned((n + kBlock - 1) / kBlock)); }
-
-template <class S>
-S* P(void* p) { return static_cast<S*>(p); }
-
 }  // namespace
+
+// ---------------------------------------------------------------------
+// Host side: geometry, launch helpers and launchers
+// ---------------------------------------------------------------------
+void fill_geometry(MemberArgs& a, int tile_w, int k_chunks, int xcd_remap)
+{
+    MemberGeom& G = a.geo;
+    G.tile_w = tile_w == 32 ? 32 : 64;
+    G.tile_h = kBlock / G.tile_w;
+    G.tiles_x = (a.g.nx + G.tile_w - 1) / G.tile_w;
+    G.tiles_y = (a.g.ny + G.tile_h - 1) / G.tile_h;
+    G.strips_x = (a.g.nx + 62) / 63;
+    G.strips_y = (a.g.ny + 3) / 4;
+    G.k_chunks = std::max(1, std::min(k_chunks, a.g.nz));
+    G.xcd_remap = xcd_remap ? 1 : 0;
+}
+
+int work_items(const MemberArgs& a, WorkKind kind)
+{
+    const MemberGeom& G = a.geo;
+    const int tiles = G.tiles_x * G.tiles_y;
+    const auto lin = [](std::size_t n) { return int((n + kBlock - 1) / kBlock); };
+    switch (kind)
+    {
+    case W_PLANE: return tiles * a.g.nz;
+    case W_CHUNK: return tiles * G.k_chunks;
+    case W_COLUMN: return tiles;
+    case W_STRIP: return G.strips_x * G.strips_y * G.k_chunks;
+    case W_WE: return lin(std::size_t(a.g.ny) * a.g.nz);
+    case W_SN: return lin(std::size_t(a.g.nx) * a.g.nz);
+    case W_SEED: return lin(std::size_t(std::max(a.g.nx, a.g.ny)) * a.g.nz);
+    case W_DIAG: return kDiagBlocksPerMember;
+    default: return 0;
+    }
+}
 
 hipEvent_t Launcher::take_event()
 {
@@ -1125,7 +1591,7 @@ void Launcher::begin(const char* name, hipEvent_t& a)
     a = nullptr;
     if (!timing) return;
     a = take_event();
-    if (a) hipEventRecord(a, stream);
+    if (a && hipEventRecord(a, stream) != hipSuccess && error == hipSuccess) error = hipErrorUnknown;
 }
 
 void Launcher::end(const char* name, hipEvent_t a)
@@ -1136,201 +1602,159 @@ void Launcher::end(const char* name, hipEvent_t a)
     if (!timing || !a) return;
     hipEvent_t b = take_event();
     if (!b) return;
-    hipEventRecord(b, stream);
+    if (hipEventRecord(b, stream) != hipSuccess && error == hipSuccess) error = hipErrorUnknown;
     if (pending) pending->push_back(TimedLaunch{name, a, b});
 }
 
-#define NIMBUS_LAUNCH(L, name, kernel, grid, ...)                               \
-    do                                                                          \
-    {                                                                           \
-        hipEvent_t ev_;                                                         \
-        (L).begin(name, ev_);                                                   \
-        hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, (L).stream, __VA_ARGS__); \
-        (L).end(name, ev_);                                                     \
-    } while (0)
-
-template <class S>
-void launch_slow_rk2(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p)
+namespace
 {
-    NIMBUS_LAUNCH(L, "slow_stage1", (k_slow_stage1<S>), cell_grid(g), g, P<S>(f.u), P<S>(f.v), P<S>(f.w),
-                  P<S>(f.rp), P<S>(f.qv), P<S>(f.qc), P<S>(f.qr), (const double*)f.base, p.coriolis_f, p.dt,
-                  P<S>(f.k1u), P<S>(f.k1v), P<S>(f.k1w), P<S>(f.su), P<S>(f.sv), P<S>(f.sw));
-    NIMBUS_LAUNCH(L, "slow_stage2", (k_slow_stage2<S>), cell_grid(g), g, P<S>(f.u), P<S>(f.v), P<S>(f.w),
-                  P<S>(f.rp), P<S>(f.qv), P<S>(f.qc), P<S>(f.qr), (const double*)f.base, p.coriolis_f, p.dt,
-                  P<S>(f.k1u), P<S>(f.k1v), P<S>(f.k1w), P<S>(f.su), P<S>(f.sv), P<S>(f.sw));
+
+template <class K, class... Args>
+void launch_map(Launcher& L, const char* name, K kernel, const BatchView& b, int subset, WorkKind kind,
+                Args... args)
+{
+    const BlockMap& m = b.maps[subset][kind];
+    if (m.blocks <= 0) return;
+    hipEvent_t ev;
+    L.begin(name, ev);
+    hipLaunchKernelGGL(kernel, dim3(unsigned(m.blocks)), dim3(kBlock), 0, L.stream, b.args, m.entries, args...);
+    L.end(name, ev);
 }
 
 template <class S>
-void launch_slow_tendencies(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, void* du,
-                            void* dv, void* dw)
+S* PS(void* p)
 {
-    NIMBUS_LAUNCH(L, "slow_tendencies", (k_slow_tendencies<S>), cell_grid(g), g, P<S>(f.u), P<S>(f.v), P<S>(f.w),
-                  P<S>(f.rp), P<S>(f.qv), P<S>(f.qc), P<S>(f.qr), (const double*)f.base, p.coriolis_f, P<S>(du),
-                  P<S>(dv), P<S>(dw));
+    return static_cast<S*>(p);
+}
+
+}  // namespace
+
+template <class S>
+void launch_slow_rk2(Launcher& L, const BatchView& b)
+{
+    launch_map(L, "slow_stage1", k_slow<S, 0>, b, S_ALL, W_CHUNK, (S*)nullptr, (S*)nullptr, (S*)nullptr);
+    launch_map(L, "slow_stage2", k_slow<S, 1>, b, S_ALL, W_CHUNK, (S*)nullptr, (S*)nullptr, (S*)nullptr);
 }
 
 template <class S>
-void launch_boundaries(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, double dt_rad)
+void launch_slow_tendencies(Launcher& L, const BatchView& b, void* du, void* dv, void* dw)
 {
-    const float dtf = float(dt_rad);
-    NIMBUS_LAUNCH(L, "bc_west_east", (k_bc_west_east<S>), linear_grid(std::size_t(g.ny) * g.nz), g, P<S>(f.u),
-                  P<S>(f.v), P<S>(f.w), P<S>(f.rp), P<S>(f.pp), f.rad_w, f.rad_e, p.bc_west, p.bc_east, dtf,
-                  1.0f / float(g.dx));
-    NIMBUS_LAUNCH(L, "bc_south_north", (k_bc_south_north<S>), linear_grid(std::size_t(g.nx) * g.nz), g, P<S>(f.u),
-                  P<S>(f.v), P<S>(f.w), P<S>(f.rp), P<S>(f.pp), f.rad_s, f.rad_n, p.bc_south, p.bc_north, dtf,
-                  1.0f / float(g.dy));
-    NIMBUS_LAUNCH(L, "bc_vertical", (k_bc_vertical<S>), column_grid(g), g, P<S>(f.u), P<S>(f.v), P<S>(f.w),
-                  p.bc_bottom, p.bc_top);
+    launch_map(L, "slow_tendencies", k_slow<S, 2>, b, S_ALL, W_CHUNK, PS<S>(du), PS<S>(dv), PS<S>(dw));
 }
 
 template <class S>
-void launch_seed_radiation(Launcher& L, const DevGrid& g, const DevFields& f)
+void launch_acoustic_substep(Launcher& L, const BatchView& b, int s, bool accumulate, bool init_at_first)
 {
-    NIMBUS_LAUNCH(L, "seed_radiation", (k_seed_radiation<S>), linear_grid(std::size_t(std::max(g.nx, g.ny)) * g.nz),
-                  g, P<S>(f.u), P<S>(f.v), f.rad_w, f.rad_e, f.rad_s, f.rad_n);
+    launch_map(L, "acoustic_substep", k_acoustic_fused<S>, b, S_EXPLICIT, W_STRIP, s, int(accumulate),
+               int(init_at_first));
+    launch_map(L, "acoustic_hevi_column", k_hevi_column<S>, b, S_HEVI, W_COLUMN, s, int(accumulate),
+               int(init_at_first));
+    launch_map(L, "acoustic_hevi_momentum", k_hevi_momentum<S>, b, S_HEVI, W_CHUNK, s);
 }
 
 template <class S>
-void launch_acoustic_substep(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, double dts,
-                             double weight, bool first, bool accumulate)
+void launch_boundaries(Launcher& L, const BatchView& b, int s)
 {
-    if (p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI)
+    launch_map(L, "bc_west_east", k_bc_west_east<S>, b, S_ALL, W_WE, s);
+    launch_map(L, "bc_south_north", k_bc_south_north<S>, b, S_ALL, W_SN, s);
+    launch_map(L, "bc_vertical", k_bc_vertical<S>, b, S_ALL, W_COLUMN, s);
+}
+
+template <class S>
+void launch_seed_radiation(Launcher& L, const BatchView& b)
+{
+    launch_map(L, "seed_radiation", k_seed_radiation<S>, b, S_ALL, W_SEED);
+}
+
+template <class S>
+void launch_commit(Launcher& L, const BatchView& b, int n_done)
+{
+    launch_map(L, "acoustic_commit", k_commit<S>, b, S_ALL, W_CHUNK, n_done);
+}
+
+template <class S>
+void launch_transport(Launcher& L, const BatchView& b, int n_scalars, const int* pd, bool into_temp)
+{
+    (void)into_temp;
+    int mask = 0;
+    for (int q = 0; q < n_scalars; ++q) mask |= (pd[q] ? 1 : 0) << q;
+    launch_map(L, "transport_flux", k_transport_flux<S>, b, S_ALL, W_CHUNK, n_scalars);
+    if (mask) launch_map(L, "transport_limiter", k_transport_limiter<S>, b, S_ALL, W_CHUNK, n_scalars, mask);
+    launch_map(L, "transport_update", k_transport_update<S>, b, S_ALL, W_CHUNK, n_scalars, mask);
+}
+
+template <class S>
+void launch_kessler_tendencies(Launcher& L, const BatchView& b, double dt, void* dth, void* dqv, void* dqc,
+                               void* dqr)
+{
+    launch_map(L, "kessler_tendencies", k_kessler_tendencies<S>, b, S_ALL, W_CHUNK, dt, PS<S>(dth), PS<S>(dqv),
+               PS<S>(dqc), PS<S>(dqr));
+}
+
+template <class S>
+void launch_large_step(Launcher& L, const BatchView& b)
+{
+    launch_slow_rk2<S>(L, b);
+    launch_boundaries<S>(L, b, -1);
+    launch_seed_radiation<S>(L, b);
+    for (int s = 0; s < b.max_substeps; ++s)
     {
-        const double alpha = 0.5 * (1.0 + p.hevi_beta);
-        NIMBUS_LAUNCH(L, "acoustic_hevi_column", (k_hevi_column<S>), column_grid(g), g, P<S>(f.u), P<S>(f.v),
-                      P<S>(f.w), P<S>(f.rp), P<S>(f.ralt), P<S>(f.pp), P<S>(f.th), P<S>(f.mfx), P<S>(f.mfy),
-                      P<S>(f.mfz), P<S>(f.rb), f.tri_c, f.tri_r, (const double*)f.base, p.bounds, dts, alpha, weight,
-                      int(first), int(accumulate));
-        NIMBUS_LAUNCH(L, "acoustic_backward", (k_acoustic_backward<S>), cell_grid(g), g, P<S>(f.u), P<S>(f.v),
-                      P<S>(f.w), P<S>(f.ralt), P<S>(f.rp), P<S>(f.pp), (const double*)f.base, p.bounds, dts, 1);
-        return;
+        launch_acoustic_substep<S>(L, b, s, true, true);
+        launch_boundaries<S>(L, b, s);
     }
-    NIMBUS_LAUNCH(L, "acoustic_forward", (k_acoustic_forward<S>), cell_grid(g), g, P<S>(f.u), P<S>(f.v), P<S>(f.w),
-                  P<S>(f.rp), P<S>(f.ralt), P<S>(f.pp), P<S>(f.th), P<S>(f.mfx), P<S>(f.mfy), P<S>(f.mfz),
-                  P<S>(f.rb), (const double*)f.base, p.bounds, dts, weight, int(first), int(accumulate),
-                  int(p.bc_top == NIMBUS_TOP_OPEN));
-    NIMBUS_LAUNCH(L, "acoustic_backward", (k_acoustic_backward<S>), cell_grid(g), g, P<S>(f.u), P<S>(f.v),
-                  P<S>(f.w), P<S>(f.ralt), P<S>(f.rp), P<S>(f.pp), (const double*)f.base, p.bounds, dts, 0);
-}
-
-template <class S>
-void launch_transport(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, int n_scalars,
-                      void* const* q_in, void* const* q_out, const int* pd)
-{
-    ScalarSet<S> set{};
-    FluxSet<S> fl{};
-    float* rls[3] = {f.rl_qv, f.rl_qc, f.rl_qr};
-    int npd = 0;
-    set.n = n_scalars;
-    for (int s = 0; s < n_scalars; ++s)
-    {
-        set.q[s] = P<S>(q_in[s]);
-        set.out[s] = P<S>(q_out[s]);
-        set.pd[s] = pd[s];
-        set.rl[s] = pd[s] ? rls[npd++] : nullptr;
-        fl.fx[s] = P<S>(f.flux[3 * s]);
-        fl.fy[s] = P<S>(f.flux[3 * s + 1]);
-        fl.fz[s] = P<S>(f.flux[3 * s + 2]);
-    }
-    const int open_top = int(p.bc_top == NIMBUS_TOP_OPEN);
-    NIMBUS_LAUNCH(L, "transport_flux", (k_transport_flux<S>), cell_grid(g), g, set, fl, P<S>(f.mfx), P<S>(f.mfy),
-                  P<S>(f.mfz), open_top);
-    if (npd > 0)
-        NIMBUS_LAUNCH(L, "transport_limiter", (k_transport_limiter<S>), cell_grid(g), g, set, fl, P<S>(f.rb),
-                      (const double*)f.base, p.dt, open_top);
-    NIMBUS_LAUNCH(L, "transport_update", (k_transport_update<S>), cell_grid(g), g, set, fl, P<S>(f.rb), P<S>(f.rp),
-                  (const double*)f.base, p.dt, open_top);
-}
-
-template <class S>
-void launch_kessler_tendencies(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, double dt,
-                               void* dth, void* dqv, void* dqc, void* dqr)
-{
-    NIMBUS_LAUNCH(L, "kessler_tendencies", (k_kessler_tendencies<S>), cell_grid(g), g, P<S>(f.th), P<S>(f.qv),
-                  P<S>(f.qc), P<S>(f.qr), P<S>(f.rp), P<S>(f.pp), (const double*)f.base, p.kessler, dt, P<S>(dth),
-                  P<S>(dqv), P<S>(dqc), P<S>(dqr));
-}
-
-template <class S>
-void launch_large_step(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p)
-{
-    const int N = p.substeps;
-    const double dts = p.dt / N;
-    const double weight = dts / p.dt;
-    launch_slow_rk2<S>(L, g, f, p);
-    launch_boundaries<S>(L, g, f, p, 0.0);
-    launch_seed_radiation<S>(L, g, f);
-    for (int n = 0; n < N; ++n)
-    {
-        launch_acoustic_substep<S>(L, g, f, p, dts, weight, n == 0, true);
-        launch_boundaries<S>(L, g, f, p, dts);
-    }
-    void* qin[4] = {f.th, f.qv, f.qc, f.qr};
-    void* qout[4] = {f.tth, f.tqv, f.tqc, f.tqr};
+    launch_commit<S>(L, b, -1);
     const int pd[4] = {0, 1, 1, 1};
-    launch_transport<S>(L, g, f, p, 4, qin, qout, pd);
-    // sedimentation CFL bound 0.8 dz / Vt_max (microphysics_step.cpp:3501-3560), state independent
-    const double limit = 0.8 * g.dz / p.kessler.vt_max;
-    const int nsub = p.dt <= limit ? 1 : int(ceil(p.dt / limit));
-    const double h = p.dt / nsub;
-    for (int n = 0; n < nsub; ++n)
-    {
-        const bool from_temp = (n % 2) == 0;
-        void* const* src = from_temp ? qout : qin;
-        void* const* dst = from_temp ? qin : qout;
-        NIMBUS_LAUNCH(L, "kessler_apply", (k_kessler_apply<S>), cell_grid(g), g, P<S>(src[0]), P<S>(src[1]),
-                      P<S>(src[2]), P<S>(src[3]), P<S>(dst[0]), P<S>(dst[1]), P<S>(dst[2]), P<S>(dst[3]), P<S>(f.rp),
-                      P<S>(f.pp), (const double*)f.base, p.kessler, h, f.rain, p.bounds, int(n == nsub - 1));
-    }
-    if (nsub % 2 == 0)
-        for (int s = 0; s < 4; ++s)
-            NIMBUS_LAUNCH(L, "copy", (k_copy<S>), linear_grid(g.cells()), g.cells(), P<S>(qout[s]), P<S>(qin[s]));
-    launch_boundaries<S>(L, g, f, p, 0.0);
+    launch_transport<S>(L, b, 4, pd, true);
+    for (int n = 0; n < b.max_kessler_sub; ++n)
+        launch_map(L, "kessler_apply", k_kessler_apply<S>, b, S_ALL, W_CHUNK, n);
+    launch_map(L, "scalars_home", k_scalars_home<S>, b, S_ALL, W_CHUNK);
+    launch_boundaries<S>(L, b, -1);
 }
 
 template <class S>
-void launch_diagnostics(Launcher& L, const DevGrid& g, const DevFields& f, double* partials, int n_partials,
-                        double* out)
+void launch_diagnostics(Launcher& L, const BatchView& b)
 {
-    NIMBUS_LAUNCH(L, "diag_partials", (k_diag_partials<S>), dim3(n_partials), g, P<S>(f.w), P<S>(f.th), P<S>(f.qc),
-                  P<S>(f.qr), (const double*)f.rain, (const double*)f.base, partials);
+    launch_map(L, "diag_partials", k_diag_partials<S>, b, S_ALL, W_DIAG);
     hipEvent_t ev;
     L.begin("diag_final", ev);
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, L.stream, (const double*)partials, n_partials, out);
+    hipLaunchKernelGGL(k_diag_final, dim3(unsigned(b.n)), dim3(64), 0, L.stream, b.args, b.n);
     L.end("diag_final", ev);
 }
 
 template <class H, class S>
 void launch_host_to_dev(Launcher& L, const DevGrid& g, const void* src, void* dst, const double* base_slot)
 {
-    NIMBUS_LAUNCH(L, "host_to_dev", (k_host_to_dev<H, S>), cell_grid(g), g, static_cast<const H*>(src), P<S>(dst),
-                  base_slot);
+    hipEvent_t ev;
+    L.begin("host_to_dev", ev);
+    hipLaunchKernelGGL((k_host_to_dev<H, S>), dim3(unsigned((g.nx + 63) / 64), unsigned(g.ny), unsigned(g.nz)),
+                       dim3(64), 0, L.stream, g, static_cast<const H*>(src), PS<S>(dst), base_slot);
+    L.end("host_to_dev", ev);
 }
 
 template <class S, class H>
 void launch_dev_to_host(Launcher& L, const DevGrid& g, const void* src, void* dst, const double* base_slot)
 {
-    NIMBUS_LAUNCH(L, "dev_to_host", (k_dev_to_host<S, H>), cell_grid(g), g, static_cast<const S*>(src),
-                  static_cast<H*>(dst), base_slot);
+    hipEvent_t ev;
+    L.begin("dev_to_host", ev);
+    hipLaunchKernelGGL((k_dev_to_host<S, H>), dim3(unsigned((g.nx + 63) / 64), unsigned(g.ny), unsigned(g.nz)),
+                       dim3(64), 0, L.stream, g, static_cast<const S*>(src), static_cast<H*>(dst), base_slot);
+    L.end("dev_to_host", ev);
 }
 
-#define NIMBUS_INSTANTIATE(S)                                                                                        \
-    template void launch_large_step<S>(Launcher&, const DevGrid&, const DevFields&, const StepParams&);             \
-    template void launch_slow_rk2<S>(Launcher&, const DevGrid&, const DevFields&, const StepParams&);               \
-    template void launch_slow_tendencies<S>(Launcher&, const DevGrid&, const DevFields&, const StepParams&, void*,   \
-                                            void*, void*);                                                           \
-    template void launch_acoustic_substep<S>(Launcher&, const DevGrid&, const DevFields&, const StepParams&, double, \
-                                             double, bool, bool);                                                    \
-    template void launch_boundaries<S>(Launcher&, const DevGrid&, const DevFields&, const StepParams&, double);     \
-    template void launch_seed_radiation<S>(Launcher&, const DevGrid&, const DevFields&);                            \
-    template void launch_transport<S>(Launcher&, const DevGrid&, const DevFields&, const StepParams&, int,          \
-                                      void* const*, void* const*, const int*);                                       \
-    template void launch_kessler_tendencies<S>(Launcher&, const DevGrid&, const DevFields&, const StepParams&,      \
-                                               double, void*, void*, void*, void*);                                  \
-    template void launch_diagnostics<S>(Launcher&, const DevGrid&, const DevFields&, double*, int, double*);        \
-    template void launch_host_to_dev<float, S>(Launcher&, const DevGrid&, const void*, void*, const double*);       \
-    template void launch_host_to_dev<double, S>(Launcher&, const DevGrid&, const void*, void*, const double*);      \
-    template void launch_dev_to_host<S, float>(Launcher&, const DevGrid&, const void*, void*, const double*);       \
+#define NIMBUS_INSTANTIATE(S)                                                                                    \
+    template void launch_large_step<S>(Launcher&, const BatchView&);                                            \
+    template void launch_slow_rk2<S>(Launcher&, const BatchView&);                                              \
+    template void launch_slow_tendencies<S>(Launcher&, const BatchView&, void*, void*, void*);                  \
+    template void launch_acoustic_substep<S>(Launcher&, const BatchView&, int, bool, bool);                     \
+    template void launch_boundaries<S>(Launcher&, const BatchView&, int);                                       \
+    template void launch_seed_radiation<S>(Launcher&, const BatchView&);                                        \
+    template void launch_commit<S>(Launcher&, const BatchView&, int);                                           \
+    template void launch_transport<S>(Launcher&, const BatchView&, int, const int*, bool);                      \
+    template void launch_kessler_tendencies<S>(Launcher&, const BatchView&, double, void*, void*, void*, void*); \
+    template void launch_diagnostics<S>(Launcher&, const BatchView&);                                           \
+    template void launch_host_to_dev<float, S>(Launcher&, const DevGrid&, const void*, void*, const double*);   \
+    template void launch_host_to_dev<double, S>(Launcher&, const DevGrid&, const void*, void*, const double*);  \
+    template void launch_dev_to_host<S, float>(Launcher&, const DevGrid&, const void*, void*, const double*);   \
     template void launch_dev_to_host<S, double>(Launcher&, const DevGrid&, const void*, void*, const double*);
 
 NIMBUS_INSTANTIATE(float)

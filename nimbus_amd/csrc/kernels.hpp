@@ -1,6 +1,11 @@
-// Launch interface of the large-step kernels. Every launcher enqueues on the
-// stream held by the Launcher, which optionally brackets each launch with HIP
-// events for per-kernel timing.
+// Launch interface of the large-step kernels.
+//
+// Every kernel works on a batch of members (independent domains of possibly
+// different shapes, one precision per batch). The batch keeps a device array
+// of MemberArgs (grid, field pointers, step parameters) and, per launch
+// shape, a block map: entry b of the map names the member and the member-local
+// work tile of workgroup b. One launch therefore covers every member of the
+// batch, and the map order decides which XCD runs which tiles.
 #pragma once
 
 #include "nimbus_internal.hpp"
@@ -31,66 +36,137 @@ struct Launcher
     void end(const char* name, hipEvent_t a);
 };
 
-// Device field set of one member. Precision-erased pointers; the kernels are
-// instantiated for float and double storage.
+// Device field set of one member (precision-erased pointers). u, v, w and
+// rho' have a second buffer: the fused explicit acoustic substep reads one
+// and writes the other, so substep s leaves them in buffer (s + 1) & 1.
 struct DevFields
 {
     void *u, *v, *w, *rp, *pp, *th, *qv, *qc, *qr;       // prognostic state (rho, p as perturbations)
+    void *u2, *v2, *w2, *rp2;                            // second acoustic buffers (u2..w2 alias the predictor)
     void *mfx, *mfy, *mfz, *rb;                          // retained mass fluxes, rho' at step begin
-    void *k1u, *k1v, *k1w, *su, *sv, *sw;                // Heun stage-1 tendencies and predictor
-    void* ralt;                                          // rho' ping-pong of the acoustic substep
+    void *k1u, *k1v, *k1w;                               // Heun stage-1 tendencies
     void *tth, *tqv, *tqc, *tqr;                         // transported scalars before microphysics
-    void* flux[12];                                      // per scalar: x, y, z face fluxes
+    void* flux[12];                                      // per scalar: x, y, z face fluxes (7 alias dead slots)
     float *rl_qv, *rl_qc, *rl_qr;                        // PD limiter factors
     double *tri_c, *tri_r;                               // HE-VI Thomas coefficients (FP64)
     float *rad_w, *rad_e, *rad_s, *rad_n;                // KW78 radiative stashes
     double* rain;                                        // accumulated surface rain [ny][nx]
     double* base;                                        // kBaseCount * nz FP64 profiles
+    double* diag;                                        // kDiagBlocksPerMember * kDiagValues partials + result
 };
 
 struct StepParams
 {
-    double dt;
+    double dt;          // large step
+    double dts;         // acoustic substep
+    double mf_weight;   // retained mass-flux weight per substep (dts / dt in the large step)
     int substeps;
     int acoustic_mode;
-    double hevi_beta;
+    double hevi_alpha;  // (1 + beta) / 2
     double coriolis_f;
     int bc_west, bc_east, bc_south, bc_north, bc_bottom, bc_top;
+    int kessler_sub;    // microphysics subcycles per large step
     KesslerConst kessler;
     Bounds bounds;
 };
 
-template <class S>
-void launch_large_step(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p);
+// Launch geometry of one member. Tiles are tile_w x (256 / tile_w) columns
+// (four waves of 64 lanes); strips are 63 owned columns plus one halo lane
+// per wave, four rows per workgroup. Column marches are split into
+// k_chunks vertical chunks.
+struct MemberGeom
+{
+    int tile_w, tile_h, tiles_x, tiles_y;
+    int strips_x, strips_y;
+    int k_chunks;
+    int xcd_remap;
+};
 
-// Individual phases (used by the per-operation dispatch entry points).
+struct MemberArgs
+{
+    DevGrid g;
+    MemberGeom geo;
+    DevFields f;
+    StepParams p;
+};
+
+// Work shapes of a launch: what one map entry (one workgroup) covers.
+enum WorkKind
+{
+    W_PLANE = 0,   // one tile of one k plane
+    W_CHUNK = 1,   // one tile, k chunk of the stencil marches
+    W_COLUMN = 2,  // one tile, whole columns
+    W_STRIP = 3,   // one strip block, k chunk
+    W_WE = 4,      // 256 (j, k) rows of the west/east faces
+    W_SN = 5,      // 256 (i, k) rows of the south/north faces
+    W_SEED = 6,    // 256 entries of the radiative stashes
+    W_DIAG = 7,    // one of kDiagBlocksPerMember reduction blocks
+    W_KINDS = 8
+};
+
+// Member subsets a launch can be restricted to.
+enum MemberSubset
+{
+    S_ALL = 0,
+    S_EXPLICIT = 1,
+    S_HEVI = 2,
+    S_SUBSETS = 3
+};
+
+struct BlockMap
+{
+    const int2* entries = nullptr;  // device: {member index, local work index}
+    int blocks = 0;
+};
+
+struct BatchView
+{
+    const MemberArgs* args = nullptr;  // device array
+    int n = 0;
+    int max_substeps = 0;
+    int max_kessler_sub = 0;
+    int elem = 4;
+    BlockMap maps[S_SUBSETS][W_KINDS];
+};
+
+// Number of local work items of `kind` for one member.
+int work_items(const MemberArgs& a, WorkKind kind);
+void fill_geometry(MemberArgs& a, int tile_w, int k_chunks, int xcd_remap);
+
+// Whole large step of every member of the batch.
 template <class S>
-void launch_slow_rk2(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p);
+void launch_large_step(Launcher& L, const BatchView& b);
+
+// Individual phases (per-operation dispatch entry points).
 template <class S>
-void launch_slow_tendencies(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, void* du,
-                            void* dv, void* dw);
+void launch_slow_rk2(Launcher& L, const BatchView& b);
 template <class S>
-void launch_acoustic_substep(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, double dts,
-                             double weight, bool first, bool accumulate);
+void launch_slow_tendencies(Launcher& L, const BatchView& b, void* du, void* dv, void* dw);
+// Substep s of the acoustic loop (0-based): reads buffer s & 1, writes (s + 1) & 1.
 template <class S>
-void launch_boundaries(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, double dt_rad);
+void launch_acoustic_substep(Launcher& L, const BatchView& b, int s, bool accumulate, bool init_at_first);
+// Boundary closure; s >= 0 selects the buffers after substep s and radiates
+// with the member's dts; s < 0 closes the home buffers without radiation.
 template <class S>
-void launch_seed_radiation(Launcher& L, const DevGrid& g, const DevFields& f);
+void launch_boundaries(Launcher& L, const BatchView& b, int s);
 template <class S>
-void launch_transport(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, int n_scalars,
-                      void* const* q_in, void* const* q_out, const int* pd);
+void launch_seed_radiation(Launcher& L, const BatchView& b);
+// Moves u, v, w, rho' from buffer 1 to buffer 0 for members whose last substep (n_done - 1) left them there.
 template <class S>
-void launch_kessler_tendencies(Launcher& L, const DevGrid& g, const DevFields& f, const StepParams& p, double dt,
-                               void* dth, void* dqv, void* dqc, void* dqr);
+void launch_commit(Launcher& L, const BatchView& b, int n_done);
 template <class S>
-void launch_diagnostics(Launcher& L, const DevGrid& g, const DevFields& f, double* partials, int n_partials,
-                        double* out);
+void launch_transport(Launcher& L, const BatchView& b, int n_scalars, const int* pd, bool into_temp);
+template <class S>
+void launch_kessler_tendencies(Launcher& L, const BatchView& b, double dt, void* dth, void* dqv, void* dqc,
+                               void* dqr);
+template <class S>
+void launch_diagnostics(Launcher& L, const BatchView& b);
 template <class H, class S>
 void launch_host_to_dev(Launcher& L, const DevGrid& g, const void* src, void* dst, const double* base_slot);
 template <class S, class H>
 void launch_dev_to_host(Launcher& L, const DevGrid& g, const void* src, void* dst, const double* base_slot);
 
 constexpr int kDiagValues = 8;
-constexpr int kDiagBlocks = 512;
+constexpr int kDiagBlocksPerMember = 64;
 
 }  // namespace nimbus

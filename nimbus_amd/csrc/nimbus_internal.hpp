@@ -37,15 +37,12 @@ struct DevGrid
     int nx, ny, nz, px;
     double dx, dy, dz;
     double idx, idy, idz;  // 1/dx, 1/dy, 1/dz
-    // Launch tiling: tile_w x (256 / tile_w) cell tiles per (j, k) plane,
-    // tiles_x * tiles_y per plane; xcd_remap orders tiles so each XCD sweeps a
-    // contiguous slab of planes.
-    int tile_w, tile_shift, tiles_x, tiles_y, tiles_plane, xcd_remap;
     __host__ __device__ std::size_t at(int i, int j, int k) const
     {
         return (std::size_t(k) * std::size_t(ny) + std::size_t(j)) * std::size_t(px) + std::size_t(i);
     }
     __host__ __device__ std::size_t cells() const { return std::size_t(px) * ny * nz; }
+    __host__ __device__ std::size_t plane() const { return std::size_t(px) * ny; }
 };
 
 // Base-state profile block in device memory (FP64), each nz long.

@@ -1,6 +1,8 @@
 // Host runtime of the HIP backend: device context, resident ensemble members
-// (device-owned state in [k][j][i] layout), hipGraph capture of whole large
-// steps, per-kernel HIP-event timing and the C ABI declared in nimbus_hip.h.
+// (device-owned state in [k][j][i] layout), batches of members that share
+// every kernel launch (device member table + block maps), hipGraph capture of
+// a batch's whole large step, per-kernel HIP-event timing and the C ABI
+// declared in nimbus_hip.h.
 #include "kernels.hpp"
 #include "nimbus_internal.hpp"
 
@@ -44,31 +46,17 @@ int pitch_for(int nx, int elem)
     return (nx + per - 1) / per * per;
 }
 
-struct Member
-{
-    nimbus_member_desc desc{};
-    std::vector<double> host_base;  // kBaseCount * nz
-    DevGrid g{};
-    int elem = 4;
-    int substeps = 0;
-    StepParams params{};
-    void* block = nullptr;  // one device allocation for everything
-    std::size_t bytes = 0;
-    DevFields f{};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    std::int64_t steps = 0;
-};
-
-// Field slots of a member allocation, in order.
+// Field slots of a member allocation, in order. The face-flux scratch of the
+// scalar transport reuses seven slots that are dead while it runs (Heun
+// stage-1 tendencies and the second acoustic buffers) plus five of its own.
 enum FieldSlot
 {
     F_U, F_V, F_W, F_RP, F_PP, F_TH, F_QV, F_QC, F_QR,
+    F_U2, F_V2, F_W2, F_RP2,
     F_MFX, F_MFY, F_MFZ, F_RB,
-    F_K1U, F_K1V, F_K1W, F_SU, F_SV, F_SW,
-    F_RALT,
+    F_K1U, F_K1V, F_K1W,
     F_TTH, F_TQV, F_TQC, F_TQR,
-    F_FLUX0, F_FLUX_END = F_FLUX0 + 12,
+    F_FLUX7, F_FLUX_END = F_FLUX7 + 5,
     F_COUNT = F_FLUX_END
 };
 
@@ -76,7 +64,7 @@ std::size_t align256(std::size_t x) { return (x + 255) / 256 * 256; }
 
 struct Layout
 {
-    std::size_t field, rl, tri, rad_w, rad_s, rain, base, total;
+    std::size_t field, rl, tri, rad_w, rad_s, rain, base, diag, total;
 };
 
 Layout layout_for(const DevGrid& g, int elem, bool hevi)
@@ -89,22 +77,9 @@ Layout layout_for(const DevGrid& g, int elem, bool hevi)
     L.rad_s = align256(std::size_t(g.nx) * g.nz * sizeof(float));
     L.rain = align256(std::size_t(g.nx) * g.ny * sizeof(double));
     L.base = align256(std::size_t(kBaseCount) * g.nz * sizeof(double));
-    L.total = L.field * F_COUNT + 3 * L.rl + 2 * L.tri + 2 * L.rad_w + 2 * L.rad_s + L.rain + L.base;
+    L.diag = align256(std::size_t(kDiagBlocksPerMember + 1) * kDiagValues * sizeof(double));
+    L.total = L.field * F_COUNT + 3 * L.rl + 2 * L.tri + 2 * L.rad_w + 2 * L.rad_s + L.rain + L.base + L.diag;
     return L;
-}
-
-// Tile width in {16, 32, 64, 128, 256}; tile height = 256 / width.
-void set_tiling(DevGrid& d, int tile_w, int remap)
-{
-    int shift = 4;
-    while ((1 << shift) < tile_w && shift < 8) ++shift;
-    d.tile_w = 1 << shift;
-    d.tile_shift = shift;
-    const int tile_h = 256 / d.tile_w;
-    d.tiles_x = (d.nx + d.tile_w - 1) / d.tile_w;
-    d.tiles_y = (d.ny + tile_h - 1) / tile_h;
-    d.tiles_plane = d.tiles_x * d.tiles_y;
-    d.xcd_remap = remap ? 1 : 0;
 }
 
 DevGrid dev_grid(const nimbus_grid& g, int elem)
@@ -120,8 +95,17 @@ DevGrid dev_grid(const nimbus_grid& g, int elem)
     d.idx = 1.0 / g.dx;
     d.idy = 1.0 / g.dy;
     d.idz = 1.0 / g.dz;
-    set_tiling(d, 32, 1);
     return d;
+}
+
+// Default vertical chunking: enough column-march workgroups to give every
+// CU several (about 2048 over the member), chunks of at least 8 planes.
+int default_k_chunks(const DevGrid& g, int tile_w)
+{
+    const int th = 256 / tile_w;
+    const int tiles = ((g.nx + tile_w - 1) / tile_w) * ((g.ny + th - 1) / th);
+    const int want = (2048 + tiles - 1) / tiles;
+    return std::max(1, std::min(want, g.nz / 8));
 }
 
 bool valid_grid(const nimbus_grid& g)
@@ -142,6 +126,40 @@ bool valid_config(const nimbus_step_config& c)
            !(c.acoustic_mode == NIMBUS_ACOUSTIC_HEVI && c.bc_top == NIMBUS_TOP_OPEN) && c.vt_max > 0 &&
            std::isfinite(c.vt_max) && c.qc0 >= 0 && c.c_auto >= 0 && c.c_accr >= 0 && c.evap_eff >= 0;
 }
+
+struct Member
+{
+    nimbus_member_desc desc{};
+    std::vector<double> host_base;  // kBaseCount * nz
+    int elem = 4;
+    MemberArgs args{};              // host copy of the device member-table entry
+    void* block = nullptr;          // one device allocation for everything
+    std::size_t bytes = 0;
+    std::int64_t steps = 0;
+    int tile_w = 64, k_chunks = 1, xcd_remap = 1;
+    const DevGrid& g() const { return args.g; }
+    const DevFields& f() const { return args.f; }
+};
+
+// A batch: members of one precision that share every launch.
+struct Batch
+{
+    std::vector<std::uint32_t> ids;
+    int elem = 4;
+    MemberArgs* d_args = nullptr;
+    int2* d_maps = nullptr;
+    BatchView view{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::uint64_t last_use = 0;
+    ~Batch()
+    {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        if (d_args) hipFree(d_args);
+        if (d_maps) hipFree(d_maps);
+    }
+};
 
 }  // namespace
 }  // namespace nimbus
@@ -165,14 +183,16 @@ struct nimbus_hip_ctx
     std::uint64_t launches = 0;
     hipEvent_t step_a = nullptr, step_b = nullptr;
     double last_step_ms = 0.0;
-    double* diag_partials = nullptr;
-    double* diag_out = nullptr;
+    std::map<std::pair<int, std::vector<std::uint32_t>>, std::unique_ptr<nimbus::Batch>> batches;
+    std::uint64_t batch_clock = 0;
 };
 
 namespace nimbus
 {
 namespace
 {
+
+constexpr std::size_t kMaxCachedBatches = 8;
 
 Launcher make_launcher(nimbus_hip_ctx* ctx)
 {
@@ -209,19 +229,25 @@ Member* find(nimbus_hip_ctx* ctx, std::uint32_t id)
     return it == ctx->members.end() ? nullptr : it->second.get();
 }
 
-void release_graph(Member& m)
+void drop_batches(nimbus_hip_ctx* ctx)
 {
-    if (m.graph_exec) hipGraphExecDestroy(m.graph_exec);
-    if (m.graph) hipGraphDestroy(m.graph);
-    m.graph_exec = nullptr;
-    m.graph = nullptr;
+    hipStreamSynchronize(ctx->stream);
+    ctx->batches.clear();
+}
+
+void set_geometry(Member& m, int tile_w, int k_chunks, int remap)
+{
+    m.tile_w = tile_w == 32 ? 32 : 64;
+    m.k_chunks = k_chunks > 0 ? std::min(k_chunks, m.args.g.nz) : default_k_chunks(m.args.g, m.tile_w);
+    m.xcd_remap = remap ? 1 : 0;
+    fill_geometry(m.args, m.tile_w, m.k_chunks, m.xcd_remap);
 }
 
 int allocate_member(const nimbus_member_desc& desc, Member& m)
 {
     m.desc = desc;
     m.elem = desc.precision == NIMBUS_PRECISION_MIXED ? 4 : 8;
-    m.g = dev_grid(desc.grid, m.elem);
+    m.args.g = dev_grid(desc.grid, m.elem);
     const int nz = desc.grid.nz;
     const bool hevi = desc.config.acoustic_mode == NIMBUS_ACOUSTIC_HEVI;
     m.host_base.assign(std::size_t(kBaseCount) * nz, 0.0);
@@ -252,12 +278,12 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
         hb[std::size_t(kBasePeDiff) * nz + k] = pe0 - p0;
         hb[std::size_t(kBaseInvRT0) * nz + k] = inv;
     }
-    m.substeps = resolve_substeps(desc.grid, m.host_base.data() + std::size_t(kBaseRho0) * nz,
-                                  m.host_base.data() + std::size_t(kBaseP0) * nz, desc.config);
-    if (m.substeps < 1)
+    const int substeps = resolve_substeps(desc.grid, m.host_base.data() + std::size_t(kBaseRho0) * nz,
+                                          m.host_base.data() + std::size_t(kBaseP0) * nz, desc.config);
+    if (substeps < 1)
         return fail(NIMBUS_EINVAL, "acoustic substep count unresolvable (non-positive base density or "
                                    "more than 1e6 substeps required)");
-    const Layout L = layout_for(m.g, m.elem, hevi);
+    const Layout L = layout_for(m.args.g, m.elem, hevi);
     m.bytes = L.total;
     hipError_t e = hipMalloc(&m.block, L.total);
     if (e != hipSuccess)
@@ -272,18 +298,21 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
         fields[s] = p;
         p += L.field;
     }
-    DevFields& f = m.f;
+    DevFields& f = m.args.f;
     f.u = fields[F_U]; f.v = fields[F_V]; f.w = fields[F_W]; f.rp = fields[F_RP]; f.pp = fields[F_PP];
     f.th = fields[F_TH]; f.qv = fields[F_QV]; f.qc = fields[F_QC]; f.qr = fields[F_QR];
+    f.u2 = fields[F_U2]; f.v2 = fields[F_V2]; f.w2 = fields[F_W2]; f.rp2 = fields[F_RP2];
     f.mfx = fields[F_MFX]; f.mfy = fields[F_MFY]; f.mfz = fields[F_MFZ]; f.rb = fields[F_RB];
     f.k1u = fields[F_K1U]; f.k1v = fields[F_K1V]; f.k1w = fields[F_K1W];
-    f.su = fields[F_SU]; f.sv = fields[F_SV]; f.sw = fields[F_SW];
-    f.ralt = fields[F_RALT];
     f.tth = fields[F_TTH]; f.tqv = fields[F_TQV]; f.tqc = fields[F_TQC]; f.tqr = fields[F_TQR];
-    for (int q = 0; q < 12; ++q) f.flux[q] = fields[F_FLUX0 + q];
+    void* flux_slots[12] = {f.k1u, f.k1v, f.k1w, f.u2, f.v2, f.w2, f.rp2, fields[F_FLUX7], fields[F_FLUX7 + 1],
+                            fields[F_FLUX7 + 2], fields[F_FLUX7 + 3], fields[F_FLUX7 + 4]};
+    for (int q = 0; q < 12; ++q) f.flux[q] = flux_slots[q];
     f.rl_qv = reinterpret_cast<float*>(p); p += L.rl;
     f.rl_qc = reinterpret_cast<float*>(p); p += L.rl;
     f.rl_qr = reinterpret_cast<float*>(p); p += L.rl;
+    f.tri_c = nullptr;
+    f.tri_r = nullptr;
     if (hevi)
     {
         f.tri_c = reinterpret_cast<double*>(p); p += L.tri;
@@ -295,44 +324,220 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     f.rad_n = reinterpret_cast<float*>(p); p += L.rad_s;
     f.rain = reinterpret_cast<double*>(p); p += L.rain;
     f.base = reinterpret_cast<double*>(p); p += L.base;
+    f.diag = reinterpret_cast<double*>(p); p += L.diag;
     // Zero everything once (not inside any graph), then upload the base state.
     e = hipMemset(m.block, 0, L.total);
     if (e == hipSuccess)
         e = hipMemcpy(f.base, m.host_base.data(), m.host_base.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "member initialisation");
-    StepParams& sp = m.params;
+    StepParams& sp = m.args.p;
     const nimbus_step_config& c = desc.config;
     sp.dt = c.dt;
-    sp.substeps = m.substeps;
+    sp.substeps = substeps;
+    sp.dts = c.dt / substeps;
+    sp.mf_weight = sp.dts / c.dt;
     sp.acoustic_mode = c.acoustic_mode;
-    sp.hevi_beta = c.hevi_beta;
+    sp.hevi_alpha = 0.5 * (1.0 + c.hevi_beta);
     sp.coriolis_f = c.coriolis_f;
     sp.bc_west = c.bc_west; sp.bc_east = c.bc_east; sp.bc_south = c.bc_south; sp.bc_north = c.bc_north;
     sp.bc_bottom = c.bc_bottom; sp.bc_top = c.bc_top;
+    // sedimentation CFL bound 0.8 dz / Vt_max (microphysics_step.cpp:3501-3560), state independent
+    const double limit = 0.8 * desc.grid.dz / c.vt_max;
+    sp.kessler_sub = c.dt <= limit ? 1 : int(std::ceil(c.dt / limit));
     sp.kessler = KesslerConst{c.qc0, c.c_auto, c.c_accr, c.evap_eff, c.vt_max};
     sp.bounds = Bounds{};
+    set_geometry(m, 64, 0, 1);
     return NIMBUS_OK;
 }
 
 void free_member(Member& m)
 {
-    release_graph(m);
     if (m.block) hipFree(m.block);
     m.block = nullptr;
 }
 
-const double* base_slot(const Member& m, int s) { return m.f.base + std::size_t(s) * m.g.nz; }
+const double* base_slot(const Member& m, int s) { return m.f().base + std::size_t(s) * m.g().nz; }
+
+// Block b of a launch over n logical work items runs on XCD b % 8; the
+// permutation hands each XCD a contiguous range of logical items, so the
+// horizontally neighbouring tiles a stencil reads share that XCD's L2.
+int xcd_logical(int b, int n)
+{
+    constexpr int kXcd = 8;
+    const int x = b % kXcd, local = b / kXcd;
+    const int q = n / kXcd, r = n % kXcd;
+    return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
+}
+
+// Builds the device member table and block maps of a batch.
+int build_batch(Batch& b, const std::vector<Member*>& ms)
+{
+    std::vector<MemberArgs> args;
+    args.reserve(ms.size());
+    b.view.max_substeps = 0;
+    b.view.max_kessler_sub = 0;
+    bool remap = true;
+    for (Member* m : ms)
+    {
+        args.push_back(m->args);
+        b.view.max_substeps = std::max(b.view.max_substeps, m->args.p.substeps);
+        b.view.max_kessler_sub = std::max(b.view.max_kessler_sub, m->args.p.kessler_sub);
+        remap = remap && m->xcd_remap;
+    }
+    std::vector<int2> all;
+    std::size_t offsets[S_SUBSETS][W_KINDS];
+    int counts[S_SUBSETS][W_KINDS];
+    for (int s = 0; s < S_SUBSETS; ++s)
+        for (int k = 0; k < W_KINDS; ++k)
+        {
+            std::vector<int2> logical;
+            for (std::size_t q = 0; q < ms.size(); ++q)
+            {
+                const int mode = ms[q]->args.p.acoustic_mode;
+                if (s == S_EXPLICIT && mode != NIMBUS_ACOUSTIC_EXPLICIT) continue;
+                if (s == S_HEVI && mode != NIMBUS_ACOUSTIC_HEVI) continue;
+                const int n = work_items(ms[q]->args, WorkKind(k));
+                for (int l = 0; l < n; ++l) logical.push_back(int2{int(q), l});
+            }
+            const int n = int(logical.size());
+            offsets[s][k] = all.size();
+            counts[s][k] = n;
+            for (int blk = 0; blk < n; ++blk) all.push_back(logical[remap ? xcd_logical(blk, n) : blk]);
+        }
+    NIMBUS_HIP_TRY(hipMalloc(&b.d_args, sizeof(MemberArgs) * args.size()));
+    NIMBUS_HIP_TRY(hipMemcpy(b.d_args, args.data(), sizeof(MemberArgs) * args.size(), hipMemcpyHostToDevice));
+    NIMBUS_HIP_TRY(hipMalloc(&b.d_maps, sizeof(int2) * std::max<std::size_t>(all.size(), 1)));
+    if (!all.empty())
+        NIMBUS_HIP_TRY(hipMemcpy(b.d_maps, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
+    b.view.args = b.d_args;
+    b.view.n = int(ms.size());
+    b.view.elem = b.elem;
+    for (int s = 0; s < S_SUBSETS; ++s)
+        for (int k = 0; k < W_KINDS; ++k)
+            b.view.maps[s][k] = BlockMap{b.d_maps + offsets[s][k], counts[s][k]};
+    return NIMBUS_OK;
+}
+
+// Cached batch for a set of member ids of one precision.
+int get_batch(nimbus_hip_ctx* ctx, const std::vector<Member*>& ms, const std::vector<std::uint32_t>& ids, Batch** out)
+{
+    std::vector<std::uint32_t> key = ids;
+    std::sort(key.begin(), key.end());
+    const int elem = ms.front()->elem;
+    auto k = std::make_pair(elem, key);
+    auto it = ctx->batches.find(k);
+    if (it == ctx->batches.end())
+    {
+        if (ctx->batches.size() >= kMaxCachedBatches)
+        {
+            auto oldest = ctx->batches.begin();
+            for (auto jt = ctx->batches.begin(); jt != ctx->batches.end(); ++jt)
+                if (jt->second->last_use < oldest->second->last_use) oldest = jt;
+            hipStreamSynchronize(ctx->stream);
+            ctx->batches.erase(oldest);
+        }
+        auto b = std::make_unique<Batch>();
+        b->ids = key;
+        b->elem = elem;
+        std::vector<Member*> sorted;
+        for (std::uint32_t id : key) sorted.push_back(find(ctx, id));
+        const int rc = build_batch(*b, sorted);
+        if (rc != NIMBUS_OK) return rc;
+        it = ctx->batches.emplace(k, std::move(b)).first;
+    }
+    it->second->last_use = ++ctx->batch_clock;
+    *out = it->second.get();
+    return NIMBUS_OK;
+}
+
+void enqueue_step(Launcher& L, const Batch& b)
+{
+    if (b.elem == 4) launch_large_step<float>(L, b.view);
+    else launch_large_step<double>(L, b.view);
+}
+
+int run_batch_step(nimbus_hip_ctx* ctx, Batch& b)
+{
+    if (ctx->graphs && !ctx->timing)
+    {
+        if (!b.exec)
+        {
+            Launcher L = make_launcher(ctx);
+            L.timing = false;
+            NIMBUS_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(L, b);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+            if (L.error != hipSuccess) return hip_fail(L.error, "step capture launch");
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            b.graph = graph;
+            NIMBUS_HIP_TRY(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+        }
+        NIMBUS_HIP_TRY(hipGraphLaunch(b.exec, ctx->stream));
+    }
+    else
+    {
+        Launcher L = make_launcher(ctx);
+        enqueue_step(L, b);
+        if (L.error != hipSuccess) return hip_fail(L.error, "step launch");
+    }
+    return NIMBUS_OK;
+}
+
+int step_members(nimbus_hip_ctx* ctx, const std::uint32_t* ids, int n_ids, int n_steps, bool wait)
+{
+    if (!ctx || (!ids && n_ids > 0) || n_ids < 0 || n_steps < 0) return fail(NIMBUS_EINVAL, "step: bad arguments");
+    std::vector<Member*> groups[2];
+    std::vector<std::uint32_t> gids[2];
+    for (int q = 0; q < n_ids; ++q)
+    {
+        Member* m = find(ctx, ids[q]);
+        if (!m) return fail(NIMBUS_ENOENT, "step: unknown member");
+        for (int r = 0; r < q; ++r)
+            if (ids[r] == ids[q]) return fail(NIMBUS_EINVAL, "step: member listed twice");
+        const int gi = m->elem == 4 ? 0 : 1;
+        groups[gi].push_back(m);
+        gids[gi].push_back(ids[q]);
+    }
+    Batch* batches[2] = {nullptr, nullptr};
+    for (int gi = 0; gi < 2; ++gi)
+        if (!groups[gi].empty())
+        {
+            const int rc = get_batch(ctx, groups[gi], gids[gi], &batches[gi]);
+            if (rc != NIMBUS_OK) return rc;
+        }
+    NIMBUS_HIP_TRY(hipEventRecord(ctx->step_a, ctx->stream));
+    for (int s = 0; s < n_steps; ++s)
+        for (Batch* b : batches)
+            if (b)
+            {
+                const int rc = run_batch_step(ctx, *b);
+                if (rc != NIMBUS_OK) return rc;
+            }
+    NIMBUS_HIP_TRY(hipEventRecord(ctx->step_b, ctx->stream));
+    for (int gi = 0; gi < 2; ++gi)
+        for (Member* m : groups[gi]) m->steps += n_steps;
+    if (wait)
+    {
+        NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        float ms = 0.0f;
+        NIMBUS_HIP_TRY(hipEventElapsedTime(&ms, ctx->step_a, ctx->step_b));
+        ctx->last_step_ms = ms;
+        harvest(ctx);
+    }
+    return NIMBUS_OK;
+}
 
 // Host field h (reference layout, element H) -> device field d via a staging buffer.
 template <class H>
 int upload_field(nimbus_hip_ctx* ctx, Member& m, const void* h, void* d, const double* subtract, void* staging)
 {
-    const std::size_t n = std::size_t(m.g.nx) * m.g.ny * m.g.nz;
+    const std::size_t n = std::size_t(m.g().nx) * m.g().ny * m.g().nz;
     NIMBUS_HIP_TRY(hipMemcpyAsync(staging, h, n * sizeof(H), hipMemcpyHostToDevice, ctx->stream));
     Launcher L = make_launcher(ctx);
     L.timing = false;
-    if (m.elem == 4) launch_host_to_dev<H, float>(L, m.g, staging, d, subtract);
-    else launch_host_to_dev<H, double>(L, m.g, staging, d, subtract);
+    if (m.elem == 4) launch_host_to_dev<H, float>(L, m.g(), staging, d, subtract);
+    else launch_host_to_dev<H, double>(L, m.g(), staging, d, subtract);
     if (L.error != hipSuccess) return hip_fail(L.error, "upload transpose");
     return NIMBUS_OK;
 }
@@ -340,11 +545,11 @@ int upload_field(nimbus_hip_ctx* ctx, Member& m, const void* h, void* d, const d
 template <class H>
 int download_field(nimbus_hip_ctx* ctx, Member& m, const void* d, void* h, const double* add, void* staging)
 {
-    const std::size_t n = std::size_t(m.g.nx) * m.g.ny * m.g.nz;
+    const std::size_t n = std::size_t(m.g().nx) * m.g().ny * m.g().nz;
     Launcher L = make_launcher(ctx);
     L.timing = false;
-    if (m.elem == 4) launch_dev_to_host<float, H>(L, m.g, d, staging, add);
-    else launch_dev_to_host<double, H>(L, m.g, d, staging, add);
+    if (m.elem == 4) launch_dev_to_host<float, H>(L, m.g(), d, staging, add);
+    else launch_dev_to_host<double, H>(L, m.g(), d, staging, add);
     if (L.error != hipSuccess) return hip_fail(L.error, "download transpose");
     NIMBUS_HIP_TRY(hipMemcpyAsync(h, staging, n * sizeof(H), hipMemcpyDeviceToHost, ctx->stream));
     NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -364,10 +569,11 @@ template <class H>
 int upload_state(nimbus_hip_ctx* ctx, Member& m, const nimbus_host_state& s)
 {
     Staging st;
-    const std::size_t n = std::size_t(m.g.nx) * m.g.ny * m.g.nz;
+    const std::size_t n = std::size_t(m.g().nx) * m.g().ny * m.g().nz;
     NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(H)));
     const void* hs[9] = {s.u, s.v, s.w, s.rho, s.p, s.theta, s.qv, s.qc, s.qr};
-    void* ds[9] = {m.f.u, m.f.v, m.f.w, m.f.rp, m.f.pp, m.f.th, m.f.qv, m.f.qc, m.f.qr};
+    const DevFields& f = m.f();
+    void* ds[9] = {f.u, f.v, f.w, f.rp, f.pp, f.th, f.qv, f.qc, f.qr};
     for (int q = 0; q < 9; ++q)
     {
         if (!hs[q]) return fail(NIMBUS_EINVAL, "upload: null field");
@@ -376,12 +582,13 @@ int upload_state(nimbus_hip_ctx* ctx, Member& m, const nimbus_host_state& s)
         if (rc != NIMBUS_OK) return rc;
         NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
-    const std::size_t cols = std::size_t(m.g.nx) * m.g.ny;
+    const std::size_t cols = std::size_t(m.g().nx) * m.g().ny;
     std::vector<double> rain(cols, 0.0);
     if (s.rain_mm)
-        for (int i = 0; i < m.g.nx; ++i)
-            for (int j = 0; j < m.g.ny; ++j) rain[std::size_t(j) * m.g.nx + i] = s.rain_mm[std::size_t(i) * m.g.ny + j];
-    NIMBUS_HIP_TRY(hipMemcpy(m.f.rain, rain.data(), cols * sizeof(double), hipMemcpyHostToDevice));
+        for (int i = 0; i < m.g().nx; ++i)
+            for (int j = 0; j < m.g().ny; ++j)
+                rain[std::size_t(j) * m.g().nx + i] = s.rain_mm[std::size_t(i) * m.g().ny + j];
+    NIMBUS_HIP_TRY(hipMemcpy(f.rain, rain.data(), cols * sizeof(double), hipMemcpyHostToDevice));
     return NIMBUS_OK;
 }
 
@@ -389,10 +596,11 @@ template <class H>
 int download_state(nimbus_hip_ctx* ctx, Member& m, const nimbus_host_state& s)
 {
     Staging st;
-    const std::size_t n = std::size_t(m.g.nx) * m.g.ny * m.g.nz;
+    const std::size_t n = std::size_t(m.g().nx) * m.g().ny * m.g().nz;
     NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(H)));
     void* hs[9] = {s.u, s.v, s.w, s.rho, s.p, s.theta, s.qv, s.qc, s.qr};
-    void* ds[9] = {m.f.u, m.f.v, m.f.w, m.f.rp, m.f.pp, m.f.th, m.f.qv, m.f.qc, m.f.qr};
+    const DevFields& f = m.f();
+    void* ds[9] = {f.u, f.v, f.w, f.rp, f.pp, f.th, f.qv, f.qc, f.qr};
     for (int q = 0; q < 9; ++q)
     {
         if (!hs[q]) continue;
@@ -402,91 +610,35 @@ int download_state(nimbus_hip_ctx* ctx, Member& m, const nimbus_host_state& s)
     }
     if (s.rain_mm)
     {
-        const std::size_t cols = std::size_t(m.g.nx) * m.g.ny;
+        const std::size_t cols = std::size_t(m.g().nx) * m.g().ny;
         std::vector<double> rain(cols);
-        NIMBUS_HIP_TRY(hipMemcpy(rain.data(), m.f.rain, cols * sizeof(double), hipMemcpyDeviceToHost));
-        for (int i = 0; i < m.g.nx; ++i)
-            for (int j = 0; j < m.g.ny; ++j) s.rain_mm[std::size_t(i) * m.g.ny + j] = rain[std::size_t(j) * m.g.nx + i];
+        NIMBUS_HIP_TRY(hipMemcpy(rain.data(), f.rain, cols * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < m.g().nx; ++i)
+            for (int j = 0; j < m.g().ny; ++j)
+                s.rain_mm[std::size_t(i) * m.g().ny + j] = rain[std::size_t(j) * m.g().nx + i];
     }
     return NIMBUS_OK;
 }
 
-void enqueue_step(Launcher& L, Member& m)
-{
-    if (m.elem == 4) launch_large_step<float>(L, m.g, m.f, m.params);
-    else launch_large_step<double>(L, m.g, m.f, m.params);
-}
-
-int run_member_step(nimbus_hip_ctx* ctx, Member& m)
-{
-    if (ctx->graphs && !ctx->timing)
-    {
-        if (!m.graph_exec)
-        {
-            Launcher L = make_launcher(ctx);
-            L.timing = false;
-            NIMBUS_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(L, m);
-            hipGraph_t graph = nullptr;
-            const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-            if (L.error != hipSuccess) return hip_fail(L.error, "step capture launch");
-            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-            m.graph = graph;
-            NIMBUS_HIP_TRY(hipGraphInstantiate(&m.graph_exec, m.graph, nullptr, nullptr, 0));
-        }
-        NIMBUS_HIP_TRY(hipGraphLaunch(m.graph_exec, ctx->stream));
-    }
-    else
-    {
-        Launcher L = make_launcher(ctx);
-        enqueue_step(L, m);
-        if (L.error != hipSuccess) return hip_fail(L.error, "step launch");
-    }
-    ++m.steps;
-    return NIMBUS_OK;
-}
-
-int step_members(nimbus_hip_ctx* ctx, const std::uint32_t* ids, int n_ids, int n_steps, bool wait)
-{
-    if (!ctx || (!ids && n_ids > 0) || n_ids < 0 || n_steps < 0) return fail(NIMBUS_EINVAL, "step: bad arguments");
-    std::vector<Member*> ms;
-    for (int q = 0; q < n_ids; ++q)
-    {
-        Member* m = find(ctx, ids[q]);
-        if (!m) return fail(NIMBUS_ENOENT, "step: unknown member");
-        ms.push_back(m);
-    }
-    NIMBUS_HIP_TRY(hipEventRecord(ctx->step_a, ctx->stream));
-    for (int s = 0; s < n_steps; ++s)
-        for (Member* m : ms)
-        {
-            const int rc = run_member_step(ctx, *m);
-            if (rc != NIMBUS_OK) return rc;
-        }
-    NIMBUS_HIP_TRY(hipEventRecord(ctx->step_b, ctx->stream));
-    if (wait)
-    {
-        NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        float ms = 0.0f;
-        NIMBUS_HIP_TRY(hipEventElapsedTime(&ms, ctx->step_a, ctx->step_b));
-        ctx->last_step_ms = ms;
-        harvest(ctx);
-    }
-    return NIMBUS_OK;
-}
-
-// A scratch member for one per-operation dispatch: mixed precision, given
-// grid and base profiles, destroyed when the dispatch returns.
+// A scratch member for one per-operation dispatch (mixed precision, given
+// grid and base profiles) with its own one-member batch; both are released
+// when the dispatch returns.
 struct ScratchMember
 {
-    nimbus_hip_ctx* ctx;
     Member m;
+    Batch batch;
     ~ScratchMember() { free_member(m); }
+    int finish_setup()
+    {
+        batch.elem = m.elem;
+        std::vector<Member*> one{&m};
+        return build_batch(batch, one);
+    }
 };
 
-int make_scratch(nimbus_hip_ctx* ctx, ScratchMember& sm, int nx, int ny, int nz, double dx, double dy, double dz,
-                 const double* rho0, const double* p0, const double* th0, const double* qv0, const double* u0,
-                 const double* v0, int acoustic_mode)
+int make_scratch(ScratchMember& sm, int nx, int ny, int nz, double dx, double dy, double dz, const double* rho0,
+                 const double* p0, const double* th0, const double* qv0, const double* u0, const double* v0,
+                 int acoustic_mode)
 {
     nimbus_member_desc d{};
     d.grid = nimbus_grid{nx, ny, nz, dx, dy, dz};
@@ -498,7 +650,6 @@ int make_scratch(nimbus_hip_ctx* ctx, ScratchMember& sm, int nx, int ny, int nz,
     d.precision = NIMBUS_PRECISION_MIXED;
     d.base = nimbus_base_state{rho0, p0, th0, qv0, u0, v0};
     if (!valid_grid(d.grid)) return fail(NIMBUS_EINVAL, "dispatch: invalid grid");
-    sm.ctx = ctx;
     return allocate_member(d, sm.m);
 }
 
@@ -532,8 +683,6 @@ int nimbus_hip_create(int device, nimbus_hip_ctx** out)
     NIMBUS_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     NIMBUS_HIP_TRY(hipEventCreate(&ctx->step_a));
     NIMBUS_HIP_TRY(hipEventCreate(&ctx->step_b));
-    NIMBUS_HIP_TRY(hipMalloc(&ctx->diag_partials, sizeof(double) * kDiagBlocks * kDiagValues));
-    NIMBUS_HIP_TRY(hipMalloc(&ctx->diag_out, sizeof(double) * kDiagValues));
     *out = ctx.release();
     return NIMBUS_OK;
 }
@@ -543,14 +692,13 @@ int nimbus_hip_destroy(nimbus_hip_ctx* ctx)
     if (!ctx) return NIMBUS_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    ctx->batches.clear();
     for (auto& kv : ctx->members) free_member(*kv.second);
     ctx->members.clear();
     harvest(ctx);
     for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
     if (ctx->step_a) hipEventDestroy(ctx->step_a);
     if (ctx->step_b) hipEventDestroy(ctx->step_b);
-    if (ctx->diag_partials) hipFree(ctx->diag_partials);
-    if (ctx->diag_out) hipFree(ctx->diag_out);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return NIMBUS_OK;
@@ -607,6 +755,12 @@ int nimbus_hip_member_destroy(nimbus_hip_ctx* ctx, uint32_t id)
     Member* m = find(ctx, id);
     if (!m) return fail(NIMBUS_ENOENT, "member_destroy: unknown member");
     hipStreamSynchronize(ctx->stream);
+    for (auto it = ctx->batches.begin(); it != ctx->batches.end();)
+    {
+        const auto& key = it->first.second;
+        if (std::find(key.begin(), key.end(), id) != key.end()) it = ctx->batches.erase(it);
+        else ++it;
+    }
     free_member(*m);
     ctx->members.erase(id);
     return NIMBUS_OK;
@@ -618,6 +772,7 @@ static int member_io(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* 
     Member* m = find(ctx, id);
     if (!m) return fail(NIMBUS_ENOENT, "member io: unknown member");
     hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     if (up) return elem_host == 4 ? upload_state<float>(ctx, *m, *s) : upload_state<double>(ctx, *m, *s);
     return elem_host == 4 ? download_state<float>(ctx, *m, *s) : download_state<double>(ctx, *m, *s);
 }
@@ -640,15 +795,22 @@ int nimbus_hip_member_diag(nimbus_hip_ctx* ctx, uint32_t id, nimbus_diag* d)
     if (!ctx || !d) return fail(NIMBUS_EINVAL, "diag: null argument");
     Member* m = find(ctx, id);
     if (!m) return fail(NIMBUS_ENOENT, "diag: unknown member");
+    hipSetDevice(ctx->device);
+    Batch* b = nullptr;
+    std::vector<Member*> one{m};
+    std::vector<std::uint32_t> ids{id};
+    int rc = get_batch(ctx, one, ids, &b);
+    if (rc != NIMBUS_OK) return rc;
     Launcher L = make_launcher(ctx);
     L.timing = false;
-    if (m->elem == 4) launch_diagnostics<float>(L, m->g, m->f, ctx->diag_partials, kDiagBlocks, ctx->diag_out);
-    else launch_diagnostics<double>(L, m->g, m->f, ctx->diag_partials, kDiagBlocks, ctx->diag_out);
+    if (m->elem == 4) launch_diagnostics<float>(L, b->view);
+    else launch_diagnostics<double>(L, b->view);
     if (L.error != hipSuccess) return hip_fail(L.error, "diag launch");
     double v[kDiagValues];
-    NIMBUS_HIP_TRY(hipMemcpyAsync(v, ctx->diag_out, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    NIMBUS_HIP_TRY(hipMemcpyAsync(v, m->f().diag + std::size_t(kDiagBlocksPerMember) * kDiagValues, sizeof(v),
+                                  hipMemcpyDeviceToHost, ctx->stream));
     NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const double cells = double(m->g.nx - 2) * double(m->g.ny - 2) * double(m->g.nz);
+    const double cells = double(m->g().nx - 2) * double(m->g().ny - 2) * double(m->g().nz);
     d->max_w = v[0];
     d->min_w = v[1];
     d->rms_theta_p = std::sqrt(v[2] / cells);
@@ -682,21 +844,20 @@ int nimbus_hip_synchronize(nimbus_hip_ctx* ctx)
     return NIMBUS_OK;
 }
 
-int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int xcd_remap)
+int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int k_chunks, int xcd_remap)
 {
     if (!ctx) return fail(NIMBUS_EINVAL, "set_tiling: null context");
     Member* m = find(ctx, id);
     if (!m) return fail(NIMBUS_ENOENT, "set_tiling: unknown member");
-    if (tile_w != 16 && tile_w != 32 && tile_w != 64 && tile_w != 128 && tile_w != 256)
-        return fail(NIMBUS_EINVAL, "set_tiling: tile width must be 16, 32, 64, 128 or 256");
-    hipStreamSynchronize(ctx->stream);
-    set_tiling(m->g, tile_w, xcd_remap);
-    release_graph(*m);
+    if (tile_w != 32 && tile_w != 64) return fail(NIMBUS_EINVAL, "set_tiling: tile width must be 32 or 64");
+    if (k_chunks < 1 || k_chunks > m->g().nz) return fail(NIMBUS_EINVAL, "set_tiling: k_chunks must be in 1..nz");
+    drop_batches(ctx);
+    set_geometry(*m, tile_w, k_chunks, xcd_remap);
     return NIMBUS_OK;
 }
 
-int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int* best_tile_w, int* best_remap,
-                               double* best_ms)
+int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int* best_tile_w, int* best_k_chunks,
+                               int* best_remap, double* best_ms)
 {
     if (!ctx || trials < 1) return fail(NIMBUS_EINVAL, "autotune: bad arguments");
     Member* m = find(ctx, id);
@@ -711,35 +872,42 @@ int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int
     const bool timing = ctx->timing;
     ctx->timing = false;
     double best = 1e300;
-    int bw = m->g.tile_w, br = m->g.xcd_remap;
+    int bw = m->tile_w, bk = m->k_chunks, br = m->xcd_remap;
     const std::uint32_t ids[1] = {id};
-    for (int w : {32, 64, 128, 256, 16})
-        for (int r : {0, 1})
-        {
-            set_tiling(m->g, w, r);
-            release_graph(*m);
-            int rc = step_members(ctx, ids, 1, 1, true);  // warm-up (and graph capture)
-            if (rc == NIMBUS_OK) rc = step_members(ctx, ids, 1, trials, true);
-            if (rc != NIMBUS_OK)
+    const int nz = m->g().nz;
+    std::vector<int> chunks;
+    for (int c : {1, 2, 4, 6, 8, 12, 16})
+        if (c <= std::max(1, nz / 4)) chunks.push_back(c);
+    for (int w : {64, 32})
+        for (int c : chunks)
+            for (int r : {1, 0})
             {
-                ctx->timing = timing;
-                return rc;
+                drop_batches(ctx);
+                set_geometry(*m, w, c, r);
+                int rc = step_members(ctx, ids, 1, 1, true);  // warm-up (and graph capture)
+                if (rc == NIMBUS_OK) rc = step_members(ctx, ids, 1, trials, true);
+                if (rc != NIMBUS_OK)
+                {
+                    ctx->timing = timing;
+                    return rc;
+                }
+                const double ms = ctx->last_step_ms / trials;
+                if (ms < best)
+                {
+                    best = ms;
+                    bw = w;
+                    bk = c;
+                    br = r;
+                }
+                NIMBUS_HIP_TRY(hipMemcpyAsync(m->block, saved, m->bytes, hipMemcpyDeviceToDevice, ctx->stream));
             }
-            const double ms = ctx->last_step_ms / trials;
-            if (ms < best)
-            {
-                best = ms;
-                bw = w;
-                br = r;
-            }
-            NIMBUS_HIP_TRY(hipMemcpyAsync(m->block, saved, m->bytes, hipMemcpyDeviceToDevice, ctx->stream));
-        }
     ctx->timing = timing;
     NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
     m->steps = steps0;
-    set_tiling(m->g, bw, br);
-    release_graph(*m);
+    drop_batches(ctx);
+    set_geometry(*m, bw, bk, br);
     if (best_tile_w) *best_tile_w = bw;
+    if (best_k_chunks) *best_k_chunks = bk;
     if (best_remap) *best_remap = br;
     if (best_ms) *best_ms = best;
     return NIMBUS_OK;
@@ -767,10 +935,7 @@ int nimbus_hip_kernel_timing(nimbus_hip_ctx* ctx, int max_rows, char (*names)[48
     for (auto& kv : ctx->table)
     {
         if (r >= max_rows) break;
-        if (names)
-        {
-            std::snprintf(names[r], 48, "%s", kv.first.c_str());
-        }
+        if (names) std::snprintf(names[r], 48, "%s", kv.first.c_str());
         if (launches) launches[r] = kv.second.launches;
         if (total_ms) total_ms[r] = kv.second.ms;
         ++r;
@@ -804,20 +969,28 @@ int nimbus_hip_dispatch_acoustic_substep_fused(nimbus_hip_ctx* ctx, const nimbus
         !(d->wind_clamp_v >= 0.0f) || std::fabs(d->gamma_host - kGamma) > 1e-12)
         return fail(NIMBUS_EINVAL, "acoustic: invalid dispatch");
     hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     ScratchMember sm{};
-    int rc = make_scratch(ctx, sm, d->nx, d->ny, d->nz, d->dx_host, d->dy_host, d->dz_host, d->rho0_base_host,
+    int rc = make_scratch(sm, d->nx, d->ny, d->nz, d->dx_host, d->dy_host, d->dz_host, d->rho0_base_host,
                           d->p0_base_host, nullptr, nullptr, nullptr, nullptr, NIMBUS_ACOUSTIC_EXPLICIT);
     if (rc != NIMBUS_OK) return rc;
     Member& m = sm.m;
-    m.params.bounds = Bounds{d->rho_min, d->rho_max, d->p_min, d->p_max, d->p_ref_fallback, d->wind_clamp_h,
-                             d->wind_clamp_v};
+    StepParams& p = m.args.p;
+    p.bounds = Bounds{d->rho_min, d->rho_max, d->p_min, d->p_max, d->p_ref_fallback, d->wind_clamp_h,
+                      d->wind_clamp_v};
+    p.substeps = 1;
+    p.dts = d->dt_small_host;
+    p.dt = d->dt_small_host;
+    p.mf_weight = 0.0;
+    if ((rc = sm.finish_setup()) != NIMBUS_OK) return rc;
     std::vector<float> zero(std::size_t(d->nx) * d->ny * d->nz, 0.0f);
     nimbus_host_state hs{d->u, d->v, d->w, d->rho, d->p, const_cast<float*>(d->theta_data), zero.data(), zero.data(),
                          zero.data(), nullptr};
     rc = upload_state<float>(ctx, m, hs);
     if (rc != NIMBUS_OK) return rc;
     Launcher L = make_launcher(ctx);
-    launch_acoustic_substep<float>(L, m.g, m.f, m.params, d->dt_small_host, 0.0, false, false);
+    launch_acoustic_substep<float>(L, sm.batch.view, 0, false, false);
+    launch_commit<float>(L, sm.batch.view, 1);
     if (L.error != hipSuccess) return hip_fail(L.error, "acoustic launch");
     nimbus_host_state out{d->u, d->v, d->w, d->rho, d->p, nullptr, nullptr, nullptr, nullptr, nullptr};
     rc = download_state<float>(ctx, m, out);
@@ -830,7 +1003,7 @@ int nimbus_hip_dispatch_acoustic_substeps_batched(nimbus_hip_ctx* ctx,
 {
     if (!ctx || !d) return fail(NIMBUS_EINVAL, "batched acoustic: null argument");
     if (!all_nonnull<const void*>({d->u, d->v, d->w, d->rho, d->p, d->theta_data, d->rho0_base_host, d->p0_base_host}) ||
-        d->n_substeps < 1 || !(d->dt_small > 0.0) ||
+        d->n_substeps < 1 || d->n_substeps > 10000 || !(d->dt_small > 0.0) ||
         (d->accumulate_mass_flux && !all_nonnull<const void*>({d->mass_flux_x, d->mass_flux_y, d->mass_flux_z})))
         return fail(NIMBUS_EINVAL, "batched acoustic: invalid dispatch");
     nimbus_step_config cfg{};
@@ -842,16 +1015,22 @@ int nimbus_hip_dispatch_acoustic_substeps_batched(nimbus_hip_ctx* ctx,
     cfg.vt_max = 20.0;
     if (!valid_config(cfg)) return fail(NIMBUS_EINVAL, "batched acoustic: invalid boundary or mode");
     hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     ScratchMember sm{};
-    int rc = make_scratch(ctx, sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->rho0_base_host, d->p0_base_host,
-                          nullptr, nullptr, nullptr, nullptr, d->acoustic_mode);
+    int rc = make_scratch(sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->rho0_base_host, d->p0_base_host, nullptr,
+                          nullptr, nullptr, nullptr, d->acoustic_mode);
     if (rc != NIMBUS_OK) return rc;
     Member& m = sm.m;
-    StepParams& p = m.params;
+    StepParams& p = m.args.p;
     p.acoustic_mode = d->acoustic_mode;
-    p.hevi_beta = d->hevi_beta;
+    p.hevi_alpha = 0.5 * (1.0 + d->hevi_beta);
     p.bc_west = d->bc_west; p.bc_east = d->bc_east; p.bc_south = d->bc_south; p.bc_north = d->bc_north;
     p.bc_bottom = d->bc_bottom; p.bc_top = d->bc_top;
+    p.substeps = d->n_substeps;
+    p.dts = d->dt_small;
+    p.dt = d->dt_small * d->n_substeps;
+    p.mf_weight = d->mass_flux_weight;
+    if ((rc = sm.finish_setup()) != NIMBUS_OK) return rc;
     std::vector<float> zero(std::size_t(d->nx) * d->ny * d->nz, 0.0f);
     nimbus_host_state hs{d->u, d->v, d->w, d->rho, d->p, const_cast<float*>(d->theta_data), zero.data(), zero.data(),
                          zero.data(), nullptr};
@@ -860,10 +1039,11 @@ int nimbus_hip_dispatch_acoustic_substeps_batched(nimbus_hip_ctx* ctx,
     Staging st;
     const std::size_t n = zero.size();
     NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(float)));
+    const DevFields& f = m.f();
     if (d->accumulate_mass_flux)
     {
         const void* src[3] = {d->mass_flux_x, d->mass_flux_y, d->mass_flux_z};
-        void* dst[3] = {m.f.mfx, m.f.mfy, m.f.mfz};
+        void* dst[3] = {f.mfx, f.mfy, f.mfz};
         for (int q = 0; q < 3; ++q)
         {
             rc = upload_field<float>(ctx, m, src[q], dst[q], nullptr, st.p);
@@ -872,13 +1052,13 @@ int nimbus_hip_dispatch_acoustic_substeps_batched(nimbus_hip_ctx* ctx,
         }
     }
     Launcher L = make_launcher(ctx);
-    launch_seed_radiation<float>(L, m.g, m.f);
+    launch_seed_radiation<float>(L, sm.batch.view);
     for (int s = 0; s < d->n_substeps; ++s)
     {
-        launch_acoustic_substep<float>(L, m.g, m.f, p, d->dt_small, d->mass_flux_weight, false,
-                                       d->accumulate_mass_flux != 0);
-        launch_boundaries<float>(L, m.g, m.f, p, d->dt_small);
+        launch_acoustic_substep<float>(L, sm.batch.view, s, d->accumulate_mass_flux != 0, false);
+        launch_boundaries<float>(L, sm.batch.view, s);
     }
+    launch_commit<float>(L, sm.batch.view, d->n_substeps);
     if (L.error != hipSuccess) return hip_fail(L.error, "batched acoustic launch");
     nimbus_host_state out{d->u, d->v, d->w, d->rho, d->p, nullptr, nullptr, nullptr, nullptr, nullptr};
     rc = download_state<float>(ctx, m, out);
@@ -886,7 +1066,7 @@ int nimbus_hip_dispatch_acoustic_substeps_batched(nimbus_hip_ctx* ctx,
     if (d->accumulate_mass_flux)
     {
         void* dst[3] = {d->mass_flux_x, d->mass_flux_y, d->mass_flux_z};
-        const void* src[3] = {m.f.mfx, m.f.mfy, m.f.mfz};
+        const void* src[3] = {f.mfx, f.mfy, f.mfz};
         for (int q = 0; q < 3; ++q)
         {
             rc = download_field<float>(ctx, m, src[q], dst[q], nullptr, st.p);
@@ -904,13 +1084,15 @@ int nimbus_hip_dispatch_slow_tendencies(nimbus_hip_ctx* ctx, const nimbus_slow_t
                                    d->rho0_base, d->du_dt_data, d->dv_dt_data, d->dw_dt_data}))
         return fail(NIMBUS_EINVAL, "slow tendencies: null field");
     hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     std::vector<double> p0(d->nz, kP00);
     ScratchMember sm{};
-    int rc = make_scratch(ctx, sm, d->nr, d->nth, d->nz, d->dx, d->dy, d->dz, d->rho0_base, p0.data(), nullptr,
+    int rc = make_scratch(sm, d->nr, d->nth, d->nz, d->dx, d->dy, d->dz, d->rho0_base, p0.data(), nullptr,
                           d->qv0_base, d->u0_base, d->v0_base, NIMBUS_ACOUSTIC_EXPLICIT);
     if (rc != NIMBUS_OK) return rc;
     Member& m = sm.m;
-    m.params.coriolis_f = d->coriolis_f;
+    m.args.p.coriolis_f = d->coriolis_f;
+    if ((rc = sm.finish_setup()) != NIMBUS_OK) return rc;
     const std::size_t n = std::size_t(d->nr) * d->nth * d->nz;
     std::vector<float> zero(n, 0.0f);
     nimbus_host_state hs{const_cast<float*>(d->u_data), const_cast<float*>(d->v_data), const_cast<float*>(d->w_data),
@@ -918,13 +1100,14 @@ int nimbus_hip_dispatch_slow_tendencies(nimbus_hip_ctx* ctx, const nimbus_slow_t
                          const_cast<float*>(d->qc_data), const_cast<float*>(d->qr_data), nullptr};
     rc = upload_state<float>(ctx, m, hs);
     if (rc != NIMBUS_OK) return rc;
+    const DevFields& f = m.f();
     Launcher L = make_launcher(ctx);
-    launch_slow_tendencies<float>(L, m.g, m.f, m.params, m.f.k1u, m.f.k1v, m.f.k1w);
+    launch_slow_tendencies<float>(L, sm.batch.view, f.k1u, f.k1v, f.k1w);
     if (L.error != hipSuccess) return hip_fail(L.error, "slow tendencies launch");
     Staging st;
     NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(float)));
     void* outs[3] = {d->du_dt_data, d->dv_dt_data, d->dw_dt_data};
-    void* srcs[3] = {m.f.k1u, m.f.k1v, m.f.k1w};
+    void* srcs[3] = {f.k1u, f.k1v, f.k1w};
     for (int q = 0; q < 3; ++q)
     {
         rc = download_field<float>(ctx, m, srcs[q], outs[q], nullptr, st.p);
@@ -940,14 +1123,16 @@ int nimbus_hip_dispatch_slow_rk2(nimbus_hip_ctx* ctx, const nimbus_slow_rk2_disp
     if (!all_nonnull<const void*>({d->u, d->v, d->w, d->rho, d->qv, d->qc, d->qr, d->rho0_base}) || !(d->dt_large > 0))
         return fail(NIMBUS_EINVAL, "slow rk2: invalid dispatch");
     hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     std::vector<double> p0(d->nz, kP00);
     ScratchMember sm{};
-    int rc = make_scratch(ctx, sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->rho0_base, p0.data(), nullptr,
+    int rc = make_scratch(sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->rho0_base, p0.data(), nullptr,
                           d->qv0_base, d->u0_base, d->v0_base, NIMBUS_ACOUSTIC_EXPLICIT);
     if (rc != NIMBUS_OK) return rc;
     Member& m = sm.m;
-    m.params.coriolis_f = d->coriolis_f;
-    m.params.dt = d->dt_large;
+    m.args.p.coriolis_f = d->coriolis_f;
+    m.args.p.dt = d->dt_large;
+    if ((rc = sm.finish_setup()) != NIMBUS_OK) return rc;
     const std::size_t n = std::size_t(d->nx) * d->ny * d->nz;
     std::vector<float> zero(n, 0.0f);
     nimbus_host_state hs{d->u, d->v, d->w, const_cast<float*>(d->rho), zero.data(), zero.data(),
@@ -955,7 +1140,7 @@ int nimbus_hip_dispatch_slow_rk2(nimbus_hip_ctx* ctx, const nimbus_slow_rk2_disp
     rc = upload_state<float>(ctx, m, hs);
     if (rc != NIMBUS_OK) return rc;
     Launcher L = make_launcher(ctx);
-    launch_slow_rk2<float>(L, m.g, m.f, m.params);
+    launch_slow_rk2<float>(L, sm.batch.view);
     if (L.error != hipSuccess) return hip_fail(L.error, "slow rk2 launch");
     nimbus_host_state out{d->u, d->v, d->w, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     rc = download_state<float>(ctx, m, out);
@@ -979,29 +1164,31 @@ int nimbus_hip_dispatch_advect_scalars(nimbus_hip_ctx* ctx, const nimbus_advect_
     if (npd > 3) return fail(NIMBUS_EINVAL, "advect: at most three positive-definite scalars");
     if (d->n_scalars == 0) return NIMBUS_OK;
     hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     std::vector<double> rho0(d->nz, 0.0), p0(d->nz, kP00);
     ScratchMember sm{};
-    int rc = make_scratch(ctx, sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, rho0.data(), p0.data(), nullptr,
-                          nullptr, nullptr, nullptr, NIMBUS_ACOUSTIC_EXPLICIT);
-    if (rc == NIMBUS_EINVAL) return rc;
+    int rc = make_scratch(sm, d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, rho0.data(), p0.data(), nullptr, nullptr,
+                          nullptr, nullptr, NIMBUS_ACOUSTIC_EXPLICIT);
     if (rc != NIMBUS_OK) return rc;
     Member& m = sm.m;
-    m.params.dt = d->dt;
-    m.params.bc_top = d->bc_top;
+    m.args.p.dt = d->dt;
+    m.args.p.bc_top = d->bc_top;
+    if ((rc = sm.finish_setup()) != NIMBUS_OK) return rc;
     const std::size_t n = std::size_t(d->nx) * d->ny * d->nz;
     Staging st;
     NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(float)));
+    const DevFields& f = m.f();
     // rho0 = 0 in the scratch member, so rho' slots hold the full densities.
     const void* src[5] = {d->mass_flux_x, d->mass_flux_y, d->mass_flux_z, d->rho_begin, d->rho_end};
-    void* dst[5] = {m.f.mfx, m.f.mfy, m.f.mfz, m.f.rb, m.f.rp};
+    void* dst[5] = {f.mfx, f.mfy, f.mfz, f.rb, f.rp};
     for (int q = 0; q < 5; ++q)
     {
         rc = upload_field<float>(ctx, m, src[q], dst[q], nullptr, st.p);
         if (rc != NIMBUS_OK) return rc;
         NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
-    void* qin[4] = {m.f.th, m.f.qv, m.f.qc, m.f.qr};
-    void* qout[4] = {m.f.tth, m.f.tqv, m.f.tqc, m.f.tqr};
+    void* qin[4] = {f.th, f.qv, f.qc, f.qr};
+    void* qout[4] = {f.tth, f.tqv, f.tqc, f.tqr};
     int pd[4] = {0, 0, 0, 0};
     for (int s = 0; s < d->n_scalars; ++s)
     {
@@ -1011,7 +1198,7 @@ int nimbus_hip_dispatch_advect_scalars(nimbus_hip_ctx* ctx, const nimbus_advect_
         pd[s] = d->positive_definite[s] != 0;
     }
     Launcher L = make_launcher(ctx);
-    launch_transport<float>(L, m.g, m.f, m.params, d->n_scalars, qin, qout, pd);
+    launch_transport<float>(L, sm.batch.view, d->n_scalars, pd, true);
     if (L.error != hipSuccess) return hip_fail(L.error, "advect launch");
     for (int s = 0; s < d->n_scalars; ++s)
     {
@@ -1030,7 +1217,7 @@ int nimbus_hip_dispatch_kessler(nimbus_hip_ctx* ctx, const nimbus_kessler_dispat
         !(d->dt > 0.0) || !(d->dz > 0.0) || !(d->vt_max > 0.0))
         return fail(NIMBUS_EINVAL, "kessler: invalid dispatch");
     hipSetDevice(ctx->device);
-    std::vector<double> rho0(d->nz, 0.0), p0(d->nz, 0.0);
+    hipStreamSynchronize(ctx->stream);
     ScratchMember sm{};
     // p0 = rho0 = 0: the perturbation slots carry the full fields.
     nimbus_member_desc desc{};
@@ -1041,12 +1228,13 @@ int nimbus_hip_dispatch_kessler(nimbus_hip_ctx* ctx, const nimbus_kessler_dispat
     desc.precision = NIMBUS_PRECISION_MIXED;
     std::vector<double> one(d->nz, 1.0);
     desc.base = nimbus_base_state{one.data(), one.data(), nullptr, nullptr, nullptr, nullptr};
-    sm.ctx = ctx;
     int rc = allocate_member(desc, sm.m);
     if (rc != NIMBUS_OK) return rc;
     Member& m = sm.m;
-    NIMBUS_HIP_TRY(hipMemset(m.f.base, 0, sizeof(double) * kBaseCount * d->nz));
-    m.params.kessler = KesslerConst{d->qc0, d->c_auto, d->c_accr, d->evap_eff, d->vt_max};
+    const DevFields& f = m.f();
+    NIMBUS_HIP_TRY(hipMemset(f.base, 0, sizeof(double) * kBaseCount * d->nz));
+    m.args.p.kessler = KesslerConst{d->qc0, d->c_auto, d->c_accr, d->evap_eff, d->vt_max};
+    if ((rc = sm.finish_setup()) != NIMBUS_OK) return rc;
     const std::size_t n = std::size_t(d->nx) * d->ny * d->nz;
     std::vector<float> zero(n, 0.0f);
     nimbus_host_state hs{zero.data(), zero.data(), zero.data(), const_cast<float*>(d->rho), const_cast<float*>(d->p),
@@ -1056,12 +1244,12 @@ int nimbus_hip_dispatch_kessler(nimbus_hip_ctx* ctx, const nimbus_kessler_dispat
     rc = upload_state<float>(ctx, m, hs);
     if (rc != NIMBUS_OK) return rc;
     Launcher L = make_launcher(ctx);
-    launch_kessler_tendencies<float>(L, m.g, m.f, m.params, d->dt, m.f.k1u, m.f.k1v, m.f.k1w, m.f.su);
+    launch_kessler_tendencies<float>(L, sm.batch.view, d->dt, f.k1u, f.k1v, f.k1w, f.u2);
     if (L.error != hipSuccess) return hip_fail(L.error, "kessler launch");
     Staging st;
     NIMBUS_HIP_TRY(hipMalloc(&st.p, n * sizeof(float)));
     void* outs[4] = {d->dtheta_dt, d->dqv_dt, d->dqc_dt, d->dqr_dt};
-    void* srcs[4] = {m.f.k1u, m.f.k1v, m.f.k1w, m.f.su};
+    void* srcs[4] = {f.k1u, f.k1v, f.k1w, f.u2};
     for (int q = 0; q < 4; ++q)
     {
         rc = download_field<float>(ctx, m, srcs[q], outs[q], nullptr, st.p);

@@ -168,9 +168,10 @@ def load_library(path=LIB_PATH):
         "nimbus_hip_step_async": (ctypes.c_int, [vp, P(ctypes.c_uint32), ctypes.c_int, ctypes.c_int]),
         "nimbus_hip_synchronize": (ctypes.c_int, [vp]),
         "nimbus_hip_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
-        "nimbus_hip_member_set_tiling": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]),
+        "nimbus_hip_member_set_tiling": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                                        ctypes.c_int]),
         "nimbus_hip_member_autotune": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_int, P(ctypes.c_int),
-                                                      P(ctypes.c_int), P(ctypes.c_double)]),
+                                                      P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_double)]),
         "nimbus_hip_set_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "nimbus_hip_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int, vp, P(ctypes.c_uint64),
                                                     P(ctypes.c_double)]),
@@ -280,6 +281,13 @@ def initial_state(grid, params, dtype=np.float32):
     fill = lib.nimbus_ic_fill_f32 if np.dtype(dtype) == np.float32 else lib.nimbus_ic_fill_f64
     check(fill(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(hs)), "nimbus_ic_fill")
     return st
+
+
+def member_bytes(grid, config, base, precision=PRECISION_MIXED):
+    """Device bytes a member of this shape holds (admission planning; no device needed)."""
+    b = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in base.items()}
+    desc = MemberDesc(grid, config, precision, base_struct(b))
+    return int(load_library().nimbus_hip_member_bytes(ctypes.byref(desc)))
 
 
 def resolve_substeps(grid, base, config):
@@ -392,15 +400,18 @@ class Member:
         check(self.ctx.lib.nimbus_hip_member_diag(self.ctx.handle, self.id, ctypes.byref(d)), "diag")
         return {n: getattr(d, n) for n, _ in Diag._fields_}
 
-    def set_tiling(self, tile_w, xcd_remap):
-        check(self.ctx.lib.nimbus_hip_member_set_tiling(self.ctx.handle, self.id, int(tile_w), int(xcd_remap)),
-              "set_tiling")
+    def set_tiling(self, tile_w, k_chunks, xcd_remap):
+        """Launch geometry: tile_w in {32, 64}, k_chunks in 1..nz, XCD-contiguous order on/off."""
+        check(self.ctx.lib.nimbus_hip_member_set_tiling(self.ctx.handle, self.id, int(tile_w), int(k_chunks),
+                                                        int(xcd_remap)), "set_tiling")
 
     def autotune(self, trials=2):
-        w, r, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        """Times each launch-geometry candidate on a saved copy of the state; returns (tile_w, k_chunks, remap, ms)."""
+        w, c, r, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         check(self.ctx.lib.nimbus_hip_member_autotune(self.ctx.handle, self.id, int(trials), ctypes.byref(w),
-                                                      ctypes.byref(r), ctypes.byref(ms)), "autotune")
-        return w.value, r.value, ms.value
+                                                      ctypes.byref(c), ctypes.byref(r), ctypes.byref(ms)),
+              "autotune")
+        return w.value, c.value, r.value, ms.value
 
     def destroy(self):
         if self.id:

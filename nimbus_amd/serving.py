@@ -1,0 +1,233 @@
+"""Ensemble serving: continuous batching of member runs on one device.
+
+Requests (grid, perturbed sounding, simulated duration) arrive over time. A
+request is admitted as soon as it has arrived and its device footprint fits
+the HBM budget next to the members already resident (first come, first
+served: a request that does not fit holds back the ones behind it). All
+resident members advance together as one batch, so every kernel launch of a
+large step covers all of them; each round advances the batch by
+min(quantum, fewest steps any member still needs) large steps, after which
+finished members report their diagnostics, free their memory and make room
+for waiting requests. A member's arithmetic does not depend on what else is
+in the batch, so a result is bitwise the same alone and in the mix.
+
+The initial state of an arrived request is generated on a host thread while
+the device works on the current batch.
+"""
+import collections
+import concurrent.futures
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import lib as L
+from . import workloads as W
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of the run
+    workload: W.Workload    # grid, dt, substeps, steps, sounding, seed/member
+
+
+def poisson_requests(seed=4, rate=2.0, horizon=600.0, sizes=(48, 64, 96, 128, 160, 192, 224, 256),
+                     levels=(40, 60, 80), dx_range=(500.0, 1000.0), minutes=(15.0, 90.0), substeps=6):
+    """Seeded Poisson arrivals (rate per second over `horizon` seconds). Each
+    request draws nx = ny from `sizes`, nz from `levels`, dx uniform in
+    `dx_range` (dt = 6 dx / 1000 s, 20 km top), a perturbed sounding and a
+    simulated duration uniform in `minutes`."""
+    s = W.SeededStream(seed)
+    out = []
+    t = 0.0
+    while True:
+        t += -math.log(max(1.0 - s.uniform(), 1e-300)) / rate
+        if t > horizon:
+            break
+        n = s.choice(list(sizes))
+        nz = s.choice(list(levels))
+        dx = round(s.between(*dx_range), 1)
+        dt = 6.0 * dx / 1000.0
+        duration = 60.0 * s.between(*minutes)
+        rid = len(out)
+        wl = W.Workload(f"req{rid}", n, n, nz, dx, 20000.0 / nz, dt, substeps, max(1, int(round(duration / dt))),
+                        seed=seed, member=rid, sounding=W.perturbed_sounding(s))
+        out.append(Request(rid, t, wl))
+    return out
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    cells: int
+    steps: int
+    arrival: float
+    admitted: float
+    completed: float
+    diag: dict
+    fields: dict = None
+
+    @property
+    def latency(self):
+        return self.completed - self.arrival
+
+
+@dataclasses.dataclass
+class Report:
+    results: list
+    rejected: list
+    wall_seconds: float
+    device_step_seconds: float
+    rounds: int
+    peak_bytes: int
+    peak_members: int
+
+    @property
+    def cell_updates(self):
+        return sum(r.cells * r.steps for r in self.results)
+
+    def latency_percentile(self, q):
+        lat = sorted(r.latency for r in self.results)
+        if not lat:
+            return float("nan")
+        return float(np.percentile(np.array(lat), q))
+
+    def summary(self):
+        return {"requests": len(self.results), "rejected": len(self.rejected),
+                "cell_updates": self.cell_updates, "wall_seconds": round(self.wall_seconds, 3),
+                "cell_updates_per_s": self.cell_updates / self.wall_seconds if self.wall_seconds > 0 else 0.0,
+                "device_step_seconds": round(self.device_step_seconds, 3),
+                "p50_latency_s": round(self.latency_percentile(50), 3),
+                "p99_latency_s": round(self.latency_percentile(99), 3),
+                "rounds": self.rounds, "peak_resident_bytes": self.peak_bytes,
+                "peak_resident_members": self.peak_members}
+
+
+class _Slot:
+    __slots__ = ("req", "member", "remaining", "bytes", "admitted")
+
+    def __init__(self, req, member, nbytes, admitted):
+        self.req = req
+        self.member = member
+        self.remaining = req.workload.steps
+        self.bytes = nbytes
+        self.admitted = admitted
+
+
+class Server:
+    """Continuous-batching request loop over one device context."""
+
+    def __init__(self, ctx, budget_bytes=200e9, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI,
+                 quantum=16, graphs=True, host_workers=4):
+        self.ctx = ctx
+        self.budget = int(budget_bytes)
+        self.precision = precision
+        self.mode = acoustic_mode
+        self.quantum = int(quantum)
+        self.graphs = graphs
+        self.host_workers = host_workers
+        self.dtype = np.float32 if precision == L.PRECISION_MIXED else np.float64
+        self._bytes = {}
+
+    def footprint(self, req):
+        wl = req.workload
+        key = (wl.nx, wl.ny, wl.nz)
+        if key not in self._bytes:
+            self._bytes[key] = L.member_bytes(wl.grid, wl.config(acoustic_mode=self.mode), wl.base(),
+                                              self.precision)
+        return self._bytes[key]
+
+    def _prepare(self, req):
+        wl = req.workload
+        return wl.base(), wl.initial_state(self.dtype)
+
+    def run(self, requests, time_scale=1.0, keep_fields=(), clock=time.perf_counter, sleep=time.sleep):
+        """Serve `requests` (arrival times in seconds, compressed by `time_scale`).
+        Returns a Report; results in completion order. `keep_fields` lists
+        request ids whose final fields are downloaded."""
+        keep = set(keep_fields)
+        queue = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        waiting = collections.deque()
+        active = []
+        results, rejected = [], []
+        used = peak_bytes = peak_members = rounds = 0
+        dev_seconds = 0.0
+        pool = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, self.host_workers))
+        prepared = {}
+        self.ctx.set_graphs(self.graphs)
+        start = clock()
+        now = lambda: (clock() - start) * time_scale
+        try:
+            while queue or waiting or active:
+                t = now()
+                while queue and queue[0].arrival <= t:
+                    r = queue.popleft()
+                    prepared[r.rid] = pool.submit(self._prepare, r)
+                    waiting.append(r)
+                while waiting:
+                    r = waiting[0]
+                    nb = self.footprint(r)
+                    if nb > self.budget:
+                        rejected.append(r.rid)
+                        waiting.popleft()
+                        prepared.pop(r.rid).cancel()
+                        continue
+                    if used + nb > self.budget:
+                        break
+                    waiting.popleft()
+                    base, st = prepared.pop(r.rid).result()
+                    wl = r.workload
+                    m = self.ctx.member(wl.grid, wl.config(acoustic_mode=self.mode), base, self.precision)
+                    m.upload(st)
+                    active.append(_Slot(r, m, nb, now()))
+                    used += nb
+                peak_bytes = max(peak_bytes, used)
+                peak_members = max(peak_members, len(active))
+                if not active:
+                    if queue:
+                        wait = (queue[0].arrival - now()) / time_scale
+                        if wait > 0:
+                            sleep(min(wait, 0.05))
+                    continue
+                n = min(self.quantum, min(s.remaining for s in active))
+                self.ctx.step([s.member for s in active], n)
+                dev_seconds += self.ctx.last_step_ms() / 1e3
+                rounds += 1
+                still = []
+                for s in active:
+                    s.remaining -= n
+                    if s.remaining > 0:
+                        still.append(s)
+                        continue
+                    d = s.member.diag()
+                    fields = s.member.download() if s.req.rid in keep else None
+                    wl = s.req.workload
+                    results.append(Result(s.req.rid, wl.cells, wl.steps, s.req.arrival, s.admitted, now(), d,
+                                          fields))
+                    s.member.destroy()
+                    used -= s.bytes
+                active = still
+        finally:
+            for s in active:
+                s.member.destroy()
+            pool.shutdown(wait=True, cancel_futures=True)
+            self.ctx.set_graphs(False)
+        wall = (clock() - start)
+        return Report(results, rejected, wall, dev_seconds, rounds, peak_bytes, peak_members)
+
+
+def run_alone(ctx, req, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI, graphs=True):
+    """One request on its own (the reference for 'same alone and in the mix' and FP64 reruns)."""
+    wl = req.workload
+    dtype = np.float32 if precision == L.PRECISION_MIXED else np.float64
+    m = ctx.member(wl.grid, wl.config(acoustic_mode=acoustic_mode), wl.base(), precision)
+    ctx.set_graphs(graphs)
+    try:
+        m.upload(wl.initial_state(dtype))
+        ctx.step([m], wl.steps)
+        return m.download(), m.diag()
+    finally:
+        ctx.set_graphs(False)
+        m.destroy()

@@ -295,3 +295,82 @@ def test_member_alone_equals_member_in_mix(hip_ctx):
     for f in L.FIELDS:
         assert np.array_equal(mixed[f], alone[f]), f
     assert d_mixed["max_w"] == d_alone["max_w"] and d_mixed["rms_theta_p"] == d_alone["rms_theta_p"]
+
+
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_large_step_odd_substep_count_fp64(hip_ctx, mode):
+    """An odd substep count leaves the acoustic state in the second buffer; the
+    commit must bring it home before transport and microphysics."""
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, dtype=np.float64)
+    cfg = L.step_config(2.0, 5, acoustic_mode=mode)
+    assert L.resolve_substeps(g, base, cfg) % 2 == 1
+    n_steps = 3
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
+    out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, n_steps)
+    scale = {"u": 30.0, "v": 30.0, "w": 10.0, "rho": 1.0, "p": 1e5, "theta": 300.0, "qv": 0.02, "qc": 1e-3,
+             "qr": 1e-3}
+    for f in L.FIELDS:
+        err = max_abs(out[f], ref[f]) / scale[f]
+        print(f"odd substeps mode {mode} {f}: scaled max diff {err:.3e}")
+        assert err < 1e-9, f
+
+
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_launch_geometry_does_not_change_results(hip_ctx, mode):
+    """Tile width, vertical chunking and XCD ordering only change which
+    workgroup computes a cell: results are bitwise identical (nx > 64 so
+    several tiles and strips meet)."""
+    g, base, st = small_case(nx=70, ny=13, nz=17, cloudy=True, wind_noise=1.0)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode)
+    results = []
+    for w, kc, r in [(64, 1, 1), (32, 3, 0), (64, 7, 1), (32, 17, 1)]:
+        m = hip_ctx.member(g, cfg, base, L.PRECISION_MIXED)
+        try:
+            m.upload(st)
+            m.set_tiling(w, kc, r)
+            hip_ctx.step([m], 2)
+            results.append(m.download())
+        finally:
+            m.destroy()
+    for res in results[1:]:
+        for f in L.FIELDS + ["rain_mm"]:
+            assert np.array_equal(res[f], results[0][f]), f
+
+
+def test_autotune_restores_state(hip_ctx):
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True)
+    cfg = L.step_config(3.0, 6)
+    m = hip_ctx.member(g, cfg, base, L.PRECISION_MIXED)
+    try:
+        m.upload(st)
+        hip_ctx.step([m], 1)
+        before = m.download()
+        w, kc, r, ms = m.autotune(1)
+        after = m.download()
+        assert w in (32, 64) and 1 <= kc <= g.nz and r in (0, 1) and ms > 0
+        assert m.diag()["steps"] == 1
+        for f in L.FIELDS:
+            assert np.array_equal(before[f], after[f]), f
+    finally:
+        m.destroy()
+
+
+def test_fp64_and_mixed_members_in_one_call(hip_ctx):
+    g, base, st64 = small_case(nx=20, ny=18, nz=14, cloudy=True, dtype=np.float64)
+    st32 = {k: (v.astype(np.float32) if k in L.FIELDS else v) for k, v in st64.items()}
+    cfg = L.step_config(3.0, 6, acoustic_mode=L.ACOUSTIC_HEVI)
+    alone64, _ = _run_member(hip_ctx, g, base, cfg, st64, L.PRECISION_FP64, 2)
+    alone32, _ = _run_member(hip_ctx, g, base, cfg, st32, L.PRECISION_MIXED, 2)
+    a = hip_ctx.member(g, cfg, base, L.PRECISION_FP64)
+    b = hip_ctx.member(g, cfg, base, L.PRECISION_MIXED)
+    try:
+        a.upload(st64)
+        b.upload(st32)
+        hip_ctx.step([b, a], 2)
+        ra, rb = a.download(), b.download()
+    finally:
+        a.destroy()
+        b.destroy()
+    for f in L.FIELDS:
+        assert np.array_equal(ra[f], alone64[f]), f
+        assert np.array_equal(rb[f], alone32[f]), f

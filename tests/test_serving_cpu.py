@@ -1,0 +1,126 @@
+"""Serving loop logic on the CPU with a stand-in device context: admission by
+HBM budget in arrival order, rejection of requests larger than the budget,
+step accounting per member and latency percentiles. No device is touched."""
+import numpy as np
+import pytest
+
+from nimbus_amd import lib as L
+from nimbus_amd import serving as SV
+from nimbus_amd import workloads as W
+
+
+class FakeMember:
+    def __init__(self, ctx, grid, config, base, precision):
+        self.ctx = ctx
+        self.grid = grid
+        self.steps = 0
+        self.destroyed = False
+        self.uploaded = False
+        ctx.created.append(self)
+
+    def upload(self, st):
+        self.uploaded = True
+
+    def download(self):
+        return {"steps": self.steps}
+
+    def diag(self):
+        return {"steps": self.steps, "max_w": 0.0}
+
+    def destroy(self):
+        assert not self.destroyed
+        self.destroyed = True
+
+
+class FakeContext:
+    """Records batches; each step 'takes' 1 ms of device time."""
+
+    def __init__(self):
+        self.created = []
+        self.batches = []
+        self.ms = 0.0
+        self.graphs = False
+
+    def member(self, grid, config, base, precision=L.PRECISION_MIXED):
+        return FakeMember(self, grid, config, base, precision)
+
+    def step(self, members, n_steps=1, wait=True):
+        assert all(m.uploaded and not m.destroyed for m in members)
+        self.batches.append((len(members), n_steps))
+        for m in members:
+            m.steps += n_steps
+        self.ms = float(n_steps)
+
+    def last_step_ms(self):
+        return self.ms
+
+    def set_graphs(self, enable):
+        self.graphs = bool(enable)
+
+
+def small_requests(n, steps, arrival_gap=0.0, size=48, nz=40):
+    out = []
+    for r in range(n):
+        wl = W.Workload(f"t{r}", size, size, nz, 1000.0, 20000.0 / nz, 6.0, 6, steps[r % len(steps)],
+                        seed=9, member=r)
+        out.append(SV.Request(r, r * arrival_gap, wl))
+    return out
+
+
+def test_poisson_stream_is_seeded_and_in_range():
+    a = SV.poisson_requests(seed=4, rate=2.0, horizon=60.0)
+    b = SV.poisson_requests(seed=4, rate=2.0, horizon=60.0)
+    assert [r.arrival for r in a] == [r.arrival for r in b]
+    assert 80 <= len(a) <= 160  # 120 expected
+    assert all(0.0 < r.arrival <= 60.0 for r in a)
+    assert all(np.diff([r.arrival for r in a]) > 0)
+    for r in a:
+        wl = r.workload
+        assert 48 <= wl.nx <= 256 and wl.nx == wl.ny and wl.nz in (40, 60, 80)
+        assert 500.0 <= wl.dx <= 1000.0
+        assert 15 * 60 - wl.dt <= wl.steps * wl.dt <= 90 * 60 + wl.dt
+
+
+def test_admission_respects_budget_and_order():
+    reqs = small_requests(6, steps=[5, 9, 3])
+    ctx = FakeContext()
+    srv = SV.Server(ctx, quantum=4, host_workers=2)
+    one = srv.footprint(reqs[0])
+    srv.budget = 2 * one
+    rep = srv.run(reqs, time_scale=1e6, sleep=lambda s: None)
+    assert sorted(r.rid for r in rep.results) == list(range(6))
+    assert rep.rejected == []
+    assert rep.peak_members <= 2 and rep.peak_bytes <= 2 * one
+    assert max(n for n, _ in ctx.batches) == 2
+    # admission in arrival order: member objects were created in request order
+    assert [m.grid.nx for m in ctx.created] == [48] * 6
+    for res in rep.results:
+        assert res.diag["steps"] == res.steps
+        assert res.admitted >= res.arrival and res.completed >= res.admitted
+    # every round advanced by at most the quantum and never past a member's end
+    assert all(0 < n <= 4 for _, n in ctx.batches)
+    assert all(m.destroyed for m in ctx.created)
+
+
+def test_oversized_request_is_rejected():
+    reqs = small_requests(3, steps=[2])
+    big = W.Workload("big", 128, 128, 80, 1000.0, 250.0, 6.0, 6, 2, seed=1, member=7)
+    reqs.append(SV.Request(3, 0.0, big))
+    ctx = FakeContext()
+    srv = SV.Server(ctx, quantum=8, host_workers=1)
+    srv.budget = srv.footprint(reqs[0]) * 3
+    rep = srv.run(reqs, time_scale=1e6, sleep=lambda s: None)
+    assert rep.rejected == [3]
+    assert sorted(r.rid for r in rep.results) == [0, 1, 2]
+
+
+def test_latency_percentiles_and_throughput_accounting():
+    reqs = small_requests(10, steps=[3, 7], arrival_gap=0.001)
+    ctx = FakeContext()
+    srv = SV.Server(ctx, quantum=2, host_workers=2)
+    rep = srv.run(reqs, time_scale=1.0)
+    s = rep.summary()
+    assert s["requests"] == 10
+    assert 0.0 <= s["p50_latency_s"] <= s["p99_latency_s"]
+    assert s["cell_updates"] == sum(r.workload.cells * r.workload.steps for r in reqs)
+    assert rep.device_step_seconds == pytest.approx(sum(n for _, n in ctx.batches) / 1e3)
