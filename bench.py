@@ -12,7 +12,7 @@ ensemble run as one batch, and a short continuous-batching serving run with
 p50/p99 completion latency. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
-                  [--serving-seconds S] [--no-ensemble] [--no-cpu-baseline]
+                  [--serving-seconds S] [--no-ensemble] [--accuracy c1|c2|''] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -185,6 +185,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=20.0,
                     help="arrival window of the serving run (600 = the full 10-minute stream; 0 skips it)")
+    ap.add_argument("--accuracy", default="c1", help="workload of the 1-hour mixed-vs-FP64 error run ('' skips it)")
     ap.add_argument("--profile", choices=["hevi", "explicit"], default=None,
                     help="only the mixed member in this mode: warm-up + timed steps, for rocprofv3 passes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -274,6 +275,10 @@ def main():
         result["ensemble_c3"] = ensemble_run(ctx, L, W, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
     if args.serving_seconds > 0:
         result["serving_c4"] = serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
+    if args.accuracy:
+        from nimbus_amd import accuracy as ACC
+        acc = ACC.one_hour(ctx, W.NAMED[args.accuracy], L.ACOUSTIC_HEVI)
+        result["accuracy_1h"] = acc
     ctx.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(wl)

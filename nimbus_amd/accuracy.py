@@ -1,0 +1,93 @@
+"""Error of the mixed-precision path against the all-FP64 path after a long run.
+
+The FP64 device path is the reference here: it agrees with the CPU FP64 oracle
+to ~1e-9 of each field's scale over several large steps, and a simulated hour
+of the 160x160x80 or 400x400x100 workloads is out of the CPU oracle's reach
+within a run. Both runs start from the same seeded state; errors are reported
+for every output field and for the run-level diagnostics (domain maximum of
+w, RMS theta', accumulated surface rain).
+
+  python -m nimbus_amd.accuracy [--workload c1] [--mode hevi|explicit] [--steps N]
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import lib as L
+from . import workloads as W
+
+
+def run(ctx, wl, precision, mode, steps, graphs=True):
+    dtype = np.float32 if precision == L.PRECISION_MIXED else np.float64
+    m = ctx.member(wl.grid, wl.config(acoustic_mode=mode), wl.base(), precision)
+    try:
+        m.upload(wl.initial_state(dtype))
+        ctx.set_graphs(graphs)
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:  # chunks keep each call short
+            n = min(200, steps - done)
+            ctx.step([m], n)
+            done += n
+        seconds = time.perf_counter() - t0
+        return m.download(), m.diag(), seconds
+    finally:
+        ctx.set_graphs(False)
+        m.destroy()
+
+
+def compare(wl, mixed, fp64, d_mixed, d_fp64):
+    """Per-field and diagnostic errors of `mixed` against `fp64` on the physical cells."""
+    base = wl.base()
+    phys = (slice(1, -1), slice(1, -1), slice(None))
+    out = {"fields_max_abs": {}, "fields_rms": {}}
+    for f in L.FIELDS:
+        d = np.asarray(mixed[f], np.float64)[phys] - np.asarray(fp64[f], np.float64)[phys]
+        out["fields_max_abs"][f] = float(np.max(np.abs(d)))
+        out["fields_rms"][f] = float(np.sqrt(np.mean(d * d)))
+    th0 = base["theta0"][None, None, :]
+    tp_m = np.asarray(mixed["theta"], np.float64)[phys] - th0
+    tp_d = np.asarray(fp64["theta"], np.float64)[phys] - th0
+    rain_m = float(np.sum(mixed["rain_mm"][1:-1, 1:-1]))
+    rain_d = float(np.sum(fp64["rain_mm"][1:-1, 1:-1]))
+    out.update({
+        "max_w_mixed": d_mixed["max_w"], "max_w_fp64": d_fp64["max_w"],
+        "max_w_error": abs(d_mixed["max_w"] - d_fp64["max_w"]),
+        "w_pointwise_max_error": out["fields_max_abs"]["w"],
+        "rms_theta_p_mixed": d_mixed["rms_theta_p"], "rms_theta_p_fp64": d_fp64["rms_theta_p"],
+        "rms_theta_p_error": abs(d_mixed["rms_theta_p"] - d_fp64["rms_theta_p"]),
+        "theta_p_rms_difference": float(np.sqrt(np.mean((tp_m - tp_d) ** 2))),
+        "rain_total_mm_mixed": rain_m, "rain_total_mm_fp64": rain_d,
+        "rain_relative_error": abs(rain_m - rain_d) / max(abs(rain_d), 1e-30),
+    })
+    out["targets_met"] = bool(out["max_w_error"] < 0.5 and out["rms_theta_p_error"] < 0.05)
+    return out
+
+
+def one_hour(ctx, wl, mode=L.ACOUSTIC_HEVI, steps=None):
+    steps = int(steps or round(3600.0 / wl.dt))
+    mixed, d_mixed, t_m = run(ctx, wl, L.PRECISION_MIXED, mode, steps)
+    fp64, d_fp64, t_d = run(ctx, wl, L.PRECISION_FP64, mode, steps)
+    res = compare(wl, mixed, fp64, d_mixed, d_fp64)
+    res.update({"workload": wl.describe(), "simulated_seconds": steps * wl.dt, "large_steps": steps,
+                "acoustic_mode": "hevi" if mode == L.ACOUSTIC_HEVI else "explicit",
+                "wall_seconds_mixed": round(t_m, 3), "wall_seconds_fp64": round(t_d, 3)})
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--mode", default="hevi", choices=["hevi", "explicit"])
+    ap.add_argument("--steps", type=int, default=0)
+    args = ap.parse_args()
+    ctx = L.Context(0)
+    mode = L.ACOUSTIC_HEVI if args.mode == "hevi" else L.ACOUSTIC_EXPLICIT
+    print(json.dumps(one_hour(ctx, W.NAMED[args.workload], mode, args.steps or None)))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

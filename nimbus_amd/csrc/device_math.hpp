@@ -36,6 +36,24 @@ struct Eps<double>
     static constexpr double value = DBL_EPSILON;
 };
 
+// Power and exponential of the FP32 bulk through the hardware base-2
+// logarithm and exponential (v_log_f32, v_exp_f32: about 1 ulp each; the
+// composed result is within a few ulp for the arguments used here); FP64
+// keeps the library functions. x must be >= 0 for fpow (fpow(0, y > 0) = 0).
+template <class A>
+__device__ __forceinline__ A fpow(A x, A y)
+{
+    if constexpr (sizeof(A) == sizeof(float)) return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+    else return pow(x, y);
+}
+
+template <class A>
+__device__ __forceinline__ A fexp(A x)
+{
+    if constexpr (sizeof(A) == sizeof(float)) return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+    else return exp(x);
+}
+
 // ---- thermodynamics (src/microphysics/base/thermodynamics.hpp:29-470) ----
 
 template <class A>
@@ -44,7 +62,7 @@ __device__ __forceinline__ A theta_to_temperature(A theta, A p)
     if (!finite(theta) || !finite(p)) return A(kT0);
     const A th = clampv(theta, A(150), A(700));
     const A pp = clampv(p, A(100), A(200000));
-    const A T = th * pow(pp / A(kP00), A(kKappa));
+    const A T = th * fpow(pp / A(kP00), A(kKappa));
     return finite(T) ? T : A(kT0);
 }
 
@@ -53,7 +71,7 @@ __device__ __forceinline__ A theta_tendency_from_temperature(A dT, A p)
 {
     if (!finite(dT) || !finite(p)) return A(0);
     const A pp = clampv(p, A(100), A(200000));
-    const A r = dT * pow(A(kP00) / pp, A(kKappa));
+    const A r = dT * fpow(A(kP00) / pp, A(kKappa));
     return finite(r) ? r : A(0);
 }
 
@@ -62,7 +80,7 @@ __device__ __forceinline__ A qvs_water(A T, A p)
 {
     if (!finite(T) || !finite(p)) return A(0);
     const A Tc = clampv(T, A(150), A(400)) - A(kT0);
-    A es = A(611.21) * exp((A(18.678) - Tc / A(234.5)) * Tc / (A(257.14) + Tc));
+    A es = A(611.21) * fexp((A(18.678) - Tc / A(234.5)) * Tc / (A(257.14) + Tc));
     es = finite(es) ? fmax(A(0), es) : A(0);
     const A pp = clampv(p, A(100), A(200000));
     const A q = A(0.62198) * es / fmax(pp - es, A(1));
@@ -149,7 +167,7 @@ template <class A>
 __device__ __forceinline__ A kessler_rain_speed(A q, A rho)
 {
     if (q < A(1.0e-9)) return A(0);
-    return A(36.34) * pow(A(1.0e-3) * rho * q, A(0.1346));
+    return A(36.34) * fpow(A(1.0e-3) * rho * q, A(0.1346));
 }
 
 __device__ __forceinline__ double eos_pressure(double rho, double theta)
@@ -183,17 +201,19 @@ __device__ __forceinline__ double weno5_face(double a, double b, double c, doubl
 
 // Offset form: the face value is c + result, with all differences taken
 // relative to c and scaled by the stencil magnitude so FP32 keeps the
-// smoothness-indicator arithmetic bounded (weno5_face.hpp:88-120).
+// smoothness-indicator arithmetic bounded (weno5_face.hpp:88-120). The
+// reciprocals are the hardware v_rcp_f32 (1 ulp): the stencil scale cancels
+// exactly in form and the weights only need relative accuracy, while a
+// correctly rounded division costs eleven instructions per reciprocal.
 __device__ __forceinline__ float weno5_offset(float a, float b, float c, float d, float e)
 {
     const float mag = fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d))), fabsf(e));
     const float scale = fmaxf(mag, 1e-20f);
     if (!isfinite(scale)) return 0.0f;
-    const float inv = 1.0f / scale;
-    auto diff = [=](float x) {
-        const float dlt = x - c;
-        return isfinite(dlt) ? dlt * inv : x * inv - c * inv;
-    };
+    const float inv = __builtin_amdgcn_rcpf(scale);
+    // differences of finite physical scalars cannot overflow; a non-finite
+    // stencil yields a non-finite offset, which the final check turns into 0
+    auto diff = [=](float x) { return (x - c) * inv; };
     const float ac = diff(a), bc = diff(b), dc = diff(d), ec = diff(e);
     const float p0 = (1.0f / 3.0f) * ac - (7.0f / 6.0f) * bc;
     const float p1 = -(1.0f / 6.0f) * bc + (1.0f / 3.0f) * dc;
@@ -209,10 +229,11 @@ __device__ __forceinline__ float weno5_offset(float a, float b, float c, float d
     const float epsv = 1e-6f * rel * rel + flo * flo;
     // With the stencil scaled to O(1), eps + beta >= 1e-6 (or >= 1 for an
     // all-tiny stencil), so 1 / (eps + beta)^2 stays below 1e12: the weights
-    // need no further normalisation and reciprocals replace the divisions.
-    const float t0 = __frcp_rn(epsv + b0), t1 = __frcp_rn(epsv + b1), t2 = __frcp_rn(epsv + b2);
+    // need no further normalisation.
+    const float t0 = __builtin_amdgcn_rcpf(epsv + b0), t1 = __builtin_amdgcn_rcpf(epsv + b1),
+                t2 = __builtin_amdgcn_rcpf(epsv + b2);
     const float a0 = 0.1f * t0 * t0, a1 = 0.6f * t1 * t1, a2 = 0.3f * t2 * t2;
-    const float off = scale * ((a0 * p0 + a1 * p1 + a2 * p2) * __frcp_rn(a0 + a1 + a2));
+    const float off = scale * ((a0 * p0 + a1 * p1 + a2 * p2) * __builtin_amdgcn_rcpf(a0 + a1 + a2));
     return isfinite(off) ? off : 0.0f;
 }
 

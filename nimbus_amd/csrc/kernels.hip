@@ -7,14 +7,14 @@
 // water species are stored whole.
 //
 // Launch shapes. Every launch covers all members of a batch: workgroup b
-// reads entry b of a block map {member, local tile}. Stencil kernels give
-// each 256-thread workgroup a tile of columns (four waves) and march k
-// upward through one vertical chunk, holding the vertical stencil of the
-// marched fields in registers, so each plane is fetched from HBM once per
-// tile and the horizontal halo comes from L1/L2. The fused acoustic
-// substep works on strips: a wave owns 63 consecutive columns of one row and
-// its 64th lane computes the eastern neighbour column, whose pressure the
-// momentum update needs, exchanged by a lane shuffle.
+// reads entry b of a block map {member, local work item}. Stencil and
+// pointwise kernels give each thread one cell and each 256-thread workgroup
+// a tile of 64 x 4 (or 32 x 8) columns at one level; the work items of a
+// column tile are numbered level-fastest within vertical chunks, so the
+// workgroups an XCD runs back to back are vertical neighbours and a
+// stencil's planes above and below are still in that XCD's L2 while every
+// thread stays independent (full occupancy, no serial march). The HE-VI
+// column solve gives each thread one whole column.
 //
 // Precision: S is the storage type. Continuity, EOS, pressure gradient,
 // base-state terms, slow tendencies, mass-flux accumulation and the HE-VI
@@ -90,51 +90,47 @@ __device__ __forceinline__ Work work_of(const int2* __restrict__ map)
     return Work{e.x, e.y};
 }
 
-// Column tile of a tile_w x tile_h workgroup, vertical chunk kc of n_chunks.
-struct Tile
+// One thread per (i, j) column of a tile_w x tile_h tile of columns.
+struct Column
 {
-    int i, j, k0, k1;
+    int i, j;
 };
 
-__device__ __forceinline__ Tile tile_of(const MemberArgs& A, int local, int n_chunks)
+__device__ __forceinline__ Column column_of(const MemberArgs& A, int local)
 {
     const MemberGeom& G = A.geo;
-    const int tiles = G.tiles_x * G.tiles_y;
-    const int kc = local / tiles, t = local - kc * tiles;
-    const int ty = t / G.tiles_x, tx = t - ty * G.tiles_x;
+    const int ty = local / G.tiles_x, tx = local - ty * G.tiles_x;
     const int tid = int(threadIdx.x);
-    Tile r;
-    r.i = tx * G.tile_w + (tid & (G.tile_w - 1));
-    r.j = ty * G.tile_h + tid / G.tile_w;
-    const int nz = A.g.nz;
-    r.k0 = (kc * nz) / n_chunks;
-    r.k1 = ((kc + 1) * nz) / n_chunks;
-    return r;
+    return Column{tx * G.tile_w + (tid & (G.tile_w - 1)), ty * G.tile_h + tid / G.tile_w};
 }
 
-// Strip of 63 owned columns per wave, four rows per workgroup; `west` puts
-// the halo lane on the west (lane 0) instead of the east (lane 63).
-struct Strip
+// One cell per thread: tile t of the member at level k. The local work
+// index runs k fastest within a vertical chunk (k_chunks chunks of kz
+// levels), then over the column tiles, then over the chunks, so the
+// workgroups an XCD runs back to back are the vertical neighbours of one
+// column tile and a stencil's k-1 .. k+3 planes are still in its L2.
+struct Cell
 {
-    int i, j, lane, k0, k1;
-    bool own;
+    int i, j, k;
+    bool valid;
 };
 
-__device__ __forceinline__ Strip strip_of(const MemberArgs& A, int local, bool west)
+__device__ __forceinline__ Cell cell_of(const MemberArgs& A, int local)
 {
     const MemberGeom& G = A.geo;
-    const int strips = G.strips_x * G.strips_y;
-    const int kc = local / strips, t = local - kc * strips;
-    const int sy = t / G.strips_x, sx = t - sy * G.strips_x;
-    Strip r;
-    r.lane = int(threadIdx.x & 63u);
-    r.j = sy * 4 + int(threadIdx.x >> 6);
-    r.i = sx * 63 + r.lane - (west ? 1 : 0);
-    r.own = west ? r.lane > 0 : r.lane < 63;
     const int nz = A.g.nz;
-    r.k0 = (kc * nz) / G.k_chunks;
-    r.k1 = ((kc + 1) * nz) / G.k_chunks;
-    return r;
+    const int kz = (nz + G.k_chunks - 1) / G.k_chunks;
+    const int tiles = G.tiles_x * G.tiles_y;
+    const int kk = local % kz, rest = local / kz;
+    const int t = rest % tiles, kc = rest / tiles;
+    const int ty = t / G.tiles_x, tx = t - ty * G.tiles_x;
+    const int tid = int(threadIdx.x);
+    Cell c;
+    c.k = kc * kz + kk;
+    c.i = tx * G.tile_w + (tid & (G.tile_w - 1));
+    c.j = ty * G.tile_h + tid / G.tile_w;
+    c.valid = c.k < nz && c.i < A.g.nx && c.j < A.g.ny;
+    return c;
 }
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
@@ -208,25 +204,7 @@ __device__ void slow_tendency_cell(const DevGrid& g, const SlowFields<S>& F, con
     }
 }
 
-// Vertical window of seven values k-3 .. k+3 of one column (clamped reads).
-template <class S>
-struct Win7
-{
-    double v[7];
-    __device__ __forceinline__ void fill(const S* __restrict__ a, std::size_t col, std::size_t sz, int k, int nz)
-    {
-#pragma unroll
-        for (int o = 0; o < 7; ++o) v[o] = double(a[col + std::size_t(clampi(k + o - 3, 0, nz - 1)) * sz]);
-    }
-    __device__ __forceinline__ void shift(const S* __restrict__ a, std::size_t col, std::size_t sz, int k, int nz)
-    {
-#pragma unroll
-        for (int o = 0; o < 6; ++o) v[o] = v[o + 1];
-        v[6] = double(a[col + std::size_t(clampi(k + 4, 0, nz - 1)) * sz]);
-    }
-};
-
-// Heun slow RK2 (dynamics.cpp:5178-5292, 2055-2095) in two column marches.
+// Heun slow RK2 (dynamics.cpp:5178-5292, 2055-2095), one cell per thread.
 // MODE 0: stage 1 at u_n -> k1 and predictor u_n + dt k1 (into u2, v2, w2).
 // MODE 1: stage 2 at the predictor, average in storage precision, apply to u_n.
 // MODE 2: tendencies only (per-operation dispatch) into out_u/v/w.
@@ -237,10 +215,10 @@ __global__ __launch_bounds__(kBlock) void k_slow(const MemberArgs* __restrict__ 
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const int i = cl.i, j = cl.j, k = cl.k;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const bool valid = t.i < NX && t.j < NY;
-    const int i = min(t.i, NX - 1), j = min(t.j, NY - 1);
     const DevFields& f = A.f;
     SlowFields<S> F;
     F.u = P<S>(MODE == 1 ? f.u2 : f.u);
@@ -250,62 +228,54 @@ __global__ __launch_bounds__(kBlock) void k_slow(const MemberArgs* __restrict__ 
     F.qv = P<S>(f.qv);
     F.qc = P<S>(f.qc);
     F.qr = P<S>(f.qr);
-    const double* base = f.base;
-    const double dt = A.p.dt, fcor = A.p.coriolis_f;
+    const double dt = A.p.dt;
     const std::size_t sz = g.plane();
     const std::size_t col = g.at(i, j, 0);
-    const bool hinterior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
-    Win7<S> U, V, Wz;
-    U.fill(F.u, col, sz, t.k0, NZ);
-    V.fill(F.v, col, sz, t.k0, NZ);
-    Wz.fill(F.w, col, sz, t.k0, NZ);
-    auto at_k = [&](const S* a, int k) { return double(a[col + std::size_t(clampi(k, 0, NZ - 1)) * sz]); };
-    double rp0 = at_k(F.rp, t.k0), qv0 = at_k(F.qv, t.k0), l0 = at_k(F.qc, t.k0) + at_k(F.qr, t.k0);
-    for (int k = t.k0; k < t.k1; ++k)
+    const std::size_t c = col + std::size_t(k) * sz;
+    double du = 0.0, dv = 0.0, dw = 0.0;
+    if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k < NZ - 1)
     {
-        const std::size_t c = col + std::size_t(k) * sz;
-        const double rp1 = at_k(F.rp, k + 1), qv1 = at_k(F.qv, k + 1);
-        const double l1 = at_k(F.qc, k + 1) + at_k(F.qr, k + 1);
-        double du = 0.0, dv = 0.0, dw = 0.0;
-        if (hinterior && k < NZ - 1)
-            slow_tendency_cell<S>(g, F, base, fcor, i, j, k, U.v, V.v, Wz.v, rp0, rp1, qv0, qv1, l0, l1, du, dv, dw);
-        if (valid)
+        double uz[7], vz[7], wz[7];
+#pragma unroll
+        for (int o = 0; o < 7; ++o)
         {
-            if constexpr (MODE == 0)
-            {
-                const S a = S(du), b = S(dv), d = S(dw);
-                P<S>(f.k1u)[c] = a;
-                P<S>(f.k1v)[c] = b;
-                P<S>(f.k1w)[c] = d;
-                P<S>(f.u2)[c] = S(U.v[3] + double(a) * dt);
-                P<S>(f.v2)[c] = S(V.v[3] + double(b) * dt);
-                P<S>(f.w2)[c] = S(Wz.v[3] + double(d) * dt);
-            }
-            else if constexpr (MODE == 1)
-            {
-                const S au = S(S(0.5) * (S(du) + P<S>(f.k1u)[c]));
-                const S av = S(S(0.5) * (S(dv) + P<S>(f.k1v)[c]));
-                const S aw = S(S(0.5) * (S(dw) + P<S>(f.k1w)[c]));
-                S* u = P<S>(f.u);
-                S* v = P<S>(f.v);
-                S* w = P<S>(f.w);
-                u[c] = S(double(u[c]) + finite0(double(au)) * dt);
-                v[c] = S(double(v[c]) + finite0(double(av)) * dt);
-                w[c] = S(double(w[c]) + finite0(double(aw)) * dt);
-            }
-            else
-            {
-                out_u[c] = S(du);
-                out_v[c] = S(dv);
-                out_w[c] = S(dw);
-            }
+            const std::size_t ck = col + std::size_t(clampi(k + o - 3, 0, NZ - 1)) * sz;
+            uz[o] = double(F.u[ck]);
+            vz[o] = double(F.v[ck]);
+            wz[o] = double(F.w[ck]);
         }
-        rp0 = rp1;
-        qv0 = qv1;
-        l0 = l1;
-        U.shift(F.u, col, sz, k, NZ);
-        V.shift(F.v, col, sz, k, NZ);
-        Wz.shift(F.w, col, sz, k, NZ);
+        const std::size_t c1 = c + sz;
+        slow_tendency_cell<S>(g, F, f.base, A.p.coriolis_f, i, j, k, uz, vz, wz, double(F.rp[c]), double(F.rp[c1]),
+                              double(F.qv[c]), double(F.qv[c1]), double(F.qc[c]) + double(F.qr[c]),
+                              double(F.qc[c1]) + double(F.qr[c1]), du, dv, dw);
+    }
+    if constexpr (MODE == 0)
+    {
+        const S a = S(du), b = S(dv), d = S(dw);
+        P<S>(f.k1u)[c] = a;
+        P<S>(f.k1v)[c] = b;
+        P<S>(f.k1w)[c] = d;
+        P<S>(f.u2)[c] = S(double(F.u[c]) + double(a) * dt);
+        P<S>(f.v2)[c] = S(double(F.v[c]) + double(b) * dt);
+        P<S>(f.w2)[c] = S(double(F.w[c]) + double(d) * dt);
+    }
+    else if constexpr (MODE == 1)
+    {
+        const S au = S(S(0.5) * (S(du) + P<S>(f.k1u)[c]));
+        const S av = S(S(0.5) * (S(dv) + P<S>(f.k1v)[c]));
+        const S aw = S(S(0.5) * (S(dw) + P<S>(f.k1w)[c]));
+        S* u = P<S>(f.u);
+        S* v = P<S>(f.v);
+        S* w = P<S>(f.w);
+        u[c] = S(double(u[c]) + finite0(double(au)) * dt);
+        v[c] = S(double(v[c]) + finite0(double(av)) * dt);
+        w[c] = S(double(w[c]) + finite0(double(aw)) * dt);
+    }
+    else
+    {
+        out_u[c] = S(du);
+        out_v[c] = S(dv);
+        out_w[c] = S(dw);
     }
 }
 
@@ -318,13 +288,35 @@ __global__ __launch_bounds__(kBlock) void k_slow(const MemberArgs* __restrict__ 
 // p - p0 = pe0 expm1(gamma log1p(x)) + (pe0 - p0) keeps FP32 relative
 // accuracy on the perturbation rather than on the full pressure.
 // ---------------------------------------------------------------------
+// (1 + x)^gamma - 1 in FP32. For |x| < 1/8 (every realistic perturbation)
+// it is the binomial series sum_{n=1..8} C(gamma, n) x^n, whose truncation
+// error is below 2e-10 relative, evaluated by Horner's rule (about 1 ulp);
+// larger |x| take expm1(gamma log1p(x)).
+__device__ __forceinline__ float pow1p_gamma_m1(float x)
+{
+    if (fabsf(x) < 0.125f)
+    {
+        // C(gamma, n) for gamma = cp / (cp - Rd) = 1004 / 717
+        float p = 3.214725001e-03f;
+        p = fmaf(x, p, -4.592693052e-03f);
+        p = fmaf(x, p, 6.989304557e-03f);
+        p = fmaf(x, p, -1.164974359e-02f);
+        p = fmaf(x, p, 2.240575685e-02f);
+        p = fmaf(x, p, -5.602415924e-02f);
+        p = fmaf(x, p, 2.802510849e-01f);
+        p = fmaf(x, p, 1.400278940e+00f);
+        return x * p;
+    }
+    return expm1f(float(kGamma) * log1pf(x));
+}
+
 template <class S>
 __device__ __forceinline__ double eos_full_pressure(const double* base, int nz, int k, double rho, double theta)
 {
     if constexpr (sizeof(S) == sizeof(float))
     {
         const double x = rho * theta * base[std::size_t(kBaseInvRT0) * nz + k] - 1.0;
-        const float e = expm1f(float(kGamma) * log1pf(float(x)));
+        const float e = pow1p_gamma_m1(float(x));
         return base[std::size_t(kBaseP0) * nz + k] + (double(e) * base[std::size_t(kBasePe0) * nz + k] +
                                                       base[std::size_t(kBasePeDiff) * nz + k]);
     }
@@ -359,20 +351,19 @@ __device__ __forceinline__ RhoP<S> density_eos(const Bounds& bd, const double* b
 }
 
 // ---------------------------------------------------------------------
-// Fused explicit acoustic substep (acoustic_cgrid_fast.cpp:12-219) with the
-// retained mass-flux accumulation of the pre-continuity state
-// (advection_cartesian.cpp:610-830). Substep s reads u, v, w, rho' from
-// buffer s & 1 and writes buffer (s + 1) & 1, so no workgroup reads what
-// another writes. Per wave and plane: forward pass (continuity, clamp, EOS)
-// of the own column, of the northern neighbour column (recomputed) and, in
-// lane 63, of the eastern neighbour; the momentum update of plane k runs
-// once the forward pass of plane k + 1 is known.
+// Explicit acoustic substep (acoustic_cgrid_fast.cpp:12-219) as a forward
+// pass (retained mass-flux accumulation of the pre-continuity state,
+// advection_cartesian.cpp:610-830; continuity; density clamp; EOS) and a
+// backward pass (momentum with the new pressure). Substep s reads rho' from
+// buffer s & 1 and writes buffer (s + 1) & 1, so no workgroup reads a
+// neighbour's rho' after it changed; u, v, w are updated in place by the
+// backward pass, which reads only its own cell's velocities.
 // ---------------------------------------------------------------------
 template <class S>
 struct AcousticBufs
 {
     const S *uo, *vo, *wo, *ro, *th;
-    S *un, *vn, *wn, *rn, *pp, *mfx, *mfy, *mfz, *rb;
+    S *rn, *pp, *mfx, *mfy, *mfz, *rb;
 };
 
 template <class S>
@@ -467,40 +458,22 @@ __device__ __forceinline__ void momentum_update(const DevGrid& g, const double* 
 }
 
 template <class S>
-__device__ __forceinline__ RhoP<S> shfl_down1(RhoP<S> v)
-{
-    RhoP<S> o;
-    o.r = __shfl_down(v.r, 1);
-    o.p = __shfl_down(v.p, 1);
-    return o;
-}
-
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_acoustic_fused(const MemberArgs* __restrict__ ma,
-                                                           const int2* __restrict__ map, int s, int accumulate,
-                                                           int init)
+__global__ __launch_bounds__(kBlock) void k_acoustic_forward(const MemberArgs* __restrict__ ma,
+                                                             const int2* __restrict__ map, int s, int accumulate,
+                                                             int init)
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     if (s >= A.p.substeps) return;
-    const DevGrid& g = A.g;
-    const Strip st = strip_of(A, wk.local, false);
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    if (st.j >= NY) return;  // whole wave (rows are wave-uniform); no workgroup barrier follows
-    const int j = st.j;
-    const bool have = st.i < NX;
-    const bool own = st.own && have;
-    const int i = min(st.i, NX - 1);
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
     const DevFields& f = A.f;
     const int par = s & 1;
     AcousticBufs<S> B;
-    B.uo = P<S>(par ? f.u2 : f.u);
-    B.vo = P<S>(par ? f.v2 : f.v);
-    B.wo = P<S>(par ? f.w2 : f.w);
+    B.uo = P<S>(f.u);
+    B.vo = P<S>(f.v);
+    B.wo = P<S>(f.w);
     B.ro = P<S>(par ? f.rp2 : f.rp);
-    B.un = P<S>(par ? f.u : f.u2);
-    B.vn = P<S>(par ? f.v : f.v2);
-    B.wn = P<S>(par ? f.w : f.w2);
     B.rn = P<S>(par ? f.rp : f.rp2);
     B.th = P<S>(f.th);
     B.pp = P<S>(f.pp);
@@ -508,44 +481,25 @@ __global__ __launch_bounds__(kBlock) void k_acoustic_fused(const MemberArgs* __r
     B.mfy = P<S>(f.mfy);
     B.mfz = P<S>(f.mfz);
     B.rb = P<S>(f.rb);
-    const double* base = f.base;
-    const Bounds bd = A.p.bounds;
-    const double dts = A.p.dts, weight = A.p.mf_weight;
-    const bool acc = accumulate != 0, first = init != 0 && s == 0;
-    const bool top_open = A.p.bc_top == NIMBUS_TOP_OPEN;
-    const bool yn = j + 1 < NY;
-    const int k0 = st.k0, k1 = st.k1;
-    RhoP<S> cur = acoustic_forward<S>(g, B, base, bd, dts, weight, i, j, k0, own, acc, first, top_open);
-    RhoP<S> ycur{};
-    if (yn) ycur = acoustic_forward<S>(g, B, base, bd, dts, weight, i, j + 1, k0, false, acc, first, top_open);
-    for (int k = k0; k < k1; ++k)
-    {
-        RhoP<S> nxt{}, ynxt{};
-        if (k + 1 < NZ)
-            nxt = acoustic_forward<S>(g, B, base, bd, dts, weight, i, j, k + 1, own && k + 1 < k1, acc, first,
-                                      top_open);
-        if (yn && k + 1 < k1)
-            ynxt = acoustic_forward<S>(g, B, base, bd, dts, weight, i, j + 1, k + 1, false, acc, first, top_open);
-        const RhoP<S> xcur = shfl_down1(cur);
-        if (own)
-        {
-            const std::size_t c = g.at(i, j, k);
-            momentum_update<S>(g, base, bd, dts, i, j, k, cur, xcur, ycur, nxt, double(B.uo[c]), double(B.vo[c]),
-                               double(B.wo[c]), false, B.un, B.vn, B.wn, c);
-        }
-        cur = nxt;
-        ycur = ynxt;
-    }
+    acoustic_forward<S>(A.g, B, f.base, A.p.bounds, A.p.dts, A.p.mf_weight, cl.i, cl.j, cl.k, true,
+                        accumulate != 0, init != 0 && s == 0, A.p.bc_top == NIMBUS_TOP_OPEN);
 }
 
 // ---------------------------------------------------------------------
 // HE-VI column kernel: horizontal continuity explicit, vertical w-rho-p
-// coupling off-centred implicit (alpha = (1+beta)/2), p linearised about
-// p* = EOS(rho*), FP64 Thomas sweep per column, flux-form density update and
-// EOS re-diagnosis. Ghost columns get the reference's pass-2 clamp/EOS only.
+// coupling off-centred implicit (alpha = (1+beta)/2): the vertical pressure
+// gradient acts on alpha p^{n+1} + (1 - alpha) p^n with p^{n+1} linearised
+// about p* = EOS(rho*); FP64 Thomas sweep per column, flux-form density
+// update and EOS re-diagnosis. Ghost columns get the reference's pass-2 clamp/EOS only.
 // rho' is read from buffer s & 1 and written to (s + 1) & 1; u, v, w stay in
 // their home buffers (w is only touched by its own column).
 // ---------------------------------------------------------------------
+template <class S>
+struct HeviLevel
+{
+    S rp, rpxp, rpxm, rpyp, rpym, u, um, v, vm, w, th, pn, mfx, mfy;
+};
+
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __restrict__ ma,
                                                         const int2* __restrict__ map, int s, int accumulate, int init)
@@ -554,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
     const MemberArgs& A = ma[wk.m];
     if (s >= A.p.substeps) return;
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, 1);
+    const Column t = column_of(A, wk.local);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     if (t.i >= NX || t.j >= NY) return;  // no barrier in this kernel
     const int i = t.i, j = t.j;
@@ -580,68 +534,104 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
     const double* rho0 = slot(base, kBaseRho0, NZ);
     const double* p0 = slot(base, kBaseP0, NZ);
-    const bool interior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
     const double idz = g.idz;
     const double a1 = 1.0 - alpha;
-    // ---- upward sweep: mass fluxes, rho*, p*, Thomas forward elimination ----
-    double rb_prev = 0.0, C_prev = 0.0, ps_prev = 0.0, wo_prev = 0.0, e_prev = 0.0;
-    double rb_prev2 = 0.0, cp_prev = 0.0, rr_prev = 0.0;
-    bool ok = true;
-    for (int k = 0; k < NZ; ++k)
+    if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
     {
-        const std::size_t c = g.at(i, j, k);
-        const double r0 = rho0[k];
-        const double rc = r0 + double(rp_in[c]);
-        if (acc)
+        // ghost column: mass fluxes, density unchanged (clamped), EOS pressure
+        for (int k = 0; k < NZ; ++k)
         {
-            if (i < NX - 1)
+            const std::size_t c = g.at(i, j, k);
+            const double r0 = rho0[k];
+            const double rc = r0 + double(rp_in[c]);
+            if (acc)
             {
-                const double fl = 0.5 * (rc + (r0 + double(rp_in[c + 1]))) * double(u[c]);
-                mfx[c] = S((first ? 0.0 : double(mfx[c])) + weight * fl);
+                if (i < NX - 1)
+                {
+                    const double fl = 0.5 * (rc + (r0 + double(rp_in[c + 1]))) * double(u[c]);
+                    mfx[c] = S((first ? 0.0 : double(mfx[c])) + weight * fl);
+                }
+                if (j < NY - 1)
+                {
+                    const double fl = 0.5 * (rc + (r0 + double(rp_in[c + sy]))) * double(v[c]);
+                    mfy[c] = S((first ? 0.0 : double(mfy[c])) + weight * fl);
+                }
+                if (first) mfz[c] = S(0);
             }
-            if (j < NY - 1)
-            {
-                const double fl = 0.5 * (rc + (r0 + double(rp_in[c + sy]))) * double(v[c]);
-                mfy[c] = S((first ? 0.0 : double(mfy[c])) + weight * fl);
-            }
-            if (first && !interior) mfz[c] = S(0);
-        }
-        if (first) rb[c] = rp_in[c];
-        if (!interior)
-        {
+            if (first) rb[c] = rp_in[c];
             const RhoP<S> o = density_eos<S>(bd, base, NZ, k, rc, double(th[c]));
             rp_out[c] = o.r;
             pp[c] = o.p;
-            continue;
         }
-        const double rxp = r0 + double(rp_in[c + 1]), rxm = r0 + double(rp_in[c - 1]);
-        const double ryp = r0 + double(rp_in[c + sy]), rym = r0 + double(rp_in[c - sy]);
-        const double dh = -((0.5 * (rc + rxp) * double(u[c]) - 0.5 * (rxm + rc) * double(u[c - 1])) * g.idx +
-                            (0.5 * (rc + ryp) * double(v[c]) - 0.5 * (rym + rc) * double(v[c - sy])) * g.idy);
+        return;
+    }
+    // ---- upward sweep: mass fluxes, rho*, p*, Thomas forward elimination.
+    // The inputs of levels k + 1 and k + 2 are loaded before level k is
+    // computed and stored, so every thread keeps loads of three levels in flight.
+    auto load_level = [&](int k) {
+        const std::size_t c = g.at(i, j, k);
+        HeviLevel<S> L;
+        L.rp = rp_in[c]; L.rpxp = rp_in[c + 1]; L.rpxm = rp_in[c - 1];
+        L.rpyp = rp_in[c + sy]; L.rpym = rp_in[c - sy];
+        L.u = u[c]; L.um = u[c - 1]; L.v = v[c]; L.vm = v[c - sy];
+        L.w = w[c]; L.th = th[c]; L.pn = pp[c];
+        L.mfx = acc && !first ? mfx[c] : S(0);
+        L.mfy = acc && !first ? mfy[c] : S(0);
+        return L;
+    };
+    double rb_prev = 0.0, C_prev = 0.0, ps_prev = 0.0, pn_prev = 0.0, wo_prev = 0.0, e_prev = 0.0;
+    double rb_prev2 = 0.0, cp_prev = 0.0, rr_prev = 0.0;
+    bool ok = true;
+    HeviLevel<S> cur = load_level(0);
+    HeviLevel<S> nxt{};
+    if (1 < NZ) nxt = load_level(1);
+    for (int k = 0; k < NZ; ++k)
+    {
+        HeviLevel<S> nxt2{};
+        if (k + 2 < NZ) nxt2 = load_level(k + 2);
+        const std::size_t c = g.at(i, j, k);
+        const double r0 = rho0[k];
+        const double rc = r0 + double(cur.rp);
+        const double rxp = r0 + double(cur.rpxp), rxm = r0 + double(cur.rpxm);
+        const double ryp = r0 + double(cur.rpyp), rym = r0 + double(cur.rpym);
+        if (acc)
+        {
+            mfx[c] = S((first ? 0.0 : double(cur.mfx)) + weight * (0.5 * (rc + rxp) * double(cur.u)));
+            mfy[c] = S((first ? 0.0 : double(cur.mfy)) + weight * (0.5 * (rc + ryp) * double(cur.v)));
+        }
+        if (first) rb[c] = cur.rp;
+        const double dh = -((0.5 * (rc + rxp) * double(cur.u) - 0.5 * (rxm + rc) * double(cur.um)) * g.idx +
+                            (0.5 * (rc + ryp) * double(cur.v) - 0.5 * (rym + rc) * double(cur.vm)) * g.idy);
         const double rstar = rc + dts * finite0(dh);
-        const double ps = dev::eos_pressure(rstar, double(th[c]));
-        const double C = kGamma * ps / rstar;
-        const double wo = double(w[c]);
+        // linearisation point p* = EOS(rho*): FP64 power law in FP64 storage,
+        // the perturbation form (FP32 relative accuracy on p* - p0) in the mixed mode
+        const double ps = eos_full_pressure<S>(base, NZ, k, rstar, double(cur.th));
+        const double C = kGamma * ps * (1.0 / rstar);
+        const double wo = double(cur.w);
         // face density above level k (zero at the lid)
-        const double rbk = k < NZ - 1 ? 0.5 * (rc + (rho0[k + 1] + double(rp_in[c + sz]))) : 0.0;
+        const double rbk = k < NZ - 1 ? 0.5 * (rc + (rho0[k + 1] + double(nxt.rp))) : 0.0;
         const double e = -a1 * dts * idz * (rbk * wo - rb_prev * wo_prev);
         rp_out[c] = S(rstar - r0);
+        const double pnk = double(cur.pn);
         if (k >= 1)
         {
             // row r = k-1 (face k-1/2): unknowns w_{r-1}, w_r, w_{r+1}
             const double rbr = rb_prev;
-            const double Ak = dts * alpha / (rbr * g.dz);
+            const double inv_rbr = 1.0 / rbr;
             const double Bc = alpha * dts * idz;
+            const double Ak = Bc * inv_rbr;
             const double Aa = (k - 1) > 0 ? -Ak * Bc * C_prev * rb_prev2 : 0.0;
             const double U = (k - 1) < NZ - 2 ? -Ak * Bc * C * rbk : 0.0;
             const double D = 1.0 + Ak * Bc * (C + C_prev) * rbr;
-            const double G = (ps - ps_prev) * idz - (p0[k] - p0[k - 1]) * idz;
-            const double R = wo_prev - dts / rbr * G - Ak * (C * e - C_prev * e_prev);
+            // PGF on alpha p* + (1 - alpha) p^n (p^n: substep-start pressure), base gradient removed
+            const double G = (alpha * ((ps - ps_prev) - (p0[k] - p0[k - 1])) + a1 * (pnk - pn_prev)) * idz;
+            const double R = wo_prev - dts * inv_rbr * G - Ak * (C * e - C_prev * e_prev);
             const double piv = D - Aa * cp_prev;
             if (!(piv > 0.0) || !isfinite(piv)) ok = false;
-            const double cpr = U / piv;
-            const double rrr = (R - Aa * rr_prev) / piv;
-            const std::size_t cr = g.at(i, j, k - 1);
+            const double inv_piv = 1.0 / piv;
+            const double cpr = U * inv_piv;
+            const double rrr = (R - Aa * rr_prev) * inv_piv;
+            const std::size_t cr = c - sz;
             tc[cr] = cpr;
             tr[cr] = rrr;
             cp_prev = cpr;
@@ -651,57 +641,94 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
         rb_prev = rbk;
         C_prev = C;
         ps_prev = ps;
+        pn_prev = pnk;
         wo_prev = wo;
         e_prev = e;
+        cur = nxt;
+        nxt = nxt2;
     }
-    if (!interior) return;
-    // ---- downward sweep: back substitution, density update, weighted z mass flux ----
+    // ---- downward sweep: back substitution, density update, weighted z mass flux.
+    // Face r's inputs (and cell r's, finalised one iteration later) are
+    // loaded one iteration ahead.
+    struct Down
+    {
+        double w, tr, tc, rp_lo;
+        S rstar, th, mfz;
+    };
+    auto load_down = [&](int r) {
+        Down d{};
+        const std::size_t cr = g.at(i, j, r);
+        d.w = double(w[cr]);
+        d.tr = tr[cr];
+        d.tc = tc[cr];
+        d.rp_lo = double(rp_in[cr]);
+        d.rstar = rp_out[cr];
+        d.th = th[cr];
+        d.mfz = acc && !first ? mfz[cr] : S(0);
+        return d;
+    };
     double w_above = 0.0, wo_above = 0.0, rb_above = 0.0;  // face r+1 quantities (stored w)
     double x_above = 0.0;                                  // face r+1 Thomas solution (unclamped)
+    // cell NZ-1 (finalised in the first iteration) and face NZ-2
+    const std::size_t ctop = g.at(i, j, NZ - 1);
+    double rp_hi = double(rp_in[ctop]);
+    S rstar_c = rp_out[ctop], th_c = th[ctop], mfz_c = acc && !first ? mfz[ctop] : S(0);
+    Down dn = load_down(NZ - 2);
     for (int r = NZ - 2; r >= -1; --r)
     {
+        Down nx{};
+        if (r - 1 >= 0) nx = load_down(r - 1);
         double wn = 0.0, wo = 0.0, rbr = 0.0;
         if (r >= 0)
         {
             const std::size_t cr = g.at(i, j, r);
-            wo = double(w[cr]);
-            const double x = ok ? tr[cr] - tc[cr] * x_above : wo;
+            wo = dn.w;
+            const double x = ok ? dn.tr - dn.tc * x_above : wo;
             x_above = x;
             wn = clamp_store<S>(x, 0.0, -bd.wind_v, bd.wind_v);
             w[cr] = S(wn);
             wn = double(S(wn));
-            rbr = 0.5 * ((rho0[r] + double(rp_in[cr])) + (rho0[r + 1] + double(rp_in[cr + sz])));
+            rbr = 0.5 * ((rho0[r] + dn.rp_lo) + (rho0[r + 1] + rp_hi));
         }
         // finalise cell r+1: top face r+1 (zero at the lid), bottom face r
         const int kc = r + 1;
         const std::size_t cc = g.at(i, j, kc);
         const double ft = kc < NZ - 1 ? rb_above * (alpha * w_above + a1 * wo_above) : 0.0;
         const double fb = r >= 0 ? rbr * (alpha * wn + a1 * wo) : 0.0;
-        if (acc) mfz[cc] = S((first ? 0.0 : double(mfz[cc])) + weight * ft);
-        const double rstar = rho0[kc] + double(rp_out[cc]);
-        const RhoP<S> o = density_eos<S>(bd, base, NZ, kc, rstar - dts * idz * (ft - fb), double(th[cc]));
+        if (acc) mfz[cc] = S((first ? 0.0 : double(mfz_c)) + weight * ft);
+        const double rstar = rho0[kc] + double(rstar_c);
+        const RhoP<S> o = density_eos<S>(bd, base, NZ, kc, rstar - dts * idz * (ft - fb), double(th_c));
         rp_out[cc] = o.r;
         pp[cc] = o.p;
         w_above = wn;
         wo_above = wo;
         rb_above = rbr;
+        if (r >= 0)
+        {
+            rp_hi = dn.rp_lo;
+            rstar_c = dn.rstar;
+            th_c = dn.th;
+            mfz_c = dn.mfz;
+        }
+        dn = nx;
     }
 }
 
-// HE-VI backward step for u and v with the new pressure; w is clamped only
-// where the column solve did not set it (ghost columns, lid).
+// Backward momentum update of substep s with the new rho' (buffer
+// (s + 1) & 1) and pressure, in place. HE-VI members keep the interior w the
+// column solve set; their ghost-column and lid w get the clamp only.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_hevi_momentum(const MemberArgs* __restrict__ ma,
-                                                          const int2* __restrict__ map, int s)
+__global__ __launch_bounds__(kBlock) void k_momentum(const MemberArgs* __restrict__ ma,
+                                                     const int2* __restrict__ map, int s)
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     if (s >= A.p.substeps) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    if (t.i >= NX || t.j >= NY) return;
-    const int i = t.i, j = t.j;
+    const int i = cl.i, j = cl.j, k = cl.k;
     const DevFields& f = A.f;
     const S* __restrict__ rn = P<S>((s & 1) ? f.rp : f.rp2);
     const S* __restrict__ pp = P<S>(f.pp);
@@ -709,16 +736,13 @@ __global__ __launch_bounds__(kBlock) void k_hevi_momentum(const MemberArgs* __re
     S* v = P<S>(f.v);
     S* w = P<S>(f.w);
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
-    for (int k = t.k0; k < t.k1; ++k)
-    {
-        const std::size_t c = g.at(i, j, k);
-        const RhoP<S> cc{rn[c], pp[c]};
-        const RhoP<S> xp = i < NX - 1 ? RhoP<S>{rn[c + 1], pp[c + 1]} : cc;
-        const RhoP<S> yp = j < NY - 1 ? RhoP<S>{rn[c + sy], pp[c + sy]} : cc;
-        const RhoP<S> zp = k < NZ - 1 ? RhoP<S>{rn[c + sz], pp[c + sz]} : cc;
-        momentum_update<S>(g, f.base, A.p.bounds, A.p.dts, i, j, k, cc, xp, yp, zp, double(u[c]), double(v[c]),
-                           double(w[c]), true, u, v, w, c);
-    }
+    const std::size_t c = g.at(i, j, k);
+    const RhoP<S> cc{rn[c], pp[c]};
+    const RhoP<S> xp = i < NX - 1 ? RhoP<S>{rn[c + 1], pp[c + 1]} : cc;
+    const RhoP<S> yp = j < NY - 1 ? RhoP<S>{rn[c + sy], pp[c + sy]} : cc;
+    const RhoP<S> zp = k < NZ - 1 ? RhoP<S>{rn[c + sz], pp[c + sz]} : cc;
+    momentum_update<S>(g, f.base, A.p.bounds, A.p.dts, i, j, k, cc, xp, yp, zp, double(u[c]), double(v[c]),
+                       double(w[c]), A.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI, u, v, w, c);
 }
 
 // ---------------------------------------------------------------------
@@ -742,18 +766,17 @@ struct BcBufs
     float dt_rad;
 };
 
-// Buffers current after substep s (s < 0: home buffers, no radiation).
+// Buffers current after substep s (s < 0: home buffers, no radiation):
+// u, v, w live in their home buffers, rho' in buffer (s + 1) & 1.
 template <class S>
 __device__ __forceinline__ bool bc_buffers(const MemberArgs& A, int s, BcBufs<S>& b)
 {
     const DevFields& f = A.f;
     if (s >= 0 && s >= A.p.substeps) return false;
     const int par = s >= 0 ? ((s + 1) & 1) : 0;
-    const bool explicit_mode = A.p.acoustic_mode == NIMBUS_ACOUSTIC_EXPLICIT;
-    const int pu = explicit_mode ? par : 0;
-    b.u = P<S>(pu ? f.u2 : f.u);
-    b.v = P<S>(pu ? f.v2 : f.v);
-    b.w = P<S>(pu ? f.w2 : f.w);
+    b.u = P<S>(f.u);
+    b.v = P<S>(f.v);
+    b.w = P<S>(f.w);
     b.rp = P<S>(par ? f.rp2 : f.rp);
     b.pp = P<S>(f.pp);
     b.dt_rad = s >= 0 ? float(A.p.dts) : 0.0f;
@@ -911,7 +934,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_vertical(const MemberArgs* __rest
     BcBufs<S> b;
     if (!bc_buffers<S>(A, s, b)) return;
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, 1);
+    const Column t = column_of(A, wk.local);
     if (t.i >= g.nx || t.j >= g.ny) return;
     const int NZ = g.nz;
     if (A.p.bc_bottom == NIMBUS_BOTTOM_NO_SLIP)
@@ -950,9 +973,8 @@ __global__ __launch_bounds__(kBlock) void k_seed_radiation(const MemberArgs* __r
     }
 }
 
-// Moves u, v, w (explicit mode) and rho' from buffer 1 to buffer 0 when the
-// member's substep count leaves them there (n_done odd; n_done < 0 means the
-// member's own count).
+// Moves rho' from buffer 1 to buffer 0 when the member's substep count
+// leaves it there (n_done odd; n_done < 0 means the member's own count).
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_commit(const MemberArgs* __restrict__ ma, const int2* __restrict__ map,
                                                    int n_done)
@@ -961,22 +983,10 @@ __global__ __launch_bounds__(kBlock) void k_commit(const MemberArgs* __restrict_
     const MemberArgs& A = ma[wk.m];
     const int n = n_done < 0 ? A.p.substeps : min(n_done, A.p.substeps);
     if ((n & 1) == 0) return;
-    const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
-    if (t.i >= g.nx || t.j >= g.ny) return;
-    const DevFields& f = A.f;
-    const bool all = A.p.acoustic_mode == NIMBUS_ACOUSTIC_EXPLICIT;
-    for (int k = t.k0; k < t.k1; ++k)
-    {
-        const std::size_t c = g.at(t.i, t.j, k);
-        P<S>(f.rp)[c] = P<S>(f.rp2)[c];
-        if (all)
-        {
-            P<S>(f.u)[c] = P<S>(f.u2)[c];
-            P<S>(f.v)[c] = P<S>(f.v2)[c];
-            P<S>(f.w)[c] = P<S>(f.w2)[c];
-        }
-    }
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
+    P<S>(A.f.rp)[c] = P<S>(A.f.rp2)[c];
 }
 
 // ---------------------------------------------------------------------
@@ -1007,8 +1017,7 @@ __device__ __forceinline__ float* limiter_slot(const DevFields& f, int pd_mask, 
 
 // Face flux pass (advect_conservative_scalar_flux): each cell writes the raw
 // flux through its own upper x, y and z faces, stored in the storage type
-// like the reference's SFX/SFY/SFZ scratch. The own column's scalar is held
-// as a window k-2 .. k+3.
+// like the reference's SFX/SFY/SFZ scratch.
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_transport_flux(const MemberArgs* __restrict__ ma,
                                                            const int2* __restrict__ map, int n_scalars)
@@ -1016,58 +1025,44 @@ __global__ __launch_bounds__(kBlock) void k_transport_flux(const MemberArgs* __r
     using A_ = typename AccT<S>::type;
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const bool valid = t.i < NX && t.j < NY;
-    const int i = min(t.i, NX - 1), j = min(t.j, NY - 1);
+    const int i = cl.i, j = cl.j, k = cl.k;
     const DevFields& f = A.f;
-    const std::size_t sz = g.plane();
-    const std::size_t col = g.at(i, j, 0);
+    const std::ptrdiff_t c = std::ptrdiff_t(g.at(i, j, k));
     const bool open_top = A.p.bc_top == NIMBUS_TOP_OPEN;
-    const S* __restrict__ mfx = P<S>(f.mfx);
-    const S* __restrict__ mfy = P<S>(f.mfy);
-    const S* __restrict__ mfz = P<S>(f.mfz);
-    int xo[6], yo[6];
+    const A_ mx = A_(P<S>(f.mfx)[c]), my = A_(P<S>(f.mfy)[c]), mz = A_(P<S>(f.mfz)[c]);
+    std::ptrdiff_t xo[6], yo[6], zo[6];
 #pragma unroll
     for (int o = 0; o < 6; ++o)
     {
         xo[o] = clampi(i + o - 2, 0, NX - 1) - i;
-        yo[o] = (clampi(j + o - 2, 0, NY - 1) - j) * g.px;
+        yo[o] = std::ptrdiff_t(clampi(j + o - 2, 0, NY - 1) - j) * g.px;
+        zo[o] = std::ptrdiff_t(clampi(k + o - 2, 0, NZ - 1) - k) * std::ptrdiff_t(g.plane());
     }
-    for (int q = 0; q < n_scalars; ++q)
+#pragma unroll
+    for (int q = 0; q < kMaxScalars; ++q)
     {
+        if (q >= n_scalars) break;
         const S* __restrict__ qs = scalar_in<S>(f, q);
-        S* __restrict__ fx = P<S>(f.flux[3 * q]);
-        S* __restrict__ fy = P<S>(f.flux[3 * q + 1]);
-        S* __restrict__ fz = P<S>(f.flux[3 * q + 2]);
-        A_ zw[6];
-#pragma unroll
-        for (int o = 0; o < 5; ++o) zw[o + 1] = A_(qs[col + std::size_t(clampi(t.k0 + o - 2, 0, NZ - 1)) * sz]);
-        for (int k = t.k0; k < t.k1; ++k)
-        {
-#pragma unroll
-            for (int o = 0; o < 5; ++o) zw[o] = zw[o + 1];
-            zw[5] = A_(qs[col + std::size_t(clampi(k + 3, 0, NZ - 1)) * sz]);
-            const std::size_t c = col + std::size_t(k) * sz;
-            S sx = S(0), sy_ = S(0), sz_ = S(0);
-            if (i <= NX - 2)
-                sx = S(dev::face_flux6<A_>(A_(qs[c + xo[0]]), A_(qs[c + xo[1]]), zw[2], A_(qs[c + xo[3]]),
-                                           A_(qs[c + xo[4]]), A_(qs[c + xo[5]]), i, NX, A_(mfx[c])));
-            if (j <= NY - 2)
-                sy_ = S(dev::face_flux6<A_>(A_(qs[c + yo[0]]), A_(qs[c + yo[1]]), zw[2], A_(qs[c + yo[3]]),
-                                            A_(qs[c + yo[4]]), A_(qs[c + yo[5]]), j, NY, A_(mfy[c])));
-            if (k <= NZ - 2)
-                sz_ = S(dev::face_flux6<A_>(zw[0], zw[1], zw[2], zw[3], zw[4], zw[5], k, NZ, A_(mfz[c])));
-            else if (open_top)
-                sz_ = S(A_(mfz[c]) * zw[2]);
-            if (valid)
-            {
-                fx[c] = sx;
-                fy[c] = sy_;
-                fz[c] = sz_;
-            }
-        }
+        const A_ q0 = A_(qs[c]);
+        S sx = S(0), sy_ = S(0), sz_ = S(0);
+        if (i <= NX - 2)
+            sx = S(dev::face_flux6<A_>(A_(qs[c + xo[0]]), A_(qs[c + xo[1]]), q0, A_(qs[c + xo[3]]), A_(qs[c + xo[4]]),
+                                       A_(qs[c + xo[5]]), i, NX, mx));
+        if (j <= NY - 2)
+            sy_ = S(dev::face_flux6<A_>(A_(qs[c + yo[0]]), A_(qs[c + yo[1]]), q0, A_(qs[c + yo[3]]),
+                                        A_(qs[c + yo[4]]), A_(qs[c + yo[5]]), j, NY, my));
+        if (k <= NZ - 2)
+            sz_ = S(dev::face_flux6<A_>(A_(qs[c + zo[0]]), A_(qs[c + zo[1]]), q0, A_(qs[c + zo[3]]),
+                                        A_(qs[c + zo[4]]), A_(qs[c + zo[5]]), k, NZ, mz));
+        else if (open_top)
+            sz_ = S(mz * q0);
+        P<S>(f.flux[3 * q])[c] = sx;
+        P<S>(f.flux[3 * q + 1])[c] = sy_;
+        P<S>(f.flux[3 * q + 2])[c] = sz_;
     }
 }
 
@@ -1086,43 +1081,32 @@ __global__ __launch_bounds__(kBlock) void k_transport_limiter(const MemberArgs* 
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    if (t.i >= NX || t.j >= NY) return;
-    const int i = t.i, j = t.j;
+    const int i = cl.i, j = cl.j, k = cl.k;
     const DevFields& f = A.f;
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
+    const std::size_t c = g.at(i, j, k);
     const bool open_top = A.p.bc_top == NIMBUS_TOP_OPEN;
-    const double dt = A.p.dt;
-    const double* rho0 = slot(f.base, kBaseRho0, NZ);
-    const S* rbp = P<S>(f.rb);
+    const double rho_b = slot(f.base, kBaseRho0, NZ)[k] + double(P<S>(f.rb)[c]);
     for (int q = 0; q < n_scalars; ++q)
     {
         if (!((pd_mask >> q) & 1)) continue;
         const S* __restrict__ fx = P<S>(f.flux[3 * q]);
         const S* __restrict__ fy = P<S>(f.flux[3 * q + 1]);
         const S* __restrict__ fz = P<S>(f.flux[3 * q + 2]);
-        const S* __restrict__ qs = scalar_in<S>(f, q);
-        float* rl = limiter_slot(f, pd_mask, q);
-        S fz_below = t.k0 >= 1 ? fz[g.at(i, j, t.k0 - 1)] : S(0);
-        for (int k = t.k0; k < t.k1; ++k)
-        {
-            const std::size_t c = g.at(i, j, k);
-            const S fzc = fz[c];
-            double out_rate = 0.0;
-            if (i <= NX - 2 && fx[c] > S(0)) out_rate += double(fx[c]) * g.idx;
-            if (i >= 1 && fx[c - 1] < S(0)) out_rate -= double(fx[c - 1]) * g.idx;
-            if (j <= NY - 2 && fy[c] > S(0)) out_rate += double(fy[c]) * g.idy;
-            if (j >= 1 && fy[c - sy] < S(0)) out_rate -= double(fy[c - sy]) * g.idy;
-            if ((k <= NZ - 2 || (open_top && k == NZ - 1)) && fzc > S(0)) out_rate += double(fzc) * g.idz;
-            if (k >= 1 && fz_below < S(0)) out_rate -= double(fz_below) * g.idz;
-            double r = 1.0;
-            const double rho_b = rho0[k] + double(rbp[c]);
-            if (out_rate > 0.0) r = fmin(1.0, rho_b * fmax(double(qs[c]), 0.0) / dt / out_rate);
-            rl[c] = pd_store(r);
-            fz_below = fzc;
-        }
+        double out_rate = 0.0;
+        if (i <= NX - 2 && fx[c] > S(0)) out_rate += double(fx[c]) * g.idx;
+        if (i >= 1 && fx[c - 1] < S(0)) out_rate -= double(fx[c - 1]) * g.idx;
+        if (j <= NY - 2 && fy[c] > S(0)) out_rate += double(fy[c]) * g.idy;
+        if (j >= 1 && fy[c - sy] < S(0)) out_rate -= double(fy[c - sy]) * g.idy;
+        if ((k <= NZ - 2 || (open_top && k == NZ - 1)) && fz[c] > S(0)) out_rate += double(fz[c]) * g.idz;
+        if (k >= 1 && fz[c - sz] < S(0)) out_rate -= double(fz[c - sz]) * g.idz;
+        double r = 1.0;
+        if (out_rate > 0.0) r = fmin(1.0, rho_b * fmax(double(scalar_in<S>(f, q)[c]), 0.0) / A.p.dt / out_rate);
+        limiter_slot(f, pd_mask, q)[c] = pd_store(r);
     }
 }
 
@@ -1135,76 +1119,63 @@ __global__ __launch_bounds__(kBlock) void k_transport_update(const MemberArgs* _
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    if (t.i >= NX || t.j >= NY) return;
-    const int i = t.i, j = t.j;
+    const int i = cl.i, j = cl.j, k = cl.k;
     const DevFields& f = A.f;
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
+    const std::size_t c = g.at(i, j, k);
     if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
     {
-        for (int q = 0; q < n_scalars; ++q)
-        {
-            const S* qs = scalar_in<S>(f, q);
-            S* out = scalar_out<S>(f, q);
-            for (int k = t.k0; k < t.k1; ++k) out[g.at(i, j, k)] = qs[g.at(i, j, k)];
-        }
+        for (int q = 0; q < n_scalars; ++q) scalar_out<S>(f, q)[c] = scalar_in<S>(f, q)[c];
         return;
     }
     const bool open_top = A.p.bc_top == NIMBUS_TOP_OPEN;
     const double dt = A.p.dt;
-    const double* rho0 = slot(f.base, kBaseRho0, NZ);
-    const S* rbp = P<S>(f.rb);
-    const S* rep = P<S>(f.rp);
+    const double r0 = slot(f.base, kBaseRho0, NZ)[k];
+    const double rho_b = r0 + double(P<S>(f.rb)[c]);
+    const double rho_e = r0 + double(P<S>(f.rp)[c]);
+    const double den = rho_e > 0.05 ? rho_e : 0.05;
+    const bool has_top = k <= NZ - 2 || (open_top && k == NZ - 1);
     const double ix = g.idx, iy = g.idy, iz = g.idz;
     for (int q = 0; q < n_scalars; ++q)
     {
         const S* __restrict__ fx = P<S>(f.flux[3 * q]);
         const S* __restrict__ fy = P<S>(f.flux[3 * q + 1]);
         const S* __restrict__ fz = P<S>(f.flux[3 * q + 2]);
-        const S* __restrict__ qs = scalar_in<S>(f, q);
-        S* __restrict__ out = scalar_out<S>(f, q);
         const bool pd = (pd_mask >> q) & 1;
         const float* rl = pd ? limiter_slot(f, pd_mask, q) : nullptr;
         auto lim = [&](S raw, std::size_t lo, std::size_t hi) -> double {
             if (!pd || raw == S(0)) return double(raw);
             return double(raw) * double(S(raw > S(0) ? rl[lo] : rl[hi]));
         };
-        for (int k = t.k0; k < t.k1; ++k)
+        const double sxp = lim(fx[c], c, c + 1);
+        const double sxm = lim(fx[c - 1], c - 1, c);
+        const double syp = lim(fy[c], c, c + sy);
+        const double sym = lim(fy[c - sy], c - sy, c);
+        const double szp = has_top ? lim(fz[c], c, k < NZ - 1 ? c + sz : c) : 0.0;
+        const double szm = k >= 1 ? lim(fz[c - sz], c - sz, c) : 0.0;
+        const double q0 = double(scalar_in<S>(f, q)[c]);
+        double rho_phi;
+        if (pd)
         {
-            const std::size_t c = g.at(i, j, k);
-            const double r0 = rho0[k];
-            const double rho_b = r0 + double(rbp[c]);
-            const double rho_e = r0 + double(rep[c]);
-            const double den = rho_e > 0.05 ? rho_e : 0.05;
-            const bool has_top = k <= NZ - 2 || (open_top && k == NZ - 1);
-            const double sxp = lim(fx[c], c, c + 1);
-            const double sxm = lim(fx[c - 1], c - 1, c);
-            const double syp = lim(fy[c], c, c + sy);
-            const double sym = lim(fy[c - sy], c - sy, c);
-            const double szp = has_top ? lim(fz[c], c, k < NZ - 1 ? c + sz : c) : 0.0;
-            const double szm = k >= 1 ? lim(fz[c - sz], c - sz, c) : 0.0;
-            const double q0 = double(qs[c]);
-            double rho_phi;
-            if (pd)
-            {
-                double outg = 0.0, inc = 0.0;
-                if (sxp >= 0.0) outg += sxp * ix; else inc -= sxp * ix;
-                if (sxm <= 0.0) outg -= sxm * ix; else inc += sxm * ix;
-                if (syp >= 0.0) outg += syp * iy; else inc -= syp * iy;
-                if (sym <= 0.0) outg -= sym * iy; else inc += sym * iy;
-                if (has_top) { if (szp >= 0.0) outg += szp * iz; else inc -= szp * iz; }
-                if (k >= 1) { if (szm <= 0.0) outg -= szm * iz; else inc += szm * iz; }
-                rho_phi = (rho_b * q0 - dt * outg) + dt * inc;
-            }
-            else
-            {
-                const double div = (sxp - sxm) * ix + (syp - sym) * iy + (szp - szm) * iz;
-                rho_phi = rho_b * q0 - dt * div;
-            }
-            out[c] = S(rho_phi / den);
+            double outg = 0.0, inc = 0.0;
+            if (sxp >= 0.0) outg += sxp * ix; else inc -= sxp * ix;
+            if (sxm <= 0.0) outg -= sxm * ix; else inc += sxm * ix;
+            if (syp >= 0.0) outg += syp * iy; else inc -= syp * iy;
+            if (sym <= 0.0) outg -= sym * iy; else inc += sym * iy;
+            if (has_top) { if (szp >= 0.0) outg += szp * iz; else inc -= szp * iz; }
+            if (k >= 1) { if (szm <= 0.0) outg -= szm * iz; else inc += szm * iz; }
+            rho_phi = (rho_b * q0 - dt * outg) + dt * inc;
         }
+        else
+        {
+            const double div = (sxp - sxm) * ix + (syp - sym) * iy + (szp - szm) * iz;
+            rho_phi = rho_b * q0 - dt * div;
+        }
+        scalar_out<S>(f, q)[c] = S(rho_phi / den);
     }
 }
 
@@ -1245,7 +1216,7 @@ __device__ KesslerOut<S> kessler_cell(double dz, const KesslerConst& kc, double 
     S prod = S(0);
     if (qcs > S(0))
     {
-        const A factor = A(1) / (A(1) + A(kc.c_accr) * dtA * pow(fmax(A(0), A(qr)), A(0.875)));
+        const A factor = A(1) / (A(1) + A(kc.c_accr) * dtA * dev::fpow(fmax(A(0), A(qr)), A(0.875)));
         const A amount = A(qcs) * (A(1) - factor) + factor * A(kc.c_auto) * dtA * fmax(A(qcs) - A(kc.qc0), A(0));
         prod = S(dev::clampv(amount, A(0), A(qcs)) / dtA);
         o.dqc = S(o.dqc - prod);
@@ -1259,7 +1230,7 @@ __device__ KesslerOut<S> kessler_cell(double dz, const KesslerConst& kc, double 
             const A qr_av = A(qr) + A(prod) * dtA;
             const A rcgs = A(1.0e-3) * rho;
             const A rc = rcgs * fmax(qr_av, A(0));
-            const A vent = (A(1.6) + A(124.9) * pow(rc, A(0.2046))) * pow(rc, A(0.525));
+            const A vent = (A(1.6) + A(124.9) * dev::fpow(rc, A(0.2046))) * dev::fpow(rc, A(0.525));
             const A resist = A(2.55e8) / (p * qsat) + A(5.4e5);
             const A raw = A(kc.evap_eff) * (vent / resist) * ((qsat - A(qvs)) / (rcgs * qsat));
             const A gsat = (A(kLv) * A(kLv) * qsat) / (A(kCp) * A(kRv) * fmax(A(Ts) * A(Ts), A(1)));
@@ -1300,32 +1271,29 @@ __global__ __launch_bounds__(kBlock) void k_kessler_tendencies(const MemberArgs*
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    if (t.i >= NX || t.j >= NY) return;
-    const int i = t.i, j = t.j;
+    const int i = cl.i, j = cl.j, k = cl.k;
     const DevFields& f = A.f;
+    const std::size_t c = g.at(i, j, k);
+    if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
+    {
+        dth[c] = S(0); dqv[c] = S(0); dqc[c] = S(0); dqr[c] = S(0);
+        return;
+    }
     const double* rho0 = slot(f.base, kBaseRho0, NZ);
     const double* p0 = slot(f.base, kBaseP0, NZ);
     const S *th = P<S>(f.th), *qv = P<S>(f.qv), *qc = P<S>(f.qc), *qr = P<S>(f.qr), *rp = P<S>(f.rp),
             *pp = P<S>(f.pp);
     const std::size_t sz = g.plane();
-    for (int k = t.k0; k < t.k1; ++k)
-    {
-        const std::size_t c = g.at(i, j, k);
-        if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
-        {
-            dth[c] = S(0); dqv[c] = S(0); dqc[c] = S(0); dqr[c] = S(0);
-            continue;
-        }
-        const bool up = k + 1 < NZ;
-        const KesslerOut<S> o =
-            kessler_cell<S>(g.dz, A.p.kessler, dt, p0[k] + double(pp[c]), rho0[k] + double(rp[c]),
-                            up ? rho0[k + 1] + double(rp[c + sz]) : 0.0, th[c], qv[c], qc[c], qr[c],
-                            up ? qr[c + sz] : S(0), up);
-        dth[c] = o.dth; dqv[c] = o.dqv; dqc[c] = o.dqc; dqr[c] = o.dqr;
-    }
+    const bool up = k + 1 < NZ;
+    const KesslerOut<S> o =
+        kessler_cell<S>(g.dz, A.p.kessler, dt, p0[k] + double(pp[c]), rho0[k] + double(rp[c]),
+                        up ? rho0[k + 1] + double(rp[c + sz]) : 0.0, th[c], qv[c], qc[c], qr[c],
+                        up ? qr[c + sz] : S(0), up);
+    dth[c] = o.dth; dqv[c] = o.dqv; dqc[c] = o.dqc; dqr[c] = o.dqr;
 }
 
 // Fused microphysics commit: Kessler tendencies, bounded apply
@@ -1333,8 +1301,7 @@ __global__ __launch_bounds__(kBlock) void k_kessler_tendencies(const MemberArgs*
 // (microphysics_step.cpp:1917-2024; kessler.cpp:533-561) and the EOS
 // pressure projection of every cell (eos_apply.cpp:50-110). Subcycle n reads
 // the transported slots (n even) or the home slots (n odd) and writes the
-// other set; a column marches downward so the level above is still the
-// subcycle's input.
+// other set, so the level above is still the subcycle's input.
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __restrict__ ma,
                                                           const int2* __restrict__ map, int n)
@@ -1342,11 +1309,11 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     if (n >= A.p.kessler_sub) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
     const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    if (t.i >= NX || t.j >= NY) return;
-    const int i = t.i, j = t.j;
+    const int i = cl.i, j = cl.j, k = cl.k;
     const DevFields& f = A.f;
     const bool from_temp = (n & 1) == 0;
     const S* th_in = P<S>(from_temp ? f.tth : f.th);
@@ -1365,47 +1332,37 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     const Bounds bd = A.p.bounds;
     const double h = A.p.dt / A.p.kessler_sub;
     const bool do_eos = n == A.p.kessler_sub - 1;
-    const bool phys = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
-    const std::size_t col = g.at(i, j, 0), sz = g.plane();
-    // level above the chunk (input values of this subcycle)
-    S qr_up = t.k1 < NZ ? qr_in[col + std::size_t(t.k1) * sz] : S(0);
-    double rho_up = t.k1 < NZ ? rho0[t.k1] + double(rp[col + std::size_t(t.k1) * sz]) : 0.0;
-    for (int k = t.k1 - 1; k >= t.k0; --k)
+    const std::size_t c = g.at(i, j, k), sz = g.plane();
+    const double rho = rho0[k] + double(rp[c]);
+    S thn = th_in[c], qvn = qv_in[c], qcn = qc_in[c], qrn = qr_in[c];
+    if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1)
     {
-        const std::size_t c = col + std::size_t(k) * sz;
-        const double rho = rho0[k] + double(rp[c]);
-        S thn = th_in[c], qvn = qv_in[c], qcn = qc_in[c], qrn = qr_in[c];
-        const S qr_here = qrn;
-        if (phys)
+        const bool up = k + 1 < NZ;
+        const KesslerOut<S> o =
+            kessler_cell<S>(g.dz, kc, h, p0[k] + double(pp[c]), rho, up ? rho0[k + 1] + double(rp[c + sz]) : 0.0,
+                            thn, qvn, qcn, qrn, up ? qr_in[c + sz] : S(0), up);
+        if (k == 0)
         {
-            const bool up = k + 1 < NZ;
-            const KesslerOut<S> o = kessler_cell<S>(g.dz, kc, h, p0[k] + double(pp[c]), rho, up ? rho_up : 0.0,
-                                                    thn, qvn, qcn, qrn, up ? qr_up : S(0), up);
-            if (k == 0)
+            const double q0 = double(qrn);
+            if (q0 > 0.0)
             {
-                const double q0 = double(qrn);
-                if (q0 > 0.0)
-                {
-                    double Vt = fmin(double(dev::kessler_rain_speed<double>(q0, rho)) * sqrt(1.2 / rho), kc.vt_max);
-                    const double sink = double(S(S(Vt) * qrn)) / g.dz;
-                    f.rain[std::size_t(j) * NX + i] += sink * rho * g.dz * h;
-                }
+                double Vt = fmin(double(dev::kessler_rain_speed<double>(q0, rho)) * sqrt(1.2 / rho), kc.vt_max);
+                const double sink = double(S(S(Vt) * qrn)) / g.dz;
+                f.rain[std::size_t(j) * NX + i] += sink * rho * g.dz * h;
             }
-            const double step = fmin(fmax(double(o.dth) * h, -50.0), 50.0);
-            thn = S(clamp_store<S>(double(thn) + step, 300.0, 200.0f, 700.0f));
-            qvn = S(clamp_store<S>(double(qvn) + double(o.dqv) * h, 0.0, 0.0f, 0.05f));
-            qcn = S(clamp_store<S>(double(qcn) + double(o.dqc) * h, 0.0, 0.0f, 0.05f));
-            qrn = S(clamp_store<S>(double(qrn) + double(o.dqr) * h, 0.0, 0.0f, 0.05f));
         }
-        th[c] = thn; qv[c] = qvn; qc[c] = qcn; qr[c] = qrn;
-        const double thv = double(thn);
-        if (do_eos && rho > 0.0 && thv > 0.0)
-        {
-            const double pd = dev::eos_pressure(rho, thv);
-            if (isfinite(pd) && pd > 0.0) pp[c] = S(clamp_store<S>(pd, 101325.0, bd.p_min, bd.p_max) - p0[k]);
-        }
-        qr_up = qr_here;
-        rho_up = rho;
+        const double step = fmin(fmax(double(o.dth) * h, -50.0), 50.0);
+        thn = S(clamp_store<S>(double(thn) + step, 300.0, 200.0f, 700.0f));
+        qvn = S(clamp_store<S>(double(qvn) + double(o.dqv) * h, 0.0, 0.0f, 0.05f));
+        qcn = S(clamp_store<S>(double(qcn) + double(o.dqc) * h, 0.0, 0.0f, 0.05f));
+        qrn = S(clamp_store<S>(double(qrn) + double(o.dqr) * h, 0.0, 0.0f, 0.05f));
+    }
+    th[c] = thn; qv[c] = qvn; qc[c] = qcn; qr[c] = qrn;
+    const double thv = double(thn);
+    if (do_eos && rho > 0.0 && thv > 0.0)
+    {
+        const double pd = dev::eos_pressure(rho, thv);
+        if (isfinite(pd) && pd > 0.0) pp[c] = S(clamp_store<S>(pd, 101325.0, bd.p_min, bd.p_max) - p0[k]);
     }
 }
 
@@ -1417,18 +1374,14 @@ __global__ __launch_bounds__(kBlock) void k_scalars_home(const MemberArgs* __res
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     if (A.p.kessler_sub & 1) return;
-    const DevGrid& g = A.g;
-    const Tile t = tile_of(A, wk.local, A.geo.k_chunks);
-    if (t.i >= g.nx || t.j >= g.ny) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
     const DevFields& f = A.f;
-    for (int k = t.k0; k < t.k1; ++k)
-    {
-        const std::size_t c = g.at(t.i, t.j, k);
-        P<S>(f.th)[c] = P<S>(f.tth)[c];
-        P<S>(f.qv)[c] = P<S>(f.tqv)[c];
-        P<S>(f.qc)[c] = P<S>(f.tqc)[c];
-        P<S>(f.qr)[c] = P<S>(f.tqr)[c];
-    }
+    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
+    P<S>(f.th)[c] = P<S>(f.tth)[c];
+    P<S>(f.qv)[c] = P<S>(f.tqv)[c];
+    P<S>(f.qc)[c] = P<S>(f.tqc)[c];
+    P<S>(f.qr)[c] = P<S>(f.tqr)[c];
 }
 
 // ---------------------------------------------------------------------
@@ -1547,8 +1500,6 @@ void fill_geometry(MemberArgs& a, int tile_w, int k_chunks, int xcd_remap)
     G.tile_h = kBlock / G.tile_w;
     G.tiles_x = (a.g.nx + G.tile_w - 1) / G.tile_w;
     G.tiles_y = (a.g.ny + G.tile_h - 1) / G.tile_h;
-    G.strips_x = (a.g.nx + 62) / 63;
-    G.strips_y = (a.g.ny + 3) / 4;
     G.k_chunks = std::max(1, std::min(k_chunks, a.g.nz));
     G.xcd_remap = xcd_remap ? 1 : 0;
 }
@@ -1560,10 +1511,8 @@ int work_items(const MemberArgs& a, WorkKind kind)
     const auto lin = [](std::size_t n) { return int((n + kBlock - 1) / kBlock); };
     switch (kind)
     {
-    case W_PLANE: return tiles * a.g.nz;
-    case W_CHUNK: return tiles * G.k_chunks;
+    case W_CELL: return tiles * G.k_chunks * ((a.g.nz + G.k_chunks - 1) / G.k_chunks);
     case W_COLUMN: return tiles;
-    case W_STRIP: return G.strips_x * G.strips_y * G.k_chunks;
     case W_WE: return lin(std::size_t(a.g.ny) * a.g.nz);
     case W_SN: return lin(std::size_t(a.g.nx) * a.g.nz);
     case W_SEED: return lin(std::size_t(std::max(a.g.nx, a.g.ny)) * a.g.nz);
@@ -1632,24 +1581,24 @@ S* PS(void* p)
 template <class S>
 void launch_slow_rk2(Launcher& L, const BatchView& b)
 {
-    launch_map(L, "slow_stage1", k_slow<S, 0>, b, S_ALL, W_CHUNK, (S*)nullptr, (S*)nullptr, (S*)nullptr);
-    launch_map(L, "slow_stage2", k_slow<S, 1>, b, S_ALL, W_CHUNK, (S*)nullptr, (S*)nullptr, (S*)nullptr);
+    launch_map(L, "slow_stage1", k_slow<S, 0>, b, S_ALL, W_CELL, (S*)nullptr, (S*)nullptr, (S*)nullptr);
+    launch_map(L, "slow_stage2", k_slow<S, 1>, b, S_ALL, W_CELL, (S*)nullptr, (S*)nullptr, (S*)nullptr);
 }
 
 template <class S>
 void launch_slow_tendencies(Launcher& L, const BatchView& b, void* du, void* dv, void* dw)
 {
-    launch_map(L, "slow_tendencies", k_slow<S, 2>, b, S_ALL, W_CHUNK, PS<S>(du), PS<S>(dv), PS<S>(dw));
+    launch_map(L, "slow_tendencies", k_slow<S, 2>, b, S_ALL, W_CELL, PS<S>(du), PS<S>(dv), PS<S>(dw));
 }
 
 template <class S>
 void launch_acoustic_substep(Launcher& L, const BatchView& b, int s, bool accumulate, bool init_at_first)
 {
-    launch_map(L, "acoustic_substep", k_acoustic_fused<S>, b, S_EXPLICIT, W_STRIP, s, int(accumulate),
+    launch_map(L, "acoustic_forward", k_acoustic_forward<S>, b, S_EXPLICIT, W_CELL, s, int(accumulate),
                int(init_at_first));
     launch_map(L, "acoustic_hevi_column", k_hevi_column<S>, b, S_HEVI, W_COLUMN, s, int(accumulate),
                int(init_at_first));
-    launch_map(L, "acoustic_hevi_momentum", k_hevi_momentum<S>, b, S_HEVI, W_CHUNK, s);
+    launch_map(L, "acoustic_momentum", k_momentum<S>, b, S_ALL, W_CELL, s);
 }
 
 template <class S>
@@ -1669,7 +1618,7 @@ void launch_seed_radiation(Launcher& L, const BatchView& b)
 template <class S>
 void launch_commit(Launcher& L, const BatchView& b, int n_done)
 {
-    launch_map(L, "acoustic_commit", k_commit<S>, b, S_ALL, W_CHUNK, n_done);
+    launch_map(L, "acoustic_commit", k_commit<S>, b, S_ALL, W_CELL, n_done);
 }
 
 template <class S>
@@ -1678,16 +1627,16 @@ void launch_transport(Launcher& L, const BatchView& b, int n_scalars, const int*
     (void)into_temp;
     int mask = 0;
     for (int q = 0; q < n_scalars; ++q) mask |= (pd[q] ? 1 : 0) << q;
-    launch_map(L, "transport_flux", k_transport_flux<S>, b, S_ALL, W_CHUNK, n_scalars);
-    if (mask) launch_map(L, "transport_limiter", k_transport_limiter<S>, b, S_ALL, W_CHUNK, n_scalars, mask);
-    launch_map(L, "transport_update", k_transport_update<S>, b, S_ALL, W_CHUNK, n_scalars, mask);
+    launch_map(L, "transport_flux", k_transport_flux<S>, b, S_ALL, W_CELL, n_scalars);
+    if (mask) launch_map(L, "transport_limiter", k_transport_limiter<S>, b, S_ALL, W_CELL, n_scalars, mask);
+    launch_map(L, "transport_update", k_transport_update<S>, b, S_ALL, W_CELL, n_scalars, mask);
 }
 
 template <class S>
 void launch_kessler_tendencies(Launcher& L, const BatchView& b, double dt, void* dth, void* dqv, void* dqc,
                                void* dqr)
 {
-    launch_map(L, "kessler_tendencies", k_kessler_tendencies<S>, b, S_ALL, W_CHUNK, dt, PS<S>(dth), PS<S>(dqv),
+    launch_map(L, "kessler_tendencies", k_kessler_tendencies<S>, b, S_ALL, W_CELL, dt, PS<S>(dth), PS<S>(dqv),
                PS<S>(dqc), PS<S>(dqr));
 }
 
@@ -1706,8 +1655,8 @@ void launch_large_step(Launcher& L, const BatchView& b)
     const int pd[4] = {0, 1, 1, 1};
     launch_transport<S>(L, b, 4, pd, true);
     for (int n = 0; n < b.max_kessler_sub; ++n)
-        launch_map(L, "kessler_apply", k_kessler_apply<S>, b, S_ALL, W_CHUNK, n);
-    launch_map(L, "scalars_home", k_scalars_home<S>, b, S_ALL, W_CHUNK);
+        launch_map(L, "kessler_apply", k_kessler_apply<S>, b, S_ALL, W_CELL, n);
+    launch_map(L, "scalars_home", k_scalars_home<S>, b, S_ALL, W_CELL);
     launch_boundaries<S>(L, b, -1);
 }
 

@@ -36,13 +36,12 @@ struct Launcher
     void end(const char* name, hipEvent_t a);
 };
 
-// Device field set of one member (precision-erased pointers). u, v, w and
-// rho' have a second buffer: the fused explicit acoustic substep reads one
-// and writes the other, so substep s leaves them in buffer (s + 1) & 1.
+// Device field set of one member (precision-erased pointers). rho' has a
+// second buffer: acoustic substep s reads buffer s & 1 and writes (s + 1) & 1.
 struct DevFields
 {
     void *u, *v, *w, *rp, *pp, *th, *qv, *qc, *qr;       // prognostic state (rho, p as perturbations)
-    void *u2, *v2, *w2, *rp2;                            // second acoustic buffers (u2..w2 alias the predictor)
+    void *u2, *v2, *w2, *rp2;                            // Heun predictor u, v, w; second rho' buffer
     void *mfx, *mfy, *mfz, *rb;                          // retained mass fluxes, rho' at step begin
     void *k1u, *k1v, *k1w;                               // Heun stage-1 tendencies
     void *tth, *tqv, *tqc, *tqr;                         // transported scalars before microphysics
@@ -70,14 +69,13 @@ struct StepParams
     Bounds bounds;
 };
 
-// Launch geometry of one member. Tiles are tile_w x (256 / tile_w) columns
-// (four waves of 64 lanes); strips are 63 owned columns plus one halo lane
-// per wave, four rows per workgroup. Column marches are split into
-// k_chunks vertical chunks.
+// Launch geometry of one member: workgroups of tile_w x (256 / tile_w)
+// columns (four waves of 64 lanes). Per-cell kernels visit the levels of a
+// column tile consecutively within each of k_chunks vertical chunks
+// (k_chunks = nz reproduces plane-by-plane order).
 struct MemberGeom
 {
     int tile_w, tile_h, tiles_x, tiles_y;
-    int strips_x, strips_y;
     int k_chunks;
     int xcd_remap;
 };
@@ -93,15 +91,13 @@ struct MemberArgs
 // Work shapes of a launch: what one map entry (one workgroup) covers.
 enum WorkKind
 {
-    W_PLANE = 0,   // one tile of one k plane
-    W_CHUNK = 1,   // one tile, k chunk of the stencil marches
-    W_COLUMN = 2,  // one tile, whole columns
-    W_STRIP = 3,   // one strip block, k chunk
-    W_WE = 4,      // 256 (j, k) rows of the west/east faces
-    W_SN = 5,      // 256 (i, k) rows of the south/north faces
-    W_SEED = 6,    // 256 entries of the radiative stashes
-    W_DIAG = 7,    // one of kDiagBlocksPerMember reduction blocks
-    W_KINDS = 8
+    W_CELL = 0,    // one tile of columns at one level (one cell per thread)
+    W_COLUMN = 1,  // one tile, whole columns (one column per thread)
+    W_WE = 2,      // 256 (j, k) rows of the west/east faces
+    W_SN = 3,      // 256 (i, k) rows of the south/north faces
+    W_SEED = 4,    // 256 entries of the radiative stashes
+    W_DIAG = 5,    // one of kDiagBlocksPerMember reduction blocks
+    W_KINDS = 6
 };
 
 // Member subsets a launch can be restricted to.
@@ -151,7 +147,7 @@ template <class S>
 void launch_boundaries(Launcher& L, const BatchView& b, int s);
 template <class S>
 void launch_seed_radiation(Launcher& L, const BatchView& b);
-// Moves u, v, w, rho' from buffer 1 to buffer 0 for members whose last substep (n_done - 1) left them there.
+// Moves rho' from buffer 1 to buffer 0 for members whose last substep (n_done - 1) left it there.
 template <class S>
 void launch_commit(Launcher& L, const BatchView& b, int n_done);
 template <class S>

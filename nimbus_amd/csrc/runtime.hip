@@ -98,14 +98,13 @@ DevGrid dev_grid(const nimbus_grid& g, int elem)
     return d;
 }
 
-// Default vertical chunking: enough column-march workgroups to give every
-// CU several (about 2048 over the member), chunks of at least 8 planes.
+// Default ordering of the per-cell work: vertical chunks of about 20 levels,
+// so a column tile's levels run back to back while the neighbouring tiles
+// in y (one tile row later) are still close enough to share the L2.
 int default_k_chunks(const DevGrid& g, int tile_w)
 {
-    const int th = 256 / tile_w;
-    const int tiles = ((g.nx + tile_w - 1) / tile_w) * ((g.ny + th - 1) / th);
-    const int want = (2048 + tiles - 1) / tiles;
-    return std::max(1, std::min(want, g.nz / 8));
+    (void)tile_w;
+    return std::max(1, (g.nz + 10) / 20);
 }
 
 bool valid_grid(const nimbus_grid& g)
@@ -876,8 +875,9 @@ int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int
     const std::uint32_t ids[1] = {id};
     const int nz = m->g().nz;
     std::vector<int> chunks;
-    for (int c : {1, 2, 4, 6, 8, 12, 16})
+    for (int c : {1, 2, 3, 5, 8, 12})
         if (c <= std::max(1, nz / 4)) chunks.push_back(c);
+    chunks.push_back(nz);  // plane-by-plane order
     for (int w : {64, 32})
         for (int c : chunks)
             for (int r : {1, 0})

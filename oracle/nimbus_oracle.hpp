@@ -806,7 +806,8 @@ inline bool thomas_solve(int n, const double* a, const double* d, const double* 
 
 // Coefficients of the HE-VI w system for one column. Inputs per level k:
 // rho_bar[k] (face density k+1/2, k = 0..nz-2), C[k] = dp/drho at fixed theta
-// (cell k), G[k] explicit vertical PGF at face k+1/2 from p*, e[k] explicit
+// (cell k), G[k] explicit vertical PGF at face k+1/2 from alpha p* + (1 - alpha) p^n minus
+// the base-state gradient, e[k] explicit
 // part of the vertical density change of cell k, w_old[k]. Unknowns are the
 // interior faces k = 0..nz-2 (ground and lid fluxes are zero).
 inline void hevi_column_system(int nz, double dt, double dz, double alpha, const double* rho_bar,
@@ -828,11 +829,17 @@ inline void hevi_column_system(int nz, double dt, double dz, double alpha, const
 
 // Vertically implicit (HE-VI) acoustic substep. Horizontal continuity and
 // momentum are explicit forward-backward as in the reference substep; the
-// vertical w-rho-p coupling is off-centred implicit (alpha = (1+beta)/2)
-// with p linearised about p* = EOS(rho*), solved per column by an FP64 Thomas
-// sweep, after which rho is updated in flux form and p re-diagnosed from the
-// EOS. The vertical mass flux alpha*w_new + (1-alpha)*w_old is returned in
-// `wflux_weighted` for the transport mass-flux accumulation.
+// vertical w-rho-p coupling is off-centred implicit (alpha = (1+beta)/2):
+// the vertical pressure gradient acts on alpha p^{n+1} + (1 - alpha) p^n,
+// where p^n is the pressure at the start of the substep and p^{n+1} is
+// linearised about p* = EOS(rho*), rho* = rho^n plus the explicit horizontal
+// divergence. (Off-centring against p* instead of p^n makes the horizontal
+// divergence enter the old-level part and is unstable: von Neumann
+// amplification up to 1.24 at horizontal Courant 0.7, alpha = 0.55.) The w
+// system is solved per column by an FP64 Thomas sweep, after which rho is
+// updated in flux form and p re-diagnosed from the EOS. The vertical mass
+// flux alpha*w_new + (1-alpha)*w_old is returned in `wz_eff` for the
+// transport mass-flux accumulation.
 template <class S>
 void acoustic_substep_hevi(const Grid& g, const Base& b, State<S>& s, double dts, double beta,
                            std::vector<double>& wz_eff)
@@ -863,7 +870,9 @@ void acoustic_substep_hevi(const Grid& g, const Base& b, State<S>& s, double dts
             for (int k = 0; k < NZ - 1; ++k)
             {
                 rb[k] = 0.5 * (double(rho_old[g.at(i, j, k)]) + double(rho_old[g.at(i, j, k + 1)]));
-                G[k] = (pstar[k + 1] - pstar[k]) * inv_dz - (b.p0[k + 1] - b.p0[k]) * inv_dz;
+                const double pn0 = double(s.p[g.at(i, j, k)]), pn1 = double(s.p[g.at(i, j, k + 1)]);
+                G[k] = (alpha * (pstar[k + 1] - pstar[k]) + (1.0 - alpha) * (pn1 - pn0)) * inv_dz -
+                       (b.p0[k + 1] - b.p0[k]) * inv_dz;
             }
             rb[NZ - 1] = 0.0;
             for (int k = 0; k < NZ; ++k)

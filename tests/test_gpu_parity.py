@@ -218,19 +218,36 @@ def _run_member(ctx, g, base, cfg, st, precision, n_steps):
         m.destroy()
 
 
+# The reference keeps the KW78 radiative boundary state in FP32 even when the
+# solver runs in FP64 (boundary_conditions_cartesian_cgrid.cpp), so two FP64
+# runs whose arithmetic differs only in operation order (FMA contraction on
+# the device, none in the oracle) may round a stashed face velocity to
+# neighbouring FP32 values. One FP32 ulp of a face velocity then reaches u, v
+# and w next to the radiative faces within the step; the other fields see it
+# only through the density. With non-radiative lateral boundaries the bound is
+# 1e-9 of each field's scale everywhere.
+FP64_SCALE = {"u": 30.0, "v": 30.0, "w": 10.0, "rho": 1.0, "p": 1e5, "theta": 300.0, "qv": 0.02, "qc": 1e-3,
+              "qr": 1e-3}
+
+
+def check_fp64(out, ref, label, radiative):
+    for f in L.FIELDS:
+        tol = (4 * 2.0 ** -23 if f in ("u", "v", "w") else 1e-8) if radiative else 1e-9
+        d = np.abs(np.asarray(out[f], np.float64) - np.asarray(ref[f], np.float64)) / FP64_SCALE[f]
+        err = float(d.max())
+        loc = np.unravel_index(int(np.argmax(d)), d.shape)
+        print(f"{label} {f}: scaled max diff {err:.3e} at (i, j, k) = {tuple(int(x) for x in loc)}")
+        assert err < tol, f
+
+@pytest.mark.parametrize("lateral", [L.BC_RADIATIVE, L.BC_RIGID_SLIP])
 @pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
-def test_large_step_fp64_matches_fp64_oracle(hip_ctx, mode):
+def test_large_step_fp64_matches_fp64_oracle(hip_ctx, mode, lateral):
     g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, dtype=np.float64)
-    cfg = L.step_config(3.0, 6, acoustic_mode=mode)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode, lateral=lateral)
     n_steps = 4
     ref, n = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
     out, diag = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, n_steps)
-    scale = {"u": 30.0, "v": 30.0, "w": 10.0, "rho": 1.0, "p": 1e5, "theta": 300.0, "qv": 0.02, "qc": 1e-3,
-             "qr": 1e-3}
-    for f in L.FIELDS:
-        err = max_abs(out[f], ref[f]) / scale[f]
-        print(f"fp64 mode {mode} {f}: scaled max diff {err:.3e}")
-        assert err < 1e-9, f
+    check_fp64(out, ref, f"fp64 mode {mode} lateral {lateral}", lateral == L.BC_RADIATIVE)
     assert max_abs(out["rain_mm"], ref["rain_mm"]) < 1e-9 * max(1.0, float(np.max(ref["rain_mm"])))
 
 
@@ -302,17 +319,12 @@ def test_large_step_odd_substep_count_fp64(hip_ctx, mode):
     """An odd substep count leaves the acoustic state in the second buffer; the
     commit must bring it home before transport and microphysics."""
     g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, dtype=np.float64)
-    cfg = L.step_config(2.0, 5, acoustic_mode=mode)
+    cfg = L.step_config(2.0, 5, acoustic_mode=mode, lateral=L.BC_ZERO_GRADIENT)
     assert L.resolve_substeps(g, base, cfg) % 2 == 1
     n_steps = 3
     ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
     out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, n_steps)
-    scale = {"u": 30.0, "v": 30.0, "w": 10.0, "rho": 1.0, "p": 1e5, "theta": 300.0, "qv": 0.02, "qc": 1e-3,
-             "qr": 1e-3}
-    for f in L.FIELDS:
-        err = max_abs(out[f], ref[f]) / scale[f]
-        print(f"odd substeps mode {mode} {f}: scaled max diff {err:.3e}")
-        assert err < 1e-9, f
+    check_fp64(out, ref, f"odd substeps mode {mode}", False)
 
 
 @pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
@@ -333,7 +345,7 @@ def test_launch_geometry_does_not_change_results(hip_ctx, mode):
         finally:
             m.destroy()
     for res in results[1:]:
-        for f in L.FIELDS + ["rain_mm"]:
+        for f in list(L.FIELDS) + ["rain_mm"]:
             assert np.array_equal(res[f], results[0][f]), f
 
 

@@ -16,6 +16,8 @@ import ctypes
 import numpy as np
 import pytest
 
+from nimbus_amd import lib as L
+from nimbus_amd import workloads as W
 from oracle import oracle_py as O
 
 DP = ctypes.POINTER(ctypes.c_double)
@@ -161,6 +163,9 @@ def test_hevi_vertical_mode_amplification_matches_theta_method(beta, courant):
     dt = courant * dz / c
     k = np.arange(nz)
     st["rho"][:, :, :] = 1.0 + eps * np.cos(m * np.pi * (k + 0.5) / nz)
+    # the substep-start pressure is the EOS of the state, as the large step leaves it
+    eos = np.vectorize(O.lib().oracle_eos_pressure)
+    st["p"] = eos(st["rho"], st["theta"])
     n_steps = 4
     # horizontally uniform columns with zero-gradient lateral closure between
     # substeps: only the vertical acoustic mode is active
@@ -175,6 +180,50 @@ def test_hevi_vertical_mode_amplification_matches_theta_method(beta, courant):
     assert np.max(np.abs(got - expect)) < 1e-3 * eps, (got[:4], expect[:4])
     if beta > 0:
         assert abs(gfac) < 1.0
+
+
+
+def _acoustic_pulse(nx=40, ny=5, nz=40, dx=1000.0, dz=500.0, amp=1e-3):
+    g = L.make_grid(nx, ny, nz, dx, dx, dz)
+    base = L.base_state(g)
+    st = L.initial_state(g, L.ic_params(g, bubbles=[], noise_amp=0.0), np.float64)
+    x = (np.arange(nx) - nx / 2)[:, None, None]
+    z = (np.arange(nz) - nz / 3)[None, None, :]
+    st["rho"] = st["rho"] * (1.0 + amp * np.exp(-(x ** 2 + z ** 2) / 8.0))
+    return O.grid(nx, ny, nz, dx, dx, dz), base, st
+
+
+def test_hevi_two_dimensional_acoustics_stay_bounded():
+    """An x-z density pulse under HE-VI substeps alone (horizontal Courant 0.35,
+    vertical 0.7, beta = 0.1) for 600 substeps: the off-centring acts on the
+    substep-start pressure, so the coupled horizontal-vertical acoustic modes
+    do not grow (off-centring against the horizontally updated p* would grow
+    them by up to 24 % per substep)."""
+    g, base, st = _acoustic_pulse()
+    dts, n = 1.0, 100
+    cfg = O.config(6 * n * dts, 6 * n, acoustic_mode=1, hevi_beta=0.1, lateral=2)
+    peaks = []
+    for _ in range(6):
+        st, _ = O.acoustic_substeps_batched(g, base, cfg, st, n, dts, 0.0, dtype=np.float64)
+        peaks.append(float(np.max(np.abs(st["w"]))))
+    assert np.all(np.isfinite(peaks))
+    assert max(peaks[1:]) < 1.5 * peaks[0], peaks
+
+
+def test_hevi_large_steps_track_the_explicit_scheme():
+    """Method change only: on a coarse warm-bubble case the HE-VI and explicit
+    large steps give the same updraft to within 10 % after 20 steps (6 vs 10
+    acoustic substeps per 6 s step)."""
+    wl = W.Workload("t", 24, 24, 40, 1000.0, 500.0, 6.0, 6, 20)
+    g = wl.grid
+    og = O.grid(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz)
+    peaks = {}
+    for mode in (0, 1):
+        cfg = O.from_step_config(wl.config(acoustic_mode=mode))
+        st, _ = O.large_steps(og, wl.base(), cfg, wl.initial_state(np.float64), 20, np.float64)
+        peaks[mode] = float(np.max(st["w"]))
+    assert 0.5 < peaks[0] < 5.0
+    assert abs(peaks[1] - peaks[0]) < 0.1 * peaks[0], peaks
 
 
 # ------------------------------------------------------------------ transport
