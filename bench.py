@@ -40,12 +40,39 @@ HBM_PEAK_GBS = 8000.0
 # its flux (7 s), limiter (1 s) and update (5 s) passes; Kessler with
 # sedimentation, apply and EOS 11 s; slow stage 1 + predictor 16 s; stage 2 +
 # average + apply 28 s. Face-flux and Thomas scratch are not algorithmic.
-ALGO_BYTES_PER_CELL = {
-    "acoustic_substep": 17, "acoustic_hevi_column": 13, "acoustic_hevi_momentum": 4,
-    "transport_flux": 7, "transport_limiter": 1, "transport_update": 5,
-    "kessler_apply": 11,
-    "slow_stage1": 16, "slow_stage2": 28,
+# Algorithmic HBM traffic per cell of each kernel (one launch): storage
+# elements read or written once, plus FP64 bytes. Neighbour values that the
+# caches serve are not counted. Per acoustic mode where the kernel differs.
+ALGO_PER_CELL = {
+    # read u v w rho' theta mfx mfy mfz; write rho'(new) p' mfx mfy mfz
+    "acoustic_forward": (13, 0),
+    # read rho'(new) p' u v w; write u v, and w for explicit members
+    "acoustic_momentum": {"explicit": (8, 0), "hevi": (7, 0)},
+    # read rho' u v w theta p' mfx mfy mfz; write rho' p' w mfx mfy mfz;
+    # the FP64 Thomas coefficients c', d' are written and read back (32 B)
+    "acoustic_hevi_column": (15, 32),
+    # read mfx mfy mfz and four scalars; write 3 face fluxes per scalar
+    "transport_flux": (19, 0),
+    # read rho_b, and fluxes and value of the three moisture scalars; write 3 limiter ratios
+    "transport_limiter": (16, 0),
+    # read rho_b rho_e, 12 fluxes, 3 ratios, 4 scalars; write 4 scalars
+    "transport_update": (25, 0),
+    # read theta qv qc qr rho' p'; write theta qv qc qr p'
+    "kessler_apply": (11, 0),
+    # read u v w rho' qv qc qr; write k1 (3) and the predictor (3)
+    "slow_stage1": (13, 0),
+    # read predictor (3) rho' qv qc qr, k1 (3), u v w; write u v w
+    "slow_stage2": (16, 0),
 }
+
+
+def algo_bytes_per_cell(name, mode, elem):
+    e = ALGO_PER_CELL.get(name)
+    if isinstance(e, dict):
+        e = e.get(mode)
+    if e is None:
+        return None
+    return e[0] * elem + e[1]
 
 
 def cpu_info():
@@ -129,11 +156,11 @@ def measured_traffic(workload, mode, kernel):
 def roofline_for(tab, cells, elem, workload, mode):
     """Dominant kernel of a step: algorithmic bytes per launch / measured launch time."""
     name, (launches, ms) = max(tab.items(), key=lambda kv: kv[1][1])
-    per_cell = ALGO_BYTES_PER_CELL.get(name)
+    per_cell = algo_bytes_per_cell(name, mode, elem)
     if per_cell is None:
         return None
     avg_ms = ms / launches
-    algo = per_cell * elem * cells
+    algo = per_cell * cells
     achieved = algo / (avg_ms / 1e3) / 1e9
     traffic = measured_traffic(workload, mode, name)
     total_ms = sum(v[1] for v in tab.values())

@@ -62,7 +62,7 @@ typedef struct nimbus_hip_ctx nimbus_hip_ctx;
 
 typedef struct
 {
-    int nx, ny, nz;       /* cells including lateral ghosts */
+    int nx, ny, nz;       /* cells including lateral ghosts; nx, ny >= 5, 4 <= nz <= 1024 */
     double dx, dy, dz;    /* metres */
 } nimbus_grid;
 
@@ -76,6 +76,17 @@ typedef struct
     double coriolis_f;    /* f-plane parameter (1/s) */
     int bc_west, bc_east, bc_south, bc_north, bc_bottom, bc_top;
     double qc0, c_auto, c_accr, evap_eff, vt_max;
+    /* Rayleigh sponges applied after the EOS projection
+     * (src/core/infra/rayleigh_damping.cpp:403-470,985-1080,1160-1440): u, v
+     * relax to the base wind, w to 0 and theta to theta0 with rate
+     * alpha sin^2 ramps; rho and p are not relaxed. sponge = 0 disables all. */
+    int sponge;
+    double sponge_z_d;            /* upper sponge bottom (m); alpha rises to sponge_alpha at the lid */
+    double sponge_alpha;          /* 1/s (0.05) */
+    int sponge_w_cells;           /* w-only absorber depth under the lid (cells, 8) */
+    double sponge_w_alpha;        /* 1/s (0.2) */
+    int sponge_lateral_cells;     /* lateral band width (cells; 10 km in the presets) */
+    double sponge_lateral_alpha;  /* 1/s (0.05) */
 } nimbus_step_config;
 
 /* FP64 base-state profiles at cell centres, each nz long

@@ -40,10 +40,23 @@ struct AccT
     using type = S;
 };
 
-template <class S>
-__device__ __forceinline__ S* P(void* p)
+// Device arrays are addressed as global memory (address space 1). The
+// pointers live in the member argument table, so without the explicit
+// address space the compiler emits flat loads and stores, which also count
+// against the LDS/scalar counter and force full waits inside pipelined loops.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ gptr<T> G(T* p)
 {
-    return static_cast<S*>(p);
+    return (gptr<T>)(p);
+}
+
+template <class S>
+__device__ __forceinline__ gptr<S> P(void* p)
+{
+    return (gptr<S>)(static_cast<S*>(p));
 }
 
 __device__ __forceinline__ double finite0(double x) { return isfinite(x) ? x : 0.0; }
@@ -74,7 +87,10 @@ __device__ __forceinline__ double clamp_full(double x, double fallback, float lo
     return fmin(fmax(x, double(lo)), double(hi));
 }
 
-__device__ __forceinline__ const double* slot(const double* base, int s, int nz) { return base + std::size_t(s) * nz; }
+__device__ __forceinline__ gptr<const double> slot(const double* base, int s, int nz)
+{
+    return G(base) + std::size_t(s) * nz;
+}
 
 // ---------------------------------------------------------------------
 // Work decoding
@@ -98,10 +114,10 @@ struct Column
 
 __device__ __forceinline__ Column column_of(const MemberArgs& A, int local)
 {
-    const MemberGeom& G = A.geo;
-    const int ty = local / G.tiles_x, tx = local - ty * G.tiles_x;
+    const MemberGeom& geo = A.geo;
+    const int ty = local / geo.tiles_x, tx = local - ty * geo.tiles_x;
     const int tid = int(threadIdx.x);
-    return Column{tx * G.tile_w + (tid & (G.tile_w - 1)), ty * G.tile_h + tid / G.tile_w};
+    return Column{tx * geo.tile_w + (tid & (geo.tile_w - 1)), ty * geo.tile_h + tid / geo.tile_w};
 }
 
 // One cell per thread: tile t of the member at level k. The local work
@@ -117,18 +133,18 @@ struct Cell
 
 __device__ __forceinline__ Cell cell_of(const MemberArgs& A, int local)
 {
-    const MemberGeom& G = A.geo;
+    const MemberGeom& geo = A.geo;
     const int nz = A.g.nz;
-    const int kz = (nz + G.k_chunks - 1) / G.k_chunks;
-    const int tiles = G.tiles_x * G.tiles_y;
+    const int kz = (nz + geo.k_chunks - 1) / geo.k_chunks;
+    const int tiles = geo.tiles_x * geo.tiles_y;
     const int kk = local % kz, rest = local / kz;
     const int t = rest % tiles, kc = rest / tiles;
-    const int ty = t / G.tiles_x, tx = t - ty * G.tiles_x;
+    const int ty = t / geo.tiles_x, tx = t - ty * geo.tiles_x;
     const int tid = int(threadIdx.x);
     Cell c;
     c.k = kc * kz + kk;
-    c.i = tx * G.tile_w + (tid & (G.tile_w - 1));
-    c.j = ty * G.tile_h + tid / G.tile_w;
+    c.i = tx * geo.tile_w + (tid & (geo.tile_w - 1));
+    c.j = ty * geo.tile_h + tid / geo.tile_w;
     c.valid = c.k < nz && c.i < A.g.nx && c.j < A.g.ny;
     return c;
 }
@@ -143,7 +159,7 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? l
 template <class S>
 struct SlowFields
 {
-    const S *u, *v, *w, *rp, *qv, *qc, *qr;
+    gptr<const S> u, v, w, rp, qv, qc, qr;
 };
 
 template <class S>
@@ -156,7 +172,7 @@ __device__ void slow_tendency_cell(const DevGrid& g, const SlowFields<S>& F, con
     const double ix = g.idx, iy = g.idy, iz = g.idz;
     const std::size_t c = g.at(i, j, k);
     const std::ptrdiff_t sx = 1, sy = g.px, sz = std::ptrdiff_t(g.plane());
-    auto R = [&](const S* a, std::ptrdiff_t o) { return double(a[std::ptrdiff_t(c) + o]); };
+    auto R = [&](gptr<const S> a, std::ptrdiff_t o) { return double(a[std::ptrdiff_t(c) + o]); };
     // u at x-face
     {
         const double ux = uz[3];
@@ -192,8 +208,8 @@ __device__ void slow_tendency_cell(const DevGrid& g, const SlowFields<S>& F, con
         const double ddy = dev::upwind_axis<double>([&](int o) { return R(F.w, o * sy); }, j, NY, vy, iy);
         const double ddz = k == 0 ? (wz >= 0.0 ? wz_[3] * iz : (wz_[4] - wz_[3]) * iz)
                                   : dev::upwind_axis<double>([&](int o) { return wz_[3 + o]; }, k, NZ, wz, iz);
-        const double* rho0 = slot(base, kBaseRho0, NZ);
-        const double* qv0s = slot(base, kBaseQv0Stored, NZ);
+        const auto rho0 = slot(base, kBaseRho0, NZ);
+        const auto qv0s = slot(base, kBaseQv0Stored, NZ);
         const double rpf = 0.5 * (rpk + rpk1);
         const double rf = 0.5 * (rho0[k] + rho0[k + 1]) + rpf;
         const double buoy = -kG * rpf / safe_rho(rf);
@@ -264,9 +280,9 @@ __global__ __launch_bounds__(kBlock) void k_slow(const MemberArgs* __restrict__ 
         const S au = S(S(0.5) * (S(du) + P<S>(f.k1u)[c]));
         const S av = S(S(0.5) * (S(dv) + P<S>(f.k1v)[c]));
         const S aw = S(S(0.5) * (S(dw) + P<S>(f.k1w)[c]));
-        S* u = P<S>(f.u);
-        S* v = P<S>(f.v);
-        S* w = P<S>(f.w);
+        const auto u = P<S>(f.u);
+        const auto v = P<S>(f.v);
+        const auto w = P<S>(f.w);
         u[c] = S(double(u[c]) + finite0(double(au)) * dt);
         v[c] = S(double(v[c]) + finite0(double(av)) * dt);
         w[c] = S(double(w[c]) + finite0(double(aw)) * dt);
@@ -310,15 +326,58 @@ __device__ __forceinline__ float pow1p_gamma_m1(float x)
     return expm1f(float(kGamma) * log1pf(x));
 }
 
-template <class S>
-__device__ __forceinline__ double eos_full_pressure(const double* base, int nz, int k, double rho, double theta)
+// Base-state profiles used by the EOS: rho0, p0, 1 / (rho0 theta0),
+// pe0 = EOS(rho0, theta0) and pe0 - p0 at level k, read from the member's
+// FP64 base table in global memory (GProf) or from a copy staged in LDS
+// (LProf, layout [rho0 | p0 | 1/(rho0 theta0) | pe0 | pe0 - p0] x nz).
+struct GProf
+{
+    gptr<const double> b;
+    int nz;
+    __device__ double rho0(int k) const { return b[std::size_t(kBaseRho0) * nz + k]; }
+    __device__ double p0(int k) const { return b[std::size_t(kBaseP0) * nz + k]; }
+    __device__ double inv_rt0(int k) const { return b[std::size_t(kBaseInvRT0) * nz + k]; }
+    __device__ double pe0(int k) const { return b[std::size_t(kBasePe0) * nz + k]; }
+    __device__ double pe_diff(int k) const { return b[std::size_t(kBasePeDiff) * nz + k]; }
+};
+
+constexpr int kLdsProfiles = 5;
+
+struct LProf
+{
+    const double* s;
+    int nz;
+    __device__ double rho0(int k) const { return s[k]; }
+    __device__ double p0(int k) const { return s[nz + k]; }
+    __device__ double inv_rt0(int k) const { return s[2 * nz + k]; }
+    __device__ double pe0(int k) const { return s[3 * nz + k]; }
+    __device__ double pe_diff(int k) const { return s[4 * nz + k]; }
+};
+
+__device__ __forceinline__ GProf gprof(const double* base, int nz) { return GProf{G(base), nz}; }
+
+// Copies the LProf profiles of a member into LDS (all threads of the block take part).
+__device__ __forceinline__ LProf stage_profiles(double* lds, const double* base, int nz)
+{
+    const gptr<const double> b = G(base);
+    constexpr int slots[kLdsProfiles] = {kBaseRho0, kBaseP0, kBaseInvRT0, kBasePe0, kBasePeDiff};
+    for (int t = int(threadIdx.x); t < kLdsProfiles * nz; t += int(blockDim.x))
+    {
+        const int q = t / nz, k = t - q * nz;
+        lds[t] = b[std::size_t(slots[q]) * nz + k];
+    }
+    __syncthreads();
+    return LProf{lds, nz};
+}
+
+template <class S, class Prof>
+__device__ __forceinline__ double eos_full_pressure(const Prof& pr, int k, double rho, double theta)
 {
     if constexpr (sizeof(S) == sizeof(float))
     {
-        const double x = rho * theta * base[std::size_t(kBaseInvRT0) * nz + k] - 1.0;
+        const double x = rho * theta * pr.inv_rt0(k) - 1.0;
         const float e = pow1p_gamma_m1(float(x));
-        return base[std::size_t(kBaseP0) * nz + k] + (double(e) * base[std::size_t(kBasePe0) * nz + k] +
-                                                      base[std::size_t(kBasePeDiff) * nz + k]);
+        return pr.p0(k) + (double(e) * pr.pe0(k) + pr.pe_diff(k));
     }
     else
     {
@@ -332,19 +391,18 @@ struct RhoP
     S r, p;  // stored rho' and p'
 };
 
-template <class S>
-__device__ __forceinline__ RhoP<S> density_eos(const Bounds& bd, const double* base, int nz, int k, double rn,
-                                               double theta)
+template <class S, class Prof>
+__device__ __forceinline__ RhoP<S> density_eos(const Bounds& bd, const Prof& pr, int k, double rn, double theta)
 {
-    const double rho0k = base[std::size_t(kBaseRho0) * nz + k];
-    const double p0k = base[std::size_t(kBaseP0) * nz + k];
+    const double rho0k = pr.rho0(k);
+    const double p0k = pr.p0(k);
     if (!isfinite(rn) || rn <= 0.0) rn = fmax(0.1, rho0k);
     const double rs = clamp_full(rn, 1.0, bd.rho_min, bd.rho_max);
     RhoP<S> o;
     o.r = S(rs - rho0k);
     const double rho_used = rho0k + double(o.r);
     double pn = double(bd.p_fallback);
-    if (theta > 0.0) pn = eos_full_pressure<S>(base, nz, k, rho_used, theta);
+    if (theta > 0.0) pn = eos_full_pressure<S>(pr, k, rho_used, theta);
     if (!isfinite(pn) || pn <= 0.0) pn = double(bd.p_fallback);
     o.p = S(clamp_full(pn, 101325.0, bd.p_min, bd.p_max) - p0k);
     return o;
@@ -362,8 +420,8 @@ __device__ __forceinline__ RhoP<S> density_eos(const Bounds& bd, const double* b
 template <class S>
 struct AcousticBufs
 {
-    const S *uo, *vo, *wo, *ro, *th;
-    S *rn, *pp, *mfx, *mfy, *mfz, *rb;
+    gptr<const S> uo, vo, wo, ro, th;
+    gptr<S> rn, pp, mfx, mfy, mfz, rb;
 };
 
 template <class S>
@@ -375,7 +433,7 @@ __device__ __forceinline__ RhoP<S> acoustic_forward(const DevGrid& g, const Acou
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const std::size_t c = g.at(i, j, k);
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
-    const double* rho0 = slot(base, kBaseRho0, NZ);
+    const auto rho0 = slot(base, kBaseRho0, NZ);
     const double r0 = rho0[k];
     const double rc = r0 + double(B.ro[c]);
     const double rxp = i < NX - 1 ? r0 + double(B.ro[c + 1]) : 0.0;
@@ -406,7 +464,7 @@ __device__ __forceinline__ RhoP<S> acoustic_forward(const DevGrid& g, const Acou
         d = -((fxp - fxm) * g.idx + (fyp - fym) * g.idy + (fzp - fzm) * g.idz);
     }
     const S drho = S(finite0(d));
-    const RhoP<S> o = density_eos<S>(bd, base, NZ, k, rc + double(drho) * dts, double(B.th[c]));
+    const RhoP<S> o = density_eos<S>(bd, gprof(base, NZ), k, rc + double(drho) * dts, double(B.th[c]));
     if (store)
     {
         B.rn[c] = o.r;
@@ -420,11 +478,11 @@ __device__ __forceinline__ RhoP<S> acoustic_forward(const DevGrid& g, const Acou
 template <class S>
 __device__ __forceinline__ void momentum_update(const DevGrid& g, const double* __restrict__ base, const Bounds& bd,
                                                 double dts, int i, int j, int k, RhoP<S> cc, RhoP<S> xp, RhoP<S> yp,
-                                                RhoP<S> zp, double u, double v, double w, bool skip_w, S* un, S* vn,
-                                                S* wn, std::size_t c)
+                                                RhoP<S> zp, double u, double v, double w, bool skip_w, gptr<S> un,
+                                                gptr<S> vn, gptr<S> wn, std::size_t c)
 {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const double* rho0 = slot(base, kBaseRho0, NZ);
+    const auto rho0 = slot(base, kBaseRho0, NZ);
     double du = 0.0, dv = 0.0, dw = 0.0;
     const bool interior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k < NZ - 1;
     if (interior)
@@ -506,34 +564,35 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
-    if (s >= A.p.substeps) return;
+    if (s >= A.p.substeps) return;  // uniform over the block (one member per block)
     const DevGrid& g = A.g;
     const Column t = column_of(A, wk.local);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    if (t.i >= NX || t.j >= NY) return;  // no barrier in this kernel
+    // Base-state profiles in LDS: their per-level reads then do not wait
+    // behind the prefetched field loads of the levels above.
+    extern __shared__ double s_prof[];
+    const LProf pr = stage_profiles(s_prof, A.f.base, NZ);
+    if (t.i >= NX || t.j >= NY) return;  // no barrier after this point
     const int i = t.i, j = t.j;
     const DevFields& f = A.f;
     const int par = s & 1;
-    const S* __restrict__ u = P<S>(f.u);
-    const S* __restrict__ v = P<S>(f.v);
-    S* __restrict__ w = P<S>(f.w);
-    const S* __restrict__ rp_in = P<S>(par ? f.rp2 : f.rp);
-    S* __restrict__ rp_out = P<S>(par ? f.rp : f.rp2);
-    S* __restrict__ pp = P<S>(f.pp);
-    const S* __restrict__ th = P<S>(f.th);
-    S* __restrict__ mfx = P<S>(f.mfx);
-    S* __restrict__ mfy = P<S>(f.mfy);
-    S* __restrict__ mfz = P<S>(f.mfz);
-    S* __restrict__ rb = P<S>(f.rb);
-    double* __restrict__ tc = f.tri_c;
-    double* __restrict__ tr = f.tri_r;
-    const double* base = f.base;
+    const gptr<const S> __restrict__ u = P<S>(f.u);
+    const gptr<const S> __restrict__ v = P<S>(f.v);
+    const gptr<S> __restrict__ w = P<S>(f.w);
+    const gptr<const S> __restrict__ rp_in = P<S>(par ? f.rp2 : f.rp);
+    const gptr<S> __restrict__ rp_out = P<S>(par ? f.rp : f.rp2);
+    const gptr<S> __restrict__ pp = P<S>(f.pp);
+    const gptr<const S> __restrict__ th = P<S>(f.th);
+    const gptr<S> __restrict__ mfx = P<S>(f.mfx);
+    const gptr<S> __restrict__ mfy = P<S>(f.mfy);
+    const gptr<S> __restrict__ mfz = P<S>(f.mfz);
+    const gptr<S> __restrict__ rb = P<S>(f.rb);
+    const gptr<double> __restrict__ tc = G(f.tri_c);
+    const gptr<double> __restrict__ tr = G(f.tri_r);
     const Bounds bd = A.p.bounds;
     const double dts = A.p.dts, alpha = A.p.hevi_alpha, weight = A.p.mf_weight;
     const bool acc = accumulate != 0, first = init != 0 && s == 0;
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
-    const double* rho0 = slot(base, kBaseRho0, NZ);
-    const double* p0 = slot(base, kBaseP0, NZ);
     const double idz = g.idz;
     const double a1 = 1.0 - alpha;
     if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
@@ -542,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
         for (int k = 0; k < NZ; ++k)
         {
             const std::size_t c = g.at(i, j, k);
-            const double r0 = rho0[k];
+            const double r0 = pr.rho0(k);
             const double rc = r0 + double(rp_in[c]);
             if (acc)
             {
@@ -559,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
                 if (first) mfz[c] = S(0);
             }
             if (first) rb[c] = rp_in[c];
-            const RhoP<S> o = density_eos<S>(bd, base, NZ, k, rc, double(th[c]));
+            const RhoP<S> o = density_eos<S>(bd, pr, k, rc, double(th[c]));
             rp_out[c] = o.r;
             pp[c] = o.p;
         }
@@ -567,7 +626,10 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
     }
     // ---- upward sweep: mass fluxes, rho*, p*, Thomas forward elimination.
     // The inputs of levels k + 1 and k + 2 are loaded before level k is
-    // computed and stored, so every thread keeps loads of three levels in flight.
+    // computed and stored, so every thread keeps loads of three levels in
+    // flight. The loads are unconditional (levels clamped to the column, mass
+    // fluxes read even when they are then ignored), so no branch splits the
+    // load stream and the wait for level k leaves the younger loads in flight.
     auto load_level = [&](int k) {
         const std::size_t c = g.at(i, j, k);
         HeviLevel<S> L;
@@ -575,22 +637,22 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
         L.rpyp = rp_in[c + sy]; L.rpym = rp_in[c - sy];
         L.u = u[c]; L.um = u[c - 1]; L.v = v[c]; L.vm = v[c - sy];
         L.w = w[c]; L.th = th[c]; L.pn = pp[c];
-        L.mfx = acc && !first ? mfx[c] : S(0);
-        L.mfy = acc && !first ? mfy[c] : S(0);
+        L.mfx = mfx[c];
+        L.mfy = mfy[c];
         return L;
     };
     double rb_prev = 0.0, C_prev = 0.0, ps_prev = 0.0, pn_prev = 0.0, wo_prev = 0.0, e_prev = 0.0;
     double rb_prev2 = 0.0, cp_prev = 0.0, rr_prev = 0.0;
     bool ok = true;
     HeviLevel<S> cur = load_level(0);
-    HeviLevel<S> nxt{};
-    if (1 < NZ) nxt = load_level(1);
+    HeviLevel<S> nxt = load_level(1);  // NZ >= 4
+    // unrolled by the prefetch depth, so the level registers rotate by renaming
+#pragma unroll 3
     for (int k = 0; k < NZ; ++k)
     {
-        HeviLevel<S> nxt2{};
-        if (k + 2 < NZ) nxt2 = load_level(k + 2);
+        const HeviLevel<S> nxt2 = load_level(min(k + 2, NZ - 1));
         const std::size_t c = g.at(i, j, k);
-        const double r0 = rho0[k];
+        const double r0 = pr.rho0(k);
         const double rc = r0 + double(cur.rp);
         const double rxp = r0 + double(cur.rpxp), rxm = r0 + double(cur.rpxm);
         const double ryp = r0 + double(cur.rpyp), rym = r0 + double(cur.rpym);
@@ -605,11 +667,11 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
         const double rstar = rc + dts * finite0(dh);
         // linearisation point p* = EOS(rho*): FP64 power law in FP64 storage,
         // the perturbation form (FP32 relative accuracy on p* - p0) in the mixed mode
-        const double ps = eos_full_pressure<S>(base, NZ, k, rstar, double(cur.th));
+        const double ps = eos_full_pressure<S>(pr, k, rstar, double(cur.th));
         const double C = kGamma * ps * (1.0 / rstar);
         const double wo = double(cur.w);
         // face density above level k (zero at the lid)
-        const double rbk = k < NZ - 1 ? 0.5 * (rc + (rho0[k + 1] + double(nxt.rp))) : 0.0;
+        const double rbk = k < NZ - 1 ? 0.5 * (rc + (pr.rho0(k + 1) + double(nxt.rp))) : 0.0;
         const double e = -a1 * dts * idz * (rbk * wo - rb_prev * wo_prev);
         rp_out[c] = S(rstar - r0);
         const double pnk = double(cur.pn);
@@ -624,7 +686,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
             const double U = (k - 1) < NZ - 2 ? -Ak * Bc * C * rbk : 0.0;
             const double D = 1.0 + Ak * Bc * (C + C_prev) * rbr;
             // PGF on alpha p* + (1 - alpha) p^n (p^n: substep-start pressure), base gradient removed
-            const double G = (alpha * ((ps - ps_prev) - (p0[k] - p0[k - 1])) + a1 * (pnk - pn_prev)) * idz;
+            const double G = (alpha * ((ps - ps_prev) - (pr.p0(k) - pr.p0(k - 1))) + a1 * (pnk - pn_prev)) * idz;
             const double R = wo_prev - dts * inv_rbr * G - Ak * (C * e - C_prev * e_prev);
             const double piv = D - Aa * cp_prev;
             if (!(piv > 0.0) || !isfinite(piv)) ok = false;
@@ -652,19 +714,19 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
     // loaded one iteration ahead.
     struct Down
     {
-        double w, tr, tc, rp_lo;
-        S rstar, th, mfz;
+        double tr, tc;
+        S w, rp_lo, rstar, th, mfz;
     };
     auto load_down = [&](int r) {
-        Down d{};
+        Down d;
         const std::size_t cr = g.at(i, j, r);
-        d.w = double(w[cr]);
+        d.w = w[cr];
         d.tr = tr[cr];
         d.tc = tc[cr];
-        d.rp_lo = double(rp_in[cr]);
+        d.rp_lo = rp_in[cr];
         d.rstar = rp_out[cr];
         d.th = th[cr];
-        d.mfz = acc && !first ? mfz[cr] : S(0);
+        d.mfz = mfz[cr];
         return d;
     };
     double w_above = 0.0, wo_above = 0.0, rb_above = 0.0;  // face r+1 quantities (stored w)
@@ -672,23 +734,23 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
     // cell NZ-1 (finalised in the first iteration) and face NZ-2
     const std::size_t ctop = g.at(i, j, NZ - 1);
     double rp_hi = double(rp_in[ctop]);
-    S rstar_c = rp_out[ctop], th_c = th[ctop], mfz_c = acc && !first ? mfz[ctop] : S(0);
+    S rstar_c = rp_out[ctop], th_c = th[ctop], mfz_c = mfz[ctop];
     Down dn = load_down(NZ - 2);
+#pragma unroll 2
     for (int r = NZ - 2; r >= -1; --r)
     {
-        Down nx{};
-        if (r - 1 >= 0) nx = load_down(r - 1);
+        const Down nx = load_down(max(r - 1, 0));
         double wn = 0.0, wo = 0.0, rbr = 0.0;
         if (r >= 0)
         {
             const std::size_t cr = g.at(i, j, r);
-            wo = dn.w;
+            wo = double(dn.w);
             const double x = ok ? dn.tr - dn.tc * x_above : wo;
             x_above = x;
             wn = clamp_store<S>(x, 0.0, -bd.wind_v, bd.wind_v);
             w[cr] = S(wn);
             wn = double(S(wn));
-            rbr = 0.5 * ((rho0[r] + dn.rp_lo) + (rho0[r + 1] + rp_hi));
+            rbr = 0.5 * ((pr.rho0(r) + double(dn.rp_lo)) + (pr.rho0(r + 1) + rp_hi));
         }
         // finalise cell r+1: top face r+1 (zero at the lid), bottom face r
         const int kc = r + 1;
@@ -696,8 +758,8 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
         const double ft = kc < NZ - 1 ? rb_above * (alpha * w_above + a1 * wo_above) : 0.0;
         const double fb = r >= 0 ? rbr * (alpha * wn + a1 * wo) : 0.0;
         if (acc) mfz[cc] = S((first ? 0.0 : double(mfz_c)) + weight * ft);
-        const double rstar = rho0[kc] + double(rstar_c);
-        const RhoP<S> o = density_eos<S>(bd, base, NZ, kc, rstar - dts * idz * (ft - fb), double(th_c));
+        const double rstar = pr.rho0(kc) + double(rstar_c);
+        const RhoP<S> o = density_eos<S>(bd, pr, kc, rstar - dts * idz * (ft - fb), double(th_c));
         rp_out[cc] = o.r;
         pp[cc] = o.p;
         w_above = wn;
@@ -705,7 +767,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
         rb_above = rbr;
         if (r >= 0)
         {
-            rp_hi = dn.rp_lo;
+            rp_hi = double(dn.rp_lo);
             rstar_c = dn.rstar;
             th_c = dn.th;
             mfz_c = dn.mfz;
@@ -730,11 +792,11 @@ __global__ __launch_bounds__(kBlock) void k_momentum(const MemberArgs* __restric
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const int i = cl.i, j = cl.j, k = cl.k;
     const DevFields& f = A.f;
-    const S* __restrict__ rn = P<S>((s & 1) ? f.rp : f.rp2);
-    const S* __restrict__ pp = P<S>(f.pp);
-    S* u = P<S>(f.u);
-    S* v = P<S>(f.v);
-    S* w = P<S>(f.w);
+    const gptr<const S> __restrict__ rn = P<S>((s & 1) ? f.rp : f.rp2);
+    const gptr<const S> __restrict__ pp = P<S>(f.pp);
+    const auto u = P<S>(f.u);
+    const auto v = P<S>(f.v);
+    const auto w = P<S>(f.w);
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
     const std::size_t c = g.at(i, j, k);
     const RhoP<S> cc{rn[c], pp[c]};
@@ -762,7 +824,7 @@ __device__ __forceinline__ float kw78_face(float prev, float phase, float grad, 
 template <class S>
 struct BcBufs
 {
-    S *u, *v, *w, *rp, *pp;
+    gptr<S> u, v, w, rp, pp;
     float dt_rad;
 };
 
@@ -799,7 +861,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_west_east(const MemberArgs* __res
     const int bcw = A.p.bc_west, bce = A.p.bc_east;
     const bool radiate = b.dt_rad > 0.0f;
     const float cstar = 30.0f, inv_dx = 1.0f / float(g.dx);
-    S *u = b.u, *v = b.v, *w = b.w, *rp = b.rp, *pp = b.pp;
+    const auto u = b.u, v = b.v, w = b.w, rp = b.rp, pp = b.pp;
     const std::size_t row = g.at(0, j, k);
     auto cp_rp = [&](std::size_t d, std::size_t src) { rp[d] = rp[src]; pp[d] = pp[src]; };
     {
@@ -815,7 +877,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_west_east(const MemberArgs* __res
             {
                 if (radiate)
                 {
-                    float& up = A.f.rad_w[std::size_t(j) * NZ + k];
+                    auto& up = G(A.f.rad_w)[std::size_t(j) * NZ + k];
                     const float un = kw78_face(up, up - 0.0f - cstar, float(u[in]) - up, b.dt_rad, inv_dx, true);
                     u[c] = S(un);
                     up = un;
@@ -840,7 +902,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_west_east(const MemberArgs* __res
             if (bce == NIMBUS_BC_RIGID || bce == NIMBUS_BC_RIGID_SLIP) u[a] = S(0);
             if (bce == NIMBUS_BC_RADIATIVE && radiate)
             {
-                float& up = A.f.rad_e[std::size_t(j) * NZ + k];
+                auto& up = G(A.f.rad_e)[std::size_t(j) * NZ + k];
                 const float un = kw78_face(up, up - 0.0f + cstar, up - float(u[am]), b.dt_rad, inv_dx, false);
                 u[a] = S(un);
                 up = un;
@@ -870,7 +932,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_south_north(const MemberArgs* __r
     const int bcs = A.p.bc_south, bcn = A.p.bc_north;
     const bool radiate = b.dt_rad > 0.0f;
     const float cstar = 30.0f, inv_dy = 1.0f / float(g.dy);
-    S *u = b.u, *v = b.v, *w = b.w, *rp = b.rp, *pp = b.pp;
+    const auto u = b.u, v = b.v, w = b.w, rp = b.rp, pp = b.pp;
     const std::size_t sy = std::size_t(g.px);
     auto cp_rp = [&](std::size_t d, std::size_t src) { rp[d] = rp[src]; pp[d] = pp[src]; };
     {
@@ -886,7 +948,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_south_north(const MemberArgs* __r
             {
                 if (radiate)
                 {
-                    float& vp = A.f.rad_s[std::size_t(i) * NZ + k];
+                    auto& vp = G(A.f.rad_s)[std::size_t(i) * NZ + k];
                     const float vn = kw78_face(vp, vp - 0.0f - cstar, float(v[in]) - vp, b.dt_rad, inv_dy, true);
                     v[c] = S(vn);
                     vp = vn;
@@ -911,7 +973,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_south_north(const MemberArgs* __r
             if (bcn == NIMBUS_BC_RIGID || bcn == NIMBUS_BC_RIGID_SLIP) v[a] = S(0);
             if (bcn == NIMBUS_BC_RADIATIVE && radiate)
             {
-                float& vp = A.f.rad_n[std::size_t(i) * NZ + k];
+                auto& vp = G(A.f.rad_n)[std::size_t(i) * NZ + k];
                 const float vn = kw78_face(vp, vp - 0.0f + cstar, vp - float(v[am]), b.dt_rad, inv_dy, false);
                 v[a] = S(vn);
                 vp = vn;
@@ -957,19 +1019,19 @@ __global__ __launch_bounds__(kBlock) void k_seed_radiation(const MemberArgs* __r
     const DevGrid& g = A.g;
     const int t = wk.local * kBlock + int(threadIdx.x);
     const int NZ = g.nz;
-    const S* u = P<S>(A.f.u);
-    const S* v = P<S>(A.f.v);
+    const auto u = P<S>(A.f.u);
+    const auto v = P<S>(A.f.v);
     if (t < g.ny * NZ)
     {
         const int j = t / NZ, k = t - j * NZ;
-        A.f.rad_w[t] = float(u[g.at(0, j, k)]);
-        A.f.rad_e[t] = float(u[g.at(g.nx - 2, j, k)]);
+        G(A.f.rad_w)[t] = float(u[g.at(0, j, k)]);
+        G(A.f.rad_e)[t] = float(u[g.at(g.nx - 2, j, k)]);
     }
     if (t < g.nx * NZ)
     {
         const int i = t / NZ, k = t - i * NZ;
-        A.f.rad_s[t] = float(v[g.at(i, 0, k)]);
-        A.f.rad_n[t] = float(v[g.at(i, g.ny - 2, k)]);
+        G(A.f.rad_s)[t] = float(v[g.at(i, 0, k)]);
+        G(A.f.rad_n)[t] = float(v[g.at(i, g.ny - 2, k)]);
     }
 }
 
@@ -998,21 +1060,21 @@ __global__ __launch_bounds__(kBlock) void k_commit(const MemberArgs* __restrict_
 constexpr int kMaxScalars = 4;
 
 template <class S>
-__device__ __forceinline__ const S* scalar_in(const DevFields& f, int q)
+__device__ __forceinline__ gptr<const S> scalar_in(const DevFields& f, int q)
 {
     return P<S>(q == 0 ? f.th : q == 1 ? f.qv : q == 2 ? f.qc : f.qr);
 }
 template <class S>
-__device__ __forceinline__ S* scalar_out(const DevFields& f, int q)
+__device__ __forceinline__ gptr<S> scalar_out(const DevFields& f, int q)
 {
     return P<S>(q == 0 ? f.tth : q == 1 ? f.tqv : q == 2 ? f.tqc : f.tqr);
 }
 // Limiter factor slot of scalar q: the PD scalars take rl_qv, rl_qc, rl_qr in order.
-__device__ __forceinline__ float* limiter_slot(const DevFields& f, int pd_mask, int q)
+__device__ __forceinline__ gptr<float> limiter_slot(const DevFields& f, int pd_mask, int q)
 {
     int n = 0;
     for (int r = 0; r < q; ++r) n += (pd_mask >> r) & 1;
-    return n == 0 ? f.rl_qv : n == 1 ? f.rl_qc : f.rl_qr;
+    return G(n == 0 ? f.rl_qv : n == 1 ? f.rl_qc : f.rl_qr);
 }
 
 // Face flux pass (advect_conservative_scalar_flux): each cell writes the raw
@@ -1046,7 +1108,7 @@ __global__ __launch_bounds__(kBlock) void k_transport_flux(const MemberArgs* __r
     for (int q = 0; q < kMaxScalars; ++q)
     {
         if (q >= n_scalars) break;
-        const S* __restrict__ qs = scalar_in<S>(f, q);
+        const gptr<const S> __restrict__ qs = scalar_in<S>(f, q);
         const A_ q0 = A_(qs[c]);
         S sx = S(0), sy_ = S(0), sz_ = S(0);
         if (i <= NX - 2)
@@ -1094,9 +1156,9 @@ __global__ __launch_bounds__(kBlock) void k_transport_limiter(const MemberArgs* 
     for (int q = 0; q < n_scalars; ++q)
     {
         if (!((pd_mask >> q) & 1)) continue;
-        const S* __restrict__ fx = P<S>(f.flux[3 * q]);
-        const S* __restrict__ fy = P<S>(f.flux[3 * q + 1]);
-        const S* __restrict__ fz = P<S>(f.flux[3 * q + 2]);
+        const gptr<const S> __restrict__ fx = P<S>(f.flux[3 * q]);
+        const gptr<const S> __restrict__ fy = P<S>(f.flux[3 * q + 1]);
+        const gptr<const S> __restrict__ fz = P<S>(f.flux[3 * q + 2]);
         double out_rate = 0.0;
         if (i <= NX - 2 && fx[c] > S(0)) out_rate += double(fx[c]) * g.idx;
         if (i >= 1 && fx[c - 1] < S(0)) out_rate -= double(fx[c - 1]) * g.idx;
@@ -1142,11 +1204,11 @@ __global__ __launch_bounds__(kBlock) void k_transport_update(const MemberArgs* _
     const double ix = g.idx, iy = g.idy, iz = g.idz;
     for (int q = 0; q < n_scalars; ++q)
     {
-        const S* __restrict__ fx = P<S>(f.flux[3 * q]);
-        const S* __restrict__ fy = P<S>(f.flux[3 * q + 1]);
-        const S* __restrict__ fz = P<S>(f.flux[3 * q + 2]);
+        const gptr<const S> __restrict__ fx = P<S>(f.flux[3 * q]);
+        const gptr<const S> __restrict__ fy = P<S>(f.flux[3 * q + 1]);
+        const gptr<const S> __restrict__ fz = P<S>(f.flux[3 * q + 2]);
         const bool pd = (pd_mask >> q) & 1;
-        const float* rl = pd ? limiter_slot(f, pd_mask, q) : nullptr;
+        const gptr<const float> rl = pd ? limiter_slot(f, pd_mask, q) : nullptr;
         auto lim = [&](S raw, std::size_t lo, std::size_t hi) -> double {
             if (!pd || raw == S(0)) return double(raw);
             return double(raw) * double(S(raw > S(0) ? rl[lo] : rl[hi]));
@@ -1283,10 +1345,10 @@ __global__ __launch_bounds__(kBlock) void k_kessler_tendencies(const MemberArgs*
         dth[c] = S(0); dqv[c] = S(0); dqc[c] = S(0); dqr[c] = S(0);
         return;
     }
-    const double* rho0 = slot(f.base, kBaseRho0, NZ);
-    const double* p0 = slot(f.base, kBaseP0, NZ);
-    const S *th = P<S>(f.th), *qv = P<S>(f.qv), *qc = P<S>(f.qc), *qr = P<S>(f.qr), *rp = P<S>(f.rp),
-            *pp = P<S>(f.pp);
+    const auto rho0 = slot(f.base, kBaseRho0, NZ);
+    const auto p0 = slot(f.base, kBaseP0, NZ);
+    const auto th = P<S>(f.th), qv = P<S>(f.qv), qc = P<S>(f.qc), qr = P<S>(f.qr), rp = P<S>(f.rp),
+            pp = P<S>(f.pp);
     const std::size_t sz = g.plane();
     const bool up = k + 1 < NZ;
     const KesslerOut<S> o =
@@ -1316,18 +1378,18 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     const int i = cl.i, j = cl.j, k = cl.k;
     const DevFields& f = A.f;
     const bool from_temp = (n & 1) == 0;
-    const S* th_in = P<S>(from_temp ? f.tth : f.th);
-    const S* qv_in = P<S>(from_temp ? f.tqv : f.qv);
-    const S* qc_in = P<S>(from_temp ? f.tqc : f.qc);
-    const S* qr_in = P<S>(from_temp ? f.tqr : f.qr);
-    S* th = P<S>(from_temp ? f.th : f.tth);
-    S* qv = P<S>(from_temp ? f.qv : f.tqv);
-    S* qc = P<S>(from_temp ? f.qc : f.tqc);
-    S* qr = P<S>(from_temp ? f.qr : f.tqr);
-    const S* rp = P<S>(f.rp);
-    S* pp = P<S>(f.pp);
-    const double* rho0 = slot(f.base, kBaseRho0, NZ);
-    const double* p0 = slot(f.base, kBaseP0, NZ);
+    const auto th_in = P<S>(from_temp ? f.tth : f.th);
+    const auto qv_in = P<S>(from_temp ? f.tqv : f.qv);
+    const auto qc_in = P<S>(from_temp ? f.tqc : f.qc);
+    const auto qr_in = P<S>(from_temp ? f.tqr : f.qr);
+    const auto th = P<S>(from_temp ? f.th : f.tth);
+    const auto qv = P<S>(from_temp ? f.qv : f.tqv);
+    const auto qc = P<S>(from_temp ? f.qc : f.tqc);
+    const auto qr = P<S>(from_temp ? f.qr : f.tqr);
+    const auto rp = P<S>(f.rp);
+    const auto pp = P<S>(f.pp);
+    const auto rho0 = slot(f.base, kBaseRho0, NZ);
+    const auto p0 = slot(f.base, kBaseP0, NZ);
     const KesslerConst kc = A.p.kessler;
     const Bounds bd = A.p.bounds;
     const double h = A.p.dt / A.p.kessler_sub;
@@ -1348,7 +1410,7 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
             {
                 double Vt = fmin(double(dev::kessler_rain_speed<double>(q0, rho)) * sqrt(1.2 / rho), kc.vt_max);
                 const double sink = double(S(S(Vt) * qrn)) / g.dz;
-                f.rain[std::size_t(j) * NX + i] += sink * rho * g.dz * h;
+                G(f.rain)[std::size_t(j) * NX + i] += sink * rho * g.dz * h;
             }
         }
         const double step = fmin(fmax(double(o.dth) * h, -50.0), 50.0);
@@ -1385,6 +1447,56 @@ __global__ __launch_bounds__(kBlock) void k_scalars_home(const MemberArgs* __res
 }
 
 // ---------------------------------------------------------------------
+// Rayleigh sponges after the EOS projection (rayleigh_damping.cpp:985-1080,
+// 1360-1440): upper sponge on u, v (to the base wind), w (to 0) and theta
+// (to theta0), the w-only absorber under the lid, then the lateral band;
+// each relaxation x += f (target - x) with the exact decay factor
+// f = 1 - exp(-alpha dt) precomputed per level, rounded to storage.
+// ---------------------------------------------------------------------
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_sponge(const MemberArgs* __restrict__ ma, const int2* __restrict__ map)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (!A.p.sponge) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const DevGrid& g = A.g;
+    const int NZ = g.nz, k = cl.k;
+    const DevFields& f = A.f;
+    const double* base = f.base;
+    const double fu = slot(base, kBaseSpongeUp, NZ)[k], fw = slot(base, kBaseSpongeUpW, NZ)[k];
+    const double fwo = slot(base, kBaseSpongeW, NZ)[k];
+    const int dist = min(min(cl.i, g.nx - 1 - cl.i), min(cl.j, g.ny - 1 - cl.j));
+    const double fl = dist < A.p.sponge_lateral ? G(f.sponge_lat)[dist] : 0.0;
+    if (fu == 0.0 && fw == 0.0 && fwo == 0.0 && fl == 0.0) return;
+    const double u0 = slot(base, kBaseU0, NZ)[k], v0 = slot(base, kBaseV0, NZ)[k];
+    const double th0 = slot(base, kBaseTh0, NZ)[k];
+    const std::size_t c = g.at(cl.i, cl.j, k);
+    const auto u = P<S>(f.u);
+    const auto v = P<S>(f.v);
+    const auto w = P<S>(f.w);
+    const auto th = P<S>(f.th);
+    S uc = u[c], vc = v[c], wc = w[c], tc = th[c];
+    auto relax = [](S& x, double target, double fac) {
+        if (fac > 0.0) x = S(double(x) + fac * (target - double(x)));
+    };
+    relax(uc, u0, fu);
+    relax(vc, v0, fu);
+    relax(wc, 0.0, fw);
+    relax(tc, th0, fu);
+    relax(wc, 0.0, fwo);
+    relax(uc, u0, fl);
+    relax(vc, v0, fl);
+    relax(wc, 0.0, fl);
+    relax(tc, th0, fl);
+    u[c] = uc;
+    v[c] = vc;
+    w[c] = wc;
+    th[c] = tc;
+}
+
+// ---------------------------------------------------------------------
 // Deterministic diagnostics: kDiagBlocksPerMember fixed-shape partials per
 // member, then one workgroup per member reduces them in a fixed order, so a
 // member's numbers do not depend on what else is in the batch.
@@ -1398,10 +1510,10 @@ __global__ __launch_bounds__(kBlock) void k_diag_partials(const MemberArgs* __re
     const MemberArgs& A = ma[wk.m];
     const DevGrid& g = A.g;
     const DevFields& f = A.f;
-    const S* w = P<S>(f.w);
-    const S* th = P<S>(f.th);
-    const S* qc = P<S>(f.qc);
-    const S* qr = P<S>(f.qr);
+    const auto w = P<S>(f.w);
+    const auto th = P<S>(f.th);
+    const auto qc = P<S>(f.qc);
+    const auto qr = P<S>(f.qr);
     double v[kDiagValues] = {-1e300, 1e300, 0.0, 0.0, 0.0, 0.0, -1e300, 0.0};
     const std::size_t ncell = std::size_t(g.nx) * g.ny * g.nz;
     const std::size_t stride = std::size_t(kDiagBlocksPerMember) * kBlock;
@@ -1421,7 +1533,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_partials(const MemberArgs* __re
         v[5] += double(qr[c]);
         if (k == 0)
         {
-            const double r = f.rain[std::size_t(j) * g.nx + i];
+            const double r = G(f.rain)[std::size_t(j) * g.nx + i];
             v[3] += r;
             v[6] = fmax(v[6], r);
         }
@@ -1444,18 +1556,18 @@ __global__ __launch_bounds__(kBlock) void k_diag_partials(const MemberArgs* __re
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < kDiagValues; ++q) f.diag[std::size_t(wk.local) * kDiagValues + q] = sm[q][0];
+        for (int q = 0; q < kDiagValues; ++q) G(f.diag)[std::size_t(wk.local) * kDiagValues + q] = sm[q][0];
 }
 
 // One workgroup per member (blockIdx.x = member index).
 __global__ void k_diag_final(const MemberArgs* __restrict__ ma, int n_members)
 {
     if (threadIdx.x != 0 || int(blockIdx.x) >= n_members) return;
-    double* d = ma[blockIdx.x].f.diag;
+    const auto d = G(ma[blockIdx.x].f.diag);
     double v[kDiagValues] = {-1e300, 1e300, 0.0, 0.0, 0.0, 0.0, -1e300, 0.0};
     for (int b = 0; b < kDiagBlocksPerMember; ++b)
     {
-        const double* p = d + std::size_t(b) * kDiagValues;
+        const auto p = d + std::size_t(b) * kDiagValues;
         v[0] = fmax(v[0], p[0]);
         v[1] = fmin(v[1], p[1]);
         v[6] = fmax(v[6], p[6]);
@@ -1559,16 +1671,27 @@ namespace
 {
 
 template <class K, class... Args>
-void launch_map(Launcher& L, const char* name, K kernel, const BatchView& b, int subset, WorkKind kind,
-                Args... args)
+void launch_map_lds(Launcher& L, const char* name, K kernel, const BatchView& b, int subset, WorkKind kind,
+                    std::size_t lds_bytes, Args... args)
 {
     const BlockMap& m = b.maps[subset][kind];
     if (m.blocks <= 0) return;
     hipEvent_t ev;
     L.begin(name, ev);
-    hipLaunchKernelGGL(kernel, dim3(unsigned(m.blocks)), dim3(kBlock), 0, L.stream, b.args, m.entries, args...);
+    hipLaunchKernelGGL(kernel, dim3(unsigned(m.blocks)), dim3(kBlock), lds_bytes, L.stream, b.args, m.entries,
+                       args...);
     L.end(name, ev);
 }
+
+template <class K, class... Args>
+void launch_map(Launcher& L, const char* name, K kernel, const BatchView& b, int subset, WorkKind kind,
+                Args... args)
+{
+    launch_map_lds(L, name, kernel, b, subset, kind, 0, args...);
+}
+
+// LDS bytes of the staged base-state profiles for the deepest member of a batch.
+std::size_t profile_lds_bytes(const BatchView& b) { return std::size_t(kLdsProfiles) * b.max_nz * sizeof(double); }
 
 template <class S>
 S* PS(void* p)
@@ -1596,8 +1719,8 @@ void launch_acoustic_substep(Launcher& L, const BatchView& b, int s, bool accumu
 {
     launch_map(L, "acoustic_forward", k_acoustic_forward<S>, b, S_EXPLICIT, W_CELL, s, int(accumulate),
                int(init_at_first));
-    launch_map(L, "acoustic_hevi_column", k_hevi_column<S>, b, S_HEVI, W_COLUMN, s, int(accumulate),
-               int(init_at_first));
+    launch_map_lds(L, "acoustic_hevi_column", k_hevi_column<S>, b, S_HEVI, W_COLUMN, profile_lds_bytes(b), s,
+                   int(accumulate), int(init_at_first));
     launch_map(L, "acoustic_momentum", k_momentum<S>, b, S_ALL, W_CELL, s);
 }
 
@@ -1657,6 +1780,7 @@ void launch_large_step(Launcher& L, const BatchView& b)
     for (int n = 0; n < b.max_kessler_sub; ++n)
         launch_map(L, "kessler_apply", k_kessler_apply<S>, b, S_ALL, W_CELL, n);
     launch_map(L, "scalars_home", k_scalars_home<S>, b, S_ALL, W_CELL);
+    if (b.any_sponge) launch_map(L, "sponge", k_sponge<S>, b, S_ALL, W_CELL);
     launch_boundaries<S>(L, b, -1);
 }
 

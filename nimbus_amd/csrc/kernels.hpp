@@ -52,6 +52,7 @@ struct DevFields
     double* rain;                                        // accumulated surface rain [ny][nx]
     double* base;                                        // kBaseCount * nz FP64 profiles
     double* diag;                                        // kDiagBlocksPerMember * kDiagValues partials + result
+    double* sponge_lat;                                  // lateral sponge factor by distance to the nearest face
 };
 
 struct StepParams
@@ -65,6 +66,8 @@ struct StepParams
     double coriolis_f;
     int bc_west, bc_east, bc_south, bc_north, bc_bottom, bc_top;
     int kessler_sub;    // microphysics subcycles per large step
+    int sponge;         // Rayleigh sponges on
+    int sponge_lateral; // lateral band width (cells)
     KesslerConst kessler;
     Bounds bounds;
 };
@@ -121,6 +124,8 @@ struct BatchView
     int n = 0;
     int max_substeps = 0;
     int max_kessler_sub = 0;
+    int any_sponge = 0;
+    int max_nz = 0;
     int elem = 4;
     BlockMap maps[S_SUBSETS][W_KINDS];
 };

@@ -57,8 +57,15 @@ enum BaseSlot
     kBasePe0 = 6,          // EOS(rho0, theta0): reference point of the perturbation EOS
     kBasePeDiff = 7,       // EOS(rho0, theta0) - p0
     kBaseInvRT0 = 8,       // 1 / (rho0 theta0)
-    kBaseCount = 9
+    kBaseSpongeUp = 9,     // upper sponge decay factor 1 - exp(-alpha dt) at cell centres
+    kBaseSpongeUpW = 10,   // the same at w faces (z = (k+1) dz)
+    kBaseSpongeW = 11,     // w-only absorber factor under the lid
+    kBaseCount = 12
 };
+
+// Deepest supported column: kernels stage five FP64 base-state profiles of
+// the deepest member in LDS (5 * 8 * 1024 B = 40 KiB per workgroup).
+constexpr int kMaxLevels = 1024;
 
 // Prognostic bounds (include/core/field/field_bounds.hpp:10-20).
 struct Bounds

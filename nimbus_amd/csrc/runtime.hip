@@ -64,10 +64,17 @@ std::size_t align256(std::size_t x) { return (x + 255) / 256 * 256; }
 
 struct Layout
 {
-    std::size_t field, rl, tri, rad_w, rad_s, rain, base, diag, total;
+    std::size_t field, rl, tri, rad_w, rad_s, rain, base, diag, lat, total;
 };
 
-Layout layout_for(const DevGrid& g, int elem, bool hevi)
+// Lateral sponge band width in cells (0 when the sponge or its band is off).
+int lateral_cells(const nimbus_step_config& c, const nimbus_grid& g)
+{
+    if (!c.sponge || !(c.sponge_lateral_alpha > 0.0)) return 0;
+    return std::max(0, std::min(c.sponge_lateral_cells, std::max(g.nx, g.ny)));
+}
+
+Layout layout_for(const DevGrid& g, int elem, bool hevi, int lateral = 0)
 {
     Layout L{};
     L.field = align256(g.cells() * std::size_t(elem));
@@ -78,7 +85,9 @@ Layout layout_for(const DevGrid& g, int elem, bool hevi)
     L.rain = align256(std::size_t(g.nx) * g.ny * sizeof(double));
     L.base = align256(std::size_t(kBaseCount) * g.nz * sizeof(double));
     L.diag = align256(std::size_t(kDiagBlocksPerMember + 1) * kDiagValues * sizeof(double));
-    L.total = L.field * F_COUNT + 3 * L.rl + 2 * L.tri + 2 * L.rad_w + 2 * L.rad_s + L.rain + L.base + L.diag;
+    L.lat = align256(std::size_t(std::max(lateral, 1)) * sizeof(double));
+    L.total = L.field * F_COUNT + 3 * L.rl + 2 * L.tri + 2 * L.rad_w + 2 * L.rad_s + L.rain + L.base + L.diag +
+              L.lat;
     return L;
 }
 
@@ -109,7 +118,7 @@ int default_k_chunks(const DevGrid& g, int tile_w)
 
 bool valid_grid(const nimbus_grid& g)
 {
-    return g.nx >= 5 && g.ny >= 5 && g.nz >= 4 && std::isfinite(g.dx) && g.dx > 0 && std::isfinite(g.dy) && g.dy > 0 &&
+    return g.nx >= 5 && g.ny >= 5 && g.nz >= 4 && g.nz <= kMaxLevels && std::isfinite(g.dx) && g.dx > 0 && std::isfinite(g.dy) && g.dy > 0 &&
            std::isfinite(g.dz) && g.dz > 0 && std::size_t(g.nx) * g.ny * g.nz < (std::size_t(1) << 31);
 }
 
@@ -123,7 +132,11 @@ bool valid_config(const nimbus_step_config& c)
            (c.bc_bottom == NIMBUS_BOTTOM_FREE_SLIP || c.bc_bottom == NIMBUS_BOTTOM_NO_SLIP) &&
            (c.bc_top == NIMBUS_TOP_RIGID || c.bc_top == NIMBUS_TOP_OPEN) &&
            !(c.acoustic_mode == NIMBUS_ACOUSTIC_HEVI && c.bc_top == NIMBUS_TOP_OPEN) && c.vt_max > 0 &&
-           std::isfinite(c.vt_max) && c.qc0 >= 0 && c.c_auto >= 0 && c.c_accr >= 0 && c.evap_eff >= 0;
+           std::isfinite(c.vt_max) && c.qc0 >= 0 && c.c_auto >= 0 && c.c_accr >= 0 && c.evap_eff >= 0 &&
+           (!c.sponge || (std::isfinite(c.sponge_z_d) && std::isfinite(c.sponge_alpha) && c.sponge_alpha >= 0 &&
+                          c.sponge_w_cells >= 0 && std::isfinite(c.sponge_w_alpha) && c.sponge_w_alpha >= 0 &&
+                          c.sponge_lateral_cells >= 0 && std::isfinite(c.sponge_lateral_alpha) &&
+                          c.sponge_lateral_alpha >= 0));
 }
 
 struct Member
@@ -282,7 +295,40 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     if (substeps < 1)
         return fail(NIMBUS_EINVAL, "acoustic substep count unresolvable (non-positive base density or "
                                    "more than 1e6 substeps required)");
-    const Layout L = layout_for(m.args.g, m.elem, hevi);
+    const int W = lateral_cells(desc.config, desc.grid);
+    // Rayleigh sponge decay factors 1 - exp(-alpha dt) (rayleigh_damping.cpp:145-163,425-486,1174-1183)
+    std::vector<double> lat(std::max(W, 1), 0.0);
+    if (desc.config.sponge)
+    {
+        const nimbus_step_config& c = desc.config;
+        const double pi_half = 1.5707963267948966, dtl = c.dt;
+        auto factor = [dtl](double alpha) { return alpha > 0.0 ? -std::expm1(-alpha * dtl) : 0.0; };
+        const double z_top = nz * desc.grid.dz, depth = z_top - c.sponge_z_d;
+        auto alpha_at = [&](double z) {
+            if (depth <= 0.0 || z <= c.sponge_z_d) return 0.0;
+            const double sn = std::sin(pi_half * std::min(1.0, (z - c.sponge_z_d) / depth));
+            return c.sponge_alpha * sn * sn;
+        };
+        double* hb = m.host_base.data();
+        for (int k = 0; k < nz; ++k)
+        {
+            hb[std::size_t(kBaseSpongeUp) * nz + k] = factor(alpha_at((k + 0.5) * desc.grid.dz));
+            hb[std::size_t(kBaseSpongeUpW) * nz + k] = factor(alpha_at((k + 1.0) * desc.grid.dz));
+        }
+        const int n = std::min(std::max(c.sponge_w_cells, 0), nz);
+        for (int d = 0; d < n && c.sponge_w_alpha > 0.0; ++d)
+        {
+            const double frac = n > 1 ? 1.0 - double(d) / double(n - 1) : 1.0;
+            const double sn = std::sin(pi_half * frac);
+            hb[std::size_t(kBaseSpongeW) * nz + (nz - 1 - d)] = factor(c.sponge_w_alpha * sn * sn);
+        }
+        for (int d = 0; d < W; ++d)
+        {
+            const double sn = std::sin(pi_half * (1.0 - double(d) / double(W)));
+            lat[d] = factor(c.sponge_lateral_alpha * sn * sn);
+        }
+    }
+    const Layout L = layout_for(m.args.g, m.elem, hevi, W);
     m.bytes = L.total;
     hipError_t e = hipMalloc(&m.block, L.total);
     if (e != hipSuccess)
@@ -324,10 +370,12 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     f.rain = reinterpret_cast<double*>(p); p += L.rain;
     f.base = reinterpret_cast<double*>(p); p += L.base;
     f.diag = reinterpret_cast<double*>(p); p += L.diag;
+    f.sponge_lat = reinterpret_cast<double*>(p); p += L.lat;
     // Zero everything once (not inside any graph), then upload the base state.
     e = hipMemset(m.block, 0, L.total);
     if (e == hipSuccess)
         e = hipMemcpy(f.base, m.host_base.data(), m.host_base.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(f.sponge_lat, lat.data(), lat.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "member initialisation");
     StepParams& sp = m.args.p;
     const nimbus_step_config& c = desc.config;
@@ -344,6 +392,8 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     const double limit = 0.8 * desc.grid.dz / c.vt_max;
     sp.kessler_sub = c.dt <= limit ? 1 : int(std::ceil(c.dt / limit));
     sp.kessler = KesslerConst{c.qc0, c.c_auto, c.c_accr, c.evap_eff, c.vt_max};
+    sp.sponge = c.sponge ? 1 : 0;
+    sp.sponge_lateral = W;
     sp.bounds = Bounds{};
     set_geometry(m, 64, 0, 1);
     return NIMBUS_OK;
@@ -375,12 +425,16 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
     args.reserve(ms.size());
     b.view.max_substeps = 0;
     b.view.max_kessler_sub = 0;
+    b.view.any_sponge = 0;
+    b.view.max_nz = 0;
     bool remap = true;
     for (Member* m : ms)
     {
         args.push_back(m->args);
         b.view.max_substeps = std::max(b.view.max_substeps, m->args.p.substeps);
         b.view.max_kessler_sub = std::max(b.view.max_kessler_sub, m->args.p.kessler_sub);
+        b.view.any_sponge = b.view.any_sponge || m->args.p.sponge;
+        b.view.max_nz = std::max(b.view.max_nz, m->args.g.nz);
         remap = remap && m->xcd_remap;
     }
     std::vector<int2> all;
@@ -719,7 +773,9 @@ uint64_t nimbus_hip_member_bytes(const nimbus_member_desc* desc)
 {
     if (!desc || !valid_grid(desc->grid)) return 0;
     const int elem = desc->precision == NIMBUS_PRECISION_MIXED ? 4 : 8;
-    return layout_for(dev_grid(desc->grid, elem), elem, desc->config.acoustic_mode == NIMBUS_ACOUSTIC_HEVI).total;
+    return layout_for(dev_grid(desc->grid, elem), elem, desc->config.acoustic_mode == NIMBUS_ACOUSTIC_HEVI,
+                      lateral_cells(desc->config, desc->grid))
+        .total;
 }
 
 int nimbus_hip_member_create(nimbus_hip_ctx* ctx, const nimbus_member_desc* desc, uint32_t* id)

@@ -43,7 +43,10 @@ class StepConfig(ctypes.Structure):
                 ("bc_west", ctypes.c_int), ("bc_east", ctypes.c_int), ("bc_south", ctypes.c_int),
                 ("bc_north", ctypes.c_int), ("bc_bottom", ctypes.c_int), ("bc_top", ctypes.c_int),
                 ("qc0", ctypes.c_double), ("c_auto", ctypes.c_double), ("c_accr", ctypes.c_double),
-                ("evap_eff", ctypes.c_double), ("vt_max", ctypes.c_double)]
+                ("evap_eff", ctypes.c_double), ("vt_max", ctypes.c_double),
+                ("sponge", ctypes.c_int), ("sponge_z_d", ctypes.c_double), ("sponge_alpha", ctypes.c_double),
+                ("sponge_w_cells", ctypes.c_int), ("sponge_w_alpha", ctypes.c_double),
+                ("sponge_lateral_cells", ctypes.c_int), ("sponge_lateral_alpha", ctypes.c_double)]
 
 
 class BaseState(ctypes.Structure):
@@ -222,11 +225,25 @@ def make_grid(nx, ny, nz, dx, dy, dz):
 
 def step_config(dt, substeps, acoustic_mode=ACOUSTIC_EXPLICIT, hevi_beta=0.1, coriolis_f=0.0,
                 lateral=BC_RADIATIVE, bottom=BOTTOM_FREE_SLIP, top=TOP_RIGID,
-                qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0):
+                qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None):
+    """Large-step configuration. `sponge` is None (off) or a dict of the
+    Rayleigh sponge settings (see sponge_preset)."""
     lat = lateral if isinstance(lateral, (tuple, list)) else (lateral,) * 4
+    sp = sponge or {}
     return StepConfig(float(dt), int(substeps), int(acoustic_mode), float(hevi_beta), float(coriolis_f),
                       lat[0], lat[1], lat[2], lat[3], int(bottom), int(top),
-                      qc0, c_auto, c_accr, evap_eff, vt_max)
+                      qc0, c_auto, c_accr, evap_eff, vt_max,
+                      int(bool(sponge)), float(sp.get("z_d", 12000.0)), float(sp.get("alpha", 0.05)),
+                      int(sp.get("w_cells", 8)), float(sp.get("w_alpha", 0.2)),
+                      int(sp.get("lateral_cells", 0)), float(sp.get("lateral_alpha", 0.05)))
+
+
+def sponge_preset(dx):
+    """The reference Kessler presets' sponges (configs/*meterResolution/kessler/baseline.yaml:
+    damping): upper sponge from 12 km at 0.05 1/s, 8-cell w-only absorber at 0.2 1/s under
+    the lid, and a 10 km lateral band at 0.05 1/s."""
+    return {"z_d": 12000.0, "alpha": 0.05, "w_cells": 8, "w_alpha": 0.2,
+            "lateral_cells": int(round(10000.0 / dx)), "lateral_alpha": 0.05}
 
 
 def base_state(grid, sounding=None):

@@ -9,7 +9,11 @@ c3  32-member ensemble of uneven grids drawn from seed 3 (nx = ny in {48, 64, 96
     minutes) with per-member sounding perturbations
 All use the analytic Weisman-Klemp-type sounding with a quarter-circle
 hodograph, +2 K bubbles of 10 km x 1.4 km radius centred 1.4 km above ground,
-and seeded uniform +-0.1 K theta noise.
+and seeded uniform +-0.1 K theta noise. Unless a workload sets sponge=False,
+its large steps carry the Rayleigh sponges of the reference Kessler presets
+(upper sponge from 12 km, 8-cell w absorber, 10 km lateral band; see
+lib.sponge_preset), without which hour-long runs reflect gravity waves off the
+lid and the lateral boundaries.
 """
 import dataclasses
 
@@ -33,6 +37,7 @@ class Workload:
     seed: int = 0
     member: int = 0
     sounding: dict = dataclasses.field(default_factory=dict)
+    sponge: bool = True
 
     @property
     def grid(self):
@@ -43,6 +48,8 @@ class Workload:
         return self.nx * self.ny * self.nz
 
     def config(self, acoustic_mode=L.ACOUSTIC_EXPLICIT, **kw):
+        if self.sponge:
+            kw.setdefault("sponge", L.sponge_preset(self.dx))
         return L.step_config(self.dt, self.substeps, acoustic_mode=acoustic_mode, **kw)
 
     def sounding_struct(self):

@@ -89,6 +89,13 @@ struct Config
     int bc_bottom = kBottomFreeSlip, bc_top = kTopRigid;
     // Kessler parameters (kessler.hpp:40-46)
     double qc0 = 1.0e-3, c_auto = 1.0e-3, c_accr = 2.2, evap_eff = 1.0, vt_max = 20.0;
+    // Rayleigh sponges (runtime_config.hpp:114-122; kessler presets' damping block)
+    int sponge = 0;
+    double sponge_z_d = 12000.0, sponge_alpha = 0.05;
+    int sponge_w_cells = 8;
+    double sponge_w_alpha = 0.2;
+    int sponge_lateral_cells = 0;
+    double sponge_lateral_alpha = 0.05;
 };
 
 template <class S>
@@ -1221,6 +1228,93 @@ void microphysics_step(const Grid& g, const Config& cfg, State<S>& s, double dt)
     }
 }
 
+// ----------------------------------------------------------------------
+// Rayleigh sponges after the EOS projection (dynamics.cpp:7325-7340 order):
+// the upper sponge relaxes u, v to the base wind, w to 0 and theta to
+// theta0 with alpha(z) = alpha_max sin^2[(pi/2) min(1, (z - z_d)/(z_top - z_d))]
+// (w at its face height z_w = (k+1) dz) (rayleigh_damping.cpp:425-465,
+// 985-1040); the w-only absorber relaxes w in the top N cells with
+// alpha = alpha_w sin^2[(pi/2)(1 - d/(N-1))], d = distance from the lid
+// (:467-486, 1055-1080); the lateral band relaxes u, v, w, theta with
+// alpha = alpha_l sin^2[(pi/2)(1 - d/W)], d = distance to the nearest
+// lateral face in cells (:1174-1183, 1360-1440). Each pass uses the exact
+// decay factor 1 - exp(-alpha dt) (:145-163) and rounds to storage.
+// ----------------------------------------------------------------------
+struct SpongeProfiles
+{
+    std::vector<double> upper, upper_w, w_only, lateral;
+};
+
+inline SpongeProfiles sponge_profiles(const Grid& g, const Config& cfg, double dt)
+{
+    const double pi_half = 1.5707963267948966;
+    SpongeProfiles sp;
+    sp.upper.assign(g.nz, 0.0);
+    sp.upper_w.assign(g.nz, 0.0);
+    sp.w_only.assign(g.nz, 0.0);
+    auto factor = [dt](double alpha) { return alpha > 0.0 ? -std::expm1(-alpha * dt) : 0.0; };
+    const double z_top = g.nz * g.dz, depth = z_top - cfg.sponge_z_d;
+    auto alpha_at = [&](double z) {
+        if (depth <= 0.0 || z <= cfg.sponge_z_d) return 0.0;
+        const double s = std::sin(pi_half * std::min(1.0, (z - cfg.sponge_z_d) / depth));
+        return cfg.sponge_alpha * s * s;
+    };
+    for (int k = 0; k < g.nz; ++k)
+    {
+        sp.upper[k] = factor(alpha_at((k + 0.5) * g.dz));
+        sp.upper_w[k] = factor(alpha_at((k + 1.0) * g.dz));
+    }
+    const int n = std::min(std::max(cfg.sponge_w_cells, 0), g.nz);
+    for (int d = 0; d < n && cfg.sponge_w_alpha > 0.0; ++d)
+    {
+        const double frac = n > 1 ? 1.0 - double(d) / double(n - 1) : 1.0;
+        const double s = std::sin(pi_half * frac);
+        sp.w_only[g.nz - 1 - d] = factor(cfg.sponge_w_alpha * s * s);
+    }
+    const int W = cfg.sponge_lateral_alpha > 0.0 ? std::max(cfg.sponge_lateral_cells, 0) : 0;
+    sp.lateral.assign(W, 0.0);
+    for (int d = 0; d < W; ++d)
+    {
+        const double s = std::sin(pi_half * (1.0 - double(d) / double(W)));
+        sp.lateral[d] = factor(cfg.sponge_lateral_alpha * s * s);
+    }
+    return sp;
+}
+
+template <class S>
+void apply_sponges(const Grid& g, const Base& b, const Config& cfg, State<S>& s, double dt)
+{
+    if (!cfg.sponge) return;
+    const SpongeProfiles sp = sponge_profiles(g, cfg, dt);
+    const int W = int(sp.lateral.size());
+    auto relax = [](S& x, double target, double f) {
+        if (f > 0.0) x = S(double(x) + f * (target - double(x)));
+    };
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+        {
+            const int dist = std::min(std::min(i, g.nx - 1 - i), std::min(j, g.ny - 1 - j));
+            for (int k = 0; k < g.nz; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                relax(s.u[c], b.u0[k], sp.upper[k]);
+                relax(s.v[c], b.v0[k], sp.upper[k]);
+                relax(s.w[c], 0.0, sp.upper_w[k]);
+                relax(s.theta[c], b.theta0[k], sp.upper[k]);
+                relax(s.w[c], 0.0, sp.w_only[k]);
+                if (dist < W)
+                {
+                    const double f = sp.lateral[dist];
+                    relax(s.u[c], b.u0[k], f);
+                    relax(s.v[c], b.v0[k], f);
+                    relax(s.w[c], 0.0, f);
+                    relax(s.theta[c], b.theta0[k], f);
+                }
+            }
+        }
+}
+
 // EOS pressure projection after physics (src/core/orchestration/dynamics/eos_apply.cpp:50-110).
 template <class S>
 void eos_projection(State<S>& s)
@@ -1283,6 +1377,7 @@ int large_step(const Grid& g, const Base& b, const Config& cfg, State<S>& s)
     s.qr.swap(tmp);
     microphysics_step(g, cfg, s, cfg.dt);
     eos_projection(s);
+    apply_sponges(g, b, cfg, s, cfg.dt);
     apply_boundaries(g, cfg, s, 0.0);
     return N;
 }

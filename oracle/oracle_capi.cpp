@@ -21,6 +21,12 @@ struct oc_config
     double coriolis_f;
     int bc_west, bc_east, bc_south, bc_north, bc_bottom, bc_top;
     double qc0, c_auto, c_accr, evap_eff, vt_max;
+    int sponge;
+    double sponge_z_d, sponge_alpha;
+    int sponge_w_cells;
+    double sponge_w_alpha;
+    int sponge_lateral_cells;
+    double sponge_lateral_alpha;
 };
 
 struct oc_base
@@ -54,6 +60,9 @@ Config to_config(const oc_config* c)
     k.bc_west = c->bc_west; k.bc_east = c->bc_east; k.bc_south = c->bc_south; k.bc_north = c->bc_north;
     k.bc_bottom = c->bc_bottom; k.bc_top = c->bc_top;
     k.qc0 = c->qc0; k.c_auto = c->c_auto; k.c_accr = c->c_accr; k.evap_eff = c->evap_eff; k.vt_max = c->vt_max;
+    k.sponge = c->sponge; k.sponge_z_d = c->sponge_z_d; k.sponge_alpha = c->sponge_alpha;
+    k.sponge_w_cells = c->sponge_w_cells; k.sponge_w_alpha = c->sponge_w_alpha;
+    k.sponge_lateral_cells = c->sponge_lateral_cells; k.sponge_lateral_alpha = c->sponge_lateral_alpha;
     return k;
 }
 
@@ -336,6 +345,17 @@ int oracle_microphysics_step_f64(const oc_grid* gp, const oc_config* cp, const o
     State<double> s;
     load(g, st, s);
     microphysics_step(g, to_config(cp), s, dt);
+    store(g, s, st);
+    return 0;
+}
+
+int oracle_apply_sponges_f64(const oc_grid* gp, const oc_base* bp, const oc_config* cp, const oc_state* st, double dt)
+{
+    const Grid g = to_grid(gp);
+    const Base b = to_base(bp, g.nz);
+    State<double> s;
+    load(g, st, s);
+    apply_sponges(g, b, to_config(cp), s, dt);
     store(g, s, st);
     return 0;
 }

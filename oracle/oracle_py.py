@@ -27,7 +27,10 @@ class OcConfig(ctypes.Structure):
                 ("bc_west", ctypes.c_int), ("bc_east", ctypes.c_int), ("bc_south", ctypes.c_int),
                 ("bc_north", ctypes.c_int), ("bc_bottom", ctypes.c_int), ("bc_top", ctypes.c_int),
                 ("qc0", ctypes.c_double), ("c_auto", ctypes.c_double), ("c_accr", ctypes.c_double),
-                ("evap_eff", ctypes.c_double), ("vt_max", ctypes.c_double)]
+                ("evap_eff", ctypes.c_double), ("vt_max", ctypes.c_double),
+                ("sponge", ctypes.c_int), ("sponge_z_d", ctypes.c_double), ("sponge_alpha", ctypes.c_double),
+                ("sponge_w_cells", ctypes.c_int), ("sponge_w_alpha", ctypes.c_double),
+                ("sponge_lateral_cells", ctypes.c_int), ("sponge_lateral_alpha", ctypes.c_double)]
 
 
 class OcBase(ctypes.Structure):
@@ -51,6 +54,7 @@ def lib():
         for name in ("oracle_large_steps_f32", "oracle_large_steps_f64"):
             getattr(L, name).argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), ctypes.c_int]
         L.oracle_resolve_substeps.argtypes = [P(OcGrid), P(OcBase), P(OcConfig)]
+        L.oracle_apply_sponges_f64.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), ctypes.c_double]
         for name in ("oracle_acoustic_substep_f32", "oracle_acoustic_substep_f64"):
             getattr(L, name).argtypes = [P(OcGrid), P(OcBase), P(OcState), ctypes.c_double]
         for name in ("oracle_hevi_substep_f64", "oracle_hevi_substep_f32"):
@@ -95,10 +99,14 @@ def grid(nx, ny, nz, dx, dy, dz):
 
 
 def config(dt, substeps, acoustic_mode=0, hevi_beta=0.1, coriolis_f=0.0, lateral=1, bottom=0, top=0,
-           qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0):
+           qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None):
     lat = lateral if isinstance(lateral, (tuple, list)) else (lateral,) * 4
+    sp = sponge or {}
     return OcConfig(float(dt), int(substeps), int(acoustic_mode), float(hevi_beta), float(coriolis_f),
-                    lat[0], lat[1], lat[2], lat[3], int(bottom), int(top), qc0, c_auto, c_accr, evap_eff, vt_max)
+                    lat[0], lat[1], lat[2], lat[3], int(bottom), int(top), qc0, c_auto, c_accr, evap_eff, vt_max,
+                    int(bool(sponge)), float(sp.get("z_d", 12000.0)), float(sp.get("alpha", 0.05)),
+                    int(sp.get("w_cells", 8)), float(sp.get("w_alpha", 0.2)), int(sp.get("lateral_cells", 0)),
+                    float(sp.get("lateral_alpha", 0.05)))
 
 
 def from_step_config(c):
@@ -217,6 +225,15 @@ def microphysics_step(g, cfg, st, dt, dtype=np.float32):
     s = copy_state(st, dtype)
     fn = lib().oracle_microphysics_step_f32 if np.dtype(dtype) == np.float32 else lib().oracle_microphysics_step_f64
     fn(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)), float(dt))
+    return s
+
+
+def apply_sponges(g, base, cfg, st, dt):
+    """The Rayleigh sponge pass alone (FP64) on a copy of st."""
+    s = copy_state(st, np.float64)
+    b = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in base.items()}
+    lib().oracle_apply_sponges_f64(ctypes.byref(g), ctypes.byref(base_struct(b)), ctypes.byref(cfg),
+                                   ctypes.byref(_state_struct(s)), float(dt))
     return s
 
 

@@ -251,10 +251,31 @@ def test_large_step_fp64_matches_fp64_oracle(hip_ctx, mode, lateral):
     assert max_abs(out["rain_mm"], ref["rain_mm"]) < 1e-9 * max(1.0, float(np.max(ref["rain_mm"])))
 
 
+# Sponge settings scaled to the 7 km deep test domain: upper sponge from 4 km,
+# 4-cell w absorber, 3-cell lateral band.
+SMALL_SPONGE = {"z_d": 4000.0, "alpha": 0.05, "w_cells": 4, "w_alpha": 0.2, "lateral_cells": 3,
+                "lateral_alpha": 0.05}
+
+
 @pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
-def test_large_step_mixed_close_to_fp64_oracle(hip_ctx, mode):
+def test_large_step_with_sponges_fp64_matches_fp64_oracle(hip_ctx, mode):
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, wind_noise=1.0, dtype=np.float64)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode, lateral=L.BC_RIGID_SLIP, sponge=SMALL_SPONGE)
+    n_steps = 4
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
+    out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, n_steps)
+    check_fp64(out, ref, f"sponge mode {mode}", False)
+    # the sponge did work in this case
+    plain, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(L.step_config(
+        3.0, 6, acoustic_mode=mode, lateral=L.BC_RIGID_SLIP)), st, n_steps, np.float64)
+    assert max_abs(plain["w"], ref["w"]) > 1e-3
+
+
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+@pytest.mark.parametrize("sponge", [None, SMALL_SPONGE])
+def test_large_step_mixed_close_to_fp64_oracle(hip_ctx, mode, sponge):
     g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, dtype=np.float64)
-    cfg = L.step_config(3.0, 6, acoustic_mode=mode)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode, sponge=sponge)
     n_steps = 4
     ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
     out, diag = _run_member(hip_ctx, g, base, cfg, {k: (v.astype(np.float32) if k in L.FIELDS else v)
