@@ -269,7 +269,11 @@ def main():
     fp64_value = runs["fp64_explicit"]["value"]
     n_sub = head["substeps"]
     ms_per_step = head["ms_per_step"]
-    step_bytes = (68 + 17 * n_sub) * 4 * wl.cells
+    # algorithmic bytes of one mixed HE-VI step: the table above summed over the
+    # kernels that ran (kernel tables cover 2 steps)
+    step_bytes = sum(launches / 2 * algo_bytes_per_cell(name, "hevi", 4) * wl.cells
+                     for name, (launches, _) in tables["mixed_hevi"].items()
+                     if algo_bytes_per_cell(name, "hevi", 4) is not None)
 
     def per_step(tab):
         return {k: round(v[1] / 2, 4) for k, v in sorted(tab.items(), key=lambda kv: -kv[1][1])}

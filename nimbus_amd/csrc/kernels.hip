@@ -60,7 +60,9 @@ __device__ __forceinline__ gptr<S> P(void* p)
 }
 
 __device__ __forceinline__ double finite0(double x) { return isfinite(x) ? x : 0.0; }
+__device__ __forceinline__ float finite0(float x) { return isfinite(x) ? x : 0.0f; }
 __device__ __forceinline__ double safe_rho(double r) { return isfinite(r) && r > 1.0e-6 ? r : 1.0; }
+__device__ __forceinline__ float safe_rho(float r) { return isfinite(r) && r > 1.0e-6f ? r : 1.0f; }
 
 // The reference sanitises the FP32 stored value (field_bounds.hpp:41-53).
 template <class S>
@@ -152,9 +154,10 @@ __device__ __forceinline__ Cell cell_of(const MemberArgs& A, int local)
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
 // ---------------------------------------------------------------------
-// Slow tendencies (cartesian_cgrid_3d.cpp:1064-1356) at one cell. The own
-// column's u, v, w are supplied as vertical windows k-3 .. k+3; everything
-// else is read directly.
+// Slow tendencies (cartesian_cgrid_3d.cpp:1064-1356) at one cell, in the
+// arithmetic type T of the storage (FP32 for the mixed mode, with buoyancy
+// in perturbation form; FP64 for FP64 storage). The own column's u, v, w are
+// supplied as vertical windows k-3 .. k+3; everything else is read directly.
 // ---------------------------------------------------------------------
 template <class S>
 struct SlowFields
@@ -162,60 +165,58 @@ struct SlowFields
     gptr<const S> u, v, w, rp, qv, qc, qr;
 };
 
-template <class S>
+template <class S, class T>
 __device__ void slow_tendency_cell(const DevGrid& g, const SlowFields<S>& F, const double* __restrict__ base,
-                                   double fcor, int i, int j, int k, const double* uz, const double* vz,
-                                   const double* wz_, double rpk, double rpk1, double qvk, double qvk1, double lk,
-                                   double lk1, double& du, double& dv, double& dw)
+                                   double fcor_d, int i, int j, int k, const T* uz, const T* vz, const T* wz_,
+                                   T rpk, T rpk1, T qvk, T qvk1, T lk, T lk1, T& du, T& dv, T& dw)
 {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const double ix = g.idx, iy = g.idy, iz = g.idz;
+    const T ix = T(g.idx), iy = T(g.idy), iz = T(g.idz), fcor = T(fcor_d);
+    const T q = T(0.25), h = T(0.5);
     const std::size_t c = g.at(i, j, k);
     const std::ptrdiff_t sx = 1, sy = g.px, sz = std::ptrdiff_t(g.plane());
-    auto R = [&](gptr<const S> a, std::ptrdiff_t o) { return double(a[std::ptrdiff_t(c) + o]); };
+    auto R = [&](gptr<const S> a, std::ptrdiff_t o) { return T(a[std::ptrdiff_t(c) + o]); };
     // u at x-face
     {
-        const double ux = uz[3];
-        const double vy = 0.25 * (R(F.v, -sy) + vz[3] + R(F.v, sx - sy) + R(F.v, sx));
-        const double wz = k == 0 ? 0.25 * (wz_[3] + R(F.w, sx))
-                                 : 0.25 * (wz_[2] + wz_[3] + R(F.w, sx - sz) + R(F.w, sx));
-        const double ddx = dev::upwind_axis<double>([&](int o) { return R(F.u, o); }, i, NX, ux, ix);
-        const double ddy = dev::upwind_axis<double>([&](int o) { return R(F.u, o * sy); }, j, NY, vy, iy);
-        const double ddz = k == 0 ? (wz >= 0.0 ? 0.0 : (uz[4] - uz[3]) * iz)
-                                  : dev::upwind_axis<double>([&](int o) { return uz[3 + o]; }, k, NZ, wz, iz);
-        const double cor = fcor * (vy - slot(base, kBaseV0, NZ)[k]);
+        const T ux = uz[3];
+        const T vy = q * (R(F.v, -sy) + vz[3] + R(F.v, sx - sy) + R(F.v, sx));
+        const T wz = k == 0 ? q * (wz_[3] + R(F.w, sx)) : q * (wz_[2] + wz_[3] + R(F.w, sx - sz) + R(F.w, sx));
+        const T ddx = dev::upwind_axis<T>([&](int o) { return R(F.u, o); }, i, NX, ux, ix);
+        const T ddy = dev::upwind_axis<T>([&](int o) { return R(F.u, o * sy); }, j, NY, vy, iy);
+        const T ddz = k == 0 ? (wz >= T(0) ? T(0) : (uz[4] - uz[3]) * iz)
+                             : dev::upwind_axis<T>([&](int o) { return uz[3 + o]; }, k, NZ, wz, iz);
+        const T cor = fcor * (vy - T(slot(base, kBaseV0, NZ)[k]));
         du = finite0(-ux * ddx - vy * ddy - wz * ddz + cor);
     }
     // v at y-face
     {
-        const double vy = vz[3];
-        const double ux = 0.25 * (R(F.u, -sx) + uz[3] + R(F.u, -sx + sy) + R(F.u, sy));
-        const double wz = k == 0 ? 0.25 * (wz_[3] + R(F.w, sy))
-                                 : 0.25 * (wz_[2] + wz_[3] + R(F.w, sy - sz) + R(F.w, sy));
-        const double ddx = dev::upwind_axis<double>([&](int o) { return R(F.v, o); }, i, NX, ux, ix);
-        const double ddy = dev::upwind_axis<double>([&](int o) { return R(F.v, o * sy); }, j, NY, vy, iy);
-        const double ddz = k == 0 ? (wz >= 0.0 ? 0.0 : (vz[4] - vz[3]) * iz)
-                                  : dev::upwind_axis<double>([&](int o) { return vz[3 + o]; }, k, NZ, wz, iz);
-        const double cor = fcor * (ux - slot(base, kBaseU0, NZ)[k]);
+        const T vy = vz[3];
+        const T ux = q * (R(F.u, -sx) + uz[3] + R(F.u, -sx + sy) + R(F.u, sy));
+        const T wz = k == 0 ? q * (wz_[3] + R(F.w, sy)) : q * (wz_[2] + wz_[3] + R(F.w, sy - sz) + R(F.w, sy));
+        const T ddx = dev::upwind_axis<T>([&](int o) { return R(F.v, o); }, i, NX, ux, ix);
+        const T ddy = dev::upwind_axis<T>([&](int o) { return R(F.v, o * sy); }, j, NY, vy, iy);
+        const T ddz = k == 0 ? (wz >= T(0) ? T(0) : (vz[4] - vz[3]) * iz)
+                             : dev::upwind_axis<T>([&](int o) { return vz[3 + o]; }, k, NZ, wz, iz);
+        const T cor = fcor * (ux - T(slot(base, kBaseU0, NZ)[k]));
         dv = finite0(-ux * ddx - vy * ddy - wz * ddz - cor);
     }
-    // w at z-face with buoyancy, moisture buoyancy and condensate loading
+    // w at z-face with buoyancy -g rho'/rho, moisture buoyancy and condensate loading
     {
-        const double wz = wz_[3];
-        const double ux = 0.25 * (R(F.u, -sx) + uz[3] + R(F.u, -sx + sz) + uz[4]);
-        const double vy = 0.25 * (R(F.v, -sy) + vz[3] + R(F.v, -sy + sz) + vz[4]);
-        const double ddx = dev::upwind_axis<double>([&](int o) { return R(F.w, o); }, i, NX, ux, ix);
-        const double ddy = dev::upwind_axis<double>([&](int o) { return R(F.w, o * sy); }, j, NY, vy, iy);
-        const double ddz = k == 0 ? (wz >= 0.0 ? wz_[3] * iz : (wz_[4] - wz_[3]) * iz)
-                                  : dev::upwind_axis<double>([&](int o) { return wz_[3 + o]; }, k, NZ, wz, iz);
+        const T wz = wz_[3];
+        const T ux = q * (R(F.u, -sx) + uz[3] + R(F.u, -sx + sz) + uz[4]);
+        const T vy = q * (R(F.v, -sy) + vz[3] + R(F.v, -sy + sz) + vz[4]);
+        const T ddx = dev::upwind_axis<T>([&](int o) { return R(F.w, o); }, i, NX, ux, ix);
+        const T ddy = dev::upwind_axis<T>([&](int o) { return R(F.w, o * sy); }, j, NY, vy, iy);
+        const T ddz = k == 0 ? (wz >= T(0) ? wz_[3] * iz : (wz_[4] - wz_[3]) * iz)
+                             : dev::upwind_axis<T>([&](int o) { return wz_[3 + o]; }, k, NZ, wz, iz);
         const auto rho0 = slot(base, kBaseRho0, NZ);
         const auto qv0s = slot(base, kBaseQv0Stored, NZ);
-        const double rpf = 0.5 * (rpk + rpk1);
-        const double rf = 0.5 * (rho0[k] + rho0[k + 1]) + rpf;
-        const double buoy = -kG * rpf / safe_rho(rf);
-        const double qvf = 0.5 * (qvk + qvk1);
-        const double mbuoy = kG * 0.608 * (qvf - 0.5 * (qv0s[k] + qv0s[k + 1]));
-        const double load = -kG * (0.5 * (lk + lk1));
+        const T rpf = h * (rpk + rpk1);
+        const T rf = T(0.5 * (rho0[k] + rho0[k + 1])) + rpf;
+        const T buoy = -T(kG) * rpf / safe_rho(rf);
+        const T qvf = h * (qvk + qvk1);
+        const T mbuoy = T(kG * 0.608) * (qvf - T(0.5 * (qv0s[k] + qv0s[k + 1])));
+        const T load = -T(kG) * (h * (lk + lk1));
         dw = finite0(-ux * ddx - vy * ddy - wz * ddz + buoy + mbuoy + load);
     }
 }
@@ -248,22 +249,23 @@ __global__ __launch_bounds__(kBlock) void k_slow(const MemberArgs* __restrict__ 
     const std::size_t sz = g.plane();
     const std::size_t col = g.at(i, j, 0);
     const std::size_t c = col + std::size_t(k) * sz;
-    double du = 0.0, dv = 0.0, dw = 0.0;
+    using T = typename AccT<S>::type;
+    T du = T(0), dv = T(0), dw = T(0);
     if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k < NZ - 1)
     {
-        double uz[7], vz[7], wz[7];
+        T uz[7], vz[7], wz[7];
 #pragma unroll
         for (int o = 0; o < 7; ++o)
         {
             const std::size_t ck = col + std::size_t(clampi(k + o - 3, 0, NZ - 1)) * sz;
-            uz[o] = double(F.u[ck]);
-            vz[o] = double(F.v[ck]);
-            wz[o] = double(F.w[ck]);
+            uz[o] = T(F.u[ck]);
+            vz[o] = T(F.v[ck]);
+            wz[o] = T(F.w[ck]);
         }
         const std::size_t c1 = c + sz;
-        slow_tendency_cell<S>(g, F, f.base, A.p.coriolis_f, i, j, k, uz, vz, wz, double(F.rp[c]), double(F.rp[c1]),
-                              double(F.qv[c]), double(F.qv[c1]), double(F.qc[c]) + double(F.qr[c]),
-                              double(F.qc[c1]) + double(F.qr[c1]), du, dv, dw);
+        slow_tendency_cell<S, T>(g, F, f.base, A.p.coriolis_f, i, j, k, uz, vz, wz, T(F.rp[c]), T(F.rp[c1]),
+                                 T(F.qv[c]), T(F.qv[c1]), T(F.qc[c]) + T(F.qr[c]), T(F.qc[c1]) + T(F.qr[c1]), du, dv,
+                                 dw);
     }
     if constexpr (MODE == 0)
     {
