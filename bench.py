@@ -48,9 +48,11 @@ ALGO_PER_CELL = {
     "acoustic_forward": (13, 0),
     # read rho'(new) p' u v w; write u v, and w for explicit members
     "acoustic_momentum": {"explicit": (8, 0), "hevi": (7, 0)},
-    # read rho' u v w theta p' mfx mfy mfz; write rho' p' w mfx mfy mfz;
+    # HE-VI horizontal part: read rho' u v theta mfx mfy; write mfx mfy rho* p*
+    "acoustic_hevi_explicit": (10, 0),
+    # HE-VI column solve: read rho* p* p' w rho' theta mfz; write w rho' p' mfz;
     # the FP64 Thomas coefficients c', d' are written and read back (32 B)
-    "acoustic_hevi_column": (15, 32),
+    "acoustic_hevi_column": (11, 32),
     # read mfx mfy mfz and four scalars; write 3 face fluxes per scalar
     "transport_flux": (19, 0),
     # read rho_b, and fluxes and value of the three moisture scalars; write 3 limiter ratios

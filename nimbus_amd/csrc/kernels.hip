@@ -153,70 +153,133 @@ __device__ __forceinline__ Cell cell_of(const MemberArgs& A, int local)
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
+// Keeps a loaded value where it is: the compiler may otherwise sink a load
+// into the branch that uses it, which puts a full memory wait inside that
+// branch. Pinning every value of a stencil after its loads are issued means
+// one wait for all of them.
+template <class T>
+__device__ __forceinline__ void pin(T& x)
+{
+    asm volatile("" : "+v"(x));
+}
+
 // ---------------------------------------------------------------------
 // Slow tendencies (cartesian_cgrid_3d.cpp:1064-1356) at one cell, in the
 // arithmetic type T of the storage (FP32 for the mixed mode, with buoyancy
-// in perturbation form; FP64 for FP64 storage). The own column's u, v, w are
-// supplied as vertical windows k-3 .. k+3; everything else is read directly.
+// in perturbation form; FP64 for FP64 storage). Every input is loaded first
+// (SlowStencil, indices clamped to the domain), so a cell's loads are all in
+// flight together and the UP5/UP3/UP1 choice near the walls only selects
+// among registers.
 // ---------------------------------------------------------------------
-template <class S>
-struct SlowFields
+template <class T>
+struct SlowStencil
 {
-    gptr<const S> u, v, w, rp, qv, qc, qr;
+    // 7-point lines through the cell, index 3 = the cell: along x, y and z
+    T ux[7], uy[7], uz[7], vx[7], vy[7], vz[7], wx[7], wy[7], wz[7];
+    // cross-face neighbours for the interpolated advecting velocities
+    T v_xp_ym, w_xp_zm, u_xm_yp, w_yp_zm, u_xm_zp, v_ym_zp;
+    T rp0, rp1, qv0, qv1, l0, l1;  // cell and the one above
+    // base state: v0, u0 at k; rho0 and the stored qv0 averaged to the face above
+    T v0, u0, rho0f, qv0f;
 };
 
 template <class S, class T>
-__device__ void slow_tendency_cell(const DevGrid& g, const SlowFields<S>& F, const double* __restrict__ base,
-                                   double fcor_d, int i, int j, int k, const T* uz, const T* vz, const T* wz_,
-                                   T rpk, T rpk1, T qvk, T qvk1, T lk, T lk1, T& du, T& dv, T& dw)
+__device__ __forceinline__ void load_slow_stencil(const DevGrid& g, gptr<const S> u, gptr<const S> v,
+                                                  gptr<const S> w, gptr<const S> rp, gptr<const S> qv,
+                                                  gptr<const S> qc, gptr<const S> qr, const double* base, int i,
+                                                  int j, int k, SlowStencil<T>& L)
+{
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const std::size_t c = g.at(i, j, k);
+    const std::ptrdiff_t sy = g.px, sz = std::ptrdiff_t(g.plane());
+#pragma unroll
+    for (int o = 0; o < 7; ++o)
+    {
+        const std::ptrdiff_t ox = clampi(i + o - 3, 0, NX - 1) - i;
+        const std::ptrdiff_t oy = std::ptrdiff_t(clampi(j + o - 3, 0, NY - 1) - j) * sy;
+        const std::ptrdiff_t oz = std::ptrdiff_t(clampi(k + o - 3, 0, NZ - 1) - k) * sz;
+        L.ux[o] = T(u[c + ox]); L.vx[o] = T(v[c + ox]); L.wx[o] = T(w[c + ox]);
+        L.uy[o] = T(u[c + oy]); L.vy[o] = T(v[c + oy]); L.wy[o] = T(w[c + oy]);
+        L.uz[o] = T(u[c + oz]); L.vz[o] = T(v[c + oz]); L.wz[o] = T(w[c + oz]);
+    }
+    const std::ptrdiff_t xm = i >= 1 ? -1 : 0, xp = i < NX - 1 ? 1 : 0;
+    const std::ptrdiff_t ym = j >= 1 ? -sy : 0, yp = j < NY - 1 ? sy : 0;
+    const std::ptrdiff_t zm = k >= 1 ? -sz : 0, zp = k < NZ - 1 ? sz : 0;
+    L.v_xp_ym = T(v[c + xp + ym]);
+    L.w_xp_zm = T(w[c + xp + zm]);
+    L.u_xm_yp = T(u[c + xm + yp]);
+    L.w_yp_zm = T(w[c + yp + zm]);
+    L.u_xm_zp = T(u[c + xm + zp]);
+    L.v_ym_zp = T(v[c + ym + zp]);
+    L.rp0 = T(rp[c]); L.rp1 = T(rp[c + zp]);
+    L.qv0 = T(qv[c]); L.qv1 = T(qv[c + zp]);
+    L.l0 = T(qc[c]) + T(qr[c]);
+    L.l1 = T(qc[c + zp]) + T(qr[c + zp]);
+    const int k1 = k < NZ - 1 ? k + 1 : k;
+    const auto rho0 = slot(base, kBaseRho0, NZ);
+    const auto qv0s = slot(base, kBaseQv0Stored, NZ);
+    L.v0 = T(slot(base, kBaseV0, NZ)[k]);
+    L.u0 = T(slot(base, kBaseU0, NZ)[k]);
+    L.rho0f = T(0.5 * (rho0[k] + rho0[k1]));
+    L.qv0f = T(0.5 * (qv0s[k] + qv0s[k1]));
+#pragma unroll
+    for (int o = 0; o < 7; ++o)
+    {
+        pin(L.ux[o]); pin(L.uy[o]); pin(L.uz[o]); pin(L.vx[o]); pin(L.vy[o]); pin(L.vz[o]);
+        pin(L.wx[o]); pin(L.wy[o]); pin(L.wz[o]);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T up7(const T* s, int pos, int n, T a, T inv)
+{
+    return dev::upwind_axis<T>([&](int o) { return s[3 + o]; }, pos, n, a, inv);
+}
+
+// Tendencies of an interior cell (1 <= i <= NX-2, 1 <= j <= NY-2, k <= NZ-2).
+template <class T>
+__device__ __forceinline__ void slow_tendency_cell(const DevGrid& g, const SlowStencil<T>& L, double fcor_d,
+                                                   int i, int j, int k, T& du, T& dv, T& dw)
 {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const T ix = T(g.idx), iy = T(g.idy), iz = T(g.idz), fcor = T(fcor_d);
     const T q = T(0.25), h = T(0.5);
-    const std::size_t c = g.at(i, j, k);
-    const std::ptrdiff_t sx = 1, sy = g.px, sz = std::ptrdiff_t(g.plane());
-    auto R = [&](gptr<const S> a, std::ptrdiff_t o) { return T(a[std::ptrdiff_t(c) + o]); };
     // u at x-face
     {
-        const T ux = uz[3];
-        const T vy = q * (R(F.v, -sy) + vz[3] + R(F.v, sx - sy) + R(F.v, sx));
-        const T wz = k == 0 ? q * (wz_[3] + R(F.w, sx)) : q * (wz_[2] + wz_[3] + R(F.w, sx - sz) + R(F.w, sx));
-        const T ddx = dev::upwind_axis<T>([&](int o) { return R(F.u, o); }, i, NX, ux, ix);
-        const T ddy = dev::upwind_axis<T>([&](int o) { return R(F.u, o * sy); }, j, NY, vy, iy);
-        const T ddz = k == 0 ? (wz >= T(0) ? T(0) : (uz[4] - uz[3]) * iz)
-                             : dev::upwind_axis<T>([&](int o) { return uz[3 + o]; }, k, NZ, wz, iz);
-        const T cor = fcor * (vy - T(slot(base, kBaseV0, NZ)[k]));
+        const T ux = L.ux[3];
+        const T vy = q * (L.vy[2] + L.vx[3] + L.v_xp_ym + L.vx[4]);
+        const T wz = k == 0 ? q * (L.wz[3] + L.wx[4]) : q * (L.wz[2] + L.wz[3] + L.w_xp_zm + L.wx[4]);
+        const T ddx = up7(L.ux, i, NX, ux, ix);
+        const T ddy = up7(L.uy, j, NY, vy, iy);
+        const T ddz = k == 0 ? (wz >= T(0) ? T(0) : (L.uz[4] - L.uz[3]) * iz) : up7(L.uz, k, NZ, wz, iz);
+        const T cor = fcor * (vy - L.v0);
         du = finite0(-ux * ddx - vy * ddy - wz * ddz + cor);
     }
     // v at y-face
     {
-        const T vy = vz[3];
-        const T ux = q * (R(F.u, -sx) + uz[3] + R(F.u, -sx + sy) + R(F.u, sy));
-        const T wz = k == 0 ? q * (wz_[3] + R(F.w, sy)) : q * (wz_[2] + wz_[3] + R(F.w, sy - sz) + R(F.w, sy));
-        const T ddx = dev::upwind_axis<T>([&](int o) { return R(F.v, o); }, i, NX, ux, ix);
-        const T ddy = dev::upwind_axis<T>([&](int o) { return R(F.v, o * sy); }, j, NY, vy, iy);
-        const T ddz = k == 0 ? (wz >= T(0) ? T(0) : (vz[4] - vz[3]) * iz)
-                             : dev::upwind_axis<T>([&](int o) { return vz[3 + o]; }, k, NZ, wz, iz);
-        const T cor = fcor * (ux - T(slot(base, kBaseU0, NZ)[k]));
+        const T vy = L.vy[3];
+        const T ux = q * (L.ux[2] + L.uy[3] + L.u_xm_yp + L.uy[4]);
+        const T wz = k == 0 ? q * (L.wz[3] + L.wy[4]) : q * (L.wz[2] + L.wz[3] + L.w_yp_zm + L.wy[4]);
+        const T ddx = up7(L.vx, i, NX, ux, ix);
+        const T ddy = up7(L.vy, j, NY, vy, iy);
+        const T ddz = k == 0 ? (wz >= T(0) ? T(0) : (L.vz[4] - L.vz[3]) * iz) : up7(L.vz, k, NZ, wz, iz);
+        const T cor = fcor * (ux - L.u0);
         dv = finite0(-ux * ddx - vy * ddy - wz * ddz - cor);
     }
     // w at z-face with buoyancy -g rho'/rho, moisture buoyancy and condensate loading
     {
-        const T wz = wz_[3];
-        const T ux = q * (R(F.u, -sx) + uz[3] + R(F.u, -sx + sz) + uz[4]);
-        const T vy = q * (R(F.v, -sy) + vz[3] + R(F.v, -sy + sz) + vz[4]);
-        const T ddx = dev::upwind_axis<T>([&](int o) { return R(F.w, o); }, i, NX, ux, ix);
-        const T ddy = dev::upwind_axis<T>([&](int o) { return R(F.w, o * sy); }, j, NY, vy, iy);
-        const T ddz = k == 0 ? (wz >= T(0) ? wz_[3] * iz : (wz_[4] - wz_[3]) * iz)
-                             : dev::upwind_axis<T>([&](int o) { return wz_[3 + o]; }, k, NZ, wz, iz);
-        const auto rho0 = slot(base, kBaseRho0, NZ);
-        const auto qv0s = slot(base, kBaseQv0Stored, NZ);
-        const T rpf = h * (rpk + rpk1);
-        const T rf = T(0.5 * (rho0[k] + rho0[k + 1])) + rpf;
+        const T wz = L.wz[3];
+        const T ux = q * (L.ux[2] + L.uz[3] + L.u_xm_zp + L.uz[4]);
+        const T vy = q * (L.vy[2] + L.vz[3] + L.v_ym_zp + L.vz[4]);
+        const T ddx = up7(L.wx, i, NX, ux, ix);
+        const T ddy = up7(L.wy, j, NY, vy, iy);
+        const T ddz = k == 0 ? (wz >= T(0) ? L.wz[3] * iz : (L.wz[4] - L.wz[3]) * iz) : up7(L.wz, k, NZ, wz, iz);
+        const T rpf = h * (L.rp0 + L.rp1);
+        const T rf = L.rho0f + rpf;
         const T buoy = -T(kG) * rpf / safe_rho(rf);
-        const T qvf = h * (qvk + qvk1);
-        const T mbuoy = T(kG * 0.608) * (qvf - T(0.5 * (qv0s[k] + qv0s[k + 1])));
-        const T load = -T(kG) * (h * (lk + lk1));
+        const T qvf = h * (L.qv0 + L.qv1);
+        const T mbuoy = T(kG * 0.608) * (qvf - L.qv0f);
+        const T load = -T(kG) * (h * (L.l0 + L.l1));
         dw = finite0(-ux * ddx - vy * ddy - wz * ddz + buoy + mbuoy + load);
     }
 }
@@ -237,57 +300,42 @@ __global__ __launch_bounds__(kBlock) void k_slow(const MemberArgs* __restrict__ 
     const int i = cl.i, j = cl.j, k = cl.k;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const DevFields& f = A.f;
-    SlowFields<S> F;
-    F.u = P<S>(MODE == 1 ? f.u2 : f.u);
-    F.v = P<S>(MODE == 1 ? f.v2 : f.v);
-    F.w = P<S>(MODE == 1 ? f.w2 : f.w);
-    F.rp = P<S>(f.rp);
-    F.qv = P<S>(f.qv);
-    F.qc = P<S>(f.qc);
-    F.qr = P<S>(f.qr);
     const double dt = A.p.dt;
-    const std::size_t sz = g.plane();
-    const std::size_t col = g.at(i, j, 0);
-    const std::size_t c = col + std::size_t(k) * sz;
+    const std::size_t c = g.at(i, j, k);
     using T = typename AccT<S>::type;
+    SlowStencil<T> L;
+    load_slow_stencil<S, T>(g, P<S>(MODE == 1 ? f.u2 : f.u), P<S>(MODE == 1 ? f.v2 : f.v),
+                            P<S>(MODE == 1 ? f.w2 : f.w), P<S>(f.rp), P<S>(f.qv), P<S>(f.qc), P<S>(f.qr), f.base, i, j,
+                            k, L);
+    // the stage-2 operands, loaded with the stencil
+    S k1u = S(0), k1v = S(0), k1w = S(0), un = S(0), vn = S(0), wn = S(0);
+    if constexpr (MODE == 1)
+    {
+        k1u = P<S>(f.k1u)[c]; k1v = P<S>(f.k1v)[c]; k1w = P<S>(f.k1w)[c];
+        un = P<S>(f.u)[c]; vn = P<S>(f.v)[c]; wn = P<S>(f.w)[c];
+    }
     T du = T(0), dv = T(0), dw = T(0);
     if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k < NZ - 1)
-    {
-        T uz[7], vz[7], wz[7];
-#pragma unroll
-        for (int o = 0; o < 7; ++o)
-        {
-            const std::size_t ck = col + std::size_t(clampi(k + o - 3, 0, NZ - 1)) * sz;
-            uz[o] = T(F.u[ck]);
-            vz[o] = T(F.v[ck]);
-            wz[o] = T(F.w[ck]);
-        }
-        const std::size_t c1 = c + sz;
-        slow_tendency_cell<S, T>(g, F, f.base, A.p.coriolis_f, i, j, k, uz, vz, wz, T(F.rp[c]), T(F.rp[c1]),
-                                 T(F.qv[c]), T(F.qv[c1]), T(F.qc[c]) + T(F.qr[c]), T(F.qc[c1]) + T(F.qr[c1]), du, dv,
-                                 dw);
-    }
+        slow_tendency_cell<T>(g, L, A.p.coriolis_f, i, j, k, du, dv, dw);
     if constexpr (MODE == 0)
     {
         const S a = S(du), b = S(dv), d = S(dw);
         P<S>(f.k1u)[c] = a;
         P<S>(f.k1v)[c] = b;
         P<S>(f.k1w)[c] = d;
-        P<S>(f.u2)[c] = S(double(F.u[c]) + double(a) * dt);
-        P<S>(f.v2)[c] = S(double(F.v[c]) + double(b) * dt);
-        P<S>(f.w2)[c] = S(double(F.w[c]) + double(d) * dt);
+        // the window centres are the cell's own u, v, w (exact in T)
+        P<S>(f.u2)[c] = S(double(S(L.uz[3])) + double(a) * dt);
+        P<S>(f.v2)[c] = S(double(S(L.vz[3])) + double(b) * dt);
+        P<S>(f.w2)[c] = S(double(S(L.wz[3])) + double(d) * dt);
     }
     else if constexpr (MODE == 1)
     {
-        const S au = S(S(0.5) * (S(du) + P<S>(f.k1u)[c]));
-        const S av = S(S(0.5) * (S(dv) + P<S>(f.k1v)[c]));
-        const S aw = S(S(0.5) * (S(dw) + P<S>(f.k1w)[c]));
-        const auto u = P<S>(f.u);
-        const auto v = P<S>(f.v);
-        const auto w = P<S>(f.w);
-        u[c] = S(double(u[c]) + finite0(double(au)) * dt);
-        v[c] = S(double(v[c]) + finite0(double(av)) * dt);
-        w[c] = S(double(w[c]) + finite0(double(aw)) * dt);
+        const S au = S(S(0.5) * (S(du) + k1u));
+        const S av = S(S(0.5) * (S(dv) + k1v));
+        const S aw = S(S(0.5) * (S(dw) + k1w));
+        P<S>(f.u)[c] = S(double(un) + finite0(double(au)) * dt);
+        P<S>(f.v)[c] = S(double(vn) + finite0(double(av)) * dt);
+        P<S>(f.w)[c] = S(double(wn) + finite0(double(aw)) * dt);
     }
     else
     {
@@ -357,6 +405,23 @@ struct LProf
 };
 
 __device__ __forceinline__ GProf gprof(const double* base, int nz) { return GProf{G(base), nz}; }
+
+// The same profile values of one level, loaded once into registers.
+struct VProf
+{
+    double r0, p0_, irt, pe0_, ped;
+    __device__ double rho0(int) const { return r0; }
+    __device__ double p0(int) const { return p0_; }
+    __device__ double inv_rt0(int) const { return irt; }
+    __device__ double pe0(int) const { return pe0_; }
+    __device__ double pe_diff(int) const { return ped; }
+};
+
+__device__ __forceinline__ VProf vprof(const double* base, int nz, int k)
+{
+    const GProf g = gprof(base, nz);
+    return VProf{g.rho0(k), g.p0(k), g.inv_rt0(k), g.pe0(k), g.pe_diff(k)};
+}
 
 // Copies the LProf profiles of a member into LDS (all threads of the block take part).
 __device__ __forceinline__ LProf stage_profiles(double* lds, const double* base, int nz)
@@ -435,38 +500,52 @@ __device__ __forceinline__ RhoP<S> acoustic_forward(const DevGrid& g, const Acou
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     const std::size_t c = g.at(i, j, k);
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
+    // all loads first, neighbours clamped to the domain (masked below)
+    const std::size_t cxp = i < NX - 1 ? c + 1 : c, cyp = j < NY - 1 ? c + sy : c, czp = k < NZ - 1 ? c + sz : c;
+    const std::size_t cxm = i >= 1 ? c - 1 : c, cym = j >= 1 ? c - sy : c, czm = k >= 1 ? c - sz : c;
+    const int kp = k < NZ - 1 ? k + 1 : k, km = k >= 1 ? k - 1 : k;
+    S ro_c = B.ro[c], ro_xp = B.ro[cxp], ro_yp = B.ro[cyp], ro_zp = B.ro[czp];
+    S ro_xm = B.ro[cxm], ro_ym = B.ro[cym], ro_zm = B.ro[czm];
+    S u_c = B.uo[c], v_c = B.vo[c], w_c = B.wo[c], u_xm = B.uo[cxm], v_ym = B.vo[cym], w_zm = B.wo[czm];
+    S th_c = B.th[c];
+    const bool acc = store && accumulate;
+    S mfx0 = B.mfx[c], mfy0 = B.mfy[c], mfz0 = B.mfz[c];  // used only when accumulating onto them
     const auto rho0 = slot(base, kBaseRho0, NZ);
-    const double r0 = rho0[k];
-    const double rc = r0 + double(B.ro[c]);
-    const double rxp = i < NX - 1 ? r0 + double(B.ro[c + 1]) : 0.0;
-    const double ryp = j < NY - 1 ? r0 + double(B.ro[c + sy]) : 0.0;
-    const double rzp = k < NZ - 1 ? rho0[k + 1] + double(B.ro[c + sz]) : 0.0;
-    const double uc = double(B.uo[c]), vc = double(B.vo[c]), wc = double(B.wo[c]);
-    if (store && accumulate)
+    VProf pv = vprof(base, NZ, k);
+    const double r0 = pv.r0, r0p = rho0[kp], r0m = rho0[km];
+    pin(ro_c); pin(ro_xp); pin(ro_yp); pin(ro_zp); pin(ro_xm); pin(ro_ym); pin(ro_zm);
+    pin(u_c); pin(v_c); pin(w_c); pin(u_xm); pin(v_ym); pin(w_zm); pin(th_c);
+    pin(mfx0); pin(mfy0); pin(mfz0);
+    const double rc = r0 + double(ro_c);
+    const double rxp = i < NX - 1 ? r0 + double(ro_xp) : 0.0;
+    const double ryp = j < NY - 1 ? r0 + double(ro_yp) : 0.0;
+    const double rzp = k < NZ - 1 ? r0p + double(ro_zp) : 0.0;
+    const double uc = double(u_c), vc = double(v_c), wc = double(w_c);
+    if (acc)
     {
-        if (i < NX - 1) B.mfx[c] = S((first ? 0.0 : double(B.mfx[c])) + weight * (0.5 * (rc + rxp) * uc));
-        if (j < NY - 1) B.mfy[c] = S((first ? 0.0 : double(B.mfy[c])) + weight * (0.5 * (rc + ryp) * vc));
+        if (i < NX - 1) B.mfx[c] = S((first ? 0.0 : double(mfx0)) + weight * (0.5 * (rc + rxp) * uc));
+        if (j < NY - 1) B.mfy[c] = S((first ? 0.0 : double(mfy0)) + weight * (0.5 * (rc + ryp) * vc));
         double fl = 0.0;
         if (k < NZ - 1) fl = 0.5 * (rc + rzp) * wc;
         else if (top_open) fl = rc * wc;
-        B.mfz[c] = S((first ? 0.0 : double(B.mfz[c])) + weight * fl);
+        B.mfz[c] = S((first ? 0.0 : double(mfz0)) + weight * fl);
     }
-    if (store && first) B.rb[c] = B.ro[c];
+    if (store && first) B.rb[c] = ro_c;
     double d = 0.0;
     if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1)
     {
-        const double rxm = r0 + double(B.ro[c - 1]);
-        const double rym = r0 + double(B.ro[c - sy]);
+        const double rxm = r0 + double(ro_xm);
+        const double rym = r0 + double(ro_ym);
         const double fxp = 0.5 * (rc + rxp) * uc;
-        const double fxm = 0.5 * (rxm + rc) * double(B.uo[c - 1]);
+        const double fxm = 0.5 * (rxm + rc) * double(u_xm);
         const double fyp = 0.5 * (rc + ryp) * vc;
-        const double fym = 0.5 * (rym + rc) * double(B.vo[c - sy]);
+        const double fym = 0.5 * (rym + rc) * double(v_ym);
         const double fzp = k + 1 < NZ ? 0.5 * (rc + rzp) * wc : rc * wc;
-        const double fzm = k > 0 ? 0.5 * ((rho0[k - 1] + double(B.ro[c - sz])) + rc) * double(B.wo[c - sz]) : 0.0;
+        const double fzm = k > 0 ? 0.5 * ((r0m + double(ro_zm)) + rc) * double(w_zm) : 0.0;
         d = -((fxp - fxm) * g.idx + (fyp - fym) * g.idy + (fzp - fzm) * g.idz);
     }
     const S drho = S(finite0(d));
-    const RhoP<S> o = density_eos<S>(bd, gprof(base, NZ), k, rc + double(drho) * dts, double(B.th[c]));
+    const RhoP<S> o = density_eos<S>(bd, pv, k, rc + double(drho) * dts, double(th_c));
     if (store)
     {
         B.rn[c] = o.r;
@@ -478,22 +557,20 @@ __device__ __forceinline__ RhoP<S> acoustic_forward(const DevGrid& g, const Acou
 // Backward momentum update of one cell with the new rho' and p' of the cell
 // and its +x, +y, +z neighbours (acoustic_cgrid_fast.cpp:159-215).
 template <class S>
-__device__ __forceinline__ void momentum_update(const DevGrid& g, const double* __restrict__ base, const Bounds& bd,
+__device__ __forceinline__ void momentum_update(const DevGrid& g, double r0, double r0p, const Bounds& bd,
                                                 double dts, int i, int j, int k, RhoP<S> cc, RhoP<S> xp, RhoP<S> yp,
                                                 RhoP<S> zp, double u, double v, double w, bool skip_w, gptr<S> un,
                                                 gptr<S> vn, gptr<S> wn, std::size_t c)
 {
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const auto rho0 = slot(base, kBaseRho0, NZ);
     double du = 0.0, dv = 0.0, dw = 0.0;
     const bool interior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k < NZ - 1;
     if (interior)
     {
-        const double r0 = rho0[k];
         const double rc = r0 + double(cc.r);
         const double rx = 0.5 * (rc + (r0 + double(xp.r)));
         const double ry = 0.5 * (rc + (r0 + double(yp.r)));
-        const double rz = 0.5 * (rc + (rho0[k + 1] + double(zp.r)));
+        const double rz = 0.5 * (rc + (r0p + double(zp.r)));
         if constexpr (sizeof(S) == sizeof(float))
         {
             // FP32 tendency, as the reference rounds it to FP32 before the update
@@ -546,18 +623,103 @@ __global__ __launch_bounds__(kBlock) void k_acoustic_forward(const MemberArgs* _
 }
 
 // ---------------------------------------------------------------------
-// HE-VI column kernel: horizontal continuity explicit, vertical w-rho-p
-// coupling off-centred implicit (alpha = (1+beta)/2): the vertical pressure
-// gradient acts on alpha p^{n+1} + (1 - alpha) p^n with p^{n+1} linearised
-// about p* = EOS(rho*); FP64 Thomas sweep per column, flux-form density
-// update and EOS re-diagnosis. Ghost columns get the reference's pass-2 clamp/EOS only.
-// rho' is read from buffer s & 1 and written to (s + 1) & 1; u, v, w stay in
-// their home buffers (w is only touched by its own column).
+// HE-VI substep: horizontal continuity explicit, vertical w-rho-p coupling
+// off-centred implicit (alpha = (1+beta)/2): the vertical pressure gradient
+// acts on alpha p^{n+1} + (1 - alpha) p^n with p^{n+1} linearised about
+// p* = EOS(rho*); FP64 Thomas sweep per column, flux-form density update and
+// EOS re-diagnosis. Two kernels: k_hevi_explicit (one thread per cell) does
+// the horizontal part, k_hevi_column (one thread per column) the vertical
+// solve. rho' is read from buffer s & 1 and written to (s + 1) & 1; u, v, w
+// stay in their home buffers (w is only touched by its own column).
 // ---------------------------------------------------------------------
+
+// Part 1, per cell: retained horizontal mass fluxes, the density after the
+// explicit horizontal divergence rho* and its EOS pressure p*, stored as
+// perturbations (rho* - rho0 in the new-density buffer, p* - p0 in the u2
+// scratch field, which is dead between the slow stage 2 and transport).
+// Ghost columns get the reference's pass-2 clamp/EOS here and are not
+// touched by the column solve.
 template <class S>
-struct HeviLevel
+__global__ __launch_bounds__(kBlock) void k_hevi_explicit(const MemberArgs* __restrict__ ma,
+                                                          const int2* __restrict__ map, int s, int accumulate,
+                                                          int init)
 {
-    S rp, rpxp, rpxm, rpyp, rpym, u, um, v, vm, w, th, pn, mfx, mfy;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (s >= A.p.substeps) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const DevGrid& g = A.g;
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const int i = cl.i, j = cl.j, k = cl.k;
+    const DevFields& f = A.f;
+    const int par = s & 1;
+    const auto u = P<S>(f.u), v = P<S>(f.v), th = P<S>(f.th);
+    const auto rp_in = P<S>(par ? f.rp2 : f.rp);
+    const auto rp_out = P<S>(par ? f.rp : f.rp2);
+    const auto mfx = P<S>(f.mfx), mfy = P<S>(f.mfy);
+    const bool acc = accumulate != 0, first = init != 0 && s == 0;
+    const double dts = A.p.dts, weight = A.p.mf_weight;
+    const std::size_t sy = std::size_t(g.px);
+    const std::size_t c = g.at(i, j, k);
+    const std::size_t cxp = i < NX - 1 ? c + 1 : c, cyp = j < NY - 1 ? c + sy : c;
+    const std::size_t cxm = i >= 1 ? c - 1 : c, cym = j >= 1 ? c - sy : c;
+    S rpc = rp_in[c], rpxp = rp_in[cxp], rpyp = rp_in[cyp], rpxm = rp_in[cxm], rpym = rp_in[cym];
+    S uc = u[c], um = u[cxm], vc = v[c], vm = v[cym], thc = th[c];
+    S mfx0 = mfx[c], mfy0 = mfy[c];  // used only when accumulating onto them
+    const VProf pv = vprof(f.base, NZ, k);
+    pin(rpc); pin(rpxp); pin(rpyp); pin(rpxm); pin(rpym); pin(uc); pin(um); pin(vc); pin(vm); pin(thc);
+    pin(mfx0); pin(mfy0);
+    const double r0 = pv.r0;
+    const double rc = r0 + double(rpc);
+    const double rxp = r0 + double(rpxp), ryp = r0 + double(rpyp);
+    if (acc)
+    {
+        if (i < NX - 1) mfx[c] = S((first ? 0.0 : double(mfx0)) + weight * (0.5 * (rc + rxp) * double(uc)));
+        if (j < NY - 1) mfy[c] = S((first ? 0.0 : double(mfy0)) + weight * (0.5 * (rc + ryp) * double(vc)));
+    }
+    if (first) P<S>(f.rb)[c] = rpc;
+    if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
+    {
+        // ghost column: density unchanged (clamped), EOS pressure, no vertical flux
+        if (acc && first) P<S>(f.mfz)[c] = S(0);
+        const RhoP<S> o = density_eos<S>(A.p.bounds, pv, k, rc, double(thc));
+        rp_out[c] = o.r;
+        P<S>(f.pp)[c] = o.p;
+        return;
+    }
+    const double rxm = r0 + double(rpxm), rym = r0 + double(rpym);
+    const double dh = -((0.5 * (rc + rxp) * double(uc) - 0.5 * (rxm + rc) * double(um)) * g.idx +
+                        (0.5 * (rc + ryp) * double(vc) - 0.5 * (rym + rc) * double(vm)) * g.idy);
+    const double rstar = rc + dts * finite0(dh);
+    // linearisation point p* = EOS(rho*): FP64 power law in FP64 storage, the
+    // perturbation form (FP32 relative accuracy on p* - p0) in the mixed mode
+    const double ps = eos_full_pressure<S>(pv, k, rstar, double(thc));
+    rp_out[c] = S(rstar - r0);
+    P<S>(f.u2)[c] = S(ps - pv.p0_);
+}
+
+// Part 2, one thread per interior column. Up-sweep: each row (face) of the
+// w system from its two cells' rho*, p*, substep-start p and face
+// densities, eliminated at once (FP64 c', d' kept in tri_c / tri_r).
+// Down-sweep: back substitution, off-centred vertical mass flux, final
+// density and EOS pressure. The inputs of the next kRing - 1 levels are in
+// flight while a level is processed: a ring of kRing register sets, the loop
+// unrolled by kRing so the ring rotates by renaming, every load unconditional
+// (levels clamped to the column), the base-state profiles in LDS.
+constexpr int kRing = 4;
+
+template <class S>
+struct HeviUp
+{
+    S rs, ps, pn, w, rp;  // rho* - rho0, p* - p0, substep-start p', w, substep-start rho'
+};
+
+template <class S>
+struct HeviDown
+{
+    double tr, tc;
+    S w, rp, rs, th, mfz;
 };
 
 template <class S>
@@ -570,211 +732,149 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
     const DevGrid& g = A.g;
     const Column t = column_of(A, wk.local);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    // Base-state profiles in LDS: their per-level reads then do not wait
-    // behind the prefetched field loads of the levels above.
     extern __shared__ double s_prof[];
     const LProf pr = stage_profiles(s_prof, A.f.base, NZ);
-    if (t.i >= NX || t.j >= NY) return;  // no barrier after this point
     const int i = t.i, j = t.j;
+    if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1)) return;  // no barrier after this point
     const DevFields& f = A.f;
     const int par = s & 1;
-    const gptr<const S> __restrict__ u = P<S>(f.u);
-    const gptr<const S> __restrict__ v = P<S>(f.v);
     const gptr<S> __restrict__ w = P<S>(f.w);
     const gptr<const S> __restrict__ rp_in = P<S>(par ? f.rp2 : f.rp);
     const gptr<S> __restrict__ rp_out = P<S>(par ? f.rp : f.rp2);
     const gptr<S> __restrict__ pp = P<S>(f.pp);
+    const gptr<const S> __restrict__ psb = P<S>(f.u2);
     const gptr<const S> __restrict__ th = P<S>(f.th);
-    const gptr<S> __restrict__ mfx = P<S>(f.mfx);
-    const gptr<S> __restrict__ mfy = P<S>(f.mfy);
     const gptr<S> __restrict__ mfz = P<S>(f.mfz);
-    const gptr<S> __restrict__ rb = P<S>(f.rb);
     const gptr<double> __restrict__ tc = G(f.tri_c);
     const gptr<double> __restrict__ tr = G(f.tri_r);
     const Bounds bd = A.p.bounds;
     const double dts = A.p.dts, alpha = A.p.hevi_alpha, weight = A.p.mf_weight;
     const bool acc = accumulate != 0, first = init != 0 && s == 0;
-    const std::size_t sy = std::size_t(g.px), sz = g.plane();
+    const std::size_t c0 = g.at(i, j, 0), sz = g.plane();
     const double idz = g.idz;
     const double a1 = 1.0 - alpha;
-    if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
-    {
-        // ghost column: mass fluxes, density unchanged (clamped), EOS pressure
-        for (int k = 0; k < NZ; ++k)
-        {
-            const std::size_t c = g.at(i, j, k);
-            const double r0 = pr.rho0(k);
-            const double rc = r0 + double(rp_in[c]);
-            if (acc)
-            {
-                if (i < NX - 1)
-                {
-                    const double fl = 0.5 * (rc + (r0 + double(rp_in[c + 1]))) * double(u[c]);
-                    mfx[c] = S((first ? 0.0 : double(mfx[c])) + weight * fl);
-                }
-                if (j < NY - 1)
-                {
-                    const double fl = 0.5 * (rc + (r0 + double(rp_in[c + sy]))) * double(v[c]);
-                    mfy[c] = S((first ? 0.0 : double(mfy[c])) + weight * fl);
-                }
-                if (first) mfz[c] = S(0);
-            }
-            if (first) rb[c] = rp_in[c];
-            const RhoP<S> o = density_eos<S>(bd, pr, k, rc, double(th[c]));
-            rp_out[c] = o.r;
-            pp[c] = o.p;
-        }
-        return;
-    }
-    // ---- upward sweep: mass fluxes, rho*, p*, Thomas forward elimination.
-    // The inputs of levels k + 1 and k + 2 are loaded before level k is
-    // computed and stored, so every thread keeps loads of three levels in
-    // flight. The loads are unconditional (levels clamped to the column, mass
-    // fluxes read even when they are then ignored), so no branch splits the
-    // load stream and the wait for level k leaves the younger loads in flight.
-    auto load_level = [&](int k) {
-        const std::size_t c = g.at(i, j, k);
-        HeviLevel<S> L;
-        L.rp = rp_in[c]; L.rpxp = rp_in[c + 1]; L.rpxm = rp_in[c - 1];
-        L.rpyp = rp_in[c + sy]; L.rpym = rp_in[c - sy];
-        L.u = u[c]; L.um = u[c - 1]; L.v = v[c]; L.vm = v[c - sy];
-        L.w = w[c]; L.th = th[c]; L.pn = pp[c];
-        L.mfx = mfx[c];
-        L.mfy = mfy[c];
+
+    // ---- up-sweep
+    auto load_up = [&](int k) {
+        const std::size_t c = c0 + std::size_t(k) * sz;
+        HeviUp<S> L;
+        L.rs = rp_out[c]; L.ps = psb[c]; L.pn = pp[c]; L.w = w[c]; L.rp = rp_in[c];
         return L;
     };
+    HeviUp<S> ring[kRing];
+#pragma unroll
+    for (int q = 0; q < kRing; ++q) ring[q] = load_up(min(q, NZ - 1));
     double rb_prev = 0.0, C_prev = 0.0, ps_prev = 0.0, pn_prev = 0.0, wo_prev = 0.0, e_prev = 0.0;
     double rb_prev2 = 0.0, cp_prev = 0.0, rr_prev = 0.0;
     bool ok = true;
-    HeviLevel<S> cur = load_level(0);
-    HeviLevel<S> nxt = load_level(1);  // NZ >= 4
-    // unrolled by the prefetch depth, so the level registers rotate by renaming
-#pragma unroll 3
-    for (int k = 0; k < NZ; ++k)
+    for (int k0 = 0; k0 < NZ; k0 += kRing)
     {
-        const HeviLevel<S> nxt2 = load_level(min(k + 2, NZ - 1));
-        const std::size_t c = g.at(i, j, k);
-        const double r0 = pr.rho0(k);
-        const double rc = r0 + double(cur.rp);
-        const double rxp = r0 + double(cur.rpxp), rxm = r0 + double(cur.rpxm);
-        const double ryp = r0 + double(cur.rpyp), rym = r0 + double(cur.rpym);
-        if (acc)
+#pragma unroll
+        for (int q = 0; q < kRing; ++q)
         {
-            mfx[c] = S((first ? 0.0 : double(cur.mfx)) + weight * (0.5 * (rc + rxp) * double(cur.u)));
-            mfy[c] = S((first ? 0.0 : double(cur.mfy)) + weight * (0.5 * (rc + ryp) * double(cur.v)));
+            const int k = k0 + q;
+            const HeviUp<S> cur = ring[q];
+            const HeviUp<S> nxt = ring[(q + 1) % kRing];  // level k + 1
+            if (k < NZ)  // not a loop exit, and the refill below reuses cur's registers
+            {
+                const double r0 = pr.rho0(k);
+                const double rc = r0 + double(cur.rp);
+                const double rstar = r0 + double(cur.rs);
+                const double ps = pr.p0(k) + double(cur.ps);
+                const double C = kGamma * ps * (1.0 / rstar);
+                const double wo = double(cur.w);
+                // face density above level k (zero at the lid)
+                const double rbk = k < NZ - 1 ? 0.5 * (rc + (pr.rho0(k + 1) + double(nxt.rp))) : 0.0;
+                const double e = -a1 * dts * idz * (rbk * wo - rb_prev * wo_prev);
+                const double pnk = double(cur.pn);
+                if (k >= 1)
+                {
+                    // row r = k-1 (face k-1/2): unknowns w_{r-1}, w_r, w_{r+1}
+                    const double rbr = rb_prev;
+                    const double inv_rbr = 1.0 / rbr;
+                    const double Bc = alpha * dts * idz;
+                    const double Ak = Bc * inv_rbr;
+                    const double Aa = (k - 1) > 0 ? -Ak * Bc * C_prev * rb_prev2 : 0.0;
+                    const double U = (k - 1) < NZ - 2 ? -Ak * Bc * C * rbk : 0.0;
+                    const double D = 1.0 + Ak * Bc * (C + C_prev) * rbr;
+                    // PGF on alpha p* + (1 - alpha) p^n (p^n: substep-start pressure), base gradient removed
+                    const double G_ = (alpha * ((ps - ps_prev) - (pr.p0(k) - pr.p0(k - 1))) + a1 * (pnk - pn_prev)) * idz;
+                    const double R = wo_prev - dts * inv_rbr * G_ - Ak * (C * e - C_prev * e_prev);
+                    const double piv = D - Aa * cp_prev;
+                    if (!(piv > 0.0) || !isfinite(piv)) ok = false;
+                    const double inv_piv = 1.0 / piv;
+                    const double cpr = U * inv_piv;
+                    const double rrr = (R - Aa * rr_prev) * inv_piv;
+                    const std::size_t cr = c0 + std::size_t(k - 1) * sz;
+                    tc[cr] = cpr;
+                    tr[cr] = rrr;
+                    cp_prev = cpr;
+                    rr_prev = rrr;
+                }
+                rb_prev2 = rb_prev;
+                rb_prev = rbk;
+                C_prev = C;
+                ps_prev = ps;
+                pn_prev = pnk;
+                wo_prev = wo;
+                e_prev = e;
+            }
+            ring[q] = load_up(min(k + kRing, NZ - 1));
         }
-        if (first) rb[c] = cur.rp;
-        const double dh = -((0.5 * (rc + rxp) * double(cur.u) - 0.5 * (rxm + rc) * double(cur.um)) * g.idx +
-                            (0.5 * (rc + ryp) * double(cur.v) - 0.5 * (rym + rc) * double(cur.vm)) * g.idy);
-        const double rstar = rc + dts * finite0(dh);
-        // linearisation point p* = EOS(rho*): FP64 power law in FP64 storage,
-        // the perturbation form (FP32 relative accuracy on p* - p0) in the mixed mode
-        const double ps = eos_full_pressure<S>(pr, k, rstar, double(cur.th));
-        const double C = kGamma * ps * (1.0 / rstar);
-        const double wo = double(cur.w);
-        // face density above level k (zero at the lid)
-        const double rbk = k < NZ - 1 ? 0.5 * (rc + (pr.rho0(k + 1) + double(nxt.rp))) : 0.0;
-        const double e = -a1 * dts * idz * (rbk * wo - rb_prev * wo_prev);
-        rp_out[c] = S(rstar - r0);
-        const double pnk = double(cur.pn);
-        if (k >= 1)
-        {
-            // row r = k-1 (face k-1/2): unknowns w_{r-1}, w_r, w_{r+1}
-            const double rbr = rb_prev;
-            const double inv_rbr = 1.0 / rbr;
-            const double Bc = alpha * dts * idz;
-            const double Ak = Bc * inv_rbr;
-            const double Aa = (k - 1) > 0 ? -Ak * Bc * C_prev * rb_prev2 : 0.0;
-            const double U = (k - 1) < NZ - 2 ? -Ak * Bc * C * rbk : 0.0;
-            const double D = 1.0 + Ak * Bc * (C + C_prev) * rbr;
-            // PGF on alpha p* + (1 - alpha) p^n (p^n: substep-start pressure), base gradient removed
-            const double G = (alpha * ((ps - ps_prev) - (pr.p0(k) - pr.p0(k - 1))) + a1 * (pnk - pn_prev)) * idz;
-            const double R = wo_prev - dts * inv_rbr * G - Ak * (C * e - C_prev * e_prev);
-            const double piv = D - Aa * cp_prev;
-            if (!(piv > 0.0) || !isfinite(piv)) ok = false;
-            const double inv_piv = 1.0 / piv;
-            const double cpr = U * inv_piv;
-            const double rrr = (R - Aa * rr_prev) * inv_piv;
-            const std::size_t cr = c - sz;
-            tc[cr] = cpr;
-            tr[cr] = rrr;
-            cp_prev = cpr;
-            rr_prev = rrr;
-        }
-        rb_prev2 = rb_prev;
-        rb_prev = rbk;
-        C_prev = C;
-        ps_prev = ps;
-        pn_prev = pnk;
-        wo_prev = wo;
-        e_prev = e;
-        cur = nxt;
-        nxt = nxt2;
     }
-    // ---- downward sweep: back substitution, density update, weighted z mass flux.
-    // Face r's inputs (and cell r's, finalised one iteration later) are
-    // loaded one iteration ahead.
-    struct Down
-    {
-        double tr, tc;
-        S w, rp_lo, rstar, th, mfz;
-    };
+
+    // ---- down-sweep: iteration t handles face r = NZ-2-t (none at r = -1)
+    // and finalises cell r+1 (top face r+1, zero at the lid; bottom face r).
     auto load_down = [&](int r) {
-        Down d;
-        const std::size_t cr = g.at(i, j, r);
-        d.w = w[cr];
-        d.tr = tr[cr];
-        d.tc = tc[cr];
-        d.rp_lo = rp_in[cr];
-        d.rstar = rp_out[cr];
-        d.th = th[cr];
+        const std::size_t cr = c0 + std::size_t(r) * sz;
+        HeviDown<S> d;
+        d.w = w[cr]; d.tr = tr[cr]; d.tc = tc[cr]; d.rp = rp_in[cr]; d.rs = rp_out[cr]; d.th = th[cr];
         d.mfz = mfz[cr];
         return d;
     };
     double w_above = 0.0, wo_above = 0.0, rb_above = 0.0;  // face r+1 quantities (stored w)
     double x_above = 0.0;                                  // face r+1 Thomas solution (unclamped)
-    // cell NZ-1 (finalised in the first iteration) and face NZ-2
-    const std::size_t ctop = g.at(i, j, NZ - 1);
-    double rp_hi = double(rp_in[ctop]);
-    S rstar_c = rp_out[ctop], th_c = th[ctop], mfz_c = mfz[ctop];
-    Down dn = load_down(NZ - 2);
-#pragma unroll 2
-    for (int r = NZ - 2; r >= -1; --r)
+    HeviDown<S> cell = load_down(NZ - 1);                  // cell r+1 (its w, tr, tc are not used)
+    HeviDown<S> dring[kRing];
+#pragma unroll
+    for (int q = 0; q < kRing; ++q) dring[q] = load_down(max(NZ - 2 - q, 0));
+    for (int t0 = 0; t0 < NZ; t0 += kRing)
     {
-        const Down nx = load_down(max(r - 1, 0));
-        double wn = 0.0, wo = 0.0, rbr = 0.0;
-        if (r >= 0)
+#pragma unroll
+        for (int q = 0; q < kRing; ++q)
         {
-            const std::size_t cr = g.at(i, j, r);
-            wo = double(dn.w);
-            const double x = ok ? dn.tr - dn.tc * x_above : wo;
-            x_above = x;
-            wn = clamp_store<S>(x, 0.0, -bd.wind_v, bd.wind_v);
-            w[cr] = S(wn);
-            wn = double(S(wn));
-            rbr = 0.5 * ((pr.rho0(r) + double(dn.rp_lo)) + (pr.rho0(r + 1) + rp_hi));
+            const int r = NZ - 2 - (t0 + q);
+            const HeviDown<S> dn = dring[q];
+            if (r >= -1)
+            {
+                double wn = 0.0, wo = 0.0, rbr = 0.0;
+                if (r >= 0)
+                {
+                    const std::size_t cr = c0 + std::size_t(r) * sz;
+                    wo = double(dn.w);
+                    const double x = ok ? dn.tr - dn.tc * x_above : wo;
+                    x_above = x;
+                    wn = clamp_store<S>(x, 0.0, -bd.wind_v, bd.wind_v);
+                    w[cr] = S(wn);
+                    wn = double(S(wn));
+                    rbr = 0.5 * ((pr.rho0(r) + double(dn.rp)) + (pr.rho0(r + 1) + double(cell.rp)));
+                }
+                const int kc = r + 1;
+                const std::size_t cc = c0 + std::size_t(kc) * sz;
+                const double ft = kc < NZ - 1 ? rb_above * (alpha * w_above + a1 * wo_above) : 0.0;
+                const double fb = r >= 0 ? rbr * (alpha * wn + a1 * wo) : 0.0;
+                if (acc) mfz[cc] = S((first ? 0.0 : double(cell.mfz)) + weight * ft);
+                const double rstar = pr.rho0(kc) + double(cell.rs);
+                const RhoP<S> o = density_eos<S>(bd, pr, kc, rstar - dts * idz * (ft - fb), double(cell.th));
+                rp_out[cc] = o.r;
+                pp[cc] = o.p;
+                w_above = wn;
+                wo_above = wo;
+                rb_above = rbr;
+                cell = dn;
+            }
+            dring[q] = load_down(max(r - kRing, 0));
         }
-        // finalise cell r+1: top face r+1 (zero at the lid), bottom face r
-        const int kc = r + 1;
-        const std::size_t cc = g.at(i, j, kc);
-        const double ft = kc < NZ - 1 ? rb_above * (alpha * w_above + a1 * wo_above) : 0.0;
-        const double fb = r >= 0 ? rbr * (alpha * wn + a1 * wo) : 0.0;
-        if (acc) mfz[cc] = S((first ? 0.0 : double(mfz_c)) + weight * ft);
-        const double rstar = pr.rho0(kc) + double(rstar_c);
-        const RhoP<S> o = density_eos<S>(bd, pr, kc, rstar - dts * idz * (ft - fb), double(th_c));
-        rp_out[cc] = o.r;
-        pp[cc] = o.p;
-        w_above = wn;
-        wo_above = wo;
-        rb_above = rbr;
-        if (r >= 0)
-        {
-            rp_hi = double(dn.rp_lo);
-            rstar_c = dn.rstar;
-            th_c = dn.th;
-            mfz_c = dn.mfz;
-        }
-        dn = nx;
     }
 }
 
@@ -801,12 +901,16 @@ __global__ __launch_bounds__(kBlock) void k_momentum(const MemberArgs* __restric
     const auto w = P<S>(f.w);
     const std::size_t sy = std::size_t(g.px), sz = g.plane();
     const std::size_t c = g.at(i, j, k);
-    const RhoP<S> cc{rn[c], pp[c]};
-    const RhoP<S> xp = i < NX - 1 ? RhoP<S>{rn[c + 1], pp[c + 1]} : cc;
-    const RhoP<S> yp = j < NY - 1 ? RhoP<S>{rn[c + sy], pp[c + sy]} : cc;
-    const RhoP<S> zp = k < NZ - 1 ? RhoP<S>{rn[c + sz], pp[c + sz]} : cc;
-    momentum_update<S>(g, f.base, A.p.bounds, A.p.dts, i, j, k, cc, xp, yp, zp, double(u[c]), double(v[c]),
-                       double(w[c]), A.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI, u, v, w, c);
+    // neighbours clamped to the domain (the cell itself stands in for a missing one)
+    const std::size_t cxp = i < NX - 1 ? c + 1 : c, cyp = j < NY - 1 ? c + sy : c, czp = k < NZ - 1 ? c + sz : c;
+    RhoP<S> cc{rn[c], pp[c]}, xp{rn[cxp], pp[cxp]}, yp{rn[cyp], pp[cyp]}, zp{rn[czp], pp[czp]};
+    S uc = u[c], vc = v[c], wc = w[c];
+    const auto rho0 = slot(f.base, kBaseRho0, NZ);
+    double r0 = rho0[k], r0p = rho0[k < NZ - 1 ? k + 1 : k];
+    pin(cc.r); pin(cc.p); pin(xp.r); pin(xp.p); pin(yp.r); pin(yp.p); pin(zp.r); pin(zp.p);
+    pin(uc); pin(vc); pin(wc); pin(r0); pin(r0p);
+    momentum_update<S>(g, r0, r0p, A.p.bounds, A.p.dts, i, j, k, cc, xp, yp, zp, double(uc), double(vc), double(wc),
+                       A.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI, u, v, w, c);
 }
 
 // ---------------------------------------------------------------------
@@ -847,6 +951,75 @@ __device__ __forceinline__ bool bc_buffers(const MemberArgs& A, int s, BcBufs<S>
     return true;
 }
 
+// One lateral boundary pair of one row: the low face (ghost cell c, first
+// interior cell in) and the high face (interior face cell a, the one inside
+// it am, ghost cell gi) for the face-normal velocity n and the tangential
+// t1, t2. Every value is loaded before anything is stored, so the row costs
+// one round trip (periodic copies read a and in, which the other face does
+// not modify before, in the reference's order, they are read).
+template <class S>
+__device__ __forceinline__ void lateral_pair(gptr<S> n, gptr<S> t1, gptr<S> t2, gptr<S> rp, gptr<S> pp,
+                                             gptr<float> stash_lo, gptr<float> stash_hi, int bc_lo, int bc_hi,
+                                             bool radiate, float dt_rad, float inv_d, std::size_t c,
+                                             std::size_t in, std::size_t am, std::size_t a, std::size_t gi)
+{
+    const float cstar = 30.0f;
+    S n_in = n[in], t1_in = t1[in], t2_in = t2[in], rp_in = rp[in], pp_in = pp[in];
+    S n_a = n[a], t1_a = t1[a], t2_a = t2[a], rp_a = rp[a], pp_a = pp[a], n_am = n[am];
+    float s_lo = *stash_lo, s_hi = *stash_hi;
+    pin(n_in); pin(t1_in); pin(t2_in); pin(rp_in); pin(pp_in);
+    pin(n_a); pin(t1_a); pin(t2_a); pin(rp_a); pin(pp_a); pin(n_am); pin(s_lo); pin(s_hi);
+    // low face
+    if (bc_lo == NIMBUS_BC_PERIODIC)
+    {
+        n[c] = n_a; t1[c] = t1_a; t2[c] = t2_a; rp[c] = rp_a; pp[c] = pp_a;
+    }
+    else
+    {
+        if (bc_lo == NIMBUS_BC_RADIATIVE)
+        {
+            if (radiate)
+            {
+                const float nn = kw78_face(s_lo, s_lo - 0.0f - cstar, float(n_in) - s_lo, dt_rad, inv_d, true);
+                n[c] = S(nn);
+                *stash_lo = nn;
+            }
+        }
+        else n[c] = bc_lo == NIMBUS_BC_ZERO_GRADIENT ? n_in : S(0);
+        const bool rigid = bc_lo == NIMBUS_BC_RIGID;
+        t1[c] = rigid ? S(0) : t1_in;
+        t2[c] = rigid ? S(0) : t2_in;
+        rp[c] = rp_in;
+        pp[c] = pp_in;
+    }
+    // high face
+    if (bc_hi == NIMBUS_BC_PERIODIC)
+    {
+        n[gi] = n_in; t1[gi] = t1_in; t2[gi] = t2_in; rp[gi] = rp_in; pp[gi] = pp_in;
+    }
+    else
+    {
+        S na = n_a;
+        if (bc_hi == NIMBUS_BC_RIGID || bc_hi == NIMBUS_BC_RIGID_SLIP) na = S(0);
+        if (bc_hi == NIMBUS_BC_RADIATIVE && radiate)
+        {
+            const float nn = kw78_face(s_hi, s_hi - 0.0f + cstar, s_hi - float(n_am), dt_rad, inv_d, false);
+            na = S(nn);
+            *stash_hi = nn;
+        }
+        else if (bc_hi == NIMBUS_BC_ZERO_GRADIENT) na = n_am;
+        if (bc_hi == NIMBUS_BC_RIGID || bc_hi == NIMBUS_BC_RIGID_SLIP || (bc_hi == NIMBUS_BC_RADIATIVE && radiate) ||
+            bc_hi == NIMBUS_BC_ZERO_GRADIENT)
+            n[a] = na;
+        n[gi] = (bc_hi == NIMBUS_BC_RADIATIVE || bc_hi == NIMBUS_BC_ZERO_GRADIENT) ? na : S(0);
+        const bool rigid = bc_hi == NIMBUS_BC_RIGID;
+        t1[gi] = rigid ? S(0) : t1_a;
+        t2[gi] = rigid ? S(0) : t2_a;
+        rp[gi] = rp_a;
+        pp[gi] = pp_a;
+    }
+}
+
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_bc_west_east(const MemberArgs* __restrict__ ma,
                                                          const int2* __restrict__ map, int s)
@@ -860,62 +1033,11 @@ __global__ __launch_bounds__(kBlock) void k_bc_west_east(const MemberArgs* __res
     if (t >= g.ny * g.nz) return;
     const int k = t / g.ny, j = t - k * g.ny;
     const int NX = g.nx, NZ = g.nz;
-    const int bcw = A.p.bc_west, bce = A.p.bc_east;
-    const bool radiate = b.dt_rad > 0.0f;
-    const float cstar = 30.0f, inv_dx = 1.0f / float(g.dx);
-    const auto u = b.u, v = b.v, w = b.w, rp = b.rp, pp = b.pp;
     const std::size_t row = g.at(0, j, k);
-    auto cp_rp = [&](std::size_t d, std::size_t src) { rp[d] = rp[src]; pp[d] = pp[src]; };
-    {
-        const std::size_t c = row, in = row + 1;
-        if (bcw == NIMBUS_BC_PERIODIC)
-        {
-            const std::size_t src = row + (NX - 2);
-            u[c] = u[src]; v[c] = v[src]; w[c] = w[src]; cp_rp(c, src);
-        }
-        else
-        {
-            if (bcw == NIMBUS_BC_RADIATIVE)
-            {
-                if (radiate)
-                {
-                    auto& up = G(A.f.rad_w)[std::size_t(j) * NZ + k];
-                    const float un = kw78_face(up, up - 0.0f - cstar, float(u[in]) - up, b.dt_rad, inv_dx, true);
-                    u[c] = S(un);
-                    up = un;
-                }
-            }
-            else if (bcw == NIMBUS_BC_ZERO_GRADIENT) u[c] = u[in];
-            else u[c] = S(0);
-            if (bcw == NIMBUS_BC_RIGID) { v[c] = S(0); w[c] = S(0); }
-            else { v[c] = v[in]; w[c] = w[in]; }
-            cp_rp(c, in);
-        }
-    }
-    {
-        const std::size_t gi = row + (NX - 1), a = row + (NX - 2), am = row + (NX - 3);
-        if (bce == NIMBUS_BC_PERIODIC)
-        {
-            const std::size_t src = row + 1;
-            u[gi] = u[src]; v[gi] = v[src]; w[gi] = w[src]; cp_rp(gi, src);
-        }
-        else
-        {
-            if (bce == NIMBUS_BC_RIGID || bce == NIMBUS_BC_RIGID_SLIP) u[a] = S(0);
-            if (bce == NIMBUS_BC_RADIATIVE && radiate)
-            {
-                auto& up = G(A.f.rad_e)[std::size_t(j) * NZ + k];
-                const float un = kw78_face(up, up - 0.0f + cstar, up - float(u[am]), b.dt_rad, inv_dx, false);
-                u[a] = S(un);
-                up = un;
-            }
-            else if (bce == NIMBUS_BC_ZERO_GRADIENT) u[a] = u[am];
-            u[gi] = (bce == NIMBUS_BC_RADIATIVE || bce == NIMBUS_BC_ZERO_GRADIENT) ? u[a] : S(0);
-            if (bce == NIMBUS_BC_RIGID) { v[gi] = S(0); w[gi] = S(0); }
-            else { v[gi] = v[a]; w[gi] = w[a]; }
-            cp_rp(gi, a);
-        }
-    }
+    const std::size_t st = std::size_t(j) * NZ + k;
+    lateral_pair<S>(b.u, b.v, b.w, b.rp, b.pp, G(A.f.rad_w) + st, G(A.f.rad_e) + st, A.p.bc_west, A.p.bc_east,
+                    b.dt_rad > 0.0f, b.dt_rad, 1.0f / float(g.dx), row, row + 1, row + (NX - 3), row + (NX - 2),
+                    row + (NX - 1));
 }
 
 template <class S>
@@ -931,62 +1053,12 @@ __global__ __launch_bounds__(kBlock) void k_bc_south_north(const MemberArgs* __r
     if (t >= g.nx * g.nz) return;
     const int k = t / g.nx, i = t - k * g.nx;
     const int NY = g.ny, NZ = g.nz;
-    const int bcs = A.p.bc_south, bcn = A.p.bc_north;
-    const bool radiate = b.dt_rad > 0.0f;
-    const float cstar = 30.0f, inv_dy = 1.0f / float(g.dy);
-    const auto u = b.u, v = b.v, w = b.w, rp = b.rp, pp = b.pp;
     const std::size_t sy = std::size_t(g.px);
-    auto cp_rp = [&](std::size_t d, std::size_t src) { rp[d] = rp[src]; pp[d] = pp[src]; };
-    {
-        const std::size_t c = g.at(i, 0, k), in = c + sy;
-        if (bcs == NIMBUS_BC_PERIODIC)
-        {
-            const std::size_t src = g.at(i, NY - 2, k);
-            v[c] = v[src]; u[c] = u[src]; w[c] = w[src]; cp_rp(c, src);
-        }
-        else
-        {
-            if (bcs == NIMBUS_BC_RADIATIVE)
-            {
-                if (radiate)
-                {
-                    auto& vp = G(A.f.rad_s)[std::size_t(i) * NZ + k];
-                    const float vn = kw78_face(vp, vp - 0.0f - cstar, float(v[in]) - vp, b.dt_rad, inv_dy, true);
-                    v[c] = S(vn);
-                    vp = vn;
-                }
-            }
-            else if (bcs == NIMBUS_BC_ZERO_GRADIENT) v[c] = v[in];
-            else v[c] = S(0);
-            if (bcs == NIMBUS_BC_RIGID) { u[c] = S(0); w[c] = S(0); }
-            else { u[c] = u[in]; w[c] = w[in]; }
-            cp_rp(c, in);
-        }
-    }
-    {
-        const std::size_t gj = g.at(i, NY - 1, k), a = g.at(i, NY - 2, k), am = g.at(i, NY - 3, k);
-        if (bcn == NIMBUS_BC_PERIODIC)
-        {
-            const std::size_t src = g.at(i, 1, k);
-            v[gj] = v[src]; u[gj] = u[src]; w[gj] = w[src]; cp_rp(gj, src);
-        }
-        else
-        {
-            if (bcn == NIMBUS_BC_RIGID || bcn == NIMBUS_BC_RIGID_SLIP) v[a] = S(0);
-            if (bcn == NIMBUS_BC_RADIATIVE && radiate)
-            {
-                auto& vp = G(A.f.rad_n)[std::size_t(i) * NZ + k];
-                const float vn = kw78_face(vp, vp - 0.0f + cstar, vp - float(v[am]), b.dt_rad, inv_dy, false);
-                v[a] = S(vn);
-                vp = vn;
-            }
-            else if (bcn == NIMBUS_BC_ZERO_GRADIENT) v[a] = v[am];
-            v[gj] = (bcn == NIMBUS_BC_RADIATIVE || bcn == NIMBUS_BC_ZERO_GRADIENT) ? v[a] : S(0);
-            if (bcn == NIMBUS_BC_RIGID) { u[gj] = S(0); w[gj] = S(0); }
-            else { u[gj] = u[a]; w[gj] = w[a]; }
-            cp_rp(gj, a);
-        }
-    }
+    const std::size_t c = g.at(i, 0, k);
+    const std::size_t st = std::size_t(i) * NZ + k;
+    lateral_pair<S>(b.v, b.u, b.w, b.rp, b.pp, G(A.f.rad_s) + st, G(A.f.rad_n) + st, A.p.bc_south, A.p.bc_north,
+                    b.dt_rad > 0.0f, b.dt_rad, 1.0f / float(g.dy), c, c + sy, c + (NY - 3) * sy, c + (NY - 2) * sy,
+                    c + (NY - 1) * sy);
 }
 
 template <class S>
@@ -1001,15 +1073,16 @@ __global__ __launch_bounds__(kBlock) void k_bc_vertical(const MemberArgs* __rest
     const Column t = column_of(A, wk.local);
     if (t.i >= g.nx || t.j >= g.ny) return;
     const int NZ = g.nz;
+    const std::size_t top = g.at(t.i, t.j, NZ - 1), tm = g.at(t.i, t.j, NZ - 2);
+    const S wt = b.w[tm], ut = b.u[tm], vt = b.v[tm];  // loads before the stores
     if (A.p.bc_bottom == NIMBUS_BOTTOM_NO_SLIP)
     {
         b.u[g.at(t.i, t.j, 0)] = S(0);
         b.v[g.at(t.i, t.j, 0)] = S(0);
     }
-    const std::size_t top = g.at(t.i, t.j, NZ - 1), tm = g.at(t.i, t.j, NZ - 2);
-    b.w[top] = A.p.bc_top == NIMBUS_TOP_OPEN ? b.w[tm] : S(0);
-    b.u[top] = b.u[tm];
-    b.v[top] = b.v[tm];
+    b.w[top] = A.p.bc_top == NIMBUS_TOP_OPEN ? wt : S(0);
+    b.u[top] = ut;
+    b.v[top] = vt;
 }
 
 template <class S>
@@ -1106,22 +1179,38 @@ __global__ __launch_bounds__(kBlock) void k_transport_flux(const MemberArgs* __r
         yo[o] = std::ptrdiff_t(clampi(j + o - 2, 0, NY - 1) - j) * g.px;
         zo[o] = std::ptrdiff_t(clampi(k + o - 2, 0, NZ - 1) - k) * std::ptrdiff_t(g.plane());
     }
+    // Phase 1: the 16 stencil values of every scalar (offsets clamped to the
+    // domain, so every load is valid and none waits behind a branch).
+    A_ v[kMaxScalars][16];
 #pragma unroll
     for (int q = 0; q < kMaxScalars; ++q)
     {
         if (q >= n_scalars) break;
-        const gptr<const S> __restrict__ qs = scalar_in<S>(f, q);
-        const A_ q0 = A_(qs[c]);
+        const gptr<const S> qs = scalar_in<S>(f, q);
+        v[q][0] = A_(qs[c]);
+#pragma unroll
+        for (int o = 0; o < 5; ++o)
+        {
+            const int oo = o < 2 ? o : o + 1;  // stencil points -2, -1, +1, +2, +3
+            v[q][1 + o] = A_(qs[c + xo[oo]]);
+            v[q][6 + o] = A_(qs[c + yo[oo]]);
+            v[q][11 + o] = A_(qs[c + zo[oo]]);
+        }
+    }
+    // Phase 2: one WENO5 face per direction and scalar (-2 -1 0 +1 +2 +3 about the face).
+#pragma unroll
+    for (int q = 0; q < kMaxScalars; ++q)
+    {
+        if (q >= n_scalars) break;
+        const A_* x = &v[q][1];
+        const A_* y = &v[q][6];
+        const A_* z = &v[q][11];
+        const A_ q0 = v[q][0];
         S sx = S(0), sy_ = S(0), sz_ = S(0);
-        if (i <= NX - 2)
-            sx = S(dev::face_flux6<A_>(A_(qs[c + xo[0]]), A_(qs[c + xo[1]]), q0, A_(qs[c + xo[3]]), A_(qs[c + xo[4]]),
-                                       A_(qs[c + xo[5]]), i, NX, mx));
-        if (j <= NY - 2)
-            sy_ = S(dev::face_flux6<A_>(A_(qs[c + yo[0]]), A_(qs[c + yo[1]]), q0, A_(qs[c + yo[3]]),
-                                        A_(qs[c + yo[4]]), A_(qs[c + yo[5]]), j, NY, my));
+        if (i <= NX - 2) sx = S(dev::face_flux6<A_>(x[0], x[1], q0, x[2], x[3], x[4], i, NX, mx));
+        if (j <= NY - 2) sy_ = S(dev::face_flux6<A_>(y[0], y[1], q0, y[2], y[3], y[4], j, NY, my));
         if (k <= NZ - 2)
-            sz_ = S(dev::face_flux6<A_>(A_(qs[c + zo[0]]), A_(qs[c + zo[1]]), q0, A_(qs[c + zo[3]]),
-                                        A_(qs[c + zo[4]]), A_(qs[c + zo[5]]), k, NZ, mz));
+            sz_ = S(dev::face_flux6<A_>(z[0], z[1], q0, z[2], z[3], z[4], k, NZ, mz));
         else if (open_top)
             sz_ = S(mz * q0);
         P<S>(f.flux[3 * q])[c] = sx;
@@ -1155,21 +1244,39 @@ __global__ __launch_bounds__(kBlock) void k_transport_limiter(const MemberArgs* 
     const std::size_t c = g.at(i, j, k);
     const bool open_top = A.p.bc_top == NIMBUS_TOP_OPEN;
     const double rho_b = slot(f.base, kBaseRho0, NZ)[k] + double(P<S>(f.rb)[c]);
-    for (int q = 0; q < n_scalars; ++q)
+    // Every load is unconditional (a missing lower neighbour reads the cell
+    // itself and is masked), so the loads of a cell issue back to back and no
+    // branch on a loaded value sits between them.
+    const std::size_t cxm = i >= 1 ? c - 1 : c, cym = j >= 1 ? c - sy : c, czm = k >= 1 ? c - sz : c;
+    const bool hx = i <= NX - 2, hy = j <= NY - 2, hz = k <= NZ - 2 || (open_top && k == NZ - 1);
+    const bool lx = i >= 1, ly = j >= 1, lz = k >= 1;
+    S fl[kMaxScalars][6], qs[kMaxScalars];
+#pragma unroll
+    for (int q = 0; q < kMaxScalars; ++q)
     {
+        if (q >= n_scalars) break;
         if (!((pd_mask >> q) & 1)) continue;
-        const gptr<const S> __restrict__ fx = P<S>(f.flux[3 * q]);
-        const gptr<const S> __restrict__ fy = P<S>(f.flux[3 * q + 1]);
-        const gptr<const S> __restrict__ fz = P<S>(f.flux[3 * q + 2]);
+        const gptr<const S> fx = P<S>(f.flux[3 * q]);
+        const gptr<const S> fy = P<S>(f.flux[3 * q + 1]);
+        const gptr<const S> fz = P<S>(f.flux[3 * q + 2]);
+        fl[q][0] = fx[c]; fl[q][1] = fx[cxm]; fl[q][2] = fy[c]; fl[q][3] = fy[cym];
+        fl[q][4] = fz[c]; fl[q][5] = fz[czm];
+        qs[q] = scalar_in<S>(f, q)[c];
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxScalars; ++q)
+    {
+        if (q >= n_scalars) break;
+        if (!((pd_mask >> q) & 1)) continue;
         double out_rate = 0.0;
-        if (i <= NX - 2 && fx[c] > S(0)) out_rate += double(fx[c]) * g.idx;
-        if (i >= 1 && fx[c - 1] < S(0)) out_rate -= double(fx[c - 1]) * g.idx;
-        if (j <= NY - 2 && fy[c] > S(0)) out_rate += double(fy[c]) * g.idy;
-        if (j >= 1 && fy[c - sy] < S(0)) out_rate -= double(fy[c - sy]) * g.idy;
-        if ((k <= NZ - 2 || (open_top && k == NZ - 1)) && fz[c] > S(0)) out_rate += double(fz[c]) * g.idz;
-        if (k >= 1 && fz[c - sz] < S(0)) out_rate -= double(fz[c - sz]) * g.idz;
+        out_rate += hx && fl[q][0] > S(0) ? double(fl[q][0]) * g.idx : 0.0;
+        out_rate -= lx && fl[q][1] < S(0) ? double(fl[q][1]) * g.idx : 0.0;
+        out_rate += hy && fl[q][2] > S(0) ? double(fl[q][2]) * g.idy : 0.0;
+        out_rate -= ly && fl[q][3] < S(0) ? double(fl[q][3]) * g.idy : 0.0;
+        out_rate += hz && fl[q][4] > S(0) ? double(fl[q][4]) * g.idz : 0.0;
+        out_rate -= lz && fl[q][5] < S(0) ? double(fl[q][5]) * g.idz : 0.0;
         double r = 1.0;
-        if (out_rate > 0.0) r = fmin(1.0, rho_b * fmax(double(scalar_in<S>(f, q)[c]), 0.0) / A.p.dt / out_rate);
+        if (out_rate > 0.0) r = fmin(1.0, rho_b * fmax(double(qs[q]), 0.0) / A.p.dt / out_rate);
         limiter_slot(f, pd_mask, q)[c] = pd_store(r);
     }
 }
@@ -1193,7 +1300,9 @@ __global__ __launch_bounds__(kBlock) void k_transport_update(const MemberArgs* _
     const std::size_t c = g.at(i, j, k);
     if (!(i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1))
     {
-        for (int q = 0; q < n_scalars; ++q) scalar_out<S>(f, q)[c] = scalar_in<S>(f, q)[c];
+#pragma unroll
+        for (int q = 0; q < kMaxScalars; ++q)
+            if (q < n_scalars) scalar_out<S>(f, q)[c] = scalar_in<S>(f, q)[c];
         return;
     }
     const bool open_top = A.p.bc_top == NIMBUS_TOP_OPEN;
@@ -1203,35 +1312,64 @@ __global__ __launch_bounds__(kBlock) void k_transport_update(const MemberArgs* _
     const double rho_e = r0 + double(P<S>(f.rp)[c]);
     const double den = rho_e > 0.05 ? rho_e : 0.05;
     const bool has_top = k <= NZ - 2 || (open_top && k == NZ - 1);
+    const std::size_t czp = k < NZ - 1 ? c + sz : c, czm = k >= 1 ? c - sz : c;
     const double ix = g.idx, iy = g.idy, iz = g.idz;
-    for (int q = 0; q < n_scalars; ++q)
+    // Phase 1: every load of every scalar, unconditionally (interior column:
+    // c +- 1 and c +- sy exist; vertical neighbours clamped; a scalar without
+    // the limiter reads some limiter field and ignores it). Doing them before
+    // any store lets them all be in flight at once.
+    S fl[kMaxScalars][6], qs[kMaxScalars];
+    float rr[kMaxScalars][7];
+#pragma unroll
+    for (int q = 0; q < kMaxScalars; ++q)
     {
-        const gptr<const S> __restrict__ fx = P<S>(f.flux[3 * q]);
-        const gptr<const S> __restrict__ fy = P<S>(f.flux[3 * q + 1]);
-        const gptr<const S> __restrict__ fz = P<S>(f.flux[3 * q + 2]);
+        if (q >= n_scalars) break;
+        const gptr<const S> fx = P<S>(f.flux[3 * q]);
+        const gptr<const S> fy = P<S>(f.flux[3 * q + 1]);
+        const gptr<const S> fz = P<S>(f.flux[3 * q + 2]);
+        fl[q][0] = fx[c]; fl[q][1] = fx[c - 1]; fl[q][2] = fy[c]; fl[q][3] = fy[c - sy];
+        fl[q][4] = fz[c]; fl[q][5] = fz[czm];
+        qs[q] = scalar_in<S>(f, q)[c];
+        const gptr<const float> rl = limiter_slot(f, pd_mask, q);
+        rr[q][0] = rl[c]; rr[q][1] = rl[c + 1]; rr[q][2] = rl[c - 1]; rr[q][3] = rl[c + sy];
+        rr[q][4] = rl[c - sy]; rr[q][5] = rl[czp]; rr[q][6] = rl[czm];
+    }
+    // Phase 2: limited face fluxes and the conservative update.
+#pragma unroll
+    for (int q = 0; q < kMaxScalars; ++q)
+    {
+        if (q >= n_scalars) break;
         const bool pd = (pd_mask >> q) & 1;
-        const gptr<const float> rl = pd ? limiter_slot(f, pd_mask, q) : nullptr;
-        auto lim = [&](S raw, std::size_t lo, std::size_t hi) -> double {
-            if (!pd || raw == S(0)) return double(raw);
-            return double(raw) * double(S(raw > S(0) ? rl[lo] : rl[hi]));
+        // a face flux scaled by its donor cell's ratio (lo: the cell below/left
+        // of the face, hi: the one above/right)
+        auto lim = [&](S raw, float r_lo, float r_hi) -> double {
+            const float r = raw > S(0) ? r_lo : r_hi;
+            return pd && raw != S(0) ? double(raw) * double(S(r)) : double(raw);
         };
-        const double sxp = lim(fx[c], c, c + 1);
-        const double sxm = lim(fx[c - 1], c - 1, c);
-        const double syp = lim(fy[c], c, c + sy);
-        const double sym = lim(fy[c - sy], c - sy, c);
-        const double szp = has_top ? lim(fz[c], c, k < NZ - 1 ? c + sz : c) : 0.0;
-        const double szm = k >= 1 ? lim(fz[c - sz], c - sz, c) : 0.0;
-        const double q0 = double(scalar_in<S>(f, q)[c]);
+        const double sxp = lim(fl[q][0], rr[q][0], rr[q][1]);
+        const double sxm = lim(fl[q][1], rr[q][2], rr[q][0]);
+        const double syp = lim(fl[q][2], rr[q][0], rr[q][3]);
+        const double sym = lim(fl[q][3], rr[q][4], rr[q][0]);
+        const double szp = has_top ? lim(fl[q][4], rr[q][0], rr[q][5]) : 0.0;
+        const double szm = k >= 1 ? lim(fl[q][5], rr[q][6], rr[q][0]) : 0.0;
+        const double q0 = double(qs[q]);
         double rho_phi;
         if (pd)
         {
+            // outgoing, then incoming (adding a signed zero leaves a sum unchanged)
             double outg = 0.0, inc = 0.0;
-            if (sxp >= 0.0) outg += sxp * ix; else inc -= sxp * ix;
-            if (sxm <= 0.0) outg -= sxm * ix; else inc += sxm * ix;
-            if (syp >= 0.0) outg += syp * iy; else inc -= syp * iy;
-            if (sym <= 0.0) outg -= sym * iy; else inc += sym * iy;
-            if (has_top) { if (szp >= 0.0) outg += szp * iz; else inc -= szp * iz; }
-            if (k >= 1) { if (szm <= 0.0) outg -= szm * iz; else inc += szm * iz; }
+            outg += sxp >= 0.0 ? sxp * ix : 0.0;
+            inc -= sxp >= 0.0 ? 0.0 : sxp * ix;
+            outg -= sxm <= 0.0 ? sxm * ix : 0.0;
+            inc += sxm <= 0.0 ? 0.0 : sxm * ix;
+            outg += syp >= 0.0 ? syp * iy : 0.0;
+            inc -= syp >= 0.0 ? 0.0 : syp * iy;
+            outg -= sym <= 0.0 ? sym * iy : 0.0;
+            inc += sym <= 0.0 ? 0.0 : sym * iy;
+            outg += has_top && szp >= 0.0 ? szp * iz : 0.0;
+            inc -= has_top && szp < 0.0 ? szp * iz : 0.0;
+            outg -= k >= 1 && szm <= 0.0 ? szm * iz : 0.0;
+            inc += k >= 1 && szm > 0.0 ? szm * iz : 0.0;
             rho_phi = (rho_b * q0 - dt * outg) + dt * inc;
         }
         else
@@ -1397,14 +1535,18 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     const double h = A.p.dt / A.p.kessler_sub;
     const bool do_eos = n == A.p.kessler_sub - 1;
     const std::size_t c = g.at(i, j, k), sz = g.plane();
-    const double rho = rho0[k] + double(rp[c]);
+    // all loads first (the level above clamped at the lid, unused there)
+    const bool up = k + 1 < NZ;
+    const std::size_t cu = up ? c + sz : c;
     S thn = th_in[c], qvn = qv_in[c], qcn = qc_in[c], qrn = qr_in[c];
+    S rpc = rp[c], ppc = pp[c], rpu = rp[cu], qru = qr_in[cu];
+    const double r0 = rho0[k], r0u = rho0[up ? k + 1 : k], p0k = p0[k];
+    pin(thn); pin(qvn); pin(qcn); pin(qrn); pin(rpc); pin(ppc); pin(rpu); pin(qru);
+    const double rho = r0 + double(rpc);
     if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1)
     {
-        const bool up = k + 1 < NZ;
-        const KesslerOut<S> o =
-            kessler_cell<S>(g.dz, kc, h, p0[k] + double(pp[c]), rho, up ? rho0[k + 1] + double(rp[c + sz]) : 0.0,
-                            thn, qvn, qcn, qrn, up ? qr_in[c + sz] : S(0), up);
+        const KesslerOut<S> o = kessler_cell<S>(g.dz, kc, h, p0k + double(ppc), rho, up ? r0u + double(rpu) : 0.0,
+                                                thn, qvn, qcn, qrn, up ? qru : S(0), up);
         if (k == 0)
         {
             const double q0 = double(qrn);
@@ -1426,7 +1568,7 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     if (do_eos && rho > 0.0 && thv > 0.0)
     {
         const double pd = dev::eos_pressure(rho, thv);
-        if (isfinite(pd) && pd > 0.0) pp[c] = S(clamp_store<S>(pd, 101325.0, bd.p_min, bd.p_max) - p0[k]);
+        if (isfinite(pd) && pd > 0.0) pp[c] = S(clamp_store<S>(pd, 101325.0, bd.p_min, bd.p_max) - p0k);
     }
 }
 
@@ -1720,6 +1862,8 @@ template <class S>
 void launch_acoustic_substep(Launcher& L, const BatchView& b, int s, bool accumulate, bool init_at_first)
 {
     launch_map(L, "acoustic_forward", k_acoustic_forward<S>, b, S_EXPLICIT, W_CELL, s, int(accumulate),
+               int(init_at_first));
+    launch_map(L, "acoustic_hevi_explicit", k_hevi_explicit<S>, b, S_HEVI, W_CELL, s, int(accumulate),
                int(init_at_first));
     launch_map_lds(L, "acoustic_hevi_column", k_hevi_column<S>, b, S_HEVI, W_COLUMN, profile_lds_bytes(b), s,
                    int(accumulate), int(init_at_first));
