@@ -57,6 +57,9 @@ enum
 };
 enum { NIMBUS_BOTTOM_FREE_SLIP = 0, NIMBUS_BOTTOM_NO_SLIP = 1 };
 enum { NIMBUS_TOP_RIGID = 0, NIMBUS_TOP_OPEN = 1 };
+/* Smagorinsky modes and filter widths (turbulence/runtime.cpp:49-72). */
+enum { NIMBUS_SGS_3D = 0, NIMBUS_SGS_HORIZONTAL = 1, NIMBUS_SGS_VERTICAL = 2 };
+enum { NIMBUS_SGS_FILTER_CUBIC_ROOT = 0, NIMBUS_SGS_FILTER_DX = 1 };
 
 typedef struct nimbus_hip_ctx nimbus_hip_ctx;
 
@@ -87,6 +90,19 @@ typedef struct
     double sponge_w_alpha;        /* 1/s (0.2) */
     int sponge_lateral_cells;     /* lateral band width (cells; 10 km in the presets) */
     double sponge_lateral_alpha;  /* 1/s (0.05) */
+    /* Smagorinsky SGS after the microphysics and before the EOS projection
+     * (turbulence/schemes/smagorinsky/smagorinsky.cpp:495-580,
+     * turbulence/base/eddy_viscosity.cpp:1253-2090, core/orchestration/dynamics/
+     * dynamics.cpp:7160-7262, sgs_apply.cpp:610-790): closure and conservative
+     * momentum/theta/qv tendencies recomputed every SGS substep of sgs_dt (the
+     * remainder last). sgs = 0 disables it. */
+    int sgs;
+    int sgs_mode;                 /* NIMBUS_SGS_3D, NIMBUS_SGS_HORIZONTAL ("2d" in the presets), NIMBUS_SGS_VERTICAL */
+    int sgs_filter;               /* NIMBUS_SGS_FILTER_CUBIC_ROOT (default), NIMBUS_SGS_FILTER_DX */
+    double sgs_cs;                /* 0.18 */
+    double sgs_pr_t, sgs_sc_t;    /* 1/3, 1/3 */
+    double sgs_nu_max, sgs_k_max; /* caps, 1000 m^2/s */
+    double sgs_dt;                /* longest SGS substep (1 s) */
 } nimbus_step_config;
 
 /* FP64 base-state profiles at cell centres, each nz long

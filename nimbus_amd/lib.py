@@ -22,6 +22,8 @@ ACOUSTIC_HEVI = 1
 BC_PERIODIC, BC_RADIATIVE, BC_ZERO_GRADIENT, BC_RIGID, BC_RIGID_SLIP = 0, 1, 2, 3, 4
 BOTTOM_FREE_SLIP, BOTTOM_NO_SLIP = 0, 1
 TOP_RIGID, TOP_OPEN = 0, 1
+SGS_3D, SGS_HORIZONTAL, SGS_VERTICAL = 0, 1, 2
+SGS_FILTER_CUBIC_ROOT, SGS_FILTER_DX = 0, 1
 
 FIELDS = ("u", "v", "w", "rho", "p", "theta", "qv", "qc", "qr")
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -46,7 +48,10 @@ class StepConfig(ctypes.Structure):
                 ("evap_eff", ctypes.c_double), ("vt_max", ctypes.c_double),
                 ("sponge", ctypes.c_int), ("sponge_z_d", ctypes.c_double), ("sponge_alpha", ctypes.c_double),
                 ("sponge_w_cells", ctypes.c_int), ("sponge_w_alpha", ctypes.c_double),
-                ("sponge_lateral_cells", ctypes.c_int), ("sponge_lateral_alpha", ctypes.c_double)]
+                ("sponge_lateral_cells", ctypes.c_int), ("sponge_lateral_alpha", ctypes.c_double),
+                ("sgs", ctypes.c_int), ("sgs_mode", ctypes.c_int), ("sgs_filter", ctypes.c_int),
+                ("sgs_cs", ctypes.c_double), ("sgs_pr_t", ctypes.c_double), ("sgs_sc_t", ctypes.c_double),
+                ("sgs_nu_max", ctypes.c_double), ("sgs_k_max", ctypes.c_double), ("sgs_dt", ctypes.c_double)]
 
 
 class BaseState(ctypes.Structure):
@@ -225,17 +230,22 @@ def make_grid(nx, ny, nz, dx, dy, dz):
 
 def step_config(dt, substeps, acoustic_mode=ACOUSTIC_EXPLICIT, hevi_beta=0.1, coriolis_f=0.0,
                 lateral=BC_RADIATIVE, bottom=BOTTOM_FREE_SLIP, top=TOP_RIGID,
-                qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None):
+                qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None, sgs=None):
     """Large-step configuration. `sponge` is None (off) or a dict of the
-    Rayleigh sponge settings (see sponge_preset)."""
+    Rayleigh sponge settings (see sponge_preset); `sgs` is None (off) or a
+    dict of Smagorinsky settings (see sgs_preset)."""
     lat = lateral if isinstance(lateral, (tuple, list)) else (lateral,) * 4
     sp = sponge or {}
+    sg = sgs or {}
     return StepConfig(float(dt), int(substeps), int(acoustic_mode), float(hevi_beta), float(coriolis_f),
                       lat[0], lat[1], lat[2], lat[3], int(bottom), int(top),
                       qc0, c_auto, c_accr, evap_eff, vt_max,
                       int(bool(sponge)), float(sp.get("z_d", 12000.0)), float(sp.get("alpha", 0.05)),
                       int(sp.get("w_cells", 8)), float(sp.get("w_alpha", 0.2)),
-                      int(sp.get("lateral_cells", 0)), float(sp.get("lateral_alpha", 0.05)))
+                      int(sp.get("lateral_cells", 0)), float(sp.get("lateral_alpha", 0.05)),
+                      int(bool(sgs)), int(sg.get("mode", SGS_3D)), int(sg.get("filter", SGS_FILTER_CUBIC_ROOT)),
+                      float(sg.get("cs", 0.18)), float(sg.get("pr_t", 1.0 / 3.0)), float(sg.get("sc_t", 1.0 / 3.0)),
+                      float(sg.get("nu_max", 1000.0)), float(sg.get("k_max", 1000.0)), float(sg.get("dt", 1.0)))
 
 
 def sponge_preset(dx):
@@ -244,6 +254,14 @@ def sponge_preset(dx):
     the lid, and a 10 km lateral band at 0.05 1/s."""
     return {"z_d": 12000.0, "alpha": 0.05, "w_cells": 8, "w_alpha": 0.2,
             "lateral_cells": int(round(10000.0 / dx)), "lateral_alpha": 0.05}
+
+
+def sgs_preset():
+    """The reference Kessler presets' turbulence block (configs/*meterResolution/kessler/
+    baseline.yaml: turbulence): Smagorinsky, mode 2d (horizontal only), Cs 0.18,
+    Pr_t = Sc_t = 1/3, cube-root filter width, caps 1000 m^2/s, 1 s SGS substeps."""
+    return {"mode": SGS_HORIZONTAL, "filter": SGS_FILTER_CUBIC_ROOT, "cs": 0.18, "pr_t": 1.0 / 3.0,
+            "sc_t": 1.0 / 3.0, "nu_max": 1000.0, "k_max": 1000.0, "dt": 1.0}
 
 
 def base_state(grid, sounding=None):

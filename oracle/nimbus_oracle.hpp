@@ -78,6 +78,18 @@ struct Base
     std::vector<double> rho0, p0, theta0, qv0, u0, v0;
 };
 
+enum SgsMode
+{
+    kSgs3d = 0,
+    kSgsHorizontal = 1,  // "2d"
+    kSgsVertical = 2
+};
+enum SgsFilter
+{
+    kFilterCubicRoot = 0,  // (dx dy dz)^(1/3)
+    kFilterDx = 1          // sqrt(dx dy)
+};
+
 struct Config
 {
     double dt = 1.0;
@@ -96,6 +108,13 @@ struct Config
     double sponge_w_alpha = 0.2;
     int sponge_lateral_cells = 0;
     double sponge_lateral_alpha = 0.05;
+    // Smagorinsky SGS (include/turbulence/turbulence_base.hpp:50-70; presets'
+    // turbulence block: smagorinsky, mode 2d = horizontal_only, Cs 0.18, Pr_t = Sc_t = 1/3)
+    int sgs = 0;
+    int sgs_mode = kSgs3d;
+    int sgs_filter = kFilterCubicRoot;
+    double sgs_cs = 0.18, sgs_pr_t = 1.0 / 3.0, sgs_sc_t = 1.0 / 3.0;
+    double sgs_nu_max = 1000.0, sgs_k_max = 1000.0, sgs_dt = 1.0;
 };
 
 template <class S>
@@ -1315,6 +1334,348 @@ void apply_sponges(const Grid& g, const Base& b, const Config& cfg, State<S>& s,
         }
 }
 
+// ----------------------------------------------------------------------
+// Smagorinsky SGS on the C-grid, flat terrain, stability correction "none"
+// (src/turbulence/schemes/smagorinsky/smagorinsky.cpp:495-580,
+// src/turbulence/base/eddy_viscosity.cpp:1253-1527 closure, 1532-1948
+// momentum stress divergence, 1953-2090 scalar flux divergence;
+// src/core/orchestration/dynamics/dynamics.cpp:7160-7262 split application,
+// sgs_apply.cpp:610-790 commit). Each SGS substep (min(remaining, dt_sgs))
+// recomputes nu_t = min(Cs^2 Delta^2 |S|, nu_max) at every cell centre from the
+// staggered strain (squares of the edge strains averaged to the centre),
+// K_theta = min(nu_t / Pr_t, K_max), K_q = min(nu_t / Sc_t, K_max), then the
+// conservative live-density tendencies: momentum from the stress divergence
+// (1/rho_face) d_j(2 rho nu S_ij) with edge viscosities, zero shear stress at
+// the ground and lid; theta and qv from (1/rho) d_j(rho K d_j phi) with
+// zero-flux ground/lid and closed lateral walls; and commits
+// x += dt_sub * dx/dt with the storage clamps. The coefficient and tendency
+// fields are stored in the storage type, as the reference's FP32 fields.
+// ----------------------------------------------------------------------
+struct SgsNeighbours
+{
+    int nx, ny;
+    bool wx, wy;
+    int ip(int i) const { return wx ? (i <= 1 ? nx - 2 : std::min(i - 1, nx - 2)) : std::max(i - 1, 0); }
+    int in(int i) const { return wx ? (i >= nx - 2 ? 1 : std::max(i + 1, 1)) : std::min(i + 1, nx - 1); }
+    int jp(int j) const { return wy ? (j <= 1 ? ny - 2 : std::min(j - 1, ny - 2)) : std::max(j - 1, 0); }
+    int jn(int j) const { return wy ? (j >= ny - 2 ? 1 : std::max(j + 1, 1)) : std::min(j + 1, ny - 1); }
+};
+
+inline SgsNeighbours sgs_neighbours(const Grid& g, const Config& cfg)
+{
+    return SgsNeighbours{g.nx, g.ny, cfg.bc_west == kPeriodic && cfg.bc_east == kPeriodic,
+                         cfg.bc_south == kPeriodic && cfg.bc_north == kPeriodic};
+}
+
+inline bool open_lateral(int bc) { return bc == kRadiative || bc == kZeroGradient; }
+
+// |S|^2 at the centre of cell (i, j, k) (eddy_viscosity.cpp:1253-1400).
+template <class S>
+double sgs_strain_sq(const Grid& g, const SgsNeighbours& nb, int mode, const State<S>& s, int i, int j, int k)
+{
+    const int NZ = g.nz;
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    auto U = [&](int a, int b, int c) { return double(s.u[g.at(a, b, c)]); };
+    auto V = [&](int a, int b, int c) { return double(s.v[g.at(a, b, c)]); };
+    auto W = [&](int a, int b, int c) { return c < 0 ? 0.0 : double(s.w[g.at(a, b, c)]); };
+    const int im = nb.ip(i), jm = nb.jp(j);
+    const double S11 = (U(i, j, k) - U(im, j, k)) / dx;
+    const double S22 = (V(i, j, k) - V(i, jm, k)) / dy;
+    const double S33 = (W(i, j, k) - W(i, j, k - 1)) / dz;
+    auto s12 = [&](int ii, int jj) {
+        const int jj1 = nb.jn(jj), ii1 = nb.in(ii);
+        return 0.5 * ((U(ii, jj1, k) - U(ii, jj, k)) / dy + (V(ii1, jj, k) - V(ii, jj, k)) / dx);
+    };
+    struct G2 { double a, b; };  // (d(u|v)/dz, dw/d(x|y)) at an edge
+    auto xz = [&](int ii, int kk) {
+        if (kk < 0 || kk >= NZ - 1) return G2{0.0, 0.0};
+        const int ii1 = nb.in(ii);
+        return G2{(U(ii, j, kk + 1) - U(ii, j, kk)) / dz, (W(ii1, j, kk) - W(ii, j, kk)) / dx};
+    };
+    auto yz = [&](int jj, int kk) {
+        if (kk < 0 || kk >= NZ - 1) return G2{0.0, 0.0};
+        const int jj1 = nb.jn(jj);
+        return G2{(V(i, jj, kk + 1) - V(i, jj, kk)) / dz, (W(i, jj1, kk) - W(i, jj, kk)) / dy};
+    };
+    const double a = s12(im, jm), b = s12(im, j), c = s12(i, jm), d = s12(i, j);
+    const double S12_sq = 0.25 * (a * a + b * b + c * c + d * d);
+    const G2 xa = xz(im, k - 1), xb = xz(im, k), xc = xz(i, k - 1), xd = xz(i, k);
+    const G2 ya = yz(jm, k - 1), yb = yz(jm, k), yc = yz(j, k - 1), yd = yz(j, k);
+    auto avg_b = [](G2 p, G2 q, G2 r, G2 t) { return 0.25 * (p.b * p.b + q.b * q.b + r.b * r.b + t.b * t.b); };
+    auto avg_a = [](G2 p, G2 q, G2 r, G2 t) { return 0.25 * (p.a * p.a + q.a * q.a + r.a * r.a + t.a * t.a); };
+    if (mode == kSgsHorizontal)
+        return 2.0 * (S11 * S11 + S22 * S22) + 4.0 * S12_sq + avg_b(xa, xb, xc, xd) + avg_b(ya, yb, yc, yd);
+    if (mode == kSgsVertical) return avg_a(xa, xb, xc, xd) + avg_a(ya, yb, yc, yd) + 2.0 * S33 * S33;
+    auto sym = [](G2 p) { const double e = 0.5 * (p.a + p.b); return e * e; };
+    const double S13_sq = 0.25 * (sym(xa) + sym(xb) + sym(xc) + sym(xd));
+    const double S23_sq = 0.25 * (sym(ya) + sym(yb) + sym(yc) + sym(yd));
+    return 2.0 * (S11 * S11 + S22 * S22 + S33 * S33) + 4.0 * (S12_sq + S13_sq + S23_sq);
+}
+
+inline double sgs_filter_width(const Grid& g, const Config& cfg)
+{
+    return cfg.sgs_filter == kFilterDx ? std::sqrt(g.dx * g.dy) : std::cbrt(g.dx * g.dy * g.dz);
+}
+
+// nu_t, K_theta, K_q at every cell (smagorinsky.cpp:495-580).
+template <class S>
+void sgs_closure(const Grid& g, const Config& cfg, const State<S>& s, std::vector<S>& nu, std::vector<S>& kth,
+                 std::vector<S>& kq)
+{
+    const SgsNeighbours nb = sgs_neighbours(g, cfg);
+    const double delta = sgs_filter_width(g, cfg);
+    const double c2d2 = cfg.sgs_cs * cfg.sgs_cs * delta * delta;
+    nu.assign(g.cells(), S(0));
+    kth.assign(g.cells(), S(0));
+    kq.assign(g.cells(), S(0));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                const double n = std::min(c2d2 * std::sqrt(sgs_strain_sq(g, nb, cfg.sgs_mode, s, i, j, k)),
+                                          cfg.sgs_nu_max);
+                nu[c] = S(n);
+                kth[c] = S(std::min(n / cfg.sgs_pr_t, cfg.sgs_k_max));
+                kq[c] = S(std::min(n / cfg.sgs_sc_t, cfg.sgs_k_max));
+            }
+}
+
+// du/dt, dv/dt, dw/dt (eddy_viscosity.cpp:1532-1948); zero outside the face ranges.
+template <class S>
+void sgs_momentum_tendencies(const Grid& g, const Config& cfg, const State<S>& s, const std::vector<S>& nu,
+                             std::vector<S>& du, std::vector<S>& dv, std::vector<S>& dw)
+{
+    const SgsNeighbours nb = sgs_neighbours(g, cfg);
+    const bool use_h = cfg.sgs_mode != kSgsVertical, use_v = cfg.sgs_mode != kSgsHorizontal;
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    auto U = [&](int a, int b, int c) { return double(s.u[g.at(a, b, c)]); };
+    auto V = [&](int a, int b, int c) { return double(s.v[g.at(a, b, c)]); };
+    auto W = [&](int a, int b, int c) { return c < 0 ? 0.0 : double(s.w[g.at(a, b, c)]); };
+    auto R = [&](int a, int b, int c) { return double(s.rho[g.at(a, b, c)]); };
+    auto MU = [&](int a, int b, int c) { return R(a, b, c) * double(nu[g.at(a, b, c)]); };
+    auto mu_xy = [&](int a, int b, int c) {
+        const int a1 = nb.in(a), b1 = nb.jn(b);
+        return 0.25 * (MU(a, b, c) + MU(a1, b, c) + MU(a, b1, c) + MU(a1, b1, c));
+    };
+    // uniform dz: the thickness-weighted average of the two levels is their mean
+    auto mu_xz = [&](int a, int b, int c) {
+        const int a1 = nb.in(a);
+        const double lower = 0.5 * (MU(a, b, c) + MU(a1, b, c)), upper = 0.5 * (MU(a, b, c + 1) + MU(a1, b, c + 1));
+        return (dz * lower + dz * upper) / (dz + dz);
+    };
+    auto mu_yz = [&](int a, int b, int c) {
+        const int b1 = nb.jn(b);
+        const double lower = 0.5 * (MU(a, b, c) + MU(a, b1, c)), upper = 0.5 * (MU(a, b, c + 1) + MU(a, b1, c + 1));
+        return (dz * lower + dz * upper) / (dz + dz);
+    };
+    du.assign(g.cells(), S(0));
+    dv.assign(g.cells(), S(0));
+    dw.assign(g.cells(), S(0));
+    const int i_hi = nb.wx ? NX - 2 : NX - 3;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= i_hi; ++i)
+        for (int j = 1; j <= NY - 2; ++j)
+        {
+            const int jm = nb.jp(j), im = nb.ip(i), ip = nb.in(i);
+            double lower_T13 = 0.0;
+            for (int k = 0; k < NZ - 1; ++k)
+            {
+                double div = 0.0;
+                if (use_h)
+                {
+                    const double S11_r = (U(ip, j, k) - U(i, j, k)) / dx, S11_l = (U(i, j, k) - U(im, j, k)) / dx;
+                    div += (2.0 * MU(ip, j, k) * S11_r - 2.0 * MU(i, j, k) * S11_l) / dx;
+                    auto T12 = [&](int je) {
+                        const int i1 = nb.in(i), je1 = nb.jn(je);
+                        const double S12 = 0.5 * ((U(i, je1, k) - U(i, je, k)) / dy + (V(i1, je, k) - V(i, je, k)) / dx);
+                        return 2.0 * mu_xy(i, je, k) * S12;
+                    };
+                    div += (T12(j) - T12(jm)) / dy;
+                }
+                if (use_v)
+                {
+                    auto T13 = [&](int ke) {
+                        if (ke < 0 || ke >= NZ - 1) return 0.0;
+                        const double dudz = (U(i, j, ke + 1) - U(i, j, ke)) / dz;
+                        const double sg = use_h ? dudz + (W(ip, j, ke) - W(i, j, ke)) / dx : dudz;
+                        return mu_xz(i, j, ke) * sg;
+                    };
+                    const double upper = T13(k);
+                    div += (upper - lower_T13) / dz;
+                    lower_T13 = upper;
+                }
+                du[g.at(i, j, k)] = S(div / (0.5 * (R(i, j, k) + R(nb.in(i), j, k))));
+            }
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= NX - 2; ++i)
+        for (int j = 1; j <= NY - 2; ++j)
+        {
+            if (!nb.wy && j >= NY - 2) continue;
+            const int jm = nb.jp(j), jp = nb.jn(j);
+            double lower_T23 = 0.0;
+            for (int k = 0; k < NZ - 1; ++k)
+            {
+                double div = 0.0;
+                if (use_h)
+                {
+                    const double S22_n = (V(i, jp, k) - V(i, j, k)) / dy, S22_s = (V(i, j, k) - V(i, jm, k)) / dy;
+                    div += (2.0 * MU(i, jp, k) * S22_n - 2.0 * MU(i, j, k) * S22_s) / dy;
+                    auto T21 = [&](int ie) {
+                        const int ie1 = nb.in(ie), j1 = nb.jn(j);
+                        const double S12 = 0.5 * ((U(ie, j1, k) - U(ie, j, k)) / dy + (V(ie1, j, k) - V(ie, j, k)) / dx);
+                        return 2.0 * mu_xy(ie, j, k) * S12;
+                    };
+                    div += (T21(i) - T21(nb.ip(i))) / dx;
+                }
+                if (use_v)
+                {
+                    auto T23 = [&](int ke) {
+                        if (ke < 0 || ke >= NZ - 1) return 0.0;
+                        const int j1 = nb.jn(j);
+                        const double dvdz = (V(i, j, ke + 1) - V(i, j, ke)) / dz;
+                        const double sg = use_h ? dvdz + (W(i, j1, ke) - W(i, j, ke)) / dy : dvdz;
+                        return mu_yz(i, j, ke) * sg;
+                    };
+                    const double upper = T23(k);
+                    div += (upper - lower_T23) / dz;
+                    lower_T23 = upper;
+                }
+                dv[g.at(i, j, k)] = S(div / (0.5 * (R(i, j, k) + R(i, nb.jn(j), k))));
+            }
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= NX - 2; ++i)
+        for (int j = 1; j <= NY - 2; ++j)
+        {
+            double lower_T33 = use_v ? 2.0 * MU(i, j, 0) * ((W(i, j, 0) - W(i, j, -1)) / dz) : 0.0;
+            for (int k = 0; k < NZ - 1; ++k)
+            {
+                const double mass = 0.5 * (R(i, j, k) * dz + R(i, j, k + 1) * dz);
+                double force = 0.0;
+                if (use_v)
+                {
+                    const double T33_t = 2.0 * MU(i, j, k + 1) * ((W(i, j, k + 1) - W(i, j, k)) / dz);
+                    force += T33_t - lower_T33;
+                    lower_T33 = T33_t;
+                }
+                if (use_h)
+                {
+                    auto T31 = [&](int ie) {
+                        const int ie1 = nb.in(ie);
+                        const double dwdx = (W(ie1, j, k) - W(ie, j, k)) / dx;
+                        const double sg = use_v ? (U(ie, j, k + 1) - U(ie, j, k)) / dz + dwdx : dwdx;
+                        return mu_xz(ie, j, k) * sg;
+                    };
+                    force += dz * (T31(i) - T31(nb.ip(i))) / dx;
+                    auto T32 = [&](int je) {
+                        const int je1 = nb.jn(je);
+                        const double dwdy = (W(i, je1, k) - W(i, je, k)) / dy;
+                        const double sg = use_v ? (V(i, je, k + 1) - V(i, je, k)) / dz + dwdy : dwdy;
+                        return mu_yz(i, je, k) * sg;
+                    };
+                    force += dz * (T32(j) - T32(nb.jp(j))) / dy;
+                }
+                dw[g.at(i, j, k)] = S(force / mass);
+            }
+        }
+}
+
+// d(phi)/dt of a scalar with diffusivity K (eddy_viscosity.cpp:1953-2090).
+template <class S>
+void sgs_scalar_tendency(const Grid& g, const Config& cfg, const State<S>& s, const std::vector<S>& K,
+                         const std::vector<S>& phi, std::vector<S>& out)
+{
+    const SgsNeighbours nb = sgs_neighbours(g, cfg);
+    const bool use_h = cfg.sgs_mode != kSgsVertical, use_v = cfg.sgs_mode != kSgsHorizontal;
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    const bool ow = open_lateral(cfg.bc_west), oe = open_lateral(cfg.bc_east);
+    const bool os = open_lateral(cfg.bc_south), on = open_lateral(cfg.bc_north);
+    auto T = [&](int a, int b, int c) { return double(s.rho[g.at(a, b, c)]) * double(K[g.at(a, b, c)]); };
+    auto P = [&](int a, int b, int c) { return double(phi[g.at(a, b, c)]); };
+    out.assign(g.cells(), S(0));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i <= NX - 2; ++i)
+        for (int j = 1; j <= NY - 2; ++j)
+        {
+            const int jm = !nb.wy && !os && j == 1 ? j : nb.jp(j);
+            const int jp = !nb.wy && !on && j == NY - 2 ? j : nb.jn(j);
+            const int im = !nb.wx && !ow && i == 1 ? i : nb.ip(i);
+            const int ip = !nb.wx && !oe && i == NX - 2 ? i : nb.in(i);
+            for (int k = 0; k < NZ; ++k)
+            {
+                double div = 0.0;
+                if (use_h)
+                {
+                    const double Fe = 0.5 * (T(i, j, k) + T(ip, j, k)) * (P(ip, j, k) - P(i, j, k)) / dx;
+                    const double Fw = 0.5 * (T(im, j, k) + T(i, j, k)) * (P(i, j, k) - P(im, j, k)) / dx;
+                    const double Fn = 0.5 * (T(i, j, k) + T(i, jp, k)) * (P(i, jp, k) - P(i, j, k)) / dy;
+                    const double Fs = 0.5 * (T(i, jm, k) + T(i, j, k)) * (P(i, j, k) - P(i, jm, k)) / dy;
+                    div += (Fe - Fw) / dx + (Fn - Fs) / dy;
+                }
+                if (use_v)
+                {
+                    const double Ft = k < NZ - 1 ? 0.5 * (T(i, j, k) + T(i, j, k + 1)) * (P(i, j, k + 1) - P(i, j, k)) / dz
+                                                 : 0.0;
+                    const double Fb = k > 0 ? 0.5 * (T(i, j, k - 1) + T(i, j, k)) * (P(i, j, k) - P(i, j, k - 1)) / dz
+                                            : 0.0;
+                    div += (Ft - Fb) / dz;
+                }
+                out[g.at(i, j, k)] = S(div / double(s.rho[g.at(i, j, k)]));
+            }
+        }
+}
+
+// One SGS substep: closure, tendencies, commit with the storage clamps
+// (sgs_apply.cpp:610-790).
+template <class S>
+void sgs_substep(const Grid& g, const Config& cfg, State<S>& s, double h)
+{
+    std::vector<S> nu, kth, kq, du, dv, dw, dth, dqv;
+    sgs_closure(g, cfg, s, nu, kth, kq);
+    sgs_momentum_tendencies(g, cfg, s, nu, du, dv, dw);
+    sgs_scalar_tendency(g, cfg, s, kth, s.theta, dth);
+    sgs_scalar_tendency(g, cfg, s, kq, s.qv, dqv);
+    auto fin = [](double x) { return std::isfinite(x) ? x : 0.0; };
+#pragma omp parallel for schedule(static)
+    for (std::size_t c = 0; c < g.cells(); ++c)
+    {
+        s.u[c] = S(clamp_store<S>(double(s.u[c]) + fin(double(du[c])) * h, 0.0, -kWindH, kWindH));
+        s.v[c] = S(clamp_store<S>(double(s.v[c]) + fin(double(dv[c])) * h, 0.0, -kWindH, kWindH));
+        s.w[c] = S(clamp_store<S>(double(s.w[c]) + fin(double(dw[c])) * h, 0.0, -kWindV, kWindV));
+        s.theta[c] = S(clamp_store<S>(double(s.theta[c]) + fin(double(dth[c])) * h, 300.0, kThetaMin, kThetaMax));
+        s.qv[c] = S(clamp_store<S>(double(s.qv[c]) + fin(double(dqv[c])) * h, 0.0, 0.0f, kQvMax));
+    }
+}
+
+// Substep lengths of the split SGS integration: dt_sgs each, the remainder last
+// (dynamics.cpp:7160-7220; the explicit stability limit is far above dt_sgs for
+// nu_t, K <= 1000 m^2/s at the grid spacings used here and is not applied).
+inline std::vector<double> sgs_substeps(const Config& cfg)
+{
+    std::vector<double> out;
+    double done = 0.0;
+    while (done < cfg.dt && out.size() < 1000)
+    {
+        const double h = std::min(cfg.dt - done, cfg.sgs_dt);
+        if (!(h > 64.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, cfg.dt))) break;
+        out.push_back(h);
+        done += h;
+    }
+    return out;
+}
+
+template <class S>
+void apply_sgs(const Grid& g, const Config& cfg, State<S>& s)
+{
+    if (!cfg.sgs) return;
+    for (double h : sgs_substeps(cfg)) sgs_substep(g, cfg, s, h);
+}
+
 // EOS pressure projection after physics (src/core/orchestration/dynamics/eos_apply.cpp:50-110).
 template <class S>
 void eos_projection(State<S>& s)
@@ -1376,6 +1737,7 @@ int large_step(const Grid& g, const Base& b, const Config& cfg, State<S>& s)
     transport_scalar(g, cfg, s.qr, tmp, mfx, mfy, mfz, rho_begin, s.rho, cfg.dt, true);
     s.qr.swap(tmp);
     microphysics_step(g, cfg, s, cfg.dt);
+    apply_sgs(g, cfg, s);
     eos_projection(s);
     apply_sponges(g, b, cfg, s, cfg.dt);
     apply_boundaries(g, cfg, s, 0.0);

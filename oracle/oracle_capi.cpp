@@ -27,6 +27,8 @@ struct oc_config
     double sponge_w_alpha;
     int sponge_lateral_cells;
     double sponge_lateral_alpha;
+    int sgs, sgs_mode, sgs_filter;
+    double sgs_cs, sgs_pr_t, sgs_sc_t, sgs_nu_max, sgs_k_max, sgs_dt;
 };
 
 struct oc_base
@@ -63,6 +65,9 @@ Config to_config(const oc_config* c)
     k.sponge = c->sponge; k.sponge_z_d = c->sponge_z_d; k.sponge_alpha = c->sponge_alpha;
     k.sponge_w_cells = c->sponge_w_cells; k.sponge_w_alpha = c->sponge_w_alpha;
     k.sponge_lateral_cells = c->sponge_lateral_cells; k.sponge_lateral_alpha = c->sponge_lateral_alpha;
+    k.sgs = c->sgs; k.sgs_mode = c->sgs_mode; k.sgs_filter = c->sgs_filter; k.sgs_cs = c->sgs_cs;
+    k.sgs_pr_t = c->sgs_pr_t; k.sgs_sc_t = c->sgs_sc_t; k.sgs_nu_max = c->sgs_nu_max; k.sgs_k_max = c->sgs_k_max;
+    k.sgs_dt = c->sgs_dt;
     return k;
 }
 
@@ -356,6 +361,48 @@ int oracle_apply_sponges_f64(const oc_grid* gp, const oc_base* bp, const oc_conf
     State<double> s;
     load(g, st, s);
     apply_sponges(g, b, to_config(cp), s, dt);
+    store(g, s, st);
+    return 0;
+}
+
+// Smagorinsky closure and tendencies of one state, no commit (FP64).
+int oracle_sgs_terms_f64(const oc_grid* gp, const oc_config* cp, const oc_state* st, double* nu, double* kth,
+                         double* kq, double* du, double* dv, double* dw, double* dth, double* dqv)
+{
+    const Grid g = to_grid(gp);
+    const Config c = to_config(cp);
+    State<double> s;
+    load(g, st, s);
+    std::vector<double> a, b, d, e, f, h, x, y;
+    sgs_closure(g, c, s, a, b, d);
+    sgs_momentum_tendencies(g, c, s, a, e, f, h);
+    sgs_scalar_tendency(g, c, s, b, s.theta, x);
+    sgs_scalar_tendency(g, c, s, d, s.qv, y);
+    const std::size_t n = g.cells();
+    std::copy(a.begin(), a.end(), nu); std::copy(b.begin(), b.end(), kth); std::copy(d.begin(), d.end(), kq);
+    std::copy(e.begin(), e.end(), du); std::copy(f.begin(), f.end(), dv); std::copy(h.begin(), h.end(), dw);
+    std::copy(x.begin(), x.end(), dth); std::copy(y.begin(), y.end(), dqv);
+    (void)n;
+    return 0;
+}
+
+// The split SGS integration over one large step (all SGS substeps).
+int oracle_apply_sgs_f64(const oc_grid* gp, const oc_config* cp, const oc_state* st)
+{
+    const Grid g = to_grid(gp);
+    State<double> s;
+    load(g, st, s);
+    apply_sgs(g, to_config(cp), s);
+    store(g, s, st);
+    return 0;
+}
+
+int oracle_apply_sgs_f32(const oc_grid* gp, const oc_config* cp, const oc_state* st)
+{
+    const Grid g = to_grid(gp);
+    State<float> s;
+    load(g, st, s);
+    apply_sgs(g, to_config(cp), s);
     store(g, s, st);
     return 0;
 }

@@ -30,7 +30,10 @@ class OcConfig(ctypes.Structure):
                 ("evap_eff", ctypes.c_double), ("vt_max", ctypes.c_double),
                 ("sponge", ctypes.c_int), ("sponge_z_d", ctypes.c_double), ("sponge_alpha", ctypes.c_double),
                 ("sponge_w_cells", ctypes.c_int), ("sponge_w_alpha", ctypes.c_double),
-                ("sponge_lateral_cells", ctypes.c_int), ("sponge_lateral_alpha", ctypes.c_double)]
+                ("sponge_lateral_cells", ctypes.c_int), ("sponge_lateral_alpha", ctypes.c_double),
+                ("sgs", ctypes.c_int), ("sgs_mode", ctypes.c_int), ("sgs_filter", ctypes.c_int),
+                ("sgs_cs", ctypes.c_double), ("sgs_pr_t", ctypes.c_double), ("sgs_sc_t", ctypes.c_double),
+                ("sgs_nu_max", ctypes.c_double), ("sgs_k_max", ctypes.c_double), ("sgs_dt", ctypes.c_double)]
 
 
 class OcBase(ctypes.Structure):
@@ -55,6 +58,9 @@ def lib():
             getattr(L, name).argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), ctypes.c_int]
         L.oracle_resolve_substeps.argtypes = [P(OcGrid), P(OcBase), P(OcConfig)]
         L.oracle_apply_sponges_f64.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), ctypes.c_double]
+        L.oracle_sgs_terms_f64.argtypes = [P(OcGrid), P(OcConfig), P(OcState)] + [_dp] * 8
+        for name in ("oracle_apply_sgs_f64", "oracle_apply_sgs_f32"):
+            getattr(L, name).argtypes = [P(OcGrid), P(OcConfig), P(OcState)]
         for name in ("oracle_acoustic_substep_f32", "oracle_acoustic_substep_f64"):
             getattr(L, name).argtypes = [P(OcGrid), P(OcBase), P(OcState), ctypes.c_double]
         for name in ("oracle_hevi_substep_f64", "oracle_hevi_substep_f32"):
@@ -99,14 +105,18 @@ def grid(nx, ny, nz, dx, dy, dz):
 
 
 def config(dt, substeps, acoustic_mode=0, hevi_beta=0.1, coriolis_f=0.0, lateral=1, bottom=0, top=0,
-           qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None):
+           qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None, sgs=None):
     lat = lateral if isinstance(lateral, (tuple, list)) else (lateral,) * 4
     sp = sponge or {}
+    sg = sgs or {}
     return OcConfig(float(dt), int(substeps), int(acoustic_mode), float(hevi_beta), float(coriolis_f),
                     lat[0], lat[1], lat[2], lat[3], int(bottom), int(top), qc0, c_auto, c_accr, evap_eff, vt_max,
                     int(bool(sponge)), float(sp.get("z_d", 12000.0)), float(sp.get("alpha", 0.05)),
                     int(sp.get("w_cells", 8)), float(sp.get("w_alpha", 0.2)), int(sp.get("lateral_cells", 0)),
-                    float(sp.get("lateral_alpha", 0.05)))
+                    float(sp.get("lateral_alpha", 0.05)),
+                    int(bool(sgs)), int(sg.get("mode", 0)), int(sg.get("filter", 0)), float(sg.get("cs", 0.18)),
+                    float(sg.get("pr_t", 1.0 / 3.0)), float(sg.get("sc_t", 1.0 / 3.0)), float(sg.get("nu_max", 1000.0)),
+                    float(sg.get("k_max", 1000.0)), float(sg.get("dt", 1.0)))
 
 
 def from_step_config(c):
@@ -234,6 +244,25 @@ def apply_sponges(g, base, cfg, st, dt):
     b = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in base.items()}
     lib().oracle_apply_sponges_f64(ctypes.byref(g), ctypes.byref(base_struct(b)), ctypes.byref(cfg),
                                    ctypes.byref(_state_struct(s)), float(dt))
+    return s
+
+
+def sgs_terms(g, cfg, st):
+    """Smagorinsky closure and tendencies of a state (FP64, no commit):
+    dict nu, k_theta, k_q, du, dv, dw, dtheta, dqv."""
+    s = copy_state(st, np.float64)
+    names = ("nu", "k_theta", "k_q", "du", "dv", "dw", "dtheta", "dqv")
+    out = {n: np.zeros(st["u"].shape) for n in names}
+    lib().oracle_sgs_terms_f64(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)),
+                               *[out[n].ctypes.data_as(_dp) for n in names])
+    return out
+
+
+def apply_sgs(g, cfg, st, dtype=np.float64):
+    """The split SGS integration of one large step on a copy of st."""
+    s = copy_state(st, dtype)
+    fn = lib().oracle_apply_sgs_f64 if np.dtype(dtype) == np.float64 else lib().oracle_apply_sgs_f32
+    fn(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)))
     return s
 
 

@@ -1,0 +1,205 @@
+"""Smagorinsky SGS of the oracle against closed forms and exact discrete identities.
+
+* A uniform flow has no strain: the viscosity and every tendency vanish and
+  the SGS step leaves the state bitwise unchanged.
+* A horizontal linear shear u = a y has |S| = |a| at every interior centre
+  (four xy-edge strains of a/2), so nu = Cs^2 Delta^2 |a| exactly, with
+  Delta the cube root of the cell volume or sqrt(dx dy); its stress is
+  uniform, so the momentum tendency vanishes in the interior.
+* A vertical shear u = a z has no horizontal strain ("2d" mode) and
+  |S| = |a| in the 3-D and vertical modes.
+* With that uniform viscosity and a uniform density, the flux-form scalar
+  diffusion of theta = A sin(kx) is K times the 3-point Laplacian, whose
+  eigenvalue (2 cos(k dx) - 2)/dx^2 is exact.
+* With closed walls the flux form conserves the scalar mass sum(rho phi)
+  and never increases its variance sum(rho phi^2).
+(eddy_viscosity.cpp:1253-1527, 1532-1948, 1953-2090; smagorinsky.cpp:495-580.)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from nimbus_amd import lib as L
+from oracle import oracle_py as O
+
+NX, NY, NZ = 14, 13, 9
+DX, DY, DZ = 500.0, 400.0, 250.0
+
+
+def _grid():
+    return O.grid(NX, NY, NZ, DX, DY, DZ)
+
+
+def _state(u=0.0, v=0.0, w=0.0, theta=300.0, qv=0.01, rho=1.0):
+    shape = (NX, NY, NZ)
+    st = {f: np.zeros(shape) for f in L.FIELDS}
+    for name, val in (("u", u), ("v", v), ("w", w), ("theta", theta), ("qv", qv), ("rho", rho)):
+        st[name] = np.broadcast_to(np.asarray(val, dtype=np.float64), shape).copy()
+    st["p"][:] = 1e5
+    return st
+
+
+def _cfg(mode=L.SGS_HORIZONTAL, filt=L.SGS_FILTER_CUBIC_ROOT, lateral=L.BC_RIGID, dt=3.0):
+    sg = dict(L.sgs_preset(), mode=mode, filter=filt)
+    return O.from_step_config(L.step_config(dt, 6, lateral=lateral, sgs=sg))
+
+
+def test_uniform_flow_has_no_sgs():
+    st = _state(u=12.0, v=-3.0, theta=301.5, qv=0.012)
+    t = O.sgs_terms(_grid(), _cfg(L.SGS_3D), st)
+    for name in t:
+        assert np.all(t[name] == 0.0), name
+    out = O.apply_sgs(_grid(), _cfg(L.SGS_3D), st)
+    for f in L.FIELDS:
+        assert np.array_equal(out[f], st[f]), f
+
+
+@pytest.mark.parametrize("filt", [L.SGS_FILTER_CUBIC_ROOT, L.SGS_FILTER_DX])
+def test_horizontal_shear_viscosity_closed_form(filt):
+    a = 4e-3
+    y = np.arange(NY)[None, :, None] * DY
+    st = _state(u=a * y + np.zeros((NX, NY, NZ)))
+    t = O.sgs_terms(_grid(), _cfg(L.SGS_HORIZONTAL, filt), st)
+    delta = (DX * DY * DZ) ** (1.0 / 3.0) if filt == L.SGS_FILTER_CUBIC_ROOT else math.sqrt(DX * DY)
+    nu = 0.18 ** 2 * delta ** 2 * a
+    inner = t["nu"][2:-2, 2:-2, :]
+    assert np.allclose(inner, nu, rtol=1e-12, atol=0), (inner.min(), inner.max(), nu)
+    assert np.allclose(t["k_theta"][2:-2, 2:-2, :], 3.0 * nu, rtol=1e-12)
+    # uniform stress: no momentum tendency away from the walls
+    assert np.max(np.abs(t["du"][2:-3, 2:-2, :-1])) < 1e-12 * a * DY
+    assert np.max(np.abs(t["dv"][2:-2, 2:-3, :-1])) < 1e-15
+
+
+def test_vertical_shear_modes():
+    a = 5e-3
+    z = (np.arange(NZ)[None, None, :] + 0.5) * DZ
+    st = _state(u=a * z + np.zeros((NX, NY, NZ)))
+    inner = (slice(2, -2), slice(2, -2), slice(1, -1))
+    delta = (DX * DY * DZ) ** (1.0 / 3.0)
+    t_h = O.sgs_terms(_grid(), _cfg(L.SGS_HORIZONTAL), st)
+    assert np.all(t_h["nu"][inner] == 0.0)
+    for mode in (L.SGS_3D, L.SGS_VERTICAL):
+        t = O.sgs_terms(_grid(), _cfg(mode), st)
+        assert np.allclose(t["nu"][inner], 0.18 ** 2 * delta ** 2 * a, rtol=1e-12), mode
+
+
+def test_scalar_diffusion_eigenvalue():
+    a = 4e-3
+    y = np.arange(NY)[None, :, None] * DY
+    x = np.arange(NX)[:, None, None] * DX
+    kx = 2.0 * math.pi / (8 * DX)
+    amp = 0.7
+    st = _state(u=a * y + np.zeros((NX, NY, NZ)), theta=300.0 + amp * np.sin(kx * x) + np.zeros((NX, NY, NZ)))
+    t = O.sgs_terms(_grid(), _cfg(L.SGS_HORIZONTAL), st)
+    delta = (DX * DY * DZ) ** (1.0 / 3.0)
+    K = 3.0 * 0.18 ** 2 * delta ** 2 * a
+    lam = (2.0 * math.cos(kx * DX) - 2.0) / DX ** 2
+    expect = K * lam * amp * np.sin(kx * x) + np.zeros((NX, NY, NZ))
+    sl = (slice(3, -3), slice(3, -3), slice(None))
+    assert np.allclose(t["dtheta"][sl], expect[sl], rtol=1e-9, atol=1e-12 * abs(K * lam * amp))
+
+
+@pytest.mark.parametrize("mode", [L.SGS_3D, L.SGS_HORIZONTAL])
+def test_scalar_flux_form_conserves_and_dissipates(mode):
+    rng = np.random.default_rng(5)
+    shape = (NX, NY, NZ)
+    st = _state(u=5.0 * rng.standard_normal(shape), v=5.0 * rng.standard_normal(shape),
+                w=2.0 * rng.standard_normal(shape), theta=300.0 + rng.standard_normal(shape),
+                qv=0.01 + 0.002 * rng.standard_normal(shape), rho=1.0 + 0.1 * rng.random(shape))
+    t = O.sgs_terms(_grid(), _cfg(mode, lateral=L.BC_RIGID), st)
+    inner = (slice(1, -1), slice(1, -1), slice(None))
+    rho = st["rho"][inner]
+    for phi, d in (("theta", "dtheta"), ("qv", "dqv")):
+        rate = rho * t[d][inner]
+        scale = np.sum(np.abs(rate))
+        assert scale > 0.0
+        assert abs(np.sum(rate)) < 1e-12 * scale, phi  # closed walls: no flux through the boundary
+        anomaly = st[phi][inner] - np.mean(st[phi][inner])
+        assert np.sum(rate * anomaly) < 0.0, phi  # variance decays
+    assert np.all(t["nu"] >= 0.0) and np.all(t["nu"] <= 1000.0)
+
+
+def test_sgs_substeps_follow_dt_sgs():
+    st = _state(u=np.random.default_rng(1).standard_normal((NX, NY, NZ)))
+    # dt = 2.5 s with dt_sgs = 1 s: substeps 1, 1, 0.5 -- three closure refreshes;
+    # the same as applying dt = 1, 1, 0.5 one after the other
+    full = O.apply_sgs(_grid(), _cfg(dt=2.5), st)
+    part = st
+    for dt in (1.0, 1.0, 0.5):
+        part = O.apply_sgs(_grid(), _cfg(dt=dt), part)
+    for f in L.FIELDS:
+        assert np.array_equal(full[f], part[f]), f
+
+
+@pytest.mark.parametrize("mode", [L.SGS_HORIZONTAL, L.SGS_3D])
+@pytest.mark.parametrize("axis", ["y", "x"])
+def test_rigid_rotation_has_no_stress(mode, axis):
+    """Solid-body rotation u = -W y, v = W x has zero strain (the stress tensor
+    is symmetric), so it feels no SGS force even where the viscosity varies;
+    w = sin(...) along y (x) makes nu vary along the direction in which the
+    u (v) equation differences its shear stress."""
+    om = 2e-3
+    x = np.arange(NX)[:, None, None] * DX + np.zeros((NX, NY, NZ))
+    y = np.arange(NY)[None, :, None] * DY + np.zeros((NX, NY, NZ))
+    s_ = y if axis == "y" else x
+    L_ = DY if axis == "y" else DX
+    w = 3.0 * np.sin(2.0 * math.pi * s_ / (6.0 * L_))
+    st = _state(u=-om * y, v=om * x, w=w)
+    t = O.sgs_terms(_grid(), _cfg(mode, lateral=L.BC_RIGID), st)
+    sl = (slice(2, -3), slice(2, -3), slice(0, -1))
+    assert np.std(t["nu"][sl]) > 0.05 * np.mean(t["nu"][sl])  # the viscosity does vary
+    scale = np.max(t["nu"][sl]) * om / min(DX, DY)
+    comp = "du" if axis == "y" else "dv"
+    assert np.max(np.abs(t[comp][sl])) < 1e-9 * scale, (comp, np.max(np.abs(t[comp][sl])), scale)
+
+
+def test_normal_stress_one_dimensional():
+    """u = u(x) only, horizontal mode: |S| = sqrt(2) |du/dx| at each centre
+    and du/dt at face i is (2 mu_{i+1} S11_{i+1} - 2 mu_i S11_i) / (rho dx)
+    with mu = rho nu, S11_i = (u_i - u_{i-1}) / dx (a 1-D variable-coefficient
+    diffusion)."""
+    x = np.arange(NX) * DX
+    prof = 8.0 * np.sin(2.0 * math.pi * x / (5.3 * DX)) + 1e-3 * x
+    st = _state(u=prof[:, None, None] + np.zeros((NX, NY, NZ)))
+    t = O.sgs_terms(_grid(), _cfg(L.SGS_HORIZONTAL), st)
+    delta = (DX * DY * DZ) ** (1.0 / 3.0)
+    s11 = np.zeros(NX)
+    s11[1:] = (prof[1:] - prof[:-1]) / DX
+    nu = 0.18 ** 2 * delta ** 2 * np.sqrt(2.0) * np.abs(s11)
+    for i in range(2, NX - 3):
+        expect = (2.0 * nu[i + 1] * s11[i + 1] - 2.0 * nu[i] * s11[i]) / DX
+        got = t["du"][i, 5, 3]
+        assert abs(got - expect) < 1e-9 * abs(expect) + 1e-15, (i, got, expect)
+
+
+def test_shear_stress_with_two_dimensional_viscosity():
+    """u = u(y), w = w(x), v = 0, horizontal mode, uniform density: at an
+    interior level |S|^2 = 2 (s_{j-1}^2 + s_j^2) + (g_{i-1}^2 + g_i^2) / 2 with
+    s_j = (u_{j+1} - u_j) / (2 dy) and g_i = (w_{i+1} - w_i) / dx, and
+    du/dt = (T(i, j) - T(i, j-1)) / dy with T(i, e) = 2 s_e times the mean
+    viscosity of the four centres around the xy-edge (i, e)."""
+    y = np.arange(NY) * DY
+    x = np.arange(NX) * DX
+    up = 6.0 * np.sin(2.0 * math.pi * y / (4.7 * DY))
+    wp = 2.0 * np.cos(2.0 * math.pi * x / (5.9 * DX))
+    st = _state(u=up[None, :, None] + np.zeros((NX, NY, NZ)), w=wp[:, None, None] + np.zeros((NX, NY, NZ)))
+    t = O.sgs_terms(_grid(), _cfg(L.SGS_HORIZONTAL), st)
+    delta = (DX * DY * DZ) ** (1.0 / 3.0)
+    s = np.zeros(NY)
+    s[:-1] = 0.5 * (up[1:] - up[:-1]) / DY
+    gx = np.zeros(NX)
+    gx[:-1] = (wp[1:] - wp[:-1]) / DX
+    nu = np.zeros((NX, NY))
+    for i in range(1, NX - 1):
+        for j in range(1, NY - 1):
+            nu[i, j] = 0.18 ** 2 * delta ** 2 * math.sqrt(2.0 * (s[j - 1] ** 2 + s[j] ** 2)
+                                                          + 0.5 * (gx[i - 1] ** 2 + gx[i] ** 2))
+    k = 4
+    assert np.allclose(t["nu"][1:-1, 1:-1, k], nu[1:-1, 1:-1], rtol=1e-12)
+    for i in range(2, NX - 3):
+        for j in range(2, NY - 3):
+            edge = lambda e: 2.0 * s[e] * 0.25 * (nu[i, e] + nu[i + 1, e] + nu[i, e + 1] + nu[i + 1, e + 1])
+            expect = (edge(j) - edge(j - 1)) / DY
+            got = t["du"][i, j, k]
+            assert abs(got - expect) < 1e-9 * abs(expect) + 1e-14, (i, j, got, expect)
