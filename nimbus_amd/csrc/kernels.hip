@@ -1533,7 +1533,7 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     const KesslerConst kc = A.p.kessler;
     const Bounds bd = A.p.bounds;
     const double h = A.p.dt / A.p.kessler_sub;
-    const bool do_eos = n == A.p.kessler_sub - 1;
+    const bool do_eos = n == A.p.kessler_sub - 1 && A.p.sgs_n == 0;  // with the SGS on, after it
     const std::size_t c = g.at(i, j, k), sz = g.plane();
     // all loads first (the level above clamped at the lid, unused there)
     const bool up = k + 1 < NZ;
@@ -1588,6 +1588,317 @@ __global__ __launch_bounds__(kBlock) void k_scalars_home(const MemberArgs* __res
     P<S>(f.qv)[c] = P<S>(f.tqv)[c];
     P<S>(f.qc)[c] = P<S>(f.tqc)[c];
     P<S>(f.qr)[c] = P<S>(f.tqr)[c];
+}
+
+// ---------------------------------------------------------------------
+// Smagorinsky SGS (smagorinsky.cpp:495-580; eddy_viscosity.cpp:1253-1527
+// closure, 1532-1948 momentum, 1953-2090 scalars; sgs_apply.cpp:610-790
+// commit), applied after the microphysics in sgs_n substeps, each three
+// per-cell passes: closure (nu, K_theta, K_q into the dead k1 fields),
+// tendencies (u2, v2, w2, rp2, mfx, all dead after transport), commit (with
+// the EOS projection after the last). FP64 arithmetic on storage-type values,
+// as the reference's CPU path; the per-edge stresses a reference column
+// march carries from level to level are recomputed here from the same
+// operands, so the values are identical.
+// ---------------------------------------------------------------------
+template <class S>
+struct SgsView
+{
+    gptr<const S> u, v, w, rho_p, th, qv;
+    const double* rho0;  // base slot, global
+    DevGrid g;
+    int wx, wy;
+    __device__ int ip(int i) const { return wx ? (i <= 1 ? g.nx - 2 : min(i - 1, g.nx - 2)) : max(i - 1, 0); }
+    __device__ int in(int i) const { return wx ? (i >= g.nx - 2 ? 1 : max(i + 1, 1)) : min(i + 1, g.nx - 1); }
+    __device__ int jp(int j) const { return wy ? (j <= 1 ? g.ny - 2 : min(j - 1, g.ny - 2)) : max(j - 1, 0); }
+    __device__ int jn(int j) const { return wy ? (j >= g.ny - 2 ? 1 : max(j + 1, 1)) : min(j + 1, g.ny - 1); }
+    __device__ double U(int i, int j, int k) const { return double(u[g.at(i, j, k)]); }
+    __device__ double V(int i, int j, int k) const { return double(v[g.at(i, j, k)]); }
+    __device__ double W(int i, int j, int k) const { return k < 0 ? 0.0 : double(w[g.at(i, j, k)]); }
+    // live density: rho0 + rho', rounded to storage as the reference's whole-value field
+    __device__ double R(int i, int j, int k) const
+    {
+        return double(S(slot(rho0, kBaseRho0, g.nz)[k] + double(rho_p[g.at(i, j, k)])));
+    }
+};
+
+template <class S>
+__device__ SgsView<S> sgs_view(const MemberArgs& A)
+{
+    const DevFields& f = A.f;
+    SgsView<S> s;
+    s.u = P<S>(f.u); s.v = P<S>(f.v); s.w = P<S>(f.w); s.rho_p = P<S>(f.rp); s.th = P<S>(f.th); s.qv = P<S>(f.qv);
+    s.rho0 = f.base;
+    s.g = A.g;
+    s.wx = A.p.sgs_wrap_x;
+    s.wy = A.p.sgs_wrap_y;
+    return s;
+}
+
+template <class S>
+__device__ double sgs_strain_sq(const SgsView<S>& s, int mode, int i, int j, int k)
+{
+    const int NZ = s.g.nz;
+    const double dx = s.g.dx, dy = s.g.dy, dz = s.g.dz;
+    const int im = s.ip(i), jm = s.jp(j);
+    const double S11 = (s.U(i, j, k) - s.U(im, j, k)) / dx;
+    const double S22 = (s.V(i, j, k) - s.V(i, jm, k)) / dy;
+    const double S33 = (s.W(i, j, k) - s.W(i, j, k - 1)) / dz;
+    auto s12 = [&](int ii, int jj) {
+        const int jj1 = s.jn(jj), ii1 = s.in(ii);
+        return 0.5 * ((s.U(ii, jj1, k) - s.U(ii, jj, k)) / dy + (s.V(ii1, jj, k) - s.V(ii, jj, k)) / dx);
+    };
+    struct G2
+    {
+        double a, b;
+    };
+    auto xz = [&](int ii, int kk) {
+        if (kk < 0 || kk >= NZ - 1) return G2{0.0, 0.0};
+        const int ii1 = s.in(ii);
+        return G2{(s.U(ii, j, kk + 1) - s.U(ii, j, kk)) / dz, (s.W(ii1, j, kk) - s.W(ii, j, kk)) / dx};
+    };
+    auto yz = [&](int jj, int kk) {
+        if (kk < 0 || kk >= NZ - 1) return G2{0.0, 0.0};
+        const int jj1 = s.jn(jj);
+        return G2{(s.V(i, jj, kk + 1) - s.V(i, jj, kk)) / dz, (s.W(i, jj1, kk) - s.W(i, jj, kk)) / dy};
+    };
+    const double a = s12(im, jm), b = s12(im, j), c = s12(i, jm), d = s12(i, j);
+    const double S12_sq = 0.25 * (a * a + b * b + c * c + d * d);
+    const G2 xa = xz(im, k - 1), xb = xz(im, k), xc = xz(i, k - 1), xd = xz(i, k);
+    const G2 ya = yz(jm, k - 1), yb = yz(jm, k), yc = yz(j, k - 1), yd = yz(j, k);
+    auto avg_b = [](G2 p, G2 q, G2 r, G2 t) { return 0.25 * (p.b * p.b + q.b * q.b + r.b * r.b + t.b * t.b); };
+    auto avg_a = [](G2 p, G2 q, G2 r, G2 t) { return 0.25 * (p.a * p.a + q.a * q.a + r.a * r.a + t.a * t.a); };
+    if (mode == NIMBUS_SGS_HORIZONTAL)
+        return 2.0 * (S11 * S11 + S22 * S22) + 4.0 * S12_sq + avg_b(xa, xb, xc, xd) + avg_b(ya, yb, yc, yd);
+    if (mode == NIMBUS_SGS_VERTICAL) return avg_a(xa, xb, xc, xd) + avg_a(ya, yb, yc, yd) + 2.0 * S33 * S33;
+    auto sym = [](G2 p) {
+        const double e = 0.5 * (p.a + p.b);
+        return e * e;
+    };
+    const double S13_sq = 0.25 * (sym(xa) + sym(xb) + sym(xc) + sym(xd));
+    const double S23_sq = 0.25 * (sym(ya) + sym(yb) + sym(yc) + sym(yd));
+    return 2.0 * (S11 * S11 + S22 * S22 + S33 * S33) + 4.0 * (S12_sq + S13_sq + S23_sq);
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_sgs_closure(const MemberArgs* __restrict__ ma,
+                                                        const int2* __restrict__ map, int n)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (n >= A.p.sgs_n) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const SgsView<S> sv = sgs_view<S>(A);
+    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
+    const double nu = fmin(A.p.sgs_c2d2 * sqrt(sgs_strain_sq<S>(sv, A.p.sgs_mode, cl.i, cl.j, cl.k)), A.p.sgs_nu_max);
+    P<S>(A.f.k1u)[c] = S(nu);
+    P<S>(A.f.k1v)[c] = S(fmin(nu * A.p.sgs_inv_pr, A.p.sgs_k_max));
+    P<S>(A.f.k1w)[c] = S(fmin(nu * A.p.sgs_inv_sc, A.p.sgs_k_max));
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_sgs_tendency(const MemberArgs* __restrict__ ma,
+                                                         const int2* __restrict__ map, int n)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (n >= A.p.sgs_n) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const SgsView<S> s = sgs_view<S>(A);
+    const DevGrid& g = A.g;
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const int i = cl.i, j = cl.j, k = cl.k;
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    const bool use_h = A.p.sgs_mode != NIMBUS_SGS_VERTICAL, use_v = A.p.sgs_mode != NIMBUS_SGS_HORIZONTAL;
+    const gptr<const S> nu = P<S>(A.f.k1u), kth = P<S>(A.f.k1v), kq = P<S>(A.f.k1w);
+    auto MU = [&](int a, int b, int cc) { return s.R(a, b, cc) * double(nu[g.at(a, b, cc)]); };
+    auto mu_xy = [&](int a, int b, int cc) {
+        const int a1 = s.in(a), b1 = s.jn(b);
+        return 0.25 * (MU(a, b, cc) + MU(a1, b, cc) + MU(a, b1, cc) + MU(a1, b1, cc));
+    };
+    auto mu_xz = [&](int a, int b, int cc) {
+        const int a1 = s.in(a);
+        const double lower = 0.5 * (MU(a, b, cc) + MU(a1, b, cc)), upper = 0.5 * (MU(a, b, cc + 1) + MU(a1, b, cc + 1));
+        return (dz * lower + dz * upper) / (dz + dz);
+    };
+    auto mu_yz = [&](int a, int b, int cc) {
+        const int b1 = s.jn(b);
+        const double lower = 0.5 * (MU(a, b, cc) + MU(a, b1, cc)), upper = 0.5 * (MU(a, b, cc + 1) + MU(a, b1, cc + 1));
+        return (dz * lower + dz * upper) / (dz + dz);
+    };
+    const std::size_t c = g.at(i, j, k);
+    // du/dt at the x-face i (eddy_viscosity.cpp:1676-1750)
+    double du = 0.0;
+    if (i >= 1 && i <= (s.wx ? NX - 2 : NX - 3) && j >= 1 && j <= NY - 2 && k < NZ - 1)
+    {
+        const int jm = s.jp(j), im = s.ip(i), ipp = s.in(i);
+        double div = 0.0;
+        if (use_h)
+        {
+            const double S11_r = (s.U(ipp, j, k) - s.U(i, j, k)) / dx, S11_l = (s.U(i, j, k) - s.U(im, j, k)) / dx;
+            div += (2.0 * MU(ipp, j, k) * S11_r - 2.0 * MU(i, j, k) * S11_l) / dx;
+            auto T12 = [&](int je) {
+                const int i1 = s.in(i), je1 = s.jn(je);
+                const double S12 =
+                    0.5 * ((s.U(i, je1, k) - s.U(i, je, k)) / dy + (s.V(i1, je, k) - s.V(i, je, k)) / dx);
+                return 2.0 * mu_xy(i, je, k) * S12;
+            };
+            div += (T12(j) - T12(jm)) / dy;
+        }
+        if (use_v)
+        {
+            auto T13 = [&](int ke) {
+                if (ke < 0 || ke >= NZ - 1) return 0.0;
+                const double dudz = (s.U(i, j, ke + 1) - s.U(i, j, ke)) / dz;
+                const double sg = use_h ? dudz + (s.W(ipp, j, ke) - s.W(i, j, ke)) / dx : dudz;
+                return mu_xz(i, j, ke) * sg;
+            };
+            div += (T13(k) - T13(k - 1)) / dz;
+        }
+        du = div / (0.5 * (s.R(i, j, k) + s.R(s.in(i), j, k)));
+    }
+    // dv/dt at the y-face j (1752-1826)
+    double dv = 0.0;
+    if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && (s.wy || j < NY - 2) && k < NZ - 1)
+    {
+        const int jm = s.jp(j), jpp = s.jn(j);
+        double div = 0.0;
+        if (use_h)
+        {
+            const double S22_n = (s.V(i, jpp, k) - s.V(i, j, k)) / dy, S22_s = (s.V(i, j, k) - s.V(i, jm, k)) / dy;
+            div += (2.0 * MU(i, jpp, k) * S22_n - 2.0 * MU(i, j, k) * S22_s) / dy;
+            auto T21 = [&](int ie) {
+                const int ie1 = s.in(ie), j1 = s.jn(j);
+                const double S12 =
+                    0.5 * ((s.U(ie, j1, k) - s.U(ie, j, k)) / dy + (s.V(ie1, j, k) - s.V(ie, j, k)) / dx);
+                return 2.0 * mu_xy(ie, j, k) * S12;
+            };
+            div += (T21(i) - T21(s.ip(i))) / dx;
+        }
+        if (use_v)
+        {
+            auto T23 = [&](int ke) {
+                if (ke < 0 || ke >= NZ - 1) return 0.0;
+                const int j1 = s.jn(j);
+                const double dvdz = (s.V(i, j, ke + 1) - s.V(i, j, ke)) / dz;
+                const double sg = use_h ? dvdz + (s.W(i, j1, ke) - s.W(i, j, ke)) / dy : dvdz;
+                return mu_yz(i, j, ke) * sg;
+            };
+            div += (T23(k) - T23(k - 1)) / dz;
+        }
+        dv = div / (0.5 * (s.R(i, j, k) + s.R(i, s.jn(j), k)));
+    }
+    // dw/dt at the z-face k (1828-1945)
+    double dw = 0.0;
+    if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && k < NZ - 1)
+    {
+        const double mass = 0.5 * (s.R(i, j, k) * dz + s.R(i, j, k + 1) * dz);
+        double force = 0.0;
+        if (use_v)
+        {
+            const double T33_t = 2.0 * MU(i, j, k + 1) * ((s.W(i, j, k + 1) - s.W(i, j, k)) / dz);
+            const double T33_b = 2.0 * MU(i, j, k) * ((s.W(i, j, k) - s.W(i, j, k - 1)) / dz);
+            force += T33_t - T33_b;
+        }
+        if (use_h)
+        {
+            auto T31 = [&](int ie) {
+                const int ie1 = s.in(ie);
+                const double dwdx = (s.W(ie1, j, k) - s.W(ie, j, k)) / dx;
+                const double sg = use_v ? (s.U(ie, j, k + 1) - s.U(ie, j, k)) / dz + dwdx : dwdx;
+                return mu_xz(ie, j, k) * sg;
+            };
+            force += dz * (T31(i) - T31(s.ip(i))) / dx;
+            auto T32 = [&](int je) {
+                const int je1 = s.jn(je);
+                const double dwdy = (s.W(i, je1, k) - s.W(i, je, k)) / dy;
+                const double sg = use_v ? (s.V(i, je, k + 1) - s.V(i, je, k)) / dz + dwdy : dwdy;
+                return mu_yz(i, je, k) * sg;
+            };
+            force += dz * (T32(j) - T32(s.jp(j))) / dy;
+        }
+        dw = force / mass;
+    }
+    // d(theta)/dt and d(qv)/dt (1953-2090)
+    double dth = 0.0, dqv = 0.0;
+    if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2)
+    {
+        const bool ow = A.p.sgs_open[0], oe = A.p.sgs_open[1], os = A.p.sgs_open[2], on = A.p.sgs_open[3];
+        const int jm = !s.wy && !os && j == 1 ? j : s.jp(j);
+        const int jpp = !s.wy && !on && j == NY - 2 ? j : s.jn(j);
+        const int im = !s.wx && !ow && i == 1 ? i : s.ip(i);
+        const int ipp = !s.wx && !oe && i == NX - 2 ? i : s.in(i);
+        auto scalar = [&](gptr<const S> K, gptr<const S> phi) {
+            auto T = [&](int a, int b, int cc) { return s.R(a, b, cc) * double(K[g.at(a, b, cc)]); };
+            auto Ph = [&](int a, int b, int cc) { return double(phi[g.at(a, b, cc)]); };
+            double div = 0.0;
+            if (use_h)
+            {
+                const double Fe = 0.5 * (T(i, j, k) + T(ipp, j, k)) * (Ph(ipp, j, k) - Ph(i, j, k)) / dx;
+                const double Fw = 0.5 * (T(im, j, k) + T(i, j, k)) * (Ph(i, j, k) - Ph(im, j, k)) / dx;
+                const double Fn = 0.5 * (T(i, j, k) + T(i, jpp, k)) * (Ph(i, jpp, k) - Ph(i, j, k)) / dy;
+                const double Fs = 0.5 * (T(i, jm, k) + T(i, j, k)) * (Ph(i, j, k) - Ph(i, jm, k)) / dy;
+                div += (Fe - Fw) / dx + (Fn - Fs) / dy;
+            }
+            if (use_v)
+            {
+                const double Ft =
+                    k < NZ - 1 ? 0.5 * (T(i, j, k) + T(i, j, k + 1)) * (Ph(i, j, k + 1) - Ph(i, j, k)) / dz : 0.0;
+                const double Fb =
+                    k > 0 ? 0.5 * (T(i, j, k - 1) + T(i, j, k)) * (Ph(i, j, k) - Ph(i, j, k - 1)) / dz : 0.0;
+                div += (Ft - Fb) / dz;
+            }
+            return div / s.R(i, j, k);
+        };
+        dth = scalar(kth, s.th);
+        dqv = scalar(kq, s.qv);
+    }
+    P<S>(A.f.u2)[c] = S(du);
+    P<S>(A.f.v2)[c] = S(dv);
+    P<S>(A.f.w2)[c] = S(dw);
+    P<S>(A.f.rp2)[c] = S(dth);
+    P<S>(A.f.mfx)[c] = S(dqv);
+}
+
+// Commit of SGS substep n (sgs_apply.cpp:610-790); after the last one the
+// EOS projection the Kessler pass leaves out when the SGS is on.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_sgs_apply(const MemberArgs* __restrict__ ma,
+                                                      const int2* __restrict__ map, int n)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (n >= A.p.sgs_n) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const DevFields& f = A.f;
+    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
+    const double h = n == A.p.sgs_n - 1 ? A.p.sgs_h_last : A.p.sgs_h;
+    const auto u = P<S>(f.u), v = P<S>(f.v), w = P<S>(f.w), th = P<S>(f.th), qv = P<S>(f.qv);
+    const S tu = P<S>(f.u2)[c], tv = P<S>(f.v2)[c], tw = P<S>(f.w2)[c], tt = P<S>(f.rp2)[c], tq = P<S>(f.mfx)[c];
+    const S uo = u[c], vo = v[c], wo = w[c], to = th[c], qo = qv[c];
+    const Bounds& bd = A.p.bounds;
+    u[c] = S(clamp_store<S>(double(uo) + finite0(double(tu)) * h, 0.0, -bd.wind_h, bd.wind_h));
+    v[c] = S(clamp_store<S>(double(vo) + finite0(double(tv)) * h, 0.0, -bd.wind_h, bd.wind_h));
+    w[c] = S(clamp_store<S>(double(wo) + finite0(double(tw)) * h, 0.0, -bd.wind_v, bd.wind_v));
+    const S thn = S(clamp_store<S>(double(to) + finite0(double(tt)) * h, 300.0, 200.0f, 700.0f));
+    th[c] = thn;
+    qv[c] = S(clamp_store<S>(double(qo) + finite0(double(tq)) * h, 0.0, 0.0f, 0.05f));
+    if (n == A.p.sgs_n - 1)
+    {
+        const int k = cl.k;
+        const double r0 = slot(f.base, kBaseRho0, A.g.nz)[k], p0k = slot(f.base, kBaseP0, A.g.nz)[k];
+        const double rho = r0 + double(P<S>(f.rp)[c]);
+        const double thv = double(thn);
+        if (rho > 0.0 && thv > 0.0)
+        {
+            const double pd = dev::eos_pressure(rho, thv);
+            if (isfinite(pd) && pd > 0.0)
+                P<S>(f.pp)[c] = S(clamp_store<S>(pd, 101325.0, A.p.bounds.p_min, A.p.bounds.p_max) - p0k);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -1926,6 +2237,12 @@ void launch_large_step(Launcher& L, const BatchView& b)
     for (int n = 0; n < b.max_kessler_sub; ++n)
         launch_map(L, "kessler_apply", k_kessler_apply<S>, b, S_ALL, W_CELL, n);
     launch_map(L, "scalars_home", k_scalars_home<S>, b, S_ALL, W_CELL);
+    for (int n = 0; n < b.max_sgs_n; ++n)
+    {
+        launch_map(L, "sgs_closure", k_sgs_closure<S>, b, S_ALL, W_CELL, n);
+        launch_map(L, "sgs_tendency", k_sgs_tendency<S>, b, S_ALL, W_CELL, n);
+        launch_map(L, "sgs_apply", k_sgs_apply<S>, b, S_ALL, W_CELL, n);
+    }
     if (b.any_sponge) launch_map(L, "sponge", k_sponge<S>, b, S_ALL, W_CELL);
     launch_boundaries<S>(L, b, -1);
 }

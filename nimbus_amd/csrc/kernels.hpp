@@ -68,6 +68,16 @@ struct StepParams
     int kessler_sub;    // microphysics subcycles per large step
     int sponge;         // Rayleigh sponges on
     int sponge_lateral; // lateral band width (cells)
+    // Smagorinsky SGS: sgs_n substeps of sgs_h seconds, the last sgs_h_last;
+    // sgs_n = 0 when off
+    int sgs_n;
+    double sgs_h, sgs_h_last;
+    int sgs_mode;                 // NIMBUS_SGS_*
+    double sgs_c2d2;              // Cs^2 Delta^2
+    double sgs_inv_pr, sgs_inv_sc; // 1 / Pr_t, 1 / Sc_t
+    double sgs_nu_max, sgs_k_max;
+    int sgs_wrap_x, sgs_wrap_y;   // periodic topology
+    int sgs_open[4];              // west, east, south, north lateral faces open (radiative or zero-gradient)
     KesslerConst kessler;
     Bounds bounds;
 };
@@ -125,6 +135,7 @@ struct BatchView
     int max_substeps = 0;
     int max_kessler_sub = 0;
     int any_sponge = 0;
+    int max_sgs_n = 0;
     int max_nz = 0;
     int elem = 4;
     BlockMap maps[S_SUBSETS][W_KINDS];

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -136,7 +137,13 @@ bool valid_config(const nimbus_step_config& c)
            (!c.sponge || (std::isfinite(c.sponge_z_d) && std::isfinite(c.sponge_alpha) && c.sponge_alpha >= 0 &&
                           c.sponge_w_cells >= 0 && std::isfinite(c.sponge_w_alpha) && c.sponge_w_alpha >= 0 &&
                           c.sponge_lateral_cells >= 0 && std::isfinite(c.sponge_lateral_alpha) &&
-                          c.sponge_lateral_alpha >= 0));
+                          c.sponge_lateral_alpha >= 0)) &&
+           (!c.sgs || (c.sgs_mode >= NIMBUS_SGS_3D && c.sgs_mode <= NIMBUS_SGS_VERTICAL &&
+                       (c.sgs_filter == NIMBUS_SGS_FILTER_CUBIC_ROOT || c.sgs_filter == NIMBUS_SGS_FILTER_DX) &&
+                       std::isfinite(c.sgs_cs) && c.sgs_cs >= 0 && std::isfinite(c.sgs_pr_t) && c.sgs_pr_t > 0 &&
+                       std::isfinite(c.sgs_sc_t) && c.sgs_sc_t > 0 && std::isfinite(c.sgs_nu_max) &&
+                       c.sgs_nu_max >= 0 && std::isfinite(c.sgs_k_max) && c.sgs_k_max >= 0 &&
+                       std::isfinite(c.sgs_dt) && c.sgs_dt > 0 && c.dt / c.sgs_dt <= 1000.0));
 }
 
 struct Member
@@ -394,6 +401,36 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     sp.kessler = KesslerConst{c.qc0, c.c_auto, c.c_accr, c.evap_eff, c.vt_max};
     sp.sponge = c.sponge ? 1 : 0;
     sp.sponge_lateral = W;
+    // Smagorinsky SGS substeps: dt_sgs each, the remainder last (the same
+    // accumulation as the CPU reference's loop)
+    sp.sgs_n = 0;
+    if (c.sgs)
+    {
+        double done = 0.0, h = 0.0, first = 0.0;
+        while (done < c.dt && sp.sgs_n < 1000)
+        {
+            h = std::min(c.dt - done, c.sgs_dt);
+            if (!(h > 64.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, c.dt))) break;
+            if (sp.sgs_n == 0) first = h;
+            ++sp.sgs_n;
+            done += h;
+        }
+        sp.sgs_h = first;
+        sp.sgs_h_last = h;
+        const double delta = c.sgs_filter == NIMBUS_SGS_FILTER_DX ? std::sqrt(desc.grid.dx * desc.grid.dy)
+                                                                  : std::cbrt(desc.grid.dx * desc.grid.dy * desc.grid.dz);
+        sp.sgs_c2d2 = c.sgs_cs * c.sgs_cs * delta * delta;
+        sp.sgs_inv_pr = 1.0 / c.sgs_pr_t;
+        sp.sgs_inv_sc = 1.0 / c.sgs_sc_t;
+        sp.sgs_mode = c.sgs_mode;
+        sp.sgs_nu_max = c.sgs_nu_max;
+        sp.sgs_k_max = c.sgs_k_max;
+        sp.sgs_wrap_x = c.bc_west == NIMBUS_BC_PERIODIC && c.bc_east == NIMBUS_BC_PERIODIC;
+        sp.sgs_wrap_y = c.bc_south == NIMBUS_BC_PERIODIC && c.bc_north == NIMBUS_BC_PERIODIC;
+        const int lat[4] = {c.bc_west, c.bc_east, c.bc_south, c.bc_north};
+        for (int q = 0; q < 4; ++q)
+            sp.sgs_open[q] = lat[q] == NIMBUS_BC_RADIATIVE || lat[q] == NIMBUS_BC_ZERO_GRADIENT;
+    }
     sp.bounds = Bounds{};
     set_geometry(m, 64, 0, 1);
     return NIMBUS_OK;
@@ -427,6 +464,7 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
     b.view.max_kessler_sub = 0;
     b.view.any_sponge = 0;
     b.view.max_nz = 0;
+    b.view.max_sgs_n = 0;
     bool remap = true;
     for (Member* m : ms)
     {
@@ -435,6 +473,7 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
         b.view.max_kessler_sub = std::max(b.view.max_kessler_sub, m->args.p.kessler_sub);
         b.view.any_sponge = b.view.any_sponge || m->args.p.sponge;
         b.view.max_nz = std::max(b.view.max_nz, m->args.g.nz);
+        b.view.max_sgs_n = std::max(b.view.max_sgs_n, m->args.p.sgs_n);
         remap = remap && m->xcd_remap;
     }
     std::vector<int2> all;

@@ -230,9 +230,11 @@ FP64_SCALE = {"u": 30.0, "v": 30.0, "w": 10.0, "rho": 1.0, "p": 1e5, "theta": 30
               "qr": 1e-3}
 
 
-def check_fp64(out, ref, label, radiative):
+def check_fp64(out, ref, label, radiative, diffusive=False):
+    """diffusive: an SGS closure also carries a stash-rounded face velocity
+    into theta and the water species next to the radiative faces."""
     for f in L.FIELDS:
-        tol = (4 * 2.0 ** -23 if f in ("u", "v", "w") else 1e-8) if radiative else 1e-9
+        tol = (4 * 2.0 ** -23 if f in ("u", "v", "w") or diffusive else 1e-8) if radiative else 1e-9
         d = np.abs(np.asarray(out[f], np.float64) - np.asarray(ref[f], np.float64)) / FP64_SCALE[f]
         err = float(d.max())
         loc = np.unravel_index(int(np.argmax(d)), d.shape)
@@ -271,11 +273,37 @@ def test_large_step_with_sponges_fp64_matches_fp64_oracle(hip_ctx, mode):
     assert max_abs(plain["w"], ref["w"]) > 1e-3
 
 
+@pytest.mark.parametrize("sgs_mode", [L.SGS_HORIZONTAL, L.SGS_3D])
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_large_step_with_smagorinsky_fp64_matches_fp64_oracle(hip_ctx, mode, sgs_mode):
+    """Smagorinsky SGS after the microphysics (three SGS substeps of a 2.5 s
+    step with dt_sgs = 1 s: 1, 1, 0.5), then the EOS projection and sponges."""
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, wind_noise=2.0, dtype=np.float64)
+    sgs = dict(L.sgs_preset(), mode=sgs_mode)
+    cfg = L.step_config(2.5, 6, acoustic_mode=mode, lateral=L.BC_RIGID_SLIP, sponge=SMALL_SPONGE, sgs=sgs)
+    n_steps = 3
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
+    out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, n_steps)
+    check_fp64(out, ref, f"sgs {sgs_mode} mode {mode}", False)
+    plain, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(L.step_config(
+        2.5, 6, acoustic_mode=mode, lateral=L.BC_RIGID_SLIP, sponge=SMALL_SPONGE)), st, n_steps, np.float64)
+    assert max_abs(plain["u"], ref["u"]) > 1e-3  # the SGS did work in this case
+
+
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_large_step_with_smagorinsky_radiative_fp64(hip_ctx, mode):
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, wind_noise=2.0, dtype=np.float64)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode, sgs=L.sgs_preset())
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, 3, np.float64)
+    out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, 3)
+    check_fp64(out, ref, f"sgs radiative mode {mode}", True, diffusive=True)
+
+
 @pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
 @pytest.mark.parametrize("sponge", [None, SMALL_SPONGE])
 def test_large_step_mixed_close_to_fp64_oracle(hip_ctx, mode, sponge):
     g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, dtype=np.float64)
-    cfg = L.step_config(3.0, 6, acoustic_mode=mode, sponge=sponge)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode, sponge=sponge, sgs=L.sgs_preset() if sponge else None)
     n_steps = 4
     ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
     out, diag = _run_member(hip_ctx, g, base, cfg, {k: (v.astype(np.float32) if k in L.FIELDS else v)
