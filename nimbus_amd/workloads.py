@@ -9,11 +9,11 @@ c3  32-member ensemble of uneven grids drawn from seed 3 (nx = ny in {48, 64, 96
     minutes) with per-member sounding perturbations
 All use the analytic Weisman-Klemp-type sounding with a quarter-circle
 hodograph, +2 K bubbles of 10 km x 1.4 km radius centred 1.4 km above ground,
-and seeded uniform +-0.1 K theta noise. Unless a workload sets sponge=False,
-its large steps carry the Rayleigh sponges of the reference Kessler presets
-(upper sponge from 12 km, 8-cell w absorber, 10 km lateral band; see
-lib.sponge_preset), without which hour-long runs reflect gravity waves off the
-lid and the lateral boundaries.
+and seeded uniform +-0.1 K theta noise. Unless a workload sets sponge=False
+or sgs=False, its large steps carry the Rayleigh sponges (upper sponge from
+12 km, 8-cell w absorber, 10 km lateral band; lib.sponge_preset) and the
+horizontal Smagorinsky closure (lib.sgs_preset) of the reference Kessler
+presets.
 """
 import dataclasses
 
@@ -38,6 +38,7 @@ class Workload:
     member: int = 0
     sounding: dict = dataclasses.field(default_factory=dict)
     sponge: bool = True
+    sgs: bool = True
 
     @property
     def grid(self):
@@ -50,6 +51,8 @@ class Workload:
     def config(self, acoustic_mode=L.ACOUSTIC_EXPLICIT, **kw):
         if self.sponge:
             kw.setdefault("sponge", L.sponge_preset(self.dx))
+        if self.sgs:
+            kw.setdefault("sgs", L.sgs_preset())
         return L.step_config(self.dt, self.substeps, acoustic_mode=acoustic_mode, **kw)
 
     def sounding_struct(self):
