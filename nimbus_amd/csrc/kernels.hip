@@ -1604,22 +1604,21 @@ __global__ __launch_bounds__(kBlock) void k_scalars_home(const MemberArgs* __res
 template <class S>
 struct SgsView
 {
+    using T = typename AccT<S>::type;  // FP32 arithmetic for FP32 storage, FP64 for FP64
     gptr<const S> u, v, w, rho_p, th, qv;
-    const double* rho0;  // base slot, global
-    DevGrid g;
+    gptr<const double> rho0;
+    int nx, ny, nz, px, plane;
     int wx, wy;
-    __device__ int ip(int i) const { return wx ? (i <= 1 ? g.nx - 2 : min(i - 1, g.nx - 2)) : max(i - 1, 0); }
-    __device__ int in(int i) const { return wx ? (i >= g.nx - 2 ? 1 : max(i + 1, 1)) : min(i + 1, g.nx - 1); }
-    __device__ int jp(int j) const { return wy ? (j <= 1 ? g.ny - 2 : min(j - 1, g.ny - 2)) : max(j - 1, 0); }
-    __device__ int jn(int j) const { return wy ? (j >= g.ny - 2 ? 1 : max(j + 1, 1)) : min(j + 1, g.ny - 1); }
-    __device__ double U(int i, int j, int k) const { return double(u[g.at(i, j, k)]); }
-    __device__ double V(int i, int j, int k) const { return double(v[g.at(i, j, k)]); }
-    __device__ double W(int i, int j, int k) const { return k < 0 ? 0.0 : double(w[g.at(i, j, k)]); }
+    __device__ int at(int i, int j, int k) const { return k * plane + j * px + i; }
+    __device__ int ip(int i) const { return wx ? (i <= 1 ? nx - 2 : min(i - 1, nx - 2)) : max(i - 1, 0); }
+    __device__ int in(int i) const { return wx ? (i >= nx - 2 ? 1 : max(i + 1, 1)) : min(i + 1, nx - 1); }
+    __device__ int jp(int j) const { return wy ? (j <= 1 ? ny - 2 : min(j - 1, ny - 2)) : max(j - 1, 0); }
+    __device__ int jn(int j) const { return wy ? (j >= ny - 2 ? 1 : max(j + 1, 1)) : min(j + 1, ny - 1); }
+    __device__ T U(int i, int j, int k) const { return T(u[at(i, j, k)]); }
+    __device__ T V(int i, int j, int k) const { return T(v[at(i, j, k)]); }
+    __device__ T W(int i, int j, int k) const { return k < 0 ? T(0) : T(w[at(i, j, k)]); }
     // live density: rho0 + rho', rounded to storage as the reference's whole-value field
-    __device__ double R(int i, int j, int k) const
-    {
-        return double(S(slot(rho0, kBaseRho0, g.nz)[k] + double(rho_p[g.at(i, j, k)])));
-    }
+    __device__ T R(int i, int j, int k) const { return T(S(rho0[k] + double(rho_p[at(i, j, k)]))); }
 };
 
 template <class S>
@@ -1628,201 +1627,327 @@ __device__ SgsView<S> sgs_view(const MemberArgs& A)
     const DevFields& f = A.f;
     SgsView<S> s;
     s.u = P<S>(f.u); s.v = P<S>(f.v); s.w = P<S>(f.w); s.rho_p = P<S>(f.rp); s.th = P<S>(f.th); s.qv = P<S>(f.qv);
-    s.rho0 = f.base;
-    s.g = A.g;
+    s.rho0 = slot(f.base, kBaseRho0, A.g.nz);
+    s.nx = A.g.nx; s.ny = A.g.ny; s.nz = A.g.nz; s.px = A.g.px; s.plane = int(A.g.plane());
     s.wx = A.p.sgs_wrap_x;
     s.wy = A.p.sgs_wrap_y;
     return s;
 }
 
+// |S|^2 at the centre of cell (i, j, k) (eddy_viscosity.cpp:1253-1400). The
+// neighbour indices are the reference's (clamped, or wrapped on periodic
+// axes) and every value is loaded before the arithmetic; edge strains of
+// edges on or above the lid and below the ground are zero.
 template <class S>
-__device__ double sgs_strain_sq(const SgsView<S>& s, int mode, int i, int j, int k)
+__device__ typename AccT<S>::type sgs_strain_sq(const SgsView<S>& s, int mode, typename AccT<S>::type idx,
+                                                typename AccT<S>::type idy, typename AccT<S>::type idz, int i, int j,
+                                                int k)
 {
-    const int NZ = s.g.nz;
-    const double dx = s.g.dx, dy = s.g.dy, dz = s.g.dz;
+    using T = typename AccT<S>::type;
+    const int NZ = s.nz;
+    const T h = T(0.5), q = T(0.25), two = T(2);
     const int im = s.ip(i), jm = s.jp(j);
-    const double S11 = (s.U(i, j, k) - s.U(im, j, k)) / dx;
-    const double S22 = (s.V(i, j, k) - s.V(i, jm, k)) / dy;
-    const double S33 = (s.W(i, j, k) - s.W(i, j, k - 1)) / dz;
-    auto s12 = [&](int ii, int jj) {
-        const int jj1 = s.jn(jj), ii1 = s.in(ii);
-        return 0.5 * ((s.U(ii, jj1, k) - s.U(ii, jj, k)) / dy + (s.V(ii1, jj, k) - s.V(ii, jj, k)) / dx);
+    const int ii[2] = {im, i}, jj[2] = {jm, j};
+    const bool kv[2] = {k >= 1, k <= NZ - 2};  // edges k-1/2 and k+1/2 exist
+    const int kl[2] = {k >= 1 ? k - 1 : 0, k <= NZ - 2 ? k : NZ - 2};
+    T u0 = s.U(i, j, k), uw = s.U(im, j, k), v0 = s.V(i, j, k), vs = s.V(i, jm, k);
+    T w0 = T(s.w[s.at(i, j, k)]), wb = T(s.w[s.at(i, j, k >= 1 ? k - 1 : 0)]);
+    T a12[2][2][4];  // s12 operands: u(ii, jn(jj)), u(ii, jj), v(in(ii), jj), v(ii, jj)
+    T xzv[2][2][4];  // xz operands: u(ii, kk+1), u(ii, kk), w(in(ii), kk), w(ii, kk)
+    T yzv[2][2][4];  // yz operands: v(jj, kk+1), v(jj, kk), w(jn(jj), kk), w(jj, kk)
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+        {
+            a12[x][y][0] = s.U(ii[x], s.jn(jj[y]), k);
+            a12[x][y][1] = s.U(ii[x], jj[y], k);
+            a12[x][y][2] = s.V(s.in(ii[x]), jj[y], k);
+            a12[x][y][3] = s.V(ii[x], jj[y], k);
+        }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int z = 0; z < 2; ++z)
+        {
+            const int kk = kl[z];
+            xzv[x][z][0] = s.U(ii[x], j, kk + 1);
+            xzv[x][z][1] = s.U(ii[x], j, kk);
+            xzv[x][z][2] = T(s.w[s.at(s.in(ii[x]), j, kk)]);
+            xzv[x][z][3] = T(s.w[s.at(ii[x], j, kk)]);
+            yzv[x][z][0] = s.V(i, jj[x], kk + 1);
+            yzv[x][z][1] = s.V(i, jj[x], kk);
+            yzv[x][z][2] = T(s.w[s.at(i, s.jn(jj[x]), kk)]);
+            yzv[x][z][3] = T(s.w[s.at(i, jj[x], kk)]);
+        }
+    pin(u0); pin(uw); pin(v0); pin(vs); pin(w0); pin(wb);
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int o = 0; o < 4; ++o)
+            {
+                pin(a12[x][y][o]);
+                pin(xzv[x][y][o]);
+                pin(yzv[x][y][o]);
+            }
+    const T S11 = (u0 - uw) * idx;
+    const T S22 = (v0 - vs) * idy;
+    const T S33 = (w0 - (k >= 1 ? wb : T(0))) * idz;
+    auto s12 = [&](int x, int y) {
+        return h * ((a12[x][y][0] - a12[x][y][1]) * idy + (a12[x][y][2] - a12[x][y][3]) * idx);
     };
     struct G2
     {
-        double a, b;
+        T a, b;
     };
-    auto xz = [&](int ii, int kk) {
-        if (kk < 0 || kk >= NZ - 1) return G2{0.0, 0.0};
-        const int ii1 = s.in(ii);
-        return G2{(s.U(ii, j, kk + 1) - s.U(ii, j, kk)) / dz, (s.W(ii1, j, kk) - s.W(ii, j, kk)) / dx};
+    auto xz = [&](int x, int z) {
+        if (!kv[z]) return G2{T(0), T(0)};
+        return G2{(xzv[x][z][0] - xzv[x][z][1]) * idz, (xzv[x][z][2] - xzv[x][z][3]) * idx};
     };
-    auto yz = [&](int jj, int kk) {
-        if (kk < 0 || kk >= NZ - 1) return G2{0.0, 0.0};
-        const int jj1 = s.jn(jj);
-        return G2{(s.V(i, jj, kk + 1) - s.V(i, jj, kk)) / dz, (s.W(i, jj1, kk) - s.W(i, jj, kk)) / dy};
+    auto yz = [&](int y, int z) {
+        if (!kv[z]) return G2{T(0), T(0)};
+        return G2{(yzv[y][z][0] - yzv[y][z][1]) * idz, (yzv[y][z][2] - yzv[y][z][3]) * idy};
     };
-    const double a = s12(im, jm), b = s12(im, j), c = s12(i, jm), d = s12(i, j);
-    const double S12_sq = 0.25 * (a * a + b * b + c * c + d * d);
-    const G2 xa = xz(im, k - 1), xb = xz(im, k), xc = xz(i, k - 1), xd = xz(i, k);
-    const G2 ya = yz(jm, k - 1), yb = yz(jm, k), yc = yz(j, k - 1), yd = yz(j, k);
-    auto avg_b = [](G2 p, G2 q, G2 r, G2 t) { return 0.25 * (p.b * p.b + q.b * q.b + r.b * r.b + t.b * t.b); };
-    auto avg_a = [](G2 p, G2 q, G2 r, G2 t) { return 0.25 * (p.a * p.a + q.a * q.a + r.a * r.a + t.a * t.a); };
+    const T a = s12(0, 0), b = s12(0, 1), c = s12(1, 0), d = s12(1, 1);
+    const T S12_sq = q * (a * a + b * b + c * c + d * d);
+    // edges: (west or centre column) x (below k: z = 0, above: z = 1)
+    const G2 xa = xz(0, 0), xb = xz(0, 1), xc = xz(1, 0), xd = xz(1, 1);
+    const G2 ya = yz(0, 0), yb = yz(0, 1), yc = yz(1, 0), yd = yz(1, 1);
+    auto avg_b = [&](G2 p, G2 r, G2 t, G2 z) { return q * (p.b * p.b + r.b * r.b + t.b * t.b + z.b * z.b); };
+    auto avg_a = [&](G2 p, G2 r, G2 t, G2 z) { return q * (p.a * p.a + r.a * r.a + t.a * t.a + z.a * z.a); };
     if (mode == NIMBUS_SGS_HORIZONTAL)
-        return 2.0 * (S11 * S11 + S22 * S22) + 4.0 * S12_sq + avg_b(xa, xb, xc, xd) + avg_b(ya, yb, yc, yd);
-    if (mode == NIMBUS_SGS_VERTICAL) return avg_a(xa, xb, xc, xd) + avg_a(ya, yb, yc, yd) + 2.0 * S33 * S33;
-    auto sym = [](G2 p) {
-        const double e = 0.5 * (p.a + p.b);
+        return two * (S11 * S11 + S22 * S22) + T(4) * S12_sq + avg_b(xa, xb, xc, xd) + avg_b(ya, yb, yc, yd);
+    if (mode == NIMBUS_SGS_VERTICAL) return avg_a(xa, xb, xc, xd) + avg_a(ya, yb, yc, yd) + two * S33 * S33;
+    auto sym = [&](G2 p) {
+        const T e = h * (p.a + p.b);
         return e * e;
     };
-    const double S13_sq = 0.25 * (sym(xa) + sym(xb) + sym(xc) + sym(xd));
-    const double S23_sq = 0.25 * (sym(ya) + sym(yb) + sym(yc) + sym(yd));
-    return 2.0 * (S11 * S11 + S22 * S22 + S33 * S33) + 4.0 * (S12_sq + S13_sq + S23_sq);
+    const T S13_sq = q * (sym(xa) + sym(xb) + sym(xc) + sym(xd));
+    const T S23_sq = q * (sym(ya) + sym(yb) + sym(yc) + sym(yd));
+    return two * (S11 * S11 + S22 * S22 + S33 * S33) + T(4) * (S12_sq + S13_sq + S23_sq);
 }
+
+// 1 / spacing in the arithmetic type: FP64 divides by the spacing as the
+// reference does; FP32 multiplies by the rounded reciprocal.
+template <class T>
+struct Inv
+{
+    T x, y, z;
+};
 
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_sgs_closure(const MemberArgs* __restrict__ ma,
                                                         const int2* __restrict__ map, int n)
 {
+    using T = typename AccT<S>::type;
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     if (n >= A.p.sgs_n) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
     const SgsView<S> sv = sgs_view<S>(A);
-    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
-    const double nu = fmin(A.p.sgs_c2d2 * sqrt(sgs_strain_sq<S>(sv, A.p.sgs_mode, cl.i, cl.j, cl.k)), A.p.sgs_nu_max);
+    const int c = sv.at(cl.i, cl.j, cl.k);
+    T str;
+    if constexpr (sizeof(T) == sizeof(double))
+        str = sgs_strain_sq<S>(sv, A.p.sgs_mode, A.g.idx, A.g.idy, A.g.idz, cl.i, cl.j, cl.k);
+    else
+        str = sgs_strain_sq<S>(sv, A.p.sgs_mode, T(A.g.idx), T(A.g.idy), T(A.g.idz), cl.i, cl.j, cl.k);
+    const double nu = fmin(A.p.sgs_c2d2 * sqrt(double(str)), A.p.sgs_nu_max);
     P<S>(A.f.k1u)[c] = S(nu);
     P<S>(A.f.k1v)[c] = S(fmin(nu * A.p.sgs_inv_pr, A.p.sgs_k_max));
     P<S>(A.f.k1w)[c] = S(fmin(nu * A.p.sgs_inv_sc, A.p.sgs_k_max));
 }
 
+// Momentum tendencies (eddy_viscosity.cpp:1532-1948) into u2, v2, w2. Every
+// operand is one of the 27 cells around (i, j, k) reached by the reference's
+// neighbour rules (west iw = prev(i), east ie = next(i), likewise in y; next
+// of prev is the cell itself on every face these loops visit); the 59 the
+// formulas use are loaded before any arithmetic.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_sgs_tendency(const MemberArgs* __restrict__ ma,
+__global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __restrict__ ma,
                                                          const int2* __restrict__ map, int n)
 {
+    using T = typename AccT<S>::type;
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     if (n >= A.p.sgs_n) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
     const SgsView<S> s = sgs_view<S>(A);
-    const DevGrid& g = A.g;
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const int NX = s.nx, NY = s.ny, NZ = s.nz;
     const int i = cl.i, j = cl.j, k = cl.k;
-    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    const int c = s.at(i, j, k);
+    const bool has_u = i >= 1 && i <= (s.wx ? NX - 2 : NX - 3) && j >= 1 && j <= NY - 2 && k < NZ - 1;
+    const bool has_v = i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && (s.wy || j < NY - 2) && k < NZ - 1;
+    const bool has_w = i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && k < NZ - 1;
+    if (!(has_u || has_v || has_w))
+    {
+        P<S>(A.f.u2)[c] = S(0);
+        P<S>(A.f.v2)[c] = S(0);
+        P<S>(A.f.w2)[c] = S(0);
+        return;
+    }
+    const int iw = s.ip(i), ie = s.in(i), js = s.jp(j), jn = s.jn(j);
+    const int km = k > 0 ? k - 1 : k, kp = k + 1;  // k <= NZ - 2 here
+    const bool lo = k > 0;                          // a face below exists
+    // cell offsets: X[0..2] = iw, i, ie; Y[0..2] = js, j, jn; Z[0..2] = km, k, kp
+    auto at3 = [&](int a, int b, int cc) {
+        const int X = a == 0 ? iw : (a == 1 ? i : ie);
+        const int Y = b == 0 ? js : (b == 1 ? j : jn);
+        const int Z = cc == 0 ? km : (cc == 1 ? k : kp);
+        return s.at(X, Y, Z);
+    };
+    const gptr<const S> nuf = P<S>(A.f.k1u);
+    // U, V, W at the nine cells each equation uses; nu and rho' at sixteen
+    T u_ = T(s.u[at3(1, 1, 1)]), u_e = T(s.u[at3(2, 1, 1)]), u_w = T(s.u[at3(0, 1, 1)]);
+    T u_n = T(s.u[at3(1, 2, 1)]), u_s = T(s.u[at3(1, 0, 1)]), u_wn = T(s.u[at3(0, 2, 1)]);
+    T u_t = T(s.u[at3(1, 1, 2)]), u_b = T(s.u[at3(1, 1, 0)]), u_wt = T(s.u[at3(0, 1, 2)]);
+    T v_ = T(s.v[at3(1, 1, 1)]), v_n = T(s.v[at3(1, 2, 1)]), v_s = T(s.v[at3(1, 0, 1)]);
+    T v_e = T(s.v[at3(2, 1, 1)]), v_w = T(s.v[at3(0, 1, 1)]), v_es = T(s.v[at3(2, 0, 1)]);
+    T v_t = T(s.v[at3(1, 1, 2)]), v_b = T(s.v[at3(1, 1, 0)]), v_st = T(s.v[at3(1, 0, 2)]);
+    T w_ = T(s.w[at3(1, 1, 1)]), w_e = T(s.w[at3(2, 1, 1)]), w_w = T(s.w[at3(0, 1, 1)]);
+    T w_n = T(s.w[at3(1, 2, 1)]), w_s = T(s.w[at3(1, 0, 1)]), w_t = T(s.w[at3(1, 1, 2)]);
+    T w_b = T(s.w[at3(1, 1, 0)]), w_eb = T(s.w[at3(2, 1, 0)]), w_nb = T(s.w[at3(1, 2, 0)]);
+    constexpr int kMu = 16;
+    const int mu_pts[kMu][3] = {{1, 1, 1}, {2, 1, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {2, 2, 1}, {2, 0, 1}, {0, 2, 1},
+                                {1, 1, 2}, {1, 1, 0}, {2, 1, 2}, {2, 1, 0}, {0, 1, 2}, {1, 2, 2}, {1, 2, 0}, {1, 0, 2}};
+    T nu_[kMu], rp_[kMu];
+#pragma unroll
+    for (int q = 0; q < kMu; ++q)
+    {
+        const int idx = at3(mu_pts[q][0], mu_pts[q][1], mu_pts[q][2]);
+        nu_[q] = T(nuf[idx]);
+        rp_[q] = T(s.rho_p[idx]);
+    }
+    const double r0k = s.rho0[k], r0m = s.rho0[km], r0p = s.rho0[kp];
+    pin(u_); pin(u_e); pin(u_w); pin(u_n); pin(u_s); pin(u_wn); pin(u_t); pin(u_b); pin(u_wt);
+    pin(v_); pin(v_n); pin(v_s); pin(v_e); pin(v_w); pin(v_es); pin(v_t); pin(v_b); pin(v_st);
+    pin(w_); pin(w_e); pin(w_w); pin(w_n); pin(w_s); pin(w_t); pin(w_b); pin(w_eb); pin(w_nb);
+    // live density and rho nu at the sixteen cells (density rounded to storage as the reference's field)
+    T R_[kMu], M_[kMu];
+#pragma unroll
+    for (int q = 0; q < kMu; ++q)
+    {
+        const double r0 = mu_pts[q][2] == 1 ? r0k : (mu_pts[q][2] == 2 ? r0p : r0m);
+        R_[q] = T(S(r0 + double(rp_[q])));
+        M_[q] = R_[q] * nu_[q];
+    }
+    // named mu = rho nu: c, e, w, n, s, en, es, wn, t, b, et, eb, wt, nt, nb, st
+    const T Mc = M_[0], Me = M_[1], Mw = M_[2], Mn = M_[3], Ms = M_[4], Men = M_[5], Mes = M_[6], Mwn = M_[7];
+    const T Mt = M_[8], Mb = M_[9], Met = M_[10], Meb = M_[11], Mwt = M_[12], Mnt = M_[13], Mnb = M_[14], Mst = M_[15];
+    const T dx = T(A.g.dx), dy = T(A.g.dy), dz = T(A.g.dz);
+    const T h = T(0.5), q4 = T(0.25), two = T(2);
+    // x / d: a division in FP64 (as the reference), a multiply by 1/d in FP32
+    const T rdx = T(1) / dx, rdy = T(1) / dy, rdz = T(1) / dz;
+    auto ovx = [&](T x) { return sizeof(T) == sizeof(double) ? x / dx : x * rdx; };
+    auto ovy = [&](T x) { return sizeof(T) == sizeof(double) ? x / dy : x * rdy; };
+    auto ovz = [&](T x) { return sizeof(T) == sizeof(double) ? x / dz : x * rdz; };
     const bool use_h = A.p.sgs_mode != NIMBUS_SGS_VERTICAL, use_v = A.p.sgs_mode != NIMBUS_SGS_HORIZONTAL;
-    const gptr<const S> nu = P<S>(A.f.k1u), kth = P<S>(A.f.k1v), kq = P<S>(A.f.k1w);
-    auto MU = [&](int a, int b, int cc) { return s.R(a, b, cc) * double(nu[g.at(a, b, cc)]); };
-    auto mu_xy = [&](int a, int b, int cc) {
-        const int a1 = s.in(a), b1 = s.jn(b);
-        return 0.25 * (MU(a, b, cc) + MU(a1, b, cc) + MU(a, b1, cc) + MU(a1, b1, cc));
-    };
-    auto mu_xz = [&](int a, int b, int cc) {
-        const int a1 = s.in(a);
-        const double lower = 0.5 * (MU(a, b, cc) + MU(a1, b, cc)), upper = 0.5 * (MU(a, b, cc + 1) + MU(a1, b, cc + 1));
-        return (dz * lower + dz * upper) / (dz + dz);
-    };
-    auto mu_yz = [&](int a, int b, int cc) {
-        const int b1 = s.jn(b);
-        const double lower = 0.5 * (MU(a, b, cc) + MU(a, b1, cc)), upper = 0.5 * (MU(a, b, cc + 1) + MU(a, b1, cc + 1));
-        return (dz * lower + dz * upper) / (dz + dz);
-    };
-    const std::size_t c = g.at(i, j, k);
-    // du/dt at the x-face i (eddy_viscosity.cpp:1676-1750)
-    double du = 0.0;
-    if (i >= 1 && i <= (s.wx ? NX - 2 : NX - 3) && j >= 1 && j <= NY - 2 && k < NZ - 1)
+    const T wb = lo ? w_b : T(0);   // w below the cell (zero at the ground)
+    const T web = lo ? w_eb : T(0);
+    const T wnb = lo ? w_nb : T(0);
+    auto vert_mean = [&](T lower, T upper) { return (dz * lower + dz * upper) / (dz + dz); };
+    T du = T(0);
+    if (has_u)
     {
-        const int jm = s.jp(j), im = s.ip(i), ipp = s.in(i);
-        double div = 0.0;
+        T div = T(0);
         if (use_h)
         {
-            const double S11_r = (s.U(ipp, j, k) - s.U(i, j, k)) / dx, S11_l = (s.U(i, j, k) - s.U(im, j, k)) / dx;
-            div += (2.0 * MU(ipp, j, k) * S11_r - 2.0 * MU(i, j, k) * S11_l) / dx;
-            auto T12 = [&](int je) {
-                const int i1 = s.in(i), je1 = s.jn(je);
-                const double S12 =
-                    0.5 * ((s.U(i, je1, k) - s.U(i, je, k)) / dy + (s.V(i1, je, k) - s.V(i, je, k)) / dx);
-                return 2.0 * mu_xy(i, je, k) * S12;
-            };
-            div += (T12(j) - T12(jm)) / dy;
+            const T S11_r = ovx((u_e - u_)), S11_l = ovx((u_ - u_w));
+            div += ovx((two * Me * S11_r - two * Mc * S11_l));
+            const T T12n = two * (q4 * (Mc + Me + Mn + Men)) * (h * (ovy((u_n - u_)) + ovx((v_e - v_))));
+            const T T12s = two * (q4 * (Ms + Mes + Mc + Me)) * (h * (ovy((u_ - u_s)) + ovx((v_es - v_s))));
+            div += ovy((T12n - T12s));
         }
         if (use_v)
         {
-            auto T13 = [&](int ke) {
-                if (ke < 0 || ke >= NZ - 1) return 0.0;
-                const double dudz = (s.U(i, j, ke + 1) - s.U(i, j, ke)) / dz;
-                const double sg = use_h ? dudz + (s.W(ipp, j, ke) - s.W(i, j, ke)) / dx : dudz;
-                return mu_xz(i, j, ke) * sg;
-            };
-            div += (T13(k) - T13(k - 1)) / dz;
+            const T T13t = vert_mean(h * (Mc + Me), h * (Mt + Met)) *
+                           (ovz((u_t - u_)) + (use_h ? ovx((w_e - w_)) : T(0)));
+            const T T13b = lo ? vert_mean(h * (Mb + Meb), h * (Mc + Me)) *
+                                    (ovz((u_ - u_b)) + (use_h ? ovx((web - wb)) : T(0)))
+                              : T(0);
+            div += ovz((T13t - T13b));
         }
-        du = div / (0.5 * (s.R(i, j, k) + s.R(s.in(i), j, k)));
+        du = div / (h * (R_[0] + R_[1]));
     }
-    // dv/dt at the y-face j (1752-1826)
-    double dv = 0.0;
-    if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && (s.wy || j < NY - 2) && k < NZ - 1)
+    T dv = T(0);
+    if (has_v)
     {
-        const int jm = s.jp(j), jpp = s.jn(j);
-        double div = 0.0;
+        T div = T(0);
         if (use_h)
         {
-            const double S22_n = (s.V(i, jpp, k) - s.V(i, j, k)) / dy, S22_s = (s.V(i, j, k) - s.V(i, jm, k)) / dy;
-            div += (2.0 * MU(i, jpp, k) * S22_n - 2.0 * MU(i, j, k) * S22_s) / dy;
-            auto T21 = [&](int ie) {
-                const int ie1 = s.in(ie), j1 = s.jn(j);
-                const double S12 =
-                    0.5 * ((s.U(ie, j1, k) - s.U(ie, j, k)) / dy + (s.V(ie1, j, k) - s.V(ie, j, k)) / dx);
-                return 2.0 * mu_xy(ie, j, k) * S12;
-            };
-            div += (T21(i) - T21(s.ip(i))) / dx;
+            const T S22_n = ovy((v_n - v_)), S22_s = ovy((v_ - v_s));
+            div += ovy((two * Mn * S22_n - two * Mc * S22_s));
+            const T T21e = two * (q4 * (Mc + Me + Mn + Men)) * (h * (ovy((u_n - u_)) + ovx((v_e - v_))));
+            const T T21w = two * (q4 * (Mw + Mc + Mwn + Mn)) * (h * (ovy((u_wn - u_w)) + ovx((v_ - v_w))));
+            div += ovx((T21e - T21w));
         }
         if (use_v)
         {
-            auto T23 = [&](int ke) {
-                if (ke < 0 || ke >= NZ - 1) return 0.0;
-                const int j1 = s.jn(j);
-                const double dvdz = (s.V(i, j, ke + 1) - s.V(i, j, ke)) / dz;
-                const double sg = use_h ? dvdz + (s.W(i, j1, ke) - s.W(i, j, ke)) / dy : dvdz;
-                return mu_yz(i, j, ke) * sg;
-            };
-            div += (T23(k) - T23(k - 1)) / dz;
+            const T T23t = vert_mean(h * (Mc + Mn), h * (Mt + Mnt)) *
+                           (ovz((v_t - v_)) + (use_h ? ovy((w_n - w_)) : T(0)));
+            const T T23b = lo ? vert_mean(h * (Mb + Mnb), h * (Mc + Mn)) *
+                                    (ovz((v_ - v_b)) + (use_h ? ovy((wnb - wb)) : T(0)))
+                              : T(0);
+            div += ovz((T23t - T23b));
         }
-        dv = div / (0.5 * (s.R(i, j, k) + s.R(i, s.jn(j), k)));
+        dv = div / (h * (R_[0] + R_[3]));
     }
-    // dw/dt at the z-face k (1828-1945)
-    double dw = 0.0;
-    if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && k < NZ - 1)
+    T dw = T(0);
+    if (has_w)
     {
-        const double mass = 0.5 * (s.R(i, j, k) * dz + s.R(i, j, k + 1) * dz);
-        double force = 0.0;
+        const T mass = h * (R_[0] * dz + R_[8] * dz);
+        T force = T(0);
         if (use_v)
         {
-            const double T33_t = 2.0 * MU(i, j, k + 1) * ((s.W(i, j, k + 1) - s.W(i, j, k)) / dz);
-            const double T33_b = 2.0 * MU(i, j, k) * ((s.W(i, j, k) - s.W(i, j, k - 1)) / dz);
+            const T T33_t = two * Mt * (ovz((w_t - w_)));
+            const T T33_b = two * Mc * (ovz((w_ - wb)));
             force += T33_t - T33_b;
         }
         if (use_h)
         {
-            auto T31 = [&](int ie) {
-                const int ie1 = s.in(ie);
-                const double dwdx = (s.W(ie1, j, k) - s.W(ie, j, k)) / dx;
-                const double sg = use_v ? (s.U(ie, j, k + 1) - s.U(ie, j, k)) / dz + dwdx : dwdx;
-                return mu_xz(ie, j, k) * sg;
-            };
-            force += dz * (T31(i) - T31(s.ip(i))) / dx;
-            auto T32 = [&](int je) {
-                const int je1 = s.jn(je);
-                const double dwdy = (s.W(i, je1, k) - s.W(i, je, k)) / dy;
-                const double sg = use_v ? (s.V(i, je, k + 1) - s.V(i, je, k)) / dz + dwdy : dwdy;
-                return mu_yz(i, je, k) * sg;
-            };
-            force += dz * (T32(j) - T32(s.jp(j))) / dy;
+            const T T31e = vert_mean(h * (Mc + Me), h * (Mt + Met)) *
+                           ((use_v ? ovz((u_t - u_)) : T(0)) + ovx((w_e - w_)));
+            const T T31w = vert_mean(h * (Mw + Mc), h * (Mwt + Mt)) *
+                           ((use_v ? ovz((u_wt - u_w)) : T(0)) + ovx((w_ - w_w)));
+            force += ovx(dz * (T31e - T31w));
+            const T T32n = vert_mean(h * (Mc + Mn), h * (Mt + Mnt)) *
+                           ((use_v ? ovz((v_t - v_)) : T(0)) + ovy((w_n - w_)));
+            const T T32s = vert_mean(h * (Ms + Mc), h * (Mst + Mt)) *
+                           ((use_v ? ovz((v_st - v_s)) : T(0)) + ovy((w_ - w_s)));
+            force += ovy(dz * (T32n - T32s));
         }
         dw = force / mass;
     }
-    // d(theta)/dt and d(qv)/dt (1953-2090)
-    double dth = 0.0, dqv = 0.0;
+    P<S>(A.f.u2)[c] = S(du);
+    P<S>(A.f.v2)[c] = S(dv);
+    P<S>(A.f.w2)[c] = S(dw);
+}
+
+// Scalar tendencies of theta and qv (eddy_viscosity.cpp:1953-2090) into rp2, mfx.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __restrict__ ma,
+                                                        const int2* __restrict__ map, int n)
+{
+    using T = typename AccT<S>::type;
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (n >= A.p.sgs_n) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const SgsView<S> s = sgs_view<S>(A);
+    const int NX = s.nx, NY = s.ny, NZ = s.nz;
+    const int i = cl.i, j = cl.j, k = cl.k;
+    const T dx = T(A.g.dx), dy = T(A.g.dy), dz = T(A.g.dz), h = T(0.5);
+    const T rdx = T(1) / dx, rdy = T(1) / dy, rdz = T(1) / dz;
+    auto ovx = [&](T x) { return sizeof(T) == sizeof(double) ? x / dx : x * rdx; };
+    auto ovy = [&](T x) { return sizeof(T) == sizeof(double) ? x / dy : x * rdy; };
+    auto ovz = [&](T x) { return sizeof(T) == sizeof(double) ? x / dz : x * rdz; };
+    const bool use_h = A.p.sgs_mode != NIMBUS_SGS_VERTICAL, use_v = A.p.sgs_mode != NIMBUS_SGS_HORIZONTAL;
+    const int c = s.at(i, j, k);
+    T dth = T(0), dqv = T(0);
     if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2)
     {
         const bool ow = A.p.sgs_open[0], oe = A.p.sgs_open[1], os = A.p.sgs_open[2], on = A.p.sgs_open[3];
@@ -1830,34 +1955,46 @@ __global__ __launch_bounds__(kBlock) void k_sgs_tendency(const MemberArgs* __res
         const int jpp = !s.wy && !on && j == NY - 2 ? j : s.jn(j);
         const int im = !s.wx && !ow && i == 1 ? i : s.ip(i);
         const int ipp = !s.wx && !oe && i == NX - 2 ? i : s.in(i);
-        auto scalar = [&](gptr<const S> K, gptr<const S> phi) {
-            auto T = [&](int a, int b, int cc) { return s.R(a, b, cc) * double(K[g.at(a, b, cc)]); };
-            auto Ph = [&](int a, int b, int cc) { return double(phi[g.at(a, b, cc)]); };
-            double div = 0.0;
+        // the seven-point star, loaded before any arithmetic (vertical neighbours clamped, masked below)
+        const int km = k > 0 ? k - 1 : k, kp = k < NZ - 1 ? k + 1 : k;
+        const int pts[7] = {c, s.at(ipp, j, k), s.at(im, j, k), s.at(i, jpp, k), s.at(i, jm, k), s.at(i, j, kp),
+                            s.at(i, j, km)};
+        const int kk[7] = {k, k, k, k, k, kp, km};
+        const gptr<const S> K1 = P<S>(A.f.k1v), K2 = P<S>(A.f.k1w);
+        T r[7], k1[7], k2[7], p1[7], p2[7];
+#pragma unroll
+        for (int o = 0; o < 7; ++o)
+        {
+            r[o] = T(S(s.rho0[kk[o]] + double(s.rho_p[pts[o]])));
+            k1[o] = T(K1[pts[o]]);
+            k2[o] = T(K2[pts[o]]);
+            p1[o] = T(s.th[pts[o]]);
+            p2[o] = T(s.qv[pts[o]]);
+        }
+        auto scalar = [&](const T* K, const T* P_) {
+            T tr[7];
+#pragma unroll
+            for (int o = 0; o < 7; ++o) tr[o] = r[o] * K[o];
+            T div = T(0);
             if (use_h)
             {
-                const double Fe = 0.5 * (T(i, j, k) + T(ipp, j, k)) * (Ph(ipp, j, k) - Ph(i, j, k)) / dx;
-                const double Fw = 0.5 * (T(im, j, k) + T(i, j, k)) * (Ph(i, j, k) - Ph(im, j, k)) / dx;
-                const double Fn = 0.5 * (T(i, j, k) + T(i, jpp, k)) * (Ph(i, jpp, k) - Ph(i, j, k)) / dy;
-                const double Fs = 0.5 * (T(i, jm, k) + T(i, j, k)) * (Ph(i, j, k) - Ph(i, jm, k)) / dy;
-                div += (Fe - Fw) / dx + (Fn - Fs) / dy;
+                const T Fe = ovx(h * (tr[0] + tr[1]) * (P_[1] - P_[0]));
+                const T Fw = ovx(h * (tr[2] + tr[0]) * (P_[0] - P_[2]));
+                const T Fn = ovy(h * (tr[0] + tr[3]) * (P_[3] - P_[0]));
+                const T Fs = ovy(h * (tr[4] + tr[0]) * (P_[0] - P_[4]));
+                div += ovx((Fe - Fw)) + ovy((Fn - Fs));
             }
             if (use_v)
             {
-                const double Ft =
-                    k < NZ - 1 ? 0.5 * (T(i, j, k) + T(i, j, k + 1)) * (Ph(i, j, k + 1) - Ph(i, j, k)) / dz : 0.0;
-                const double Fb =
-                    k > 0 ? 0.5 * (T(i, j, k - 1) + T(i, j, k)) * (Ph(i, j, k) - Ph(i, j, k - 1)) / dz : 0.0;
-                div += (Ft - Fb) / dz;
+                const T Ft = k < NZ - 1 ? ovz(h * (tr[0] + tr[5]) * (P_[5] - P_[0])) : T(0);
+                const T Fb = k > 0 ? ovz(h * (tr[6] + tr[0]) * (P_[0] - P_[6])) : T(0);
+                div += ovz((Ft - Fb));
             }
-            return div / s.R(i, j, k);
+            return div / r[0];
         };
-        dth = scalar(kth, s.th);
-        dqv = scalar(kq, s.qv);
+        dth = scalar(k1, p1);
+        dqv = scalar(k2, p2);
     }
-    P<S>(A.f.u2)[c] = S(du);
-    P<S>(A.f.v2)[c] = S(dv);
-    P<S>(A.f.w2)[c] = S(dw);
     P<S>(A.f.rp2)[c] = S(dth);
     P<S>(A.f.mfx)[c] = S(dqv);
 }
@@ -1874,7 +2011,7 @@ __global__ __launch_bounds__(kBlock) void k_sgs_apply(const MemberArgs* __restri
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
     const DevFields& f = A.f;
-    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
+    const int c = int(A.g.at(cl.i, cl.j, cl.k));
     const double h = n == A.p.sgs_n - 1 ? A.p.sgs_h_last : A.p.sgs_h;
     const auto u = P<S>(f.u), v = P<S>(f.v), w = P<S>(f.w), th = P<S>(f.th), qv = P<S>(f.qv);
     const S tu = P<S>(f.u2)[c], tv = P<S>(f.v2)[c], tw = P<S>(f.w2)[c], tt = P<S>(f.rp2)[c], tq = P<S>(f.mfx)[c];
@@ -2240,7 +2377,8 @@ void launch_large_step(Launcher& L, const BatchView& b)
     for (int n = 0; n < b.max_sgs_n; ++n)
     {
         launch_map(L, "sgs_closure", k_sgs_closure<S>, b, S_ALL, W_CELL, n);
-        launch_map(L, "sgs_tendency", k_sgs_tendency<S>, b, S_ALL, W_CELL, n);
+        launch_map(L, "sgs_momentum", k_sgs_momentum<S>, b, S_ALL, W_CELL, n);
+        launch_map(L, "sgs_scalars", k_sgs_scalars<S>, b, S_ALL, W_CELL, n);
         launch_map(L, "sgs_apply", k_sgs_apply<S>, b, S_ALL, W_CELL, n);
     }
     if (b.any_sponge) launch_map(L, "sponge", k_sponge<S>, b, S_ALL, W_CELL);
