@@ -1594,13 +1594,30 @@ __global__ __launch_bounds__(kBlock) void k_scalars_home(const MemberArgs* __res
 // Smagorinsky SGS (smagorinsky.cpp:495-580; eddy_viscosity.cpp:1253-1527
 // closure, 1532-1948 momentum, 1953-2090 scalars; sgs_apply.cpp:610-790
 // commit), applied after the microphysics in sgs_n substeps, each three
-// per-cell passes: closure (nu, K_theta, K_q into the dead k1 fields),
-// tendencies (u2, v2, w2, rp2, mfx, all dead after transport), commit (with
-// the EOS projection after the last). FP64 arithmetic on storage-type values,
-// as the reference's CPU path; the per-edge stresses a reference column
-// march carries from level to level are recomputed here from the same
-// operands, so the values are identical.
+// per-cell passes: closure (nu, K_theta, K_q into the dead k1 fields), then
+// momentum and scalar tendencies, each committed with the storage clamps
+// into the other of two buffer sets (home / scratch ping-pong), with the EOS
+// projection after the last. FP64 arithmetic for FP64 storage, as the
+// reference's CPU path; FP32 for the mixed mode. The per-edge stresses a
+// reference column march carries from level to level are recomputed here
+// from the same operands, so the values are identical.
 // ---------------------------------------------------------------------
+// EOS projection of one cell after the SGS commit (eos_apply.cpp:50-110).
+template <class S>
+__device__ __forceinline__ void sgs_eos(const MemberArgs& A, int c, int k, S theta)
+{
+    const DevFields& f = A.f;
+    const double r0 = slot(f.base, kBaseRho0, A.g.nz)[k], p0k = slot(f.base, kBaseP0, A.g.nz)[k];
+    const double rho = r0 + double(P<S>(f.rp)[c]);
+    const double thv = double(theta);
+    if (rho > 0.0 && thv > 0.0)
+    {
+        const double pd = dev::eos_pressure(rho, thv);
+        if (isfinite(pd) && pd > 0.0)
+            P<S>(f.pp)[c] = S(clamp_store<S>(pd, 101325.0, A.p.bounds.p_min, A.p.bounds.p_max) - p0k);
+    }
+}
+
 template <class S>
 struct SgsView
 {
@@ -1621,12 +1638,28 @@ struct SgsView
     __device__ T R(int i, int j, int k) const { return T(S(rho0[k] + double(rho_p[at(i, j, k)]))); }
 };
 
+// SGS substep n reads u, v, w, theta, qv from the home fields when n is even
+// and from the scratch set (u2, v2, w2, rp2, mfx: dead after transport) when
+// n is odd, and writes the other set, so the commit is part of the tendency
+// passes; an odd count leaves the state in the scratch set for k_sgs_home.
+struct SgsSet
+{
+    void *u, *v, *w, *th, *qv;
+};
+
+__device__ __forceinline__ SgsSet sgs_set(const DevFields& f, bool scratch)
+{
+    return scratch ? SgsSet{f.u2, f.v2, f.w2, f.rp2, f.mfx} : SgsSet{f.u, f.v, f.w, f.th, f.qv};
+}
+
 template <class S>
-__device__ SgsView<S> sgs_view(const MemberArgs& A)
+__device__ SgsView<S> sgs_view(const MemberArgs& A, int n)
 {
     const DevFields& f = A.f;
+    const SgsSet cur = sgs_set(f, n & 1);
     SgsView<S> s;
-    s.u = P<S>(f.u); s.v = P<S>(f.v); s.w = P<S>(f.w); s.rho_p = P<S>(f.rp); s.th = P<S>(f.th); s.qv = P<S>(f.qv);
+    s.u = P<S>(cur.u); s.v = P<S>(cur.v); s.w = P<S>(cur.w); s.rho_p = P<S>(f.rp); s.th = P<S>(cur.th);
+    s.qv = P<S>(cur.qv);
     s.rho0 = slot(f.base, kBaseRho0, A.g.nz);
     s.nx = A.g.nx; s.ny = A.g.ny; s.nz = A.g.nz; s.px = A.g.px; s.plane = int(A.g.plane());
     s.wx = A.p.sgs_wrap_x;
@@ -1747,7 +1780,7 @@ __global__ __launch_bounds__(kBlock) void k_sgs_closure(const MemberArgs* __rest
     if (n >= A.p.sgs_n) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
-    const SgsView<S> sv = sgs_view<S>(A);
+    const SgsView<S> sv = sgs_view<S>(A, n);
     const int c = sv.at(cl.i, cl.j, cl.k);
     T str;
     if constexpr (sizeof(T) == sizeof(double))
@@ -1760,7 +1793,8 @@ __global__ __launch_bounds__(kBlock) void k_sgs_closure(const MemberArgs* __rest
     P<S>(A.f.k1w)[c] = S(fmin(nu * A.p.sgs_inv_sc, A.p.sgs_k_max));
 }
 
-// Momentum tendencies (eddy_viscosity.cpp:1532-1948) into u2, v2, w2. Every
+// Momentum tendencies (eddy_viscosity.cpp:1532-1948), committed into the
+// other buffer set (sgs_set) with the storage clamps. Every
 // operand is one of the 27 cells around (i, j, k) reached by the reference's
 // neighbour rules (west iw = prev(i), east ie = next(i), likewise in y; next
 // of prev is the cell itself on every face these loops visit); the 59 the
@@ -1775,18 +1809,26 @@ __global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __res
     if (n >= A.p.sgs_n) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
-    const SgsView<S> s = sgs_view<S>(A);
+    const SgsView<S> s = sgs_view<S>(A, n);
     const int NX = s.nx, NY = s.ny, NZ = s.nz;
     const int i = cl.i, j = cl.j, k = cl.k;
     const int c = s.at(i, j, k);
     const bool has_u = i >= 1 && i <= (s.wx ? NX - 2 : NX - 3) && j >= 1 && j <= NY - 2 && k < NZ - 1;
     const bool has_v = i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && (s.wy || j < NY - 2) && k < NZ - 1;
     const bool has_w = i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && k < NZ - 1;
+    const SgsSet nxt = sgs_set(A.f, !(n & 1));
+    const double hstep = n == A.p.sgs_n - 1 ? A.p.sgs_h_last : A.p.sgs_h;
+    const Bounds& bd = A.p.bounds;
+    // commit x + h dx/dt with the storage clamps (sgs_apply.cpp:610-660); the
+    // tendency is rounded to storage first, as the reference's tendency field
+    auto commit = [&](T du, T dv, T dw) {
+        P<S>(nxt.u)[c] = S(clamp_store<S>(double(s.u[c]) + finite0(double(S(du))) * hstep, 0.0, -bd.wind_h, bd.wind_h));
+        P<S>(nxt.v)[c] = S(clamp_store<S>(double(s.v[c]) + finite0(double(S(dv))) * hstep, 0.0, -bd.wind_h, bd.wind_h));
+        P<S>(nxt.w)[c] = S(clamp_store<S>(double(s.w[c]) + finite0(double(S(dw))) * hstep, 0.0, -bd.wind_v, bd.wind_v));
+    };
     if (!(has_u || has_v || has_w))
     {
-        P<S>(A.f.u2)[c] = S(0);
-        P<S>(A.f.v2)[c] = S(0);
-        P<S>(A.f.w2)[c] = S(0);
+        commit(T(0), T(0), T(0));
         return;
     }
     const int iw = s.ip(i), ie = s.in(i), js = s.jp(j), jn = s.jn(j);
@@ -1921,12 +1963,12 @@ __global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __res
         }
         dw = force / mass;
     }
-    P<S>(A.f.u2)[c] = S(du);
-    P<S>(A.f.v2)[c] = S(dv);
-    P<S>(A.f.w2)[c] = S(dw);
+    commit(du, dv, dw);
 }
 
-// Scalar tendencies of theta and qv (eddy_viscosity.cpp:1953-2090) into rp2, mfx.
+// Scalar tendencies of theta and qv (eddy_viscosity.cpp:1953-2090),
+// committed into the other buffer set; the EOS projection after the last
+// substep when it lands in the home fields.
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __restrict__ ma,
                                                         const int2* __restrict__ map, int n)
@@ -1937,7 +1979,7 @@ __global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __rest
     if (n >= A.p.sgs_n) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
-    const SgsView<S> s = sgs_view<S>(A);
+    const SgsView<S> s = sgs_view<S>(A, n);
     const int NX = s.nx, NY = s.ny, NZ = s.nz;
     const int i = cl.i, j = cl.j, k = cl.k;
     const T dx = T(A.g.dx), dy = T(A.g.dy), dz = T(A.g.dz), h = T(0.5);
@@ -1995,47 +2037,34 @@ __global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __rest
         dth = scalar(k1, p1);
         dqv = scalar(k2, p2);
     }
-    P<S>(A.f.rp2)[c] = S(dth);
-    P<S>(A.f.mfx)[c] = S(dqv);
+    const SgsSet nxt = sgs_set(A.f, !(n & 1));
+    const double hstep = n == A.p.sgs_n - 1 ? A.p.sgs_h_last : A.p.sgs_h;
+    const S thn = S(clamp_store<S>(double(s.th[c]) + finite0(double(S(dth))) * hstep, 300.0, 200.0f, 700.0f));
+    P<S>(nxt.th)[c] = thn;
+    P<S>(nxt.qv)[c] = S(clamp_store<S>(double(s.qv[c]) + finite0(double(S(dqv))) * hstep, 0.0, 0.0f, 0.05f));
+    // an even count ends in the home fields: the EOS projection follows the last commit here
+    if (n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0) sgs_eos<S>(A, c, k, thn);
 }
 
-// Commit of SGS substep n (sgs_apply.cpp:610-790); after the last one the
-// EOS projection the Kessler pass leaves out when the SGS is on.
+// Odd SGS counts: the state of the last substep back to the home fields,
+// then the EOS projection.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_sgs_apply(const MemberArgs* __restrict__ ma,
-                                                      const int2* __restrict__ map, int n)
+__global__ __launch_bounds__(kBlock) void k_sgs_home(const MemberArgs* __restrict__ ma, const int2* __restrict__ map)
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
-    if (n >= A.p.sgs_n) return;
+    if ((A.p.sgs_n & 1) == 0) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
     const DevFields& f = A.f;
     const int c = int(A.g.at(cl.i, cl.j, cl.k));
-    const double h = n == A.p.sgs_n - 1 ? A.p.sgs_h_last : A.p.sgs_h;
-    const auto u = P<S>(f.u), v = P<S>(f.v), w = P<S>(f.w), th = P<S>(f.th), qv = P<S>(f.qv);
-    const S tu = P<S>(f.u2)[c], tv = P<S>(f.v2)[c], tw = P<S>(f.w2)[c], tt = P<S>(f.rp2)[c], tq = P<S>(f.mfx)[c];
-    const S uo = u[c], vo = v[c], wo = w[c], to = th[c], qo = qv[c];
-    const Bounds& bd = A.p.bounds;
-    u[c] = S(clamp_store<S>(double(uo) + finite0(double(tu)) * h, 0.0, -bd.wind_h, bd.wind_h));
-    v[c] = S(clamp_store<S>(double(vo) + finite0(double(tv)) * h, 0.0, -bd.wind_h, bd.wind_h));
-    w[c] = S(clamp_store<S>(double(wo) + finite0(double(tw)) * h, 0.0, -bd.wind_v, bd.wind_v));
-    const S thn = S(clamp_store<S>(double(to) + finite0(double(tt)) * h, 300.0, 200.0f, 700.0f));
-    th[c] = thn;
-    qv[c] = S(clamp_store<S>(double(qo) + finite0(double(tq)) * h, 0.0, 0.0f, 0.05f));
-    if (n == A.p.sgs_n - 1)
-    {
-        const int k = cl.k;
-        const double r0 = slot(f.base, kBaseRho0, A.g.nz)[k], p0k = slot(f.base, kBaseP0, A.g.nz)[k];
-        const double rho = r0 + double(P<S>(f.rp)[c]);
-        const double thv = double(thn);
-        if (rho > 0.0 && thv > 0.0)
-        {
-            const double pd = dev::eos_pressure(rho, thv);
-            if (isfinite(pd) && pd > 0.0)
-                P<S>(f.pp)[c] = S(clamp_store<S>(pd, 101325.0, A.p.bounds.p_min, A.p.bounds.p_max) - p0k);
-        }
-    }
+    const S u = P<S>(f.u2)[c], v = P<S>(f.v2)[c], w = P<S>(f.w2)[c], th = P<S>(f.rp2)[c], qv = P<S>(f.mfx)[c];
+    P<S>(f.u)[c] = u;
+    P<S>(f.v)[c] = v;
+    P<S>(f.w)[c] = w;
+    P<S>(f.th)[c] = th;
+    P<S>(f.qv)[c] = qv;
+    sgs_eos<S>(A, c, cl.k, th);
 }
 
 // ---------------------------------------------------------------------
@@ -2379,8 +2408,8 @@ void launch_large_step(Launcher& L, const BatchView& b)
         launch_map(L, "sgs_closure", k_sgs_closure<S>, b, S_ALL, W_CELL, n);
         launch_map(L, "sgs_momentum", k_sgs_momentum<S>, b, S_ALL, W_CELL, n);
         launch_map(L, "sgs_scalars", k_sgs_scalars<S>, b, S_ALL, W_CELL, n);
-        launch_map(L, "sgs_apply", k_sgs_apply<S>, b, S_ALL, W_CELL, n);
     }
+    if (b.max_sgs_n > 0) launch_map(L, "sgs_home", k_sgs_home<S>, b, S_ALL, W_CELL);
     if (b.any_sponge) launch_map(L, "sponge", k_sponge<S>, b, S_ALL, W_CELL);
     launch_boundaries<S>(L, b, -1);
 }
