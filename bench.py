@@ -12,7 +12,7 @@ ensemble run as one batch, and a short continuous-batching serving run with
 p50/p99 completion latency. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
-                  [--serving-seconds S] [--no-ensemble] [--accuracy c1|c2|''] [--no-cpu-baseline]
+                  [--serving-seconds S] [--no-ensemble] [--accuracy c2,c1|''] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -65,6 +65,12 @@ ALGO_PER_CELL = {
     "slow_stage1": (13, 0),
     # read predictor (3) rho' qv qc qr, k1 (3), u v w; write u v w
     "slow_stage2": (16, 0),
+    # Smagorinsky substep: read u v w; write nu K_theta K_q
+    "sgs_closure": (6, 0),
+    # read u v w rho' nu; write u v w (the other buffer set)
+    "sgs_momentum": (8, 0),
+    # read theta qv rho' K_theta K_q; write theta qv (p' of the last substep not counted)
+    "sgs_scalars": (7, 0),
 }
 
 
@@ -116,7 +122,7 @@ def kernel_table(ctx, m, steps):
     return tab
 
 
-def cpu_baseline(wl, sample_steps=2):
+def cpu_baseline(wl, sample_steps=6):
     """The oracle in the reference's CPU arithmetic (FP32 storage, FP64
     arithmetic, explicit acoustic substeps) on the host cores."""
     from oracle import oracle_py as O
@@ -214,7 +220,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=20.0,
                     help="arrival window of the serving run (600 = the full 10-minute stream; 0 skips it)")
-    ap.add_argument("--accuracy", default="c1", help="workload of the 1-hour mixed-vs-FP64 error run ('' skips it)")
+    ap.add_argument("--accuracy", default="c2,c1",
+                    help="workloads of the 1-hour mixed-vs-FP64 error runs, comma-separated; the first is "
+                         "reported as accuracy_1h ('' skips them)")
     ap.add_argument("--profile", choices=["hevi", "explicit"], default=None,
                     help="only the mixed member in this mode: warm-up + timed steps, for rocprofv3 passes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -310,8 +318,9 @@ def main():
         result["serving_c4"] = serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
     if args.accuracy:
         from nimbus_amd import accuracy as ACC
-        acc = ACC.one_hour(ctx, W.NAMED[args.accuracy], L.ACOUSTIC_HEVI)
-        result["accuracy_1h"] = acc
+        for q, name in enumerate(n for n in args.accuracy.split(",") if n):
+            acc = ACC.one_hour(ctx, W.NAMED[name], L.ACOUSTIC_HEVI)
+            result["accuracy_1h" if q == 0 else f"accuracy_1h_{name}"] = acc
     ctx.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(wl)
