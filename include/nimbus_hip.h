@@ -189,10 +189,89 @@ int nimbus_ic_base_state(const nimbus_grid* g, const nimbus_sounding* s, double*
 /* Full initial state: base + bubbles (RH preserving) + seeded noise; p = p0, rho from the EOS. */
 int nimbus_ic_fill_f32(const nimbus_grid* g, const nimbus_ic_params* prm, const nimbus_host_state* out);
 int nimbus_ic_fill_f64(const nimbus_grid* g, const nimbus_ic_params* prm, const nimbus_host_state* out);
+/* The same fill over caller-supplied base profiles (nz each; a CM1 sounding's
+ * nimbus_cm1_base_state output, say): bubbles, noise and the RH-preserving
+ * moistening as above, with prm->sounding used only for its qv_cap. */
+int nimbus_ic_fill_base_f32(const nimbus_grid* g, const nimbus_base_state* base, const nimbus_ic_params* prm,
+                            const nimbus_host_state* out);
+int nimbus_ic_fill_base_f64(const nimbus_grid* g, const nimbus_base_state* base, const nimbus_ic_params* prm,
+                            const nimbus_host_state* out);
 /* Counter-based uniform [0,1) hash of (seed, member, i, j, k) used for the noise. */
 double nimbus_ic_uniform(uint64_t seed, uint32_t member, int i, int j, int k);
 /* Effective acoustic substep count (split_explicit.cpp:67-156; HE-VI checks only the horizontal CFL). */
 int nimbus_resolve_substeps(const nimbus_grid* g, const nimbus_base_state* b, const nimbus_step_config* c);
+
+/* ---------------- CM1 input_sounding (host only, no device needed) ----------------
+ * CM1's native text sounding: a header "p_sfc[hPa] theta_sfc[K] qv_sfc[g/kg]",
+ * then one row "z[m] theta[K] qv[g/kg] u[m/s] v[m/s]" per level, the first at
+ * z = 0; '!' and '#' start comments, Fortran 'd'/'D' exponents are accepted
+ * (src/soundings/schemes/cm1/cm1_sounding.hpp:5-15). In memory the column is
+ * in SI units (Pa, K, kg/kg, m, m/s) with the pressure and temperature the
+ * reader reconstructs hydrostatically from the header. Arrays are
+ * caller-owned with `capacity` entries each. */
+#define NIMBUS_CM1_MAX_LEVELS 100000           /* soundings/sounding_input_limits.hpp:10 */
+#define NIMBUS_CM1_MAX_BYTES (8u * 1024u * 1024u) /* soundings/sounding_input_limits.hpp:9 */
+
+typedef struct
+{
+    int n;                             /* levels after the header (output of parse/read) */
+    int capacity;                      /* entries each array below holds */
+    double p_sfc, theta_sfc, qv_sfc;   /* header: Pa, K, kg/kg */
+    double *z, *theta, *qv, *u, *v;    /* the file's columns */
+    double *p, *T;                     /* hydrostatic reconstruction (Pa, K); may be null for format */
+} nimbus_cm1_sounding;
+
+enum
+{
+    NIMBUS_INTERP_LINEAR = 0,          /* soundings_base.hpp:95 interpolation_method 0 */
+    NIMBUS_INTERP_SPLINE = 1,          /* monotone PCHIP */
+    NIMBUS_INTERP_LOG_PRESSURE = 2     /* linear, with log-linear pressure */
+};
+
+typedef struct
+{
+    int method;                        /* NIMBUS_INTERP_* (default linear) */
+    int extrapolate_below, extrapolate_above; /* default 0, 0 */
+} nimbus_interp;
+
+/* Parse and validate CM1 text, reconstruct p and T (Exner integration of
+ * theta_v by the trapezoid rule from the header pressure) and apply the
+ * reader's quality control, refusing rather than shortening a profile
+ * (Cm1SoundingScheme::load_sounding, cm1_sounding.cpp:159-281). Fails with
+ * NIMBUS_EINVAL and a line-numbered message; when capacity is too small,
+ * out->n holds the level count needed. */
+int nimbus_cm1_parse(const char* text, size_t len, nimbus_cm1_sounding* out);
+/* The same from a regular file of at most NIMBUS_CM1_MAX_BYTES
+ * (Cm1SoundingScheme::initialize, cm1_sounding.cpp:138-157). */
+int nimbus_cm1_read(const char* path, nimbus_cm1_sounding* out);
+/* Write the column as CM1 text (header from p_sfc, theta_sfc, qv_sfc; rows
+ * from z, theta, qv, u, v) with round-trip precision into buf (cap bytes, NUL
+ * terminated); *len receives the length without the NUL, also when cap is too
+ * small (then NIMBUS_EINVAL). No reference counterpart: the writer the
+ * exchange of soundings with the reference needs. */
+int nimbus_cm1_format(const nimbus_cm1_sounding* s, char* buf, size_t cap, size_t* len);
+/* Model column at heights z[0..nz) (strictly increasing, >= 0; dz the grid
+ * spacing): interpolation to z (Cm1SoundingScheme::interpolate_to_heights,
+ * cm1_sounding.cpp:283-312; sounding_algorithms.cpp:427-641;
+ * sharpy_interpolation.cpp:13-407), the coverage check of the model top
+ * interface and the dry Exner re-integration that keeps theta exact
+ * (FileSoundingSource::build and sounding_from_data,
+ * init/sounding/sources/file_sounding.cpp:63-327,344-438). Outputs the base
+ * profiles rho0, p0, theta0, qv0, u0, v0 and temperature T0 (nz each; T0 may
+ * be null). */
+int nimbus_cm1_column(const nimbus_cm1_sounding* s, const nimbus_interp* how, int nz, const double* z, double dz,
+                      double* rho0, double* p0, double* theta0, double* qv0, double* u0, double* v0, double* T0);
+/* nimbus_cm1_column at the cell centres z_k = (k + 1/2) dz of grid g: the
+ * base state the reference assigns from a sounding
+ * (core/orchestration/dynamics/equations.cpp:288-305). */
+int nimbus_cm1_base_state(const nimbus_grid* g, const nimbus_cm1_sounding* s, const nimbus_interp* how,
+                          double* rho0, double* p0, double* theta0, double* qv0, double* u0, double* v0);
+/* The analytic sounding sampled at heights z[0..n) (z[0] = 0) as a CM1
+ * column: theta, qv and the hodograph wind at each height, the header from
+ * the surface values; p and T (when non-null) from the same hydrostatic
+ * integration as nimbus_ic_base_state. No reference counterpart: it exports
+ * the seeded workloads' sounding in the reference's input format. */
+int nimbus_ic_sounding_levels(const nimbus_sounding* s, int n, const double* z, nimbus_cm1_sounding* out);
 
 /* ---------------- resident ensemble members ---------------- */
 int nimbus_hip_member_create(nimbus_hip_ctx* ctx, const nimbus_member_desc* desc, uint32_t* id);

@@ -149,12 +149,15 @@ int base_state(const nimbus_grid& g, const nimbus_sounding& s, double* rho0, dou
 }
 
 template <class S>
-int fill_state(const nimbus_grid& g, const nimbus_ic_params& prm, const nimbus_host_state& out)
+int fill_from_base(const nimbus_grid& g, const nimbus_base_state& b, const nimbus_ic_params& prm,
+                   const nimbus_host_state& out)
 {
     if (g.nx < 5 || g.ny < 5 || g.nz < 4) return NIMBUS_EINVAL;
-    std::vector<double> rho0(g.nz), p0(g.nz), th0(g.nz), qv0(g.nz), u0(g.nz), v0(g.nz);
-    const int rc = base_state(g, prm.sounding, rho0.data(), p0.data(), th0.data(), qv0.data(), u0.data(), v0.data());
-    if (rc != NIMBUS_OK) return rc;
+    if (!b.rho0 || !b.p0 || !b.theta0 || !b.qv0 || !b.u0 || !b.v0) return NIMBUS_EINVAL;
+    for (int k = 0; k < g.nz; ++k)
+        if (!(b.p0[k] > 0.0) || !(b.theta0[k] > 0.0) || !std::isfinite(b.p0[k]) || !std::isfinite(b.theta0[k]))
+            return NIMBUS_EINVAL;
+    const double *p0 = b.p0, *th0 = b.theta0, *qv0 = b.qv0, *u0 = b.u0, *v0 = b.v0;
     S* u = static_cast<S*>(out.u);
     S* v = static_cast<S*>(out.v);
     S* w = static_cast<S*>(out.w);
@@ -211,8 +214,68 @@ int fill_state(const nimbus_grid& g, const nimbus_ic_params& prm, const nimbus_h
     return NIMBUS_OK;
 }
 
+template <class S>
+int fill_state(const nimbus_grid& g, const nimbus_ic_params& prm, const nimbus_host_state& out)
+{
+    if (g.nx < 5 || g.ny < 5 || g.nz < 4) return NIMBUS_EINVAL;
+    std::vector<double> rho0(g.nz), p0(g.nz), th0(g.nz), qv0(g.nz), u0(g.nz), v0(g.nz);
+    const int rc = base_state(g, prm.sounding, rho0.data(), p0.data(), th0.data(), qv0.data(), u0.data(), v0.data());
+    if (rc != NIMBUS_OK) return rc;
+    const nimbus_base_state b{rho0.data(), p0.data(), th0.data(), qv0.data(), u0.data(), v0.data()};
+    return fill_from_base<S>(g, b, prm, out);
+}
+
 template int fill_state<float>(const nimbus_grid&, const nimbus_ic_params&, const nimbus_host_state&);
 template int fill_state<double>(const nimbus_grid&, const nimbus_ic_params&, const nimbus_host_state&);
+template int fill_from_base<float>(const nimbus_grid&, const nimbus_base_state&, const nimbus_ic_params&,
+                                   const nimbus_host_state&);
+template int fill_from_base<double>(const nimbus_grid&, const nimbus_base_state&, const nimbus_ic_params&,
+                                    const nimbus_host_state&);
+
+// The analytic sounding at heights z (z[0] = 0) in CM1 column form: the
+// same midpoint integration of d(pi)/dz = -g / (cp theta_v) as base_state,
+// carried from one requested height to the next.
+int sounding_levels(const nimbus_sounding& s, int n, const double* z, nimbus_cm1_sounding& out)
+{
+    if (n < 1 || !z || !(z[0] == 0.0)) return NIMBUS_EINVAL;
+    if (out.capacity < n || !out.z || !out.theta || !out.qv || !out.u || !out.v) return NIMBUS_EINVAL;
+    for (int k = 1; k < n; ++k)
+        if (!(z[k] > z[k - 1]) || !std::isfinite(z[k])) return NIMBUS_EINVAL;
+    const double kappa = kRd / kCp;
+    auto thv_at = [&](double zq, double piq) {
+        const double th = wk_theta(s, zq);
+        const double p = kP00 * std::pow(piq, 1.0 / kappa);
+        return th * (1.0 + 0.608 * sounding_qv(s, zq, th * piq, p));
+    };
+    double zc = 0.0;
+    double pi = std::pow(s.p_sfc / kP00, kappa);
+    for (int k = 0; k < n; ++k)
+    {
+        const int nsub = std::max(1, int(std::ceil((z[k] - zc) / 10.0)));
+        const double h = (z[k] - zc) / nsub;
+        for (int q = 0; q < nsub && z[k] > zc; ++q)
+        {
+            const double half = pi - 0.5 * h * kG / (kCp * thv_at(zc, pi));
+            pi = pi - h * kG / (kCp * thv_at(zc + 0.5 * h, half));
+            zc += h;
+        }
+        zc = z[k];
+        const double th = wk_theta(s, z[k]);
+        const double p = kP00 * std::pow(pi, 1.0 / kappa);
+        const double T = th * pi;
+        out.z[k] = z[k];
+        out.theta[k] = th;
+        out.qv[k] = sounding_qv(s, z[k], T, p);
+        hodograph(s, z[k], out.u[k], out.v[k]);
+        if (out.p) out.p[k] = p;
+        if (out.T) out.T[k] = T;
+    }
+    out.n = n;
+    out.p_sfc = s.p_sfc;
+    out.theta_sfc = out.theta[0];
+    out.qv_sfc = out.qv[0];
+    return NIMBUS_OK;
+}
 
 int resolve_substeps(const nimbus_grid& g, const double* rho0, const double* p0, const nimbus_step_config& c)
 {
@@ -264,6 +327,26 @@ int nimbus_ic_fill_f64(const nimbus_grid* g, const nimbus_ic_params* prm, const 
 {
     if (!g || !prm || !out) return NIMBUS_EINVAL;
     return nimbus::fill_state<double>(*g, *prm, *out);
+}
+
+int nimbus_ic_fill_base_f32(const nimbus_grid* g, const nimbus_base_state* base, const nimbus_ic_params* prm,
+                            const nimbus_host_state* out)
+{
+    if (!g || !base || !prm || !out) return NIMBUS_EINVAL;
+    return nimbus::fill_from_base<float>(*g, *base, *prm, *out);
+}
+
+int nimbus_ic_fill_base_f64(const nimbus_grid* g, const nimbus_base_state* base, const nimbus_ic_params* prm,
+                            const nimbus_host_state* out)
+{
+    if (!g || !base || !prm || !out) return NIMBUS_EINVAL;
+    return nimbus::fill_from_base<double>(*g, *base, *prm, *out);
+}
+
+int nimbus_ic_sounding_levels(const nimbus_sounding* s, int n, const double* z, nimbus_cm1_sounding* out)
+{
+    if (!s || !out) return NIMBUS_EINVAL;
+    return nimbus::sounding_levels(*s, n, z, *out);
 }
 
 double nimbus_ic_uniform(uint64_t seed, uint32_t member, int i, int j, int k)

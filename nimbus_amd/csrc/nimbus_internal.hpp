@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <string>
 
 #include "nimbus_hip.h"
 
@@ -27,6 +28,12 @@ int base_state(const nimbus_grid& g, const nimbus_sounding& s, double* rho0, dou
                double* u0, double* v0);
 template <class S>
 int fill_state(const nimbus_grid& g, const nimbus_ic_params& prm, const nimbus_host_state& out);
+template <class S>
+int fill_from_base(const nimbus_grid& g, const nimbus_base_state& b, const nimbus_ic_params& prm,
+                   const nimbus_host_state& out);
+int sounding_levels(const nimbus_sounding& s, int n, const double* z, nimbus_cm1_sounding& out);
+// Records msg as the calling thread's last error (nimbus_hip_last_error) and returns code.
+int set_error(int code, const std::string& msg);
 double ic_uniform(std::uint64_t seed, std::uint32_t member, int i, int j, int k);
 int resolve_substeps(const nimbus_grid& g, const double* rho0, const double* p0, const nimbus_step_config& c);
 

@@ -181,6 +181,9 @@ struct Batch
 };
 
 }  // namespace
+
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
 }  // namespace nimbus
 
 struct nimbus_hip_ctx

@@ -77,6 +77,16 @@ class ICParams(ctypes.Structure):
                 ("noise_amp", ctypes.c_double), ("seed", ctypes.c_uint64), ("member", ctypes.c_uint32)]
 
 
+class Cm1Sounding(ctypes.Structure):
+    _fields_ = ([("n", ctypes.c_int), ("capacity", ctypes.c_int)] +
+                [(n, ctypes.c_double) for n in ("p_sfc", "theta_sfc", "qv_sfc")] +
+                [(n, _c_double_p) for n in ("z", "theta", "qv", "u", "v", "p", "T")])
+
+
+class Interp(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("extrapolate_below", ctypes.c_int), ("extrapolate_above", ctypes.c_int)]
+
+
 class MemberDesc(ctypes.Structure):
     _fields_ = [("grid", Grid), ("config", StepConfig), ("precision", ctypes.c_int), ("base", BaseState)]
 
@@ -163,6 +173,16 @@ def load_library(path=LIB_PATH):
         "nimbus_ic_base_state": (ctypes.c_int, [P(Grid), P(Sounding)] + [_c_double_p] * 6),
         "nimbus_ic_fill_f32": (ctypes.c_int, [P(Grid), P(ICParams), P(HostState)]),
         "nimbus_ic_fill_f64": (ctypes.c_int, [P(Grid), P(ICParams), P(HostState)]),
+        "nimbus_ic_fill_base_f32": (ctypes.c_int, [P(Grid), P(BaseState), P(ICParams), P(HostState)]),
+        "nimbus_ic_fill_base_f64": (ctypes.c_int, [P(Grid), P(BaseState), P(ICParams), P(HostState)]),
+        "nimbus_ic_sounding_levels": (ctypes.c_int, [P(Sounding), ctypes.c_int, _c_double_p, P(Cm1Sounding)]),
+        "nimbus_cm1_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, P(Cm1Sounding)]),
+        "nimbus_cm1_read": (ctypes.c_int, [ctypes.c_char_p, P(Cm1Sounding)]),
+        "nimbus_cm1_format": (ctypes.c_int, [P(Cm1Sounding), ctypes.c_char_p, ctypes.c_size_t,
+                                             P(ctypes.c_size_t)]),
+        "nimbus_cm1_column": (ctypes.c_int, [P(Cm1Sounding), P(Interp), ctypes.c_int, _c_double_p, ctypes.c_double]
+                              + [_c_double_p] * 7),
+        "nimbus_cm1_base_state": (ctypes.c_int, [P(Grid), P(Cm1Sounding), P(Interp)] + [_c_double_p] * 6),
         "nimbus_ic_uniform": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int]),
         "nimbus_resolve_substeps": (ctypes.c_int, [P(Grid), P(BaseState), P(StepConfig)]),
@@ -316,6 +336,129 @@ def initial_state(grid, params, dtype=np.float32):
     fill = lib.nimbus_ic_fill_f32 if np.dtype(dtype) == np.float32 else lib.nimbus_ic_fill_f64
     check(fill(ctypes.byref(grid), ctypes.byref(params), ctypes.byref(hs)), "nimbus_ic_fill")
     return st
+
+
+def initial_state_from_base(grid, base, params, dtype=np.float32):
+    """Seeded initial state (bubbles, noise) over given base profiles, e.g. a
+    CM1 sounding's cm1_base_state."""
+    lib = load_library()
+    st = new_state(grid, dtype)
+    hs = host_state_struct(st)
+    b = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in base.items()}
+    fill = lib.nimbus_ic_fill_base_f32 if np.dtype(dtype) == np.float32 else lib.nimbus_ic_fill_base_f64
+    check(fill(ctypes.byref(grid), ctypes.byref(base_struct(b)), ctypes.byref(params), ctypes.byref(hs)),
+          "nimbus_ic_fill_base")
+    return st
+
+
+# ---------------------------------------------------------------- CM1 input_sounding
+
+CM1_COLUMNS = ("z", "theta", "qv", "u", "v", "p", "T")
+INTERP_LINEAR, INTERP_SPLINE, INTERP_LOG_PRESSURE = 0, 1, 2
+
+
+def _cm1_struct(col):
+    """Cm1Sounding over the arrays of a column dict (keeps no copies: the
+    caller keeps `col` alive while the struct is in use)."""
+    n = len(col["z"])
+    s = Cm1Sounding()
+    s.n = n
+    s.capacity = n
+    s.p_sfc, s.theta_sfc, s.qv_sfc = (float(col[k]) for k in ("p_sfc", "theta_sfc", "qv_sfc"))
+    for k in CM1_COLUMNS:
+        if col.get(k) is not None:
+            setattr(s, k, dptr(col[k]))
+    return s
+
+
+def _cm1_empty(capacity):
+    col = {k: np.zeros(capacity) for k in CM1_COLUMNS}
+    col.update(p_sfc=0.0, theta_sfc=0.0, qv_sfc=0.0)
+    s = _cm1_struct(col)
+    s.n = 0
+    return col, s
+
+
+def _cm1_result(col, s):
+    out = {k: col[k][:s.n].copy() for k in CM1_COLUMNS}
+    out.update(p_sfc=s.p_sfc, theta_sfc=s.theta_sfc, qv_sfc=s.qv_sfc)
+    return out
+
+
+def cm1_parse(text):
+    """CM1 input_sounding text -> column dict (SI units; p and T reconstructed)."""
+    lib = load_library()
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    col, s = _cm1_empty(max(8, data.count(b"\n") + 2))
+    check(lib.nimbus_cm1_parse(data, len(data), ctypes.byref(s)), "nimbus_cm1_parse")
+    return _cm1_result(col, s)
+
+
+def cm1_read(path):
+    """CM1 input_sounding file -> column dict."""
+    lib = load_library()
+    size = os.path.getsize(path) if os.path.isfile(path) else 0
+    col, s = _cm1_empty(max(8, size // 8 + 2))
+    check(lib.nimbus_cm1_read(os.fsencode(path), ctypes.byref(s)), "nimbus_cm1_read")
+    return _cm1_result(col, s)
+
+
+def cm1_format(col):
+    """Column dict -> CM1 input_sounding text."""
+    lib = load_library()
+    c = {k: (np.ascontiguousarray(col[k], dtype=np.float64) if k in CM1_COLUMNS and col.get(k) is not None
+             else col.get(k)) for k in list(CM1_COLUMNS) + ["p_sfc", "theta_sfc", "qv_sfc"]}
+    s = _cm1_struct(c)
+    need = ctypes.c_size_t(0)
+    lib.nimbus_cm1_format(ctypes.byref(s), None, 0, ctypes.byref(need))
+    buf = ctypes.create_string_buffer(need.value + 1)
+    check(lib.nimbus_cm1_format(ctypes.byref(s), buf, len(buf), ctypes.byref(need)), "nimbus_cm1_format")
+    return buf.value.decode()
+
+
+def _interp(method, extrapolate_below, extrapolate_above):
+    return Interp(int(method), int(bool(extrapolate_below)), int(bool(extrapolate_above)))
+
+
+def cm1_column(col, z, dz, method=INTERP_LINEAR, extrapolate_below=False, extrapolate_above=False):
+    """Model column at heights z from a CM1 column dict: rho0, p0, theta0,
+    qv0, u0, v0 and T0."""
+    lib = load_library()
+    c = {k: (np.ascontiguousarray(col[k], dtype=np.float64) if k in CM1_COLUMNS else col[k])
+         for k in list(CM1_COLUMNS) + ["p_sfc", "theta_sfc", "qv_sfc"]}
+    s = _cm1_struct(c)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    names = ("rho0", "p0", "theta0", "qv0", "u0", "v0", "T0")
+    out = {n: np.zeros(len(z)) for n in names}
+    how = _interp(method, extrapolate_below, extrapolate_above)
+    check(lib.nimbus_cm1_column(ctypes.byref(s), ctypes.byref(how), len(z), dptr(z), float(dz),
+                                *[dptr(out[n]) for n in names]), "nimbus_cm1_column")
+    return out
+
+
+def cm1_base_state(grid, col, method=INTERP_LINEAR, extrapolate_below=False, extrapolate_above=False):
+    """Base-state profiles at the cell centres of `grid` from a CM1 column dict."""
+    lib = load_library()
+    c = {k: (np.ascontiguousarray(col[k], dtype=np.float64) if k in CM1_COLUMNS else col[k])
+         for k in list(CM1_COLUMNS) + ["p_sfc", "theta_sfc", "qv_sfc"]}
+    s = _cm1_struct(c)
+    names = ("rho0", "p0", "theta0", "qv0", "u0", "v0")
+    out = {n: np.zeros(grid.nz) for n in names}
+    how = _interp(method, extrapolate_below, extrapolate_above)
+    check(lib.nimbus_cm1_base_state(ctypes.byref(grid), ctypes.byref(s), ctypes.byref(how),
+                                    *[dptr(out[n]) for n in names]), "nimbus_cm1_base_state")
+    return out
+
+
+def sounding_levels(z, sounding=None):
+    """The analytic sounding sampled at heights z (z[0] = 0) as a CM1 column dict."""
+    lib = load_library()
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    snd = sounding if sounding is not None else default_sounding()
+    col, s = _cm1_empty(len(z))
+    check(lib.nimbus_ic_sounding_levels(ctypes.byref(snd), len(z), dptr(z), ctypes.byref(s)),
+          "nimbus_ic_sounding_levels")
+    return _cm1_result(col, s)
 
 
 def member_bytes(grid, config, base, precision=PRECISION_MIXED):

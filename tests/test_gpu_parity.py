@@ -10,6 +10,7 @@ compare the FP64 HIP path with the FP64 oracle (reordering only: rel 1e-9)
 and the mixed path with the FP64 oracle at physics-level tolerances.
 """
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -290,6 +291,28 @@ def test_large_step_with_smagorinsky_fp64_matches_fp64_oracle(hip_ctx, mode, sgs
     plain, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(L.step_config(
         dt, 6, acoustic_mode=mode, lateral=L.BC_RIGID_SLIP, sponge=SMALL_SPONGE)), st, n_steps, np.float64)
     assert max_abs(plain["u"], ref["u"]) > 1e-3  # the SGS did work in this case
+
+
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_large_steps_on_a_cm1_sounding_fp64(hip_ctx, mode):
+    """A member whose base state comes from one of the reference's CM1
+    input_soundings (high-CAPE quarter-circle) through the reader and the
+    column builder, with a warm bubble, sponges and SGS: FP64 device = FP64
+    oracle on the same base."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    col = L.cm1_read(os.path.join(here, "golden", "cm1", "HIGHCAPENEW_QUARTERCIRCLE"))
+    g = L.make_grid(24, 22, 36, 1000.0, 1000.0, 500.0)
+    base = L.cm1_base_state(g, col)
+    prm = L.ic_params(g, bubbles=[(11000.0, 10000.0, 1500.0, 5000.0, 1500.0, 3.0)], noise_amp=0.1, seed=7)
+    st = L.initial_state_from_base(g, base, prm, np.float64)
+    # periodic sides carry the sounding's sheared wind through the domain
+    sponge = dict(SMALL_SPONGE, z_d=12000.0, lateral_cells=0)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode, lateral=L.BC_PERIODIC, sponge=sponge, sgs=L.sgs_preset())
+    n_steps = 3
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
+    out, diag = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, n_steps)
+    check_fp64(out, ref, f"cm1 sounding mode {mode}", False)
+    assert diag["max_w"] > 0.05  # the bubble rises
 
 
 @pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
