@@ -273,6 +273,17 @@ int nimbus_cm1_base_state(const nimbus_grid* g, const nimbus_cm1_sounding* s, co
  * the seeded workloads' sounding in the reference's input format. */
 int nimbus_ic_sounding_levels(const nimbus_sounding* s, int n, const double* z, nimbus_cm1_sounding* out);
 
+/* ---------------- state I/O (host only) ---------------- */
+enum
+{
+    NIMBUS_NPY_F4 = 0,   /* '<f4' */
+    NIMBUS_NPY_F8 = 1    /* '<f8' */
+};
+/* Write a C-order array of 1 to 3 dimensions as a NumPy v1.0 .npy file, byte
+ * for byte what the reference's writer produces (npy::write_2d / write_3d,
+ * src/core/output/npy_writer.cpp:33-182). */
+int nimbus_npy_write(const char* path, const void* data, int dtype, int ndim, const int64_t* shape);
+
 /* ---------------- resident ensemble members ---------------- */
 int nimbus_hip_member_create(nimbus_hip_ctx* ctx, const nimbus_member_desc* desc, uint32_t* id);
 int nimbus_hip_member_destroy(nimbus_hip_ctx* ctx, uint32_t id);
@@ -281,6 +292,29 @@ uint64_t nimbus_hip_member_bytes(const nimbus_member_desc* desc);
 int nimbus_hip_member_upload(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s);
 int nimbus_hip_member_download(nimbus_hip_ctx* ctx, uint32_t id, const nimbus_host_state* s);
 int nimbus_hip_member_diag(nimbus_hip_ctx* ctx, uint32_t id, nimbus_diag* d);
+/* Field export of one member: u, v, w, theta, p, rho, qv, qc, qr as <name>.npy
+ * of shape (nx, ny, nz) '<f4' in the reference layout (full rho and p), and
+ * accumulated_rainfall.npy of shape (nx, ny), into the existing directory
+ * `dir` -- the files the reference's NPY field export writes for these
+ * fields (AsyncOutputWriter::write_npy_field, src/core/output/output_writer.cpp:
+ * 1331-1350; names src/core/output/output_config.cpp:216-253). FP64 members
+ * are rounded to FP32, as the reference's fields are stored. */
+int nimbus_hip_member_export_npy(nimbus_hip_ctx* ctx, uint32_t id, const char* dir);
+/* Member snapshot for eviction and resume: the description (grid, step
+ * configuration, precision, launch tiling), the six base profiles, the step
+ * count, and the prognostic fields and surface rain in their device storage
+ * form (perturbation rho' and p', storage precision), with an FNV-1a 64
+ * checksum. Restoring it and stepping on gives the results of never having
+ * stopped, bit for bit. The role of the reference's simulation checkpoint
+ * (src/core/runtime/simulation_checkpoint.cpp, --resume / --checkpoint-every;
+ * wire helpers include/core/runtime/checkpoint_format.hpp:22-60), not its file
+ * format: that one records the whole run configuration. */
+int nimbus_hip_member_snapshot_size(nimbus_hip_ctx* ctx, uint32_t id, uint64_t* bytes);
+int nimbus_hip_member_snapshot(nimbus_hip_ctx* ctx, uint32_t id, void* buf, uint64_t cap);
+int nimbus_hip_member_restore(nimbus_hip_ctx* ctx, const void* buf, uint64_t len, uint32_t* id);
+/* The same through a file. */
+int nimbus_hip_member_save(nimbus_hip_ctx* ctx, uint32_t id, const char* path);
+int nimbus_hip_member_load(nimbus_hip_ctx* ctx, const char* path, uint32_t* id);
 /* Advance each listed member by n_steps large steps; returns after the device finished.
  * The listed members of one precision run as one batch: every kernel launch of
  * the step covers all of them (uneven grids included), and the batch's whole

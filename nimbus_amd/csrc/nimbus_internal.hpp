@@ -34,6 +34,7 @@ int fill_from_base(const nimbus_grid& g, const nimbus_base_state& b, const nimbu
 int sounding_levels(const nimbus_sounding& s, int n, const double* z, nimbus_cm1_sounding& out);
 // Records msg as the calling thread's last error (nimbus_hip_last_error) and returns code.
 int set_error(int code, const std::string& msg);
+int write_npy(const char* path, const void* data, int dtype, int ndim, const std::int64_t* shape);
 double ic_uniform(std::uint64_t seed, std::uint32_t member, int i, int j, int k);
 int resolve_substeps(const nimbus_grid& g, const double* rho0, const double* p0, const nimbus_step_config& c);
 

@@ -13,6 +13,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <sys/stat.h>
 #include <string>
 #include <vector>
 
@@ -917,6 +918,240 @@ int nimbus_hip_member_diag(nimbus_hip_ctx* ctx, uint32_t id, nimbus_diag* d)
     d->mean_qr = v[5] / cells;
     d->steps = m->steps;
     return NIMBUS_OK;
+}
+
+// ---------------------------------------------------------------------
+// State I/O: NPY field export and member snapshots.
+// ---------------------------------------------------------------------
+
+int nimbus_hip_member_export_npy(nimbus_hip_ctx* ctx, uint32_t id, const char* dir)
+{
+    if (!ctx || !dir) return fail(NIMBUS_EINVAL, "export_npy: null argument");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "export_npy: unknown member");
+    struct stat st;
+    if (::stat(dir, &st) != 0 || !S_ISDIR(st.st_mode))
+        return fail(NIMBUS_EINVAL, std::string("export_npy: not a directory: ") + dir);
+    const int nx = m->g().nx, ny = m->g().ny, nz = m->g().nz;
+    const std::size_t n = std::size_t(nx) * ny * nz;
+    static const char* names[9] = {"u", "v", "w", "rho", "p", "theta", "qv", "qc", "qr"};
+    std::vector<std::vector<float>> f32(9, std::vector<float>(n));
+    std::vector<double> rain(std::size_t(nx) * ny);
+    if (m->elem == 4)
+    {
+        nimbus_host_state hs{f32[0].data(), f32[1].data(), f32[2].data(), f32[3].data(), f32[4].data(),
+                             f32[5].data(), f32[6].data(), f32[7].data(), f32[8].data(), rain.data()};
+        const int rc = member_io(ctx, id, &hs, false, 4);
+        if (rc != NIMBUS_OK) return rc;
+    }
+    else
+    {
+        std::vector<std::vector<double>> f64(9, std::vector<double>(n));
+        nimbus_host_state hs{f64[0].data(), f64[1].data(), f64[2].data(), f64[3].data(), f64[4].data(),
+                             f64[5].data(), f64[6].data(), f64[7].data(), f64[8].data(), rain.data()};
+        const int rc = member_io(ctx, id, &hs, false, 8);
+        if (rc != NIMBUS_OK) return rc;
+        for (int q = 0; q < 9; ++q)
+            for (std::size_t c = 0; c < n; ++c) f32[q][c] = float(f64[q][c]);
+    }
+    const std::int64_t shape3[3] = {nx, ny, nz}, shape2[2] = {nx, ny};
+    for (int q = 0; q < 9; ++q)
+    {
+        const std::string path = std::string(dir) + "/" + names[q] + ".npy";
+        const int rc = write_npy(path.c_str(), f32[q].data(), NIMBUS_NPY_F4, 3, shape3);
+        if (rc != NIMBUS_OK) return rc;
+    }
+    std::vector<float> rain32(rain.begin(), rain.end());
+    const std::string path = std::string(dir) + "/accumulated_rainfall.npy";
+    return write_npy(path.c_str(), rain32.data(), NIMBUS_NPY_F4, 2, shape2);
+}
+
+namespace
+{
+constexpr char kSnapMagic[8] = {'N', 'I', 'M', 'B', 'U', 'S', 'M', 'S'};
+constexpr std::uint32_t kSnapVersion = 1;
+
+struct SnapHead
+{
+    char magic[8];
+    std::uint32_t version, endian, grid_bytes, config_bytes;
+    std::int32_t precision, tile_w, k_chunks, xcd_remap;
+    std::int64_t steps;
+    std::uint64_t field_bytes, rain_bytes;
+};
+
+// FNV-1a, 64 bit
+std::uint64_t fnv1a(const std::uint8_t* p, std::size_t n, std::uint64_t h = 1469598103934665603ULL)
+{
+    for (std::size_t i = 0; i < n; ++i)
+    {
+        h ^= p[i];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+struct SnapSizes
+{
+    std::size_t field, rain, base, total;
+};
+
+SnapSizes snap_sizes(const Member& m)
+{
+    const bool hevi = m.desc.config.acoustic_mode == NIMBUS_ACOUSTIC_HEVI;
+    const Layout L = layout_for(m.args.g, m.elem, hevi, lateral_cells(m.desc.config, m.desc.grid));
+    SnapSizes z{};
+    z.field = L.field;
+    z.rain = std::size_t(m.g().nx) * m.g().ny * sizeof(double);
+    z.base = 6 * std::size_t(m.g().nz) * sizeof(double);
+    z.total = sizeof(SnapHead) + sizeof(nimbus_grid) + sizeof(nimbus_step_config) + z.base + 9 * z.field + z.rain +
+              sizeof(std::uint64_t);
+    return z;
+}
+
+void* prognostic(const Member& m, int q)
+{
+    const DevFields& f = m.f();
+    void* ds[9] = {f.u, f.v, f.w, f.rp, f.pp, f.th, f.qv, f.qc, f.qr};
+    return ds[q];
+}
+}  // namespace
+
+int nimbus_hip_member_snapshot_size(nimbus_hip_ctx* ctx, uint32_t id, uint64_t* bytes)
+{
+    if (!ctx || !bytes) return fail(NIMBUS_EINVAL, "snapshot_size: null argument");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "snapshot_size: unknown member");
+    *bytes = snap_sizes(*m).total;
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_member_snapshot(nimbus_hip_ctx* ctx, uint32_t id, void* buf, uint64_t cap)
+{
+    if (!ctx || !buf) return fail(NIMBUS_EINVAL, "snapshot: null argument");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "snapshot: unknown member");
+    const SnapSizes z = snap_sizes(*m);
+    if (cap < z.total) return fail(NIMBUS_EINVAL, "snapshot: buffer smaller than nimbus_hip_member_snapshot_size");
+    hipSetDevice(ctx->device);
+    NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::uint8_t* p = static_cast<std::uint8_t*>(buf);
+    SnapHead h{};
+    std::memcpy(h.magic, kSnapMagic, 8);
+    h.version = kSnapVersion;
+    h.endian = 0x01020304u;
+    h.grid_bytes = sizeof(nimbus_grid);
+    h.config_bytes = sizeof(nimbus_step_config);
+    h.precision = m->desc.precision;
+    h.tile_w = m->tile_w;
+    h.k_chunks = m->k_chunks;
+    h.xcd_remap = m->xcd_remap;
+    h.steps = m->steps;
+    h.field_bytes = z.field;
+    h.rain_bytes = z.rain;
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    std::memcpy(p, &m->desc.grid, sizeof(nimbus_grid));
+    p += sizeof(nimbus_grid);
+    std::memcpy(p, &m->desc.config, sizeof(nimbus_step_config));
+    p += sizeof(nimbus_step_config);
+    std::memcpy(p, m->host_base.data(), z.base);  // slots 0..5: rho0 p0 theta0 qv0 u0 v0
+    p += z.base;
+    for (int q = 0; q < 9; ++q, p += z.field)
+        NIMBUS_HIP_TRY(hipMemcpy(p, prognostic(*m, q), z.field, hipMemcpyDeviceToHost));
+    NIMBUS_HIP_TRY(hipMemcpy(p, m->f().rain, z.rain, hipMemcpyDeviceToHost));
+    p += z.rain;
+    const std::uint64_t sum = fnv1a(static_cast<const std::uint8_t*>(buf), z.total - sizeof(std::uint64_t));
+    std::memcpy(p, &sum, sizeof sum);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_member_restore(nimbus_hip_ctx* ctx, const void* buf, uint64_t len, uint32_t* id)
+{
+    if (!ctx || !buf || !id) return fail(NIMBUS_EINVAL, "restore: null argument");
+    const std::uint8_t* p0 = static_cast<const std::uint8_t*>(buf);
+    SnapHead h{};
+    if (len < sizeof h) return fail(NIMBUS_EINVAL, "restore: truncated snapshot");
+    std::memcpy(&h, p0, sizeof h);
+    if (std::memcmp(h.magic, kSnapMagic, 8) != 0) return fail(NIMBUS_EINVAL, "restore: not a member snapshot");
+    if (h.version != kSnapVersion || h.endian != 0x01020304u || h.grid_bytes != sizeof(nimbus_grid) ||
+        h.config_bytes != sizeof(nimbus_step_config))
+        return fail(NIMBUS_EINVAL, "restore: snapshot written by an incompatible library version");
+    const std::size_t fixed = sizeof h + sizeof(nimbus_grid) + sizeof(nimbus_step_config);
+    if (len < fixed) return fail(NIMBUS_EINVAL, "restore: truncated snapshot");
+    nimbus_member_desc d{};
+    std::memcpy(&d.grid, p0 + sizeof h, sizeof(nimbus_grid));
+    std::memcpy(&d.config, p0 + sizeof h + sizeof(nimbus_grid), sizeof(nimbus_step_config));
+    d.precision = h.precision;
+    if (!valid_grid(d.grid)) return fail(NIMBUS_EINVAL, "restore: invalid grid in snapshot");
+    const std::size_t nz = std::size_t(d.grid.nz);
+    const std::size_t base_bytes = 6 * nz * sizeof(double);
+    const std::size_t total = fixed + base_bytes + 9 * h.field_bytes + h.rain_bytes + sizeof(std::uint64_t);
+    if (len != total) return fail(NIMBUS_EINVAL, "restore: snapshot length does not match its header");
+    std::uint64_t sum = 0;
+    std::memcpy(&sum, p0 + total - sizeof sum, sizeof sum);
+    if (sum != fnv1a(p0, total - sizeof sum)) return fail(NIMBUS_EINVAL, "restore: checksum mismatch");
+    std::vector<double> prof(6 * nz);
+    std::memcpy(prof.data(), p0 + fixed, base_bytes);
+    d.base = nimbus_base_state{prof.data(), prof.data() + nz, prof.data() + 2 * nz, prof.data() + 3 * nz,
+                               prof.data() + 4 * nz, prof.data() + 5 * nz};
+    const int rc = nimbus_hip_member_create(ctx, &d, id);
+    if (rc != NIMBUS_OK) return rc;
+    Member* m = find(ctx, *id);
+    const SnapSizes z = snap_sizes(*m);
+    if (z.field != h.field_bytes || z.rain != h.rain_bytes)
+    {
+        nimbus_hip_member_destroy(ctx, *id);
+        return fail(NIMBUS_EINVAL, "restore: field layout differs from the snapshot's");
+    }
+    set_geometry(*m, h.tile_w, h.k_chunks, h.xcd_remap);
+    m->steps = h.steps;
+    const std::uint8_t* p = p0 + fixed + base_bytes;
+    for (int q = 0; q < 9; ++q, p += z.field)
+    {
+        const hipError_t e = hipMemcpy(prognostic(*m, q), p, z.field, hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+        {
+            nimbus_hip_member_destroy(ctx, *id);
+            return hip_fail(e, "restore upload");
+        }
+    }
+    const hipError_t e = hipMemcpy(m->f().rain, p, z.rain, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+    {
+        nimbus_hip_member_destroy(ctx, *id);
+        return hip_fail(e, "restore upload");
+    }
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_member_save(nimbus_hip_ctx* ctx, uint32_t id, const char* path)
+{
+    if (!path) return fail(NIMBUS_EINVAL, "save: null path");
+    std::uint64_t n = 0;
+    int rc = nimbus_hip_member_snapshot_size(ctx, id, &n);
+    if (rc != NIMBUS_OK) return rc;
+    std::vector<std::uint8_t> buf(n);
+    rc = nimbus_hip_member_snapshot(ctx, id, buf.data(), n);
+    if (rc != NIMBUS_OK) return rc;
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(NIMBUS_EINVAL, std::string("save: cannot open ") + path);
+    const bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    if (std::fclose(f) != 0 || !ok) return fail(NIMBUS_EINVAL, std::string("save: write failed: ") + path);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_member_load(nimbus_hip_ctx* ctx, const char* path, uint32_t* id)
+{
+    if (!path) return fail(NIMBUS_EINVAL, "load: null path");
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(NIMBUS_EINVAL, std::string("load: cannot open ") + path);
+    std::vector<std::uint8_t> buf;
+    std::uint8_t chunk[1 << 16];
+    std::size_t got;
+    while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+    std::fclose(f);
+    return nimbus_hip_member_restore(ctx, buf.data(), buf.size(), id);
 }
 
 int nimbus_hip_step(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, int n_steps)

@@ -9,6 +9,7 @@ unloadable library raises, there is no fallback path.
 """
 import ctypes
 import os
+import struct
 
 import numpy as np
 
@@ -176,6 +177,13 @@ def load_library(path=LIB_PATH):
         "nimbus_ic_fill_base_f32": (ctypes.c_int, [P(Grid), P(BaseState), P(ICParams), P(HostState)]),
         "nimbus_ic_fill_base_f64": (ctypes.c_int, [P(Grid), P(BaseState), P(ICParams), P(HostState)]),
         "nimbus_ic_sounding_levels": (ctypes.c_int, [P(Sounding), ctypes.c_int, _c_double_p, P(Cm1Sounding)]),
+        "nimbus_npy_write": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_int64)]),
+        "nimbus_hip_member_export_npy": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_char_p]),
+        "nimbus_hip_member_snapshot_size": (ctypes.c_int, [vp, ctypes.c_uint32, P(ctypes.c_uint64)]),
+        "nimbus_hip_member_snapshot": (ctypes.c_int, [vp, ctypes.c_uint32, vp, ctypes.c_uint64]),
+        "nimbus_hip_member_restore": (ctypes.c_int, [vp, vp, ctypes.c_uint64, P(ctypes.c_uint32)]),
+        "nimbus_hip_member_save": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_char_p]),
+        "nimbus_hip_member_load": (ctypes.c_int, [vp, ctypes.c_char_p, P(ctypes.c_uint32)]),
         "nimbus_cm1_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, P(Cm1Sounding)]),
         "nimbus_cm1_read": (ctypes.c_int, [ctypes.c_char_p, P(Cm1Sounding)]),
         "nimbus_cm1_format": (ctypes.c_int, [P(Cm1Sounding), ctypes.c_char_p, ctypes.c_size_t,
@@ -351,6 +359,25 @@ def initial_state_from_base(grid, base, params, dtype=np.float32):
     return st
 
 
+# ---------------------------------------------------------------- state I/O
+
+NPY_F4, NPY_F8 = 0, 1
+# member snapshot header: magic, version, endian mark, grid and config sizes,
+# precision, tile_w, k_chunks, xcd_remap, steps, field bytes, rain bytes
+SNAP_HEAD = "<8s4I4iq2Q"
+
+
+def npy_write(path, array):
+    """Write a float32/float64 C-order array (1-3 dims) with the library's NPY writer."""
+    a = np.ascontiguousarray(array)
+    if a.dtype not in (np.float32, np.float64) or not 1 <= a.ndim <= 3:
+        raise NimbusError("npy_write: float32/float64 arrays of 1 to 3 dimensions")
+    shape = (ctypes.c_int64 * a.ndim)(*a.shape)
+    check(load_library().nimbus_npy_write(os.fsencode(path), a.ctypes.data_as(ctypes.c_void_p),
+                                          NPY_F4 if a.dtype == np.float32 else NPY_F8, a.ndim, shape),
+          "nimbus_npy_write")
+
+
 # ---------------------------------------------------------------- CM1 input_sounding
 
 CM1_COLUMNS = ("z", "theta", "qv", "u", "v", "p", "T")
@@ -512,6 +539,24 @@ class Context:
     def member(self, grid, config, base, precision=PRECISION_MIXED):
         return Member(self, grid, config, base, precision)
 
+    def restore(self, blob):
+        """A new member from a snapshot (Member.snapshot or the file Member.save wrote)."""
+        if isinstance(blob, (str, os.PathLike)):
+            with open(blob, "rb") as f:
+                blob = f.read()
+        blob = bytes(blob)
+        head = struct.unpack_from(SNAP_HEAD, blob, 0)
+        off = struct.calcsize(SNAP_HEAD)
+        grid = Grid.from_buffer_copy(blob, off)
+        config = StepConfig.from_buffer_copy(blob, off + ctypes.sizeof(Grid))
+        off += ctypes.sizeof(Grid) + ctypes.sizeof(StepConfig)
+        prof = np.frombuffer(blob, dtype="<f8", count=6 * grid.nz, offset=off).reshape(6, grid.nz)
+        base = {n: prof[q].copy() for q, n in enumerate(("rho0", "p0", "theta0", "qv0", "u0", "v0"))}
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        mid = ctypes.c_uint32()
+        check(self.lib.nimbus_hip_member_restore(self._h, buf, len(blob), ctypes.byref(mid)), "member_restore")
+        return Member(self, grid, config, base, head[5], existing_id=mid.value)
+
     def step(self, members, n_steps=1, wait=True):
         ids = (ctypes.c_uint32 * len(members))(*[m.id for m in members])
         fn = self.lib.nimbus_hip_step if wait else self.lib.nimbus_hip_step_async
@@ -547,7 +592,7 @@ class Context:
 class Member:
     """A device-resident ensemble member."""
 
-    def __init__(self, ctx, grid, config, base, precision):
+    def __init__(self, ctx, grid, config, base, precision, existing_id=None):
         self.ctx = ctx
         self.grid = grid
         self.config = config
@@ -556,9 +601,12 @@ class Member:
         self._base = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in base.items()}
         desc = MemberDesc(grid, config, precision, base_struct(self._base))
         self.bytes = ctx.lib.nimbus_hip_member_bytes(ctypes.byref(desc))
-        mid = ctypes.c_uint32()
-        check(ctx.lib.nimbus_hip_member_create(ctx.handle, ctypes.byref(desc), ctypes.byref(mid)), "member_create")
-        self.id = mid.value
+        if existing_id is None:
+            mid = ctypes.c_uint32()
+            check(ctx.lib.nimbus_hip_member_create(ctx.handle, ctypes.byref(desc), ctypes.byref(mid)),
+                  "member_create")
+            existing_id = mid.value
+        self.id = existing_id
         self.substeps = resolve_substeps(grid, self._base, config)
 
     def upload(self, st):
@@ -577,6 +625,23 @@ class Member:
         d = Diag()
         check(self.ctx.lib.nimbus_hip_member_diag(self.ctx.handle, self.id, ctypes.byref(d)), "diag")
         return {n: getattr(d, n) for n, _ in Diag._fields_}
+
+    def export_npy(self, directory):
+        """u, v, w, theta, p, rho, qv, qc, qr and accumulated_rainfall as .npy files in `directory`."""
+        check(self.ctx.lib.nimbus_hip_member_export_npy(self.ctx.handle, self.id, os.fsencode(directory)),
+              "export_npy")
+
+    def snapshot(self):
+        """The member's snapshot as bytes (restore with Context.restore)."""
+        n = ctypes.c_uint64()
+        check(self.ctx.lib.nimbus_hip_member_snapshot_size(self.ctx.handle, self.id, ctypes.byref(n)),
+              "snapshot_size")
+        buf = ctypes.create_string_buffer(n.value)
+        check(self.ctx.lib.nimbus_hip_member_snapshot(self.ctx.handle, self.id, buf, n.value), "snapshot")
+        return buf.raw
+
+    def save(self, path):
+        check(self.ctx.lib.nimbus_hip_member_save(self.ctx.handle, self.id, os.fsencode(path)), "member_save")
 
     def set_tiling(self, tile_w, k_chunks, xcd_remap):
         """Launch geometry: tile_w in {32, 64}, k_chunks in 1..nz, XCD-contiguous order on/off."""

@@ -13,11 +13,21 @@ in the batch, so a result is bitwise the same alone and in the mix.
 
 The initial state of an arrived request is generated on a host thread while
 the device works on the current batch.
+
+With `preempt` on, a request that does not fit may evict resident members
+whose remaining work (cells x steps) is at least `preempt_ratio` times its
+own: each is snapshotted to host memory (nimbus_hip_member_snapshot),
+destroyed, and queued behind the request; when it is admitted again it is
+restored and continues bit for bit where it stopped. A member is evicted at
+most once. With `results_dir` set, every finished member's fields are
+delivered as .npy files (nimbus_hip_member_export_npy) in
+results_dir/req_<rid>/.
 """
 import collections
 import concurrent.futures
 import dataclasses
 import math
+import os
 import time
 
 import numpy as np
@@ -83,6 +93,7 @@ class Report:
     rounds: int
     peak_bytes: int
     peak_members: int
+    preemptions: int = 0
 
     @property
     def cell_updates(self):
@@ -102,11 +113,11 @@ class Report:
                 "p50_latency_s": round(self.latency_percentile(50), 3),
                 "p99_latency_s": round(self.latency_percentile(99), 3),
                 "rounds": self.rounds, "peak_resident_bytes": self.peak_bytes,
-                "peak_resident_members": self.peak_members}
+                "peak_resident_members": self.peak_members, "preemptions": self.preemptions}
 
 
 class _Slot:
-    __slots__ = ("req", "member", "remaining", "bytes", "admitted")
+    __slots__ = ("req", "member", "remaining", "bytes", "admitted", "evicted")
 
     def __init__(self, req, member, nbytes, admitted):
         self.req = req
@@ -114,13 +125,31 @@ class _Slot:
         self.remaining = req.workload.steps
         self.bytes = nbytes
         self.admitted = admitted
+        self.evicted = False
+
+    @property
+    def work(self):
+        return self.remaining * self.req.workload.cells
+
+
+class _Suspended:
+    """An evicted member: its host snapshot and the slot it came from."""
+    __slots__ = ("slot", "blob")
+
+    def __init__(self, slot, blob):
+        self.slot = slot
+        self.blob = blob
+
+    @property
+    def req(self):
+        return self.slot.req
 
 
 class Server:
     """Continuous-batching request loop over one device context."""
 
     def __init__(self, ctx, budget_bytes=200e9, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI,
-                 quantum=16, graphs=True, host_workers=4):
+                 quantum=16, graphs=True, host_workers=4, preempt=False, preempt_ratio=4.0, results_dir=None):
         self.ctx = ctx
         self.budget = int(budget_bytes)
         self.precision = precision
@@ -129,6 +158,10 @@ class Server:
         self.graphs = graphs
         self.host_workers = host_workers
         self.dtype = np.float32 if precision == L.PRECISION_MIXED else np.float64
+        self.preempt = bool(preempt)
+        self.preempt_ratio = float(preempt_ratio)
+        self.results_dir = results_dir
+        self.preemptions = 0
         self._bytes = {}
 
     def footprint(self, req):
@@ -148,6 +181,7 @@ class Server:
         Returns a Report; results in completion order. `keep_fields` lists
         request ids whose final fields are downloaded."""
         keep = set(keep_fields)
+        self.preemptions = 0
         queue = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         waiting = collections.deque()
         active = []
@@ -167,7 +201,16 @@ class Server:
                     prepared[r.rid] = pool.submit(self._prepare, r)
                     waiting.append(r)
                 while waiting:
-                    r = waiting[0]
+                    item = waiting[0]
+                    if isinstance(item, _Suspended):
+                        if used + item.slot.bytes > self.budget:
+                            break
+                        waiting.popleft()
+                        item.slot.member = self.ctx.restore(item.blob)
+                        active.append(item.slot)
+                        used += item.slot.bytes
+                        continue
+                    r = item
                     nb = self.footprint(r)
                     if nb > self.budget:
                         rejected.append(r.rid)
@@ -175,7 +218,18 @@ class Server:
                         prepared.pop(r.rid).cancel()
                         continue
                     if used + nb > self.budget:
-                        break
+                        victims = self._victims(active, r, used + nb - self.budget)
+                        if not victims:
+                            break
+                        for v in victims:
+                            blob = v.member.snapshot()
+                            v.member.destroy()
+                            v.member = None
+                            v.evicted = True
+                            active.remove(v)
+                            used -= v.bytes
+                            self.preemptions += 1
+                            waiting.insert(1, _Suspended(v, blob))
                     waiting.popleft()
                     base, st = prepared.pop(r.rid).result()
                     wl = r.workload
@@ -203,6 +257,10 @@ class Server:
                         continue
                     d = s.member.diag()
                     fields = s.member.download() if s.req.rid in keep else None
+                    if self.results_dir is not None:
+                        out = os.path.join(self.results_dir, f"req_{s.req.rid}")
+                        os.makedirs(out, exist_ok=True)
+                        s.member.export_npy(out)
                     wl = s.req.workload
                     results.append(Result(s.req.rid, wl.cells, wl.steps, s.req.arrival, s.admitted, now(), d,
                                           fields))
@@ -212,10 +270,28 @@ class Server:
         finally:
             for s in active:
                 s.member.destroy()
+            waiting.clear()
             pool.shutdown(wait=True, cancel_futures=True)
             self.ctx.set_graphs(False)
         wall = (clock() - start)
-        return Report(results, rejected, wall, dev_seconds, rounds, peak_bytes, peak_members)
+        return Report(results, rejected, wall, dev_seconds, rounds, peak_bytes, peak_members, self.preemptions)
+
+    def _victims(self, active, req, need):
+        """Resident members to evict so that `need` more bytes fit, largest
+        remaining work first, each with at least preempt_ratio times the
+        request's work and not evicted before; [] when they cannot free enough."""
+        if not self.preempt:
+            return []
+        work = req.workload.steps * req.workload.cells
+        cand = sorted((s for s in active if not s.evicted and s.work >= self.preempt_ratio * work),
+                      key=lambda s: -s.work)
+        out, freed = [], 0
+        for s in cand:
+            if freed >= need:
+                break
+            out.append(s)
+            freed += s.bytes
+        return out if freed >= need else []
 
 
 def run_alone(ctx, req, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI, graphs=True):

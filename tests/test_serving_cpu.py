@@ -1,6 +1,8 @@
 """Serving loop logic on the CPU with a stand-in device context: admission by
 HBM budget in arrival order, rejection of requests larger than the budget,
 step accounting per member and latency percentiles. No device is touched."""
+import os
+
 import numpy as np
 import pytest
 
@@ -31,18 +33,33 @@ class FakeMember:
         assert not self.destroyed
         self.destroyed = True
 
+    def snapshot(self):
+        return {"grid": self.grid, "steps": self.steps}
+
+    def export_npy(self, directory):
+        self.ctx.exported.append(directory)
+
 
 class FakeContext:
     """Records batches; each step 'takes' 1 ms of device time."""
 
     def __init__(self):
         self.created = []
+        self.restored = []
+        self.exported = []
         self.batches = []
         self.ms = 0.0
         self.graphs = False
 
     def member(self, grid, config, base, precision=L.PRECISION_MIXED):
         return FakeMember(self, grid, config, base, precision)
+
+    def restore(self, blob):
+        m = FakeMember(self, blob["grid"], None, None, None)
+        m.steps = blob["steps"]
+        m.uploaded = True
+        self.restored.append(m)
+        return m
 
     def step(self, members, n_steps=1, wait=True):
         assert all(m.uploaded and not m.destroyed for m in members)
@@ -124,3 +141,38 @@ def test_latency_percentiles_and_throughput_accounting():
     assert 0.0 <= s["p50_latency_s"] <= s["p99_latency_s"]
     assert s["cell_updates"] == sum(r.workload.cells * r.workload.steps for r in reqs)
     assert rep.device_step_seconds == pytest.approx(sum(n for _, n in ctx.batches) / 1e3)
+
+
+def test_preemption_lets_a_short_request_through_and_resumes_the_long_one(tmp_path):
+    """A long request fills the budget; a short one arrives later. Without
+    preemption it waits for the long one; with it the long member is
+    snapshotted, the short one runs, and the long one resumes from its
+    snapshot and still takes exactly its own number of steps."""
+    long_r = W.Workload("long", 48, 48, 40, 1000.0, 500.0, 6.0, 6, 400, seed=9, member=0)
+    short_r = W.Workload("short", 48, 48, 40, 1000.0, 500.0, 6.0, 6, 10, seed=9, member=1)
+    reqs = [SV.Request(0, 0.0, long_r), SV.Request(1, 0.0, short_r)]
+    one = L.member_bytes(long_r.grid, long_r.config(acoustic_mode=L.ACOUSTIC_HEVI), long_r.base())
+    for preempt in (False, True):
+        ctx = FakeContext()
+        srv = SV.Server(ctx, budget_bytes=int(1.5 * one), quantum=8, preempt=preempt,
+                        results_dir=str(tmp_path / str(preempt)))
+        # the short request is admitted only after the long one is resident
+        reqs[1].arrival = 0.0
+        rep = srv.run(reqs, clock=_stepping_clock(), sleep=lambda s: None)
+        order = [r.rid for r in rep.results]
+        by = {r.rid: r for r in rep.results}
+        assert by[0].diag["steps"] == 400 and by[1].diag["steps"] == 10
+        if preempt:
+            assert order == [1, 0] and rep.preemptions == 1 and len(ctx.restored) == 1
+        else:
+            assert order == [0, 1] and rep.preemptions == 0 and not ctx.restored
+        assert sorted(os.path.basename(d) for d in ctx.exported) == ["req_0", "req_1"]
+
+
+def _stepping_clock():
+    t = [0.0]
+
+    def clock():
+        t[0] += 1e-3
+        return t[0]
+    return clock
