@@ -65,7 +65,7 @@ typedef struct nimbus_hip_ctx nimbus_hip_ctx;
 
 typedef struct
 {
-    int nx, ny, nz;       /* cells including lateral ghosts; nx, ny >= 5, 4 <= nz <= 1024 */
+    int nx, ny, nz;       /* cells including lateral ghosts; nx, ny >= 5, 4 <= nz <= 1024, fewer than 2^28 cells (rows padded to 64 B) */
     double dx, dy, dz;    /* metres */
 } nimbus_grid;
 

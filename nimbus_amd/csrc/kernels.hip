@@ -94,6 +94,33 @@ __device__ __forceinline__ gptr<const double> slot(const double* base, int s, in
     return G(base) + std::size_t(s) * nz;
 }
 
+// One field as a raw buffer: the base address in a scalar buffer resource
+// and 32-bit byte offsets, the lane-dependent part (position in a plane) in
+// a VGPR and the plane part in an SGPR. A stencil's loads then cost no
+// 64-bit address arithmetic per operand. Offsets stay below 2^31: a member
+// field holds fewer than 2^28 padded cells (runtime grid check).
+template <class S>
+struct BufField
+{
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ S operator()(int voff, int soff) const
+    {
+        if constexpr (sizeof(S) == sizeof(float))
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+        else
+            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    }
+};
+
+// Buffer resource word 3 for gfx9-family parts: 32-bit elements, raw (stride 0) addressing.
+constexpr int kBufferWord3 = 0x00020000;
+
+template <class S>
+__device__ __forceinline__ BufField<S> buf(const void* p, std::size_t bytes)
+{
+    return BufField<S>{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), kBufferWord3)};
+}
+
 // ---------------------------------------------------------------------
 // Work decoding
 // ---------------------------------------------------------------------
@@ -1622,10 +1649,19 @@ template <class S>
 struct SgsView
 {
     using T = typename AccT<S>::type;  // FP32 arithmetic for FP32 storage, FP64 for FP64
+    static constexpr int es = int(sizeof(S));
     gptr<const S> u, v, w, rho_p, th, qv;
+    BufField<S> bu, bv, bw;            // the same u, v, w as buffers
     gptr<const double> rho0;
     int nx, ny, nz, px, plane;
     int wx, wy;
+    // byte offsets of a column position (lane part) and of a level (uniform part)
+    __device__ int xo(int i) const { return i * es; }
+    __device__ int yo(int j) const { return j * px * es; }
+    __device__ int zo(int k) const { return k * plane * es; }
+    __device__ T Ub(int i, int j, int k) const { return T(bu(xo(i) + yo(j), zo(k))); }
+    __device__ T Vb(int i, int j, int k) const { return T(bv(xo(i) + yo(j), zo(k))); }
+    __device__ T Wb(int i, int j, int k) const { return T(bw(xo(i) + yo(j), zo(k))); }
     __device__ int at(int i, int j, int k) const { return k * plane + j * px + i; }
     __device__ int ip(int i) const { return wx ? (i <= 1 ? nx - 2 : min(i - 1, nx - 2)) : max(i - 1, 0); }
     __device__ int in(int i) const { return wx ? (i >= nx - 2 ? 1 : max(i + 1, 1)) : min(i + 1, nx - 1); }
@@ -1660,6 +1696,10 @@ __device__ SgsView<S> sgs_view(const MemberArgs& A, int n)
     SgsView<S> s;
     s.u = P<S>(cur.u); s.v = P<S>(cur.v); s.w = P<S>(cur.w); s.rho_p = P<S>(f.rp); s.th = P<S>(cur.th);
     s.qv = P<S>(cur.qv);
+    const std::size_t fbytes = std::size_t(A.g.plane()) * A.g.nz * sizeof(S);
+    s.bu = buf<S>(cur.u, fbytes);
+    s.bv = buf<S>(cur.v, fbytes);
+    s.bw = buf<S>(cur.w, fbytes);
     s.rho0 = slot(f.base, kBaseRho0, A.g.nz);
     s.nx = A.g.nx; s.ny = A.g.ny; s.nz = A.g.nz; s.px = A.g.px; s.plane = int(A.g.plane());
     s.wx = A.p.sgs_wrap_x;
@@ -1671,10 +1711,10 @@ __device__ SgsView<S> sgs_view(const MemberArgs& A, int n)
 // neighbour indices are the reference's (clamped, or wrapped on periodic
 // axes) and every value is loaded before the arithmetic; edge strains of
 // edges on or above the lid and below the ground are zero.
-template <class S>
-__device__ typename AccT<S>::type sgs_strain_sq(const SgsView<S>& s, int mode, typename AccT<S>::type idx,
-                                                typename AccT<S>::type idy, typename AccT<S>::type idz, int i, int j,
-                                                int k)
+template <class S, int Mode>
+__device__ __forceinline__ typename AccT<S>::type sgs_strain_sq(const SgsView<S>& s, typename AccT<S>::type idx,
+                                                                typename AccT<S>::type idy,
+                                                                typename AccT<S>::type idz, int i, int j, int k)
 {
     using T = typename AccT<S>::type;
     const int NZ = s.nz;
@@ -1683,36 +1723,58 @@ __device__ typename AccT<S>::type sgs_strain_sq(const SgsView<S>& s, int mode, t
     const int ii[2] = {im, i}, jj[2] = {jm, j};
     const bool kv[2] = {k >= 1, k <= NZ - 2};  // edges k-1/2 and k+1/2 exist
     const int kl[2] = {k >= 1 ? k - 1 : 0, k <= NZ - 2 ? k : NZ - 2};
-    T u0 = s.U(i, j, k), uw = s.U(im, j, k), v0 = s.V(i, j, k), vs = s.V(i, jm, k);
-    T w0 = T(s.w[s.at(i, j, k)]), wb = T(s.w[s.at(i, j, k >= 1 ? k - 1 : 0)]);
-    T a12[2][2][4];  // s12 operands: u(ii, jn(jj)), u(ii, jj), v(in(ii), jj), v(ii, jj)
-    T xzv[2][2][4];  // xz operands: u(ii, kk+1), u(ii, kk), w(in(ii), kk), w(ii, kk)
-    T yzv[2][2][4];  // yz operands: v(jj, kk+1), v(jj, kk), w(jn(jj), kk), w(jj, kk)
+    // Only the operands the mode's formula uses are loaded: horizontal
+    // derivatives of u, v and w unless it is the vertical-only mode, vertical
+    // derivatives of u, v, w unless it is the horizontal-only mode.
+    constexpr bool hz = Mode != NIMBUS_SGS_VERTICAL, vt = Mode != NIMBUS_SGS_HORIZONTAL;
+    T u0 = T(0), uw = T(0), v0 = T(0), vs = T(0), w0 = T(0), wb = T(0);
+    T a12[2][2][4] = {};  // s12 operands: u(ii, jn(jj)), u(ii, jj), v(in(ii), jj), v(ii, jj)
+    T xzv[2][2][4] = {};  // xz operands: u(ii, kk+1), u(ii, kk), w(in(ii), kk), w(ii, kk)
+    T yzv[2][2][4] = {};  // yz operands: v(jj, kk+1), v(jj, kk), w(jn(jj), kk), w(jj, kk)
+    if (hz)
+    {
+        u0 = s.Ub(i, j, k);
+        uw = s.Ub(im, j, k);
+        v0 = s.Vb(i, j, k);
+        vs = s.Vb(i, jm, k);
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
-        {
-            a12[x][y][0] = s.U(ii[x], s.jn(jj[y]), k);
-            a12[x][y][1] = s.U(ii[x], jj[y], k);
-            a12[x][y][2] = s.V(s.in(ii[x]), jj[y], k);
-            a12[x][y][3] = s.V(ii[x], jj[y], k);
-        }
+            for (int y = 0; y < 2; ++y)
+            {
+                a12[x][y][0] = s.Ub(ii[x], s.jn(jj[y]), k);
+                a12[x][y][1] = s.Ub(ii[x], jj[y], k);
+                a12[x][y][2] = s.Vb(s.in(ii[x]), jj[y], k);
+                a12[x][y][3] = s.Vb(ii[x], jj[y], k);
+            }
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int z = 0; z < 2; ++z)
-        {
-            const int kk = kl[z];
-            xzv[x][z][0] = s.U(ii[x], j, kk + 1);
-            xzv[x][z][1] = s.U(ii[x], j, kk);
-            xzv[x][z][2] = T(s.w[s.at(s.in(ii[x]), j, kk)]);
-            xzv[x][z][3] = T(s.w[s.at(ii[x], j, kk)]);
-            yzv[x][z][0] = s.V(i, jj[x], kk + 1);
-            yzv[x][z][1] = s.V(i, jj[x], kk);
-            yzv[x][z][2] = T(s.w[s.at(i, s.jn(jj[x]), kk)]);
-            yzv[x][z][3] = T(s.w[s.at(i, jj[x], kk)]);
-        }
+            for (int z = 0; z < 2; ++z)
+            {
+                const int kk = kl[z];
+                xzv[x][z][2] = s.Wb(s.in(ii[x]), j, kk);
+                xzv[x][z][3] = s.Wb(ii[x], j, kk);
+                yzv[x][z][2] = s.Wb(i, s.jn(jj[x]), kk);
+                yzv[x][z][3] = s.Wb(i, jj[x], kk);
+            }
+    }
+    if (vt)
+    {
+        w0 = s.Wb(i, j, k);
+        wb = s.Wb(i, j, k >= 1 ? k - 1 : 0);
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int z = 0; z < 2; ++z)
+            {
+                const int kk = kl[z];
+                xzv[x][z][0] = s.Ub(ii[x], j, kk + 1);
+                xzv[x][z][1] = s.Ub(ii[x], j, kk);
+                yzv[x][z][0] = s.Vb(i, jj[x], kk + 1);
+                yzv[x][z][1] = s.Vb(i, jj[x], kk);
+            }
+    }
     pin(u0); pin(uw); pin(v0); pin(vs); pin(w0); pin(wb);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -1750,9 +1812,9 @@ __device__ typename AccT<S>::type sgs_strain_sq(const SgsView<S>& s, int mode, t
     const G2 ya = yz(0, 0), yb = yz(0, 1), yc = yz(1, 0), yd = yz(1, 1);
     auto avg_b = [&](G2 p, G2 r, G2 t, G2 z) { return q * (p.b * p.b + r.b * r.b + t.b * t.b + z.b * z.b); };
     auto avg_a = [&](G2 p, G2 r, G2 t, G2 z) { return q * (p.a * p.a + r.a * r.a + t.a * t.a + z.a * z.a); };
-    if (mode == NIMBUS_SGS_HORIZONTAL)
+    if constexpr (Mode == NIMBUS_SGS_HORIZONTAL)
         return two * (S11 * S11 + S22 * S22) + T(4) * S12_sq + avg_b(xa, xb, xc, xd) + avg_b(ya, yb, yc, yd);
-    if (mode == NIMBUS_SGS_VERTICAL) return avg_a(xa, xb, xc, xd) + avg_a(ya, yb, yc, yd) + two * S33 * S33;
+    if constexpr (Mode == NIMBUS_SGS_VERTICAL) return avg_a(xa, xb, xc, xd) + avg_a(ya, yb, yc, yd) + two * S33 * S33;
     auto sym = [&](G2 p) {
         const T e = h * (p.a + p.b);
         return e * e;
@@ -1782,11 +1844,15 @@ __global__ __launch_bounds__(kBlock) void k_sgs_closure(const MemberArgs* __rest
     if (!cl.valid) return;
     const SgsView<S> sv = sgs_view<S>(A, n);
     const int c = sv.at(cl.i, cl.j, cl.k);
+    const T idx = T(A.g.idx), idy = T(A.g.idy), idz = T(A.g.idz);
     T str;
-    if constexpr (sizeof(T) == sizeof(double))
-        str = sgs_strain_sq<S>(sv, A.p.sgs_mode, A.g.idx, A.g.idy, A.g.idz, cl.i, cl.j, cl.k);
+    // one code path per mode (uniform over the workgroup)
+    if (A.p.sgs_mode == NIMBUS_SGS_HORIZONTAL)
+        str = sgs_strain_sq<S, NIMBUS_SGS_HORIZONTAL>(sv, idx, idy, idz, cl.i, cl.j, cl.k);
+    else if (A.p.sgs_mode == NIMBUS_SGS_VERTICAL)
+        str = sgs_strain_sq<S, NIMBUS_SGS_VERTICAL>(sv, idx, idy, idz, cl.i, cl.j, cl.k);
     else
-        str = sgs_strain_sq<S>(sv, A.p.sgs_mode, T(A.g.idx), T(A.g.idy), T(A.g.idz), cl.i, cl.j, cl.k);
+        str = sgs_strain_sq<S, NIMBUS_SGS_3D>(sv, idx, idy, idz, cl.i, cl.j, cl.k);
     const double nu = fmin(A.p.sgs_c2d2 * sqrt(double(str)), A.p.sgs_nu_max);
     P<S>(A.f.k1u)[c] = S(nu);
     P<S>(A.f.k1v)[c] = S(fmin(nu * A.p.sgs_inv_pr, A.p.sgs_k_max));
@@ -1799,16 +1865,10 @@ __global__ __launch_bounds__(kBlock) void k_sgs_closure(const MemberArgs* __rest
 // neighbour rules (west iw = prev(i), east ie = next(i), likewise in y; next
 // of prev is the cell itself on every face these loops visit); the 59 the
 // formulas use are loaded before any arithmetic.
-template <class S>
-__global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __restrict__ ma,
-                                                         const int2* __restrict__ map, int n)
+template <class S, int Mode>
+__device__ __forceinline__ void sgs_momentum_cell(const MemberArgs& A, const Cell& cl, int n)
 {
     using T = typename AccT<S>::type;
-    const Work wk = work_of(map);
-    const MemberArgs& A = ma[wk.m];
-    if (n >= A.p.sgs_n) return;
-    const Cell cl = cell_of(A, wk.local);
-    if (!cl.valid) return;
     const SgsView<S> s = sgs_view<S>(A, n);
     const int NX = s.nx, NY = s.ny, NZ = s.nz;
     const int i = cl.i, j = cl.j, k = cl.k;
@@ -1835,23 +1895,40 @@ __global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __res
     const int km = k > 0 ? k - 1 : k, kp = k + 1;  // k <= NZ - 2 here
     const bool lo = k > 0;                          // a face below exists
     // cell offsets: X[0..2] = iw, i, ie; Y[0..2] = js, j, jn; Z[0..2] = km, k, kp
-    auto at3 = [&](int a, int b, int cc) {
-        const int X = a == 0 ? iw : (a == 1 ? i : ie);
-        const int Y = b == 0 ? js : (b == 1 ? j : jn);
-        const int Z = cc == 0 ? km : (cc == 1 ? k : kp);
-        return s.at(X, Y, Z);
-    };
-    const gptr<const S> nuf = P<S>(A.f.k1u);
+    // (bytes; the level is uniform over the workgroup, so the plane part is scalar)
+    constexpr int es = int(sizeof(S));
+    const int xs[3] = {iw * es, i * es, ie * es}, ys[3] = {js * s.px * es, j * s.px * es, jn * s.px * es};
+    const int zs[3] = {km * s.plane * es, k * s.plane * es, kp * s.plane * es};
+    const std::size_t fbytes = std::size_t(s.plane) * NZ * es;
+    const BufField<S> bu = s.bu, bv = s.bv, bw = s.bw;
+    const BufField<S> bn = buf<S>(A.f.k1u, fbytes), br = buf<S>(A.f.rp, fbytes);
+    auto ld = [&](const BufField<S>& f, int a, int b, int cc) { return T(f(xs[a] + ys[b], zs[cc])); };
     // U, V, W at the nine cells each equation uses; nu and rho' at sixteen
-    T u_ = T(s.u[at3(1, 1, 1)]), u_e = T(s.u[at3(2, 1, 1)]), u_w = T(s.u[at3(0, 1, 1)]);
-    T u_n = T(s.u[at3(1, 2, 1)]), u_s = T(s.u[at3(1, 0, 1)]), u_wn = T(s.u[at3(0, 2, 1)]);
-    T u_t = T(s.u[at3(1, 1, 2)]), u_b = T(s.u[at3(1, 1, 0)]), u_wt = T(s.u[at3(0, 1, 2)]);
-    T v_ = T(s.v[at3(1, 1, 1)]), v_n = T(s.v[at3(1, 2, 1)]), v_s = T(s.v[at3(1, 0, 1)]);
-    T v_e = T(s.v[at3(2, 1, 1)]), v_w = T(s.v[at3(0, 1, 1)]), v_es = T(s.v[at3(2, 0, 1)]);
-    T v_t = T(s.v[at3(1, 1, 2)]), v_b = T(s.v[at3(1, 1, 0)]), v_st = T(s.v[at3(1, 0, 2)]);
-    T w_ = T(s.w[at3(1, 1, 1)]), w_e = T(s.w[at3(2, 1, 1)]), w_w = T(s.w[at3(0, 1, 1)]);
-    T w_n = T(s.w[at3(1, 2, 1)]), w_s = T(s.w[at3(1, 0, 1)]), w_t = T(s.w[at3(1, 1, 2)]);
-    T w_b = T(s.w[at3(1, 1, 0)]), w_eb = T(s.w[at3(2, 1, 0)]), w_nb = T(s.w[at3(1, 2, 0)]);
+    T u_ = ld(bu, 1, 1, 1), u_e = ld(bu, 2, 1, 1), u_w = ld(bu, 0, 1, 1);
+    T u_n = ld(bu, 1, 2, 1), u_s = ld(bu, 1, 0, 1), u_wn = ld(bu, 0, 2, 1);
+    // Mode < 0: the member's mode at run time (one code path for all modes)
+    const int mode = Mode < 0 ? A.p.sgs_mode : Mode;
+    const bool use_h = mode != NIMBUS_SGS_VERTICAL, use_v = mode != NIMBUS_SGS_HORIZONTAL;
+    // operands of vertical derivatives only: not loaded in the horizontal-only mode
+    T u_t = T(0), u_b = T(0), u_wt = T(0), v_t = T(0), v_b = T(0), v_st = T(0);
+    T w_t = T(0), w_b = T(0), w_eb = T(0), w_nb = T(0);
+    if (use_v)
+    {
+        u_t = ld(bu, 1, 1, 2);
+        u_b = ld(bu, 1, 1, 0);
+        u_wt = ld(bu, 0, 1, 2);
+        v_t = ld(bv, 1, 1, 2);
+        v_b = ld(bv, 1, 1, 0);
+        v_st = ld(bv, 1, 0, 2);
+        w_t = ld(bw, 1, 1, 2);
+        w_b = ld(bw, 1, 1, 0);
+        w_eb = ld(bw, 2, 1, 0);
+        w_nb = ld(bw, 1, 2, 0);
+    }
+    T v_ = ld(bv, 1, 1, 1), v_n = ld(bv, 1, 2, 1), v_s = ld(bv, 1, 0, 1);
+    T v_e = ld(bv, 2, 1, 1), v_w = ld(bv, 0, 1, 1), v_es = ld(bv, 2, 0, 1);
+    T w_ = ld(bw, 1, 1, 1), w_e = ld(bw, 2, 1, 1), w_w = ld(bw, 0, 1, 1);
+    T w_n = ld(bw, 1, 2, 1), w_s = ld(bw, 1, 0, 1);
     constexpr int kMu = 16;
     const int mu_pts[kMu][3] = {{1, 1, 1}, {2, 1, 1}, {0, 1, 1}, {1, 2, 1}, {1, 0, 1}, {2, 2, 1}, {2, 0, 1}, {0, 2, 1},
                                 {1, 1, 2}, {1, 1, 0}, {2, 1, 2}, {2, 1, 0}, {0, 1, 2}, {1, 2, 2}, {1, 2, 0}, {1, 0, 2}};
@@ -1859,9 +1936,14 @@ __global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __res
 #pragma unroll
     for (int q = 0; q < kMu; ++q)
     {
-        const int idx = at3(mu_pts[q][0], mu_pts[q][1], mu_pts[q][2]);
-        nu_[q] = T(nuf[idx]);
-        rp_[q] = T(s.rho_p[idx]);
+        nu_[q] = T(0);
+        rp_[q] = T(0);
+        // the three cells below (b, eb, nb) enter only vertical-derivative terms
+        if (use_v || mu_pts[q][2] != 0)
+        {
+            nu_[q] = ld(bn, mu_pts[q][0], mu_pts[q][1], mu_pts[q][2]);
+            rp_[q] = ld(br, mu_pts[q][0], mu_pts[q][1], mu_pts[q][2]);
+        }
     }
     const double r0k = s.rho0[k], r0m = s.rho0[km], r0p = s.rho0[kp];
     pin(u_); pin(u_e); pin(u_w); pin(u_n); pin(u_s); pin(u_wn); pin(u_t); pin(u_b); pin(u_wt);
@@ -1886,11 +1968,21 @@ __global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __res
     auto ovx = [&](T x) { return sizeof(T) == sizeof(double) ? x / dx : x * rdx; };
     auto ovy = [&](T x) { return sizeof(T) == sizeof(double) ? x / dy : x * rdy; };
     auto ovz = [&](T x) { return sizeof(T) == sizeof(double) ? x / dz : x * rdz; };
-    const bool use_h = A.p.sgs_mode != NIMBUS_SGS_VERTICAL, use_v = A.p.sgs_mode != NIMBUS_SGS_HORIZONTAL;
     const T wb = lo ? w_b : T(0);   // w below the cell (zero at the ground)
     const T web = lo ? w_eb : T(0);
     const T wnb = lo ? w_nb : T(0);
-    auto vert_mean = [&](T lower, T upper) { return (dz * lower + dz * upper) / (dz + dz); };
+    // the reference's thickness-weighted mean of two levels; FP32 multiplies by 1 / (2 dz)
+    const T r2dz = T(1) / (dz + dz);
+    auto vert_mean = [&](T lower, T upper) {
+        return sizeof(T) == sizeof(double) ? (dz * lower + dz * upper) / (dz + dz) : (dz * lower + dz * upper) * r2dz;
+    };
+    // quotient: a division in FP64, the hardware reciprocal and a multiply in FP32
+    auto quot = [](T a, T b) {
+        if constexpr (sizeof(T) == sizeof(double))
+            return a / b;
+        else
+            return a * __builtin_amdgcn_rcpf(b);
+    };
     T du = T(0);
     if (has_u)
     {
@@ -1912,7 +2004,7 @@ __global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __res
                               : T(0);
             div += ovz((T13t - T13b));
         }
-        du = div / (h * (R_[0] + R_[1]));
+        du = quot(div, h * (R_[0] + R_[1]));
     }
     T dv = T(0);
     if (has_v)
@@ -1935,7 +2027,7 @@ __global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __res
                               : T(0);
             div += ovz((T23t - T23b));
         }
-        dv = div / (h * (R_[0] + R_[3]));
+        dv = quot(div, h * (R_[0] + R_[3]));
     }
     T dw = T(0);
     if (has_w)
@@ -1961,24 +2053,32 @@ __global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __res
                            ((use_v ? ovz((v_st - v_s)) : T(0)) + ovy((w_ - w_s)));
             force += ovy(dz * (T32n - T32s));
         }
-        dw = force / mass;
+        dw = quot(force, mass);
     }
     commit(du, dv, dw);
 }
 
-// Scalar tendencies of theta and qv (eddy_viscosity.cpp:1953-2090),
-// committed into the other buffer set; the EOS projection after the last
-// substep when it lands in the home fields.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __restrict__ ma,
-                                                        const int2* __restrict__ map, int n)
+__global__ __launch_bounds__(kBlock) void k_sgs_momentum(const MemberArgs* __restrict__ ma,
+                                                         const int2* __restrict__ map, int n)
 {
-    using T = typename AccT<S>::type;
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     if (n >= A.p.sgs_n) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
+    // a single code path with the mode read at run time: per-mode copies of
+    // this kernel raise its register count and measured slower
+    sgs_momentum_cell<S, -1>(A, cl, n);
+}
+
+// Scalar tendencies of theta and qv (eddy_viscosity.cpp:1953-2090),
+// committed into the other buffer set; the EOS projection after the last
+// substep when it lands in the home fields.
+template <class S, int Mode>
+__device__ __forceinline__ void sgs_scalars_cell(const MemberArgs& A, const Cell& cl, int n)
+{
+    using T = typename AccT<S>::type;
     const SgsView<S> s = sgs_view<S>(A, n);
     const int NX = s.nx, NY = s.ny, NZ = s.nz;
     const int i = cl.i, j = cl.j, k = cl.k;
@@ -1987,7 +2087,7 @@ __global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __rest
     auto ovx = [&](T x) { return sizeof(T) == sizeof(double) ? x / dx : x * rdx; };
     auto ovy = [&](T x) { return sizeof(T) == sizeof(double) ? x / dy : x * rdy; };
     auto ovz = [&](T x) { return sizeof(T) == sizeof(double) ? x / dz : x * rdz; };
-    const bool use_h = A.p.sgs_mode != NIMBUS_SGS_VERTICAL, use_v = A.p.sgs_mode != NIMBUS_SGS_HORIZONTAL;
+    constexpr bool use_h = Mode != NIMBUS_SGS_VERTICAL, use_v = Mode != NIMBUS_SGS_HORIZONTAL;
     const int c = s.at(i, j, k);
     T dth = T(0), dqv = T(0);
     if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2)
@@ -1999,19 +2099,29 @@ __global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __rest
         const int ipp = !s.wx && !oe && i == NX - 2 ? i : s.in(i);
         // the seven-point star, loaded before any arithmetic (vertical neighbours clamped, masked below)
         const int km = k > 0 ? k - 1 : k, kp = k < NZ - 1 ? k + 1 : k;
-        const int pts[7] = {c, s.at(ipp, j, k), s.at(im, j, k), s.at(i, jpp, k), s.at(i, jm, k), s.at(i, j, kp),
-                            s.at(i, j, km)};
+        // lane part and level part of each point's byte offset
+        const int vo[7] = {s.xo(i) + s.yo(j),   s.xo(ipp) + s.yo(j), s.xo(im) + s.yo(j), s.xo(i) + s.yo(jpp),
+                           s.xo(i) + s.yo(jm),  s.xo(i) + s.yo(j),   s.xo(i) + s.yo(j)};
         const int kk[7] = {k, k, k, k, k, kp, km};
-        const gptr<const S> K1 = P<S>(A.f.k1v), K2 = P<S>(A.f.k1w);
+        const std::size_t fbytes = std::size_t(s.plane) * NZ * sizeof(S);
+        const SgsSet cur = sgs_set(A.f, n & 1);
+        const BufField<S> bR = buf<S>(A.f.rp, fbytes), bK1 = buf<S>(A.f.k1v, fbytes), bK2 = buf<S>(A.f.k1w, fbytes);
+        const BufField<S> bT = buf<S>(cur.th, fbytes), bQ = buf<S>(cur.qv, fbytes);
         T r[7], k1[7], k2[7], p1[7], p2[7];
 #pragma unroll
         for (int o = 0; o < 7; ++o)
         {
-            r[o] = T(S(s.rho0[kk[o]] + double(s.rho_p[pts[o]])));
-            k1[o] = T(K1[pts[o]]);
-            k2[o] = T(K2[pts[o]]);
-            p1[o] = T(s.th[pts[o]]);
-            p2[o] = T(s.qv[pts[o]]);
+            r[o] = k1[o] = k2[o] = p1[o] = p2[o] = T(0);
+            // lateral points for the horizontal terms, vertical ones for the vertical terms
+            if (o == 0 || (o <= 4 ? use_h : use_v))
+            {
+                const int so = s.zo(kk[o]);
+                r[o] = T(S(s.rho0[kk[o]] + double(bR(vo[o], so))));
+                k1[o] = T(bK1(vo[o], so));
+                k2[o] = T(bK2(vo[o], so));
+                p1[o] = T(bT(vo[o], so));
+                p2[o] = T(bQ(vo[o], so));
+            }
         }
         auto scalar = [&](const T* K, const T* P_) {
             T tr[7];
@@ -2032,7 +2142,10 @@ __global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __rest
                 const T Fb = k > 0 ? ovz(h * (tr[6] + tr[0]) * (P_[0] - P_[6])) : T(0);
                 div += ovz((Ft - Fb));
             }
-            return div / r[0];
+            if constexpr (sizeof(T) == sizeof(double))
+                return div / r[0];
+            else
+                return div * __builtin_amdgcn_rcpf(r[0]);
         };
         dth = scalar(k1, p1);
         dqv = scalar(k2, p2);
@@ -2044,6 +2157,24 @@ __global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __rest
     P<S>(nxt.qv)[c] = S(clamp_store<S>(double(s.qv[c]) + finite0(double(S(dqv))) * hstep, 0.0, 0.0f, 0.05f));
     // an even count ends in the home fields: the EOS projection follows the last commit here
     if (n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0) sgs_eos<S>(A, c, k, thn);
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __restrict__ ma,
+                                                        const int2* __restrict__ map, int n)
+{
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (n >= A.p.sgs_n) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    // one code path per mode (uniform over the workgroup)
+    if (A.p.sgs_mode == NIMBUS_SGS_HORIZONTAL)
+        sgs_scalars_cell<S, NIMBUS_SGS_HORIZONTAL>(A, cl, n);
+    else if (A.p.sgs_mode == NIMBUS_SGS_VERTICAL)
+        sgs_scalars_cell<S, NIMBUS_SGS_VERTICAL>(A, cl, n);
+    else
+        sgs_scalars_cell<S, NIMBUS_SGS_3D>(A, cl, n);
 }
 
 // Odd SGS counts: the state of the last substep back to the home fields,

@@ -121,7 +121,7 @@ int default_k_chunks(const DevGrid& g, int tile_w)
 bool valid_grid(const nimbus_grid& g)
 {
     return g.nx >= 5 && g.ny >= 5 && g.nz >= 4 && g.nz <= kMaxLevels && std::isfinite(g.dx) && g.dx > 0 && std::isfinite(g.dy) && g.dy > 0 &&
-           std::isfinite(g.dz) && g.dz > 0 && std::size_t(g.nx) * g.ny * g.nz < (std::size_t(1) << 31);
+           std::isfinite(g.dz) && g.dz > 0 && dev_grid(g, 4).cells() < (std::size_t(1) << 28);
 }
 
 bool valid_config(const nimbus_step_config& c)
