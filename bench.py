@@ -192,20 +192,37 @@ def ensemble_run(ctx, L, W, precision, mode):
             "device_step_seconds": s["device_step_seconds"], "p50_completion_s": s["p50_latency_s"],
             "p99_completion_s": s["p99_latency_s"], "max_completion_s": round(lat[-1], 3),
             "rounds": s["rounds"], "acoustic_mode": "hevi" if mode == L.ACOUSTIC_HEVI else "explicit",
-            "data": "seed 3: nx = ny in {48..160}, nz in {40, 60, 80}, dx in {500, 1000} m, dt = 6 dx/1000 s, "
-                    "6 substeps, perturbed soundings, 1800 simulated s each"}
+            "data": "seed 3: nx = ny in {48..160}, nz in {40, 60, 80}, dx in {500, 1000} m, "
+                    "dt = 6 min(dx, dz)/1000 s, 6 substeps, perturbed soundings, 1800 simulated s each"}
 
 
-def serving_run(ctx, L, seconds, precision, mode):
-    """Continuous batching of seeded Poisson requests (2/s) arriving over `seconds`."""
+def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5):
+    """Continuous batching of seeded Poisson requests (2/s) arriving over `seconds`,
+    then `fp64_sample` seeded requests rerun alone in FP64: the error of each
+    served (mixed) result against its FP64 rerun."""
+    import random
     from nimbus_amd import serving as SV
+    from nimbus_amd import accuracy as ACC
     reqs = SV.poisson_requests(seed=4, rate=2.0, horizon=float(seconds))
+    sample = sorted(random.Random(4).sample([r.rid for r in reqs], min(fp64_sample, len(reqs))))
     srv = SV.Server(ctx, budget_bytes=200e9, precision=precision, acoustic_mode=mode, quantum=16)
-    rep = srv.run(reqs)
+    rep = srv.run(reqs, keep_fields=sample)
     s = rep.summary()
     s["arrival_seconds"] = seconds
     s["data"] = ("seed 4 Poisson arrivals at 2/s; nx = ny in {48..256}, nz in {40, 60, 80}, dx 500-1000 m, "
                  "15-90 simulated minutes each; 200 GB HBM budget")
+    by_id = {r.rid: r for r in reqs}
+    errors = []
+    for res in rep.results:
+        if res.fields is None:
+            continue
+        req = by_id[res.rid]
+        fields, diag = SV.run_alone(ctx, req, precision=L.PRECISION_FP64, acoustic_mode=mode)
+        e = ACC.compare(req.workload, res.fields, fields, res.diag, diag)
+        errors.append({"rid": res.rid, "grid": req.workload.describe(), "steps": req.workload.steps,
+                       "max_w_error": e["max_w_error"], "rms_theta_p_error": e["rms_theta_p_error"],
+                       "rain_relative_error": e["rain_relative_error"], "targets_met": e["targets_met"]})
+    s["fp64_rerun"] = errors
     return s
 
 
@@ -308,6 +325,11 @@ def main():
         "kernels_ms_per_step": {k: per_step(t) for k, t in tables.items()},
         "diag_after_steps": {k: diag[k] for k in ("max_w", "rms_theta_p", "rain_total_mm")},
         "roofline": roofline_for(tables["mixed_hevi"], wl.cells, 4, args.workload, "hevi"),
+        # whole-step figure with the large-step byte budget B(N) = (68 + 17 N) s per cell update
+        "roofline_step_budget": {"bytes_per_cell_update": (68 + 17 * n_sub) * 4,
+                                 "achieved": round(value * (68 + 17 * n_sub) * 4 / 1e9, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(value * (68 + 17 * n_sub) * 4 / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline_mixed_explicit": roofline_for(tables["mixed_explicit"], wl.cells, 4, args.workload, "explicit"),
         "cpu_baseline": None,
         "host": {"platform": platform.platform(), **cpu_info()},

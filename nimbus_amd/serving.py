@@ -47,7 +47,7 @@ def poisson_requests(seed=4, rate=2.0, horizon=600.0, sizes=(48, 64, 96, 128, 16
                      levels=(40, 60, 80), dx_range=(500.0, 1000.0), minutes=(15.0, 90.0), substeps=6):
     """Seeded Poisson arrivals (rate per second over `horizon` seconds). Each
     request draws nx = ny from `sizes`, nz from `levels`, dx uniform in
-    `dx_range` (dt = 6 dx / 1000 s, 20 km top), a perturbed sounding and a
+    `dx_range` (dt = workloads.long_run_dt(dx, dz), 20 km top), a perturbed sounding and a
     simulated duration uniform in `minutes`."""
     s = W.SeededStream(seed)
     out = []
@@ -59,7 +59,7 @@ def poisson_requests(seed=4, rate=2.0, horizon=600.0, sizes=(48, 64, 96, 128, 16
         n = s.choice(list(sizes))
         nz = s.choice(list(levels))
         dx = round(s.between(*dx_range), 1)
-        dt = 6.0 * dx / 1000.0
+        dt = W.long_run_dt(dx, 20000.0 / nz)
         duration = 60.0 * s.between(*minutes)
         rid = len(out)
         wl = W.Workload(f"req{rid}", n, n, nz, dx, 20000.0 / nz, dt, substeps, max(1, int(round(duration / dt))),

@@ -5,8 +5,8 @@ c1  160x160x80, dx = dy = 500 m, dz = 250 m, dt = 3 s, 6 substeps, 1 simulated h
 c2  400x400x100, dx = dy = 250 m, dz = 200 m, dt = 1.5 s, 8 substeps, 1 simulated hour,
     three warm bubbles 15 km apart
 c3  32-member ensemble of uneven grids drawn from seed 3 (nx = ny in {48, 64, 96, 128, 160},
-    nz in {40, 60, 80}, dx in {500, 1000} m, dt = 6 dx / 1000 s, 6 substeps, 30 simulated
-    minutes) with per-member sounding perturbations
+    nz in {40, 60, 80}, dx in {500, 1000} m, dt = long_run_dt(dx, dz), 6 substeps, 30
+    simulated minutes) with per-member sounding perturbations
 All use the analytic Weisman-Klemp-type sounding with a quarter-circle
 hodograph, +2 K bubbles of 10 km x 1.4 km radius centred 1.4 km above ground,
 and seeded uniform +-0.1 K theta noise. Unless a workload sets sponge=False
@@ -110,6 +110,18 @@ class SeededStream:
         return lo + (hi - lo) * self.uniform()
 
 
+def long_run_dt(dx, dz):
+    """Large step for the generated long runs (ensemble members, serving
+    requests): 6 s per km of the finer of dx and dz. The momentum advection
+    of the slow step (Heun RK2 with fifth-order upwind differences) is stable
+    only up to Courant numbers of a few tenths (linear analysis of Wicker and
+    Skamarock 2002, Mon. Wea. Rev. 130), so with updrafts of up to 50 m/s the
+    vertical step is kept at w dt / dz <= 0.3, i.e. dt <= 6 dz / 1000 s.
+    With 6 dx / 1000 s alone, a 90-minute request at dz = 500 m, dt = 5.2 s
+    blew up in both precisions after about 70 minutes."""
+    return 6.0 * min(dx, dz) / 1000.0
+
+
 def perturbed_sounding(stream):
     """Per-member sounding perturbation: surface theta +-2 K, surface qv 11-16 g/kg,
     shear 10-40 m/s, hodograph curvature 0-1."""
@@ -125,7 +137,7 @@ def ensemble_c3(n_members=32, seed=3):
         n = s.choice([48, 64, 96, 128, 160])
         nz = s.choice([40, 60, 80])
         dx = s.choice([500.0, 1000.0])
-        dt = 6.0 * dx / 1000.0
+        dt = long_run_dt(dx, 20000.0 / nz)
         members.append(Workload(f"c3m{m}", n, n, nz, dx, 20000.0 / nz, dt, 6, int(round(1800.0 / dt)),
                                 seed=seed, member=m, sounding=perturbed_sounding(s)))
     return members
