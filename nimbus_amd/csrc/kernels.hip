@@ -2495,7 +2495,8 @@ void launch_seed_radiation(Launcher& L, const BatchView& b)
 template <class S>
 void launch_commit(Launcher& L, const BatchView& b, int n_done)
 {
-    launch_map(L, "acoustic_commit", k_commit<S>, b, S_ALL, W_CELL, n_done);
+    // a whole step (n_done < 0) has nothing to commit when every member's substep count is even
+    if (n_done >= 0 || b.any_odd_substeps) launch_map(L, "acoustic_commit", k_commit<S>, b, S_ALL, W_CELL, n_done);
 }
 
 template <class S>
@@ -2533,14 +2534,14 @@ void launch_large_step(Launcher& L, const BatchView& b)
     launch_transport<S>(L, b, 4, pd, true);
     for (int n = 0; n < b.max_kessler_sub; ++n)
         launch_map(L, "kessler_apply", k_kessler_apply<S>, b, S_ALL, W_CELL, n);
-    launch_map(L, "scalars_home", k_scalars_home<S>, b, S_ALL, W_CELL);
+    if (b.any_even_kessler_sub) launch_map(L, "scalars_home", k_scalars_home<S>, b, S_ALL, W_CELL);
     for (int n = 0; n < b.max_sgs_n; ++n)
     {
         launch_map(L, "sgs_closure", k_sgs_closure<S>, b, S_ALL, W_CELL, n);
         launch_map(L, "sgs_momentum", k_sgs_momentum<S>, b, S_ALL, W_CELL, n);
         launch_map(L, "sgs_scalars", k_sgs_scalars<S>, b, S_ALL, W_CELL, n);
     }
-    if (b.max_sgs_n > 0) launch_map(L, "sgs_home", k_sgs_home<S>, b, S_ALL, W_CELL);
+    if (b.any_odd_sgs_n) launch_map(L, "sgs_home", k_sgs_home<S>, b, S_ALL, W_CELL);
     if (b.any_sponge) launch_map(L, "sponge", k_sponge<S>, b, S_ALL, W_CELL);
     launch_boundaries<S>(L, b, -1);
 }

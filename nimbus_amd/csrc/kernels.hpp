@@ -136,6 +136,10 @@ struct BatchView
     int max_kessler_sub = 0;
     int any_sponge = 0;
     int max_sgs_n = 0;
+    // parities that decide whether a copy-home kernel has work: some member
+    // with an odd substep count (acoustic commit), an even microphysics
+    // subcycle count (scalars home), an odd SGS substep count (SGS home)
+    int any_odd_substeps = 0, any_even_kessler_sub = 0, any_odd_sgs_n = 0;
     int max_nz = 0;
     int elem = 4;
     BlockMap maps[S_SUBSETS][W_KINDS];

@@ -469,6 +469,7 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
     b.view.any_sponge = 0;
     b.view.max_nz = 0;
     b.view.max_sgs_n = 0;
+    b.view.any_odd_substeps = b.view.any_even_kessler_sub = b.view.any_odd_sgs_n = 0;
     bool remap = true;
     for (Member* m : ms)
     {
@@ -478,6 +479,9 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
         b.view.any_sponge = b.view.any_sponge || m->args.p.sponge;
         b.view.max_nz = std::max(b.view.max_nz, m->args.g.nz);
         b.view.max_sgs_n = std::max(b.view.max_sgs_n, m->args.p.sgs_n);
+        b.view.any_odd_substeps |= m->args.p.substeps & 1;
+        b.view.any_even_kessler_sub |= (m->args.p.kessler_sub & 1) == 0;
+        b.view.any_odd_sgs_n |= m->args.p.sgs_n & 1;
         remap = remap && m->xcd_remap;
     }
     std::vector<int2> all;
