@@ -1630,6 +1630,64 @@ __global__ __launch_bounds__(kBlock) void k_scalars_home(const MemberArgs* __res
 // from the same operands, so the values are identical.
 // ---------------------------------------------------------------------
 // EOS projection of one cell after the SGS commit (eos_apply.cpp:50-110).
+// ---------------------------------------------------------------------
+// Rayleigh sponges after the EOS projection (rayleigh_damping.cpp:985-1080,
+// 1360-1440): upper sponge on u, v (to the base wind), w (to 0) and theta
+// (to theta0), the w-only absorber under the lid, then the lateral band;
+// each relaxation x += f (target - x) with the exact decay factor
+// f = 1 - exp(-alpha dt) precomputed per level, rounded to storage. Members
+// with SGS apply them in the pass that commits their last SGS substep;
+// others in k_sponge.
+// ---------------------------------------------------------------------
+struct SpongeAt
+{
+    double fu, fw, fwo, fl;  // upper sponge at centres and at w faces, w-only absorber, lateral band
+    double u0, v0, th0;      // targets
+};
+
+__device__ __forceinline__ SpongeAt sponge_at(const MemberArgs& A, int i, int j, int k)
+{
+    const DevGrid& g = A.g;
+    const int NZ = g.nz;
+    const double* base = A.f.base;
+    SpongeAt z;
+    z.fu = slot(base, kBaseSpongeUp, NZ)[k];
+    z.fw = slot(base, kBaseSpongeUpW, NZ)[k];
+    z.fwo = slot(base, kBaseSpongeW, NZ)[k];
+    const int dist = min(min(i, g.nx - 1 - i), min(j, g.ny - 1 - j));
+    z.fl = dist < A.p.sponge_lateral ? G(A.f.sponge_lat)[dist] : 0.0;
+    z.u0 = slot(base, kBaseU0, NZ)[k];
+    z.v0 = slot(base, kBaseV0, NZ)[k];
+    z.th0 = slot(base, kBaseTh0, NZ)[k];
+    return z;
+}
+
+template <class S>
+__device__ __forceinline__ void sponge_relax(S& x, double target, double fac)
+{
+    if (fac > 0.0) x = S(double(x) + fac * (target - double(x)));
+}
+
+// per field in the reference's order: upper sponge, w-only absorber, lateral band
+template <class S>
+__device__ __forceinline__ void sponge_wind(const SpongeAt& z, S& u, S& v, S& w)
+{
+    sponge_relax(u, z.u0, z.fu);
+    sponge_relax(v, z.v0, z.fu);
+    sponge_relax(w, 0.0, z.fw);
+    sponge_relax(w, 0.0, z.fwo);
+    sponge_relax(u, z.u0, z.fl);
+    sponge_relax(v, z.v0, z.fl);
+    sponge_relax(w, 0.0, z.fl);
+}
+
+template <class S>
+__device__ __forceinline__ void sponge_theta(const SpongeAt& z, S& th)
+{
+    sponge_relax(th, z.th0, z.fu);
+    sponge_relax(th, z.th0, z.fl);
+}
+
 template <class S>
 __device__ __forceinline__ void sgs_eos(const MemberArgs& A, int c, int k, S theta)
 {
@@ -1881,10 +1939,16 @@ __device__ __forceinline__ void sgs_momentum_cell(const MemberArgs& A, const Cel
     const Bounds& bd = A.p.bounds;
     // commit x + h dx/dt with the storage clamps (sgs_apply.cpp:610-660); the
     // tendency is rounded to storage first, as the reference's tendency field
+    // the last substep of an even count lands in the home fields: the sponges follow it here
+    const bool sponge_here = A.p.sponge && n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0;
     auto commit = [&](T du, T dv, T dw) {
-        P<S>(nxt.u)[c] = S(clamp_store<S>(double(s.u[c]) + finite0(double(S(du))) * hstep, 0.0, -bd.wind_h, bd.wind_h));
-        P<S>(nxt.v)[c] = S(clamp_store<S>(double(s.v[c]) + finite0(double(S(dv))) * hstep, 0.0, -bd.wind_h, bd.wind_h));
-        P<S>(nxt.w)[c] = S(clamp_store<S>(double(s.w[c]) + finite0(double(S(dw))) * hstep, 0.0, -bd.wind_v, bd.wind_v));
+        S un = S(clamp_store<S>(double(s.u[c]) + finite0(double(S(du))) * hstep, 0.0, -bd.wind_h, bd.wind_h));
+        S vn = S(clamp_store<S>(double(s.v[c]) + finite0(double(S(dv))) * hstep, 0.0, -bd.wind_h, bd.wind_h));
+        S wn = S(clamp_store<S>(double(s.w[c]) + finite0(double(S(dw))) * hstep, 0.0, -bd.wind_v, bd.wind_v));
+        if (sponge_here) sponge_wind<S>(sponge_at(A, i, j, k), un, vn, wn);
+        P<S>(nxt.u)[c] = un;
+        P<S>(nxt.v)[c] = vn;
+        P<S>(nxt.w)[c] = wn;
     };
     if (!(has_u || has_v || has_w))
     {
@@ -2152,11 +2216,16 @@ __device__ __forceinline__ void sgs_scalars_cell(const MemberArgs& A, const Cell
     }
     const SgsSet nxt = sgs_set(A.f, !(n & 1));
     const double hstep = n == A.p.sgs_n - 1 ? A.p.sgs_h_last : A.p.sgs_h;
-    const S thn = S(clamp_store<S>(double(s.th[c]) + finite0(double(S(dth))) * hstep, 300.0, 200.0f, 700.0f));
-    P<S>(nxt.th)[c] = thn;
+    S thn = S(clamp_store<S>(double(s.th[c]) + finite0(double(S(dth))) * hstep, 300.0, 200.0f, 700.0f));
     P<S>(nxt.qv)[c] = S(clamp_store<S>(double(s.qv[c]) + finite0(double(S(dqv))) * hstep, 0.0, 0.0f, 0.05f));
-    // an even count ends in the home fields: the EOS projection follows the last commit here
-    if (n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0) sgs_eos<S>(A, c, k, thn);
+    // an even count ends in the home fields: the EOS projection, then the
+    // sponges, follow the last commit here
+    if (n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0)
+    {
+        sgs_eos<S>(A, c, k, thn);
+        if (A.p.sponge) sponge_theta<S>(sponge_at(A, i, j, k), thn);
+    }
+    P<S>(nxt.th)[c] = thn;
 }
 
 template <class S>
@@ -2189,59 +2258,42 @@ __global__ __launch_bounds__(kBlock) void k_sgs_home(const MemberArgs* __restric
     if (!cl.valid) return;
     const DevFields& f = A.f;
     const int c = int(A.g.at(cl.i, cl.j, cl.k));
-    const S u = P<S>(f.u2)[c], v = P<S>(f.v2)[c], w = P<S>(f.w2)[c], th = P<S>(f.rp2)[c], qv = P<S>(f.mfx)[c];
+    S u = P<S>(f.u2)[c], v = P<S>(f.v2)[c], w = P<S>(f.w2)[c], th = P<S>(f.rp2)[c];
+    const S qv = P<S>(f.mfx)[c];
+    sgs_eos<S>(A, c, cl.k, th);
+    if (A.p.sponge)
+    {
+        const SpongeAt z = sponge_at(A, cl.i, cl.j, cl.k);
+        sponge_wind<S>(z, u, v, w);
+        sponge_theta<S>(z, th);
+    }
     P<S>(f.u)[c] = u;
     P<S>(f.v)[c] = v;
     P<S>(f.w)[c] = w;
     P<S>(f.th)[c] = th;
     P<S>(f.qv)[c] = qv;
-    sgs_eos<S>(A, c, cl.k, th);
 }
 
-// ---------------------------------------------------------------------
-// Rayleigh sponges after the EOS projection (rayleigh_damping.cpp:985-1080,
-// 1360-1440): upper sponge on u, v (to the base wind), w (to 0) and theta
-// (to theta0), the w-only absorber under the lid, then the lateral band;
-// each relaxation x += f (target - x) with the exact decay factor
-// f = 1 - exp(-alpha dt) precomputed per level, rounded to storage.
-// ---------------------------------------------------------------------
+// Sponges of the members without SGS (the others apply them with their last SGS substep).
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_sponge(const MemberArgs* __restrict__ ma, const int2* __restrict__ map)
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
-    if (!A.p.sponge) return;
+    if (!A.p.sponge || A.p.sgs_n > 0) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
-    const DevGrid& g = A.g;
-    const int NZ = g.nz, k = cl.k;
+    const SpongeAt z = sponge_at(A, cl.i, cl.j, cl.k);
+    if (z.fu == 0.0 && z.fw == 0.0 && z.fwo == 0.0 && z.fl == 0.0) return;
     const DevFields& f = A.f;
-    const double* base = f.base;
-    const double fu = slot(base, kBaseSpongeUp, NZ)[k], fw = slot(base, kBaseSpongeUpW, NZ)[k];
-    const double fwo = slot(base, kBaseSpongeW, NZ)[k];
-    const int dist = min(min(cl.i, g.nx - 1 - cl.i), min(cl.j, g.ny - 1 - cl.j));
-    const double fl = dist < A.p.sponge_lateral ? G(f.sponge_lat)[dist] : 0.0;
-    if (fu == 0.0 && fw == 0.0 && fwo == 0.0 && fl == 0.0) return;
-    const double u0 = slot(base, kBaseU0, NZ)[k], v0 = slot(base, kBaseV0, NZ)[k];
-    const double th0 = slot(base, kBaseTh0, NZ)[k];
-    const std::size_t c = g.at(cl.i, cl.j, k);
+    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
     const auto u = P<S>(f.u);
     const auto v = P<S>(f.v);
     const auto w = P<S>(f.w);
     const auto th = P<S>(f.th);
     S uc = u[c], vc = v[c], wc = w[c], tc = th[c];
-    auto relax = [](S& x, double target, double fac) {
-        if (fac > 0.0) x = S(double(x) + fac * (target - double(x)));
-    };
-    relax(uc, u0, fu);
-    relax(vc, v0, fu);
-    relax(wc, 0.0, fw);
-    relax(tc, th0, fu);
-    relax(wc, 0.0, fwo);
-    relax(uc, u0, fl);
-    relax(vc, v0, fl);
-    relax(wc, 0.0, fl);
-    relax(tc, th0, fl);
+    sponge_wind<S>(z, uc, vc, wc);
+    sponge_theta<S>(z, tc);
     u[c] = uc;
     v[c] = vc;
     w[c] = wc;
@@ -2542,7 +2594,7 @@ void launch_large_step(Launcher& L, const BatchView& b)
         launch_map(L, "sgs_scalars", k_sgs_scalars<S>, b, S_ALL, W_CELL, n);
     }
     if (b.any_odd_sgs_n) launch_map(L, "sgs_home", k_sgs_home<S>, b, S_ALL, W_CELL);
-    if (b.any_sponge) launch_map(L, "sponge", k_sponge<S>, b, S_ALL, W_CELL);
+    if (b.any_sponge_unfused) launch_map(L, "sponge", k_sponge<S>, b, S_ALL, W_CELL);
     launch_boundaries<S>(L, b, -1);
 }
 

@@ -140,6 +140,7 @@ struct BatchView
     // with an odd substep count (acoustic commit), an even microphysics
     // subcycle count (scalars home), an odd SGS substep count (SGS home)
     int any_odd_substeps = 0, any_even_kessler_sub = 0, any_odd_sgs_n = 0;
+    int any_sponge_unfused = 0;  // a member with sponges and no SGS pass to apply them in
     int max_nz = 0;
     int elem = 4;
     BlockMap maps[S_SUBSETS][W_KINDS];

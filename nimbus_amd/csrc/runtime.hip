@@ -470,6 +470,7 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
     b.view.max_nz = 0;
     b.view.max_sgs_n = 0;
     b.view.any_odd_substeps = b.view.any_even_kessler_sub = b.view.any_odd_sgs_n = 0;
+    b.view.any_sponge_unfused = 0;
     bool remap = true;
     for (Member* m : ms)
     {
@@ -482,6 +483,7 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
         b.view.any_odd_substeps |= m->args.p.substeps & 1;
         b.view.any_even_kessler_sub |= (m->args.p.kessler_sub & 1) == 0;
         b.view.any_odd_sgs_n |= m->args.p.sgs_n & 1;
+        b.view.any_sponge_unfused |= m->args.p.sponge && m->args.p.sgs_n == 0;
         remap = remap && m->xcd_remap;
     }
     std::vector<int2> all;
