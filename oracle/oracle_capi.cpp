@@ -366,14 +366,17 @@ int oracle_apply_sponges_f64(const oc_grid* gp, const oc_base* bp, const oc_conf
 }
 
 // Smagorinsky closure and tendencies of one state, no commit (FP64).
-int oracle_sgs_terms_f64(const oc_grid* gp, const oc_config* cp, const oc_state* st, double* nu, double* kth,
-                         double* kq, double* du, double* dv, double* dw, double* dth, double* dqv)
+}  // extern "C"
+
+template <class S>
+static int sgs_terms(const oc_grid* gp, const oc_config* cp, const oc_state* st, double* nu, double* kth,
+                     double* kq, double* du, double* dv, double* dw, double* dth, double* dqv)
 {
     const Grid g = to_grid(gp);
     const Config c = to_config(cp);
-    State<double> s;
+    State<S> s;
     load(g, st, s);
-    std::vector<double> a, b, d, e, f, h, x, y;
+    std::vector<S> a, b, d, e, f, h, x, y;
     sgs_closure(g, c, s, a, b, d);
     sgs_momentum_tendencies(g, c, s, a, e, f, h);
     sgs_scalar_tendency(g, c, s, b, s.theta, x);
@@ -384,6 +387,21 @@ int oracle_sgs_terms_f64(const oc_grid* gp, const oc_config* cp, const oc_state*
     std::copy(x.begin(), x.end(), dth); std::copy(y.begin(), y.end(), dqv);
     (void)n;
     return 0;
+}
+
+extern "C" {
+
+int oracle_sgs_terms_f64(const oc_grid* gp, const oc_config* cp, const oc_state* st, double* nu, double* kth,
+                         double* kq, double* du, double* dv, double* dw, double* dth, double* dqv)
+{
+    return sgs_terms<double>(gp, cp, st, nu, kth, kq, du, dv, dw, dth, dqv);
+}
+
+// FP32 storage (the reference's fields): coefficients and tendencies rounded to FP32.
+int oracle_sgs_terms_f32(const oc_grid* gp, const oc_config* cp, const oc_state* st, double* nu, double* kth,
+                         double* kq, double* du, double* dv, double* dw, double* dth, double* dqv)
+{
+    return sgs_terms<float>(gp, cp, st, nu, kth, kq, du, dv, dw, dth, dqv);
 }
 
 // The split SGS integration over one large step (all SGS substeps).

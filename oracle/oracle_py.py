@@ -58,7 +58,8 @@ def lib():
             getattr(L, name).argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), ctypes.c_int]
         L.oracle_resolve_substeps.argtypes = [P(OcGrid), P(OcBase), P(OcConfig)]
         L.oracle_apply_sponges_f64.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), ctypes.c_double]
-        L.oracle_sgs_terms_f64.argtypes = [P(OcGrid), P(OcConfig), P(OcState)] + [_dp] * 8
+        for name in ("oracle_sgs_terms_f64", "oracle_sgs_terms_f32"):
+            getattr(L, name).argtypes = [P(OcGrid), P(OcConfig), P(OcState)] + [_dp] * 8
         for name in ("oracle_apply_sgs_f64", "oracle_apply_sgs_f32"):
             getattr(L, name).argtypes = [P(OcGrid), P(OcConfig), P(OcState)]
         for name in ("oracle_acoustic_substep_f32", "oracle_acoustic_substep_f64"):
@@ -247,14 +248,16 @@ def apply_sponges(g, base, cfg, st, dt):
     return s
 
 
-def sgs_terms(g, cfg, st):
-    """Smagorinsky closure and tendencies of a state (FP64, no commit):
-    dict nu, k_theta, k_q, du, dv, dw, dtheta, dqv."""
-    s = copy_state(st, np.float64)
+def sgs_terms(g, cfg, st, dtype=np.float64):
+    """Smagorinsky closure and tendencies of a state (no commit): dict nu,
+    k_theta, k_q, du, dv, dw, dtheta, dqv. dtype float32 keeps the state,
+    coefficients and tendencies in FP32 storage as the reference's fields."""
+    s = copy_state(st, dtype)
     names = ("nu", "k_theta", "k_q", "du", "dv", "dw", "dtheta", "dqv")
     out = {n: np.zeros(st["u"].shape) for n in names}
-    lib().oracle_sgs_terms_f64(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)),
-                               *[out[n].ctypes.data_as(_dp) for n in names])
+    fn = lib().oracle_sgs_terms_f64 if np.dtype(dtype) == np.float64 else lib().oracle_sgs_terms_f32
+    fn(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)),
+       *[out[n].ctypes.data_as(_dp) for n in names])
     return out
 
 

@@ -16,12 +16,14 @@
 (eddy_viscosity.cpp:1253-1527, 1532-1948, 1953-2090; smagorinsky.cpp:495-580.)
 """
 import math
+import os
 
 import numpy as np
 import pytest
 
 from nimbus_amd import lib as L
 from oracle import oracle_py as O
+from tests.helpers import oracle_grid
 
 NX, NY, NZ = 14, 13, 9
 DX, DY, DZ = 500.0, 400.0, 250.0
@@ -203,3 +205,38 @@ def test_shear_stress_with_two_dimensional_viscosity():
             expect = (edge(j) - edge(j - 1)) / DY
             got = t["du"][i, j, k]
             assert abs(got - expect) < 1e-9 * abs(expect) + 1e-14, (i, j, got, expect)
+
+
+# ---------------------------------------------------------------------------
+# Against the reference's own operators: tests/golden/sgs.npz holds seeded
+# FP32 states and what the reference's eddy_viscosity.cpp computes for them
+# (tests/golden/make_sgs_golden.py): every mode, both filter widths, closed,
+# open, periodic and mixed sides, and active viscosity / diffusivity caps.
+
+_GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sgs.npz"))
+_CASES = sorted({k.split("/")[0] for k in _GOLD.files if "/" in k})
+_OUT = {"nu": "nu", "kth": "k_theta", "kq": "k_q", "du": "du", "dv": "dv", "dw": "dw", "dth": "dtheta",
+        "dqv": "dqv"}
+
+
+@pytest.mark.parametrize("case", _CASES)
+def test_sgs_terms_bit_exact_against_reference(case):
+    nx, ny, nz, dx, dy, dz = _GOLD["grid"]
+    g = L.make_grid(int(nx), int(ny), int(nz), dx, dy, dz)
+    mode, filt, cs, prt, sct, nu_max, k_max, px, py, ow, oe, os_, on = _GOLD[case + "/settings"]
+
+    def bc(periodic, open_):
+        return L.BC_PERIODIC if periodic else (L.BC_RADIATIVE if open_ else L.BC_RIGID_SLIP)
+
+    sgs = {"mode": {0: L.SGS_3D, 1: L.SGS_HORIZONTAL, 2: L.SGS_VERTICAL}[int(mode)], "filter": int(filt),
+           "cs": cs, "pr_t": prt, "sc_t": sct, "nu_max": nu_max, "k_max": k_max, "dt": 1.0}
+    cfg = O.from_step_config(L.step_config(1.0, 6, lateral=(bc(px, ow), bc(px, oe), bc(py, os_), bc(py, on)),
+                                           sgs=sgs))
+    st = {k: _GOLD[f"{case}/in/{k}"].astype(np.float64) for k in ("u", "v", "w", "rho", "theta", "qv")}
+    for k in ("p", "qc", "qr"):
+        st[k] = np.zeros_like(st["u"])
+    got = O.sgs_terms(oracle_grid(g), cfg, st, np.float32)
+    for ref_name, name in _OUT.items():
+        ref = _GOLD[f"{case}/out/{ref_name}"].astype(np.float64)
+        assert np.array_equal(got[name], ref), (case, name)
+    assert np.max(_GOLD[case + "/out/nu"]) > 0.0
