@@ -315,6 +315,29 @@ def test_large_steps_on_a_cm1_sounding_fp64(hip_ctx, mode):
     assert diag["max_w"] > 0.05  # the bubble rises
 
 
+def test_mixed_member_on_a_cm1_sounding_close_to_fp64_oracle(hip_ctx):
+    """The mixed path on the reference's medium-CAPE semicircle sounding
+    (spline interpolation to the grid) stays within the FP32 rounding bound
+    of the FP64 oracle over four HE-VI large steps with sponges and SGS."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    col = L.cm1_read(os.path.join(here, "golden", "cm1", "MEDCAPENEW_SEMICIRCLE_06"))
+    g = L.make_grid(22, 20, 36, 1000.0, 1000.0, 500.0)
+    base = L.cm1_base_state(g, col, L.INTERP_SPLINE)
+    prm = L.ic_params(g, bubbles=[(10000.0, 9000.0, 1500.0, 5000.0, 1500.0, 3.0)], noise_amp=0.1, seed=9)
+    st64 = L.initial_state_from_base(g, base, prm, np.float64)
+    cfg = L.step_config(3.0, 6, acoustic_mode=L.ACOUSTIC_HEVI, lateral=L.BC_PERIODIC,
+                        sponge=dict(SMALL_SPONGE, z_d=12000.0, lateral_cells=0), sgs=L.sgs_preset())
+    n_steps = 4
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st64, n_steps, np.float64)
+    out, _ = _run_member(hip_ctx, g, base, cfg, L.initial_state_from_base(g, base, prm, np.float32),
+                         L.PRECISION_MIXED, n_steps)
+    eps32 = 2.0 ** -23
+    for f, scale in (("u", 30.0), ("v", 30.0), ("w", 30.0), ("rho", 1.0), ("p", 1e5), ("theta", 300.0),
+                     ("qv", 0.02)):
+        d = max_abs(out[f].astype(np.float64), ref[f])
+        assert d <= scale * eps32 * 1e3, (f, d)
+
+
 @pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
 def test_large_step_with_smagorinsky_radiative_fp64(hip_ctx, mode):
     g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, wind_noise=2.0, dtype=np.float64)

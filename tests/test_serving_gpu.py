@@ -55,6 +55,25 @@ def test_preempted_members_resume_bitwise(hip_ctx):
 
 
 @pytest.mark.gpu
+def test_served_results_delivered_as_npy(hip_ctx, tmp_path):
+    """With results_dir set, every finished request's fields arrive as the
+    reference's NPY field files, equal to the fields downloaded at completion."""
+    from nimbus_amd import workloads as W
+    reqs = [SV.Request(q, 0.0, W.Workload(f"n{q}", 48, 48, 40, 1000.0, 500.0, 3.0, 6, 3 + q, seed=6, member=q))
+            for q in range(3)]
+    srv = SV.Server(hip_ctx, quantum=2, host_workers=2, results_dir=str(tmp_path))
+    rep = srv.run(reqs, keep_fields=[r.rid for r in reqs])
+    assert len(rep.results) == 3
+    for res in rep.results:
+        d = tmp_path / f"req_{res.rid}"
+        for f in L.FIELDS:
+            a = np.load(str(d / f"{f}.npy"))
+            assert a.dtype == np.float32 and np.array_equal(a, res.fields[f].astype(np.float32)), (res.rid, f)
+        r = np.load(str(d / "accumulated_rainfall.npy"))
+        assert np.array_equal(r, res.fields["rain_mm"].astype(np.float32))
+
+
+@pytest.mark.gpu
 def test_uneven_ensemble_batch_matches_members_alone(hip_ctx):
     """Members of different grids, dx and hence dt share one batch for a few steps."""
     from nimbus_amd import workloads as W
