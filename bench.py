@@ -305,6 +305,19 @@ def main():
     def per_step(tab):
         return {k: round(v[1] / 2, 4) for k, v in sorted(tab.items(), key=lambda kv: -kv[1][1])}
 
+    def per_kernel_roofline(tab, mode):
+        """Every kernel of a step with a counted byte figure: algorithmic GB/s
+        of its average launch and the fraction of the HBM peak."""
+        out = {}
+        for name, (launches, ms) in sorted(tab.items(), key=lambda kv: -kv[1][1]):
+            per_cell = algo_bytes_per_cell(name, mode, 4)
+            if per_cell is None or launches == 0 or ms <= 0:
+                continue
+            gbs = per_cell * wl.cells / (ms / launches / 1e3) / 1e9
+            out[name] = {"bytes_per_cell": per_cell, "avg_launch_ms": round(ms / launches, 4),
+                         "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        return out
+
     result = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
@@ -331,6 +344,8 @@ def main():
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": round(value * (68 + 17 * n_sub) * 4 / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline_mixed_explicit": roofline_for(tables["mixed_explicit"], wl.cells, 4, args.workload, "explicit"),
+        "kernels_roofline": {"mixed_hevi": per_kernel_roofline(tables["mixed_hevi"], "hevi"),
+                             "mixed_explicit": per_kernel_roofline(tables["mixed_explicit"], "explicit")},
         "cpu_baseline": None,
         "host": {"platform": platform.platform(), **cpu_info()},
     }
