@@ -3,6 +3,7 @@
 #include "nimbus_oracle.hpp"
 
 #include <cstring>
+#include <omp.h>
 
 extern "C" {
 
@@ -461,5 +462,9 @@ void oracle_hevi_column_system(int nz, double dt, double dz, double alpha, const
 {
     hevi_column_system(nz, dt, dz, alpha, rho_bar, C, G, e, w_old, A, D, U, R);
 }
+
+// Thread count of the OpenMP loops for the rest of the process (n >= 1).
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_max_threads() { return omp_get_max_threads(); }
 
 }  // extern "C"

@@ -97,8 +97,17 @@ def lib():
         L.oracle_thomas.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oracle_hevi_column_system.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_max_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops from now on; returns the previous count."""
+    prev = lib().oracle_max_threads()
+    lib().oracle_set_threads(int(n))
+    return prev
 
 
 def grid(nx, ny, nz, dx, dy, dz):

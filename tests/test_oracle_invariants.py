@@ -317,3 +317,19 @@ def test_substep_promotion_follows_reference_rule():
                                                    ctypes.byref(O.base_struct(base)),
                                                    ctypes.byref(O.from_step_config(cfg)))
         assert n_oracle == expect == L.resolve_substeps(g, base, cfg)
+
+
+def test_oracle_result_does_not_depend_on_thread_count():
+    # every OpenMP loop of the oracle writes disjoint cells from inputs no
+    # iteration writes, so one thread and many give the same bits
+    g, base, st, cfg = _periodic_case(np.float32)
+    og = O.grid(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz)
+    c = O.from_step_config(cfg)
+    many, _ = O.large_steps(og, base, c, st, 2, np.float32)
+    prev = O.set_threads(1)
+    try:
+        one, _ = O.large_steps(og, base, c, st, 2, np.float32)
+    finally:
+        O.set_threads(prev)
+    for k in many:
+        assert np.array_equal(many[k], one[k]), k
