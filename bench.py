@@ -53,6 +53,8 @@ ALGO_PER_CELL = {
     # HE-VI column solve: read rho* p* p' w rho' theta mfz; write w rho' p' mfz;
     # the FP64 Thomas coefficients c', d' are written and read back (32 B)
     "acoustic_hevi_column": (11, 32),
+    # the partitioned column solve of FP64 members: the same 11 fields, no coefficient arrays
+    "acoustic_hevi_column_seg": (11, 0),
     # read mfx mfy mfz and four scalars; write 3 face fluxes per scalar
     "transport_flux": (19, 0),
     # read rho_b, and fluxes and value of the three moisture scalars; write 3 limiter ratios

@@ -60,6 +60,17 @@ __device__ __forceinline__ gptr<S> P(void* p)
 }
 
 __device__ __forceinline__ double finite0(double x) { return isfinite(x) ? x : 0.0; }
+
+// 1/x in FP64: the hardware reciprocal estimate refined by two Newton-Raphson
+// steps r <- r + r (1 - x r), each of which squares the relative error, so the
+// result is within about an ulp of the quotient for finite nonzero x of
+// normal range (zero and non-finite x give a non-finite result).
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
 __device__ __forceinline__ float finite0(float x) { return isfinite(x) ? x : 0.0f; }
 __device__ __forceinline__ double safe_rho(double r) { return isfinite(r) && r > 1.0e-6 ? r : 1.0; }
 __device__ __forceinline__ float safe_rho(float r) { return isfinite(r) && r > 1.0e-6f ? r : 1.0f; }
@@ -755,8 +766,10 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
-    if (s >= A.p.substeps) return;  // uniform over the block (one member per block)
     const DevGrid& g = A.g;
+    // uniform over the block (one member per block); FP64 members of at most
+    // kSegMaxNz levels take the partitioned form
+    if (s >= A.p.substeps || hevi_partitioned(int(sizeof(S)), g.nz)) return;
     const Column t = column_of(A, wk.local);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     extern __shared__ double s_prof[];
@@ -903,6 +916,331 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
             dring[q] = load_down(max(r - kRing, 0));
         }
     }
+}
+
+// Part 2, partitioned form (FP64 members of at most kSegMaxNz levels): the
+// same rows as the sequential sweep, in FP64, solved without coefficient
+// arrays in memory. Each column's faces 0 .. nz-2 are split into runs of ls
+// consecutive faces, one thread per run (kSegCols consecutive columns of one
+// row x kSegments runs per workgroup, so every load is 16 contiguous lanes
+// of one level). A thread
+//  1. loads its run's levels once (one level below and two above for the
+//     face densities);
+//  2. forward-eliminates its rows to x_r + c1_r x_{r+1} + a1_r L' = d1_r,
+//     where L' is the last unknown of the run below (a1 carries that
+//     coupling; the first run has none, and its c1, d1 are the sequential
+//     sweep's c', d'), and back-eliminates its first row to
+//     x_{s0} = dt0 - at0 L' - ct0 L, with L its own last unknown;
+//  3. forms its row of the system of the runs' last unknowns (row p:
+//     a1 L_{p-1} + (1 - c1 at') L_p - c1 ct' L_{p+1} = d1 - c1 dt', primes for
+//     run p + 1's first row), solved by cyclic reduction over the column's
+//     run threads in LDS;
+//  4. substitutes back down its run and finalises the cells whose lower face
+//     is in it (the face above the run is run p + 1's first unknown,
+//     evaluated from the same LDS values by the same expression).
+// Quotients use rcp_nr (about an ulp) rather than correctly rounded
+// division. Every pivot of the runs and of the reduction must be positive
+// and finite, otherwise the whole column keeps its substep-start w, as in the
+// sequential sweep. A run's neighbours overwrite its halo levels only after
+// the barriers, by which time every halo value has been consumed.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_hevi_column_seg(const MemberArgs* __restrict__ ma,
+                                                            const int2* __restrict__ map, int s, int accumulate,
+                                                            int init)
+{
+    constexpr int NQ = kSegRows + 3;  // levels s0 - 1 .. s0 + ls + 1
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    const DevGrid& g = A.g;
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    // uniform over the block (one member per block): no barrier is skipped by part of it
+    if (s >= A.p.substeps || !hevi_partitioned(int(sizeof(S)), NZ)) return;
+    extern __shared__ double s_prof[];
+    const LProf pr = stage_profiles(s_prof, A.f.base, NZ);
+    double* const ex = s_prof + kLdsProfiles * NZ;         // [11][kBlock]: run ends and reduced rows
+    double* const lv = ex + 11 * kBlock;                   // [kBlock]: last unknown of each run
+    int* const okv = reinterpret_cast<int*>(lv + kBlock);  // [2][kBlock]: run and reduction verdicts
+
+    const int tid = int(threadIdx.x);
+    const int col = tid & (kSegCols - 1), p = tid / kSegCols;
+    const int row_blocks = (NX + kSegCols - 1) / kSegCols;
+    const int j = wk.local / row_blocks;
+    const int i = (wk.local - j * row_blocks) * kSegCols + col;
+    const bool interior = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1;
+    const int n = NZ - 1;  // unknown faces 0 .. NZ-2
+    const int ls = (n + kSegments - 1) / kSegments;
+    const int runs = (n + ls - 1) / ls;
+    const int s0 = p * ls;
+    const int m = interior && p < runs ? min(ls, n - s0) : 0;
+
+    const DevFields& f = A.f;
+    const int par = s & 1;
+    const gptr<S> __restrict__ w = P<S>(f.w);
+    const gptr<const S> __restrict__ rp_in = P<S>(par ? f.rp2 : f.rp);
+    const gptr<S> __restrict__ rp_out = P<S>(par ? f.rp : f.rp2);
+    const gptr<S> __restrict__ pp = P<S>(f.pp);
+    const gptr<const S> __restrict__ psb = P<S>(f.u2);
+    const gptr<const S> __restrict__ th = P<S>(f.th);
+    const gptr<S> __restrict__ mfz = P<S>(f.mfz);
+    const Bounds bd = A.p.bounds;
+    const double dts = A.p.dts, alpha = A.p.hevi_alpha, weight = A.p.mf_weight;
+    const bool acc = accumulate != 0, first = init != 0 && s == 0;
+    const double idz = g.idz;
+    const double a1 = 1.0 - alpha;
+    const std::size_t c0 = m > 0 ? g.at(i, j, 0) : 0, sz = g.plane();
+
+    // 1. level q is level s0 - 1 + q (clamped to the column for the load)
+    S rp[NQ], wo_s[NQ], rs[NQ], ps[NQ], pn[NQ], tq[NQ], mq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+    {
+        rp[q] = wo_s[q] = rs[q] = ps[q] = pn[q] = tq[q] = mq[q] = S(0);
+        if (q <= m + 2 && m > 0)
+        {
+            const std::size_t c = c0 + std::size_t(clampi(s0 - 1 + q, 0, NZ - 1)) * sz;
+            rp[q] = rp_in[c];
+            wo_s[q] = w[c];
+            if (q >= 1 && q <= m + 1)
+            {
+                rs[q] = rp_out[c];
+                ps[q] = psb[c];
+                pn[q] = pp[c];
+            }
+        }
+    }
+    auto lev = [&](int q) { return clampi(s0 - 1 + q, 0, NZ - 1); };
+    // old w of face s0 - 1 + q (none below the ground face)
+    auto wo = [&](int q) { return s0 - 1 + q >= 0 ? double(wo_s[q]) : 0.0; };
+    // full density of cell s0 - 1 + q
+    auto rc_at = [&](int q) { return pr.rho0(lev(q)) + double(rp[q]); };
+    // density of face s0 - 1 + q between cells q and q + 1 (zero below the ground face and at the lid)
+    auto rb_of = [&](int q, double rc_lo, double rc_hi) {
+        const int fq = s0 - 1 + q;
+        return fq >= 0 && fq < NZ - 1 ? 0.5 * (rc_lo + rc_hi) : 0.0;
+    };
+    // cell s0 - 1 + q: C = gamma p* / rho* and the full p*
+    auto cell_c = [&](int q, double& Cq, double& psq) {
+        const int k = lev(q);
+        const double rstar = pr.rho0(k) + double(rs[q]);
+        psq = pr.p0(k) + double(ps[q]);
+        Cq = kGamma * psq * rcp_nr(rstar);
+    };
+
+    // 2. forward elimination of rows r = s0 + t (cells r, r + 1; faces r - 1, r, r + 1),
+    // C, the old-w divergence e and the face densities rolled along as in the sequential sweep
+    double c1[kSegRows], a1s[kSegRows], d1[kSegRows];
+    bool ok = true;
+    double rb_lo = 0.0, rb_mid = 0.0, rc_nx = 0.0, C_lo = 0.0, ps_lo = 0.0, e_lo = 0.0;
+    if (m > 0)
+    {
+        const double r0 = rc_at(0), r1 = rc_at(1);
+        rc_nx = rc_at(2);
+        rb_lo = rb_of(0, r0, r1);
+        rb_mid = rb_of(1, r1, rc_nx);
+        cell_c(1, C_lo, ps_lo);
+        e_lo = -a1 * dts * idz * (rb_mid * wo(1) - rb_lo * wo(0));
+    }
+    const double Bc = alpha * dts * idz;
+    double cp = 0.0, ap = 0.0, dp = 0.0;  // the last eliminated row
+#pragma unroll
+    for (int t = 0; t < kSegRows; ++t)
+    {
+        if (t < m)
+        {
+            const int r = s0 + t;
+            const double rc_hi = rc_at(t + 3);
+            const double rb_hi = rb_of(t + 2, rc_nx, rc_hi);
+            rc_nx = rc_hi;
+            double C_hi, ps_hi;
+            cell_c(t + 2, C_hi, ps_hi);
+            const double e_hi = -a1 * dts * idz * (rb_hi * wo(t + 2) - rb_mid * wo(t + 1));
+            const double rbr = rb_mid;
+            const double inv_rbr = rcp_nr(rbr);
+            const double Ak = Bc * inv_rbr;
+            const double Aa = r > 0 ? -Ak * Bc * C_lo * rb_lo : 0.0;
+            const double U = r < NZ - 2 ? -Ak * Bc * C_hi * rb_hi : 0.0;
+            const double D = 1.0 + Ak * Bc * (C_hi + C_lo) * rbr;
+            const double G_ = (alpha * ((ps_hi - ps_lo) - (pr.p0(r + 1) - pr.p0(r))) +
+                               a1 * (double(pn[t + 2]) - double(pn[t + 1]))) *
+                              idz;
+            const double R = wo(t + 1) - dts * inv_rbr * G_ - Ak * (C_hi * e_hi - C_lo * e_lo);
+            const double piv = D - Aa * cp;
+            if (!(piv > 0.0) || !isfinite(piv)) ok = false;
+            const double inv_piv = rcp_nr(piv);
+            c1[t] = U * inv_piv;
+            a1s[t] = (t == 0 ? Aa : -Aa * ap) * inv_piv;
+            d1[t] = (R - Aa * dp) * inv_piv;
+            cp = c1[t];
+            ap = a1s[t];
+            dp = d1[t];
+            rb_lo = rb_mid;
+            rb_mid = rb_hi;
+            C_lo = C_hi;
+            ps_lo = ps_hi;
+            e_lo = e_hi;
+        }
+        else
+        {
+            c1[t] = a1s[t] = d1[t] = 0.0;
+        }
+    }
+    // the first row back-eliminated: x_{s0} = dt0 - at0 L' - ct0 L (the last row is L itself)
+    double dt0 = 0.0, at0 = 0.0, ct0 = -1.0;
+#pragma unroll
+    for (int t = kSegRows - 2; t >= 0; --t)
+    {
+        if (t < m - 1)
+        {
+            dt0 = d1[t] - c1[t] * dt0;
+            at0 = a1s[t] - c1[t] * at0;
+            ct0 = -c1[t] * ct0;
+        }
+    }
+
+    // 3. the system of the runs' last unknowns L_p, one row per run thread:
+    // a1 L_{p-1} + (1 - c1 at') L_p - c1 ct' L_{p+1} = d1 - c1 dt' (primes:
+    // run p + 1's first row), solved by cyclic reduction over the 16 run
+    // threads of the column in LDS (4 steps of stride 1, 2, 4, 8, double
+    // buffered so one barrier per step suffices). Runs past the column and
+    // non-interior columns carry identity rows.
+    double* const nb = ex;                 // [3][kBlock]: at0, ct0, dt0 of each run
+    double* const rbuf = ex + 3 * kBlock;  // [2][4][kBlock]: reduced rows (sub, dia, sup, rhs)
+    nb[0 * kBlock + tid] = at0;
+    nb[1 * kBlock + tid] = ct0;
+    nb[2 * kBlock + tid] = dt0;
+    okv[tid] = ok ? 1 : 0;
+    __syncthreads();
+    double sub = 0.0, dia = 1.0, sup = 0.0, rhs = 0.0;
+    if (m > 0)
+    {
+        const int un = tid + kSegCols;
+        const bool more = p + 1 < runs;
+        const double aN = more ? nb[0 * kBlock + un] : 0.0;
+        const double cN = more ? nb[1 * kBlock + un] : 0.0;
+        const double dN = more ? nb[2 * kBlock + un] : 0.0;
+        sub = ap;
+        dia = 1.0 - cp * aN;
+        sup = -cp * cN;
+        rhs = dp - cp * dN;
+    }
+    bool col_ok = true;
+    for (int q = 0; q < runs; ++q) col_ok = col_ok && okv[q * kSegCols + col] != 0;
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+    {
+        const int stride = 1 << st;
+        double* const b = rbuf + (st & 1) * 4 * kBlock;
+        b[0 * kBlock + tid] = sub;
+        b[1 * kBlock + tid] = dia;
+        b[2 * kBlock + tid] = sup;
+        b[3 * kBlock + tid] = rhs;
+        __syncthreads();
+        double k1 = 0.0, k2 = 0.0, nsub = 0.0, nsup = 0.0, ndia = dia, nrhs = rhs;
+        if (p - stride >= 0)
+        {
+            const int u = tid - stride * kSegCols;
+            k1 = sub * rcp_nr(b[1 * kBlock + u]);
+            nsub = -k1 * b[0 * kBlock + u];
+            ndia -= k1 * b[2 * kBlock + u];
+            nrhs -= k1 * b[3 * kBlock + u];
+        }
+        if (p + stride < kSegments)
+        {
+            const int u = tid + stride * kSegCols;
+            k2 = sup * rcp_nr(b[1 * kBlock + u]);
+            nsup = -k2 * b[2 * kBlock + u];
+            ndia -= k2 * b[0 * kBlock + u];
+            nrhs -= k2 * b[3 * kBlock + u];
+        }
+        sub = nsub;
+        sup = nsup;
+        dia = ndia;
+        rhs = nrhs;
+        if (m > 0 && (!(dia > 0.0) || !isfinite(dia))) col_ok = false;
+    }
+    const double Lp = rhs * rcp_nr(dia);
+    lv[tid] = Lp;
+    okv[kBlock + tid] = col_ok ? 1 : 0;
+    __syncthreads();
+    if (m <= 0) return;  // no barrier after this point
+    for (int q = 0; q < runs; ++q) col_ok = col_ok && okv[kBlock + q * kSegCols + col] != 0;
+
+    // 4. back substitution from the top of the run down; each cell s0 + t + 1 is
+    // finalised once both of its faces are known (the face above the run is
+    // run p + 1's first unknown, the same expression and operands as there)
+    // theta and the mass flux of the cells this run finalises (no other run
+    // writes them) are loaded only now to keep them out of the elimination's
+    // registers; rho' and rho* of levels s0 .. s0 + m + 1 are read again here rather than
+    // kept: rho' is not written by this kernel, and rho* of a cell is
+    // overwritten only by the run that finalises it, after this read
+#pragma unroll
+    for (int q = 1; q < NQ; ++q)
+    {
+        if (q <= m + 2)
+        {
+            const std::size_t c = c0 + std::size_t(lev(q)) * sz;
+            rp[q] = rp_in[c];
+            if (q >= (p == 0 ? 1 : 2) && q <= m + 1)
+            {
+                rs[q] = rp_out[c];
+                tq[q] = th[c];
+                if (acc && !first) mq[q] = mfz[c];
+            }
+        }
+    }
+    const double Lm = p > 0 ? lv[tid - kSegCols] : 0.0;
+    // old w, density and stored new w of the face above the run (face s0 + m)
+    double wo_e = 0.0, rc_e = 0.0, rc_e1 = 0.0;
+#pragma unroll
+    for (int q = 1; q < NQ - 1; ++q)
+        if (q == m + 1)
+        {
+            wo_e = wo(q);
+            rc_e = rc_at(q);
+            rc_e1 = rc_at(q + 1);
+        }
+    double rb_up = rb_of(m + 1, rc_e, rc_e1);
+    double rc_up = rc_e;  // density of the cell above the current face
+    double w_up = 0.0;
+    if (s0 + m < n)
+    {
+        const int u = tid + kSegCols;
+        const double x = col_ok ? nb[2 * kBlock + u] - nb[0 * kBlock + u] * Lp - nb[1 * kBlock + u] * lv[u] : wo_e;
+        w_up = double(S(clamp_store<S>(x, 0.0, -bd.wind_v, bd.wind_v)));
+    }
+    auto finalise = [&](int q, double rb_top, double w_top, double rb_bot, double w_bot) {
+        // cell s0 - 1 + q: upper face q, lower face q - 1
+        const int kc = s0 - 1 + q;
+        const std::size_t cc = c0 + std::size_t(kc) * sz;
+        const double ft = kc < NZ - 1 ? rb_top * (alpha * w_top + a1 * wo(q)) : 0.0;
+        const double fb = kc >= 1 ? rb_bot * (alpha * w_bot + a1 * wo(q - 1)) : 0.0;
+        if (acc) mfz[cc] = S((first ? 0.0 : double(mq[q])) + weight * ft);
+        const double rstar = pr.rho0(kc) + double(rs[q]);
+        const RhoP<S> o = density_eos<S>(bd, pr, kc, rstar - dts * idz * (ft - fb), double(tq[q]));
+        rp_out[cc] = o.r;
+        pp[cc] = o.p;
+    };
+    double x_up = Lp;  // unclamped solution of the face above
+#pragma unroll
+    for (int t = kSegRows - 1; t >= 0; --t)
+    {
+        if (t < m)
+        {
+            double x = t == m - 1 ? Lp : t == 0 ? dt0 - at0 * Lm - ct0 * Lp : d1[t] - c1[t] * x_up - a1s[t] * Lm;
+            x_up = x;
+            if (!col_ok) x = wo(t + 1);
+            const double c = clamp_store<S>(x, 0.0, -bd.wind_v, bd.wind_v);
+            w[c0 + std::size_t(s0 + t) * sz] = S(c);
+            const double wn = double(S(c));
+            const double rc_lo = rc_at(t + 1);
+            const double rb = rb_of(t + 1, rc_lo, rc_up);
+            finalise(t + 2, rb_up, w_up, rb, wn);
+            w_up = wn;
+            rb_up = rb;
+            rc_up = rc_lo;
+        }
+    }
+    if (p == 0) finalise(1, rb_up, w_up, 0.0, 0.0);  // cell 0 above the ground face
 }
 
 // Backward momentum update of substep s with the new rho' (buffer
@@ -2433,6 +2771,7 @@ int work_items(const MemberArgs& a, WorkKind kind)
     case W_SN: return lin(std::size_t(a.g.nx) * a.g.nz);
     case W_SEED: return lin(std::size_t(std::max(a.g.nx, a.g.ny)) * a.g.nz);
     case W_DIAG: return kDiagBlocksPerMember;
+    case W_SEGCOL: return ((a.g.nx + kSegCols - 1) / kSegCols) * a.g.ny;
     default: return 0;
     }
 }
@@ -2497,6 +2836,13 @@ void launch_map(Launcher& L, const char* name, K kernel, const BatchView& b, int
 // LDS bytes of the staged base-state profiles for the deepest member of a batch.
 std::size_t profile_lds_bytes(const BatchView& b) { return std::size_t(kLdsProfiles) * b.max_nz * sizeof(double); }
 
+// LDS of the partitioned column solve: the profiles, 12 doubles per thread
+// (run ends, two buffers of reduced rows, last unknowns) and 2 pivot verdicts.
+std::size_t segment_lds_bytes(const BatchView& b)
+{
+    return profile_lds_bytes(b) + std::size_t(12) * kBlock * sizeof(double) + std::size_t(2) * kBlock * sizeof(int);
+}
+
 template <class S>
 S* PS(void* p)
 {
@@ -2525,8 +2871,13 @@ void launch_acoustic_substep(Launcher& L, const BatchView& b, int s, bool accumu
                int(init_at_first));
     launch_map(L, "acoustic_hevi_explicit", k_hevi_explicit<S>, b, S_HEVI, W_CELL, s, int(accumulate),
                int(init_at_first));
-    launch_map_lds(L, "acoustic_hevi_column", k_hevi_column<S>, b, S_HEVI, W_COLUMN, profile_lds_bytes(b), s,
-                   int(accumulate), int(init_at_first));
+    if constexpr (sizeof(S) == sizeof(double))
+        if (b.any_hevi_seg)
+            launch_map_lds(L, "acoustic_hevi_column_seg", k_hevi_column_seg<S>, b, S_HEVI, W_SEGCOL,
+                           segment_lds_bytes(b), s, int(accumulate), int(init_at_first));
+    if (b.any_hevi_serial)
+        launch_map_lds(L, "acoustic_hevi_column", k_hevi_column<S>, b, S_HEVI, W_COLUMN, profile_lds_bytes(b), s,
+                       int(accumulate), int(init_at_first));
     launch_map(L, "acoustic_momentum", k_momentum<S>, b, S_ALL, W_CELL, s);
 }
 

@@ -110,8 +110,23 @@ enum WorkKind
     W_SN = 3,      // 256 (i, k) rows of the south/north faces
     W_SEED = 4,    // 256 entries of the radiative stashes
     W_DIAG = 5,    // one of kDiagBlocksPerMember reduction blocks
-    W_KINDS = 6
+    W_SEGCOL = 6,  // kSegCols consecutive columns of one row, kSegments runs of faces each
+    W_KINDS = 7
 };
+
+// Partitioned HE-VI column solve: a column of nz - 1 unknown faces is split
+// into kSegments runs of at most kSegRows faces, one thread per run. FP64
+// members with nz <= kSegMaxNz use it (measured faster than the sequential
+// sweep there, which is memory-bound on its FP64 coefficient arrays); FP32
+// members and deeper columns take the sequential sweep (for FP32 storage the
+// partitioned form is the slower one: its FP64 elimination is latency-bound
+// at the occupancy its registers allow), which keeps its FP64 coefficients in
+// two cell-sized scratch arrays.
+constexpr int kSegments = 16;
+constexpr int kSegCols = 16;
+constexpr int kSegRows = 8;
+constexpr int kSegMaxNz = kSegments * kSegRows + 1;
+constexpr bool hevi_partitioned(int elem, int nz) { return elem == 8 && nz <= kSegMaxNz; }
 
 // Member subsets a launch can be restricted to.
 enum MemberSubset
@@ -141,6 +156,8 @@ struct BatchView
     // subcycle count (scalars home), an odd SGS substep count (SGS home)
     int any_odd_substeps = 0, any_even_kessler_sub = 0, any_odd_sgs_n = 0;
     int any_sponge_unfused = 0;  // a member with sponges and no SGS pass to apply them in
+    int any_hevi_seg = 0;        // HE-VI members solved in the partitioned form
+    int any_hevi_serial = 0;     // HE-VI members solved by the sequential sweep
     int max_nz = 0;
     int elem = 4;
     BlockMap maps[S_SUBSETS][W_KINDS];

@@ -81,7 +81,8 @@ Layout layout_for(const DevGrid& g, int elem, bool hevi, int lateral = 0)
     Layout L{};
     L.field = align256(g.cells() * std::size_t(elem));
     L.rl = align256(g.cells() * sizeof(float));
-    L.tri = hevi ? align256(g.cells() * sizeof(double)) : 0;
+    // FP64 Thomas coefficients: only the sequential sweep keeps them in memory
+    L.tri = hevi && !hevi_partitioned(elem, g.nz) ? align256(g.cells() * sizeof(double)) : 0;
     L.rad_w = align256(std::size_t(g.ny) * g.nz * sizeof(float));
     L.rad_s = align256(std::size_t(g.nx) * g.nz * sizeof(float));
     L.rain = align256(std::size_t(g.nx) * g.ny * sizeof(double));
@@ -369,7 +370,7 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     f.rl_qr = reinterpret_cast<float*>(p); p += L.rl;
     f.tri_c = nullptr;
     f.tri_r = nullptr;
-    if (hevi)
+    if (L.tri)
     {
         f.tri_c = reinterpret_cast<double*>(p); p += L.tri;
         f.tri_r = reinterpret_cast<double*>(p); p += L.tri;
@@ -484,6 +485,12 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
         b.view.any_even_kessler_sub |= (m->args.p.kessler_sub & 1) == 0;
         b.view.any_odd_sgs_n |= m->args.p.sgs_n & 1;
         b.view.any_sponge_unfused |= m->args.p.sponge && m->args.p.sgs_n == 0;
+        if (m->args.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI)
+        {
+            const bool part = hevi_partitioned(m->elem, m->args.g.nz);
+            b.view.any_hevi_seg |= part;
+            b.view.any_hevi_serial |= !part;
+        }
         remap = remap && m->xcd_remap;
     }
     std::vector<int2> all;

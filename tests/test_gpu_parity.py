@@ -254,6 +254,20 @@ def test_large_step_fp64_matches_fp64_oracle(hip_ctx, mode, lateral):
     assert max_abs(out["rain_mm"], ref["rain_mm"]) < 1e-9 * max(1.0, float(np.max(ref["rain_mm"])))
 
 
+@pytest.mark.parametrize("nz", [4, 14, 17, 33, 100, 129, 130, 140])
+def test_hevi_column_depths_fp64_match_fp64_oracle(hip_ctx, nz):
+    """The vertically implicit solve of FP64 members at column depths that
+    cover both of its forms: partitioned into runs of faces (nz <= 129:
+    one-face runs, uneven last runs, 16 runs of 8 faces) and the sequential
+    sweep (nz > 129); a 20 km deep domain with wind noise."""
+    g, base, st = small_case(nx=12, ny=10, nz=nz, dz=20000.0 / nz, cloudy=True, wind_noise=1.0, dtype=np.float64)
+    cfg = L.step_config(3.0, 6, acoustic_mode=L.ACOUSTIC_HEVI, lateral=L.BC_RIGID_SLIP)
+    n_steps = 2
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, n_steps, np.float64)
+    out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, n_steps)
+    check_fp64(out, ref, f"hevi nz {nz}", False)
+
+
 # Sponge settings scaled to the 7 km deep test domain: upper sponge from 4 km,
 # 4-cell w absorber, 3-cell lateral band.
 SMALL_SPONGE = {"z_d": 4000.0, "alpha": 0.05, "w_cells": 4, "w_alpha": 0.2, "lateral_cells": 3,
