@@ -67,36 +67,56 @@ def compare(wl, mixed, fp64, d_mixed, d_fp64):
     return out
 
 
-def one_hour(ctx, wl, mode=L.ACOUSTIC_HEVI, steps=None, every_seconds=600.0):
+def one_hour(ctx, wl, mode=L.ACOUSTIC_HEVI, steps=None, every_seconds=600.0, control=True):
     """Mixed and FP64 members advanced side by side from the same state; the
     run-level errors at every `every_seconds` of simulated time, and the full
-    per-field comparison at the end."""
+    per-field comparison at the end.
+
+    control: also advance an FP64 member from the FP32-rounded initial state
+    (the state the mixed member starts from) and report its distance to the
+    FP64 run beside the mixed one. It separates what FP32 arithmetic adds
+    from how far the flow itself carries a perturbation the size of one FP32
+    rounding of the initial fields."""
     steps = int(steps or round(3600.0 / wl.dt))
     chunk = max(1, int(round(every_seconds / wl.dt)))
-    ms = {p: ctx.member(wl.grid, wl.config(acoustic_mode=mode), wl.base(), p)
-          for p in (L.PRECISION_MIXED, L.PRECISION_FP64)}
+    cfg = wl.config(acoustic_mode=mode)
+    runs = [("mixed", L.PRECISION_MIXED, np.float32), ("fp64", L.PRECISION_FP64, np.float64)]
+    if control:
+        runs.append(("control", L.PRECISION_FP64, np.float32))
+    ms = {}
     series = []
-    wall = {p: 0.0 for p in ms}
+    wall = {name: 0.0 for name, _, _ in runs}
     try:
-        ms[L.PRECISION_MIXED].upload(wl.initial_state(np.float32))
-        ms[L.PRECISION_FP64].upload(wl.initial_state(np.float64))
+        for name, prec, ic_dtype in runs:
+            ms[name] = ctx.member(wl.grid, cfg, wl.base(), prec)
+            ms[name].upload(wl.initial_state(ic_dtype))
         ctx.set_graphs(True)
         done = 0
         while done < steps:
             n = min(chunk, steps - done)
-            for p, m in ms.items():
+            for name, m in ms.items():
                 t0 = time.perf_counter()
                 ctx.step([m], n)
-                wall[p] += time.perf_counter() - t0
+                wall[name] += time.perf_counter() - t0
             done += n
-            dm, dd = ms[L.PRECISION_MIXED].diag(), ms[L.PRECISION_FP64].diag()
-            series.append({"simulated_seconds": done * wl.dt, "max_w_mixed": dm["max_w"], "max_w_fp64": dd["max_w"],
-                           "max_w_error": abs(dm["max_w"] - dd["max_w"]),
-                           "rms_theta_p_error": abs(dm["rms_theta_p"] - dd["rms_theta_p"]),
-                           "targets_met": bool(abs(dm["max_w"] - dd["max_w"]) < 0.5 and
-                                               abs(dm["rms_theta_p"] - dd["rms_theta_p"]) < 0.05)})
-        res = compare(wl, ms[L.PRECISION_MIXED].download(), ms[L.PRECISION_FP64].download(),
-                      ms[L.PRECISION_MIXED].diag(), ms[L.PRECISION_FP64].diag())
+            dm, dd = ms["mixed"].diag(), ms["fp64"].diag()
+            e = {"simulated_seconds": done * wl.dt, "max_w_mixed": dm["max_w"], "max_w_fp64": dd["max_w"],
+                 "max_w_error": abs(dm["max_w"] - dd["max_w"]),
+                 "rms_theta_p_error": abs(dm["rms_theta_p"] - dd["rms_theta_p"]),
+                 "targets_met": bool(abs(dm["max_w"] - dd["max_w"]) < 0.5 and
+                                     abs(dm["rms_theta_p"] - dd["rms_theta_p"]) < 0.05)}
+            if control:
+                dc = ms["control"].diag()
+                e["control_max_w_error"] = abs(dc["max_w"] - dd["max_w"])
+                e["control_rms_theta_p_error"] = abs(dc["rms_theta_p"] - dd["rms_theta_p"])
+            series.append(e)
+        fp64 = ms["fp64"].download()
+        res = compare(wl, ms["mixed"].download(), fp64, ms["mixed"].diag(), ms["fp64"].diag())
+        if control:
+            c = compare(wl, ms["control"].download(), fp64, ms["control"].diag(), ms["fp64"].diag())
+            res["control_fp64_from_f32_state"] = {k: c[k] for k in (
+                "max_w_error", "w_pointwise_max_error", "rms_theta_p_error", "theta_p_rms_difference",
+                "rain_relative_error")}
     finally:
         ctx.set_graphs(False)
         for m in ms.values():
@@ -105,8 +125,8 @@ def one_hour(ctx, wl, mode=L.ACOUSTIC_HEVI, steps=None, every_seconds=600.0):
     first_fail = next((e["simulated_seconds"] for e in series if not e["targets_met"]), None)
     res.update({"workload": wl.describe(), "simulated_seconds": steps * wl.dt, "large_steps": steps,
                 "acoustic_mode": "hevi" if mode == L.ACOUSTIC_HEVI else "explicit",
-                "wall_seconds_mixed": round(wall[L.PRECISION_MIXED], 3),
-                "wall_seconds_fp64": round(wall[L.PRECISION_FP64], 3),
+                "wall_seconds_mixed": round(wall["mixed"], 3),
+                "wall_seconds_fp64": round(wall["fp64"], 3),
                 "series": series, "targets_met_through_seconds": (first_fail - chunk * wl.dt) if first_fail
                 else (met[-1] if met else 0.0)})
     return res

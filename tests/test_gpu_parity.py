@@ -497,3 +497,23 @@ def test_fp64_and_mixed_members_in_one_call(hip_ctx):
     for f in L.FIELDS:
         assert np.array_equal(ra[f], alone64[f]), f
         assert np.array_equal(rb[f], alone32[f]), f
+
+
+def test_accuracy_run_reports_the_fp32_rounding_control(hip_ctx):
+    """The side-by-side accuracy run on the 64 x 64 x 40 workload for a few
+    steps: the FP64 control started from the FP32-rounded state stays much
+    closer to the FP64 run than FP32 storage does at first, and all three
+    members report finite diagnostics."""
+    from nimbus_amd import accuracy as ACC
+    from nimbus_amd import workloads as W
+    wl = W.NAMED["c0"]
+    res = ACC.one_hour(hip_ctx, wl, L.ACOUSTIC_HEVI, steps=4, every_seconds=2 * wl.dt)
+    assert len(res["series"]) == 2
+    for e in res["series"]:
+        for k in ("max_w_error", "rms_theta_p_error", "control_max_w_error", "control_rms_theta_p_error"):
+            assert np.isfinite(e[k]), k
+    c = res["control_fp64_from_f32_state"]
+    print("mixed", res["w_pointwise_max_error"], res["theta_p_rms_difference"],
+          "control", c["w_pointwise_max_error"], c["theta_p_rms_difference"])
+    assert c["theta_p_rms_difference"] < res["theta_p_rms_difference"]
+    assert c["w_pointwise_max_error"] < 1e-3
