@@ -268,6 +268,26 @@ def test_hevi_column_depths_fp64_match_fp64_oracle(hip_ctx, nz):
     check_fp64(out, ref, f"hevi nz {nz}", False)
 
 
+@pytest.mark.parametrize("lateral", [
+    (L.BC_PERIODIC, L.BC_RADIATIVE, L.BC_RIGID_SLIP, L.BC_PERIODIC),
+    (L.BC_RADIATIVE, L.BC_PERIODIC, L.BC_PERIODIC, L.BC_ZERO_GRADIENT),
+    (L.BC_RIGID, L.BC_ZERO_GRADIENT, L.BC_RADIATIVE, L.BC_RIGID),
+])
+def test_mixed_lateral_pairings_fp64_match_fp64_oracle(hip_ctx, lateral):
+    """West/east and south/north faces of different kinds, including a
+    periodic low face beside a non-periodic high one (the periodic copy reads
+    the interior normal velocity the other face then overwrites)."""
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, wind_noise=1.0, dtype=np.float64)
+    for mode in (L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI):
+        cfg = L.step_config(3.0, 6, acoustic_mode=mode, lateral=lateral)
+        ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, 3, np.float64)
+        out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, 3)
+        # with wind noise, transport carries a stash-rounded face velocity into
+        # the water species next to a radiative face, as an SGS closure does
+        rad = L.BC_RADIATIVE in lateral
+        check_fp64(out, ref, f"lateral {lateral} mode {mode}", rad, diffusive=rad)
+
+
 # Sponge settings scaled to the 7 km deep test domain: upper sponge from 4 km,
 # 4-cell w absorber, 3-cell lateral band.
 SMALL_SPONGE = {"z_d": 4000.0, "alpha": 0.05, "w_cells": 4, "w_alpha": 0.2, "lateral_cells": 3,
