@@ -76,6 +76,14 @@ ALGO_PER_CELL = {
 }
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """One line per phase on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
 def algo_bytes_per_cell(name, mode, elem):
     e = ALGO_PER_CELL.get(name)
     if isinstance(e, dict):
@@ -227,6 +235,7 @@ def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5):
         if res.fields is None:
             continue
         req = by_id[res.rid]
+        progress(f"  FP64 rerun of request {res.rid}: {req.workload.describe()}, {req.workload.steps} steps")
         fields, diag = SV.run_alone(ctx, req, precision=L.PRECISION_FP64, acoustic_mode=mode)
         e = ACC.compare(req.workload, res.fields, fields, res.diag, diag)
         errors.append({"rid": res.rid, "grid": req.workload.describe(), "steps": req.workload.steps,
@@ -292,6 +301,7 @@ def main():
                               ("mixed_explicit", L.PRECISION_MIXED, "explicit"),
                               ("fp64_explicit", L.PRECISION_FP64, "explicit"),
                               ("fp64_hevi", L.PRECISION_FP64, "hevi")):
+        progress(f"{label}: {wl.describe()}")
         steps = args.steps if prec == L.PRECISION_MIXED else max(1, min(args.steps, 10))
         m, ms, _ = run_member(ctx, L, wl, prec, modes[mode], steps, args.warmup if prec == L.PRECISION_MIXED else 1,
                               graphs)
@@ -360,18 +370,24 @@ def main():
         "host": {"platform": platform.platform(), **cpu_info()},
     }
     if not args.no_ensemble:
+        progress("ensemble: 32 members, 30 simulated minutes each")
         result["ensemble_c3"] = ensemble_run(ctx, L, W, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
     if args.serving_seconds > 0:
+        progress(f"serving: Poisson arrivals over {args.serving_seconds:g} s")
         result["serving_c4"] = serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
     if args.accuracy:
         from nimbus_amd import accuracy as ACC
         for q, name in enumerate(n for n in args.accuracy.split(",") if n):
-            acc = ACC.one_hour(ctx, W.NAMED[name], L.ACOUSTIC_HEVI)
+            progress(f"accuracy: one simulated hour of {name}, mixed / FP64 / FP64 control")
+            acc = ACC.one_hour(ctx, W.NAMED[name], L.ACOUSTIC_HEVI,
+                               progress=lambda t, name=name: progress(f"  {name}: {t:g} simulated s"))
             result["accuracy_1h" if q == 0 else f"accuracy_1h_{name}"] = acc
     ctx.close()
     if not args.no_cpu_baseline:
+        progress("cpu baseline: the oracle on the host cores")
         result["cpu_baseline"] = cpu_baseline(wl)
         result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+    progress("done")
     print(json.dumps(result))
 
 

@@ -67,7 +67,7 @@ def compare(wl, mixed, fp64, d_mixed, d_fp64):
     return out
 
 
-def one_hour(ctx, wl, mode=L.ACOUSTIC_HEVI, steps=None, every_seconds=600.0, control=True):
+def one_hour(ctx, wl, mode=L.ACOUSTIC_HEVI, steps=None, every_seconds=600.0, control=True, progress=None):
     """Mixed and FP64 members advanced side by side from the same state; the
     run-level errors at every `every_seconds` of simulated time, and the full
     per-field comparison at the end.
@@ -76,7 +76,8 @@ def one_hour(ctx, wl, mode=L.ACOUSTIC_HEVI, steps=None, every_seconds=600.0, con
     (the state the mixed member starts from) and report its distance to the
     FP64 run beside the mixed one. It separates what FP32 arithmetic adds
     from how far the flow itself carries a perturbation the size of one FP32
-    rounding of the initial fields."""
+    rounding of the initial fields. progress(simulated_seconds) is called
+    after every comparison."""
     steps = int(steps or round(3600.0 / wl.dt))
     chunk = max(1, int(round(every_seconds / wl.dt)))
     cfg = wl.config(acoustic_mode=mode)
@@ -110,6 +111,8 @@ def one_hour(ctx, wl, mode=L.ACOUSTIC_HEVI, steps=None, every_seconds=600.0, con
                 e["control_max_w_error"] = abs(dc["max_w"] - dd["max_w"])
                 e["control_rms_theta_p_error"] = abs(dc["rms_theta_p"] - dd["rms_theta_p"])
             series.append(e)
+            if progress:
+                progress(done * wl.dt)
         fp64 = ms["fp64"].download()
         res = compare(wl, ms["mixed"].download(), fp64, ms["mixed"].diag(), ms["fp64"].diag())
         if control:
