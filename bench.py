@@ -260,7 +260,9 @@ def main():
                     help="workloads of the 1-hour mixed-vs-FP64 error runs, comma-separated; the first is "
                          "reported as accuracy_1h ('' skips them)")
     ap.add_argument("--profile", choices=["hevi", "explicit"], default=None,
-                    help="only the mixed member in this mode: warm-up + timed steps, for rocprofv3 passes")
+                    help="only one member in this mode: warm-up + timed steps, for rocprofv3 passes")
+    ap.add_argument("--profile-precision", choices=["mixed", "fp64"], default="mixed",
+                    help="storage of the --profile member")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -288,9 +290,10 @@ def main():
     modes = {"hevi": L.ACOUSTIC_HEVI, "explicit": L.ACOUSTIC_EXPLICIT}
 
     if args.profile:
-        m, ms, _ = run_member(ctx, L, wl, L.PRECISION_MIXED, modes[args.profile], args.steps, args.warmup, graphs)
-        print(json.dumps({"profile": True, "workload": wl.describe(), "mode": args.profile, "steps": args.steps,
-                          "ms_per_step": ms / args.steps}))
+        prec = L.PRECISION_FP64 if args.profile_precision == "fp64" else L.PRECISION_MIXED
+        m, ms, _ = run_member(ctx, L, wl, prec, modes[args.profile], args.steps, args.warmup, graphs)
+        print(json.dumps({"profile": True, "workload": wl.describe(), "mode": args.profile,
+                          "precision": args.profile_precision, "steps": args.steps, "ms_per_step": ms / args.steps}))
         m.destroy()
         ctx.close()
         return
