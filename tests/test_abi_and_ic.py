@@ -124,3 +124,26 @@ def test_ensemble_draws_cover_the_specified_ranges():
         assert 10.0 <= s["shear"] <= 40.0 and 0.0 <= s["curvature"] <= 1.0
         assert m.steps * m.dt == pytest.approx(1800.0, abs=m.dt)
     assert [m.nx for m in W.ensemble_c3()] == [m.nx for m in members]  # seeded
+
+
+def test_hevi_coefficient_scratch_only_for_the_sequential_sweep():
+    # FP64 members of at most 129 levels solve their columns in the partitioned
+    # form and carry no FP64 Thomas coefficient arrays; FP32-storage members
+    # and deeper FP64 columns keep two FP64 arrays over the padded cells (rows
+    # padded to 64 B of the storage type)
+    def pair(nz, precision):
+        g = L.make_grid(24, 20, nz, 1000.0, 1000.0, 20000.0 / nz)
+        base = L.base_state(g)
+        b = {m: L.member_bytes(g, L.step_config(3.0, 6, acoustic_mode=m), base, precision)
+             for m in (L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI)}
+        return b[L.ACOUSTIC_HEVI] - b[L.ACOUSTIC_EXPLICIT]
+
+    def scratch(nz, elem):
+        per_row = 64 // elem
+        cells = (24 + per_row - 1) // per_row * per_row * 20 * nz
+        return 2 * ((cells * 8 + 255) // 256 * 256)
+
+    assert pair(100, L.PRECISION_FP64) == 0
+    assert pair(129, L.PRECISION_FP64) == 0
+    assert pair(130, L.PRECISION_FP64) == scratch(130, 8)
+    assert pair(100, L.PRECISION_MIXED) == scratch(100, 4)
