@@ -767,9 +767,9 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     const DevGrid& g = A.g;
-    // uniform over the block (one member per block); FP64 members of at most
-    // kSegMaxNz levels take the partitioned form
-    if (s >= A.p.substeps || hevi_partitioned(int(sizeof(S)), g.nz)) return;
+    // uniform over the block (one member per block); members the batch solves
+    // in the partitioned form are not this kernel's
+    if (s >= A.p.substeps || A.p.hevi_seg) return;
     const Column t = column_of(A, wk.local);
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     extern __shared__ double s_prof[];
@@ -918,12 +918,12 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
     }
 }
 
-// Part 2, partitioned form (FP64 members of at most kSegMaxNz levels): the
-// same rows as the sequential sweep, in FP64, solved without coefficient
-// arrays in memory. Each column's faces 0 .. nz-2 are split into runs of ls
-// consecutive faces, one thread per run (kSegCols consecutive columns of one
-// row x kSegments runs per workgroup, so every load is 16 contiguous lanes
-// of one level). A thread
+// Part 2, partitioned form (columns of at most kSegMaxNz levels, chosen per
+// batch by StepParams::hevi_seg): the same rows as the sequential sweep, in
+// FP64, solved without coefficient arrays in memory. Each column's faces
+// 0 .. nz-2 are split into runs of ls consecutive faces, one thread per run
+// (kSegCols consecutive columns of one row x kSegments runs per workgroup, so
+// every load is 16 contiguous lanes of one level). A thread
 //  1. loads its run's levels once (one level below and two above for the
 //     face densities);
 //  2. forward-eliminates its rows to x_r + c1_r x_{r+1} + a1_r L' = d1_r,
@@ -954,7 +954,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column_seg(const MemberArgs* __
     const DevGrid& g = A.g;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
     // uniform over the block (one member per block): no barrier is skipped by part of it
-    if (s >= A.p.substeps || !hevi_partitioned(int(sizeof(S)), NZ)) return;
+    if (s >= A.p.substeps || !A.p.hevi_seg || NZ > kSegMaxNz) return;
     extern __shared__ double s_prof[];
     const LProf pr = stage_profiles(s_prof, A.f.base, NZ);
     double* const ex = s_prof + kLdsProfiles * NZ;         // [11][kBlock]: run ends and reduced rows
@@ -2871,10 +2871,9 @@ void launch_acoustic_substep(Launcher& L, const BatchView& b, int s, bool accumu
                int(init_at_first));
     launch_map(L, "acoustic_hevi_explicit", k_hevi_explicit<S>, b, S_HEVI, W_CELL, s, int(accumulate),
                int(init_at_first));
-    if constexpr (sizeof(S) == sizeof(double))
-        if (b.any_hevi_seg)
-            launch_map_lds(L, "acoustic_hevi_column_seg", k_hevi_column_seg<S>, b, S_HEVI, W_SEGCOL,
-                           segment_lds_bytes(b), s, int(accumulate), int(init_at_first));
+    if (b.any_hevi_seg)
+        launch_map_lds(L, "acoustic_hevi_column_seg", k_hevi_column_seg<S>, b, S_HEVI, W_SEGCOL,
+                       segment_lds_bytes(b), s, int(accumulate), int(init_at_first));
     if (b.any_hevi_serial)
         launch_map_lds(L, "acoustic_hevi_column", k_hevi_column<S>, b, S_HEVI, W_COLUMN, profile_lds_bytes(b), s,
                        int(accumulate), int(init_at_first));

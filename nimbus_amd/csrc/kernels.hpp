@@ -62,6 +62,7 @@ struct StepParams
     double mf_weight;   // retained mass-flux weight per substep (dts / dt in the large step)
     int substeps;
     int acoustic_mode;
+    int hevi_seg;       // set per batch: the HE-VI columns are solved in the partitioned form
     double hevi_alpha;  // (1 + beta) / 2
     double coriolis_f;
     int bc_west, bc_east, bc_south, bc_north, bc_bottom, bc_top;
@@ -115,18 +116,23 @@ enum WorkKind
 };
 
 // Partitioned HE-VI column solve: a column of nz - 1 unknown faces is split
-// into kSegments runs of at most kSegRows faces, one thread per run. FP64
-// members with nz <= kSegMaxNz use it (measured faster than the sequential
-// sweep there, which is memory-bound on its FP64 coefficient arrays); FP32
-// members and deeper columns take the sequential sweep (for FP32 storage the
-// partitioned form is the slower one: its FP64 elimination is latency-bound
-// at the occupancy its registers allow), which keeps its FP64 coefficients in
-// two cell-sized scratch arrays.
+// into kSegments runs of at most kSegRows faces, one thread per run, for
+// columns of at most kSegMaxNz levels. Chosen per batch (StepParams::hevi_seg):
+// FP64 members always (measured faster than the sequential sweep, which is
+// memory-bound on its FP64 coefficient arrays), FP32-storage members in
+// batches of fewer than kSegSmallBatchColumns HE-VI columns, where one thread
+// per column cannot fill the GPU (measured: 64x64x40 0.49 -> 0.36 ms per
+// large step, 160x160x80 1.29 -> 1.23; at 400x400x100 the partitioned form's
+// latency-bound FP64 elimination is the slower one, 9.06 against 8.04).
+// Deeper columns, and FP32 members in larger batches, take the sequential
+// sweep, which keeps its FP64 coefficients in two cell-sized scratch arrays
+// (allocated for every member that may need it).
 constexpr int kSegments = 16;
 constexpr int kSegCols = 16;
 constexpr int kSegRows = 8;
 constexpr int kSegMaxNz = kSegments * kSegRows + 1;
-constexpr bool hevi_partitioned(int elem, int nz) { return elem == 8 && nz <= kSegMaxNz; }
+constexpr int kSegSmallBatchColumns = 65536;
+constexpr bool hevi_partitioned_always(int elem, int nz) { return elem == 8 && nz <= kSegMaxNz; }
 
 // Member subsets a launch can be restricted to.
 enum MemberSubset

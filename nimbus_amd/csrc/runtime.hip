@@ -82,7 +82,7 @@ Layout layout_for(const DevGrid& g, int elem, bool hevi, int lateral = 0)
     L.field = align256(g.cells() * std::size_t(elem));
     L.rl = align256(g.cells() * sizeof(float));
     // FP64 Thomas coefficients: only the sequential sweep keeps them in memory
-    L.tri = hevi && !hevi_partitioned(elem, g.nz) ? align256(g.cells() * sizeof(double)) : 0;
+    L.tri = hevi && !hevi_partitioned_always(elem, g.nz) ? align256(g.cells() * sizeof(double)) : 0;
     L.rad_w = align256(std::size_t(g.ny) * g.nz * sizeof(float));
     L.rad_s = align256(std::size_t(g.nx) * g.nz * sizeof(float));
     L.rain = align256(std::size_t(g.nx) * g.ny * sizeof(double));
@@ -396,6 +396,7 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     sp.dts = c.dt / substeps;
     sp.mf_weight = sp.dts / c.dt;
     sp.acoustic_mode = c.acoustic_mode;
+    sp.hevi_seg = 0;  // chosen per batch
     sp.hevi_alpha = 0.5 * (1.0 + c.hevi_beta);
     sp.coriolis_f = c.coriolis_f;
     sp.bc_west = c.bc_west; sp.bc_east = c.bc_east; sp.bc_south = c.bc_south; sp.bc_north = c.bc_north;
@@ -473,6 +474,11 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
     b.view.any_odd_substeps = b.view.any_even_kessler_sub = b.view.any_odd_sgs_n = 0;
     b.view.any_sponge_unfused = 0;
     bool remap = true;
+    std::size_t hevi_columns = 0;
+    for (Member* m : ms)
+        if (m->args.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI)
+            hevi_columns += std::size_t(m->args.g.nx) * m->args.g.ny;
+    const bool small_batch = hevi_columns < std::size_t(kSegSmallBatchColumns);
     for (Member* m : ms)
     {
         args.push_back(m->args);
@@ -487,7 +493,9 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
         b.view.any_sponge_unfused |= m->args.p.sponge && m->args.p.sgs_n == 0;
         if (m->args.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI)
         {
-            const bool part = hevi_partitioned(m->elem, m->args.g.nz);
+            const bool part = hevi_partitioned_always(m->elem, m->args.g.nz) ||
+                              (small_batch && m->args.g.nz <= kSegMaxNz);
+            args.back().p.hevi_seg = part ? 1 : 0;
             b.view.any_hevi_seg |= part;
             b.view.any_hevi_serial |= !part;
         }

@@ -288,6 +288,34 @@ def test_mixed_lateral_pairings_fp64_match_fp64_oracle(hip_ctx, lateral):
         check_fp64(out, ref, f"lateral {lateral} mode {mode}", rad, diffusive=rad)
 
 
+def test_mixed_column_solve_forms_agree(hip_ctx):
+    """FP32-storage members take the partitioned column solve in batches of
+    fewer than 65536 HE-VI columns and the sequential sweep in larger ones: the
+    same member stepped alone (partitioned) and beside a 264 x 256 member
+    (sequential) agrees to the FP32 tolerance of the batched-substep test."""
+    g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, wind_noise=1.0, dtype=np.float32)
+    cfg = L.step_config(3.0, 6, acoustic_mode=L.ACOUSTIC_HEVI, lateral=L.BC_RIGID_SLIP)
+    gb = L.make_grid(264, 256, 4, 1000.0, 1000.0, 500.0)
+    big = hip_ctx.member(gb, cfg, L.base_state(gb), L.PRECISION_MIXED)
+    alone = hip_ctx.member(g, cfg, base, L.PRECISION_MIXED)
+    beside = hip_ctx.member(g, cfg, base, L.PRECISION_MIXED)
+    try:
+        big.upload(L.initial_state(gb, L.ic_params(gb), np.float32))
+        alone.upload(st)
+        beside.upload(st)
+        hip_ctx.step([alone], 3)
+        hip_ctx.step([beside, big], 3)
+        a, b = alone.download(), beside.download()
+    finally:
+        for m in (big, alone, beside):
+            m.destroy()
+    for f, (rel, ab) in {"u": (1e-5, 1e-4), "v": (1e-5, 1e-4), "w": (1e-5, 1e-4), "rho": (1e-6, 1e-6),
+                         "p": (1e-6, 0.1), "theta": (1e-6, 1e-4)}.items():
+        nbad, err = within_tier(a[f], b[f], rel, ab)
+        print(f"column forms {f}: max abs diff {err:.3e}")
+        assert nbad == 0, f
+
+
 # Sponge settings scaled to the 7 km deep test domain: upper sponge from 4 km,
 # 4-cell w absorber, 3-cell lateral band.
 SMALL_SPONGE = {"z_d": 4000.0, "alpha": 0.05, "w_cells": 4, "w_alpha": 0.2, "lateral_cells": 3,
