@@ -122,8 +122,9 @@ enum WorkKind
 // memory-bound on its FP64 coefficient arrays), FP32-storage members in
 // batches of fewer than kSegSmallBatchColumns HE-VI columns, where one thread
 // per column cannot fill the GPU (measured: 64x64x40 0.49 -> 0.36 ms per
-// large step, 160x160x80 1.29 -> 1.23; at 400x400x100 the partitioned form's
-// latency-bound FP64 elimination is the slower one, 9.06 against 8.04).
+// large step, 160x160x80 1.29 -> 1.23; by 255x255 columns the partitioned
+// form's latency-bound FP64 elimination is the slower one, and at 400x400x100
+// 9.06 against 8.04).
 // Deeper columns, and FP32 members in larger batches, take the sequential
 // sweep, which keeps its FP64 coefficients in two cell-sized scratch arrays
 // (allocated for every member that may need it).
@@ -131,7 +132,7 @@ constexpr int kSegments = 16;
 constexpr int kSegCols = 16;
 constexpr int kSegRows = 8;
 constexpr int kSegMaxNz = kSegments * kSegRows + 1;
-constexpr int kSegSmallBatchColumns = 65536;
+constexpr int kSegSmallBatchColumns = 32768;
 constexpr bool hevi_partitioned_always(int elem, int nz) { return elem == 8 && nz <= kSegMaxNz; }
 
 // Member subsets a launch can be restricted to.

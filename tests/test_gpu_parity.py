@@ -290,7 +290,7 @@ def test_mixed_lateral_pairings_fp64_match_fp64_oracle(hip_ctx, lateral):
 
 def test_mixed_column_solve_forms_agree(hip_ctx):
     """FP32-storage members take the partitioned column solve in batches of
-    fewer than 65536 HE-VI columns and the sequential sweep in larger ones: the
+    fewer than 32768 HE-VI columns and the sequential sweep in larger ones: the
     same member stepped alone (partitioned) and beside a 264 x 256 member
     (sequential) agrees to the FP32 tolerance of the batched-substep test."""
     g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True, wind_noise=1.0, dtype=np.float32)
