@@ -1199,51 +1199,90 @@ double kessler_bottom_sink(const Grid& g, const KesslerParams& kp, const std::ve
     return double(S(S(Vt) * q)) / g.dz;
 }
 
-// Commit of the microphysics tendencies with the reference's bounds
-// (microphysics_step.cpp:1090-1140: theta step limited to +-50 K and clamped
-// to [200, 700] K, water species clamped to [0, 0.05]) and the surface rain
-// ledger (microphysics_step.cpp:1917-2024: precipitation flux = sink*rho*dz,
-// 1 kg m^-2 = 1 mm).
+// Longest stable microphysics step of the state (sedimentation_step_limit,
+// microphysics_step.cpp:3501-3572): the minimum over the physical cells of
+// 0.8 * layer thickness / speed, where speed is the scheme's largest explicit
+// terminal velocity for the cell's total water qv + qc + qr at its density
+// (KesslerScheme::explicit_terminal_velocity_mps, kessler.cpp:39-44:
+// min(Vt_max, kessler_rain(q, rho) sqrt(1.2 / rho))); infinity when no cell
+// has a positive speed.
 template <class S>
-void microphysics_step(const Grid& g, const Config& cfg, State<S>& s, double dt)
+double sedimentation_limit(const Grid& g, const Config& cfg, const State<S>& s)
 {
-    const KesslerParams kp{cfg.qc0, cfg.c_auto, cfg.c_accr, cfg.evap_eff, cfg.vt_max};
-    // sedimentation subcycling limit 0.8*dz/Vt_max(q_total) (microphysics_step.cpp:3501-3560)
-    double vmax = 0.0;
+    double limit = std::numeric_limits<double>::infinity();
     for (int i = 1; i < g.nx - 1; ++i)
         for (int j = 1; j < g.ny - 1; ++j)
             for (int k = 0; k < g.nz; ++k)
             {
                 const std::size_t c = g.at(i, j, k);
-                const double qt = double(s.qv[c]) + double(s.qc[c]) + double(s.qr[c]);
                 const double r = double(s.rho[c]);
-                vmax = std::max(vmax, std::min(cfg.vt_max, kessler_rain_speed(qt, r) * std::sqrt(1.2 / r)));
+                const double qt = double(s.qv[c]) + double(s.qc[c]) + double(s.qr[c]);
+                const double speed = std::min(cfg.vt_max, kessler_rain_speed(qt, r) * std::sqrt(1.2 / r));
+                if (speed > 0.0) limit = std::min(limit, 0.8 * g.dz / speed);
             }
-    const double limit = vmax > 0.0 ? 0.8 * g.dz / vmax : std::numeric_limits<double>::infinity();
-    const int nsub = dt <= limit ? 1 : int(std::ceil(dt / limit));
-    const double h = dt / nsub;
+    return limit;
+}
+
+// One microphysics step of length h: Kessler tendencies (saturation
+// adjustment, warm rain, sedimentation), the bounded commit
+// (microphysics_step.cpp:1090-1140: theta step limited to +-50 K and clamped
+// to [200, 700] K, water species clamped to [0, 0.05]) and the surface rain
+// ledger (microphysics_step.cpp:1917-2024: precipitation flux = sink*rho*dz,
+// 1 kg m^-2 = 1 mm).
+template <class S>
+void microphysics_once(const Grid& g, const Config& cfg, State<S>& s, double h)
+{
+    const KesslerParams kp{cfg.qc0, cfg.c_auto, cfg.c_accr, cfg.evap_eff, cfg.vt_max};
     std::vector<S> dth, dqv, dqc, dqr;
-    for (int n = 0; n < nsub; ++n)
-    {
-        kessler_tendencies(g, kp, s.p, s.theta, s.rho, s.qv, s.qc, s.qr, h, dth, dqv, dqc, dqr);
-        for (int i = 1; i < g.nx - 1; ++i)
-            for (int j = 1; j < g.ny - 1; ++j)
-            {
-                const double sink = kessler_bottom_sink(g, kp, s.rho, s.qr, i, j);
-                s.rain_mm[std::size_t(i) * g.ny + j] += sink * double(s.rho[g.at(i, j, 0)]) * g.dz * h;
-            }
+    kessler_tendencies(g, kp, s.p, s.theta, s.rho, s.qv, s.qc, s.qr, h, dth, dqv, dqc, dqr);
+    for (int i = 1; i < g.nx - 1; ++i)
+        for (int j = 1; j < g.ny - 1; ++j)
+        {
+            const double sink = kessler_bottom_sink(g, kp, s.rho, s.qr, i, j);
+            s.rain_mm[std::size_t(i) * g.ny + j] += sink * double(s.rho[g.at(i, j, 0)]) * g.dz * h;
+        }
 #pragma omp parallel for collapse(2) schedule(static)
-        for (int i = 1; i < g.nx - 1; ++i)
-            for (int j = 1; j < g.ny - 1; ++j)
-                for (int k = 0; k < g.nz; ++k)
-                {
-                    const std::size_t c = g.at(i, j, k);
-                    const double step = std::clamp(double(dth[c]) * h, -50.0, 50.0);
-                    s.theta[c] = S(clamp_store<S>(double(s.theta[c]) + step, 300.0, kThetaMin, kThetaMax));
-                    s.qv[c] = S(clamp_store<S>(double(s.qv[c]) + double(dqv[c]) * h, 0.0, 0.0f, kQvMax));
-                    s.qc[c] = S(clamp_store<S>(double(s.qc[c]) + double(dqc[c]) * h, 0.0, 0.0f, kQhMax));
-                    s.qr[c] = S(clamp_store<S>(double(s.qr[c]) + double(dqr[c]) * h, 0.0, 0.0f, kQhMax));
-                }
+    for (int i = 1; i < g.nx - 1; ++i)
+        for (int j = 1; j < g.ny - 1; ++j)
+            for (int k = 0; k < g.nz; ++k)
+            {
+                const std::size_t c = g.at(i, j, k);
+                const double step = std::clamp(double(dth[c]) * h, -50.0, 50.0);
+                s.theta[c] = S(clamp_store<S>(double(s.theta[c]) + step, 300.0, kThetaMin, kThetaMax));
+                s.qv[c] = S(clamp_store<S>(double(s.qv[c]) + double(dqv[c]) * h, 0.0, 0.0f, kQvMax));
+                s.qc[c] = S(clamp_store<S>(double(s.qc[c]) + double(dqc[c]) * h, 0.0, 0.0f, kQhMax));
+                s.qr[c] = S(clamp_store<S>(double(s.qr[c]) + double(dqr[c]) * h, 0.0, 0.0f, kQhMax));
+            }
+}
+
+// Microphysics over an interval with the reference's state-dependent
+// subcycling (step_microphysics, microphysics_step.cpp:3619,3700-3750): one
+// step when the interval fits under the state's limit; otherwise steps of
+// min(remaining, limit), the limit recomputed from the state after each step
+// that leaves time remaining. The step lengths are appended to `lengths`
+// when it is given.
+template <class S>
+void microphysics_step(const Grid& g, const Config& cfg, State<S>& s, double dt,
+                       std::vector<double>* lengths = nullptr)
+{
+    double limit = sedimentation_limit(g, cfg, s);
+    if (dt <= limit)
+    {
+        microphysics_once(g, cfg, s, dt);
+        if (lengths) lengths->push_back(dt);
+        return;
+    }
+    double elapsed = 0.0;
+    int n = 0;
+    while (elapsed < dt)
+    {
+        if (++n > 100000) throw std::runtime_error("microphysics interval requires too many substeps");
+        const double h = std::min(dt - elapsed, limit);
+        if (!(h > 0.0) || elapsed + h <= elapsed) throw std::runtime_error("microphysics substep cannot advance");
+        microphysics_once(g, cfg, s, h);
+        if (lengths) lengths->push_back(h);
+        elapsed += h;
+        if (elapsed < dt) limit = sedimentation_limit(g, cfg, s);
     }
 }
 

@@ -253,6 +253,17 @@ int oracle_apply_boundaries_f32(const oc_grid* gp, const oc_config* cp, const oc
     return 0;
 }
 
+int oracle_apply_boundaries_f64(const oc_grid* gp, const oc_config* cp, const oc_state* st, double dt_rad)
+{
+    const Grid g = to_grid(gp);
+    State<double> s;
+    load(g, st, s);
+    seed_radiation(g, s);
+    apply_boundaries(g, to_config(cp), s, dt_rad);
+    store(g, s, st);
+    return 0;
+}
+
 int oracle_slow_tendencies_f32(const oc_grid* gp, const oc_base* bp, const oc_config* cp, const oc_state* st,
                                float* du, float* dv, float* dw)
 {
@@ -264,6 +275,20 @@ int oracle_slow_tendencies_f32(const oc_grid* gp, const oc_base* bp, const oc_co
     std::memcpy(du, a.data(), sizeof(float) * g.cells());
     std::memcpy(dv, b.data(), sizeof(float) * g.cells());
     std::memcpy(dw, c.data(), sizeof(float) * g.cells());
+    return 0;
+}
+
+int oracle_slow_tendencies_f64(const oc_grid* gp, const oc_base* bp, const oc_config* cp, const oc_state* st,
+                               double* du, double* dv, double* dw)
+{
+    const Grid g = to_grid(gp);
+    State<double> s;
+    load(g, st, s);
+    std::vector<double> a, b, c;
+    slow_tendencies(g, to_base(bp, g.nz), to_config(cp), s, a, b, c);
+    std::memcpy(du, a.data(), sizeof(double) * g.cells());
+    std::memcpy(dv, b.data(), sizeof(double) * g.cells());
+    std::memcpy(dw, c.data(), sizeof(double) * g.cells());
     return 0;
 }
 
@@ -335,24 +360,32 @@ int oracle_kessler_tendencies_f32(const oc_grid* gp, const oc_config* cp, const 
     return 0;
 }
 
-int oracle_microphysics_step_f32(const oc_grid* gp, const oc_config* cp, const oc_state* st, double dt)
+// The microphysics interval; the lengths of its steps go to `lengths`
+// (capacity `cap`), their count is returned.
+int oracle_microphysics_step_f32(const oc_grid* gp, const oc_config* cp, const oc_state* st, double dt,
+                                 double* lengths, int cap)
 {
     const Grid g = to_grid(gp);
     State<float> s;
     load(g, st, s);
-    microphysics_step(g, to_config(cp), s, dt);
+    std::vector<double> h;
+    microphysics_step(g, to_config(cp), s, dt, &h);
     store(g, s, st);
-    return 0;
+    for (int q = 0; q < int(h.size()) && q < cap; ++q) lengths[q] = h[q];
+    return int(h.size());
 }
 
-int oracle_microphysics_step_f64(const oc_grid* gp, const oc_config* cp, const oc_state* st, double dt)
+int oracle_microphysics_step_f64(const oc_grid* gp, const oc_config* cp, const oc_state* st, double dt,
+                                 double* lengths, int cap)
 {
     const Grid g = to_grid(gp);
     State<double> s;
     load(g, st, s);
-    microphysics_step(g, to_config(cp), s, dt);
+    std::vector<double> h;
+    microphysics_step(g, to_config(cp), s, dt, &h);
     store(g, s, st);
-    return 0;
+    for (int q = 0; q < int(h.size()) && q < cap; ++q) lengths[q] = h[q];
+    return int(h.size());
 }
 
 int oracle_apply_sponges_f64(const oc_grid* gp, const oc_base* bp, const oc_config* cp, const oc_state* st, double dt)

@@ -74,6 +74,8 @@ def lib():
                                                            _dp, _dp, _dp]
         L.oracle_apply_boundaries_f32.argtypes = [P(OcGrid), P(OcConfig), P(OcState), ctypes.c_double]
         L.oracle_slow_tendencies_f32.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), _fp, _fp, _fp]
+        L.oracle_slow_tendencies_f64.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState), _dp, _dp, _dp]
+        L.oracle_apply_boundaries_f64.argtypes = [P(OcGrid), P(OcConfig), P(OcState), ctypes.c_double]
         L.oracle_slow_rk2_f32.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState)]
         L.oracle_slow_rk2_f64.argtypes = [P(OcGrid), P(OcBase), P(OcConfig), P(OcState)]
         L.oracle_transport_f32.argtypes = [P(OcGrid), P(OcConfig), _fp, _fp, _fp, _fp, _fp, _fp, _fp,
@@ -81,8 +83,10 @@ def lib():
         L.oracle_transport_f64.argtypes = [P(OcGrid), P(OcConfig), _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                            ctypes.c_double, ctypes.c_int]
         L.oracle_kessler_tendencies_f32.argtypes = [P(OcGrid), P(OcConfig)] + [_fp] * 6 + [ctypes.c_double] + [_fp] * 4
-        L.oracle_microphysics_step_f32.argtypes = [P(OcGrid), P(OcConfig), P(OcState), ctypes.c_double]
-        L.oracle_microphysics_step_f64.argtypes = [P(OcGrid), P(OcConfig), P(OcState), ctypes.c_double]
+        L.oracle_microphysics_step_f32.argtypes = [P(OcGrid), P(OcConfig), P(OcState), ctypes.c_double, _dp,
+                                                   ctypes.c_int]
+        L.oracle_microphysics_step_f64.argtypes = [P(OcGrid), P(OcConfig), P(OcState), ctypes.c_double, _dp,
+                                                   ctypes.c_int]
         for name, n in (("oracle_weno5_face", 5), ("oracle_mc_slope", 3), ("oracle_upwind_fd5", 9),
                         ("oracle_upwind_fd3", 7), ("oracle_qvs_water", 2), ("oracle_theta_to_temperature", 2),
                         ("oracle_eos_pressure", 2), ("oracle_kessler_rain_speed", 2)):
@@ -194,20 +198,25 @@ def acoustic_substeps_batched(g, base, cfg, st, n, dts, weight, mf=None, dtype=n
     return s, mf
 
 
-def apply_boundaries(g, cfg, st, dt_rad):
-    s = copy_state(st, np.float32)
-    lib().oracle_apply_boundaries_f32(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)),
-                                      float(dt_rad))
+def apply_boundaries(g, cfg, st, dt_rad, dtype=np.float32):
+    """One boundary closure on a copy of st, the radiative stash seeded from
+    the input faces; dt_rad > 0 radiates the KW78 faces over dt_rad."""
+    s = copy_state(st, dtype)
+    fn = lib().oracle_apply_boundaries_f32 if np.dtype(dtype) == np.float32 else lib().oracle_apply_boundaries_f64
+    fn(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)), float(dt_rad))
     return s
 
 
-def slow_tendencies(g, base, cfg, st):
-    s = copy_state(st, np.float32)
+def slow_tendencies(g, base, cfg, st, dtype=np.float32):
+    s = copy_state(st, dtype)
     b = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in base.items()}
-    du, dv, dw = (np.zeros(st["u"].shape, dtype=np.float32) for _ in range(3))
-    lib().oracle_slow_tendencies_f32(ctypes.byref(g), ctypes.byref(base_struct(b)), ctypes.byref(cfg),
-                                     ctypes.byref(_state_struct(s)), du.ctypes.data_as(_fp),
-                                     dv.ctypes.data_as(_fp), dw.ctypes.data_as(_fp))
+    du, dv, dw = (np.zeros(st["u"].shape, dtype=dtype) for _ in range(3))
+    if np.dtype(dtype) == np.float32:
+        fn, ptr = lib().oracle_slow_tendencies_f32, _fp
+    else:
+        fn, ptr = lib().oracle_slow_tendencies_f64, _dp
+    fn(ctypes.byref(g), ctypes.byref(base_struct(b)), ctypes.byref(cfg), ctypes.byref(_state_struct(s)),
+       du.ctypes.data_as(ptr), dv.ctypes.data_as(ptr), dw.ctypes.data_as(ptr))
     return du, dv, dw
 
 
@@ -241,11 +250,15 @@ def kessler_tendencies(g, cfg, p, theta, rho, qv, qc, qr, dt):
     return outs
 
 
-def microphysics_step(g, cfg, st, dt, dtype=np.float32):
+def microphysics_step(g, cfg, st, dt, dtype=np.float32, lengths=False):
+    """The microphysics interval dt on a copy of st; with lengths=True also
+    the lengths of the steps the state-dependent subcycling took."""
     s = copy_state(st, dtype)
     fn = lib().oracle_microphysics_step_f32 if np.dtype(dtype) == np.float32 else lib().oracle_microphysics_step_f64
-    fn(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)), float(dt))
-    return s
+    cap = 4096
+    h = np.zeros(cap)
+    n = fn(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)), float(dt), h.ctypes.data_as(_dp), cap)
+    return (s, h[:min(n, cap)].copy()) if lengths else s
 
 
 def apply_sponges(g, base, cfg, st, dt):
