@@ -30,19 +30,66 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s per large step (FP32-mixed vs FP64) + p50/p99 ensemble latency"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per cell of each kernel, in units of the storage size s
-# (4 B mixed, 8 B FP64): every field a phase must read or write, once, with
-# the phase totals of the large-step budget B(N) = (68 + 17 N) s:
-# acoustic substep 17 s (u, v, w, rho, p, theta in; u, v, w, rho, p out;
-# three retained mass fluxes read and written), split for HE-VI into the
-# column solve (13 s) and the u, v momentum update (4 s); scalar transport
-# 13 s (4 scalars, 3 mass fluxes, 2 densities in; 4 scalars out) split over
-# its flux (7 s), limiter (1 s) and update (5 s) passes; Kessler with
-# sedimentation, apply and EOS 11 s; slow stage 1 + predictor 16 s; stage 2 +
-# average + apply 28 s. Face-flux and Thomas scratch are not algorithmic.
-# Algorithmic HBM traffic per cell of each kernel (one launch): storage
-# elements read or written once, plus FP64 bytes. Neighbour values that the
-# caches serve are not counted. Per acoustic mode where the kernel differs.
+# Algorithmic bytes per cell update of each phase of the large step, in units
+# of the storage size s (4 B mixed, 8 B FP64): every field the phase must read
+# or write, once (halos, 1-D profiles and 2-D rain free). The budget of one
+# large step is B(N) = (68 + 17 N) s for N acoustic substeps:
+#   slow stage 1 + predictor 16, stage 2 + average 16, slow apply 12;
+#   each acoustic substep 17 (u, v, w, p', rho', theta in; u, v, w, p', rho'
+#   out; three retained mass fluxes read and written);
+#   scalar transport of 4 scalars 13 (9 in, 4 out);
+#   Kessler with sedimentation, apply and EOS 11 (7 in, 4 out).
+# A phase is run by the kernels listed with it; `per_step` is how many times
+# the phase runs per large step ("substeps" = once per acoustic substep).
+# The Smagorinsky SGS and the sponges are outside the budget (reported per
+# kernel only).
+PHASES = {
+    "acoustic_substep": {"bytes_s": 17, "per_step": "substeps",
+                         "kernels": ("acoustic_forward", "acoustic_hevi_explicit", "acoustic_hevi_column",
+                                     "acoustic_hevi_column_seg", "acoustic_momentum", "acoustic_commit",
+                                     "bc_west_east", "bc_south_north", "bc_vertical", "seed_radiation")},
+    "slow_rk2_apply": {"bytes_s": 44, "per_step": 1, "kernels": ("slow_stage1", "slow_stage2")},
+    "scalar_transport": {"bytes_s": 13, "per_step": 1,
+                         "kernels": ("transport_flux", "transport_limiter", "transport_update")},
+    "kessler": {"bytes_s": 11, "per_step": 1, "kernels": ("kessler_apply", "scalars_home")},
+}
+
+
+def phase_table(tab, cells, elem, n_sub, steps, traffic=None):
+    """Per phase: the budget bytes of one execution, the summed duration of its
+    kernels per execution (HIP events on the launch stream, `steps` large
+    steps), achieved GB/s and fraction of the HBM peak; `traffic` (kernel ->
+    counted HBM bytes per launch) adds the measured bytes per execution."""
+    out = {}
+    for name, ph in PHASES.items():
+        per_step = n_sub if ph["per_step"] == "substeps" else ph["per_step"]
+        execs = per_step * steps
+        ms = sum(tab[k][1] for k in ph["kernels"] if k in tab)
+        if ms <= 0:
+            continue
+        algo = ph["bytes_s"] * elem * cells
+        t = ms / execs
+        gbs = algo / (t / 1e3) / 1e9
+        row = {"kernels": {k: {"launches_per_execution": tab[k][0] / execs,
+                               "avg_launch_ms": round(tab[k][1] / tab[k][0], 4)}
+                           for k in ph["kernels"] if k in tab},
+               "bytes_per_cell": ph["bytes_s"] * elem, "algorithmic_bytes_per_execution": algo,
+               "ms_per_execution": round(t, 4), "executions_per_step": per_step,
+               "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if traffic is not None:
+            have = [k for k in row["kernels"] if k in traffic]
+            if have:
+                counted = sum(traffic[k] * row["kernels"][k]["launches_per_execution"] for k in have)
+                row["traffic_bytes_per_execution"] = int(counted)
+                row["traffic_over_budget"] = round(counted / algo, 2)
+                row["traffic_kernels_without_counters"] = [k for k in row["kernels"] if k not in traffic]
+        out[name] = row
+    return out
+
+
+# Bytes each kernel itself reads and writes once per cell (its own I/O, which
+# exceeds the phase budget where a phase is split over several kernels that
+# pass intermediates through memory). In units of s, plus FP64 bytes.
 ALGO_PER_CELL = {
     # read u v w rho' theta mfx mfy mfz; write rho'(new) p' mfx mfy mfz
     "acoustic_forward": (13, 0),
@@ -163,37 +210,31 @@ def cpu_baseline(wl, sample_steps=6):
                               "sample": "1 large step of the same grid and state", "seconds": round(el1, 3)}}
 
 
-def measured_traffic(workload, mode, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary for this workload and acoustic mode, scaled by the copy
-    calibration recorded with it; None when no matching summary exists."""
+def measured_traffic_all(workload, mode):
+    """kernel -> counted HBM bytes per launch from the committed counter summary."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{mode}.json")
     try:
         with open(path) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    k = d.get("kernels", {}).get(kernel)
-    if not k:
-        return None
-    return int(k["fetch_kib"] * 1024 * d["fetch_scale"] + k["write_kib"] * 1024 * d["write_scale"])
+    return {k: int(v["fetch_kib"] * 1024 * d["fetch_scale"] + v["write_kib"] * 1024 * d["write_scale"])
+            for k, v in d.get("kernels", {}).items()}
 
 
-def roofline_for(tab, cells, elem, workload, mode):
-    """Dominant kernel of a step: algorithmic bytes per launch / measured launch time."""
-    name, (launches, ms) = max(tab.items(), key=lambda kv: kv[1][1])
-    per_cell = algo_bytes_per_cell(name, mode, elem)
-    if per_cell is None:
-        return None
-    avg_ms = ms / launches
-    algo = per_cell * cells
-    achieved = algo / (avg_ms / 1e3) / 1e9
-    traffic = measured_traffic(workload, mode, name)
-    total_ms = sum(v[1] for v in tab.values())
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
-            "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": algo,
-            "share_of_step": round(ms / total_ms, 3)}
+def roofline_for(tab, cells, elem, workload, mode, n_sub, steps):
+    """The dominant phase of the step (largest summed kernel time): its budget
+    bytes per execution / the summed duration of its kernels per execution."""
+    phases = phase_table(tab, cells, elem, n_sub, steps, measured_traffic_all(workload, mode))
+    name, ph = max(phases.items(), key=lambda kv: kv[1]["ms_per_execution"] * kv[1]["executions_per_step"])
+    total_ms = sum(v[1] for v in tab.values()) / steps
+    return {"bound": "hbm", "achieved": ph["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ph["frac"], "traffic": ph.get("traffic_bytes_per_execution"),
+            "traffic_over_budget": ph.get("traffic_over_budget"), "phase": name,
+            "kernels": ph["kernels"], "avg_execution_ms": ph["ms_per_execution"],
+            "algorithmic_bytes_per_execution": ph["algorithmic_bytes_per_execution"],
+            "bytes_per_cell": ph["bytes_per_cell"],
+            "share_of_step": round(ph["ms_per_execution"] * ph["executions_per_step"] / total_ms, 3)}
 
 
 def ensemble_run(ctx, L, W, precision, mode):
@@ -360,15 +401,18 @@ def main():
         "step_algorithmic_GBps": round(step_bytes / (ms_per_step / 1e3) / 1e9, 1),
         "kernels_ms_per_step": {k: per_step(t) for k, t in tables.items()},
         "diag_after_steps": {k: diag[k] for k in ("max_w", "rms_theta_p", "rain_total_mm")},
-        "roofline": roofline_for(tables["mixed_hevi"], wl.cells, 4, args.workload, "hevi"),
-        # whole-step figure with the large-step byte budget B(N) = (68 + 17 N) s per cell update
+        "roofline": roofline_for(tables["mixed_hevi"], wl.cells, 4, args.workload, "hevi", n_sub, 2),
+        # whole step against the large-step byte budget B(N) = (68 + 17 N) s per cell update
         "roofline_step_budget": {"bytes_per_cell_update": (68 + 17 * n_sub) * 4,
                                  "achieved": round(value * (68 + 17 * n_sub) * 4 / 1e9, 1),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": round(value * (68 + 17 * n_sub) * 4 / 1e9 / HBM_PEAK_GBS, 4)},
-        "roofline_mixed_explicit": roofline_for(tables["mixed_explicit"], wl.cells, 4, args.workload, "explicit"),
-        "kernels_roofline": {"mixed_hevi": per_kernel_roofline(tables["mixed_hevi"], "hevi"),
-                             "mixed_explicit": per_kernel_roofline(tables["mixed_explicit"], "explicit")},
+        "roofline_mixed_explicit": roofline_for(tables["mixed_explicit"], wl.cells, 4, args.workload, "explicit",
+                                                runs["mixed_explicit"]["substeps"], 2),
+        "phases_roofline": {k: phase_table(tables[k], wl.cells, 8 if k.startswith("fp64") else 4,
+                                           runs[k]["substeps"], 2) for k in tables},
+        "kernels_io_roofline": {"mixed_hevi": per_kernel_roofline(tables["mixed_hevi"], "hevi"),
+                                "mixed_explicit": per_kernel_roofline(tables["mixed_explicit"], "explicit")},
         "cpu_baseline": None,
         "host": {"platform": platform.platform(), **cpu_info()},
     }

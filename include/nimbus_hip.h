@@ -328,8 +328,25 @@ int nimbus_hip_synchronize(nimbus_hip_ctx* ctx);
  * (tile_w 32 or 64), column marches split into k_chunks vertical chunks
  * (1 .. nz), and XCD-contiguous workgroup ordering on/off. */
 int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int k_chunks, int xcd_remap);
+/* HE-VI column-solve form of a member. It is fixed per member (never chosen
+ * by the batch), so a member's results are the same alone and in any batch.
+ * AUTO: partitioned for FP64 members of at most 129 levels and for FP32
+ * members of fewer than 32768 columns and at most 129 levels, the sequential
+ * sweep otherwise. PARTITIONED is refused above 129 levels, SEQUENTIAL for FP64
+ * members of at most 129 levels (they hold no coefficient scratch). No
+ * reference counterpart: the reference has no vertically implicit solve. */
+enum
+{
+    NIMBUS_COLUMN_SOLVE_AUTO = 0,
+    NIMBUS_COLUMN_SOLVE_SEQUENTIAL = 1,  /* one thread per column, FP64 coefficients in scratch */
+    NIMBUS_COLUMN_SOLVE_PARTITIONED = 2  /* 16 runs of faces per column, reduced system in LDS */
+};
+int nimbus_hip_member_set_column_solve(nimbus_hip_ctx* ctx, uint32_t id, int form);
+/* The member's form (SEQUENTIAL or PARTITIONED; AUTO for explicit-acoustic members). */
+int nimbus_hip_member_column_solve(nimbus_hip_ctx* ctx, uint32_t id, int* form);
 /* Autotuner: times `trials` large steps per launch-geometry candidate on a saved
- * copy of the member state (restored afterwards), keeps the fastest and reports it. */
+ * copy of the member state, keeps the fastest and reports it. Every exit,
+ * failures included, restores the member's state and step count. */
 int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int* best_tile_w, int* best_k_chunks,
                                int* best_remap, double* best_ms);
 /* Enable hipGraph capture of whole large steps (one graph per batch of members). */

@@ -919,7 +919,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column(const MemberArgs* __rest
 }
 
 // Part 2, partitioned form (columns of at most kSegMaxNz levels, chosen per
-// batch by StepParams::hevi_seg): the same rows as the sequential sweep, in
+// member by StepParams::hevi_seg): the same rows as the sequential sweep, in
 // FP64, solved without coefficient arrays in memory. Each column's faces
 // 0 .. nz-2 are split into runs of ls consecutive faces, one thread per run
 // (kSegCols consecutive columns of one row x kSegments runs per workgroup, so
@@ -1869,6 +1869,69 @@ __global__ __launch_bounds__(kBlock) void k_kessler_tendencies(const MemberArgs*
 // pressure projection of every cell (eos_apply.cpp:50-110). Subcycle n reads
 // the transported slots (n even) or the home slots (n odd) and writes the
 // other set, so the level above is still the subcycle's input.
+// Sedimentation limit of a member's state before microphysics subcycle n
+// (sedimentation_step_limit, microphysics_step.cpp:3501-3572): the minimum
+// over physical cells of 0.8 dz / speed, speed = min(Vt_max,
+// kessler_rain(qv + qc + qr, rho) sqrt(1.2 / rho)), folded into the member's
+// clock by an unsigned 64-bit minimum of the (positive) FP64 bit patterns.
+// Reads the set subcycle n reads: the transported slots for even n.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_micro_limit(const MemberArgs* __restrict__ ma,
+                                                        const int2* __restrict__ map, int n)
+{
+    __shared__ double red[kBlock];
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (!A.p.kessler_adaptive || n >= A.p.kessler_sub) return;
+    const Cell cl = cell_of(A, wk.local);
+    const DevGrid& g = A.g;
+    const DevFields& f = A.f;
+    double lim = __builtin_inf();
+    if (cl.valid && cl.i >= 1 && cl.i < g.nx - 1 && cl.j >= 1 && cl.j < g.ny - 1)
+    {
+        const bool from_temp = (n & 1) == 0;
+        const std::size_t c = g.at(cl.i, cl.j, cl.k);
+        const double qt = double(P<S>(from_temp ? f.tqv : f.qv)[c]) + double(P<S>(from_temp ? f.tqc : f.qc)[c]) +
+                          double(P<S>(from_temp ? f.tqr : f.qr)[c]);
+        const double rho = slot(f.base, kBaseRho0, g.nz)[cl.k] + double(P<S>(f.rp)[c]);
+        const double speed = fmin(A.p.kessler.vt_max, dev::kessler_rain_speed<double>(qt, rho) * sqrt(1.2 / rho));
+        if (speed > 0.0) lim = 0.8 * g.dz / speed;
+    }
+    red[threadIdx.x] = lim;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1)
+    {
+        if (int(threadIdx.x) < h) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] < __builtin_inf())
+        atomicMin(reinterpret_cast<unsigned long long*>(f.clock + 2),
+                  static_cast<unsigned long long>(__double_as_longlong(red[0])));
+}
+
+// Microphysics clock of each adaptive member before subcycle n
+// (step_microphysics, microphysics_step.cpp:3700-3750): the step is
+// min(remaining, limit), the whole interval when it fits under the limit, and
+// 0 once the interval is done (the subcycle then passes the state through);
+// the last launch of the budget takes whatever remains. Resets the limit
+// slot for the next reduction. One thread per member.
+__global__ __launch_bounds__(64) void k_micro_clock(const MemberArgs* __restrict__ ma, int n_members, int n)
+{
+    const int m = int(blockIdx.x) * 64 + int(threadIdx.x);
+    if (m >= n_members) return;
+    const MemberArgs& A = ma[m];
+    if (!A.p.kessler_adaptive || n >= A.p.kessler_sub) return;
+    double* clk = A.f.clock;
+    const double dt = A.p.dt;
+    const double elapsed = n == 0 ? 0.0 : clk[0];
+    const double limit = clk[2];
+    double h = 0.0;
+    if (elapsed < dt) h = n == A.p.kessler_sub - 1 ? dt - elapsed : fmin(dt - elapsed, limit);
+    clk[0] = elapsed + h;
+    clk[1] = h;
+    clk[2] = __builtin_inf();
+}
+
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __restrict__ ma,
                                                           const int2* __restrict__ map, int n)
@@ -1897,7 +1960,9 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     const auto p0 = slot(f.base, kBaseP0, NZ);
     const KesslerConst kc = A.p.kessler;
     const Bounds bd = A.p.bounds;
-    const double h = A.p.dt / A.p.kessler_sub;
+    // subcycle length: from the member's clock when it subcycles by its
+    // state, else the whole step (one subcycle)
+    const double h = A.p.kessler_adaptive ? G(f.clock)[1] : A.p.dt / A.p.kessler_sub;
     const bool do_eos = n == A.p.kessler_sub - 1 && A.p.sgs_n == 0;  // with the SGS on, after it
     const std::size_t c = g.at(i, j, k), sz = g.plane();
     // all loads first (the level above clamped at the lid, unused there)
@@ -1908,7 +1973,7 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     const double r0 = rho0[k], r0u = rho0[up ? k + 1 : k], p0k = p0[k];
     pin(thn); pin(qvn); pin(qcn); pin(qrn); pin(rpc); pin(ppc); pin(rpu); pin(qru);
     const double rho = r0 + double(rpc);
-    if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1)
+    if (h > 0.0 && i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1)
     {
         const KesslerOut<S> o = kessler_cell<S>(g.dz, kc, h, p0k + double(ppc), rho, up ? r0u + double(rpu) : 0.0,
                                                 thn, qvn, qcn, qrn, up ? qru : S(0), up);
@@ -2935,7 +3000,18 @@ void launch_large_step(Launcher& L, const BatchView& b)
     const int pd[4] = {0, 1, 1, 1};
     launch_transport<S>(L, b, 4, pd, true);
     for (int n = 0; n < b.max_kessler_sub; ++n)
+    {
+        if (b.any_kessler_adaptive)
+        {
+            launch_map(L, "micro_limit", k_micro_limit<S>, b, S_ALL, W_CELL, n);
+            hipEvent_t ev;
+            L.begin("micro_clock", ev);
+            hipLaunchKernelGGL(k_micro_clock, dim3(unsigned((b.n + 63) / 64)), dim3(64), 0, L.stream, b.args, b.n,
+                               n);
+            L.end("micro_clock", ev);
+        }
         launch_map(L, "kessler_apply", k_kessler_apply<S>, b, S_ALL, W_CELL, n);
+    }
     if (b.any_even_kessler_sub) launch_map(L, "scalars_home", k_scalars_home<S>, b, S_ALL, W_CELL);
     for (int n = 0; n < b.max_sgs_n; ++n)
     {

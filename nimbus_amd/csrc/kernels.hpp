@@ -53,6 +53,7 @@ struct DevFields
     double* base;                                        // kBaseCount * nz FP64 profiles
     double* diag;                                        // kDiagBlocksPerMember * kDiagValues partials + result
     double* sponge_lat;                                  // lateral sponge factor by distance to the nearest face
+    double* clock;                                       // microphysics subcycle clock: elapsed, step, limit bits
 };
 
 struct StepParams
@@ -62,11 +63,12 @@ struct StepParams
     double mf_weight;   // retained mass-flux weight per substep (dts / dt in the large step)
     int substeps;
     int acoustic_mode;
-    int hevi_seg;       // set per batch: the HE-VI columns are solved in the partitioned form
+    int hevi_seg;       // the member's HE-VI columns are solved in the partitioned form (fixed per member)
     double hevi_alpha;  // (1 + beta) / 2
     double coriolis_f;
     int bc_west, bc_east, bc_south, bc_north, bc_bottom, bc_top;
-    int kessler_sub;    // microphysics subcycles per large step
+    int kessler_sub;    // microphysics subcycle launches per large step
+    int kessler_adaptive; // subcycle lengths follow the state's sedimentation limit (device clock)
     int sponge;         // Rayleigh sponges on
     int sponge_lateral; // lateral band width (cells)
     // Smagorinsky SGS: sgs_n substeps of sgs_h seconds, the last sgs_h_last;
@@ -117,23 +119,30 @@ enum WorkKind
 
 // Partitioned HE-VI column solve: a column of nz - 1 unknown faces is split
 // into kSegments runs of at most kSegRows faces, one thread per run, for
-// columns of at most kSegMaxNz levels. Chosen per batch (StepParams::hevi_seg):
-// FP64 members always (measured faster than the sequential sweep, which is
-// memory-bound on its FP64 coefficient arrays), FP32-storage members in
-// batches of fewer than kSegSmallBatchColumns HE-VI columns, where one thread
-// per column cannot fill the GPU (measured: 64x64x40 0.49 -> 0.36 ms per
-// large step, 160x160x80 1.29 -> 1.23; by 255x255 columns the partitioned
-// form's latency-bound FP64 elimination is the slower one, and at 400x400x100
-// 9.06 against 8.04).
-// Deeper columns, and FP32 members in larger batches, take the sequential
-// sweep, which keeps its FP64 coefficients in two cell-sized scratch arrays
-// (allocated for every member that may need it).
+// columns of at most kSegMaxNz levels. The form is a property of the member,
+// fixed at creation (StepParams::hevi_seg, default from the member's own
+// shape, nimbus_hip_member_set_column_solve to override), so a member's
+// arithmetic never depends on the batch it runs in. Default: FP64 members
+// always (measured faster than the sequential sweep, which is memory-bound
+// on its FP64 coefficient arrays); FP32-storage members with fewer than
+// kSegSmallMemberColumns columns, which alone cannot fill the GPU with one
+// thread per column (measured: 64x64x40 0.49 -> 0.36 ms per large step,
+// 160x160x80 1.29 -> 1.23; by 255x255 columns the partitioned form's
+// latency-bound FP64 elimination is the slower one, and at 400x400x100
+// 9.06 against 8.04). Deeper columns take the sequential sweep, which keeps
+// its FP64 coefficients in two cell-sized scratch arrays (allocated for every
+// member that may be solved by it: FP32 storage or more than kSegMaxNz levels).
 constexpr int kSegments = 16;
 constexpr int kSegCols = 16;
 constexpr int kSegRows = 8;
 constexpr int kSegMaxNz = kSegments * kSegRows + 1;
-constexpr int kSegSmallBatchColumns = 32768;
+constexpr int kSegSmallMemberColumns = 32768;
 constexpr bool hevi_partitioned_always(int elem, int nz) { return elem == 8 && nz <= kSegMaxNz; }
+// Default column-solve form of a member: 1 = partitioned, 0 = sequential sweep.
+constexpr int hevi_default_seg(int elem, int nx, int ny, int nz)
+{
+    return nz <= kSegMaxNz && (elem == 8 || long(nx) * long(ny) < long(kSegSmallMemberColumns)) ? 1 : 0;
+}
 
 // Member subsets a launch can be restricted to.
 enum MemberSubset
@@ -163,6 +172,7 @@ struct BatchView
     // subcycle count (scalars home), an odd SGS substep count (SGS home)
     int any_odd_substeps = 0, any_even_kessler_sub = 0, any_odd_sgs_n = 0;
     int any_sponge_unfused = 0;  // a member with sponges and no SGS pass to apply them in
+    int any_kessler_adaptive = 0; // a member whose microphysics subcycles follow its sedimentation limit
     int any_hevi_seg = 0;        // HE-VI members solved in the partitioned form
     int any_hevi_serial = 0;     // HE-VI members solved by the sequential sweep
     int max_nz = 0;

@@ -66,7 +66,7 @@ std::size_t align256(std::size_t x) { return (x + 255) / 256 * 256; }
 
 struct Layout
 {
-    std::size_t field, rl, tri, rad_w, rad_s, rain, base, diag, lat, total;
+    std::size_t field, rl, tri, rad_w, rad_s, rain, base, diag, lat, clock, total;
 };
 
 // Lateral sponge band width in cells (0 when the sponge or its band is off).
@@ -89,8 +89,9 @@ Layout layout_for(const DevGrid& g, int elem, bool hevi, int lateral = 0)
     L.base = align256(std::size_t(kBaseCount) * g.nz * sizeof(double));
     L.diag = align256(std::size_t(kDiagBlocksPerMember + 1) * kDiagValues * sizeof(double));
     L.lat = align256(std::size_t(std::max(lateral, 1)) * sizeof(double));
+    L.clock = align256(4 * sizeof(double));
     L.total = L.field * F_COUNT + 3 * L.rl + 2 * L.tri + 2 * L.rad_w + 2 * L.rad_s + L.rain + L.base + L.diag +
-              L.lat;
+              L.lat + L.clock;
     return L;
 }
 
@@ -267,6 +268,32 @@ void set_geometry(Member& m, int tile_w, int k_chunks, int remap)
     fill_geometry(m.args, m.tile_w, m.k_chunks, m.xcd_remap);
 }
 
+// Fixes the HE-VI column-solve form of a member (NIMBUS_COLUMN_SOLVE_*).
+int set_column_solve(Member& m, int form)
+{
+    if (m.desc.config.acoustic_mode != NIMBUS_ACOUSTIC_HEVI)
+        return fail(NIMBUS_EINVAL, "set_column_solve: the member does not use the HE-VI acoustic mode");
+    const int nz = m.args.g.nz;
+    int seg = 0;
+    switch (form)
+    {
+    case NIMBUS_COLUMN_SOLVE_AUTO: seg = hevi_default_seg(m.elem, m.args.g.nx, m.args.g.ny, nz); break;
+    case NIMBUS_COLUMN_SOLVE_PARTITIONED:
+        if (nz > kSegMaxNz) return fail(NIMBUS_EINVAL, "set_column_solve: the partitioned solve takes at most 129 levels");
+        seg = 1;
+        break;
+    case NIMBUS_COLUMN_SOLVE_SEQUENTIAL:
+        if (!m.args.f.tri_c)
+            return fail(NIMBUS_EINVAL, "set_column_solve: FP64 members of at most 129 levels hold no scratch for the "
+                                       "sequential sweep");
+        seg = 0;
+        break;
+    default: return fail(NIMBUS_EINVAL, "set_column_solve: unknown form");
+    }
+    m.args.p.hevi_seg = seg;
+    return NIMBUS_OK;
+}
+
 int allocate_member(const nimbus_member_desc& desc, Member& m)
 {
     m.desc = desc;
@@ -383,8 +410,15 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     f.base = reinterpret_cast<double*>(p); p += L.base;
     f.diag = reinterpret_cast<double*>(p); p += L.diag;
     f.sponge_lat = reinterpret_cast<double*>(p); p += L.lat;
-    // Zero everything once (not inside any graph), then upload the base state.
+    f.clock = reinterpret_cast<double*>(p); p += L.clock;
+    // Zero everything once (not inside any graph), then upload the base state
+    // and the microphysics clock (its limit slot starts at +infinity).
     e = hipMemset(m.block, 0, L.total);
+    if (e == hipSuccess)
+    {
+        const double clock0[4] = {0.0, 0.0, std::numeric_limits<double>::infinity(), 0.0};
+        e = hipMemcpy(f.clock, clock0, sizeof clock0, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
         e = hipMemcpy(f.base, m.host_base.data(), m.host_base.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(f.sponge_lat, lat.data(), lat.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -396,14 +430,21 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
     sp.dts = c.dt / substeps;
     sp.mf_weight = sp.dts / c.dt;
     sp.acoustic_mode = c.acoustic_mode;
-    sp.hevi_seg = 0;  // chosen per batch
+    sp.hevi_seg = hevi ? hevi_default_seg(m.elem, desc.grid.nx, desc.grid.ny, nz) : 0;
     sp.hevi_alpha = 0.5 * (1.0 + c.hevi_beta);
     sp.coriolis_f = c.coriolis_f;
     sp.bc_west = c.bc_west; sp.bc_east = c.bc_east; sp.bc_south = c.bc_south; sp.bc_north = c.bc_north;
     sp.bc_bottom = c.bc_bottom; sp.bc_top = c.bc_top;
-    // sedimentation CFL bound 0.8 dz / Vt_max (microphysics_step.cpp:3501-3560), state independent
+    // Microphysics subcycling (microphysics_step.cpp:3501-3572,3700-3750): the
+    // reference's limit is the minimum over cells of 0.8 dz / speed with
+    // speed <= Vt_max, so it is never below 0.8 dz / Vt_max. A step that fits
+    // under that bound is one subcycle whatever the state; a longer one takes
+    // the state-dependent lengths min(remaining, limit) from a device-side
+    // clock, within a launch budget that covers the shortest possible limit
+    // (plus one launch for a rounding remainder).
     const double limit = 0.8 * desc.grid.dz / c.vt_max;
-    sp.kessler_sub = c.dt <= limit ? 1 : int(std::ceil(c.dt / limit));
+    sp.kessler_adaptive = c.dt <= limit ? 0 : 1;
+    sp.kessler_sub = c.dt <= limit ? 1 : int(std::ceil(c.dt / limit)) + 1;
     sp.kessler = KesslerConst{c.qc0, c.c_auto, c.c_accr, c.evap_eff, c.vt_max};
     sp.sponge = c.sponge ? 1 : 0;
     sp.sponge_lateral = W;
@@ -474,11 +515,6 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
     b.view.any_odd_substeps = b.view.any_even_kessler_sub = b.view.any_odd_sgs_n = 0;
     b.view.any_sponge_unfused = 0;
     bool remap = true;
-    std::size_t hevi_columns = 0;
-    for (Member* m : ms)
-        if (m->args.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI)
-            hevi_columns += std::size_t(m->args.g.nx) * m->args.g.ny;
-    const bool small_batch = hevi_columns < std::size_t(kSegSmallBatchColumns);
     for (Member* m : ms)
     {
         args.push_back(m->args);
@@ -491,11 +527,10 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
         b.view.any_even_kessler_sub |= (m->args.p.kessler_sub & 1) == 0;
         b.view.any_odd_sgs_n |= m->args.p.sgs_n & 1;
         b.view.any_sponge_unfused |= m->args.p.sponge && m->args.p.sgs_n == 0;
+        b.view.any_kessler_adaptive |= m->args.p.kessler_adaptive;
         if (m->args.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI)
         {
-            const bool part = hevi_partitioned_always(m->elem, m->args.g.nz) ||
-                              (small_batch && m->args.g.nz <= kSegMaxNz);
-            args.back().p.hevi_seg = part ? 1 : 0;
+            const bool part = m->args.p.hevi_seg != 0;
             b.view.any_hevi_seg |= part;
             b.view.any_hevi_serial |= !part;
         }
@@ -990,13 +1025,14 @@ int nimbus_hip_member_export_npy(nimbus_hip_ctx* ctx, uint32_t id, const char* d
 namespace
 {
 constexpr char kSnapMagic[8] = {'N', 'I', 'M', 'B', 'U', 'S', 'M', 'S'};
-constexpr std::uint32_t kSnapVersion = 1;
+constexpr std::uint32_t kSnapVersion = 2;
 
 struct SnapHead
 {
     char magic[8];
     std::uint32_t version, endian, grid_bytes, config_bytes;
     std::int32_t precision, tile_w, k_chunks, xcd_remap;
+    std::int32_t hevi_seg, reserved;
     std::int64_t steps;
     std::uint64_t field_bytes, rain_bytes;
 };
@@ -1067,6 +1103,7 @@ int nimbus_hip_member_snapshot(nimbus_hip_ctx* ctx, uint32_t id, void* buf, uint
     h.tile_w = m->tile_w;
     h.k_chunks = m->k_chunks;
     h.xcd_remap = m->xcd_remap;
+    h.hevi_seg = m->args.p.hevi_seg;
     h.steps = m->steps;
     h.field_bytes = z.field;
     h.rain_bytes = z.rain;
@@ -1126,6 +1163,15 @@ int nimbus_hip_member_restore(nimbus_hip_ctx* ctx, const void* buf, uint64_t len
         return fail(NIMBUS_EINVAL, "restore: field layout differs from the snapshot's");
     }
     set_geometry(*m, h.tile_w, h.k_chunks, h.xcd_remap);
+    if (m->desc.config.acoustic_mode == NIMBUS_ACOUSTIC_HEVI)
+    {
+        const int rc2 = set_column_solve(*m, h.hevi_seg ? NIMBUS_COLUMN_SOLVE_PARTITIONED : NIMBUS_COLUMN_SOLVE_SEQUENTIAL);
+        if (rc2 != NIMBUS_OK)
+        {
+            nimbus_hip_member_destroy(ctx, *id);
+            return rc2;
+        }
+    }
     m->steps = h.steps;
     const std::uint8_t* p = p0 + fixed + base_bytes;
     for (int q = 0; q < 9; ++q, p += z.field)
@@ -1209,6 +1255,28 @@ int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, i
     return NIMBUS_OK;
 }
 
+int nimbus_hip_member_set_column_solve(nimbus_hip_ctx* ctx, uint32_t id, int form)
+{
+    if (!ctx) return fail(NIMBUS_EINVAL, "set_column_solve: null context");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "set_column_solve: unknown member");
+    const int before = m->args.p.hevi_seg;
+    const int rc = set_column_solve(*m, form);
+    if (rc == NIMBUS_OK && m->args.p.hevi_seg != before) drop_batches(ctx);
+    return rc;
+}
+
+int nimbus_hip_member_column_solve(nimbus_hip_ctx* ctx, uint32_t id, int* form)
+{
+    if (!ctx || !form) return fail(NIMBUS_EINVAL, "column_solve: null argument");
+    Member* m = find(ctx, id);
+    if (!m) return fail(NIMBUS_ENOENT, "column_solve: unknown member");
+    *form = m->desc.config.acoustic_mode != NIMBUS_ACOUSTIC_HEVI ? NIMBUS_COLUMN_SOLVE_AUTO
+            : m->args.p.hevi_seg                                  ? NIMBUS_COLUMN_SOLVE_PARTITIONED
+                                                                  : NIMBUS_COLUMN_SOLVE_SEQUENTIAL;
+    return NIMBUS_OK;
+}
+
 int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int* best_tile_w, int* best_k_chunks,
                                int* best_remap, double* best_ms)
 {
@@ -1218,11 +1286,32 @@ int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int
     hipSetDevice(ctx->device);
     void* saved = nullptr;
     NIMBUS_HIP_TRY(hipMalloc(&saved, m->bytes));
-    Staging keep;
-    keep.p = saved;
+    // Every exit restores the member's device block, step count, launch
+    // geometry (the best one found on success) and the context's timing
+    // switch, so a failed trial leaves the member as it was.
+    struct Restore
+    {
+        nimbus_hip_ctx* ctx;
+        Member* m;
+        void* saved;
+        bool copied = false;
+        std::int64_t steps;
+        int w, k, r;
+        bool timing;
+        ~Restore()
+        {
+            hipStreamSynchronize(ctx->stream);
+            if (copied) hipMemcpy(m->block, saved, m->bytes, hipMemcpyDeviceToDevice);
+            hipFree(saved);
+            m->steps = steps;
+            ctx->timing = timing;
+            drop_batches(ctx);
+            set_geometry(*m, w, k, r);
+        }
+    } guard{ctx, m, saved, false, m->steps, m->tile_w, m->k_chunks, m->xcd_remap, ctx->timing};
     NIMBUS_HIP_TRY(hipMemcpyAsync(saved, m->block, m->bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    const std::int64_t steps0 = m->steps;
-    const bool timing = ctx->timing;
+    NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    guard.copied = true;
     ctx->timing = false;
     double best = 1e300;
     int bw = m->tile_w, bk = m->k_chunks, br = m->xcd_remap;
@@ -1240,11 +1329,7 @@ int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int
                 set_geometry(*m, w, c, r);
                 int rc = step_members(ctx, ids, 1, 1, true);  // warm-up (and graph capture)
                 if (rc == NIMBUS_OK) rc = step_members(ctx, ids, 1, trials, true);
-                if (rc != NIMBUS_OK)
-                {
-                    ctx->timing = timing;
-                    return rc;
-                }
+                if (rc != NIMBUS_OK) return rc;
                 const double ms = ctx->last_step_ms / trials;
                 if (ms < best)
                 {
@@ -1255,11 +1340,9 @@ int nimbus_hip_member_autotune(nimbus_hip_ctx* ctx, uint32_t id, int trials, int
                 }
                 NIMBUS_HIP_TRY(hipMemcpyAsync(m->block, saved, m->bytes, hipMemcpyDeviceToDevice, ctx->stream));
             }
-    ctx->timing = timing;
-    NIMBUS_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    m->steps = steps0;
-    drop_batches(ctx);
-    set_geometry(*m, bw, bk, br);
+    guard.w = bw;
+    guard.k = bk;
+    guard.r = br;
     if (best_tile_w) *best_tile_w = bw;
     if (best_k_chunks) *best_k_chunks = bk;
     if (best_remap) *best_remap = br;

@@ -21,6 +21,7 @@ PRECISION_MIXED = 1
 ACOUSTIC_EXPLICIT = 0
 ACOUSTIC_HEVI = 1
 BC_PERIODIC, BC_RADIATIVE, BC_ZERO_GRADIENT, BC_RIGID, BC_RIGID_SLIP = 0, 1, 2, 3, 4
+COLUMN_SOLVE_AUTO, COLUMN_SOLVE_SEQUENTIAL, COLUMN_SOLVE_PARTITIONED = 0, 1, 2
 BOTTOM_FREE_SLIP, BOTTOM_NO_SLIP = 0, 1
 TOP_RIGID, TOP_OPEN = 0, 1
 SGS_3D, SGS_HORIZONTAL, SGS_VERTICAL = 0, 1, 2
@@ -206,6 +207,8 @@ def load_library(path=LIB_PATH):
         "nimbus_hip_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
         "nimbus_hip_member_set_tiling": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
                                                         ctypes.c_int]),
+        "nimbus_hip_member_set_column_solve": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_int]),
+        "nimbus_hip_member_column_solve": (ctypes.c_int, [vp, ctypes.c_uint32, P(ctypes.c_int)]),
         "nimbus_hip_member_autotune": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.c_int, P(ctypes.c_int),
                                                       P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_double)]),
         "nimbus_hip_set_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -363,8 +366,9 @@ def initial_state_from_base(grid, base, params, dtype=np.float32):
 
 NPY_F4, NPY_F8 = 0, 1
 # member snapshot header: magic, version, endian mark, grid and config sizes,
-# precision, tile_w, k_chunks, xcd_remap, steps, field bytes, rain bytes
-SNAP_HEAD = "<8s4I4iq2Q"
+# precision, tile_w, k_chunks, xcd_remap, column-solve form, reserved, steps,
+# field bytes, rain bytes
+SNAP_HEAD = "<8s4I6iq2Q"
 
 
 def npy_write(path, array):
@@ -647,6 +651,17 @@ class Member:
         """Launch geometry: tile_w in {32, 64}, k_chunks in 1..nz, XCD-contiguous order on/off."""
         check(self.ctx.lib.nimbus_hip_member_set_tiling(self.ctx.handle, self.id, int(tile_w), int(k_chunks),
                                                         int(xcd_remap)), "set_tiling")
+
+    def set_column_solve(self, form):
+        """HE-VI column-solve form (COLUMN_SOLVE_*), fixed for the member until changed."""
+        check(self.ctx.lib.nimbus_hip_member_set_column_solve(self.ctx.handle, self.id, int(form)),
+              "set_column_solve")
+
+    def column_solve(self):
+        f = ctypes.c_int()
+        check(self.ctx.lib.nimbus_hip_member_column_solve(self.ctx.handle, self.id, ctypes.byref(f)),
+              "column_solve")
+        return f.value
 
     def autotune(self, trials=2):
         """Times each launch-geometry candidate on a saved copy of the state; returns (tile_w, k_chunks, remap, ms)."""
