@@ -179,35 +179,92 @@ def kernel_table(ctx, m, steps):
     return tab
 
 
-def cpu_baseline(wl, sample_steps=6):
-    """The oracle in the reference's CPU arithmetic (FP32 storage, FP64
-    arithmetic, explicit acoustic substeps) on the host cores."""
-    from oracle import oracle_py as O
-
+def _oracle_threads():
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     threads = max(1, min(threads, 16))
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    return threads
+
+
+def cpu_baseline(wl, sample_steps=10, warmup=2):
+    """The oracle in the reference's CPU arithmetic (FP32 storage, FP64
+    arithmetic, explicit acoustic substeps) on the host cores: `warmup` large
+    steps, then `sample_steps` timed; and one step on one thread."""
+    from oracle import oracle_py as O
+
+    threads = _oracle_threads()
     g = wl.grid
     og = O.grid(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz)
     cfg = O.from_step_config(wl.config())
     base = wl.base()
     st = wl.initial_state(np.float32)
-    st, _ = O.large_steps(og, base, cfg, st, 1, np.float32)  # warm-up step
+    st, _ = O.large_steps(og, base, cfg, st, warmup, np.float32)
     t0 = time.perf_counter()
-    _, n_sub = O.large_steps(og, base, cfg, st, sample_steps, np.float32)
+    st2, n_sub = O.large_steps(og, base, cfg, st, sample_steps, np.float32)
     el = time.perf_counter() - t0
     # the same arithmetic on one thread: one large step of the full grid
     prev = O.set_threads(1)
     t1 = time.perf_counter()
-    O.large_steps(og, base, cfg, st, 1, np.float32)
+    O.large_steps(og, base, cfg, st2, 1, np.float32)
     el1 = time.perf_counter() - t1
     O.set_threads(prev)
     return {"value": wl.cells * sample_steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{sample_steps} large steps of the full {wl.describe()} grid after 1 warm-up step, "
+            "sample": f"{sample_steps} large steps of the full {wl.describe()} grid after {warmup} warm-up steps, "
                       f"reference arithmetic (FP32 storage, FP64 arithmetic, explicit substeps), {n_sub} substeps",
             "seconds": round(el, 3),
+            "one_hour_extrapolated_s": round(el / sample_steps * 3600.0 / wl.dt, 1),
             "single_thread": {"value": wl.cells / el1, "unit": "cell-updates/s", "cores": 1,
                               "sample": "1 large step of the same grid and state", "seconds": round(el1, 3)}}
+
+
+def cpu_baseline_more(wl, W, b_steps=4):
+    """More of the oracle on the host cores: the FP64 B mode (FP64 storage,
+    HE-VI) on the same grid, the 64 x 64 x 40 plain FP64 method over its 10
+    steps, and the 32-member ensemble (each member 1 warm-up + 2 timed steps,
+    the rates combined as total cell updates over total time)."""
+    from oracle import oracle_py as O
+
+    threads = _oracle_threads()
+    out = {}
+    g = wl.grid
+    og = O.grid(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz)
+    import nimbus_amd.lib as L
+    cfg = O.from_step_config(wl.config(acoustic_mode=L.ACOUSTIC_HEVI))
+    st = wl.initial_state(np.float64)
+    st, _ = O.large_steps(og, wl.base(), cfg, st, 1, np.float64)
+    t0 = time.perf_counter()
+    _, n_sub = O.large_steps(og, wl.base(), cfg, st, b_steps, np.float64)
+    el = time.perf_counter() - t0
+    out["b_mode_fp64_hevi"] = {"value": wl.cells * b_steps / el, "unit": "cell-updates/s", "cores": threads,
+                               "kind": "port", "seconds": round(el, 3),
+                               "sample": f"{b_steps} FP64 HE-VI large steps of {wl.describe()} after 1 warm-up "
+                                         f"step, {n_sub} substeps"}
+    c0 = W.NAMED["c0"]
+    g0 = c0.grid
+    cfg0 = O.from_step_config(c0.config())
+    t0 = time.perf_counter()
+    _, n0 = O.large_steps(O.grid(g0.nx, g0.ny, g0.nz, g0.dx, g0.dy, g0.dz), c0.base(), cfg0,
+                          c0.initial_state(np.float64), c0.steps, np.float64)
+    el0 = time.perf_counter() - t0
+    out["c0_plain_fp64"] = {"value": c0.cells * c0.steps / el0, "unit": "cell-updates/s", "cores": threads,
+                            "kind": "port", "seconds": round(el0, 3),
+                            "sample": f"all {c0.steps} steps of {c0.describe()}, FP64, explicit, {n0} substeps"}
+    cells_steps, secs = 0, 0.0
+    for m in W.ensemble_c3():
+        gm = m.grid
+        ogm = O.grid(gm.nx, gm.ny, gm.nz, gm.dx, gm.dy, gm.dz)
+        cm = O.from_step_config(m.config())
+        s, _ = O.large_steps(ogm, m.base(), cm, m.initial_state(np.float32), 1, np.float32)
+        t0 = time.perf_counter()
+        O.large_steps(ogm, m.base(), cm, s, 2, np.float32)
+        secs += time.perf_counter() - t0
+        cells_steps += 2 * m.cells
+    out["ensemble_c3"] = {"value": cells_steps / secs, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                          "seconds": round(secs, 3),
+                          "sample": "each of the 32 members: 1 warm-up + 2 timed steps in the reference "
+                                    "arithmetic (FP32 storage, FP64 arithmetic, explicit substeps), total cell "
+                                    "updates over total time"}
+    return out
 
 
 def measured_traffic_all(workload, mode):
@@ -237,37 +294,56 @@ def roofline_for(tab, cells, elem, workload, mode, n_sub, steps):
             "share_of_step": round(ph["ms_per_execution"] * ph["executions_per_step"] / total_ms, 3)}
 
 
-def ensemble_run(ctx, L, W, precision, mode):
+def column_solve_code(L, name):
+    return {"auto": None, "sequential": L.COLUMN_SOLVE_SEQUENTIAL, "partitioned": L.COLUMN_SOLVE_PARTITIONED}[name]
+
+
+def ensemble_run(ctx, L, W, precision, mode, tuning=None, column_solve=None):
     """The 32-member uneven ensemble as one resident batch, 30 simulated minutes
     per member; members leave the batch as they finish."""
     from nimbus_amd import serving as SV
     reqs = [SV.Request(q, 0.0, wl) for q, wl in enumerate(W.ensemble_c3())]
-    srv = SV.Server(ctx, budget_bytes=250e9, precision=precision, acoustic_mode=mode, quantum=50)
+    before = ctx.batch_stats()
+    srv = SV.Server(ctx, budget_bytes=250e9, precision=precision, acoustic_mode=mode, quantum=50, tuning=tuning,
+                    column_solve=column_solve)
     rep = srv.run(reqs)
     s = rep.summary()
+    after = ctx.batch_stats()
     lat = sorted(r.latency for r in rep.results)
     return {"members": len(reqs), "cells_total": sum(r.workload.cells for r in reqs),
             "cell_updates_per_s": s["cell_updates_per_s"], "wall_seconds": s["wall_seconds"],
             "device_step_seconds": s["device_step_seconds"], "p50_completion_s": s["p50_latency_s"],
             "p99_completion_s": s["p99_latency_s"], "max_completion_s": round(lat[-1], 3),
             "rounds": s["rounds"], "acoustic_mode": "hevi" if mode == L.ACOUSTIC_HEVI else "explicit",
+            "column_solve": {None: "per-member default", L.COLUMN_SOLVE_SEQUENTIAL: "sequential",
+                             L.COLUMN_SOLVE_PARTITIONED: "partitioned"}[column_solve],
+            "autotuned_shapes": tuning.tuned if tuning else 0,
+            "membership_changes": {k: round(after[k] - before[k], 3) for k in after},
             "data": "seed 3: nx = ny in {48..160}, nz in {40, 60, 80}, dx in {500, 1000} m, "
                     "dt = 6 min(dx, dz)/1000 s, 6 substeps, perturbed soundings, 1800 simulated s each"}
 
 
-def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5):
+def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5, tuning=None, column_solve=None,
+                stop_after=None):
     """Continuous batching of seeded Poisson requests (2/s) arriving over `seconds`,
     then `fp64_sample` requests (drawn with seed 5) rerun alone in FP64: the error of each
-    served (mixed) result against its FP64 rerun."""
+    served (mixed) result against its FP64 rerun. `stop_after` (wall seconds) ends
+    the run early and reports the backlog left."""
     import random
     from nimbus_amd import serving as SV
     from nimbus_amd import accuracy as ACC
     reqs = SV.poisson_requests(seed=4, rate=2.0, horizon=float(seconds))
     sample = sorted(random.Random(5).sample([r.rid for r in reqs], min(fp64_sample, len(reqs))))
-    srv = SV.Server(ctx, budget_bytes=200e9, precision=precision, acoustic_mode=mode, quantum=16)
-    rep = srv.run(reqs, keep_fields=sample)
+    before = ctx.batch_stats()
+    srv = SV.Server(ctx, budget_bytes=200e9, precision=precision, acoustic_mode=mode, quantum=16, tuning=tuning,
+                    column_solve=column_solve, progress=lambda m: progress("  serving " + m))
+    rep = srv.run(reqs, keep_fields=sample, stop_after=stop_after)
     s = rep.summary()
+    after = ctx.batch_stats()
     s["arrival_seconds"] = seconds
+    s["stream"] = "full 10-minute stream" if seconds >= 600 else f"reduced stream: the first {seconds:g} s of arrivals"
+    s["requests_generated"] = len(reqs)
+    s["membership_changes"] = {k: round(after[k] - before[k], 3) for k in after}
     s["data"] = ("seed 4 Poisson arrivals at 2/s; nx = ny in {48..256}, nz in {40, 60, 80}, dx 500-1000 m, "
                  "15-90 simulated minutes each; 200 GB HBM budget")
     by_id = {r.rid: r for r in reqs}
@@ -297,6 +373,14 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=20.0,
                     help="arrival window of the serving run (600 = the full 10-minute stream; 0 skips it)")
+    ap.add_argument("--serving-stop-after", type=float, default=None,
+                    help="end the serving run after this many wall seconds and report the backlog")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving run (with --serving-seconds), no single-member or ensemble runs")
+    ap.add_argument("--column-solve", choices=["auto", "sequential", "partitioned"], default="sequential",
+                    help="HE-VI column-solve form of the ensemble and serving members (FP32 storage)")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="ensemble and serving members keep the default launch geometry (no autotuning)")
     ap.add_argument("--accuracy", default="c2,c1",
                     help="workloads of the 1-hour mixed-vs-FP64 error runs, comma-separated; the first is "
                          "reported as accuracy_1h ('' skips them)")
@@ -329,6 +413,20 @@ def main():
     ctx = L.Context(0)
     graphs = not args.no_graphs
     modes = {"hevi": L.ACOUSTIC_HEVI, "explicit": L.ACOUSTIC_EXPLICIT}
+    tuning = None
+    if not args.no_tuning:
+        from nimbus_amd import tuning as T
+        tuning = T.TuningCache(ctx.device_arch())
+    col = column_solve_code(L, args.column_solve)
+
+    if args.serving_only:
+        progress(f"serving: Poisson arrivals over {args.serving_seconds:g} s")
+        out = {"serving_c4": serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI,
+                                         tuning=tuning, column_solve=col, stop_after=args.serving_stop_after,
+                                         fp64_sample=0)}
+        ctx.close()
+        print(json.dumps(out))
+        return
 
     if args.profile:
         prec = L.PRECISION_FP64 if args.profile_precision == "fp64" else L.PRECISION_MIXED
@@ -418,10 +516,11 @@ def main():
     }
     if not args.no_ensemble:
         progress("ensemble: 32 members, 30 simulated minutes each")
-        result["ensemble_c3"] = ensemble_run(ctx, L, W, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
+        result["ensemble_c3"] = ensemble_run(ctx, L, W, L.PRECISION_MIXED, L.ACOUSTIC_HEVI, tuning, col)
     if args.serving_seconds > 0:
         progress(f"serving: Poisson arrivals over {args.serving_seconds:g} s")
-        result["serving_c4"] = serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI)
+        result["serving_c4"] = serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI,
+                                           tuning=tuning, column_solve=col, stop_after=args.serving_stop_after)
     if args.accuracy:
         from nimbus_amd import accuracy as ACC
         for q, name in enumerate(n for n in args.accuracy.split(",") if n):
@@ -434,6 +533,14 @@ def main():
         progress("cpu baseline: the oracle on the host cores")
         result["cpu_baseline"] = cpu_baseline(wl)
         result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
+        progress("cpu baseline: FP64 B mode, the 64x64x40 plain method, the ensemble")
+        result["cpu_baseline_more"] = cpu_baseline_more(wl, W)
+        result["gpu_over_cpu_more"] = {
+            "fp64_hevi": round(runs["fp64_hevi"]["value"] / result["cpu_baseline_more"]["b_mode_fp64_hevi"]["value"],
+                               1)}
+        if "ensemble_c3" in result:
+            result["gpu_over_cpu_more"]["ensemble"] = round(
+                result["ensemble_c3"]["cell_updates_per_s"] / result["cpu_baseline_more"]["ensemble_c3"]["value"], 1)
     progress("done")
     print(json.dumps(result))
 

@@ -178,8 +178,16 @@ const char* nimbus_hip_last_error(void);
 int nimbus_hip_create(int device, nimbus_hip_ctx** out);
 int nimbus_hip_destroy(nimbus_hip_ctx* ctx);
 int nimbus_hip_mem_info(nimbus_hip_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
-/* Counts completed dispatches per kernel family (backend_interface.hpp:46-64 analog). */
+/* The device's architecture name (e.g. "gfx950:sramecc+:xnack-"), NUL terminated in buf[n]. */
+int nimbus_hip_device_arch(nimbus_hip_ctx* ctx, char* buf, size_t n);
+/* Kernel launches made by the context (ComputeBackend::successful_dispatch_count,
+ * backend_interface.hpp:47). */
 uint64_t nimbus_hip_dispatch_count(nimbus_hip_ctx* ctx);
+/* Successful calls of one reference-contract entry point, named by the part
+ * after "nimbus_hip_dispatch_" (e.g. "kessler_pointwise"); the per-operation
+ * count behind ComputeBackend::successful_kernel_dispatch_count(KernelId)
+ * (backend_interface.hpp:60). */
+uint64_t nimbus_hip_successful_dispatch_count(nimbus_hip_ctx* ctx, const char* op);
 
 /* ---------------- initial conditions (host only, no device needed) ---------------- */
 void nimbus_default_sounding(nimbus_sounding* s);
@@ -357,6 +365,13 @@ int nimbus_hip_set_kernel_timing(nimbus_hip_ctx* ctx, int enable);
 int nimbus_hip_kernel_timing(nimbus_hip_ctx* ctx, int max_rows, char (*names)[48], uint64_t* launches,
                              double* total_ms);
 int nimbus_hip_reset_kernel_timing(nimbus_hip_ctx* ctx);
+/* Cost of membership changes so far: batch member-table builds (device
+ * allocation and upload of the table and block maps) and whole-step graph
+ * captures (capture and instantiation), counts and host milliseconds. A
+ * batch is built and captured the first time a member set steps together;
+ * the 8 most recent sets stay cached. */
+int nimbus_hip_batch_stats(nimbus_hip_ctx* ctx, uint64_t* builds, double* build_ms, uint64_t* captures,
+                           double* capture_ms);
 /* Elapsed device time (ms) of the last nimbus_hip_step call measured with HIP events. */
 double nimbus_hip_last_step_ms(nimbus_hip_ctx* ctx);
 
@@ -454,6 +469,176 @@ typedef struct
     double qc0, c_auto, c_accr, evap_eff, vt_max;
 } nimbus_kessler_dispatch;
 int nimbus_hip_dispatch_kessler(nimbus_hip_ctx* ctx, const nimbus_kessler_dispatch* d);
+
+/* ---------------- reference dispatch contracts as they are ----------------
+ * The structs below carry the field lists of the reference's device dispatch
+ * structs (include/compute/dispatch/{dynamics,microphysics,advection}.hpp)
+ * with C types (bool -> int), so an adapter implementing the reference's
+ * ComputeBackend families copies fields one for one. Arrays are caller-owned
+ * host float arrays in the reference layout; outputs are written back only
+ * when the whole call succeeded. Arithmetic: FP64 on FP32 storage (the
+ * reference CPU backend's precision). */
+
+/* Replaces DynamicsComputeKernels::dispatch_cartesian_cgrid_slow_tendencies
+ * (include/compute/dispatch/dynamics.hpp:374-420,780): UP5 momentum advection
+ * with 4-point advecting velocities, buoyancy -g (rho_f - rho0_f) / rho_f,
+ * moisture buoyancy 0.608 g (qv_f - qv0_f) when has_moisture, hydrometeor
+ * loading -g loading_f (host pre-summed) when has_loading, f-plane Coriolis on
+ * the perturbation wind, dp/dt = -u.grad p (centred) and drho/dt = 0; every
+ * output zero outside the reference's update ranges. Frame velocities
+ * subtract from the horizontal advecting velocities. Refused: coriolis_beta
+ * != 0 (beta plane). rho0/qv0/u0/v0 profiles are nz long (FP32 as the
+ * reference stores them). */
+typedef struct
+{
+    const float *u_data, *v_data, *w_data, *rho_data, *p_data;
+    const float* rho0_base_data;
+    int rho0_base_len;
+    const float* qv0_base_data;
+    int qv0_base_len;
+    const float *u0_base_data, *v0_base_data;
+    int wind0_base_len;
+    const float *qv_data, *loading_data;
+    float *du_dt_data, *dv_dt_data, *dw_dt_data, *drho_dt_data, *dp_dt_data;
+    int nr, nth, nz;
+    float dx, dy, dz, g, coriolis_f_val;
+    int has_moisture, has_loading, slow_rk2_invariant_input_residency;
+    float *du_advection_dt_data, *dv_advection_dt_data, *dw_advection_dt_data; /* optional, all or none */
+    float frame_velocity_u_ms, frame_velocity_v_ms;
+    float coriolis_beta;
+} nimbus_cartesian_cgrid_slow_tendencies_dispatch;
+int nimbus_hip_dispatch_cartesian_cgrid_slow_tendencies(nimbus_hip_ctx* ctx,
+                                                         const nimbus_cartesian_cgrid_slow_tendencies_dispatch* d);
+
+/* Observer called after the device finished and before the host arrays are
+ * written, with host copies of the committed fields (not to be retained);
+ * include/compute/dispatch/common.hpp:22-27. */
+typedef void (*nimbus_four_field_observer)(void* context, const float* field0, const float* field1,
+                                           const float* field2, const float* field3);
+
+/* Replaces DynamicsComputeKernels::dispatch_slow_apply (dynamics.hpp:328-351,760;
+ * the CPU callback dynamics.cpp:1955-2095): u += dt (du + du_pbl), v += dt (dv +
+ * dv_pbl), w += dt dw, p += dt dp; with sanitize a non-finite sum counts as
+ * zero. Null pbl arrays count as zero. */
+typedef struct
+{
+    float *u, *v, *w, *p_data;
+    const float *du_dt, *dv_dt, *dw_dt, *dp_dt;
+    const float *du_dt_pbl, *dv_dt_pbl;
+    int nx, ny, nz;
+    float dt_large;
+    nimbus_four_field_observer completed_observer;
+    void* completed_observer_context;
+    int sanitize;
+} nimbus_slow_apply_dispatch;
+int nimbus_hip_dispatch_slow_apply(nimbus_hip_ctx* ctx, const nimbus_slow_apply_dispatch* d);
+
+/* Replaces DynamicsComputeKernels::dispatch_state_relaxation_cartesian
+ * (dynamics.hpp:354-372,747): upper Rayleigh sponge, additive w-only lid
+ * absorber and lateral band on u, v, w, theta, each x += (1 - exp(-alpha dt))
+ * (x_ref - x). profiles: 7 rows of nz floats (upper alpha, w-only alpha,
+ * u_ref, v_ref, theta0, lateral alpha by distance to the nearest lateral face,
+ * upper alpha at w faces); lateral_width <= nz. */
+typedef struct
+{
+    float *u, *v, *w, *theta_data;
+    const float* profiles;
+    int profiles_len;
+    int nx, ny, nz;
+    int lateral_width;
+    int do_upper, do_w_only, do_lateral;
+    float dt_step;
+    nimbus_four_field_observer completed_observer;
+    void* completed_observer_context;
+} nimbus_state_relaxation_dispatch;
+int nimbus_hip_dispatch_state_relaxation_cartesian(nimbus_hip_ctx* ctx, const nimbus_state_relaxation_dispatch* d);
+
+/* Replaces MicrophysicsComputeKernels::dispatch_microphysics_apply
+ * (include/compute/dispatch/microphysics.hpp:326-363,688): on columns
+ * i in [i_begin, i_end), j in [j_begin, j_end): theta += clamp((dtheta + rad +
+ * pbl) dt, +-max_theta_step), bounded to [theta_min, theta_max] (fallback for
+ * non-finite values); species += tendency dt clamped to [0, qv_max_val] /
+ * [0, hydro_max_val] (0 when non-finite). Optional stats_out (stats_len >= 10,
+ * and 10 + nz + nx for the histograms): counts of non-finite tendencies,
+ * non-finite totals, limited steps, clamps, and the first clamp's record. The
+ * ice and graupel/hail species and the rad/pbl terms may be null. */
+typedef struct
+{
+    float *theta_data, *qv_data, *qc_data, *qr_data, *qi_data, *qs_data, *qg_data, *qh_data;
+    const float *dtheta, *dqv, *dqc, *dqr, *dqi, *dqs, *dqg, *dqh;
+    const float *dtheta_rad, *dtheta_pbl, *dqv_pbl;
+    int nx, ny, nz;
+    int i_begin, i_end, j_begin, j_end;
+    float dt_step, theta_fallback, theta_min, theta_max, max_theta_step, qv_max_val, hydro_max_val;
+    unsigned int* stats_out;
+    int stats_len;
+} nimbus_microphysics_apply_dispatch;
+int nimbus_hip_dispatch_microphysics_apply(nimbus_hip_ctx* ctx, const nimbus_microphysics_apply_dispatch* d);
+
+/* Replaces MicrophysicsComputeKernels::dispatch_kessler_pointwise
+ * (microphysics.hpp:436-467,726; caller kessler.cpp:213-230): warm rain on the
+ * host-adjusted temperature, qv and qc (the caller books the saturation
+ * adjustment itself): autoconversion + accretion, ventilated evaporation
+ * bounded by the saturation deficit; dtheta_dt receives the temperature
+ * tendency (the caller converts it to theta). Writes every output: zero
+ * outside the index ranges and for the qg/qh slots (which may be null). */
+typedef struct
+{
+    const float *temperature_data, *p_data, *qv_data, *qc_data, *qr_data, *rho_d_data, *qh_data;
+    float *dtheta_dt_data, *dqv_dt_data, *dqc_dt_data, *dqr_dt_data, *dqg_dt_data, *dqh_dt_data;
+    int nr, nth, nz;
+    int i_begin, i_end, j_begin, j_end, k_begin, k_end;
+    float qc0, c_auto, c_accr, evaporation_efficiency, Lv_cp, T0, dt_step;
+} nimbus_kessler_pointwise_dispatch;
+int nimbus_hip_dispatch_kessler_pointwise(nimbus_hip_ctx* ctx, const nimbus_kessler_pointwise_dispatch* d);
+
+/* Replaces MicrophysicsComputeKernels::dispatch_kessler_sedimentation
+ * (microphysics.hpp:469-492,752; kessler.cpp:465-531): adds the rain
+ * sedimentation divergence to dqr_dt, column by column in k order: the donor
+ * loses Vt qr / dz_k, the level below gains (rho_k / rho_{k-1}) Vt qr / dz_{k-1},
+ * Vt = min(36.34 (1e-3 rho qr)^0.1346 sqrt(1.2 / rho), Vt_max_rain, vt_cap).
+ * layer_thickness_m (optional) replaces dz_val per cell; rho_d_data may be
+ * null (rho = 1.2). The qg/qh slots are inert. */
+typedef struct
+{
+    const float *qr_data, *qg_data, *qh_data, *rho_d_data;
+    float *dqr_dt_data, *dqg_dt_data, *dqh_dt_data;
+    int nr, nth, nz;
+    int i_begin, i_end, j_begin, j_end, k_begin, k_end;
+    float dz_val, Vt_max_rain, vt_cap, dt_step;
+    const float* layer_thickness_m;
+} nimbus_kessler_sedimentation_dispatch;
+int nimbus_hip_dispatch_kessler_sedimentation(nimbus_hip_ctx* ctx, const nimbus_kessler_sedimentation_dispatch* d);
+
+/* Replaces AdvectionComputeKernels::dispatch_advect_scalars_conservative_cartesian_cgrid
+ * (include/compute/dispatch/advection.hpp:46-103,226-231;
+ * advection_cartesian.cpp:1652-2749): WENO5 flux-form transport of n_scalars
+ * (<= 16) with the positive-definite donor limiter. With
+ * use_supplied_mass_flux the retained time-averaged fluxes and density_begin
+ * are used and rho_data is the end density; without, the fluxes are formed
+ * from u, v, w (frame-relative) and rho_data, which is then the begin
+ * density, and the end density carries the same fluxes. q_out[s] may equal
+ * q_in[s]. Refused: weno == 0 (the TVD reconstruction is not built) and
+ * nonzero dt_kappas (chained point diffusion, as the reference's device
+ * backends refuse it). The scheme's own fields inv_dx2.. are unused. */
+typedef struct
+{
+    const float *u, *v, *w, *rho_data;
+    const float* const* q_in;
+    float* const* q_out;
+    const float* dt_kappas;
+    const int* positive_definite;
+    int n_scalars;
+    int nx, ny, nz;
+    float inv_dx, inv_dy, inv_dz, inv_dx2, inv_dy2, inv_dz2;
+    float dt_step, rho_floor;
+    const float *density_begin, *mass_flux_x, *mass_flux_y, *mass_flux_z;
+    int use_supplied_mass_flux;
+    float frame_velocity_u_ms, frame_velocity_v_ms;
+    int weno, preserve_positive_support;
+} nimbus_conservative_cartesian_cgrid_transport_dispatch;
+int nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid(
+    nimbus_hip_ctx* ctx, const nimbus_conservative_cartesian_cgrid_transport_dispatch* d);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnimbus_hip.so")
-SOURCES = ["kernels.hip", "runtime.hip", "initial_conditions.cpp", "sounding.cpp", "npy.cpp"]
+SOURCES = ["kernels.hip", "runtime.hip", "reference_dispatch.hip", "initial_conditions.cpp", "sounding.cpp", "npy.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

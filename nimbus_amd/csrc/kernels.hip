@@ -1099,10 +1099,10 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column_seg(const MemberArgs* __
 
     // 3. the system of the runs' last unknowns L_p, one row per run thread:
     // a1 L_{p-1} + (1 - c1 at') L_p - c1 ct' L_{p+1} = d1 - c1 dt' (primes:
-    // run p + 1's first row), solved by cyclic reduction over the 16 run
-    // threads of the column in LDS (4 steps of stride 1, 2, 4, 8, double
-    // buffered so one barrier per step suffices). Runs past the column and
-    // non-interior columns carry identity rows.
+    // run p + 1's first row), solved by cyclic reduction over the kSegments
+    // run threads of the column in LDS (log2(kSegments) steps of stride 1, 2,
+    // 4, ..., double buffered so one barrier per step suffices). Runs past the
+    // column and non-interior columns carry identity rows.
     double* const nb = ex;                 // [3][kBlock]: at0, ct0, dt0 of each run
     double* const rbuf = ex + 3 * kBlock;  // [2][4][kBlock]: reduced rows (sub, dia, sup, rhs)
     nb[0 * kBlock + tid] = at0;
@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(kBlock) void k_hevi_column_seg(const MemberArgs* __
     bool col_ok = true;
     for (int q = 0; q < runs; ++q) col_ok = col_ok && okv[q * kSegCols + col] != 0;
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
+    for (int st = 0; st < kSegSteps; ++st)
     {
         const int stride = 1 << st;
         double* const b = rbuf + (st & 1) * 4 * kBlock;

@@ -132,9 +132,14 @@ enum WorkKind
 // 9.06 against 8.04). Deeper columns take the sequential sweep, which keeps
 // its FP64 coefficients in two cell-sized scratch arrays (allocated for every
 // member that may be solved by it: FP32 storage or more than kSegMaxNz levels).
-constexpr int kSegments = 16;
-constexpr int kSegCols = 16;
-constexpr int kSegRows = 8;
+#ifndef NIMBUS_SEG_ROWS
+#define NIMBUS_SEG_ROWS 8
+#endif
+constexpr int kSegRows = NIMBUS_SEG_ROWS;        // faces per run (one thread)
+constexpr int kSegments = 128 / kSegRows;        // runs per column: columns of up to 129 levels
+constexpr int kSegCols = 256 / kSegments;        // columns per 256-thread workgroup
+constexpr int kSegSteps = kSegments == 64 ? 6 : kSegments == 32 ? 5 : kSegments == 16 ? 4 : kSegments == 8 ? 3 : 0;
+static_assert(kSegSteps > 0 && (1 << kSegSteps) == kSegments, "runs per column must be 8, 16, 32 or 64");
 constexpr int kSegMaxNz = kSegments * kSegRows + 1;
 constexpr int kSegSmallMemberColumns = 32768;
 constexpr bool hevi_partitioned_always(int elem, int nz) { return elem == 8 && nz <= kSegMaxNz; }

@@ -37,6 +37,10 @@ int set_error(int code, const std::string& msg);
 int write_npy(const char* path, const void* data, int dtype, int ndim, const std::int64_t* shape);
 double ic_uniform(std::uint64_t seed, std::uint32_t member, int i, int j, int k);
 int resolve_substeps(const nimbus_grid& g, const double* rho0, const double* p0, const nimbus_step_config& c);
+// Device, launch stream and dispatch counter of a context (runtime.hip).
+int context_device(nimbus_hip_ctx* ctx);
+hipStream_t context_stream(nimbus_hip_ctx* ctx);
+void count_dispatch(nimbus_hip_ctx* ctx, const char* op);
 
 // Device grid descriptor. Device fields are stored [k][j][i] with i
 // contiguous and the row pitch px (elements) padded to 64 bytes.

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <limits>
 #include <map>
 #include <memory>
@@ -187,6 +188,10 @@ struct Batch
 
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 
+int context_device(nimbus_hip_ctx* ctx);
+hipStream_t context_stream(nimbus_hip_ctx* ctx);
+void count_dispatch(nimbus_hip_ctx* ctx, const char* op);
+
 }  // namespace nimbus
 
 struct nimbus_hip_ctx
@@ -210,7 +215,19 @@ struct nimbus_hip_ctx
     double last_step_ms = 0.0;
     std::map<std::pair<int, std::vector<std::uint32_t>>, std::unique_ptr<nimbus::Batch>> batches;
     std::uint64_t batch_clock = 0;
+    std::map<std::string, std::uint64_t> op_counts;  // successful per-operation dispatches by entry point
+    // membership-change costs: batch table builds and whole-step graph
+    // captures (capture + instantiate), host wall clock
+    std::uint64_t batch_builds = 0, graph_captures = 0;
+    double batch_build_ms = 0.0, graph_capture_ms = 0.0;
 };
+
+namespace nimbus
+{
+int context_device(nimbus_hip_ctx* ctx) { return ctx->device; }
+hipStream_t context_stream(nimbus_hip_ctx* ctx) { return ctx->stream; }
+void count_dispatch(nimbus_hip_ctx* ctx, const char* op) { ++ctx->op_counts[op]; }
+}  // namespace nimbus
 
 namespace nimbus
 {
@@ -593,8 +610,11 @@ int get_batch(nimbus_hip_ctx* ctx, const std::vector<Member*>& ms, const std::ve
         b->elem = elem;
         std::vector<Member*> sorted;
         for (std::uint32_t id : key) sorted.push_back(find(ctx, id));
+        const auto t0 = std::chrono::steady_clock::now();
         const int rc = build_batch(*b, sorted);
         if (rc != NIMBUS_OK) return rc;
+        ctx->batch_builds += 1;
+        ctx->batch_build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         it = ctx->batches.emplace(k, std::move(b)).first;
     }
     it->second->last_use = ++ctx->batch_clock;
@@ -614,6 +634,7 @@ int run_batch_step(nimbus_hip_ctx* ctx, Batch& b)
     {
         if (!b.exec)
         {
+            const auto t0 = std::chrono::steady_clock::now();
             Launcher L = make_launcher(ctx);
             L.timing = false;
             NIMBUS_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -624,6 +645,9 @@ int run_batch_step(nimbus_hip_ctx* ctx, Batch& b)
             if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
             b.graph = graph;
             NIMBUS_HIP_TRY(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+            ctx->graph_captures += 1;
+            ctx->graph_capture_ms +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
         NIMBUS_HIP_TRY(hipGraphLaunch(b.exec, ctx->stream));
     }
@@ -867,6 +891,33 @@ int nimbus_hip_mem_info(nimbus_hip_ctx* ctx, uint64_t* free_bytes, uint64_t* tot
 }
 
 uint64_t nimbus_hip_dispatch_count(nimbus_hip_ctx* ctx) { return ctx ? ctx->launches : 0; }
+
+int nimbus_hip_device_arch(nimbus_hip_ctx* ctx, char* buf, size_t n)
+{
+    if (!ctx || !buf || n == 0) return fail(NIMBUS_EINVAL, "device_arch: bad arguments");
+    hipDeviceProp_t prop{};
+    NIMBUS_HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
+    std::snprintf(buf, n, "%s", prop.gcnArchName);
+    return NIMBUS_OK;
+}
+
+int nimbus_hip_batch_stats(nimbus_hip_ctx* ctx, uint64_t* builds, double* build_ms, uint64_t* captures,
+                           double* capture_ms)
+{
+    if (!ctx) return fail(NIMBUS_EINVAL, "batch_stats: null context");
+    if (builds) *builds = ctx->batch_builds;
+    if (build_ms) *build_ms = ctx->batch_build_ms;
+    if (captures) *captures = ctx->graph_captures;
+    if (capture_ms) *capture_ms = ctx->graph_capture_ms;
+    return NIMBUS_OK;
+}
+
+uint64_t nimbus_hip_successful_dispatch_count(nimbus_hip_ctx* ctx, const char* op)
+{
+    if (!ctx || !op) return 0;
+    auto it = ctx->op_counts.find(op);
+    return it == ctx->op_counts.end() ? 0 : it->second;
+}
 
 uint64_t nimbus_hip_member_bytes(const nimbus_member_desc* desc)
 {

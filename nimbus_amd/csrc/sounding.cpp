@@ -49,7 +49,7 @@ struct Bad : std::runtime_error
 
 [[noreturn]] void bad(const std::string& what) { throw Bad("CM1 input_sounding: " + what); }
 
-std::string at_line(std::size_t line) { return " at line " + std::to_string(line); }
+std::string at_line(std::size_t line) { return " (line " + std::to_string(line) + ")"; }
 
 // One substantive (non-blank after comment removal) line of the file.
 struct Line
@@ -75,7 +75,7 @@ std::vector<Line> substantive(const char* text, std::size_t len)
         if (a == std::string::npos) continue;
         const std::size_t b = line.find_last_not_of(" \t\r\n");
         out.push_back({number, line.substr(a, b - a + 1)});
-        if (out.size() > std::size_t(NIMBUS_CM1_MAX_LEVELS) + 1) bad("exceeds the 100,000-level safety limit");
+        if (out.size() > std::size_t(NIMBUS_CM1_MAX_LEVELS) + 1) bad("[level-limit] more than 100000 levels");
     }
     return out;
 }
@@ -94,8 +94,8 @@ std::vector<double> numbers(const Line& ln, std::size_t want)
         p = q == std::string::npos ? q : ln.text.find_first_not_of(ws, q);
     }
     if (tok.size() != want)
-        bad("line " + std::to_string(ln.number) + " has " + std::to_string(tok.size()) + " columns; expected " +
-            std::to_string(want));
+        bad("[columns] line " + std::to_string(ln.number) + ": " + std::to_string(tok.size()) + " values where " +
+            std::to_string(want) + " belong");
     std::vector<double> v;
     for (std::size_t c = 0; c < tok.size(); ++c)
     {
@@ -108,10 +108,10 @@ std::vector<double> numbers(const Line& ln, std::size_t want)
         errno = 0;
         char* end = nullptr;
         const double x = std::strtod(t.c_str(), &end);
-        const std::string where = "line " + std::to_string(ln.number) + ", column " + std::to_string(c + 1);
-        if (end == t.c_str() || errno == ERANGE) bad(where + " is not numeric: '" + tok[c] + "'");
+        const std::string where = " line " + std::to_string(ln.number) + ", value " + std::to_string(c + 1);
+        if (end == t.c_str() || errno == ERANGE) bad("[not-numeric]" + where + ": cannot read '" + tok[c] + "'");
         if (std::size_t(end - t.c_str()) != t.size() || !std::isfinite(x))
-            bad(where + " is not a finite number: '" + tok[c] + "'");
+            bad("[not-finite]" + where + ": '" + tok[c] + "' is not a finite value");
         v.push_back(x);
     }
     return v;
@@ -135,13 +135,13 @@ struct Column
 Column parse(const char* text, std::size_t len, double hdr[3])
 {
     const std::vector<Line> lines = substantive(text, len);
-    need(!lines.empty(), "contains no numeric rows");
-    need(lines.size() >= 6, "requires at least five vertical levels after the surface header");
+    need(!lines.empty(), "[empty] no data rows");
+    need(lines.size() >= 6, "[too-few-levels] fewer than 5 levels follow the header");
     const std::vector<double> h = numbers(lines[0], 3);
     const double ps_hpa = h[0], ths = h[1], qvs_gkg = h[2];
-    need(ps_hpa >= 100.0 && ps_hpa <= 1100.0, "surface pressure must be in [100, 1100] hPa");
-    need(ths >= 150.0 && ths <= 700.0, "surface potential temperature must be in [150, 700] K");
-    need(qvs_gkg >= 0.0 && qvs_gkg <= 100.0, "surface water-vapor mixing ratio must be in [0, 100] g/kg");
+    need(ps_hpa >= 100.0 && ps_hpa <= 1100.0, "[header-p] header pressure outside 100..1100 hPa");
+    need(ths >= 150.0 && ths <= 700.0, "[header-theta-range] header theta outside 150..700 K");
+    need(qvs_gkg >= 0.0 && qvs_gkg <= 100.0, "[header-qv-range] header qv outside 0..100 g/kg");
     hdr[0] = ps_hpa * 100.0;
     hdr[1] = ths;
     hdr[2] = qvs_gkg / 1000.0;
@@ -158,29 +158,29 @@ Column parse(const char* text, std::size_t len, double hdr[3])
         const std::vector<double> row = numbers(lines[r], 5);
         const std::size_t ln = lines[r].number;
         const double z = row[0], th = row[1], q_gkg = row[2], u = row[3], v = row[4];
-        need(z >= 0.0, "height cannot be negative" + at_line(ln));
+        need(z >= 0.0, "[z-negative] negative height" + at_line(ln));
         if (r == 1)
-            need(std::abs(z) <= 1.0e-9, "first profile level must be at z=0 m");
+            need(std::abs(z) <= 1.0e-9, "[first-z0] the first level is not at z = 0");
         else
-            need(z > c.z.back(), "heights must increase strictly" + at_line(ln));
-        need(th >= 150.0 && th <= 700.0, "potential temperature must be in [150, 700] K" + at_line(ln));
-        need(q_gkg >= 0.0 && q_gkg <= 100.0, "water-vapor mixing ratio must be in [0, 100] g/kg" + at_line(ln));
-        need(std::abs(u) <= 300.0 && std::abs(v) <= 300.0, "wind component magnitude exceeds 300 m/s" + at_line(ln));
+            need(z > c.z.back(), "[z-order] height does not increase" + at_line(ln));
+        need(th >= 150.0 && th <= 700.0, "[theta-range] theta outside 150..700 K" + at_line(ln));
+        need(q_gkg >= 0.0 && q_gkg <= 100.0, "[qv-range] qv outside 0..100 g/kg" + at_line(ln));
+        need(std::abs(u) <= 300.0 && std::abs(v) <= 300.0, "[wind-limit] a wind component beyond 300 m/s" + at_line(ln));
         if (r == 1)
         {
             // header values may be printed with fewer digits than the z = 0 row
-            need(std::abs(th - ths) <= 1.0e-3, "surface header theta does not match the z=0 row");
-            need(std::abs(q_gkg - qvs_gkg) <= 1.0e-3, "surface header qv does not match the z=0 row");
+            need(std::abs(th - ths) <= 1.0e-3, "[header-theta] header theta differs from the z = 0 level");
+            need(std::abs(q_gkg - qvs_gkg) <= 1.0e-3, "[header-qv] header qv differs from the z = 0 level");
         }
         const double q = q_gkg / 1000.0;
         const double thv = th * (1.0 + kVirtualCoef * q);
         const double dz = z - z_prev;
         exner -= (kG / kCp) * dz * 0.5 * ((1.0 / thv_prev) + (1.0 / thv));
-        need(std::isfinite(exner) && exner > 0.0, "hydrostatic Exner pressure became invalid" + at_line(ln));
+        need(std::isfinite(exner) && exner > 0.0, "[exner] Exner integration left the valid range" + at_line(ln));
         const double p = kP00 * std::pow(exner, 1.0 / kappa);
         const double T = th * exner;
         need(std::isfinite(p) && p > 0.0 && std::isfinite(T) && T > 0.0,
-             "hydrostatic thermodynamic reconstruction failed" + at_line(ln));
+             "[thermo] pressure and temperature reconstruction failed" + at_line(ln));
         c.z.push_back(z);
         c.p_pa.push_back(p);
         c.p_hpa.push_back(p / 100.0);
@@ -207,10 +207,11 @@ Column parse(const char* text, std::size_t len, double hdr[3])
         if (k > 0) ok = ok && c.z[k] > c.z[k - 1] && c.p_hpa[k] < c.p_hpa[k - 1];
         good += ok;
     }
-    if (good < 5) bad("failed quality control (" + std::to_string(good) + " valid levels)");
+    if (good < 5) bad("[quality] only " + std::to_string(good) + " levels pass quality control");
     if (good != c.z.size())
-        bad("quality control rejected one or more levels; refusing a silently shortened profile (" +
-            std::to_string(c.z.size() - good) + " outside 200-350 K, 1-1100 hPa or non-monotone)");
+        bad("[qc-shortened] quality control would drop " + std::to_string(c.z.size() - good) +
+            " levels (theta outside 200-350 K, pressure outside 1-1100 hPa or not monotone); the profile is not "
+            "shortened silently");
     return c;
 }
 
@@ -431,9 +432,9 @@ void column(const Column& s, const nimbus_interp& how, const std::vector<double>
     const double top = zt[nz - 1] + half;
     const double tol = 1.0e-10 * std::max({1.0, std::abs(top), std::abs(s.z.back())});
     if (top > s.z.back() + tol && !how.extrapolate_above)
-        bad("model top interface (" + std::to_string(top) + " m; highest scalar center " + std::to_string(zt[nz - 1]) +
-            " m) exceeds sounding top (" + std::to_string(s.z.back()) +
-            " m); refusing to extrapolate an uncovered upper half-cell");
+        bad("[top-coverage] the model's top interface at " + std::to_string(top) + " m (highest centre " +
+            std::to_string(zt[nz - 1]) + " m) lies above the sounding's top at " + std::to_string(s.z.back()) +
+            " m, and extrapolation above is off");
     const Column r = interpolate(s, zt, how);
     const double kappa = kRd / kCp;
     // theta stays exact; d(pi)/dz = -g / (cp theta) by the trapezoid rule from

@@ -151,6 +151,91 @@ class KesslerDispatch(ctypes.Structure):
                 [(n, ctypes.c_double) for n in ("dz", "dt", "qc0", "c_auto", "c_accr", "evap_eff", "vt_max")])
 
 
+# ---- reference dispatch contracts as they are (field lists of
+# include/compute/dispatch/{dynamics,microphysics,advection}.hpp) ----
+_f = ctypes.c_float
+_i = ctypes.c_int
+FourFieldObserver = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _c_float_p, _c_float_p, _c_float_p, _c_float_p)
+
+
+class CgridSlowTendenciesDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("u_data", "v_data", "w_data", "rho_data", "p_data", "rho0_base_data")] +
+                [("rho0_base_len", _i), ("qv0_base_data", _c_float_p), ("qv0_base_len", _i),
+                 ("u0_base_data", _c_float_p), ("v0_base_data", _c_float_p), ("wind0_base_len", _i),
+                 ("qv_data", _c_float_p), ("loading_data", _c_float_p)] +
+                [(n, _c_float_p) for n in ("du_dt_data", "dv_dt_data", "dw_dt_data", "drho_dt_data", "dp_dt_data")] +
+                [("nr", _i), ("nth", _i), ("nz", _i)] +
+                [(n, _f) for n in ("dx", "dy", "dz", "g", "coriolis_f_val")] +
+                [(n, _i) for n in ("has_moisture", "has_loading", "slow_rk2_invariant_input_residency")] +
+                [(n, _c_float_p) for n in ("du_advection_dt_data", "dv_advection_dt_data", "dw_advection_dt_data")] +
+                [(n, _f) for n in ("frame_velocity_u_ms", "frame_velocity_v_ms", "coriolis_beta")])
+
+
+class SlowApplyDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("u", "v", "w", "p_data", "du_dt", "dv_dt", "dw_dt", "dp_dt",
+                                          "du_dt_pbl", "dv_dt_pbl")] +
+                [("nx", _i), ("ny", _i), ("nz", _i), ("dt_large", _f), ("completed_observer", FourFieldObserver),
+                 ("completed_observer_context", ctypes.c_void_p), ("sanitize", _i)])
+
+
+class StateRelaxationDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("u", "v", "w", "theta_data", "profiles")] +
+                [(n, _i) for n in ("profiles_len", "nx", "ny", "nz", "lateral_width", "do_upper", "do_w_only",
+                                   "do_lateral")] +
+                [("dt_step", _f), ("completed_observer", FourFieldObserver),
+                 ("completed_observer_context", ctypes.c_void_p)])
+
+
+MICRO_SPECIES = ("theta_data", "qv_data", "qc_data", "qr_data", "qi_data", "qs_data", "qg_data", "qh_data")
+MICRO_TENDENCIES = ("dtheta", "dqv", "dqc", "dqr", "dqi", "dqs", "dqg", "dqh")
+
+
+class MicrophysicsApplyDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in MICRO_SPECIES + MICRO_TENDENCIES + ("dtheta_rad", "dtheta_pbl", "dqv_pbl")] +
+                [(n, _i) for n in ("nx", "ny", "nz", "i_begin", "i_end", "j_begin", "j_end")] +
+                [(n, _f) for n in ("dt_step", "theta_fallback", "theta_min", "theta_max", "max_theta_step",
+                                   "qv_max_val", "hydro_max_val")] +
+                [("stats_out", ctypes.POINTER(ctypes.c_uint)), ("stats_len", _i)])
+
+
+class KesslerPointwiseDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("temperature_data", "p_data", "qv_data", "qc_data", "qr_data",
+                                          "rho_d_data", "qh_data", "dtheta_dt_data", "dqv_dt_data", "dqc_dt_data",
+                                          "dqr_dt_data", "dqg_dt_data", "dqh_dt_data")] +
+                [(n, _i) for n in ("nr", "nth", "nz", "i_begin", "i_end", "j_begin", "j_end", "k_begin", "k_end")] +
+                [(n, _f) for n in ("qc0", "c_auto", "c_accr", "evaporation_efficiency", "Lv_cp", "T0", "dt_step")])
+
+
+class KesslerSedimentationDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("qr_data", "qg_data", "qh_data", "rho_d_data", "dqr_dt_data",
+                                          "dqg_dt_data", "dqh_dt_data")] +
+                [(n, _i) for n in ("nr", "nth", "nz", "i_begin", "i_end", "j_begin", "j_end", "k_begin", "k_end")] +
+                [(n, _f) for n in ("dz_val", "Vt_max_rain", "vt_cap", "dt_step")] +
+                [("layer_thickness_m", _c_float_p)])
+
+
+class ConservativeTransportDispatch(ctypes.Structure):
+    _fields_ = ([(n, _c_float_p) for n in ("u", "v", "w", "rho_data")] +
+                [("q_in", ctypes.POINTER(_c_float_p)), ("q_out", ctypes.POINTER(_c_float_p)),
+                 ("dt_kappas", _c_float_p), ("positive_definite", ctypes.POINTER(_i)), ("n_scalars", _i),
+                 ("nx", _i), ("ny", _i), ("nz", _i)] +
+                [(n, _f) for n in ("inv_dx", "inv_dy", "inv_dz", "inv_dx2", "inv_dy2", "inv_dz2", "dt_step",
+                                   "rho_floor")] +
+                [(n, _c_float_p) for n in ("density_begin", "mass_flux_x", "mass_flux_y", "mass_flux_z")] +
+                [("use_supplied_mass_flux", _i), ("frame_velocity_u_ms", _f), ("frame_velocity_v_ms", _f),
+                 ("weno", _i), ("preserve_positive_support", _i)])
+
+
+REFERENCE_DISPATCH = {
+    "nimbus_hip_dispatch_cartesian_cgrid_slow_tendencies": CgridSlowTendenciesDispatch,
+    "nimbus_hip_dispatch_slow_apply": SlowApplyDispatch,
+    "nimbus_hip_dispatch_state_relaxation_cartesian": StateRelaxationDispatch,
+    "nimbus_hip_dispatch_microphysics_apply": MicrophysicsApplyDispatch,
+    "nimbus_hip_dispatch_kessler_pointwise": KesslerPointwiseDispatch,
+    "nimbus_hip_dispatch_kessler_sedimentation": KesslerSedimentationDispatch,
+    "nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid": ConservativeTransportDispatch,
+}
+
 _LIB = None
 
 
@@ -223,6 +308,12 @@ def load_library(path=LIB_PATH):
         "nimbus_hip_dispatch_advect_scalars": (ctypes.c_int, [vp, P(AdvectScalarsDispatch)]),
         "nimbus_hip_dispatch_kessler": (ctypes.c_int, [vp, P(KesslerDispatch)]),
     }
+    decl["nimbus_hip_successful_dispatch_count"] = (ctypes.c_uint64, [vp, ctypes.c_char_p])
+    decl["nimbus_hip_device_arch"] = (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t])
+    decl["nimbus_hip_batch_stats"] = (ctypes.c_int, [vp, P(ctypes.c_uint64), P(ctypes.c_double), P(ctypes.c_uint64),
+                                                     P(ctypes.c_double)])
+    for name, st in REFERENCE_DISPATCH.items():
+        decl[name] = (ctypes.c_int, [vp, P(st)])
     for name, (res, args) in decl.items():
         fn = getattr(lib, name)
         fn.restype = res
@@ -588,6 +679,21 @@ class Context:
 
     def last_step_ms(self):
         return self.lib.nimbus_hip_last_step_ms(self._h)
+
+    def device_arch(self):
+        """Architecture name of the context's device, without feature suffixes (e.g. 'gfx950')."""
+        buf = ctypes.create_string_buffer(128)
+        check(self.lib.nimbus_hip_device_arch(self._h, buf, 128), "device_arch")
+        return buf.value.decode().split(":")[0]
+
+    def batch_stats(self):
+        """Membership-change costs so far: batch builds and graph captures (counts, host ms)."""
+        b, c = ctypes.c_uint64(), ctypes.c_uint64()
+        bm, cm = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.nimbus_hip_batch_stats(self._h, ctypes.byref(b), ctypes.byref(bm), ctypes.byref(c),
+                                              ctypes.byref(cm)), "batch_stats")
+        return {"batch_builds": b.value, "batch_build_ms": bm.value, "graph_captures": c.value,
+                "graph_capture_ms": cm.value}
 
     def dispatch_count(self):
         return self.lib.nimbus_hip_dispatch_count(self._h)

@@ -12,7 +12,15 @@ for waiting requests. A member's arithmetic does not depend on what else is
 in the batch, so a result is bitwise the same alone and in the mix.
 
 The initial state of an arrived request is generated on a host thread while
-the device works on the current batch.
+the device works on the current batch: a round's steps are enqueued without
+waiting (nimbus_hip_step_async), arrivals are collected and their host
+preparation started, and only then does the loop wait for the device.
+
+With a TuningCache, every admitted member gets the autotuned launch geometry
+of its shape (tuned the first time the shape is seen); the geometry never
+changes results. `column_solve` fixes the HE-VI column-solve form of every
+member the server creates (None keeps each member's own default); it is part
+of the member's arithmetic, so run_alone must be given the same value.
 
 With `preempt` on, a request that does not fit may evict resident members
 whose remaining work (cells x steps) is at least `preempt_ratio` times its
@@ -94,6 +102,7 @@ class Report:
     peak_bytes: int
     peak_members: int
     preemptions: int = 0
+    unfinished: list = dataclasses.field(default_factory=list)
 
     @property
     def cell_updates(self):
@@ -113,7 +122,11 @@ class Report:
                 "p50_latency_s": round(self.latency_percentile(50), 3),
                 "p99_latency_s": round(self.latency_percentile(99), 3),
                 "rounds": self.rounds, "peak_resident_bytes": self.peak_bytes,
-                "peak_resident_members": self.peak_members, "preemptions": self.preemptions}
+                "peak_resident_members": self.peak_members, "preemptions": self.preemptions,
+                "unfinished": len(self.unfinished),
+                "unfinished_resident": sum(1 for u in self.unfinished if u["state"] == "resident"),
+                "unfinished_waiting": sum(1 for u in self.unfinished if u["state"] == "waiting"),
+                "unfinished_oldest_age_s": max((u["age_s"] for u in self.unfinished), default=0.0)}
 
 
 class _Slot:
@@ -149,7 +162,8 @@ class Server:
     """Continuous-batching request loop over one device context."""
 
     def __init__(self, ctx, budget_bytes=200e9, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI,
-                 quantum=16, graphs=True, host_workers=4, preempt=False, preempt_ratio=4.0, results_dir=None):
+                 quantum=16, graphs=True, host_workers=4, preempt=False, preempt_ratio=4.0, results_dir=None,
+                 tuning=None, column_solve=None, progress=None):
         self.ctx = ctx
         self.budget = int(budget_bytes)
         self.precision = precision
@@ -161,6 +175,9 @@ class Server:
         self.preempt = bool(preempt)
         self.preempt_ratio = float(preempt_ratio)
         self.results_dir = results_dir
+        self.tuning = tuning
+        self.column_solve = column_solve
+        self.progress = progress
         self.preemptions = 0
         self._bytes = {}
 
@@ -176,10 +193,13 @@ class Server:
         wl = req.workload
         return wl.base(), wl.initial_state(self.dtype)
 
-    def run(self, requests, time_scale=1.0, keep_fields=(), clock=time.perf_counter, sleep=time.sleep):
+    def run(self, requests, time_scale=1.0, keep_fields=(), clock=time.perf_counter, sleep=time.sleep,
+            stop_after=None):
         """Serve `requests` (arrival times in seconds, compressed by `time_scale`).
         Returns a Report; results in completion order. `keep_fields` lists
-        request ids whose final fields are downloaded."""
+        request ids whose final fields are downloaded. With `stop_after`
+        (seconds of wall time) the run ends there and the Report lists the
+        requests still resident, waiting or not yet arrived as `unfinished`."""
         keep = set(keep_fields)
         self.preemptions = 0
         queue = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
@@ -193,8 +213,12 @@ class Server:
         self.ctx.set_graphs(self.graphs)
         start = clock()
         now = lambda: (clock() - start) * time_scale
+        stopped = False
         try:
             while queue or waiting or active:
+                if stop_after is not None and clock() - start >= stop_after:
+                    stopped = True
+                    break
                 t = now()
                 while queue and queue[0].arrival <= t:
                     r = queue.popleft()
@@ -234,6 +258,7 @@ class Server:
                     base, st = prepared.pop(r.rid).result()
                     wl = r.workload
                     m = self.ctx.member(wl.grid, wl.config(acoustic_mode=self.mode), base, self.precision)
+                    configure_member(m, self.mode, self.column_solve, self.tuning)
                     m.upload(st)
                     active.append(_Slot(r, m, nb, now()))
                     used += nb
@@ -246,9 +271,19 @@ class Server:
                             sleep(min(wait, 0.05))
                     continue
                 n = min(self.quantum, min(s.remaining for s in active))
-                self.ctx.step([s.member for s in active], n)
+                self.ctx.step([s.member for s in active], n, wait=False)
+                # while the device works: collect arrivals and start their host preparation
+                t = now()
+                while queue and queue[0].arrival <= t:
+                    r = queue.popleft()
+                    prepared[r.rid] = pool.submit(self._prepare, r)
+                    waiting.append(r)
+                self.ctx.synchronize()
                 dev_seconds += self.ctx.last_step_ms() / 1e3
                 rounds += 1
+                if self.progress is not None and rounds % 25 == 0:
+                    self.progress(f"round {rounds}: t={now():.1f} s, resident {len(active)}, waiting "
+                                  f"{len(waiting)}, done {len(results)}, not arrived {len(queue)}")
                 still = []
                 for s in active:
                     s.remaining -= n
@@ -267,6 +302,19 @@ class Server:
                     s.member.destroy()
                     used -= s.bytes
                 active = still
+            unfinished = []
+            if stopped:
+                t = now()
+                for s in active:
+                    unfinished.append({"rid": s.req.rid, "state": "resident", "age_s": round(t - s.req.arrival, 3),
+                                       "steps_left": s.remaining, "steps": s.req.workload.steps})
+                for item in waiting:
+                    r = item.req if isinstance(item, _Suspended) else item
+                    unfinished.append({"rid": r.rid, "state": "waiting", "age_s": round(t - r.arrival, 3),
+                                       "steps_left": r.workload.steps, "steps": r.workload.steps})
+                for r in queue:
+                    unfinished.append({"rid": r.rid, "state": "not arrived", "age_s": 0.0,
+                                       "steps_left": r.workload.steps, "steps": r.workload.steps})
         finally:
             for s in active:
                 s.member.destroy()
@@ -274,7 +322,9 @@ class Server:
             pool.shutdown(wait=True, cancel_futures=True)
             self.ctx.set_graphs(False)
         wall = (clock() - start)
-        return Report(results, rejected, wall, dev_seconds, rounds, peak_bytes, peak_members, self.preemptions)
+        rep = Report(results, rejected, wall, dev_seconds, rounds, peak_bytes, peak_members, self.preemptions)
+        rep.unfinished = unfinished
+        return rep
 
     def _victims(self, active, req, need):
         """Resident members to evict so that `need` more bytes fit, largest
@@ -294,11 +344,24 @@ class Server:
         return out if freed >= need else []
 
 
-def run_alone(ctx, req, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI, graphs=True):
+def configure_member(m, mode, column_solve=None, tuning=None):
+    """The server's per-member settings: the HE-VI column-solve form (part of
+    the member's arithmetic) and the cached launch geometry (not)."""
+    if column_solve is not None and mode == L.ACOUSTIC_HEVI:
+        m.set_column_solve(column_solve)
+    if tuning is not None:
+        tuning.apply(m, mode)
+
+
+def run_alone(ctx, req, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI, graphs=True, column_solve=None,
+              tuning=None):
     """One request on its own (the reference for 'same alone and in the mix' and FP64 reruns)."""
     wl = req.workload
     dtype = np.float32 if precision == L.PRECISION_MIXED else np.float64
     m = ctx.member(wl.grid, wl.config(acoustic_mode=acoustic_mode), wl.base(), precision)
+    # the column-solve form applies to the FP32-storage members it was chosen
+    # for; FP64 reruns keep their default (partitioned up to 129 levels)
+    configure_member(m, acoustic_mode, column_solve if precision == L.PRECISION_MIXED else None, tuning)
     ctx.set_graphs(graphs)
     try:
         m.upload(wl.initial_state(dtype))

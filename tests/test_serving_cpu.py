@@ -50,6 +50,7 @@ class FakeContext:
         self.batches = []
         self.ms = 0.0
         self.graphs = False
+        self.pending = False
 
     def member(self, grid, config, base, precision=L.PRECISION_MIXED):
         return FakeMember(self, grid, config, base, precision)
@@ -63,10 +64,15 @@ class FakeContext:
 
     def step(self, members, n_steps=1, wait=True):
         assert all(m.uploaded and not m.destroyed for m in members)
+        assert not self.pending, "a step was enqueued while another was not waited for"
         self.batches.append((len(members), n_steps))
         for m in members:
             m.steps += n_steps
         self.ms = float(n_steps)
+        self.pending = not wait
+
+    def synchronize(self):
+        self.pending = False
 
     def last_step_ms(self):
         return self.ms
@@ -176,3 +182,35 @@ def _stepping_clock():
         t[0] += 1e-3
         return t[0]
     return clock
+
+
+class TunableMember:
+    def __init__(self, grid, precision):
+        self.grid = grid
+        self.precision = precision
+        self.autotuned = 0
+        self.tiling = None
+
+    def autotune(self, trials):
+        self.autotuned += 1
+        self.tiling = (32, 3, 0)
+        return 32, 3, 0, 1.25
+
+    def set_tiling(self, w, c, r):
+        self.tiling = (w, c, r)
+
+
+def test_tuning_cache_tunes_each_shape_once_and_persists(tmp_path):
+    from nimbus_amd import tuning as T
+    cache = T.TuningCache("gfx950", directory=str(tmp_path))
+    g = L.make_grid(48, 48, 40, 1000.0, 1000.0, 500.0)
+    a, b = TunableMember(g, L.PRECISION_MIXED), TunableMember(g, L.PRECISION_MIXED)
+    assert cache.apply(a, L.ACOUSTIC_HEVI) == (32, 3, 0) and a.autotuned == 1
+    assert cache.apply(b, L.ACOUSTIC_HEVI) == (32, 3, 0) and b.autotuned == 0 and b.tiling == (32, 3, 0)
+    other = TunableMember(g, L.PRECISION_FP64)
+    cache.apply(other, L.ACOUSTIC_HEVI)
+    assert other.autotuned == 1  # precision is part of the key
+    again = T.TuningCache("gfx950", directory=str(tmp_path))
+    c = TunableMember(g, L.PRECISION_MIXED)
+    again.apply(c, L.ACOUSTIC_HEVI)
+    assert c.autotuned == 0 and c.tiling == (32, 3, 0) and again.hits == 1

@@ -25,12 +25,35 @@ def reader_rows(col):
     return np.stack([col["z"], col["p"] / 100.0, col["T"], col["theta"], col["qv"], col["u"], col["v"]], 1)
 
 
-def ref_tail(msg):
-    """The reference's message without the component prefix it names."""
-    for pre in ("CM1 input_sounding: ", "CM1 input_sounding ", "FileSoundingSource: "):
-        if msg.startswith(pre):
-            return msg[len(pre):]
-    return msg
+# Which check refused, by the phrase the reference's message uses -> the tag
+# this library's message carries for the same check.
+REFUSAL_TAGS = (
+    ("failed quality control", "[quality]"),
+    ("silently shortened profile", "[qc-shortened]"),
+    ("columns; expected", "[columns]"),
+    ("is not numeric", "[not-numeric]"),
+    ("first profile level must be at z=0", "[first-z0]"),
+    ("heights must increase strictly", "[z-order]"),
+    ("at least five vertical levels", "[too-few-levels]"),
+    ("surface header theta does not match", "[header-theta]"),
+    ("wind component magnitude exceeds", "[wind-limit]"),
+    ("model top interface", "[top-coverage]"),
+)
+
+
+def same_refusal(ref_msg, ours):
+    """Our refusal is the same check the reference's message names."""
+    tags = [tag for phrase, tag in REFUSAL_TAGS if phrase in ref_msg]
+    assert len(tags) == 1, f"no single check for the reference message {ref_msg!r}"
+    return tags[0] in ours
+
+
+def test_every_reference_refusal_maps_to_one_check():
+    msgs = [c["error"] for f in GOLD["files"].values() for c in f["columns"] if c["error"]]
+    msgs += [e["read"]["error"] for e in GOLD["synthetic"].values() if e["read"]["error"]]
+    assert msgs
+    for m in msgs:
+        assert len([t for p, t in REFUSAL_TAGS if p in m]) == 1, m
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -49,7 +72,7 @@ def test_columns_bit_exact(name):
         if case["error"]:
             with pytest.raises(L.NimbusError) as e:
                 L.cm1_column(col, case["z"], case["dz"], case["method"], ex, ex)
-            assert ref_tail(case["error"]) in str(e.value)
+            assert same_refusal(case["error"], str(e.value))
             continue
         out = L.cm1_column(col, case["z"], case["dz"], case["method"], ex, ex)
         rows = np.stack([np.array(case["z"])] + [out[k] for k in ("theta0", "T0", "p0", "rho0", "qv0", "u0", "v0")], 1)
@@ -62,7 +85,7 @@ def test_synthetic_texts(name):
     if e["read"]["error"]:
         with pytest.raises(L.NimbusError) as err:
             L.cm1_parse(e["text"])
-        assert ref_tail(e["read"]["error"]) in str(err.value)
+        assert same_refusal(e["read"]["error"], str(err.value))
         return
     col = L.cm1_parse(e["text"])
     assert np.array_equal(reader_rows(col), np.array(e["read"]["rows"]))
@@ -79,7 +102,7 @@ def test_base_state_is_the_column_at_cell_centres():
     c = L.cm1_column(col, (np.arange(38) + 0.5) * 500.0, 500.0, L.INTERP_SPLINE)
     for k in ("rho0", "p0", "theta0", "qv0", "u0", "v0"):
         assert np.array_equal(b[k], c[k]), k
-    with pytest.raises(L.NimbusError, match="upper half-cell"):
+    with pytest.raises(L.NimbusError, match=r"\[top-coverage\]"):
         L.cm1_base_state(L.make_grid(16, 16, 40, 500.0, 500.0, 500.0), col)
 
 
