@@ -640,6 +640,24 @@ typedef struct
 int nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid(
     nimbus_hip_ctx* ctx, const nimbus_conservative_cartesian_cgrid_transport_dispatch* d);
 
+/* Allocation-free variant of nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Every array in *d (winds, densities, mass fluxes,
+ * q_in[s], q_out[s]) is a DEVICE pointer owned by the caller; the pointer
+ * arrays q_in/q_out, dt_kappas and positive_definite stay in host memory.
+ * The face fluxes, limiter ratios and (standalone mode) the formed mass
+ * fluxes live in one static device array per GPU of 8 floats per cell for at
+ * most NIMBUS_NOALLOC_MAX_CELLS cells (32 MiB), held until the process ends;
+ * larger grids are refused with NIMBUS_EINVAL. The call makes no hipMalloc,
+ * hipFree or other allocation, runs on the context stream and returns when
+ * the device finished. Calls that overlap on one GPU share that array and
+ * race; successive calls need no clearing (every scratch element is written
+ * before it is read). */
+#define NIMBUS_NOALLOC_MAX_CELLS (1u << 20)
+int nimbus_hip_advect_scalars_conservative_noalloc(nimbus_hip_ctx* ctx,
+                                                   const nimbus_conservative_cartesian_cgrid_transport_dispatch* d);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -723,6 +724,34 @@ void enqueue_transport(const nimbus_conservative_cartesian_cgrid_transport_dispa
     }
 }
 
+// Device-resident scratch of the allocation-free transport entry point: one
+// array per GPU, held for the life of the process. The host-side name is a
+// stand-in, not a device address; kernels get the address that
+// hipGetSymbolAddress returns for the current device.
+constexpr std::size_t kNoAllocMaxCells = std::size_t(1) << 20;
+static __device__ __align__(256) float scratch[kTransportScratchFields * kNoAllocMaxCells];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch address on `device` (the current device), looked up once.
+int scratch_on(int device, float** out)
+{
+    if (device < 0 || device >= kMaxDevices) return set_error(NIMBUS_ENODEV, "no-alloc transport: device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device])
+    {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(NIMBUS_EHIP, std::string("no-alloc transport: scratch address: ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<float*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return NIMBUS_OK;
+}
+
 }  // namespace
 }  // namespace nimbus
 
@@ -990,6 +1019,36 @@ int nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid(
     if (rc == NIMBUS_OK) REF_TRY(hipStreamSynchronize(s));
     if (rc == NIMBUS_OK) count_dispatch(ctx, "advect_scalars_conservative_cartesian_cgrid");
     return rc;
+}
+
+int nimbus_hip_advect_scalars_conservative_noalloc(nimbus_hip_ctx* ctx,
+                                                   const nimbus_conservative_cartesian_cgrid_transport_dispatch* d)
+{
+    if (!ctx || !d) return set_error(NIMBUS_EINVAL, "no-alloc transport: null argument");
+    int rc = validate_transport(*d);
+    if (rc != NIMBUS_OK) return rc;
+    const std::size_t n = std::size_t(d->nx) * d->ny * d->nz;
+    if (n > kNoAllocMaxCells)
+        return set_error(NIMBUS_EINVAL, "no-alloc transport: " + std::to_string(n) + " cells exceed the scratch maximum of " +
+                                            std::to_string(kNoAllocMaxCells) + " cells");
+    const int device = context_device(ctx);
+    REF_TRY(hipSetDevice(device));
+    float* scr = nullptr;
+    rc = scratch_on(device, &scr);
+    if (rc != NIMBUS_OK) return rc;
+    hipStream_t s = context_stream(ctx);
+    const bool sup = d->use_supplied_mass_flux != 0;
+    TransportArrays t{};
+    t.rho = d->rho_data;
+    if (sup) { t.mfx = d->mass_flux_x; t.mfy = d->mass_flux_y; t.mfz = d->mass_flux_z; t.rho_begin = d->density_begin; }
+    else { t.u = d->u; t.v = d->v; t.w = d->w; }
+    t.q_in = d->q_in;
+    t.q_out = d->q_out;
+    enqueue_transport(*d, t, scr, s);
+    REF_TRY(hipGetLastError());
+    REF_TRY(hipStreamSynchronize(s));
+    count_dispatch(ctx, "advect_scalars_conservative_noalloc");
+    return NIMBUS_OK;
 }
 
 }  // extern "C"

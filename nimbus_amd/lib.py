@@ -314,6 +314,7 @@ def load_library(path=LIB_PATH):
                                                      P(ctypes.c_double)])
     for name, st in REFERENCE_DISPATCH.items():
         decl[name] = (ctypes.c_int, [vp, P(st)])
+    decl["nimbus_hip_advect_scalars_conservative_noalloc"] = (ctypes.c_int, [vp, P(ConservativeTransportDispatch)])
     for name, (res, args) in decl.items():
         fn = getattr(lib, name)
         fn.restype = res
