@@ -153,10 +153,12 @@ def cpu_info():
     return info
 
 
-def run_member(ctx, L, wl, precision, mode, steps, warmup, graphs):
+def run_member(ctx, L, wl, precision, mode, steps, warmup, graphs, column_solve=None):
     dtype = np.float32 if precision == L.PRECISION_MIXED else np.float64
     cfg = wl.config(acoustic_mode=mode)
     m = ctx.member(wl.grid, cfg, wl.base(), precision)
+    if column_solve is not None:
+        m.set_column_solve(column_solve)
     m.upload(wl.initial_state(dtype))
     ctx.set_graphs(graphs)
     ctx.step([m], warmup)
@@ -324,7 +326,7 @@ def ensemble_run(ctx, L, W, precision, mode, tuning=None, column_solve=None):
 
 
 def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5, tuning=None, column_solve=None,
-                stop_after=None):
+                stop_after=None, max_cells=None):
     """Continuous batching of seeded Poisson requests (2/s) arriving over `seconds`,
     then `fp64_sample` requests (drawn with seed 5) rerun alone in FP64: the error of each
     served (mixed) result against its FP64 rerun. `stop_after` (wall seconds) ends
@@ -336,13 +338,18 @@ def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5, tuning=None, co
     sample = sorted(random.Random(5).sample([r.rid for r in reqs], min(fp64_sample, len(reqs))))
     before = ctx.batch_stats()
     srv = SV.Server(ctx, budget_bytes=200e9, precision=precision, acoustic_mode=mode, quantum=16, tuning=tuning,
-                    column_solve=column_solve, progress=lambda m: progress("  serving " + m))
+                    column_solve=column_solve, progress=lambda m: progress("  serving " + m),
+                    max_resident_cells=max_cells)
     rep = srv.run(reqs, keep_fields=sample, stop_after=stop_after)
     s = rep.summary()
     after = ctx.batch_stats()
     s["arrival_seconds"] = seconds
     s["stream"] = "full 10-minute stream" if seconds >= 600 else f"reduced stream: the first {seconds:g} s of arrivals"
+    if stop_after is not None:
+        s["stream"] += (f", stopped after {stop_after:g} s of wall time with {s['unfinished']} requests unfinished"
+                        if s["unfinished"] else f", finished within the {stop_after:g} s stop")
     s["requests_generated"] = len(reqs)
+    s["max_resident_cells"] = max_cells
     s["membership_changes"] = {k: round(after[k] - before[k], 3) for k in after}
     s["data"] = ("seed 4 Poisson arrivals at 2/s; nx = ny in {48..256}, nz in {40, 60, 80}, dx 500-1000 m, "
                  "15-90 simulated minutes each; 200 GB HBM budget")
@@ -377,6 +384,9 @@ def main():
                     help="end the serving run after this many wall seconds and report the backlog")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving run (with --serving-seconds), no single-member or ensemble runs")
+    ap.add_argument("--serving-max-cells", type=float, default=48e6,
+                    help="cap on the cells resident at once in the serving run besides the 200 GB HBM budget "
+                         "(0: no cap)")
     ap.add_argument("--column-solve", choices=["auto", "sequential", "partitioned"], default="sequential",
                     help="HE-VI column-solve form of the ensemble and serving members (FP32 storage)")
     ap.add_argument("--no-tuning", action="store_true",
@@ -388,6 +398,8 @@ def main():
                     help="only one member in this mode: warm-up + timed steps, for rocprofv3 passes")
     ap.add_argument("--profile-precision", choices=["mixed", "fp64"], default="mixed",
                     help="storage of the --profile member")
+    ap.add_argument("--profile-column-solve", choices=["auto", "sequential", "partitioned"], default="auto",
+                    help="HE-VI column-solve form of the --profile member (auto: the member's default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -423,16 +435,18 @@ def main():
         progress(f"serving: Poisson arrivals over {args.serving_seconds:g} s")
         out = {"serving_c4": serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI,
                                          tuning=tuning, column_solve=col, stop_after=args.serving_stop_after,
-                                         fp64_sample=0)}
+                                         fp64_sample=0, max_cells=args.serving_max_cells or None)}
         ctx.close()
         print(json.dumps(out))
         return
 
     if args.profile:
         prec = L.PRECISION_FP64 if args.profile_precision == "fp64" else L.PRECISION_MIXED
-        m, ms, _ = run_member(ctx, L, wl, prec, modes[args.profile], args.steps, args.warmup, graphs)
+        m, ms, _ = run_member(ctx, L, wl, prec, modes[args.profile], args.steps, args.warmup, graphs,
+                              column_solve_code(L, args.profile_column_solve))
         print(json.dumps({"profile": True, "workload": wl.describe(), "mode": args.profile,
-                          "precision": args.profile_precision, "steps": args.steps, "ms_per_step": ms / args.steps}))
+                          "precision": args.profile_precision, "column_solve": m.column_solve(),
+                          "steps": args.steps, "ms_per_step": ms / args.steps}))
         m.destroy()
         ctx.close()
         return
@@ -520,7 +534,8 @@ def main():
     if args.serving_seconds > 0:
         progress(f"serving: Poisson arrivals over {args.serving_seconds:g} s")
         result["serving_c4"] = serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI,
-                                           tuning=tuning, column_solve=col, stop_after=args.serving_stop_after)
+                                           tuning=tuning, column_solve=col, stop_after=args.serving_stop_after,
+                                           max_cells=args.serving_max_cells or None)
     if args.accuracy:
         from nimbus_amd import accuracy as ACC
         for q, name in enumerate(n for n in args.accuracy.split(",") if n):

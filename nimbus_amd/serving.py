@@ -16,6 +16,14 @@ the device works on the current batch: a round's steps are enqueued without
 waiting (nimbus_hip_step_async), arrivals are collected and their host
 preparation started, and only then does the loop wait for the device.
 
+`max_resident_cells` caps the cells resident at once besides the HBM budget
+(None: the budget alone decides). A batch of a few tens of million cells
+already keeps the device busy; admitting more only spreads the same
+throughput over more members, so under overload every request waits longer
+and each membership change rebuilds a larger batch. Admission stays first
+come, first served, and a request larger than the cap is admitted when
+nothing else is resident.
+
 With a TuningCache, every admitted member gets the autotuned launch geometry
 of its shape (tuned the first time the shape is seen); the geometry never
 changes results. `column_solve` fixes the HE-VI column-solve form of every
@@ -126,7 +134,15 @@ class Report:
                 "unfinished": len(self.unfinished),
                 "unfinished_resident": sum(1 for u in self.unfinished if u["state"] == "resident"),
                 "unfinished_waiting": sum(1 for u in self.unfinished if u["state"] == "waiting"),
-                "unfinished_oldest_age_s": max((u["age_s"] for u in self.unfinished), default=0.0)}
+                "unfinished_oldest_age_s": max((u["age_s"] for u in self.unfinished), default=0.0),
+                # finished work plus the steps already taken by members still resident
+                "cell_updates_incl_partial": self.cell_updates_incl_partial,
+                "cell_updates_per_s_incl_partial":
+                    self.cell_updates_incl_partial / self.wall_seconds if self.wall_seconds > 0 else 0.0}
+
+    @property
+    def cell_updates_incl_partial(self):
+        return self.cell_updates + sum((u["steps"] - u["steps_left"]) * u["cells"] for u in self.unfinished)
 
 
 class _Slot:
@@ -163,9 +179,10 @@ class Server:
 
     def __init__(self, ctx, budget_bytes=200e9, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI,
                  quantum=16, graphs=True, host_workers=4, preempt=False, preempt_ratio=4.0, results_dir=None,
-                 tuning=None, column_solve=None, progress=None):
+                 tuning=None, column_solve=None, progress=None, max_resident_cells=None):
         self.ctx = ctx
         self.budget = int(budget_bytes)
+        self.max_cells = None if max_resident_cells is None else int(max_resident_cells)
         self.precision = precision
         self.mode = acoustic_mode
         self.quantum = int(quantum)
@@ -235,6 +252,9 @@ class Server:
                         used += item.slot.bytes
                         continue
                     r = item
+                    if (self.max_cells is not None and active and
+                            sum(a.req.workload.cells for a in active) + r.workload.cells > self.max_cells):
+                        break
                     nb = self.footprint(r)
                     if nb > self.budget:
                         rejected.append(r.rid)
@@ -307,14 +327,17 @@ class Server:
                 t = now()
                 for s in active:
                     unfinished.append({"rid": s.req.rid, "state": "resident", "age_s": round(t - s.req.arrival, 3),
-                                       "steps_left": s.remaining, "steps": s.req.workload.steps})
+                                       "steps_left": s.remaining, "steps": s.req.workload.steps,
+                                       "cells": s.req.workload.cells})
                 for item in waiting:
                     r = item.req if isinstance(item, _Suspended) else item
                     unfinished.append({"rid": r.rid, "state": "waiting", "age_s": round(t - r.arrival, 3),
-                                       "steps_left": r.workload.steps, "steps": r.workload.steps})
+                                       "steps_left": r.workload.steps, "steps": r.workload.steps,
+                                       "cells": r.workload.cells})
                 for r in queue:
                     unfinished.append({"rid": r.rid, "state": "not arrived", "age_s": 0.0,
-                                       "steps_left": r.workload.steps, "steps": r.workload.steps})
+                                       "steps_left": r.workload.steps, "steps": r.workload.steps,
+                                       "cells": r.workload.cells})
         finally:
             for s in active:
                 s.member.destroy()

@@ -214,3 +214,47 @@ def test_tuning_cache_tunes_each_shape_once_and_persists(tmp_path):
     c = TunableMember(g, L.PRECISION_MIXED)
     again.apply(c, L.ACOUSTIC_HEVI)
     assert c.autotuned == 0 and c.tiling == (32, 3, 0) and again.hits == 1
+
+
+def test_stop_after_reports_the_backlog_and_the_partial_work():
+    """A run stopped by wall time lists every request not finished (resident,
+    waiting, not arrived) and counts the steps resident members already took."""
+    reqs = small_requests(4, steps=[400])
+    late = small_requests(1, steps=[5])[0]
+    reqs.append(SV.Request(4, 1e9, late.workload))  # arrives long after the stop
+    ctx = FakeContext()
+    srv = SV.Server(ctx, quantum=4, host_workers=1)
+    srv.budget = 2 * srv.footprint(reqs[0])
+    ticks = iter(range(10 ** 6))
+    rep = srv.run(reqs, clock=lambda: float(next(ticks)), sleep=lambda s: None, stop_after=40.0)
+    s = rep.summary()
+    states = sorted(u["state"] for u in rep.unfinished)
+    assert s["requests"] == 0 and s["unfinished"] == 5
+    assert states == ["not arrived", "resident", "resident", "waiting", "waiting"]
+    taken = sum(m.steps for m in ctx.created)
+    assert 0 < taken < 2 * 400
+    cells = reqs[0].workload.cells
+    assert s["cell_updates_incl_partial"] == taken * cells
+    assert s["cell_updates_per_s_incl_partial"] == pytest.approx(taken * cells / rep.wall_seconds)
+    assert all(m.destroyed for m in ctx.created)
+
+
+def test_resident_cell_cap_holds_back_requests_until_cells_free():
+    """With max_resident_cells the batch never holds more cells than the cap
+    (one request alone may exceed it); every request still finishes, in
+    arrival order of admission."""
+    reqs = small_requests(6, steps=[6, 10])
+    cells = reqs[0].workload.cells
+    ctx = FakeContext()
+    srv = SV.Server(ctx, quantum=4, host_workers=1, max_resident_cells=2 * cells + 1)
+    rep = srv.run(reqs, time_scale=1e6, sleep=lambda s: None)
+    assert sorted(r.rid for r in rep.results) == list(range(6))
+    assert rep.peak_members == 2
+    assert max(n for n, _ in ctx.batches) == 2
+    admitted = sorted(rep.results, key=lambda r: r.admitted)
+    assert [r.rid for r in admitted] == sorted(r.rid for r in admitted)
+    # a cap below one request's size still admits it when nothing is resident
+    ctx = FakeContext()
+    rep = SV.Server(ctx, quantum=4, host_workers=1, max_resident_cells=cells // 2).run(
+        reqs[:2], time_scale=1e6, sleep=lambda s: None)
+    assert len(rep.results) == 2 and rep.peak_members == 1
