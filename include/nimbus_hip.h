@@ -103,7 +103,19 @@ typedef struct
     double sgs_pr_t, sgs_sc_t;    /* 1/3, 1/3 */
     double sgs_nu_max, sgs_k_max; /* caps, 1000 m^2/s */
     double sgs_dt;                /* longest SGS substep (1 s) */
+    /* Explicit Laplacian diffusion after the SGS and before the EOS
+     * projection (core/orchestration/physics/diffusion_step.cpp:679-1030,
+     * numerics/diffusion/schemes/explicit/explicit.cpp): the live-density
+     * finite-volume tendencies of the state after the SGS, committed as
+     * field + dt tendency with the storage clamps. diffusion: 0 off,
+     * NIMBUS_DIFFUSE_SCALARS (theta, qv), NIMBUS_DIFFUSE_MOMENTUM (u, v, w),
+     * or both OR-ed (the reference's apply_to "scalars", "momentum", "all");
+     * lateral sides diffuse periodically when both faces of an axis are
+     * periodic and are closed (zero flux) otherwise. */
+    int diffusion;
+    double diffusion_kh, diffusion_kv; /* m^2/s */
 } nimbus_step_config;
+enum { NIMBUS_DIFFUSE_SCALARS = 1, NIMBUS_DIFFUSE_MOMENTUM = 2 };
 
 /* FP64 base-state profiles at cell centres, each nz long
  * (include/core/runtime/simulation_state.hpp:104-111). */
@@ -639,6 +651,48 @@ typedef struct
 } nimbus_conservative_cartesian_cgrid_transport_dispatch;
 int nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid(
     nimbus_hip_ctx* ctx, const nimbus_conservative_cartesian_cgrid_transport_dispatch* d);
+
+/* Control-volume kind of a diffused field (DiffusionFieldLocation,
+ * include/compute/dispatch/diffusion.hpp:22-28). */
+enum
+{
+    NIMBUS_DIFFUSION_SCALAR = 0,
+    NIMBUS_DIFFUSION_U_FACE = 1,
+    NIMBUS_DIFFUSION_V_FACE = 2,
+    NIMBUS_DIFFUSION_W_FACE = 3
+};
+
+/* Replaces DiffusionComputeKernels::dispatch_diffusion_batch
+ * (include/compute/dispatch/diffusion.hpp:52-69,149-209; its single-field
+ * dispatch_diffusion, :29-50, is a batch of one; equations of
+ * ExplicitDiffusionScheme, src/numerics/diffusion/schemes/explicit/
+ * explicit.cpp:119-339,1183-1433): for each of `count` fields the live-density
+ * finite-volume Laplacian tendency d(field)/dt = (1/rho_cv) div(rho K
+ * grad(field)) on the field's control volumes (scalar cells or C-grid u, v,
+ * w faces; A-grid cells when c_grid is 0), K_h horizontally, K_v (or the
+ * field's vertical_K array) vertically with layer thicknesses dz[nz]; zero
+ * flux at closed control-volume edges, wrap on periodic sides. Every
+ * tendency cell is written (zero outside the control volumes). The vertical
+ * part is rounded to FP32 before the horizontal part is added, as the
+ * reference adds its two passes; FP64 arithmetic. Refused (NIMBUS_EINVAL,
+ * tendencies untouched): nx or ny < 3, nz < 2, count outside 1..16, a
+ * location outside 0..3, non-positive or non-finite dx, dy or dz, negative or
+ * non-finite K, a non-finite field value or a non-positive density inside a
+ * control-volume domain, a negative or non-finite vertical_K there. */
+typedef struct
+{
+    const float* const* fields;
+    float* const* tendencies;
+    const int* locations;
+    int count;
+    const float* rho;
+    const float* dz;
+    int nx, ny, nz;
+    float dx, dy, K_h, K_v;
+    int c_grid, periodic_x, periodic_y;
+    const float* const* vertical_K; /* null, or count arrays (each may be null) */
+} nimbus_diffusion_batch_dispatch;
+int nimbus_hip_dispatch_diffusion_batch(nimbus_hip_ctx* ctx, const nimbus_diffusion_batch_dispatch* d);
 
 /* Allocation-free variant of nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid.
  * No counterpart in the reference: it is for callers that cannot allocate

@@ -16,6 +16,7 @@
 #include "compute/backend_interface.hpp"
 #include "compute/backend_types.hpp"
 #include "compute/dispatch/advection.hpp"
+#include "compute/dispatch/diffusion.hpp"
 #include "compute/dispatch/dynamics.hpp"
 #include "compute/dispatch/microphysics.hpp"
 #include "compute/kernel_manifest.hpp"
@@ -296,10 +297,55 @@ private:
     nimbus_hip_ctx*& ctx_;
 };
 
+class Diffusion final : public DiffusionComputeKernels
+{
+public:
+    explicit Diffusion(nimbus_hip_ctx*& ctx) : ctx_(ctx) {}
+
+    bool supports(DiffusionKernelOp op) const override { return ctx_ && op == DiffusionKernelOp::Diffusion; }
+
+    bool dispatch_diffusion(const DiffusionDispatch& d) override
+    {
+        const float* const fields[1] = {d.field};
+        float* const tendencies[1] = {d.tendency};
+        const DiffusionFieldLocation locations[1] = {d.location};
+        const float* const vertical[1] = {d.vertical_K};
+        const DiffusionBatchDispatch b{fields, tendencies, locations, 1, d.rho, d.dz, d.nx, d.ny, d.nz, d.dx, d.dy,
+                                       d.K_h, d.K_v, d.c_grid, d.periodic_x, d.periodic_y,
+                                       d.vertical_K ? vertical : nullptr};
+        return dispatch_diffusion_batch(b);
+    }
+
+    bool dispatch_diffusion_batch(const DiffusionBatchDispatch& d) override
+    {
+        if (!ctx_ || d.count < 1 || !d.locations) return false;
+        try
+        {
+            std::vector<int> locations(static_cast<std::size_t>(d.count));
+            for (int t = 0; t < d.count; ++t) locations[static_cast<std::size_t>(t)] = static_cast<int>(d.locations[t]);
+            nimbus_diffusion_batch_dispatch c{};
+            c.fields = d.fields; c.tendencies = d.tendencies; c.locations = locations.data(); c.count = d.count;
+            c.rho = d.rho; c.dz = d.dz;
+            c.nx = d.nx; c.ny = d.ny; c.nz = d.nz;
+            c.dx = d.dx; c.dy = d.dy; c.K_h = d.K_h; c.K_v = d.K_v;
+            c.c_grid = d.c_grid ? 1 : 0; c.periodic_x = d.periodic_x ? 1 : 0; c.periodic_y = d.periodic_y ? 1 : 0;
+            c.vertical_K = d.vertical_K;
+            return nimbus_hip_dispatch_diffusion_batch(ctx_, &c) == NIMBUS_OK;
+        }
+        catch (...)
+        {
+            return false;
+        }
+    }
+
+private:
+    nimbus_hip_ctx*& ctx_;
+};
+
 class HipComputeBackend final : public ComputeBackend
 {
 public:
-    HipComputeBackend() : dynamics_(ctx_), advection_(ctx_), microphysics_(ctx_) {}
+    HipComputeBackend() : dynamics_(ctx_), advection_(ctx_), microphysics_(ctx_), diffusion_(ctx_) {}
     ~HipComputeBackend() override { shutdown(); }
 
     std::string name() const override { return "hip"; }
@@ -353,6 +399,7 @@ public:
         case KernelId::KesslerSedimentation: op = "kessler_sedimentation"; break;
         case KernelId::MicrophysicsApply: op = "microphysics_apply"; break;
         case KernelId::AdvectConservativeUpdateCartesianCgrid: op = "advect_scalars_conservative_cartesian_cgrid"; break;
+        case KernelId::Diffusion: op = "diffusion_batch"; break;
         default: return 0;
         }
         return ctx_ ? nimbus_hip_successful_dispatch_count(ctx_, op) : 0;
@@ -364,6 +411,8 @@ public:
     const DynamicsComputeKernels* dynamics_kernels() const override { return ctx_ ? &dynamics_ : nullptr; }
     MicrophysicsComputeKernels* microphysics_kernels() override { return ctx_ ? &microphysics_ : nullptr; }
     const MicrophysicsComputeKernels* microphysics_kernels() const override { return ctx_ ? &microphysics_ : nullptr; }
+    DiffusionComputeKernels* diffusion_kernels() override { return ctx_ ? &diffusion_ : nullptr; }
+    const DiffusionComputeKernels* diffusion_kernels() const override { return ctx_ ? &diffusion_ : nullptr; }
 
     // The per-operation calls stage their arrays each time, so nothing is
     // resident between them: the residency hooks have no state to keep.
@@ -377,6 +426,7 @@ private:
     Dynamics dynamics_;
     Advection advection_;
     Microphysics microphysics_;
+    Diffusion diffusion_;
 };
 
 }  // namespace nimbus_hip_adapter

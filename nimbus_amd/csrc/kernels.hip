@@ -16,11 +16,12 @@
 // thread stays independent (full occupancy, no serial march). The HE-VI
 // column solve gives each thread one whole column.
 //
-// Precision: S is the storage type. Continuity, EOS, pressure gradient,
-// base-state terms, slow tendencies, mass-flux accumulation and the HE-VI
-// Thomas solve compute in FP64 for both storage types; WENO5 transport and
-// Kessler microphysics compute in the storage type (FP32 bulk in the mixed
-// mode).
+// Precision: S is the storage type. Continuity, EOS, base-state terms,
+// mass-flux accumulation and the HE-VI column system with its Thomas solve
+// compute in FP64 for both storage types. The slow tendencies (UP5
+// advection, buoyancy, Coriolis), the acoustic pressure-gradient tendency,
+// WENO5 transport, Kessler microphysics and the SGS operators compute in the
+// storage type's arithmetic (FP32 bulk in the mixed mode).
 #include "device_math.hpp"
 #include "kernels.hpp"
 
@@ -1963,7 +1964,8 @@ __global__ __launch_bounds__(kBlock) void k_kessler_apply(const MemberArgs* __re
     // subcycle length: from the member's clock when it subcycles by its
     // state, else the whole step (one subcycle)
     const double h = A.p.kessler_adaptive ? G(f.clock)[1] : A.p.dt / A.p.kessler_sub;
-    const bool do_eos = n == A.p.kessler_sub - 1 && A.p.sgs_n == 0;  // with the SGS on, after it
+    // with the SGS or the diffusion on, the projection follows them instead
+    const bool do_eos = n == A.p.kessler_sub - 1 && A.p.sgs_n == 0 && A.p.dif_apply == 0;
     const std::size_t c = g.at(i, j, k), sz = g.plane();
     // all loads first (the level above clamped at the lid, unused there)
     const bool up = k + 1 < NZ;
@@ -2343,7 +2345,7 @@ __device__ __forceinline__ void sgs_momentum_cell(const MemberArgs& A, const Cel
     // commit x + h dx/dt with the storage clamps (sgs_apply.cpp:610-660); the
     // tendency is rounded to storage first, as the reference's tendency field
     // the last substep of an even count lands in the home fields: the sponges follow it here
-    const bool sponge_here = A.p.sponge && n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0;
+    const bool sponge_here = A.p.sponge && n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0 && A.p.dif_apply == 0;
     auto commit = [&](T du, T dv, T dw) {
         S un = S(clamp_store<S>(double(s.u[c]) + finite0(double(S(du))) * hstep, 0.0, -bd.wind_h, bd.wind_h));
         S vn = S(clamp_store<S>(double(s.v[c]) + finite0(double(S(dv))) * hstep, 0.0, -bd.wind_h, bd.wind_h));
@@ -2623,7 +2625,7 @@ __device__ __forceinline__ void sgs_scalars_cell(const MemberArgs& A, const Cell
     P<S>(nxt.qv)[c] = S(clamp_store<S>(double(s.qv[c]) + finite0(double(S(dqv))) * hstep, 0.0, 0.0f, 0.05f));
     // an even count ends in the home fields: the EOS projection, then the
     // sponges, follow the last commit here
-    if (n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0)
+    if (n == A.p.sgs_n - 1 && (A.p.sgs_n & 1) == 0 && A.p.dif_apply == 0)
     {
         sgs_eos<S>(A, c, k, thn);
         if (A.p.sponge) sponge_theta<S>(sponge_at(A, i, j, k), thn);
@@ -2650,7 +2652,7 @@ __global__ __launch_bounds__(kBlock) void k_sgs_scalars(const MemberArgs* __rest
 }
 
 // Odd SGS counts: the state of the last substep back to the home fields,
-// then the EOS projection.
+// then the EOS projection and the sponges (unless the diffusion follows).
 template <class S>
 __global__ __launch_bounds__(kBlock) void k_sgs_home(const MemberArgs* __restrict__ ma, const int2* __restrict__ map)
 {
@@ -2663,8 +2665,8 @@ __global__ __launch_bounds__(kBlock) void k_sgs_home(const MemberArgs* __restric
     const int c = int(A.g.at(cl.i, cl.j, cl.k));
     S u = P<S>(f.u2)[c], v = P<S>(f.v2)[c], w = P<S>(f.w2)[c], th = P<S>(f.rp2)[c];
     const S qv = P<S>(f.mfx)[c];
-    sgs_eos<S>(A, c, cl.k, th);
-    if (A.p.sponge)
+    if (A.p.dif_apply == 0) sgs_eos<S>(A, c, cl.k, th);
+    if (A.p.sponge && A.p.dif_apply == 0)
     {
         const SpongeAt z = sponge_at(A, cl.i, cl.j, cl.k);
         sponge_wind<S>(z, u, v, w);
@@ -2683,7 +2685,7 @@ __global__ __launch_bounds__(kBlock) void k_sponge(const MemberArgs* __restrict_
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
-    if (!A.p.sponge || A.p.sgs_n > 0) return;
+    if (!A.p.sponge || A.p.sgs_n > 0 || A.p.dif_apply != 0) return;
     const Cell cl = cell_of(A, wk.local);
     if (!cl.valid) return;
     const SpongeAt z = sponge_at(A, cl.i, cl.j, cl.k);
@@ -2701,6 +2703,142 @@ __global__ __launch_bounds__(kBlock) void k_sponge(const MemberArgs* __restrict_
     v[c] = vc;
     w[c] = wc;
     th[c] = tc;
+}
+
+// ---------------------------------------------------------------------
+// Explicit Laplacian diffusion after the SGS (diffusion_step.cpp:679-1030;
+// equations of explicit.cpp:52-87,119-194,263-339,1183-1433 on the C-grid
+// control volumes): one pass writes the tendencies of the state (u, v, w into
+// the dead Heun stage-1 slots, theta and qv into two dead face-flux slots), a
+// second commits field + dt tendency with the storage clamps, then the EOS
+// projection and the sponges, which the microphysics or SGS pass leaves to it.
+// FP64 arithmetic; the vertical part is rounded to storage before the
+// horizontal part is added, as the reference adds its two passes.
+// ---------------------------------------------------------------------
+template <class S>
+__device__ double dif_tendency(const MemberArgs& A, gptr<const S> fld, int loc, int i, int j, int k)
+{
+#pragma clang fp contract(off)
+    const DevGrid& g = A.g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool px = A.p.dif_wrap_x != 0, py = A.p.dif_wrap_y != 0;
+    int i0 = 1, i1 = nx - 1, j0 = 1, j1 = ny - 1, k0 = 0, k1 = loc == 0 ? nz : nz - 1;
+    if (loc == 1) i1 = px ? nx - 1 : nx - 2;
+    if (loc == 2) j1 = py ? ny - 1 : ny - 2;
+    if (i < i0 || i >= i1 || j < j0 || j >= j1 || k < k0 || k >= k1) return 0.0;
+    const gptr<const double> rho0 = slot(A.f.base, kBaseRho0, nz);
+    const gptr<const S> rp = P<S>(A.f.rp);
+    auto rho = [&](int a, int b, int e) { return rho0[e] + double(rp[g.at(a, b, e)]); };
+    auto q = [&](int a, int b, int e) { return double(fld[g.at(a, b, e)]); };
+    const double dz = g.dz;
+    auto comp_rho = [&](int a, int b, int e) {
+        if (loc == 0) return rho(a, b, e);
+        if (loc == 1) return 0.5 * (rho(a, b, e) + rho(px && a == nx - 2 ? 1 : a + 1, b, e));
+        if (loc == 2) return 0.5 * (rho(a, b, e) + rho(a, py && b == ny - 2 ? 1 : b + 1, e));
+        return (rho(a, b, e) * dz + rho(a, b, e + 1) * dz) / (dz + dz);
+    };
+    const double qc = q(i, j, k);
+    S t = S(0);
+    const double kv = A.p.dif_kv;
+    if (kv > 0.0)
+    {
+        auto cell_tr = [&](int a, int b, int e) { return rho(a, b, e) * kv; };
+        auto comp_tr = [&](int a, int b, int e) {
+            if (loc == 1) return 0.5 * (cell_tr(a, b, e) + cell_tr(px && a == nx - 2 ? 1 : a + 1, b, e));
+            if (loc == 2) return 0.5 * (cell_tr(a, b, e) + cell_tr(a, py && b == ny - 2 ? 1 : b + 1, e));
+            return cell_tr(a, b, e);
+        };
+        const bool wc = loc == 3;
+        auto face_tr = [&](int lo) { return wc ? cell_tr(i, j, lo + 1) : 0.5 * (comp_tr(i, j, lo) + comp_tr(i, j, lo + 1)); };
+        const double dist = wc ? dz : 0.5 * (dz + dz);
+        const double mass = wc ? 0.5 * (rho(i, j, k) * dz + rho(i, j, k + 1) * dz) : comp_rho(i, j, k) * dz;
+        double down = 0.0, up = 0.0;
+        if (k > k0) down = face_tr(k - 1) * (qc - q(i, j, k - 1)) / dist;
+        if (k < k1 - 1) up = face_tr(k) * (q(i, j, k + 1) - qc) / dist;
+        t = S(double(t) + (up - down) / mass);
+    }
+    const double kh = A.p.dif_kh;
+    if (kh > 0.0)
+    {
+        const double idx2 = 1.0 / (g.dx * g.dx), idy2 = 1.0 / (g.dy * g.dy);
+        const bool wcl = i == i0 && !px, ecl = i == i1 - 1 && !px;
+        const bool scl = j == j0 && !py, ncl = j == j1 - 1 && !py;
+        const int iw = i == i0 && px ? i1 - 1 : i - 1, ie = i == i1 - 1 && px ? i0 : i + 1;
+        const int js = j == j0 && py ? j1 - 1 : j - 1, jn = j == j1 - 1 && py ? j0 : j + 1;
+        const double rc = comp_rho(i, j, k);
+        const double rw = wcl ? 0.0 : 0.5 * (rc + comp_rho(iw, j, k));
+        const double re = ecl ? 0.0 : 0.5 * (rc + comp_rho(ie, j, k));
+        const double rs = scl ? 0.0 : 0.5 * (rc + comp_rho(i, js, k));
+        const double rn = ncl ? 0.0 : 0.5 * (rc + comp_rho(i, jn, k));
+        const double xd = ((ecl ? 0.0 : re * (q(ie, j, k) - qc)) - (wcl ? 0.0 : rw * (qc - q(iw, j, k)))) * idx2;
+        const double yd = ((ncl ? 0.0 : rn * (q(i, jn, k) - qc)) - (scl ? 0.0 : rs * (qc - q(i, js, k)))) * idy2;
+        t = S(double(t) + kh * (xd + yd) / rc);
+    }
+    return double(t);
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_diffusion_tend(const MemberArgs* __restrict__ ma,
+                                                           const int2* __restrict__ map)
+{
+#pragma clang fp contract(off)
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (A.p.dif_apply == 0) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const DevFields& f = A.f;
+    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
+    if (A.p.dif_apply & NIMBUS_DIFFUSE_MOMENTUM)
+    {
+        P<S>(f.k1u)[c] = S(dif_tendency<S>(A, P<S>(f.u), 1, cl.i, cl.j, cl.k));
+        P<S>(f.k1v)[c] = S(dif_tendency<S>(A, P<S>(f.v), 2, cl.i, cl.j, cl.k));
+        P<S>(f.k1w)[c] = S(dif_tendency<S>(A, P<S>(f.w), 3, cl.i, cl.j, cl.k));
+    }
+    if (A.p.dif_apply & NIMBUS_DIFFUSE_SCALARS)
+    {
+        P<S>(f.flux[7])[c] = S(dif_tendency<S>(A, P<S>(f.th), 0, cl.i, cl.j, cl.k));
+        P<S>(f.flux[8])[c] = S(dif_tendency<S>(A, P<S>(f.qv), 0, cl.i, cl.j, cl.k));
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_diffusion_commit(const MemberArgs* __restrict__ ma,
+                                                             const int2* __restrict__ map)
+{
+#pragma clang fp contract(off)
+    const Work wk = work_of(map);
+    const MemberArgs& A = ma[wk.m];
+    if (A.p.dif_apply == 0) return;
+    const Cell cl = cell_of(A, wk.local);
+    if (!cl.valid) return;
+    const DevFields& f = A.f;
+    const std::size_t c = A.g.at(cl.i, cl.j, cl.k);
+    const Bounds& bd = A.p.bounds;
+    const double dt = A.p.dt;
+    S u = P<S>(f.u)[c], v = P<S>(f.v)[c], w = P<S>(f.w)[c], th = P<S>(f.th)[c];
+    if (A.p.dif_apply & NIMBUS_DIFFUSE_MOMENTUM)
+    {
+        u = S(clamp_store<S>(double(u) + double(P<S>(f.k1u)[c]) * dt, 0.0, -bd.wind_h, bd.wind_h));
+        v = S(clamp_store<S>(double(v) + double(P<S>(f.k1v)[c]) * dt, 0.0, -bd.wind_h, bd.wind_h));
+        w = S(clamp_store<S>(double(w) + double(P<S>(f.k1w)[c]) * dt, 0.0, -bd.wind_v, bd.wind_v));
+    }
+    if (A.p.dif_apply & NIMBUS_DIFFUSE_SCALARS)
+    {
+        th = S(clamp_store<S>(double(th) + double(P<S>(f.flux[7])[c]) * dt, 300.0, 200.0f, 700.0f));
+        P<S>(f.qv)[c] = S(clamp_store<S>(double(P<S>(f.qv)[c]) + double(P<S>(f.flux[8])[c]) * dt, 0.0, 0.0f, 0.05f));
+    }
+    sgs_eos<S>(A, int(c), cl.k, th);
+    if (A.p.sponge)
+    {
+        const SpongeAt z = sponge_at(A, cl.i, cl.j, cl.k);
+        sponge_wind<S>(z, u, v, w);
+        sponge_theta<S>(z, th);
+    }
+    P<S>(f.u)[c] = u;
+    P<S>(f.v)[c] = v;
+    P<S>(f.w)[c] = w;
+    P<S>(f.th)[c] = th;
 }
 
 // ---------------------------------------------------------------------
@@ -3020,6 +3158,11 @@ void launch_large_step(Launcher& L, const BatchView& b)
         launch_map(L, "sgs_scalars", k_sgs_scalars<S>, b, S_ALL, W_CELL, n);
     }
     if (b.any_odd_sgs_n) launch_map(L, "sgs_home", k_sgs_home<S>, b, S_ALL, W_CELL);
+    if (b.any_diffusion)
+    {
+        launch_map(L, "diffusion_tendencies", k_diffusion_tend<S>, b, S_ALL, W_CELL);
+        launch_map(L, "diffusion_commit", k_diffusion_commit<S>, b, S_ALL, W_CELL);
+    }
     if (b.any_sponge_unfused) launch_map(L, "sponge", k_sponge<S>, b, S_ALL, W_CELL);
     launch_boundaries<S>(L, b, -1);
 }

@@ -81,6 +81,12 @@ struct StepParams
     double sgs_nu_max, sgs_k_max;
     int sgs_wrap_x, sgs_wrap_y;   // periodic topology
     int sgs_open[4];              // west, east, south, north lateral faces open (radiative or zero-gradient)
+    // explicit diffusion after the SGS: NIMBUS_DIFFUSE_* bits (0 off); when on,
+    // the EOS projection and the sponges follow its commit instead of the
+    // microphysics or SGS pass
+    int dif_apply;
+    double dif_kh, dif_kv;
+    int dif_wrap_x, dif_wrap_y;
     KesslerConst kessler;
     Bounds bounds;
 };
@@ -176,7 +182,8 @@ struct BatchView
     // with an odd substep count (acoustic commit), an even microphysics
     // subcycle count (scalars home), an odd SGS substep count (SGS home)
     int any_odd_substeps = 0, any_even_kessler_sub = 0, any_odd_sgs_n = 0;
-    int any_sponge_unfused = 0;  // a member with sponges and no SGS pass to apply them in
+    int any_sponge_unfused = 0;  // a member with sponges and no SGS or diffusion pass to apply them in
+    int any_diffusion = 0;       // a member with explicit diffusion
     int any_kessler_adaptive = 0; // a member whose microphysics subcycles follow its sedimentation limit
     int any_hevi_seg = 0;        // HE-VI members solved in the partitioned form
     int any_hevi_serial = 0;     // HE-VI members solved by the sequential sweep

@@ -724,6 +724,143 @@ void enqueue_transport(const nimbus_conservative_cartesian_cgrid_transport_dispa
     }
 }
 
+// Explicit Laplacian diffusion tendency of one field (diffusion batch
+// contract, include/compute/dispatch/diffusion.hpp:29-69; equations of
+// explicit.cpp:52-87,119-194,263-339,1183-1433). One thread per cell writes
+// the cell's tendency: zero outside the field's control-volume domain,
+// otherwise the vertical part (rounded to FP32), then the horizontal part
+// added in FP64 and rounded again. Contraction is off so the FP64 operation
+// sequence is the reference's.
+struct DiffusionView
+{
+    const float *f, *rho, *dz, *kv;
+    float* out;
+    int nx, ny, nz, loc, c_grid, px, py;
+    double dx, dy, kh, kvc;
+};
+
+struct DiffDomain
+{
+    int i0, i1, j0, j1, k0, k1;
+};
+
+__host__ __device__ inline DiffDomain diff_domain(int nx, int ny, int nz, int loc, bool c_grid, bool px, bool py)
+{
+    if (!c_grid || loc == 0) return {1, nx - 1, 1, ny - 1, c_grid ? 0 : 1, c_grid ? nz : nz - 1};
+    if (loc == 1) return {1, px ? nx - 1 : nx - 2, 1, ny - 1, 0, nz - 1};
+    if (loc == 2) return {1, nx - 1, 1, py ? ny - 1 : ny - 2, 0, nz - 1};
+    return {1, nx - 1, 1, ny - 1, 0, nz - 1};
+}
+
+__global__ __launch_bounds__(kThreads) void k_ref_diffusion(DiffusionView v)
+{
+#pragma clang fp contract(off)
+    const std::size_t n = std::size_t(v.nx) * v.ny * v.nz;
+    const std::size_t c = std::size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (c >= n) return;
+    const int nx = v.nx, ny = v.ny, nz = v.nz, loc = v.loc;
+    const bool cg = v.c_grid != 0, px = v.px != 0, py = v.py != 0;
+    const int k = int(c % nz), j = int((c / nz) % ny), i = int(c / (std::size_t(nz) * ny));
+    const DiffDomain d = diff_domain(nx, ny, nz, loc, cg, px, py);
+    if (i < d.i0 || i >= d.i1 || j < d.j0 || j >= d.j1 || k < d.k0 || k >= d.k1)
+    {
+        v.out[c] = 0.0f;
+        return;
+    }
+    auto at = [=](int a, int b, int e) { return (std::size_t(a) * ny + b) * nz + e; };
+    auto rho = [&](int a, int b, int e) { return double(v.rho[at(a, b, e)]); };
+    auto dzk = [&](int e) { return double(v.dz[e]); };
+    auto fld = [&](int a, int b, int e) { return double(v.f[at(a, b, e)]); };
+    const bool stag = cg && loc != 0;
+    auto comp_rho = [&](int a, int b, int e) {
+        if (!stag) return rho(a, b, e);
+        if (loc == 1) return 0.5 * (rho(a, b, e) + rho(px && a == nx - 2 ? 1 : a + 1, b, e));
+        if (loc == 2) return 0.5 * (rho(a, b, e) + rho(a, py && b == ny - 2 ? 1 : b + 1, e));
+        return (rho(a, b, e) * dzk(e) + rho(a, b, e + 1) * dzk(e + 1)) / (dzk(e) + dzk(e + 1));
+    };
+    const double q = fld(i, j, k);
+    float t = 0.0f;
+    if (v.kvc > 0.0 || v.kv != nullptr)
+    {
+        auto kvat = [&](int a, int b, int e) { return v.kv ? double(v.kv[at(a, b, e)]) : v.kvc; };
+        auto cell_tr = [&](int a, int b, int e) { return rho(a, b, e) * kvat(a, b, e); };
+        auto comp_tr = [&](int a, int b, int e) {
+            if (!cg) return cell_tr(a, b, e);
+            if (loc == 1) return 0.5 * (cell_tr(a, b, e) + cell_tr(px && a == nx - 2 ? 1 : a + 1, b, e));
+            if (loc == 2) return 0.5 * (cell_tr(a, b, e) + cell_tr(a, py && b == ny - 2 ? 1 : b + 1, e));
+            if (loc == 3) return 0.5 * (cell_tr(a, b, e) + cell_tr(a, b, e + 1));
+            return cell_tr(a, b, e);
+        };
+        const bool wc = cg && loc == 3;
+        auto face_tr = [&](int lo) { return wc ? cell_tr(i, j, lo + 1) : 0.5 * (comp_tr(i, j, lo) + comp_tr(i, j, lo + 1)); };
+        auto dist = [&](int lo) { return wc ? dzk(lo + 1) : 0.5 * (dzk(lo) + dzk(lo + 1)); };
+        const double mass = wc ? 0.5 * (rho(i, j, k) * dzk(k) + rho(i, j, k + 1) * dzk(k + 1))
+                               : (cg ? comp_rho(i, j, k) : rho(i, j, k)) * dzk(k);
+        double down = 0.0, up = 0.0;
+        if (k > d.k0) down = face_tr(k - 1) * (q - fld(i, j, k - 1)) / dist(k - 1);
+        if (k < d.k1 - 1) up = face_tr(k) * (fld(i, j, k + 1) - q) / dist(k);
+        t = float(double(t) + (up - down) / mass);
+    }
+    if (v.kh > 0.0)
+    {
+        const double idx2 = 1.0 / (v.dx * v.dx), idy2 = 1.0 / (v.dy * v.dy);
+        const bool wcl = i == d.i0 && !px, ecl = i == d.i1 - 1 && !px;
+        const bool scl = j == d.j0 && !py, ncl = j == d.j1 - 1 && !py;
+        const int iw = i == d.i0 && px ? d.i1 - 1 : i - 1, ie = i == d.i1 - 1 && px ? d.i0 : i + 1;
+        const int js = j == d.j0 && py ? d.j1 - 1 : j - 1, jn = j == d.j1 - 1 && py ? d.j0 : j + 1;
+        const double rc = comp_rho(i, j, k);
+        const double rw = wcl ? 0.0 : 0.5 * (rc + comp_rho(iw, j, k));
+        const double re = ecl ? 0.0 : 0.5 * (rc + comp_rho(ie, j, k));
+        const double rs = scl ? 0.0 : 0.5 * (rc + comp_rho(i, js, k));
+        const double rn = ncl ? 0.0 : 0.5 * (rc + comp_rho(i, jn, k));
+        const double xd = ((ecl ? 0.0 : re * (fld(ie, j, k) - q)) - (wcl ? 0.0 : rw * (q - fld(iw, j, k)))) * idx2;
+        const double yd = ((ncl ? 0.0 : rn * (fld(i, jn, k) - q)) - (scl ? 0.0 : rs * (q - fld(i, js, k)))) * idy2;
+        t = float(double(t) + v.kh * (xd + yd) / rc);
+    }
+    v.out[c] = t;
+}
+
+constexpr int kMaxDiffusionTargets = 16;
+
+// Host-side refusals of a diffusion batch (the reference validates the same
+// inputs before its device dispatch, explicit.cpp:89-116,859-1181).
+int validate_diffusion(const nimbus_diffusion_batch_dispatch& d)
+{
+    auto fin_pos = [](float x) { return std::isfinite(x) && x > 0.0f; };
+    if (!d.fields || !d.tendencies || !d.locations || !d.rho || !d.dz || d.count < 1 ||
+        d.count > kMaxDiffusionTargets || d.nx < 3 || d.ny < 3 || d.nz < 2 || !shape_ok(d.nx, d.ny, d.nz) ||
+        !fin_pos(d.dx) || !fin_pos(d.dy) || !std::isfinite(d.K_h) || d.K_h < 0.0f || !std::isfinite(d.K_v) ||
+        d.K_v < 0.0f)
+        return set_error(NIMBUS_EINVAL, "diffusion batch: invalid dispatch");
+    for (int k = 0; k < d.nz; ++k)
+        if (!fin_pos(d.dz[k])) return set_error(NIMBUS_EINVAL, "diffusion batch: layer thickness not positive");
+    const std::size_t ny = std::size_t(d.ny), nz = std::size_t(d.nz);
+    for (int i = 1; i < d.nx - 1; ++i)
+        for (int j = 1; j < d.ny - 1; ++j)
+            for (int k = 0; k < d.nz; ++k)
+                if (!fin_pos(d.rho[(i * ny + j) * nz + k]))
+                    return set_error(NIMBUS_EINVAL, "diffusion batch: density not positive and finite");
+    for (int t = 0; t < d.count; ++t)
+    {
+        if (!d.fields[t] || !d.tendencies[t] || d.locations[t] < 0 || d.locations[t] > 3)
+            return set_error(NIMBUS_EINVAL, "diffusion batch: invalid target");
+        const DiffDomain dm = diff_domain(d.nx, d.ny, d.nz, d.locations[t], d.c_grid != 0, d.periodic_x != 0,
+                                          d.periodic_y != 0);
+        const float* kv = d.vertical_K ? d.vertical_K[t] : nullptr;
+        for (int i = dm.i0; i < dm.i1; ++i)
+            for (int j = dm.j0; j < dm.j1; ++j)
+                for (int k = dm.k0; k < dm.k1; ++k)
+                {
+                    const std::size_t c = (i * ny + j) * nz + k;
+                    if (!std::isfinite(d.fields[t][c]))
+                        return set_error(NIMBUS_EINVAL, "diffusion batch: non-finite field value");
+                    if (kv && (!std::isfinite(kv[c]) || kv[c] < 0.0f))
+                        return set_error(NIMBUS_EINVAL, "diffusion batch: vertical diffusivity negative or non-finite");
+                }
+    }
+    return NIMBUS_OK;
+}
+
 // Device-resident scratch of the allocation-free transport entry point: one
 // array per GPU, held for the life of the process. The host-side name is a
 // stand-in, not a device address; kernels get the address that
@@ -1018,6 +1155,42 @@ int nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid(
     for (int q = 0; q < d->n_scalars && rc == NIMBUS_OK; ++q) rc = back(d->q_out[q], qout[q], n, s);
     if (rc == NIMBUS_OK) REF_TRY(hipStreamSynchronize(s));
     if (rc == NIMBUS_OK) count_dispatch(ctx, "advect_scalars_conservative_cartesian_cgrid");
+    return rc;
+}
+
+int nimbus_hip_dispatch_diffusion_batch(nimbus_hip_ctx* ctx, const nimbus_diffusion_batch_dispatch* d)
+{
+    if (!ctx || !d) return set_error(NIMBUS_EINVAL, "diffusion batch: null argument");
+    int rc = validate_diffusion(*d);
+    if (rc != NIMBUS_OK) return rc;
+    hipSetDevice(context_device(ctx));
+    hipStream_t s = context_stream(ctx);
+    const std::size_t n = std::size_t(d->nx) * d->ny * d->nz;
+    DeviceArrays A;
+    float *rho = nullptr, *dz = nullptr;
+    rc = A.take(d->rho, n, s, &rho);
+    if (rc == NIMBUS_OK) rc = A.take(d->dz, std::size_t(d->nz), s, &dz);
+    std::vector<float*> out(d->count, nullptr);
+    for (int t = 0; t < d->count && rc == NIMBUS_OK; ++t)
+    {
+        float *f = nullptr, *kv = nullptr;
+        rc = A.take(d->fields[t], n, s, &f);
+        const float* kvh = d->vertical_K ? d->vertical_K[t] : nullptr;
+        if (rc == NIMBUS_OK && kvh) rc = A.take(kvh, n, s, &kv);
+        if (rc == NIMBUS_OK) rc = A.take(nullptr, n, s, &out[t]);
+        if (rc != NIMBUS_OK) break;
+        DiffusionView v{f, rho, dz, kv, out[t], d->nx, d->ny, d->nz, d->locations[t], d->c_grid != 0 ? 1 : 0,
+                        d->periodic_x != 0 ? 1 : 0, d->periodic_y != 0 ? 1 : 0, double(d->dx), double(d->dy),
+                        double(d->K_h), double(d->K_v)};
+        hipLaunchKernelGGL(k_ref_diffusion, dim3(blocks_for(n)), dim3(kThreads), 0, s, v);
+    }
+    if (rc != NIMBUS_OK) return rc;
+    REF_TRY(hipGetLastError());
+    REF_TRY(hipStreamSynchronize(s));
+    // published only once every target has been computed
+    for (int t = 0; t < d->count && rc == NIMBUS_OK; ++t) rc = back(d->tendencies[t], out[t], n, s);
+    if (rc == NIMBUS_OK) REF_TRY(hipStreamSynchronize(s));
+    if (rc == NIMBUS_OK) count_dispatch(ctx, "diffusion_batch");
     return rc;
 }
 

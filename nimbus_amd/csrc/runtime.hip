@@ -147,7 +147,10 @@ bool valid_config(const nimbus_step_config& c)
                        std::isfinite(c.sgs_cs) && c.sgs_cs >= 0 && std::isfinite(c.sgs_pr_t) && c.sgs_pr_t > 0 &&
                        std::isfinite(c.sgs_sc_t) && c.sgs_sc_t > 0 && std::isfinite(c.sgs_nu_max) &&
                        c.sgs_nu_max >= 0 && std::isfinite(c.sgs_k_max) && c.sgs_k_max >= 0 &&
-                       std::isfinite(c.sgs_dt) && c.sgs_dt > 0 && c.dt / c.sgs_dt <= 1000.0));
+                       std::isfinite(c.sgs_dt) && c.sgs_dt > 0 && c.dt / c.sgs_dt <= 1000.0)) &&
+           c.diffusion >= 0 && c.diffusion <= (NIMBUS_DIFFUSE_SCALARS | NIMBUS_DIFFUSE_MOMENTUM) &&
+           (c.diffusion == 0 || (std::isfinite(c.diffusion_kh) && c.diffusion_kh >= 0.0 &&
+                                 std::isfinite(c.diffusion_kv) && c.diffusion_kv >= 0.0));
 }
 
 struct Member
@@ -495,6 +498,12 @@ int allocate_member(const nimbus_member_desc& desc, Member& m)
         for (int q = 0; q < 4; ++q)
             sp.sgs_open[q] = lat[q] == NIMBUS_BC_RADIATIVE || lat[q] == NIMBUS_BC_ZERO_GRADIENT;
     }
+    // explicit diffusion after the SGS: off when neither coefficient is positive
+    sp.dif_apply = c.diffusion != 0 && (c.diffusion_kh > 0.0 || c.diffusion_kv > 0.0) ? c.diffusion : 0;
+    sp.dif_kh = c.diffusion_kh;
+    sp.dif_kv = c.diffusion_kv;
+    sp.dif_wrap_x = c.bc_west == NIMBUS_BC_PERIODIC && c.bc_east == NIMBUS_BC_PERIODIC;
+    sp.dif_wrap_y = c.bc_south == NIMBUS_BC_PERIODIC && c.bc_north == NIMBUS_BC_PERIODIC;
     sp.bounds = Bounds{};
     set_geometry(m, 64, 0, 1);
     return NIMBUS_OK;
@@ -531,6 +540,7 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
     b.view.max_sgs_n = 0;
     b.view.any_odd_substeps = b.view.any_even_kessler_sub = b.view.any_odd_sgs_n = 0;
     b.view.any_sponge_unfused = 0;
+    b.view.any_diffusion = 0;
     bool remap = true;
     for (Member* m : ms)
     {
@@ -543,7 +553,8 @@ int build_batch(Batch& b, const std::vector<Member*>& ms)
         b.view.any_odd_substeps |= m->args.p.substeps & 1;
         b.view.any_even_kessler_sub |= (m->args.p.kessler_sub & 1) == 0;
         b.view.any_odd_sgs_n |= m->args.p.sgs_n & 1;
-        b.view.any_sponge_unfused |= m->args.p.sponge && m->args.p.sgs_n == 0;
+        b.view.any_sponge_unfused |= m->args.p.sponge && m->args.p.sgs_n == 0 && m->args.p.dif_apply == 0;
+        b.view.any_diffusion |= m->args.p.dif_apply != 0;
         b.view.any_kessler_adaptive |= m->args.p.kessler_adaptive;
         if (m->args.p.acoustic_mode == NIMBUS_ACOUSTIC_HEVI)
         {
@@ -1076,7 +1087,7 @@ int nimbus_hip_member_export_npy(nimbus_hip_ctx* ctx, uint32_t id, const char* d
 namespace
 {
 constexpr char kSnapMagic[8] = {'N', 'I', 'M', 'B', 'U', 'S', 'M', 'S'};
-constexpr std::uint32_t kSnapVersion = 2;
+constexpr std::uint32_t kSnapVersion = 3;
 
 struct SnapHead
 {

@@ -53,7 +53,11 @@ class StepConfig(ctypes.Structure):
                 ("sponge_lateral_cells", ctypes.c_int), ("sponge_lateral_alpha", ctypes.c_double),
                 ("sgs", ctypes.c_int), ("sgs_mode", ctypes.c_int), ("sgs_filter", ctypes.c_int),
                 ("sgs_cs", ctypes.c_double), ("sgs_pr_t", ctypes.c_double), ("sgs_sc_t", ctypes.c_double),
-                ("sgs_nu_max", ctypes.c_double), ("sgs_k_max", ctypes.c_double), ("sgs_dt", ctypes.c_double)]
+                ("sgs_nu_max", ctypes.c_double), ("sgs_k_max", ctypes.c_double), ("sgs_dt", ctypes.c_double),
+                ("diffusion", ctypes.c_int), ("diffusion_kh", ctypes.c_double), ("diffusion_kv", ctypes.c_double)]
+
+
+DIFFUSE_SCALARS, DIFFUSE_MOMENTUM = 1, 2
 
 
 class BaseState(ctypes.Structure):
@@ -226,6 +230,16 @@ class ConservativeTransportDispatch(ctypes.Structure):
                  ("weno", _i), ("preserve_positive_support", _i)])
 
 
+class DiffusionBatchDispatch(ctypes.Structure):
+    _fields_ = [("fields", ctypes.POINTER(_c_float_p)), ("tendencies", ctypes.POINTER(_c_float_p)),
+                ("locations", ctypes.POINTER(_i)), ("count", _i), ("rho", _c_float_p), ("dz", _c_float_p),
+                ("nx", _i), ("ny", _i), ("nz", _i), ("dx", _f), ("dy", _f), ("K_h", _f), ("K_v", _f),
+                ("c_grid", _i), ("periodic_x", _i), ("periodic_y", _i),
+                ("vertical_K", ctypes.POINTER(_c_float_p))]
+
+
+DIFFUSION_SCALAR, DIFFUSION_U_FACE, DIFFUSION_V_FACE, DIFFUSION_W_FACE = 0, 1, 2, 3
+
 REFERENCE_DISPATCH = {
     "nimbus_hip_dispatch_cartesian_cgrid_slow_tendencies": CgridSlowTendenciesDispatch,
     "nimbus_hip_dispatch_slow_apply": SlowApplyDispatch,
@@ -234,6 +248,7 @@ REFERENCE_DISPATCH = {
     "nimbus_hip_dispatch_kessler_pointwise": KesslerPointwiseDispatch,
     "nimbus_hip_dispatch_kessler_sedimentation": KesslerSedimentationDispatch,
     "nimbus_hip_dispatch_advect_scalars_conservative_cartesian_cgrid": ConservativeTransportDispatch,
+    "nimbus_hip_dispatch_diffusion_batch": DiffusionBatchDispatch,
 }
 
 _LIB = None
@@ -353,10 +368,16 @@ def make_grid(nx, ny, nz, dx, dy, dz):
 
 def step_config(dt, substeps, acoustic_mode=ACOUSTIC_EXPLICIT, hevi_beta=0.1, coriolis_f=0.0,
                 lateral=BC_RADIATIVE, bottom=BOTTOM_FREE_SLIP, top=TOP_RIGID,
-                qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None, sgs=None):
+                qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None, sgs=None,
+                diffusion=None):
     """Large-step configuration. `sponge` is None (off) or a dict of the
     Rayleigh sponge settings (see sponge_preset); `sgs` is None (off) or a
-    dict of Smagorinsky settings (see sgs_preset)."""
+    dict of Smagorinsky settings (see sgs_preset); `diffusion` is None (off)
+    or {"apply": "scalars" | "momentum" | "all", "kh": K_h, "kv": K_v} for the
+    explicit Laplacian diffusion (m^2/s)."""
+    dif = diffusion or {}
+    dcode = {"scalars": DIFFUSE_SCALARS, "momentum": DIFFUSE_MOMENTUM,
+             "all": DIFFUSE_SCALARS | DIFFUSE_MOMENTUM}[dif.get("apply", "all")] if diffusion else 0
     lat = lateral if isinstance(lateral, (tuple, list)) else (lateral,) * 4
     sp = sponge or {}
     sg = sgs or {}
@@ -368,7 +389,8 @@ def step_config(dt, substeps, acoustic_mode=ACOUSTIC_EXPLICIT, hevi_beta=0.1, co
                       int(sp.get("lateral_cells", 0)), float(sp.get("lateral_alpha", 0.05)),
                       int(bool(sgs)), int(sg.get("mode", SGS_3D)), int(sg.get("filter", SGS_FILTER_CUBIC_ROOT)),
                       float(sg.get("cs", 0.18)), float(sg.get("pr_t", 1.0 / 3.0)), float(sg.get("sc_t", 1.0 / 3.0)),
-                      float(sg.get("nu_max", 1000.0)), float(sg.get("k_max", 1000.0)), float(sg.get("dt", 1.0)))
+                      float(sg.get("nu_max", 1000.0)), float(sg.get("k_max", 1000.0)), float(sg.get("dt", 1.0)),
+                      dcode, float(dif.get("kh", 0.0)), float(dif.get("kv", 0.0)))
 
 
 def sponge_preset(dx):

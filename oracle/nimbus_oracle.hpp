@@ -115,6 +115,11 @@ struct Config
     int sgs_filter = kFilterCubicRoot;
     double sgs_cs = 0.18, sgs_pr_t = 1.0 / 3.0, sgs_sc_t = 1.0 / 3.0;
     double sgs_nu_max = 1000.0, sgs_k_max = 1000.0, sgs_dt = 1.0;
+    // explicit Laplacian diffusion after the SGS (diffusion_step.cpp:679-1030;
+    // DiffusionConfig, include/numerics/diffusion/diffusion_base.hpp:23-40):
+    // 0 off, 1 scalars (theta, qv), 2 momentum, 3 both; K_h, K_v in m^2/s
+    int diffusion = 0;
+    double diffusion_kh = 0.0, diffusion_kv = 0.0;
 };
 
 template <class S>
@@ -1739,6 +1744,9 @@ void eos_projection(State<S>& s)
 // the final BC closure.
 // ----------------------------------------------------------------------
 template <class S>
+void apply_diffusion(const Grid& g, const Config& cfg, State<S>& s);
+
+template <class S>
 int large_step(const Grid& g, const Base& b, const Config& cfg, State<S>& s)
 {
     const int N = resolve_substeps(g, b, cfg);
@@ -1777,10 +1785,160 @@ int large_step(const Grid& g, const Base& b, const Config& cfg, State<S>& s)
     s.qr.swap(tmp);
     microphysics_step(g, cfg, s, cfg.dt);
     apply_sgs(g, cfg, s);
+    apply_diffusion(g, cfg, s);
     eos_projection(s);
     apply_sponges(g, b, cfg, s, cfg.dt);
     apply_boundaries(g, cfg, s, 0.0);
     return N;
+}
+
+// ---- explicit Laplacian diffusion tendency of one field
+// (src/numerics/diffusion/schemes/explicit/explicit.cpp: control-volume
+// domains :52-87, horizontal component :119-194, vertical component :263-339,
+// scalar and momentum wrappers :1183-1433; the device contract
+// include/compute/dispatch/diffusion.hpp:29-69). loc: 0 scalar, 1 u face,
+// 2 v face, 3 w face. Every cell of `tend` is written: zero outside the
+// field's control-volume domain; inside, the vertical part (when K_v > 0 or
+// kv is given) rounded to the storage type, then the horizontal part (K_h > 0)
+// added in FP64 and rounded again, as the reference adds the two passes. ----
+struct DiffusionDomain
+{
+    int i0, i1, j0, j1, k0, k1;
+};
+
+inline DiffusionDomain diffusion_domain(int nx, int ny, int nz, int loc, bool c_grid, bool px, bool py)
+{
+    if (!c_grid || loc == 0) return {1, nx - 1, 1, ny - 1, c_grid ? 0 : 1, c_grid ? nz : nz - 1};
+    if (loc == 1) return {1, px ? nx - 1 : nx - 2, 1, ny - 1, 0, nz - 1};
+    if (loc == 2) return {1, nx - 1, 1, py ? ny - 1 : ny - 2, 0, nz - 1};
+    return {1, nx - 1, 1, ny - 1, 0, nz - 1};
+}
+
+template <class S>
+void diffusion_tendency(int nx, int ny, int nz, double dx, double dy, const double* dz, const S* field,
+                        const S* rho, const S* kv, double K_h, double K_v, int loc, bool c_grid, bool px, bool py,
+                        S* tend)
+{
+    const auto at = [=](int i, int j, int k) { return (std::size_t(i) * ny + j) * nz + k; };
+    const std::size_t n = std::size_t(nx) * ny * nz;
+    for (std::size_t c = 0; c < n; ++c) tend[c] = S(0);
+    const DiffusionDomain d = diffusion_domain(nx, ny, nz, loc, c_grid, px, py);
+    if (d.i0 >= d.i1 || d.j0 >= d.j1 || d.k0 >= d.k1) return;
+    const bool stag = c_grid && loc != 0;
+    const auto rho_at = [&](int i, int j, int k) { return double(rho[at(i, j, k)]); };
+    const auto kv_at = [&](int i, int j, int k) { return kv ? double(kv[at(i, j, k)]) : K_v; };
+    // density of the field's control volume (u and v: the two cells it joins,
+    // the partner wrapping to column / row 1 on a periodic seam; w: the two
+    // cells weighted by their thicknesses)
+    const auto comp_rho = [&](int i, int j, int k) {
+        if (!stag) return rho_at(i, j, k);
+        if (loc == 1) return 0.5 * (rho_at(i, j, k) + rho_at(px && i == nx - 2 ? 1 : i + 1, j, k));
+        if (loc == 2) return 0.5 * (rho_at(i, j, k) + rho_at(i, py && j == ny - 2 ? 1 : j + 1, k));
+        return (rho_at(i, j, k) * dz[k] + rho_at(i, j, k + 1) * dz[k + 1]) / (dz[k] + dz[k + 1]);
+    };
+    // vertical component
+    if (K_v > 0.0 || kv != nullptr)
+    {
+        const auto cell_tr = [&](int i, int j, int k) { return rho_at(i, j, k) * kv_at(i, j, k); };
+        const auto comp_tr = [&](int i, int j, int k) {
+            if (!c_grid) return cell_tr(i, j, k);
+            if (loc == 1) return 0.5 * (cell_tr(i, j, k) + cell_tr(px && i == nx - 2 ? 1 : i + 1, j, k));
+            if (loc == 2) return 0.5 * (cell_tr(i, j, k) + cell_tr(i, py && j == ny - 2 ? 1 : j + 1, k));
+            if (loc == 3) return 0.5 * (cell_tr(i, j, k) + cell_tr(i, j, k + 1));
+            return cell_tr(i, j, k);
+        };
+        const bool wc = c_grid && loc == 3;
+        // transport through the face above level lo, the distance between the
+        // two points it joins and the control mass of level k
+        const auto face_tr = [&](int i, int j, int lo) {
+            return wc ? cell_tr(i, j, lo + 1) : 0.5 * (comp_tr(i, j, lo) + comp_tr(i, j, lo + 1));
+        };
+        const auto dist = [&](int lo) { return wc ? dz[lo + 1] : 0.5 * (dz[lo] + dz[lo + 1]); };
+        const auto mass = [&](int i, int j, int k) {
+            if (wc) return 0.5 * (rho_at(i, j, k) * dz[k] + rho_at(i, j, k + 1) * dz[k + 1]);
+            const double r = c_grid ? comp_rho(i, j, k) : rho_at(i, j, k);
+            return r * dz[k];
+        };
+        for (int i = d.i0; i < d.i1; ++i)
+            for (int j = d.j0; j < d.j1; ++j)
+                for (int k = d.k0; k < d.k1; ++k)
+                {
+                    const double q = double(field[at(i, j, k)]);
+                    double down = 0.0, up = 0.0;
+                    if (k > d.k0) down = face_tr(i, j, k - 1) * (q - double(field[at(i, j, k - 1)])) / dist(k - 1);
+                    if (k < d.k1 - 1) up = face_tr(i, j, k) * (double(field[at(i, j, k + 1)]) - q) / dist(k);
+                    tend[at(i, j, k)] = S(double(tend[at(i, j, k)]) + (up - down) / mass(i, j, k));
+                }
+    }
+    // horizontal component, closed (zero flux) at the domain's lateral faces
+    // unless periodic, where the neighbour wraps inside the domain
+    if (K_h > 0.0)
+    {
+        const double idx2 = 1.0 / (dx * dx), idy2 = 1.0 / (dy * dy);
+        for (int i = d.i0; i < d.i1; ++i)
+            for (int j = d.j0; j < d.j1; ++j)
+                for (int k = d.k0; k < d.k1; ++k)
+                {
+                    const bool wcl = i == d.i0 && !px, ecl = i == d.i1 - 1 && !px;
+                    const bool scl = j == d.j0 && !py, ncl = j == d.j1 - 1 && !py;
+                    const int iw = i == d.i0 && px ? d.i1 - 1 : i - 1, ie = i == d.i1 - 1 && px ? d.i0 : i + 1;
+                    const int js = j == d.j0 && py ? d.j1 - 1 : j - 1, jn = j == d.j1 - 1 && py ? d.j0 : j + 1;
+                    const double rc = comp_rho(i, j, k);
+                    const double rw = wcl ? 0.0 : 0.5 * (rc + comp_rho(iw, j, k));
+                    const double re = ecl ? 0.0 : 0.5 * (rc + comp_rho(ie, j, k));
+                    const double rs = scl ? 0.0 : 0.5 * (rc + comp_rho(i, js, k));
+                    const double rn = ncl ? 0.0 : 0.5 * (rc + comp_rho(i, jn, k));
+                    const double q = double(field[at(i, j, k)]);
+                    const double xd = ((ecl ? 0.0 : re * (double(field[at(ie, j, k)]) - q)) -
+                                       (wcl ? 0.0 : rw * (q - double(field[at(iw, j, k)])))) *
+                                      idx2;
+                    const double yd = ((ncl ? 0.0 : rn * (double(field[at(i, jn, k)]) - q)) -
+                                       (scl ? 0.0 : rs * (q - double(field[at(i, js, k)])))) *
+                                      idy2;
+                    tend[at(i, j, k)] = S(double(tend[at(i, j, k)]) + K_h * (xd + yd) / rc);
+                }
+    }
+}
+
+// ---- explicit diffusion in the large step (diffusion_step.cpp:679-1030):
+// the tendencies of the state after the SGS, then every cell committed as
+// field + dt tendency rounded to the storage type and clamped like the
+// reference's commit (winds, theta, qv bounds of field_bounds.hpp). Sides are
+// periodic when both faces of an axis are. ----
+template <class S>
+void apply_diffusion(const Grid& g, const Config& cfg, State<S>& s)
+{
+    if (cfg.diffusion == 0 || (!(cfg.diffusion_kh > 0.0) && !(cfg.diffusion_kv > 0.0))) return;
+    const bool px = cfg.bc_west == kPeriodic && cfg.bc_east == kPeriodic;
+    const bool py = cfg.bc_south == kPeriodic && cfg.bc_north == kPeriodic;
+    const std::vector<double> dz(std::size_t(g.nz), g.dz);
+    const std::size_t n = g.cells();
+    auto tend = [&](const std::vector<S>& f, int loc) {
+        std::vector<S> t(n);
+        diffusion_tendency<S>(g.nx, g.ny, g.nz, g.dx, g.dy, dz.data(), f.data(), s.rho.data(), nullptr,
+                              cfg.diffusion_kh, cfg.diffusion_kv, loc, true, px, py, t.data());
+        return t;
+    };
+    if (cfg.diffusion & 2)
+    {
+        const std::vector<S> du = tend(s.u, 1), dv = tend(s.v, 2), dw = tend(s.w, 3);
+        for (std::size_t c = 0; c < n; ++c)
+        {
+            s.u[c] = S(clamp_store<S>(double(s.u[c]) + double(du[c]) * cfg.dt, 0.0, -kWindH, kWindH));
+            s.v[c] = S(clamp_store<S>(double(s.v[c]) + double(dv[c]) * cfg.dt, 0.0, -kWindH, kWindH));
+            s.w[c] = S(clamp_store<S>(double(s.w[c]) + double(dw[c]) * cfg.dt, 0.0, -kWindV, kWindV));
+        }
+    }
+    if (cfg.diffusion & 1)
+    {
+        const std::vector<S> dth = tend(s.theta, 0), dqv = tend(s.qv, 0);
+        for (std::size_t c = 0; c < n; ++c)
+        {
+            s.theta[c] =
+                S(clamp_store<S>(double(s.theta[c]) + double(dth[c]) * cfg.dt, 300.0, kThetaMin, kThetaMax));
+            s.qv[c] = S(clamp_store<S>(double(s.qv[c]) + double(dqv[c]) * cfg.dt, 0.0, 0.0f, kQvMax));
+        }
+    }
 }
 
 }  // namespace oracle

@@ -30,6 +30,8 @@ struct oc_config
     double sponge_lateral_alpha;
     int sgs, sgs_mode, sgs_filter;
     double sgs_cs, sgs_pr_t, sgs_sc_t, sgs_nu_max, sgs_k_max, sgs_dt;
+    int diffusion;
+    double diffusion_kh, diffusion_kv;
 };
 
 struct oc_base
@@ -69,6 +71,7 @@ Config to_config(const oc_config* c)
     k.sgs = c->sgs; k.sgs_mode = c->sgs_mode; k.sgs_filter = c->sgs_filter; k.sgs_cs = c->sgs_cs;
     k.sgs_pr_t = c->sgs_pr_t; k.sgs_sc_t = c->sgs_sc_t; k.sgs_nu_max = c->sgs_nu_max; k.sgs_k_max = c->sgs_k_max;
     k.sgs_dt = c->sgs_dt;
+    k.diffusion = c->diffusion; k.diffusion_kh = c->diffusion_kh; k.diffusion_kv = c->diffusion_kv;
     return k;
 }
 
@@ -494,6 +497,34 @@ void oracle_hevi_column_system(int nz, double dt, double dz, double alpha, const
                                double* U, double* R)
 {
     hevi_column_system(nz, dt, dz, alpha, rho_bar, C, G, e, w_old, A, D, U, R);
+}
+
+// The large step's explicit diffusion pass alone on a state (FP64 storage).
+int oracle_apply_diffusion_f64(const oc_grid* gp, const oc_config* cp, const oc_state* st)
+{
+    const Grid g = to_grid(gp);
+    State<double> s;
+    s.allocate(g);
+    load(g, st, s);
+    apply_diffusion(g, to_config(cp), s);
+    store(g, s, st);
+    return 0;
+}
+
+// Explicit diffusion tendency of one field (dz: nz layer thicknesses; kv may be null).
+void oracle_diffusion_tendency_f32(int nx, int ny, int nz, double dx, double dy, const double* dz, const float* field,
+                                   const float* rho, const float* kv, double kh, double kvc, int loc, int c_grid,
+                                   int px, int py, float* tend)
+{
+    diffusion_tendency<float>(nx, ny, nz, dx, dy, dz, field, rho, kv, kh, kvc, loc, c_grid != 0, px != 0, py != 0,
+                              tend);
+}
+void oracle_diffusion_tendency_f64(int nx, int ny, int nz, double dx, double dy, const double* dz, const double* field,
+                                   const double* rho, const double* kv, double kh, double kvc, int loc, int c_grid,
+                                   int px, int py, double* tend)
+{
+    diffusion_tendency<double>(nx, ny, nz, dx, dy, dz, field, rho, kv, kh, kvc, loc, c_grid != 0, px != 0, py != 0,
+                               tend);
 }
 
 // Thread count of the OpenMP loops for the rest of the process (n >= 1).

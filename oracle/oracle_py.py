@@ -33,7 +33,8 @@ class OcConfig(ctypes.Structure):
                 ("sponge_lateral_cells", ctypes.c_int), ("sponge_lateral_alpha", ctypes.c_double),
                 ("sgs", ctypes.c_int), ("sgs_mode", ctypes.c_int), ("sgs_filter", ctypes.c_int),
                 ("sgs_cs", ctypes.c_double), ("sgs_pr_t", ctypes.c_double), ("sgs_sc_t", ctypes.c_double),
-                ("sgs_nu_max", ctypes.c_double), ("sgs_k_max", ctypes.c_double), ("sgs_dt", ctypes.c_double)]
+                ("sgs_nu_max", ctypes.c_double), ("sgs_k_max", ctypes.c_double), ("sgs_dt", ctypes.c_double),
+                ("diffusion", ctypes.c_int), ("diffusion_kh", ctypes.c_double), ("diffusion_kv", ctypes.c_double)]
 
 
 class OcBase(ctypes.Structure):
@@ -101,6 +102,13 @@ def lib():
         L.oracle_thomas.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oracle_hevi_column_system.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        _i = ctypes.c_int
+        _d = ctypes.c_double
+        L.oracle_diffusion_tendency_f32.argtypes = [_i, _i, _i, _d, _d, _dp, _fp, _fp, _fp, _d, _d, _i, _i, _i, _i,
+                                                    _fp]
+        L.oracle_diffusion_tendency_f64.argtypes = [_i, _i, _i, _d, _d, _dp, _dp, _dp, _dp, _d, _d, _i, _i, _i, _i,
+                                                    _dp]
+        L.oracle_apply_diffusion_f64.argtypes = [P(OcGrid), P(OcConfig), P(OcState)]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_max_threads.restype = ctypes.c_int
         _LIB = L
@@ -119,7 +127,8 @@ def grid(nx, ny, nz, dx, dy, dz):
 
 
 def config(dt, substeps, acoustic_mode=0, hevi_beta=0.1, coriolis_f=0.0, lateral=1, bottom=0, top=0,
-           qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None, sgs=None):
+           qc0=1.0e-3, c_auto=1.0e-3, c_accr=2.2, evap_eff=1.0, vt_max=20.0, sponge=None, sgs=None,
+           diffusion=None):
     lat = lateral if isinstance(lateral, (tuple, list)) else (lateral,) * 4
     sp = sponge or {}
     sg = sgs or {}
@@ -130,7 +139,17 @@ def config(dt, substeps, acoustic_mode=0, hevi_beta=0.1, coriolis_f=0.0, lateral
                     float(sp.get("lateral_alpha", 0.05)),
                     int(bool(sgs)), int(sg.get("mode", 0)), int(sg.get("filter", 0)), float(sg.get("cs", 0.18)),
                     float(sg.get("pr_t", 1.0 / 3.0)), float(sg.get("sc_t", 1.0 / 3.0)), float(sg.get("nu_max", 1000.0)),
-                    float(sg.get("k_max", 1000.0)), float(sg.get("dt", 1.0)))
+                    float(sg.get("k_max", 1000.0)), float(sg.get("dt", 1.0)),
+                    *diffusion_fields(diffusion))
+
+
+def diffusion_fields(diffusion):
+    """(apply code, K_h, K_v) of an explicit-diffusion setting {"apply": "scalars" | "momentum" | "all",
+    "kh": .., "kv": ..}, or (0, 0, 0) for None."""
+    if not diffusion:
+        return 0, 0.0, 0.0
+    code = {"scalars": 1, "momentum": 2, "all": 3}[diffusion.get("apply", "all")]
+    return code, float(diffusion.get("kh", 0.0)), float(diffusion.get("kv", 0.0))
 
 
 def from_step_config(c):
@@ -300,3 +319,30 @@ def saturation_adjust(T, p, qv, qc):
     qc_ = ctypes.c_double(qc)
     Ta = lib().oracle_saturation_adjust(T, p, ctypes.byref(qv_), ctypes.byref(qc_))
     return Ta, qv_.value, qc_.value
+
+
+def diffusion_tendency(field, rho, dz, dx, dy, K_h, K_v, location, c_grid=True, periodic_x=False, periodic_y=False,
+                       kv=None, dtype=np.float32):
+    """Explicit diffusion tendency d(field)/dt of one [nx][ny][nz] field
+    (location 0 scalar, 1 u face, 2 v face, 3 w face); kv: optional per-cell
+    vertical diffusivity. Arithmetic FP64; storage `dtype`."""
+    nx, ny, nz = field.shape
+    f = np.ascontiguousarray(field, dtype)
+    r = np.ascontiguousarray(rho, dtype)
+    k = None if kv is None else np.ascontiguousarray(kv, dtype)
+    dzv = np.ascontiguousarray(dz, np.float64)
+    out = np.empty_like(f)
+    name = "oracle_diffusion_tendency_f32" if dtype == np.float32 else "oracle_diffusion_tendency_f64"
+    ptr = _fp if dtype == np.float32 else _dp
+    getattr(lib(), name)(nx, ny, nz, float(dx), float(dy), dzv.ctypes.data_as(_dp), f.ctypes.data_as(ptr),
+                         r.ctypes.data_as(ptr), None if k is None else k.ctypes.data_as(ptr), float(K_h), float(K_v),
+                         int(location), int(bool(c_grid)), int(bool(periodic_x)), int(bool(periodic_y)),
+                         out.ctypes.data_as(ptr))
+    return out
+
+
+def apply_diffusion(g, cfg, st):
+    """The large step's explicit diffusion pass alone on a copy of st (FP64)."""
+    s = copy_state(st, np.float64)
+    lib().oracle_apply_diffusion_f64(ctypes.byref(g), ctypes.byref(cfg), ctypes.byref(_state_struct(s)))
+    return s

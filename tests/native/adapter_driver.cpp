@@ -14,7 +14,8 @@ int main()
 {
     std::unique_ptr<ComputeBackend> b = create_hip_compute_backend();
     if (!b || b->name() != "hip") return 1;
-    if (b->dynamics_kernels() || b->advection_kernels() || b->microphysics_kernels()) return 2;
+    if (b->dynamics_kernels() || b->advection_kernels() || b->microphysics_kernels() || b->diffusion_kernels())
+        return 2;
     ComputeBackendConfig cfg;
     cfg.backend = "hip";
     cfg.device_index = 0;
@@ -24,7 +25,8 @@ int main()
     if (ok)
     {
         if (!b->dynamics_kernels() ||
-            !b->dynamics_kernels()->supports(DynamicsKernelOp::CartesianCgridSlowTendencies))
+            !b->dynamics_kernels()->supports(DynamicsKernelOp::CartesianCgridSlowTendencies) ||
+            !b->diffusion_kernels() || !b->diffusion_kernels()->supports(DiffusionKernelOp::Diffusion))
             return 3;
         std::printf("initialized\n");
     }

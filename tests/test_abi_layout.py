@@ -30,6 +30,7 @@ STRUCTS = {
     "nimbus_kessler_pointwise_dispatch": L.KesslerPointwiseDispatch,
     "nimbus_kessler_sedimentation_dispatch": L.KesslerSedimentationDispatch,
     "nimbus_conservative_cartesian_cgrid_transport_dispatch": L.ConservativeTransportDispatch,
+    "nimbus_diffusion_batch_dispatch": L.DiffusionBatchDispatch,
 }
 
 
