@@ -7,12 +7,15 @@ cell-updates/s of the mixed-precision path (FP32 bulk in perturbation form,
 FP64 base state and FP64 vertically implicit w-p' Thomas solve), hipGraph
 captured, state resident in HBM; `fp64_value` is the plain all-FP64 path
 (explicit acoustic substeps, the reference method) on the same grid. Also
-reported: the other precision/method combinations, the 32-member uneven
-ensemble run as one batch, and a short continuous-batching serving run with
-p50/p99 completion latency. Prints one JSON line.
+reported: the other precision/method combinations, the operators beside the
+step (the step with explicit diffusion on, member snapshot/restore and NPY
+export, CM1 sounding I/O), the 32-member uneven ensemble run as one batch, and
+a short continuous-batching serving run with p50/p99 completion latency.
+Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
                   [--serving-seconds S] [--no-ensemble] [--accuracy c2,c1|''] [--no-cpu-baseline]
+                  [--no-next-rows]
 """
 import argparse
 import json
@@ -120,6 +123,11 @@ ALGO_PER_CELL = {
     "sgs_momentum": (8, 0),
     # read theta qv rho' K_theta K_q; write theta qv (p' of the last substep not counted)
     "sgs_scalars": (7, 0),
+    # explicit Laplacian diffusion of u v w theta qv: read the five fields and
+    # rho'; write the five tendencies
+    "diffusion_tendencies": (11, 0),
+    # read u v w theta qv, the five tendencies and rho'; write u v w theta qv p'
+    "diffusion_commit": (17, 0),
 }
 
 
@@ -153,9 +161,14 @@ def cpu_info():
     return info
 
 
-def run_member(ctx, L, wl, precision, mode, steps, warmup, graphs, column_solve=None):
+# explicit Laplacian diffusion timed beside the step (its coefficients are zero
+# in the reference's presets): u, v, w, theta and qv, K_h 50 and K_v 10 m^2/s
+DIFFUSION_TIMED = {"apply": "all", "kh": 50.0, "kv": 10.0}
+
+
+def run_member(ctx, L, wl, precision, mode, steps, warmup, graphs, column_solve=None, diffusion=None):
     dtype = np.float32 if precision == L.PRECISION_MIXED else np.float64
-    cfg = wl.config(acoustic_mode=mode)
+    cfg = wl.config(acoustic_mode=mode, diffusion=diffusion)
     m = ctx.member(wl.grid, cfg, wl.base(), precision)
     if column_solve is not None:
         m.set_column_solve(column_solve)
@@ -369,6 +382,82 @@ def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5, tuning=None, co
     return s
 
 
+def next_rows_run(ctx, L, wl, steps, warmup, graphs, base_ms):
+    """The operators beside the large step, at the same workload: the mixed
+    HE-VI step with the explicit Laplacian diffusion of u, v, w, theta and qv on
+    (K_h 50, K_v 10 m^2/s) and its two kernels against their own reads and
+    writes; the member snapshot (device to host), restore (host to device) and
+    NPY field export; the CM1 sounding text written, read and turned into the
+    base state of the grid."""
+    import shutil
+    import tempfile
+    out = {}
+    m, ms, _ = run_member(ctx, L, wl, L.PRECISION_MIXED, L.ACOUSTIC_HEVI, steps, warmup, graphs,
+                          diffusion=DIFFUSION_TIMED)
+    ms /= steps
+    tab = kernel_table(ctx, m, 2)
+    kern = {}
+    for name in ("diffusion_tendencies", "diffusion_commit"):
+        if name not in tab or tab[name][0] == 0:
+            continue
+        launches, kms = tab[name]
+        per_cell = algo_bytes_per_cell(name, "hevi", 4)
+        gbs = per_cell * wl.cells / (kms / launches / 1e3) / 1e9
+        kern[name] = {"launches_per_step": launches / 2, "avg_launch_ms": round(kms / launches, 4),
+                      "bytes_per_cell": per_cell, "achieved_GBps": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["diffusion"] = {"workload": wl.describe(),
+                        "coefficients": {"K_h": DIFFUSION_TIMED["kh"], "K_v": DIFFUSION_TIMED["kv"]},
+                        "fields": "u v w theta qv", "ms_per_step": round(ms, 4),
+                        "ms_per_step_without": round(base_ms, 4),
+                        "cell_updates_per_s": wl.cells / (ms / 1e3), "kernels": kern}
+
+    t0 = time.perf_counter()
+    blob = m.snapshot()
+    t_snap = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m2 = ctx.restore(blob)
+    t_rest = time.perf_counter() - t0
+    same = m2.snapshot() == blob
+    m2.destroy()
+    d = tempfile.mkdtemp(prefix="nimbus_npy_")
+    try:
+        t0 = time.perf_counter()
+        m.export_npy(d)
+        t_npy = time.perf_counter() - t0
+        npy_bytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    m.destroy()
+    out["state_io"] = {"snapshot_bytes": len(blob), "snapshot_s": round(t_snap, 4),
+                       "snapshot_GBps": round(len(blob) / t_snap / 1e9, 2), "restore_s": round(t_rest, 4),
+                       "restore_GBps": round(len(blob) / t_rest / 1e9, 2), "restored_snapshot_identical": same,
+                       "npy_export_bytes": npy_bytes, "npy_export_s": round(t_npy, 4),
+                       "npy_export_GBps": round(npy_bytes / t_npy / 1e9, 2),
+                       "note": "host wall time; snapshot into pageable host memory, NPY files written to /tmp"}
+
+    g = wl.grid
+    z = np.concatenate([[0.0], (np.arange(g.nz) + 0.5) * g.dz, [g.nz * g.dz]])
+    levels = L.sounding_levels(z, wl.sounding_struct())
+    reps = 200
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        text = L.cm1_format(levels)
+    t_fmt = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        col = L.cm1_parse(text)
+    t_parse = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.cm1_base_state(wl.grid, col)
+    t_base = (time.perf_counter() - t0) / reps
+    out["sounding_io"] = {"levels": len(col["z"]), "text_bytes": len(text), "write_ms": round(t_fmt * 1e3, 4),
+                          "read_ms": round(t_parse * 1e3, 4), "base_state_ms": round(t_base * 1e3, 4),
+                          "note": f"host, mean of {reps} calls through the C ABI"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -398,8 +487,12 @@ def main():
                     help="only one member in this mode: warm-up + timed steps, for rocprofv3 passes")
     ap.add_argument("--profile-precision", choices=["mixed", "fp64"], default="mixed",
                     help="storage of the --profile member")
+    ap.add_argument("--profile-diffusion", action="store_true",
+                    help="with --profile: the explicit diffusion of u, v, w, theta, qv on (K_h 50, K_v 10 m^2/s)")
     ap.add_argument("--profile-column-solve", choices=["auto", "sequential", "partitioned"], default="auto",
                     help="HE-VI column-solve form of the --profile member (auto: the member's default)")
+    ap.add_argument("--no-next-rows", action="store_true",
+                    help="skip the diffusion, state I/O and sounding I/O timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -443,9 +536,11 @@ def main():
     if args.profile:
         prec = L.PRECISION_FP64 if args.profile_precision == "fp64" else L.PRECISION_MIXED
         m, ms, _ = run_member(ctx, L, wl, prec, modes[args.profile], args.steps, args.warmup, graphs,
-                              column_solve_code(L, args.profile_column_solve))
+                              column_solve_code(L, args.profile_column_solve),
+                              DIFFUSION_TIMED if args.profile_diffusion else None)
         print(json.dumps({"profile": True, "workload": wl.describe(), "mode": args.profile,
                           "precision": args.profile_precision, "column_solve": m.column_solve(),
+                          "diffusion": DIFFUSION_TIMED if args.profile_diffusion else None,
                           "steps": args.steps, "ms_per_step": ms / args.steps}))
         m.destroy()
         ctx.close()
@@ -528,6 +623,9 @@ def main():
         "cpu_baseline": None,
         "host": {"platform": platform.platform(), **cpu_info()},
     }
+    if not args.no_next_rows:
+        progress("beside the step: diffusion, state I/O, sounding I/O")
+        result["next_rows"] = next_rows_run(ctx, L, wl, min(args.steps, 10), args.warmup, graphs, ms_per_step)
     if not args.no_ensemble:
         progress("ensemble: 32 members, 30 simulated minutes each")
         result["ensemble_c3"] = ensemble_run(ctx, L, W, L.PRECISION_MIXED, L.ACOUSTIC_HEVI, tuning, col)

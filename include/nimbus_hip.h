@@ -323,8 +323,8 @@ int nimbus_hip_member_export_npy(nimbus_hip_ctx* ctx, uint32_t id, const char* d
 /* Member snapshot for eviction and resume: the description (grid, step
  * configuration, precision, launch tiling), the six base profiles, the step
  * count, and the prognostic fields and surface rain in their device storage
- * form (perturbation rho' and p', storage precision), with an FNV-1a 64
- * checksum. Restoring it and stepping on gives the results of never having
+ * form (perturbation rho' and p', storage precision), with a 64-bit
+ * checksum (FNV-1a steps over 64-bit words in four lanes). Restoring it and stepping on gives the results of never having
  * stopped, bit for bit. The role of the reference's simulation checkpoint
  * (src/core/runtime/simulation_checkpoint.cpp, --resume / --checkpoint-every;
  * wire helpers include/core/runtime/checkpoint_format.hpp:22-60), not its file

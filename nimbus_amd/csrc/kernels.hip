@@ -2715,8 +2715,14 @@ __global__ __launch_bounds__(kBlock) void k_sponge(const MemberArgs* __restrict_
 // FP64 arithmetic; the vertical part is rounded to storage before the
 // horizontal part is added, as the reference adds its two passes.
 // ---------------------------------------------------------------------
+// Addresses are offsets from the cell's own element (neighbour columns and
+// rows precomputed once, with the periodic wrap), and every operand is loaded
+// before the arithmetic: neighbours that a closed edge or the lid leaves out
+// are read at clamped, in-domain positions and their terms selected away.
+// The arithmetic is the reference's expression for expression.
 template <class S>
-__device__ double dif_tendency(const MemberArgs& A, gptr<const S> fld, int loc, int i, int j, int k)
+__device__ __forceinline__ double dif_tendency(const MemberArgs& A, gptr<const S> fld, int loc, int i, int j,
+                                               int k)
 {
 #pragma clang fp contract(off)
     const DevGrid& g = A.g;
@@ -2726,52 +2732,80 @@ __device__ double dif_tendency(const MemberArgs& A, gptr<const S> fld, int loc, 
     if (loc == 1) i1 = px ? nx - 1 : nx - 2;
     if (loc == 2) j1 = py ? ny - 1 : ny - 2;
     if (i < i0 || i >= i1 || j < j0 || j >= j1 || k < k0 || k >= k1) return 0.0;
+    const std::ptrdiff_t sy = g.px, sz = std::ptrdiff_t(g.plane());
+    const std::size_t c = g.at(i, j, k);
     const gptr<const double> rho0 = slot(A.f.base, kBaseRho0, nz);
-    const gptr<const S> rp = P<S>(A.f.rp);
-    auto rho = [&](int a, int b, int e) { return rho0[e] + double(rp[g.at(a, b, e)]); };
-    auto q = [&](int a, int b, int e) { return double(fld[g.at(a, b, e)]); };
+    const gptr<const S> rp = P<S>(A.f.rp) + c;
+    const gptr<const S> fq = fld + c;
+    // neighbour columns (wrapped on periodic sides) and clamped levels
+    const int iw = i == i0 && px ? i1 - 1 : i - 1, ie = i == i1 - 1 && px ? i0 : i + 1;
+    const int js = j == j0 && py ? j1 - 1 : j - 1, jn = j == j1 - 1 && py ? j0 : j + 1;
+    const std::ptrdiff_t ow = iw - i, oe = ie - i, os = std::ptrdiff_t(js - j) * sy, on = std::ptrdiff_t(jn - j) * sy;
+    const int km = k > k0 ? k - 1 : k, kp = k < nz - 1 ? k + 1 : k;
+    const std::ptrdiff_t dm = std::ptrdiff_t(km - k) * sz, dp = std::ptrdiff_t(kp - k) * sz;
+    // the partner column of a u (v) control volume: the next one, wrapped
+    auto xpart = [&](int a) -> std::ptrdiff_t { return px && a == nx - 2 ? std::ptrdiff_t(1 - a) : 1; };
+    auto ypart = [&](int b) -> std::ptrdiff_t { return (py && b == ny - 2 ? std::ptrdiff_t(1 - b) : 1) * sy; };
     const double dz = g.dz;
-    auto comp_rho = [&](int a, int b, int e) {
-        if (loc == 0) return rho(a, b, e);
-        if (loc == 1) return 0.5 * (rho(a, b, e) + rho(px && a == nx - 2 ? 1 : a + 1, b, e));
-        if (loc == 2) return 0.5 * (rho(a, b, e) + rho(a, py && b == ny - 2 ? 1 : b + 1, e));
-        return (rho(a, b, e) * dz + rho(a, b, e + 1) * dz) / (dz + dz);
+    // live density at offset o (same level e as the offset's level part)
+    auto rho = [&](std::ptrdiff_t o, int e) { return rho0[e] + double(rp[o]); };
+    // control-volume density at column/row offset o of column a, row b, level e
+    auto comp_rho = [&](std::ptrdiff_t o, int a, int b, int e) {
+        const std::ptrdiff_t d = std::ptrdiff_t(e - k) * sz;
+        if (loc == 0) return rho(o + d, e);
+        if (loc == 1) return 0.5 * (rho(o + d, e) + rho(o + xpart(a) + d, e));
+        if (loc == 2) return 0.5 * (rho(o + d, e) + rho(o + ypart(b) + d, e));
+        return (rho(o + d, e) * dz + rho(o + d + sz, e + 1) * dz) / (dz + dz);
     };
-    const double qc = q(i, j, k);
+    const double qc = double(fq[0]);
     S t = S(0);
     const double kv = A.p.dif_kv;
     if (kv > 0.0)
     {
-        auto cell_tr = [&](int a, int b, int e) { return rho(a, b, e) * kv; };
-        auto comp_tr = [&](int a, int b, int e) {
-            if (loc == 1) return 0.5 * (cell_tr(a, b, e) + cell_tr(px && a == nx - 2 ? 1 : a + 1, b, e));
-            if (loc == 2) return 0.5 * (cell_tr(a, b, e) + cell_tr(a, py && b == ny - 2 ? 1 : b + 1, e));
-            return cell_tr(a, b, e);
+        const double q_m = double(fq[dm]), q_p = double(fq[dp]);
+        auto cell_tr = [&](std::ptrdiff_t o, int e) { return rho(o, e) * kv; };
+        // vertical transmissibility of the control volume at level e (offset d)
+        auto comp_tr = [&](std::ptrdiff_t d, int e) {
+            if (loc == 1) return 0.5 * (cell_tr(d, e) + cell_tr(d + xpart(i), e));
+            if (loc == 2) return 0.5 * (cell_tr(d, e) + cell_tr(d + ypart(j), e));
+            return cell_tr(d, e);
         };
         const bool wc = loc == 3;
-        auto face_tr = [&](int lo) { return wc ? cell_tr(i, j, lo + 1) : 0.5 * (comp_tr(i, j, lo) + comp_tr(i, j, lo + 1)); };
         const double dist = wc ? dz : 0.5 * (dz + dz);
-        const double mass = wc ? 0.5 * (rho(i, j, k) * dz + rho(i, j, k + 1) * dz) : comp_rho(i, j, k) * dz;
-        double down = 0.0, up = 0.0;
-        if (k > k0) down = face_tr(k - 1) * (qc - q(i, j, k - 1)) / dist;
-        if (k < k1 - 1) up = face_tr(k) * (q(i, j, k + 1) - qc) / dist;
+        double tr_down, tr_up, mass;
+        if (wc)
+        {
+            // faces of a w control volume are the cells k and k + 1
+            tr_down = cell_tr(0, k);
+            tr_up = cell_tr(sz, k + 1);
+            mass = 0.5 * (rho(0, k) * dz + rho(sz, k + 1) * dz);
+        }
+        else
+        {
+            const double tm = comp_tr(dm, km), t0 = comp_tr(0, k), tp = comp_tr(dp, kp);
+            tr_down = 0.5 * (tm + t0);
+            tr_up = 0.5 * (t0 + tp);
+            mass = comp_rho(0, i, j, k) * dz;
+        }
+        const double down = k > k0 ? tr_down * (qc - q_m) / dist : 0.0;
+        const double up = k < k1 - 1 ? tr_up * (q_p - qc) / dist : 0.0;
         t = S(double(t) + (up - down) / mass);
     }
     const double kh = A.p.dif_kh;
     if (kh > 0.0)
     {
-        const double idx2 = 1.0 / (g.dx * g.dx), idy2 = 1.0 / (g.dy * g.dy);
+        const double q_w = double(fq[ow]), q_e = double(fq[oe]), q_s = double(fq[os]), q_n = double(fq[on]);
         const bool wcl = i == i0 && !px, ecl = i == i1 - 1 && !px;
         const bool scl = j == j0 && !py, ncl = j == j1 - 1 && !py;
-        const int iw = i == i0 && px ? i1 - 1 : i - 1, ie = i == i1 - 1 && px ? i0 : i + 1;
-        const int js = j == j0 && py ? j1 - 1 : j - 1, jn = j == j1 - 1 && py ? j0 : j + 1;
-        const double rc = comp_rho(i, j, k);
-        const double rw = wcl ? 0.0 : 0.5 * (rc + comp_rho(iw, j, k));
-        const double re = ecl ? 0.0 : 0.5 * (rc + comp_rho(ie, j, k));
-        const double rs = scl ? 0.0 : 0.5 * (rc + comp_rho(i, js, k));
-        const double rn = ncl ? 0.0 : 0.5 * (rc + comp_rho(i, jn, k));
-        const double xd = ((ecl ? 0.0 : re * (q(ie, j, k) - qc)) - (wcl ? 0.0 : rw * (qc - q(iw, j, k)))) * idx2;
-        const double yd = ((ncl ? 0.0 : rn * (q(i, jn, k) - qc)) - (scl ? 0.0 : rs * (qc - q(i, js, k)))) * idy2;
+        const double rc = comp_rho(0, i, j, k);
+        const double cw = comp_rho(ow, iw, j, k), ce = comp_rho(oe, ie, j, k);
+        const double cs = comp_rho(os, i, js, k), cn = comp_rho(on, i, jn, k);
+        const double rw = wcl ? 0.0 : 0.5 * (rc + cw);
+        const double re = ecl ? 0.0 : 0.5 * (rc + ce);
+        const double rs = scl ? 0.0 : 0.5 * (rc + cs);
+        const double rn = ncl ? 0.0 : 0.5 * (rc + cn);
+        const double xd = ((ecl ? 0.0 : re * (q_e - qc)) - (wcl ? 0.0 : rw * (qc - q_w))) * A.p.dif_idx2;
+        const double yd = ((ncl ? 0.0 : rn * (q_n - qc)) - (scl ? 0.0 : rs * (qc - q_s))) * A.p.dif_idy2;
         t = S(double(t) + kh * (xd + yd) / rc);
     }
     return double(t);

@@ -86,6 +86,7 @@ struct StepParams
     // microphysics or SGS pass
     int dif_apply;
     double dif_kh, dif_kv;
+    double dif_idx2, dif_idy2;  // 1 / (dx dx), 1 / (dy dy)
     int dif_wrap_x, dif_wrap_y;
     KesslerConst kessler;
     Bounds bounds;

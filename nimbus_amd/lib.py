@@ -662,7 +662,8 @@ class Context:
         if isinstance(blob, (str, os.PathLike)):
             with open(blob, "rb") as f:
                 blob = f.read()
-        blob = bytes(blob)
+        if not isinstance(blob, (bytes, bytearray)):
+            blob = bytes(blob)
         head = struct.unpack_from(SNAP_HEAD, blob, 0)
         off = struct.calcsize(SNAP_HEAD)
         grid = Grid.from_buffer_copy(blob, off)
@@ -670,9 +671,13 @@ class Context:
         off += ctypes.sizeof(Grid) + ctypes.sizeof(StepConfig)
         prof = np.frombuffer(blob, dtype="<f8", count=6 * grid.nz, offset=off).reshape(6, grid.nz)
         base = {n: prof[q].copy() for q, n in enumerate(("rho0", "p0", "theta0", "qv0", "u0", "v0"))}
-        buf = ctypes.create_string_buffer(blob, len(blob))
+        # the library reads the snapshot in place: no host copy of the fields
+        if isinstance(blob, bytes):
+            ptr = ctypes.cast(ctypes.c_char_p(blob), ctypes.c_void_p)
+        else:
+            ptr = ctypes.cast((ctypes.c_char * len(blob)).from_buffer(blob), ctypes.c_void_p)
         mid = ctypes.c_uint32()
-        check(self.lib.nimbus_hip_member_restore(self._h, buf, len(blob), ctypes.byref(mid)), "member_restore")
+        check(self.lib.nimbus_hip_member_restore(self._h, ptr, len(blob), ctypes.byref(mid)), "member_restore")
         return Member(self, grid, config, base, head[5], existing_id=mid.value)
 
     def step(self, members, n_steps=1, wait=True):
@@ -765,13 +770,15 @@ class Member:
               "export_npy")
 
     def snapshot(self):
-        """The member's snapshot as bytes (restore with Context.restore)."""
+        """The member's snapshot as a bytearray, written in place by the library
+        (restore with Context.restore)."""
         n = ctypes.c_uint64()
         check(self.ctx.lib.nimbus_hip_member_snapshot_size(self.ctx.handle, self.id, ctypes.byref(n)),
               "snapshot_size")
-        buf = ctypes.create_string_buffer(n.value)
-        check(self.ctx.lib.nimbus_hip_member_snapshot(self.ctx.handle, self.id, buf, n.value), "snapshot")
-        return buf.raw
+        blob = bytearray(n.value)
+        ptr = ctypes.cast((ctypes.c_char * n.value).from_buffer(blob), ctypes.c_void_p)
+        check(self.ctx.lib.nimbus_hip_member_snapshot(self.ctx.handle, self.id, ptr, n.value), "snapshot")
+        return blob
 
     def save(self, path):
         check(self.ctx.lib.nimbus_hip_member_save(self.ctx.handle, self.id, os.fsencode(path)), "member_save")
