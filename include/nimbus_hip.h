@@ -346,7 +346,10 @@ int nimbus_hip_step_async(nimbus_hip_ctx* ctx, const uint32_t* ids, int n_ids, i
 int nimbus_hip_synchronize(nimbus_hip_ctx* ctx);
 /* Launch geometry of a member: column tiles of tile_w x (256 / tile_w) columns
  * (tile_w 32 or 64), column marches split into k_chunks vertical chunks
- * (1 .. nz), and XCD-contiguous workgroup ordering on/off. */
+ * (1 .. nz), and XCD-contiguous workgroup ordering on/off. tile_w 0 (the
+ * default): per-cell work in runs of 256 consecutive cells of one level's
+ * row-major plane, per-column work and the wide-stencil slow tendencies in
+ * 64 x 4 tiles (with k_chunks). */
 int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int k_chunks, int xcd_remap);
 /* HE-VI column-solve form of a member. It is fixed per member (never chosen
  * by the batch), so a member's results are the same alone and in any batch.

@@ -161,18 +161,23 @@ __device__ __forceinline__ Column column_of(const MemberArgs& A, int local)
     return Column{tx * geo.tile_w + (tid & (geo.tile_w - 1)), ty * geo.tile_h + tid / geo.tile_w};
 }
 
-// One cell per thread: tile t of the member at level k. The local work
-// index runs k fastest within a vertical chunk (k_chunks chunks of kz
-// levels), then over the column tiles, then over the chunks, so the
-// workgroups an XCD runs back to back are the vertical neighbours of one
-// column tile and a stencil's k-1 .. k+3 planes are still in its L2.
+// One cell per thread. Column tiles (tile_cell_of): tile t of the member at
+// level k; the local work index runs k fastest within a vertical chunk
+// (k_chunks chunks of kz levels), then over the column tiles, then over the
+// chunks, so the workgroups an XCD runs back to back are the vertical
+// neighbours of one column tile and a stencil's k-1 .. k+3 planes are still
+// in its L2. Linear runs (cell_of with MemberGeom::linear): 256 consecutive
+// elements of one level's row-major plane, levels in order, so a launch
+// streams through memory; the slow tendencies and the WENO5 face fluxes and
+// update (+-3 stencils in every direction) keep the column tiles, which serve
+// their j and k neighbours from L2.
 struct Cell
 {
     int i, j, k;
     bool valid;
 };
 
-__device__ __forceinline__ Cell cell_of(const MemberArgs& A, int local)
+__device__ __forceinline__ Cell tile_cell_of(const MemberArgs& A, int local)
 {
     const MemberGeom& geo = A.geo;
     const int nz = A.g.nz;
@@ -188,6 +193,29 @@ __device__ __forceinline__ Cell cell_of(const MemberArgs& A, int local)
     c.j = ty * geo.tile_h + tid / geo.tile_w;
     c.valid = c.k < nz && c.i < A.g.nx && c.j < A.g.ny;
     return c;
+}
+
+__device__ __forceinline__ Cell cell_of(const MemberArgs& A, int local)
+{
+    const MemberGeom& geo = A.geo;
+    const int nz = A.g.nz;
+    if (geo.linear)
+    {
+        // run r of kBlock consecutive elements of level k's padded plane: the
+        // level stays uniform over the workgroup, so per-level data stays
+        // scalar
+        const int k = local / geo.runs;
+        const unsigned idx = unsigned(local - k * geo.runs) * unsigned(kBlock) + threadIdx.x;
+        const unsigned px = unsigned(A.g.px);
+        const unsigned row = idx / px;
+        Cell c;
+        c.i = int(idx - row * px);
+        c.j = int(row);
+        c.k = k;
+        c.valid = k < nz && c.j < A.g.ny && c.i < A.g.nx;
+        return c;
+    }
+    return tile_cell_of(A, local);
 }
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
@@ -334,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_slow(const MemberArgs* __restrict__ 
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
     const DevGrid& g = A.g;
-    const Cell cl = cell_of(A, wk.local);
+    const Cell cl = tile_cell_of(A, wk.local);
     if (!cl.valid) return;
     const int i = cl.i, j = cl.j, k = cl.k;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
@@ -1528,7 +1556,7 @@ __global__ __launch_bounds__(kBlock) void k_transport_flux(const MemberArgs* __r
     using A_ = typename AccT<S>::type;
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
-    const Cell cl = cell_of(A, wk.local);
+    const Cell cl = tile_cell_of(A, wk.local);
     if (!cl.valid) return;
     const DevGrid& g = A.g;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
@@ -1656,7 +1684,7 @@ __global__ __launch_bounds__(kBlock) void k_transport_update(const MemberArgs* _
 {
     const Work wk = work_of(map);
     const MemberArgs& A = ma[wk.m];
-    const Cell cl = cell_of(A, wk.local);
+    const Cell cl = tile_cell_of(A, wk.local);
     if (!cl.valid) return;
     const DevGrid& g = A.g;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
@@ -2987,6 +3015,8 @@ __global__ __launch_bounds__(64) void k_dev_to_host(DevGrid g, const S* src, H* 
 void fill_geometry(MemberArgs& a, int tile_w, int k_chunks, int xcd_remap)
 {
     MemberGeom& G = a.geo;
+    G.linear = tile_w == 0 ? 1 : 0;
+    G.runs = int((a.g.plane() + kBlock - 1) / kBlock);
     G.tile_w = tile_w == 32 ? 32 : 64;
     G.tile_h = kBlock / G.tile_w;
     G.tiles_x = (a.g.nx + G.tile_w - 1) / G.tile_w;
@@ -3002,13 +3032,15 @@ int work_items(const MemberArgs& a, WorkKind kind)
     const auto lin = [](std::size_t n) { return int((n + kBlock - 1) / kBlock); };
     switch (kind)
     {
-    case W_CELL: return tiles * G.k_chunks * ((a.g.nz + G.k_chunks - 1) / G.k_chunks);
+    case W_CELL:
+        return G.linear ? G.runs * a.g.nz : tiles * G.k_chunks * ((a.g.nz + G.k_chunks - 1) / G.k_chunks);
     case W_COLUMN: return tiles;
     case W_WE: return lin(std::size_t(a.g.ny) * a.g.nz);
     case W_SN: return lin(std::size_t(a.g.nx) * a.g.nz);
     case W_SEED: return lin(std::size_t(std::max(a.g.nx, a.g.ny)) * a.g.nz);
     case W_DIAG: return kDiagBlocksPerMember;
     case W_SEGCOL: return ((a.g.nx + kSegCols - 1) / kSegCols) * a.g.ny;
+    case W_TILECELL: return tiles * G.k_chunks * ((a.g.nz + G.k_chunks - 1) / G.k_chunks);
     default: return 0;
     }
 }
@@ -3091,14 +3123,14 @@ S* PS(void* p)
 template <class S>
 void launch_slow_rk2(Launcher& L, const BatchView& b)
 {
-    launch_map(L, "slow_stage1", k_slow<S, 0>, b, S_ALL, W_CELL, (S*)nullptr, (S*)nullptr, (S*)nullptr);
-    launch_map(L, "slow_stage2", k_slow<S, 1>, b, S_ALL, W_CELL, (S*)nullptr, (S*)nullptr, (S*)nullptr);
+    launch_map(L, "slow_stage1", k_slow<S, 0>, b, S_ALL, W_TILECELL, (S*)nullptr, (S*)nullptr, (S*)nullptr);
+    launch_map(L, "slow_stage2", k_slow<S, 1>, b, S_ALL, W_TILECELL, (S*)nullptr, (S*)nullptr, (S*)nullptr);
 }
 
 template <class S>
 void launch_slow_tendencies(Launcher& L, const BatchView& b, void* du, void* dv, void* dw)
 {
-    launch_map(L, "slow_tendencies", k_slow<S, 2>, b, S_ALL, W_CELL, PS<S>(du), PS<S>(dv), PS<S>(dw));
+    launch_map(L, "slow_tendencies", k_slow<S, 2>, b, S_ALL, W_TILECELL, PS<S>(du), PS<S>(dv), PS<S>(dw));
 }
 
 template <class S>
@@ -3144,9 +3176,9 @@ void launch_transport(Launcher& L, const BatchView& b, int n_scalars, const int*
     (void)into_temp;
     int mask = 0;
     for (int q = 0; q < n_scalars; ++q) mask |= (pd[q] ? 1 : 0) << q;
-    launch_map(L, "transport_flux", k_transport_flux<S>, b, S_ALL, W_CELL, n_scalars);
+    launch_map(L, "transport_flux", k_transport_flux<S>, b, S_ALL, W_TILECELL, n_scalars);
     if (mask) launch_map(L, "transport_limiter", k_transport_limiter<S>, b, S_ALL, W_CELL, n_scalars, mask);
-    launch_map(L, "transport_update", k_transport_update<S>, b, S_ALL, W_CELL, n_scalars, mask);
+    launch_map(L, "transport_update", k_transport_update<S>, b, S_ALL, W_TILECELL, n_scalars, mask);
 }
 
 template <class S>

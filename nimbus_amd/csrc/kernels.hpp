@@ -101,6 +101,10 @@ struct MemberGeom
     int tile_w, tile_h, tiles_x, tiles_y;
     int k_chunks;
     int xcd_remap;
+    // per-cell work in runs of kBlock consecutive elements of one level's
+    // padded [j][i] plane (`runs` per level) instead of column tiles at one
+    // level; column work keeps its tiles
+    int linear, runs;
 };
 
 struct MemberArgs
@@ -114,14 +118,15 @@ struct MemberArgs
 // Work shapes of a launch: what one map entry (one workgroup) covers.
 enum WorkKind
 {
-    W_CELL = 0,    // one tile of columns at one level (one cell per thread)
+    W_CELL = 0,    // one tile of columns at one level, or one run of a level (one cell per thread)
     W_COLUMN = 1,  // one tile, whole columns (one column per thread)
     W_WE = 2,      // 256 (j, k) rows of the west/east faces
     W_SN = 3,      // 256 (i, k) rows of the south/north faces
     W_SEED = 4,    // 256 entries of the radiative stashes
     W_DIAG = 5,    // one of kDiagBlocksPerMember reduction blocks
     W_SEGCOL = 6,  // kSegCols consecutive columns of one row, kSegments runs of faces each
-    W_KINDS = 7
+    W_TILECELL = 7,  // one tile of columns at one level whatever the geometry (wide stencils)
+    W_KINDS = 8
 };
 
 // Partitioned HE-VI column solve: a column of nz - 1 unknown faces is split

@@ -784,7 +784,8 @@ class Member:
         check(self.ctx.lib.nimbus_hip_member_save(self.ctx.handle, self.id, os.fsencode(path)), "member_save")
 
     def set_tiling(self, tile_w, k_chunks, xcd_remap):
-        """Launch geometry: tile_w in {32, 64}, k_chunks in 1..nz, XCD-contiguous order on/off."""
+        """Launch geometry: tile_w in {32, 64} (column tiles), or 0 (per-cell work in runs of 256
+        consecutive cells of a level); k_chunks in 1..nz; XCD-contiguous order on/off."""
         check(self.ctx.lib.nimbus_hip_member_set_tiling(self.ctx.handle, self.id, int(tile_w), int(k_chunks),
                                                         int(xcd_remap)), "set_tiling")
 
