@@ -348,8 +348,9 @@ int nimbus_hip_synchronize(nimbus_hip_ctx* ctx);
  * (tile_w 32 or 64), column marches split into k_chunks vertical chunks
  * (1 .. nz), and XCD-contiguous workgroup ordering on/off. tile_w 0 (the
  * default): per-cell work in runs of 256 consecutive cells of one level's
- * row-major plane, per-column work and the wide-stencil slow tendencies in
- * 64 x 4 tiles (with k_chunks). */
+ * row-major plane, per-column work in runs of 256 consecutive columns, the
+ * wide-stencil passes (slow tendencies, WENO5 fluxes and update) in 64 x 4
+ * tiles with k_chunks. */
 int nimbus_hip_member_set_tiling(nimbus_hip_ctx* ctx, uint32_t id, int tile_w, int k_chunks, int xcd_remap);
 /* HE-VI column-solve form of a member. It is fixed per member (never chosen
  * by the batch), so a member's results are the same alone and in any batch.

@@ -147,7 +147,8 @@ __device__ __forceinline__ Work work_of(const int2* __restrict__ map)
     return Work{e.x, e.y};
 }
 
-// One thread per (i, j) column of a tile_w x tile_h tile of columns.
+// One thread per (i, j) column: of a tile_w x tile_h tile of columns, or of
+// a run of kBlock consecutive columns in the linear geometry.
 struct Column
 {
     int i, j;
@@ -156,6 +157,15 @@ struct Column
 __device__ __forceinline__ Column column_of(const MemberArgs& A, int local)
 {
     const MemberGeom& geo = A.geo;
+    if (geo.linear)
+    {
+        // run `local` of kBlock consecutive columns in row-major order: a
+        // level's loads of a workgroup are one or two contiguous segments
+        const unsigned col = unsigned(local) * unsigned(kBlock) + threadIdx.x;
+        const unsigned nx = unsigned(A.g.nx);
+        const unsigned j = col / nx;
+        return Column{int(col - j * nx), int(j)};
+    }
     const int ty = local / geo.tiles_x, tx = local - ty * geo.tiles_x;
     const int tid = int(threadIdx.x);
     return Column{tx * geo.tile_w + (tid & (geo.tile_w - 1)), ty * geo.tile_h + tid / geo.tile_w};
@@ -3034,7 +3044,7 @@ int work_items(const MemberArgs& a, WorkKind kind)
     {
     case W_CELL:
         return G.linear ? G.runs * a.g.nz : tiles * G.k_chunks * ((a.g.nz + G.k_chunks - 1) / G.k_chunks);
-    case W_COLUMN: return tiles;
+    case W_COLUMN: return G.linear ? int((std::size_t(a.g.nx) * a.g.ny + kBlock - 1) / kBlock) : tiles;
     case W_WE: return lin(std::size_t(a.g.ny) * a.g.nz);
     case W_SN: return lin(std::size_t(a.g.nx) * a.g.nz);
     case W_SEED: return lin(std::size_t(std::max(a.g.nx, a.g.ny)) * a.g.nz);

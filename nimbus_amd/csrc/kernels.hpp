@@ -103,7 +103,7 @@ struct MemberGeom
     int xcd_remap;
     // per-cell work in runs of kBlock consecutive elements of one level's
     // padded [j][i] plane (`runs` per level) instead of column tiles at one
-    // level; column work keeps its tiles
+    // level, per-column work in runs of kBlock consecutive columns
     int linear, runs;
 };
 

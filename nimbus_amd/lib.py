@@ -785,7 +785,8 @@ class Member:
 
     def set_tiling(self, tile_w, k_chunks, xcd_remap):
         """Launch geometry: tile_w in {32, 64} (column tiles), or 0 (per-cell work in runs of 256
-        consecutive cells of a level); k_chunks in 1..nz; XCD-contiguous order on/off."""
+        consecutive cells of a level, per-column work in runs of 256 columns); k_chunks in 1..nz;
+        XCD-contiguous order on/off."""
         check(self.ctx.lib.nimbus_hip_member_set_tiling(self.ctx.handle, self.id, int(tile_w), int(k_chunks),
                                                         int(xcd_remap)), "set_tiling")
 
