@@ -339,7 +339,7 @@ def ensemble_run(ctx, L, W, precision, mode, tuning=None, column_solve=None):
 
 
 def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5, tuning=None, column_solve=None,
-                stop_after=None, max_cells=None):
+                stop_after=None, max_cells=None, admission="fifo"):
     """Continuous batching of seeded Poisson requests (2/s) arriving over `seconds`,
     then `fp64_sample` requests (drawn with seed 5) rerun alone in FP64: the error of each
     served (mixed) result against its FP64 rerun. `stop_after` (wall seconds) ends
@@ -352,7 +352,7 @@ def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5, tuning=None, co
     before = ctx.batch_stats()
     srv = SV.Server(ctx, budget_bytes=200e9, precision=precision, acoustic_mode=mode, quantum=16, tuning=tuning,
                     column_solve=column_solve, progress=lambda m: progress("  serving " + m),
-                    max_resident_cells=max_cells)
+                    max_resident_cells=max_cells, admission=admission)
     rep = srv.run(reqs, keep_fields=sample, stop_after=stop_after)
     s = rep.summary()
     after = ctx.batch_stats()
@@ -363,6 +363,7 @@ def serving_run(ctx, L, seconds, precision, mode, fp64_sample=5, tuning=None, co
                         if s["unfinished"] else f", finished within the {stop_after:g} s stop")
     s["requests_generated"] = len(reqs)
     s["max_resident_cells"] = max_cells
+    s["admission"] = admission
     s["membership_changes"] = {k: round(after[k] - before[k], 3) for k in after}
     s["data"] = ("seed 4 Poisson arrivals at 2/s; nx = ny in {48..256}, nz in {40, 60, 80}, dx 500-1000 m, "
                  "15-90 simulated minutes each; 200 GB HBM budget")
@@ -473,6 +474,8 @@ def main():
                     help="end the serving run after this many wall seconds and report the backlog")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving run (with --serving-seconds), no single-member or ensemble runs")
+    ap.add_argument("--serving-admission", choices=["fifo", "shortest"], default="fifo",
+                    help="order of the waiting requests: arrival, or least work first")
     ap.add_argument("--serving-max-cells", type=float, default=48e6,
                     help="cap on the cells resident at once in the serving run besides the 200 GB HBM budget "
                          "(0: no cap)")
@@ -528,7 +531,8 @@ def main():
         progress(f"serving: Poisson arrivals over {args.serving_seconds:g} s")
         out = {"serving_c4": serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI,
                                          tuning=tuning, column_solve=col, stop_after=args.serving_stop_after,
-                                         fp64_sample=0, max_cells=args.serving_max_cells or None)}
+                                         fp64_sample=0, max_cells=args.serving_max_cells or None,
+                                         admission=args.serving_admission)}
         ctx.close()
         print(json.dumps(out))
         return
@@ -633,7 +637,8 @@ def main():
         progress(f"serving: Poisson arrivals over {args.serving_seconds:g} s")
         result["serving_c4"] = serving_run(ctx, L, args.serving_seconds, L.PRECISION_MIXED, L.ACOUSTIC_HEVI,
                                            tuning=tuning, column_solve=col, stop_after=args.serving_stop_after,
-                                           max_cells=args.serving_max_cells or None)
+                                           max_cells=args.serving_max_cells or None,
+                                           admission=args.serving_admission)
     if args.accuracy:
         from nimbus_amd import accuracy as ACC
         for q, name in enumerate(n for n in args.accuracy.split(",") if n):

@@ -20,9 +20,14 @@ preparation started, and only then does the loop wait for the device.
 (None: the budget alone decides). A batch of a few tens of million cells
 already keeps the device busy; admitting more only spreads the same
 throughput over more members, so under overload every request waits longer
-and each membership change rebuilds a larger batch. Admission stays first
-come, first served, and a request larger than the cap is admitted when
-nothing else is resident.
+and each membership change rebuilds a larger batch. A request larger than
+the cap is admitted when nothing else is resident.
+
+`admission` orders the waiting requests: "fifo" (first come, first served)
+or "shortest" (least work, cells x steps, first: under overload the short
+requests stop queueing behind long ones, which lowers the median completion
+time, and a long request can wait as long as shorter ones keep arriving).
+Evicted members go back to the head of the queue under either order.
 
 With a TuningCache, every admitted member gets the autotuned launch geometry
 of its shape (tuned the first time the shape is seen); the geometry never
@@ -179,8 +184,11 @@ class Server:
 
     def __init__(self, ctx, budget_bytes=200e9, precision=L.PRECISION_MIXED, acoustic_mode=L.ACOUSTIC_HEVI,
                  quantum=16, graphs=True, host_workers=4, preempt=False, preempt_ratio=4.0, results_dir=None,
-                 tuning=None, column_solve=None, progress=None, max_resident_cells=None):
+                 tuning=None, column_solve=None, progress=None, max_resident_cells=None, admission="fifo"):
+        if admission not in ("fifo", "shortest"):
+            raise ValueError(f"admission must be 'fifo' or 'shortest', not {admission!r}")
         self.ctx = ctx
+        self.admission = admission
         self.budget = int(budget_bytes)
         self.max_cells = None if max_resident_cells is None else int(max_resident_cells)
         self.precision = precision
@@ -241,6 +249,12 @@ class Server:
                     r = queue.popleft()
                     prepared[r.rid] = pool.submit(self._prepare, r)
                     waiting.append(r)
+                if self.admission == "shortest" and len(waiting) > 1:
+                    # evicted members stay in front; the new requests by work
+                    held = [w for w in waiting if isinstance(w, _Suspended)]
+                    fresh = sorted((w for w in waiting if not isinstance(w, _Suspended)),
+                                   key=lambda r: (r.workload.cells * r.workload.steps, r.arrival, r.rid))
+                    waiting = collections.deque(held + fresh)
                 while waiting:
                     item = waiting[0]
                     if isinstance(item, _Suspended):

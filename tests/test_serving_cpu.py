@@ -258,3 +258,24 @@ def test_resident_cell_cap_holds_back_requests_until_cells_free():
     rep = SV.Server(ctx, quantum=4, host_workers=1, max_resident_cells=cells // 2).run(
         reqs[:2], time_scale=1e6, sleep=lambda s: None)
     assert len(rep.results) == 2 and rep.peak_members == 1
+
+
+def test_shortest_first_admission_orders_waiting_requests_by_work():
+    """With room for one member, requests that wait are admitted least work
+    first under 'shortest' and in arrival order under 'fifo'; every request
+    still takes exactly its own number of steps."""
+    steps = [40, 30, 5, 20, 10]
+    reqs = small_requests(5, steps=steps)
+    orders = {}
+    for admission in ("fifo", "shortest"):
+        ctx = FakeContext()
+        srv = SV.Server(ctx, quantum=64, host_workers=1, admission=admission)
+        srv.budget = srv.footprint(reqs[0])
+        rep = srv.run(reqs, clock=_stepping_clock(), sleep=lambda s: None)
+        assert all(r.diag["steps"] == r.steps for r in rep.results)
+        orders[admission] = [r.rid for r in rep.results]
+    # all five arrive at once and run one at a time
+    assert orders["fifo"] == [0, 1, 2, 3, 4]
+    assert orders["shortest"] == [2, 4, 3, 1, 0]
+    with pytest.raises(ValueError):
+        SV.Server(FakeContext(), admission="lifo")
