@@ -9,11 +9,13 @@ from nimbus_amd import serving as SV
 
 
 @pytest.mark.gpu
-def test_served_results_equal_alone(hip_ctx):
+@pytest.mark.parametrize("admission", ["fifo", "shortest"])
+def test_served_results_equal_alone(hip_ctx, admission):
+    print(f"case: served results equal alone, admission {admission}", flush=True)
     reqs = SV.poisson_requests(seed=11, rate=40.0, horizon=0.25, sizes=(48, 64), levels=(40, 60),
                                minutes=(0.5, 1.5))
     assert len(reqs) >= 5
-    srv = SV.Server(hip_ctx, quantum=3, host_workers=2)
+    srv = SV.Server(hip_ctx, quantum=3, host_workers=2, admission=admission)
     srv.budget = 3 * srv.footprint(reqs[0])  # forces waiting and admission as members finish
     rep = srv.run(reqs, time_scale=1.0, keep_fields=[r.rid for r in reqs])
     s = rep.summary()
