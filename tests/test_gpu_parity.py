@@ -256,6 +256,22 @@ def test_large_step_fp64_matches_fp64_oracle(hip_ctx, mode, lateral):
     assert max_abs(out["rain_mm"], ref["rain_mm"]) < 1e-9 * max(1.0, float(np.max(ref["rain_mm"])))
 
 
+@pytest.mark.parametrize("shape", [(5, 5, 4), (5, 9, 6), (17, 5, 5)])
+@pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
+def test_smallest_grids_fp64_match_fp64_oracle(hip_ctx, mode, shape):
+    """The smallest grids a member accepts (nx, ny >= 5, nz >= 4): a single
+    interior column row, a plane far smaller than one 256-cell run (most
+    lanes of every workgroup are padding or beyond the plane), and a
+    one-workgroup column run, in both acoustic methods."""
+    nx, ny, nz = shape
+    print(f"case: smallest grid {shape} mode {mode}", flush=True)
+    g, base, st = small_case(nx=nx, ny=ny, nz=nz, dz=2000.0, cloudy=True, wind_noise=0.5, dtype=np.float64)
+    cfg = L.step_config(3.0, 6, acoustic_mode=mode, lateral=L.BC_RIGID_SLIP)
+    ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, 3, np.float64)
+    out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, 3)
+    check_fp64(out, ref, f"smallest {shape} mode {mode}", False)
+
+
 @pytest.mark.parametrize("nz", [4, 14, 17, 33, 100, 129, 130, 140])
 def test_hevi_column_depths_fp64_match_fp64_oracle(hip_ctx, nz):
     """The vertically implicit solve of FP64 members at column depths that
