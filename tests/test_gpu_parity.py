@@ -272,23 +272,30 @@ def test_smallest_grids_fp64_match_fp64_oracle(hip_ctx, mode, shape):
     check_fp64(out, ref, f"smallest {shape} mode {mode}", False)
 
 
+@pytest.mark.parametrize("precision", [L.PRECISION_FP64, L.PRECISION_MIXED])
 @pytest.mark.parametrize("mode", [L.ACOUSTIC_EXPLICIT, L.ACOUSTIC_HEVI])
-def test_non_finite_state_is_sanitised_as_the_oracle_does(hip_ctx, mode):
+def test_non_finite_state_is_sanitised_as_the_oracle_does(hip_ctx, mode, precision):
     """A NaN wind, an infinite vapour mixing ratio and a NaN temperature in
     the uploaded state go through the reference's guards (non-finite
     tendencies taken as zero, the storage clamps' fallbacks) and come out
-    finite, cell for cell as in the oracle."""
-    print(f"case: non-finite state mode {mode}", flush=True)
+    finite, cell for cell as in the FP64 oracle (FP64 members to 1e-9 of
+    each field's scale, mixed members to the FP32 rounding bound)."""
+    print(f"case: non-finite state mode {mode} precision {precision}", flush=True)
     g, base, st = small_case(nx=16, ny=14, nz=10, cloudy=True, wind_noise=0.5, dtype=np.float64)
     st["u"][6, 6, 4] = np.nan
     st["qv"][9, 7, 3] = np.inf
     st["theta"][4, 9, 5] = np.nan
     cfg = L.step_config(3.0, 6, acoustic_mode=mode, lateral=L.BC_RIGID_SLIP)
     ref, _ = O.large_steps(oracle_grid(g), base, O.from_step_config(cfg), st, 2, np.float64)
-    out, _ = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_FP64, 2)
+    dtype = np.float64 if precision == L.PRECISION_FP64 else np.float32
+    up = {k: (v.astype(dtype) if k in L.FIELDS else v) for k, v in st.items()}
+    out, _ = _run_member(hip_ctx, g, base, cfg, up, precision, 2)
     for f in L.FIELDS:
         assert np.all(np.isfinite(ref[f])) and np.all(np.isfinite(out[f])), f
-    check_fp64(out, ref, f"non-finite state mode {mode}", False)
+    if precision == L.PRECISION_FP64:
+        check_fp64(out, ref, f"non-finite state mode {mode}", False)
+    else:
+        check_mixed(out, ref, f"non-finite state mode {mode} mixed")
 
 
 @pytest.mark.parametrize("nz", [4, 14, 17, 33, 100, 129, 130, 140])
