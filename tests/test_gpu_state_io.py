@@ -22,13 +22,19 @@ def _member(ctx, precision, mode, seed=0):
     return m
 
 
+@pytest.mark.parametrize("tiling", [None, (32, 2, 1)])
 @pytest.mark.parametrize("precision,mode", [(L.PRECISION_MIXED, L.ACOUSTIC_HEVI),
                                             (L.PRECISION_FP64, L.ACOUSTIC_EXPLICIT)])
-def test_snapshot_resume_is_bitwise(hip_ctx, tmp_path, precision, mode):
-    print(f"case: snapshot precision {precision} mode {mode}", flush=True)
+def test_snapshot_resume_is_bitwise(hip_ctx, tmp_path, precision, mode, tiling):
+    """Snapshot after two steps, three more steps on the original and on the
+    restored member (from memory and from a file): identical results. The
+    launch geometry travels with the snapshot (the default linear order, or
+    32 x 8 column tiles)."""
+    print(f"case: snapshot precision {precision} mode {mode} tiling {tiling}", flush=True)
     a = _member(hip_ctx, precision, mode)
     try:
-        a.set_tiling(32, 2, 1)
+        if tiling is not None:
+            a.set_tiling(*tiling)
         hip_ctx.step([a], 2)
         blob = a.snapshot()
         a.save(str(tmp_path / "m.snap"))
