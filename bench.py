@@ -570,7 +570,7 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the large step on one device and "
+        print(f"bench.py measures one GPU: the large step runs on one device and "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
