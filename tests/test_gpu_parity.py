@@ -539,7 +539,8 @@ def test_graph_replay_matches_eager(hip_ctx):
 
 def test_member_alone_equals_member_in_mix(hip_ctx):
     """Bitwise identity of a member advanced alone and interleaved with others
-    of different grid sizes (serving's 'same alone and in the mix')."""
+    of different grid sizes (serving's 'same alone and in the mix'); one of
+    the others runs in column tiles, so the batch mixes both launch orders."""
     g, base, st = small_case(nx=20, ny=18, nz=14, cloudy=True)
     cfg = L.step_config(3.0, 6)
     alone, d_alone = _run_member(hip_ctx, g, base, cfg, st, L.PRECISION_MIXED, 3)
@@ -550,6 +551,8 @@ def test_member_alone_equals_member_in_mix(hip_ctx):
     for q in range(2):
         o = hip_ctx.member(g2, L.step_config(2.0, 4), base2, L.PRECISION_MIXED)
         o.upload(st2)
+        if q == 1:
+            o.set_tiling(64, 3, 0)
         others.append(o)
     hip_ctx.step([others[0], m, others[1]], 3)
     mixed = m.download()
